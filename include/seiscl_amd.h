@@ -1,0 +1,146 @@
+/*
+ * seiscl_amd.h -- C-ABI of the MI355X-native SeisCL time-domain FD engine.
+ *
+ * This library replaces the device runtime of the reference (Init_OpenCL /
+ * clprogram / clbuf, src/F.h:124-189, 547, 566) and its per-shot time loop
+ * (time_stepping, src/time_stepping.c:692-1012) with hand-written gfx950 HIP
+ * kernels.  Everything crossing this boundary is plain C: opaque handles,
+ * pointers, sizes and int status codes (0 = success).  No call writes to
+ * stderr; the text of the last error on the calling thread is returned by
+ * seis_last_error().
+ *
+ * Array layouts at this boundary are the reference's host layouts:
+ *   - model parameters: N[0]=NZ (fastest), then NY, then NX, unpadded
+ *     (what SeisCL.py writes after np.transpose, SeisCL/SeisCL.py:747-763);
+ *   - src_pos [nsrc][5] = x, y, z, srcid, type   (src/read_hdf5.c:307-459);
+ *   - src     [nsrc][NT];
+ *   - rec_pos [nrec][8] = x, y, z, srcid, recid, type, -, -;
+ *   - seismograms [nout][nrec][NT] (src/time_stepping.c:25-90), outputs in
+ *     the order of seis_out_name();
+ *   - gradients: unpadded like the parameters.
+ */
+#ifndef SEISCL_AMD_H
+#define SEISCL_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEIS_MAXL 8
+
+/* Modeling constants.  Field names, meaning and defaults follow the HDF5
+ * constants file read by read_csts (src/read_hdf5.c:461-552) and the csts
+ * list of SeisCL.py (SeisCL/SeisCL.py:23-30).  Fill with seis_csts_default()
+ * then override. */
+typedef struct seis_csts {
+    int ND;             /* 2: 2D P-SV, 3: 3D, 22: 2D acoustic                   */
+    int N[3];           /* [NZ, NX] (2D) or [NZ, NY, NX] (3D)                    */
+    int NT;             /* time samples                                           */
+    float dt, dh;
+    int FDORDER;        /* 2, 4, 6, 8, 10, 12                                     */
+    int MAXRELERROR;    /* 0 Taylor, 1..4 Holberg                                 */
+    int freesurf;       /* 1: free surface on top (z=0)                           */
+    int abs_type;       /* 1: CPML, 2: Cerjan taper, 0: none                     */
+    int nab;            /* absorbing width in cells                               */
+    float abpc, VPPML, NPOWER, FPML, K_MAX_CPML;
+    int L;              /* Maxwell bodies (0 = elastic)                           */
+    float f0;           /* reference frequency of the phase velocity              */
+    float FL[SEIS_MAXL];/* relaxation frequencies                                 */
+    int seisout;        /* 1 velocities, 2 pressure, 3 stresses, 4 all           */
+    int param_type;     /* 0 vp,vs,rho  1 M,mu,rho  2 Ip,Is,rho  3 complex v      */
+    int tmax;           /* forward steps, 0 -> NT   (src/read_hdf5.c:523-526)    */
+    int tmin;           /* first adjoint step bound (src/time_stepping.c:869)     */
+    int gradout;        /* 1: keep what the gradient needs                        */
+    int FP16;           /* 0: FP32; 2: FP16 storage, FP32 compute; 64: FP64      */
+} seis_csts;
+
+typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
+typedef struct seis_model seis_model;   /* one modeling case resident on a GPU   */
+
+/* ---- errors ------------------------------------------------------------- */
+const char *seis_last_error(void);
+void seis_csts_default(seis_csts *c);    /* SeisCL.py:36-83 defaults             */
+size_t seis_csts_size(void);             /* sizeof(seis_csts), for FFI checks    */
+
+/* ---- device context: replaces Init_CUDA device discovery and the two      */
+/* queues per device (src/Init_OpenCL.c:24-323, 274-287)                      */
+int seis_ctx_create(int device_ordinal, seis_ctx **ctx);
+int seis_ctx_destroy(seis_ctx *ctx);
+int seis_ctx_sync(seis_ctx *ctx);                   /* WAITQUEUE (src/F.h:634)  */
+size_t seis_ctx_footprint(const seis_ctx *ctx);     /* device bytes allocated   */
+
+/* ---- buffers: replace clbuf_create / clbuf_send / clbuf_read              */
+/* (src/clbuf.c:23-281).  Transfers are ordered on the compute stream.        */
+int seis_buf_create(seis_ctx *ctx, size_t bytes, void **dptr);
+int seis_buf_free(seis_ctx *ctx, void *dptr);
+int seis_buf_send(seis_ctx *ctx, void *dptr, const void *host, size_t bytes);
+int seis_buf_read(seis_ctx *ctx, void *host, const void *dptr, size_t bytes);
+
+/* ---- model: replaces assign_modeling_case + Init_cst/Init_model +          */
+/* the buffer/program set-up of Init_CUDA (src/assign_modeling_case.c:871-   */
+/* 1326, src/Init_model.c:40-97, src/Init_OpenCL.c:326-1217).  Parameters are */
+/* those named by param_type (vp|M|Ip, vs|mu|Is, rho, taup, taus); pointers  */
+/* the case does not use may be NULL.  Fails on an unstable dt                */
+/* (check_stability, src/assign_modeling_case.c:730-827).                     */
+int seis_model_create(seis_ctx *ctx, const seis_csts *c, const float *p0,
+                      const float *p1, const float *rho, const float *taup,
+                      const float *taus, seis_model **model);
+int seis_model_destroy(seis_model *m);
+/* Replace the material parameters of an existing model (same grid).          */
+int seis_model_set_params(seis_model *m, const float *p0, const float *p1,
+                          const float *rho, const float *taup, const float *taus);
+size_t seis_model_footprint(const seis_model *m);
+
+/* number / names of seismogram outputs for the model's seisout              */
+/* (src/assign_modeling_case.c:1244-1326; SeisCL.py to_load_names :376-408)  */
+int seis_out_count(const seis_model *m);
+const char *seis_out_name(const seis_model *m, int i);
+
+/* ---- per-shot runtime: the launch sequence of time_stepping.               */
+/* seis_shot_bind = initialize_forward H2D of src/src_pos/rec_pos            */
+/* (src/time_stepping.c:553-621); the step launches replace prog_launch of   */
+/* sources / update_v / update_s / freesurface / varsout                     */
+/* (src/time_stepping.c:785-816, src/clprogram.c:1004-1071).                  */
+int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos,
+                   const float *src, int nrec, const float *rec_pos);
+int seis_launch_init(seis_model *m);                 /* vars_init + varsoutinit */
+int seis_launch_sources(seis_model *m, int nt, int pdir);
+int seis_launch_update_v(seis_model *m);
+int seis_launch_update_s(seis_model *m);
+int seis_launch_surface(seis_model *m);
+int seis_launch_varsout(seis_model *m, int nt);
+/* D2H of the bound shot's seismograms, [nout][nrec][NT] (reduce_seis,       */
+/* src/time_stepping.c:25-90).                                                */
+int seis_shot_read(seis_model *m, float *out);
+
+/* Run forward steps t0 .. t0+nsteps-1 of the bound shot (sources, update_v, */
+/* update_s, free surface, varsout), waiting for completion; the device time */
+/* of the run is returned by seis_last_timing.                                */
+int seis_run_steps(seis_model *m, int t0, int nsteps);
+
+/* Whole forward shot: bind + init + NT steps + read (the forward half of    */
+/* time_stepping, src/time_stepping.c:747-848).                              */
+int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos,
+                      const float *src, int nrec, const float *rec_pos,
+                      float *out);
+
+/* Copy one wavefield ("vx", "sxx", ...) to the host in the reference's     */
+/* padded layout, Pi(N+FDORDER) floats (used by movout, src/time_stepping.c: */
+/* 272-350, and by parity tests).                                              */
+int seis_read_field(seis_model *m, const char *name, float *host);
+
+/* Record HIP events around every update_v / update_s launch of the next    */
+/* forward runs (costs a few microseconds per step).                          */
+int seis_model_set_profiling(seis_model *m, int on);
+
+/* Last timed forward run: device milliseconds spent in the update kernels   */
+/* and in the whole step loop (HIP events on the compute stream).             */
+int seis_last_timing(const seis_model *m, double *ms_update_v,
+                     double *ms_update_s, double *ms_loop);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1710 @@
+/*
+ * seis_oracle.c -- CPU restatement of SeisCL's time-domain finite-difference
+ * forward path (the checker for the HIP product path).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Nothing in seiscl_amd/ links or calls it.
+ *
+ * Arithmetic semantics follow the reference's OpenCL build:
+ *   - HC1..HC6, DT, DT2 and DH are float literals printed with "%9.9f"
+ *     (src/clprogram.c:323-348), reproduced by or_literal9();
+ *   - unsuffixed literals such as 1.0, 2.0, 0.5 are double, so expressions
+ *     that contain them are evaluated in double and rounded to float on
+ *     assignment, exactly as written below;
+ *   - contraction into FMA is disabled (-ffp-contract=off in oracle/Makefile)
+ *     and every sum keeps the reference's left-to-right order.
+ *
+ * Every function cites the reference file:line it restates.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "seis_oracle.h"
+
+#define PI (3.141592653589793238462643383279502884197169)
+
+/* ------------------------------------------------------------------------ */
+/* FD coefficients: src/third_party/SOFI3D/holbergcoeff.c:22-102            */
+/* ------------------------------------------------------------------------ */
+static const float HOLBERG[5][6][7] = {
+    {{23.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {8.0f, 9.0f / 8.0f, -1.0f / 24.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {6.0f, 75.0f / 64.0f, -25.0f / 384.0f, 3.0f / 640.0f, 0.0f, 0.0f, 0.0f},
+     {5.0f, 1225.0f / 1024.0f, -245.0f / 3072.0f, 49.0f / 5120.0f,
+      -5.0f / 7168.0f, 0.0f, 0.0f},
+     {5.0f, 19845.0f / 16384.0f, -735.0f / 8192.0f, 567.0f / 40960.0f,
+      -405.0f / 229376.0f, 35.0f / 294912.0f, 0.0f},
+     {4.0f, 160083.0f / 131072.0f, -12705.0f / 131072.0f,
+      22869.0f / 1310720.0f, -5445.0f / 1835008.0f, 847.0f / 2359296.0f,
+      -63.0f / 2883584.0f}},
+    {{49.7f, 1.0010f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {8.32f, 1.1382f, -0.046414f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {4.77f, 1.1965f, -0.078804f, 0.0081781f, 0.0f, 0.0f, 0.0f},
+     {3.69f, 1.2257f, -0.099537f, 0.018063f, -0.0026274f, 0.0f, 0.0f},
+     {3.19f, 1.2415f, -0.11231f, 0.026191f, -0.0064682f, 0.001191f, 0.0f},
+     {2.91f, 1.2508f, -0.12034f, 0.032131f, -0.010142f, 0.0029857f,
+      -0.00066667f}},
+    {{22.2f, 1.0050f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {5.65f, 1.1534f, -0.052806f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {3.74f, 1.2111f, -0.088313f, 0.011768f, 0.0f, 0.0f, 0.0f},
+     {3.11f, 1.2367f, -0.10815f, 0.023113f, -0.0046905f, 0.0f, 0.0f},
+     {2.80f, 1.2496f, -0.11921f, 0.031130f, -0.0093272f, 0.0025161f, 0.0f},
+     {2.62f, 1.2568f, -0.12573f, 0.036423f, -0.013132f, 0.0047484f,
+      -0.0015979f}},
+    {{15.8f, 1.0100f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {4.80f, 1.1640f, -0.057991f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {3.39f, 1.2192f, -0.094070f, 0.014608f, 0.0f, 0.0f, 0.0f},
+     {2.90f, 1.2422f, -0.11269f, 0.026140f, -0.0064054f, 0.0f, 0.0f},
+     {2.65f, 1.2534f, -0.12257f, 0.033755f, -0.011081f, 0.0036784f, 0.0f},
+     {2.51f, 1.2596f, -0.12825f, 0.038550f, -0.014763f, 0.0058619f,
+      -0.0024538f}},
+    {{9.16f, 1.0300f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {3.47f, 1.1876f, -0.072518f, 0.0f, 0.0f, 0.0f, 0.0f},
+     {2.91f, 1.2341f, -0.10569f, 0.022589f, 0.0f, 0.0f, 0.0f},
+     {2.61f, 1.2516f, -0.12085f, 0.032236f, -0.011459f, 0.0f, 0.0f},
+     {2.45f, 1.2596f, -0.12829f, 0.038533f, -0.014681f, 0.0072580f, 0.0f},
+     {2.36f, 1.2640f, -0.13239f, 0.042217f, -0.017803f, 0.0081959f,
+      -0.0051848f}},
+};
+
+int or_holberg(int fdorder, int maxrelerror, double hc[7]) {
+    if (fdorder != 2 && fdorder != 4 && fdorder != 6 && fdorder != 8 &&
+        fdorder != 10 && fdorder != 12)
+        return 1;
+    if (maxrelerror < 0 || maxrelerror > 4) return 1;
+    for (int i = 0; i <= 6; i++)
+        hc[i] = (double)HOLBERG[maxrelerror][fdorder / 2 - 1][i];
+    return 0;
+}
+
+/* A double printed as "%9.9f" and read back as a float literal: the way the
+ * reference hands HC, DT, DT2 and DH to its kernels (src/clprogram.c:323-348). */
+float or_literal9(double v) {
+    char buf[64];
+    snprintf(buf, sizeof(buf), "%9.9f", v);
+    return strtof(buf, NULL);
+}
+
+void or_kernel_hc(int fdorder, int maxrelerror, float hcf[7]) {
+    double hc[7] = {0};
+    or_holberg(fdorder, maxrelerror, hc);
+    hcf[0] = 0.0f;
+    for (int i = 1; i <= 6; i++) hcf[i] = or_literal9(hc[i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPML profiles: src/third_party/SOFI3D/CPML_COEFF.c:26-200                 */
+/* Arrays hold 2*nab floats and must be zero on entry (the reference        */
+/* allocates them zeroed, src/F.h:59-62).                                    */
+/* ------------------------------------------------------------------------ */
+int or_cpml_coeff(float NPOWER, float k_max_CPML, float FPML, float VPPML,
+                  float dh, float dt, int NAB, float *K_i, float *b_i,
+                  float *a_i, float *K_i_half, float *b_i_half,
+                  float *a_i_half) {
+    int state = 0;
+    const float npower = NPOWER;
+    const float alpha_max_PML = 2.0 * PI * (FPML / 2.0);
+    const float Rcoef = 0.0008;
+    const float a = 0.25, b = 0.75, c = 0.0;
+    float d0_i, position_norm, position_in_PML;
+    float *d_i = calloc(2 * NAB + 1, sizeof(float));
+    float *d_i_half = calloc(2 * NAB + 1, sizeof(float));
+    float *alpha_prime_i = calloc(2 * NAB + 1, sizeof(float));
+    float *alpha_prime_i_half = calloc(2 * NAB + 1, sizeof(float));
+
+    d0_i = -(npower + 1) * VPPML * log(Rcoef) / (2.0 * NAB * dh);
+    for (int i = 0; i <= NAB; i++) {
+        K_i[i] = 1.0;
+        position_in_PML = (NAB - i) * dh;
+        position_norm = position_in_PML / (NAB * dh);
+        d_i[i] = d0_i * (a * position_norm + b * pow(position_norm, npower) +
+                         c * pow(position_norm, (4)));
+        K_i[i] = 1.0 + (k_max_CPML - 1.0) * pow(position_norm, npower);
+        alpha_prime_i[i] = alpha_max_PML * (1.0 - position_norm);
+        if (alpha_prime_i[i] < 0.0) state = 1;
+        b_i[i] = exp(-(d_i[i] / K_i[i] + alpha_prime_i[i]) * dt);
+        if (fabsf(d_i[i]) > 1.0e-6)
+            a_i[i] = d_i[i] * (b_i[i] - 1.0) /
+                     (K_i[i] * (d_i[i] + K_i[i] * alpha_prime_i[i]));
+        else
+            a_i[i] = 0.0;
+        if (i <= NAB - 1) {
+            position_in_PML = (NAB - i + 0.5 - 1.0) * dh;
+            position_norm = position_in_PML / (NAB * dh);
+            int i1 = i;
+            K_i_half[i1] = 1.0;
+            d_i_half[i1] =
+                d0_i * (a * position_norm + b * pow(position_norm, npower) +
+                        c * pow(position_norm, (4)));
+            if (position_in_PML < 0.0) state = 1;
+            K_i_half[i1] = 1.0 + (k_max_CPML - 1.0) * pow(position_norm, npower);
+            alpha_prime_i_half[i1] = alpha_max_PML * (1.0 - position_norm);
+            if (alpha_prime_i_half[i1] < 0.0) state = 1;
+            b_i_half[i1] =
+                exp(-(d_i_half[i1] / K_i_half[i1] + alpha_prime_i_half[i1]) * dt);
+            if (fabs(d_i_half[i1]) > 1.0e-6)
+                a_i_half[i1] = d_i_half[i1] * (b_i_half[i1] - 1.0) /
+                               (K_i_half[i1] * (d_i_half[i1] +
+                                                K_i_half[i1] * alpha_prime_i_half[i1]));
+            /* mirror onto the far boundary */
+            int l = 2 * NAB - i - 1;
+            if (i > 0) {
+                K_i[l + 1] = K_i[i];
+                b_i[l + 1] = b_i[i];
+                if (fabs(d_i[i]) > 1.0e-6) a_i[l + 1] = a_i[i];
+            }
+            K_i_half[l] = K_i_half[i];
+            b_i_half[l] = b_i_half[i];
+            if (fabs(d_i[i]) > 1.0e-6) a_i_half[l] = a_i_half[i];
+        }
+    }
+    free(d_i);
+    free(d_i_half);
+    free(alpha_prime_i);
+    free(alpha_prime_i_half);
+    return state;
+}
+
+/* Cerjan taper: src/assign_modeling_case.c:571-584 */
+void or_taper(int nab, float abpc, float *taper) {
+    float amp = 1 - abpc / 100;
+    float a = sqrt(-log(amp) / ((nab - 1) * (nab - 1)));
+    for (int i = 1; i <= nab; i++)
+        taper[i - 1] = exp(-(a * a * (nab - i) * (nab - i)));
+}
+
+/* Relaxation constants: src/assign_modeling_case.c:536-548 */
+void or_eta(int L, const float *FL, float dt, float *eta) {
+    for (int l = 0; l < L; l++) eta[l] = (2.0 * PI * FL[l]) * dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Parameter averaging: src/assign_modeling_case.c:50-301                    */
+/* ------------------------------------------------------------------------ */
+static void dims3(const or_csts *c, int *NZ, int *NY, int *NX) {
+    if (c->ND == 3) {
+        *NZ = c->N[0];
+        *NY = c->N[1];
+        *NX = c->N[2];
+    } else {
+        *NZ = c->N[0];
+        *NY = 1;
+        *NX = c->N[1];
+    }
+}
+
+/* ave_arithmetic1 (:50-99): pout = 0.5*(p + p[+dir]); last plane copied */
+static void ave_arith1(const float *pin, float *pout, int NZ, int NY, int NX,
+                       const int dir[3]) {
+    int NX0 = 0, NY0 = 0, NZ0 = 0;
+    for (int k = 0; k < NZ - dir[0]; k++)
+        for (int j = 0; j < NY - dir[1]; j++)
+            for (int i = 0; i < NX - dir[2]; i++) {
+                long i1 = (long)i * NY * NZ + (long)j * NZ + k;
+                long i2 = (long)(i + dir[2]) * NY * NZ + (long)(j + dir[1]) * NZ +
+                          (k + dir[0]);
+                pout[i1] = 0.5 * (pin[i1] + pin[i2]);
+            }
+    if (dir[2] == 1) NX0 = NX - 1;
+    else if (dir[1] == 1) NY0 = NY - 1;
+    if (dir[0] == 1) NZ0 = NZ - 1;
+    for (int k = NZ0; k < NZ; k++)
+        for (int j = NY0; j < NY; j++)
+            for (int i = NX0; i < NX; i++) {
+                long i1 = (long)i * NY * NZ + (long)j * NZ + k;
+                pout[i1] = pin[i1];
+            }
+}
+
+/* Edge handling shared by ave_arithmetic2 (:100-173) and ave_harmonic
+ * (:225-301).  The start indices of the second copy are not reset after
+ * the first one, so only the corner line of the second edge is copied and
+ * the rest of that edge keeps the zero the buffer was allocated with. */
+static void ave_edges2(const float *pin, float *pout, int NZ, int NY, int NX,
+                       const int dir[2][3]) {
+    int NX0 = 0, NY0 = 0, NZ0 = 0;
+    for (int e = 0; e < 2; e++) {
+        if (dir[e][2] == 1) NX0 = NX - 1;
+        else if (dir[e][1] == 1) NY0 = NY - 1;
+        if (dir[e][0] == 1) NZ0 = NZ - 1;
+        for (int k = NZ0; k < NZ; k++)
+            for (int j = NY0; j < NY; j++)
+                for (int i = NX0; i < NX; i++) {
+                    long i1 = (long)i * NY * NZ + (long)j * NZ + k;
+                    pout[i1] = pin[i1];
+                }
+    }
+}
+
+static void ave_arith2(const float *pin, float *pout, int NZ, int NY, int NX,
+                       const int dir[2][3]) {
+    for (int k = 0; k < NZ - dir[0][0] - dir[1][0]; k++)
+        for (int j = 0; j < NY - dir[0][1] - dir[1][1]; j++)
+            for (int i = 0; i < NX - dir[0][2] - dir[1][2]; i++) {
+                long i1 = (long)i * NY * NZ + (long)j * NZ + k;
+                long i2 = (long)(i + dir[0][2]) * NY * NZ +
+                          (long)(j + dir[0][1]) * NZ + (k + dir[0][0]);
+                long i3 = (long)(i + dir[1][2]) * NY * NZ +
+                          (long)(j + dir[1][1]) * NZ + (k + dir[1][0]);
+                long i4 = (long)(i + dir[0][2] + dir[1][2]) * NY * NZ +
+                          (long)(j + dir[0][1] + dir[1][1]) * NZ +
+                          (k + dir[0][0] + dir[1][0]);
+                pout[i1] = 0.25 * (pin[i1] + pin[i2] + pin[i3] + pin[i4]);
+            }
+    ave_edges2(pin, pout, NZ, NY, NX, dir);
+}
+
+static void ave_harm(const float *pin, float *pout, int NZ, int NY, int NX,
+                     const int dir[2][3]) {
+    for (int k = 0; k < NZ - dir[0][0] - dir[1][0]; k++)
+        for (int j = 0; j < NY - dir[0][1] - dir[1][1]; j++)
+            for (int i = 0; i < NX - dir[0][2] - dir[1][2]; i++) {
+                long i1 = (long)i * NY * NZ + (long)j * NZ + k;
+                long i2 = (long)(i + dir[0][2]) * NY * NZ +
+                          (long)(j + dir[0][1]) * NZ + (k + dir[0][0]);
+                long i3 = (long)(i + dir[1][2]) * NY * NZ +
+                          (long)(j + dir[1][1]) * NZ + (k + dir[1][0]);
+                long i4 = (long)(i + dir[0][2] + dir[1][2]) * NY * NZ +
+                          (long)(j + dir[0][1] + dir[1][1]) * NZ +
+                          (k + dir[0][0] + dir[1][0]);
+                pout[i1] = 4.0 / (1.0 / pin[i1] + 1.0 / pin[i2] +
+                                  1.0 / pin[i3] + 1.0 / pin[i4]);
+            }
+    ave_edges2(pin, pout, NZ, NY, NX, dir);
+}
+
+/* powf(x,2) of the reference is emitted by gcc as the correctly rounded x*x */
+static inline float sq(float x) { return x * x; }
+
+/* ------------------------------------------------------------------------ */
+/* Parameter transforms, in registration order                               */
+/* (src/assign_modeling_case.c:1001-1040; transforms :305-535)               */
+/* ------------------------------------------------------------------------ */
+int or_transform(const or_csts *c, const float *p0, const float *p1,
+                 const float *rho_in, const float *taup_in,
+                 const float *taus_in, float *M, float *mu, float *rho,
+                 float *rip, float *rjp, float *rkp, float *muipkp,
+                 float *muipjp, float *mujpkp, float *taup, float *taus,
+                 float *tausipkp, float *tausipjp, float *tausjpkp) {
+    int NZ, NY, NX;
+    dims3(c, &NZ, &NY, &NX);
+    long n = (long)NZ * NY * NX;
+    int L = c->L;
+    int elastic = (c->ND == 2 || c->ND == 3);
+    float eta[OR_MAXL];
+    or_eta(L, c->FL, c->dt, eta);
+    float *tp = taup, *ts = taus;
+    float *tmp_tp = NULL, *tmp_ts = NULL;
+    if (L > 0) {
+        if (!tp) tp = tmp_tp = malloc(n * sizeof(float));
+        if (elastic && !ts) ts = tmp_ts = malloc(n * sizeof(float));
+        memcpy(tp, taup_in, n * sizeof(float));
+        if (elastic) memcpy(ts, taus_in, n * sizeof(float));
+    }
+    float pts[OR_MAXL];
+    for (int l = 0; l < L; l++) pts[l] = c->dt / eta[l];
+    float ws = 2.0 * PI * c->f0;
+
+    /* M (:380-449) */
+    for (long i = 0; i < n; i++) {
+        float v = p0[i];
+        if (c->param_type == 0) M[i] = sq(v) * rho_in[i] * c->dt / c->dh;
+        if (c->param_type == 1) M[i] = v * c->dt / c->dh;
+        else if (c->param_type == 2)
+            M[i] = sq(v / rho_in[i]) * rho_in[i] * c->dt / c->dh;
+        else if (c->param_type == 3) {
+            float thisvp = v - tp[i];
+            float thistaup = tp[i] / (v - tp[i]);
+            M[i] = sq(thisvp) * rho_in[i] * c->dt / c->dh;
+            tp[i] = thistaup;
+        }
+    }
+    if (L > 0) {
+        for (long i = 0; i < n; i++) {
+            float sumpi = 0.0;
+            for (int l = 0; l < L; l++)
+                sumpi += ((ws * ws * pts[l] * pts[l] * tp[i]) /
+                          (1.0 + ws * ws * pts[l] * pts[l]));
+            M[i] = M[i] / (1.0 + sumpi);
+        }
+    }
+    /* mu (:305-379) */
+    if (elastic) {
+        for (long i = 0; i < n; i++) {
+            float v = p1[i];
+            if (c->param_type == 0) mu[i] = sq(v) * rho_in[i] * c->dt / c->dh;
+            else if (c->param_type == 1) mu[i] = v * c->dt / c->dh;
+            else if (c->param_type == 2)
+                mu[i] = sq(v / rho_in[i]) * rho_in[i] * c->dt / c->dh;
+            else if (c->param_type == 3) {
+                float thisvs, thistaus;
+                if (v > 0) {
+                    thisvs = v - ts[i];
+                    thistaus = ts[i] / (v - ts[i]);
+                } else {
+                    thistaus = 0;
+                    thisvs = 0;
+                }
+                mu[i] = sq(thisvs) * rho_in[i] * c->dt / c->dh;
+                ts[i] = thistaus;
+            }
+        }
+        if (L > 0) {
+            float sumu = 0.0;
+            for (int l = 0; l < L; l++)
+                sumu += ((ws * ws * pts[l] * pts[l]) /
+                         (1.0 + ws * ws * pts[l] * pts[l]));
+            for (long i = 0; i < n; i++) mu[i] = mu[i] / (1.0 + sumu * ts[i]);
+        }
+    }
+    /* rho (:451-462), par_scale = 0 in FP32 */
+    for (long i = 0; i < n; i++)
+        rho[i] = 1.0 / rho_in[i] * c->dt / c->dh * 1.0; /* powf(2,-par_scale), par_scale=0 */
+    /* buoyancy averages (:464-487) */
+    {
+        int dx[3] = {0, 0, 1}, dy[3] = {0, 1, 0}, dz[3] = {1, 0, 0};
+        ave_arith1(rho, rip, NZ, NY, NX, dx);
+        if (c->ND == 3) ave_arith1(rho, rjp, NZ, NY, NX, dy);
+        ave_arith1(rho, rkp, NZ, NY, NX, dz);
+    }
+    if (elastic) {
+        int dxz[2][3] = {{0, 0, 1}, {1, 0, 0}};
+        int dyz[2][3] = {{0, 1, 0}, {1, 0, 0}};
+        int dxy[2][3] = {{0, 0, 1}, {0, 1, 0}};
+        memset(muipkp, 0, n * sizeof(float));
+        ave_harm(mu, muipkp, NZ, NY, NX, dxz);
+        if (c->ND == 3) {
+            memset(muipjp, 0, n * sizeof(float));
+            memset(mujpkp, 0, n * sizeof(float));
+            ave_harm(mu, muipjp, NZ, NY, NX, dxy);
+            ave_harm(mu, mujpkp, NZ, NY, NX, dyz);
+        }
+        if (L > 0) {
+            memset(tausipkp, 0, n * sizeof(float));
+            ave_arith2(ts, tausipkp, NZ, NY, NX, dxz);
+            if (c->ND == 3) {
+                memset(tausipjp, 0, n * sizeof(float));
+                memset(tausjpkp, 0, n * sizeof(float));
+                ave_arith2(ts, tausipjp, NZ, NY, NX, dxy);
+                ave_arith2(ts, tausjpkp, NZ, NY, NX, dyz);
+            }
+        }
+    }
+    free(tmp_tp);
+    free(tmp_ts);
+    return 0;
+}
+
+/* src/assign_modeling_case.c:730-827 */
+int or_check_stability(const or_csts *c, const float *M, const float *mu,
+                       const float *rho, float *dtstable_out) {
+    int NZ, NY, NX;
+    dims3(c, &NZ, &NY, &NX);
+    long n = (long)NZ * NY * NX;
+    float vpmax = 0, vsmax = 0, vpmin = 99999, vsmin = 99999;
+    float vmax, g, gamma, dtstable;
+    (void)vpmin;
+    (void)vsmin;
+    if (mu) {
+        for (long i = 0; i < n; i++) {
+            float thisvs = sqrt(mu[i] * rho[i]) / c->dt * c->dh;
+            if (vsmax < thisvs) vsmax = thisvs;
+            if (vsmin > thisvs && thisvs > 0.1) vsmin = thisvs;
+        }
+    }
+    if (M) {
+        for (long i = 0; i < n; i++) {
+            float thisvp = sqrt(M[i] * rho[i]) / c->dt * c->dh;
+            if (vpmax < thisvp) vpmax = thisvp;
+            if (vpmin > thisvp && thisvp > 0.1) vpmin = thisvp;
+        }
+    }
+    vmax = (vpmax == 0) ? vsmax : vpmax;
+    switch (c->FDORDER) {
+        case 2: gamma = 1.0; break;
+        case 4: gamma = 1.184614; break;
+        case 6: gamma = 1.283482; break;
+        case 8: gamma = 1.345927; break;
+        case 10: gamma = 1.38766; break;
+        case 12: gamma = 1.417065; break;
+        default: gamma = 1.0; break;
+    }
+    (void)g;
+    if (c->ND == 3) dtstable = c->dh / (gamma * sqrt(3.0) * vmax);
+    else dtstable = c->dh / (gamma * sqrt(2.0) * vmax);
+    if (dtstable_out) *dtstable_out = dtstable;
+    return c->dt > dtstable;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Output variables (src/assign_modeling_case.c:1244-1326, SeisCL.py        */
+/* to_load_names :376-408).                                                  */
+/* ------------------------------------------------------------------------ */
+static const char *OUT2[4][6] = {{"vx", "vz"},
+                                 {"p"},
+                                 {"sxx", "szz", "sxz"},
+                                 {"vx", "vz", "sxx", "szz", "sxz", "p"}};
+static const char *OUT3[4][10] = {
+    {"vx", "vy", "vz"},
+    {"p"},
+    {"sxx", "syy", "szz", "sxz", "sxy", "syz"},
+    {"vx", "vy", "vz", "sxx", "syy", "szz", "sxz", "sxy", "syz", "p"}};
+static const char *OUTAC[4][3] = {{"vx", "vz"}, {"p"}, {NULL}, {"vx", "vz", "p"}};
+static const int NOUT2[4] = {2, 1, 3, 6}, NOUT3[4] = {3, 1, 6, 10},
+                 NOUTAC[4] = {2, 1, 0, 3};
+
+int or_nout(const or_csts *c) {
+    if (c->seisout < 1 || c->seisout > 4) return 0;
+    if (c->ND == 3) return NOUT3[c->seisout - 1];
+    if (c->ND == 22) return NOUTAC[c->seisout - 1];
+    return NOUT2[c->seisout - 1];
+}
+const char *or_outname(const or_csts *c, int i) {
+    if (i < 0 || i >= or_nout(c)) return NULL;
+    if (c->ND == 3) return OUT3[c->seisout - 1][i];
+    if (c->ND == 22) return OUTAC[c->seisout - 1][i];
+    return OUT2[c->seisout - 1][i];
+}
+
+/* Wavefield variables in registration order (src/assign_modeling_case.c:1044-1070) */
+static const char *VARS2[] = {"vx", "vz", "sxx", "szz", "sxz"};
+static const char *VARS3[] = {"vx", "vy", "vz", "sxx", "szz", "sxz", "sxy", "syz", "syy"};
+static const char *VARSAC[] = {"vx", "vz", "p"};
+int or_nvars(const or_csts *c) { return c->ND == 3 ? 9 : (c->ND == 22 ? 3 : 5); }
+const char *or_varname(const or_csts *c, int i) {
+    if (i < 0 || i >= or_nvars(c)) return NULL;
+    return c->ND == 3 ? VARS3[i] : (c->ND == 22 ? VARSAC[i] : VARS2[i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Simulation state                                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const or_csts *c;
+    int FDOH, NAB, L, ND;
+    int NZ, NY, NX;        /* unpadded */
+    int NZp, NYp, NXp;     /* padded (NY=NYp=1 in 2D) */
+    long nv, np;           /* padded / unpadded cell counts */
+    float hc[7], DT, DT2, DH;
+    float eta[OR_MAXL];
+    /* wavefields */
+    float *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *syz, *sxz, *p;
+    float *rxx, *ryy, *rzz, *rxy, *ryz, *rxz, *rp;
+    /* parameters */
+    float *M, *mu, *rho, *rip, *rjp, *rkp, *muipkp, *muipjp, *mujpkp;
+    float *taup, *taus, *tausipkp, *tausipjp, *tausjpkp;
+    /* CPML */
+    float K_x[256], a_x[256], b_x[256], K_x_half[256], a_x_half[256],
+        b_x_half[256];
+    float K_y[256], a_y[256], b_y[256], K_y_half[256], a_y_half[256],
+        b_y_half[256];
+    float K_z[256], a_z[256], b_z[256], K_z_half[256], a_z_half[256],
+        b_z_half[256];
+    float taper[256];
+    float *psi_sxx_x, *psi_sxy_x, *psi_sxz_x, *psi_sxy_y, *psi_syy_y,
+        *psi_syz_y, *psi_sxz_z, *psi_syz_z, *psi_szz_z;
+    float *psi_vx_x, *psi_vy_x, *psi_vz_x, *psi_vx_y, *psi_vy_y, *psi_vz_y,
+        *psi_vx_z, *psi_vy_z, *psi_vz_z;
+    float *psi_p_x, *psi_p_z;
+} sim;
+
+static float *zalloc(long n) { return n > 0 ? calloc(n, sizeof(float)) : NULL; }
+
+/* stencils of src/header_FD.cl:87-265, summed left to right */
+static inline float Dp(const float *v, long i, long s, const float *hc, int fdoh) {
+    float acc = hc[1] * (v[i + s] - v[i]);
+    for (int k = 2; k <= fdoh; k++)
+        acc = acc + hc[k] * (v[i + k * s] - v[i - (k - 1) * s]);
+    return acc;
+}
+static inline float Dm(const float *v, long i, long s, const float *hc, int fdoh) {
+    float acc = hc[1] * (v[i] - v[i - s]);
+    for (int k = 2; k <= fdoh; k++)
+        acc = acc + hc[k] * (v[i + (k - 1) * s] - v[i - k * s]);
+    return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2D P-SV velocity update: src/update_v2D.cl:32-246                         */
+/* ------------------------------------------------------------------------ */
+static void update_v2D(sim *S) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NX = S->NXp;
+    const int cpml = S->c->abs_type == 1, tap = S->c->abs_type == 2,
+              fs = S->c->freesurf;
+    const float *hc = S->hc;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+            long indp = (long)(gidx - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+            long indv = (long)gidx * NZ + gidz;
+            float sxx_x = Dp(S->sxx, indv, NZ, hc, FDOH);
+            float szz_z = Dp(S->szz, indv, 1, hc, FDOH);
+            float sxz_z = Dm(S->sxz, indv, 1, hc, FDOH);
+            float sxz_x = Dm(S->sxz, indv, NZ, hc, FDOH);
+            if (cpml) {
+                int i, k, ind;
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    i = gidx - FDOH;
+                    k = gidz - NZ + NAB + FDOH + NAB;
+                    ind = 2 * NAB - 1 - k;
+                    float *ps = &S->psi_sxz_z[(long)i * (2 * NAB) + k];
+                    *ps = S->b_z[ind + 1] * *ps + S->a_z[ind + 1] * sxz_z;
+                    sxz_z = sxz_z / S->K_z[ind + 1] + *ps;
+                    float *pz = &S->psi_szz_z[(long)i * (2 * NAB) + k];
+                    *pz = S->b_z_half[ind] * *pz + S->a_z_half[ind] * szz_z;
+                    szz_z = szz_z / S->K_z_half[ind] + *pz;
+                } else if (!fs && gidz - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *ps = &S->psi_sxz_z[(long)i * (2 * NAB) + k];
+                    *ps = S->b_z[k] * *ps + S->a_z[k] * sxz_z;
+                    sxz_z = sxz_z / S->K_z[k] + *ps;
+                    float *pz = &S->psi_szz_z[(long)i * (2 * NAB) + k];
+                    *pz = S->b_z_half[k] * *pz + S->a_z_half[k] * szz_z;
+                    szz_z = szz_z / S->K_z_half[k] + *pz;
+                }
+                if (gidx - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *px = &S->psi_sxx_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *px = S->b_x_half[i] * *px + S->a_x_half[i] * sxx_x;
+                    sxx_x = sxx_x / S->K_x_half[i] + *px;
+                    float *pxz = &S->psi_sxz_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *pxz = S->b_x[i] * *pxz + S->a_x[i] * sxz_x;
+                    sxz_x = sxz_x / S->K_x[i] + *pxz;
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    i = gidx - NX + NAB + FDOH + NAB;
+                    k = gidz - FDOH;
+                    ind = 2 * NAB - 1 - i;
+                    float *px = &S->psi_sxx_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *px = S->b_x_half[ind] * *px + S->a_x_half[ind] * sxx_x;
+                    sxx_x = sxx_x / S->K_x_half[ind] + *px;
+                    float *pxz = &S->psi_sxz_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *pxz = S->b_x[ind + 1] * *pxz + S->a_x[ind + 1] * sxz_x;
+                    sxz_x = sxz_x / S->K_x[ind + 1] + *pxz;
+                }
+            }
+            S->vx[indv] += ((sxx_x + sxz_z) * S->rip[indp]);
+            S->vz[indv] += ((szz_z + sxz_x) * S->rkp[indp]);
+            if (tap) {
+                if (!fs && gidz - FDOH < NAB) {
+                    S->vx[indv] *= S->taper[gidz - FDOH];
+                    S->vz[indv] *= S->taper[gidz - FDOH];
+                }
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    S->vx[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                    S->vz[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                }
+                if (gidx - FDOH < NAB) {
+                    S->vx[indv] *= S->taper[gidx - FDOH];
+                    S->vz[indv] *= S->taper[gidx - FDOH];
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    S->vx[indv] *= S->taper[NX - FDOH - gidx - 1];
+                    S->vz[indv] *= S->taper[NX - FDOH - gidx - 1];
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2D P-SV stress update: src/update_s2D.cl:30-326                           */
+/* ------------------------------------------------------------------------ */
+static void update_s2D(sim *S) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NX = S->NXp,
+              LVE = S->L;
+    const int cpml = S->c->abs_type == 1, tap = S->c->abs_type == 2,
+              fs = S->c->freesurf;
+    const float *hc = S->hc, DT = S->DT, DT2 = S->DT2;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+            long indp = (long)(gidx - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+            long indv = (long)gidx * NZ + gidz;
+            float vxx = Dm(S->vx, indv, NZ, hc, FDOH);
+            float vxz = Dp(S->vx, indv, 1, hc, FDOH);
+            float vzz = Dm(S->vz, indv, 1, hc, FDOH);
+            float vzx = Dp(S->vz, indv, NZ, hc, FDOH);
+            if (cpml) {
+                int i, k, ind;
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    i = gidx - FDOH;
+                    k = gidz - NZ + NAB + FDOH + NAB;
+                    ind = 2 * NAB - 1 - k;
+                    float *p1 = &S->psi_vx_z[(long)i * (2 * NAB) + k];
+                    *p1 = S->b_z_half[ind] * *p1 + S->a_z_half[ind] * vxz;
+                    vxz = vxz / S->K_z_half[ind] + *p1;
+                    float *p2 = &S->psi_vz_z[(long)i * (2 * NAB) + k];
+                    *p2 = S->b_z[ind + 1] * *p2 + S->a_z[ind + 1] * vzz;
+                    vzz = vzz / S->K_z[ind + 1] + *p2;
+                } else if (!fs && gidz - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *p1 = &S->psi_vx_z[(long)i * (2 * NAB) + k];
+                    *p1 = S->b_z_half[k] * *p1 + S->a_z_half[k] * vxz;
+                    vxz = vxz / S->K_z_half[k] + *p1;
+                    float *p2 = &S->psi_vz_z[(long)i * (2 * NAB) + k];
+                    *p2 = S->b_z[k] * *p2 + S->a_z[k] * vzz;
+                    vzz = vzz / S->K_z[k] + *p2;
+                }
+                if (gidx - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *p1 = &S->psi_vx_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *p1 = S->b_x[i] * *p1 + S->a_x[i] * vxx;
+                    vxx = vxx / S->K_x[i] + *p1;
+                    float *p2 = &S->psi_vz_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *p2 = S->b_x_half[i] * *p2 + S->a_x_half[i] * vzx;
+                    vzx = vzx / S->K_x_half[i] + *p2;
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    i = gidx - NX + NAB + FDOH + NAB;
+                    k = gidz - FDOH;
+                    ind = 2 * NAB - 1 - i;
+                    float *p1 = &S->psi_vx_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *p1 = S->b_x[ind + 1] * *p1 + S->a_x[ind + 1] * vxx;
+                    vxx = vxx / S->K_x[ind + 1] + *p1;
+                    float *p2 = &S->psi_vz_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *p2 = S->b_x_half[ind] * *p2 + S->a_x_half[ind] * vzx;
+                    vzx = vzx / S->K_x_half[ind] + *p2;
+                }
+            }
+            if (LVE == 0) {
+                float fipkp = S->muipkp[indp];
+                float f = 2.0 * S->mu[indp];
+                float g = S->M[indp];
+                S->sxz[indv] += (fipkp * (vxz + vzx));
+                S->sxx[indv] += (g * (vxx + vzz)) - (f * vzz);
+                S->szz[indv] += (g * (vxx + vzz)) - (f * vxx);
+            } else {
+                float lM = S->M[indp], lmu = S->mu[indp], lmuipkp = S->muipkp[indp];
+                float ltaup = S->taup[indp], ltaus = S->taus[indp],
+                      ltausipkp = S->tausipkp[indp];
+                float fipkp = lmuipkp * (1.0 + (float)LVE * ltausipkp);
+                float g = lM * (1.0 + (float)LVE * ltaup);
+                float f = 2.0 * lmu * (1.0 + (float)LVE * ltaus);
+                float dipkp = lmuipkp * ltausipkp / DT;
+                float d = 2.0 * lmu * ltaus / DT;
+                float e = lM * ltaup / DT;
+                float sumrxz = 0, sumrxx = 0, sumrzz = 0;
+                for (int l = 0; l < LVE; l++) {
+                    long indr = (long)l * NX * NZ + indv;
+                    sumrxz += S->rxz[indr];
+                    sumrxx += S->rxx[indr];
+                    sumrzz += S->rzz[indr];
+                }
+                float lsxz = (fipkp * (vxz + vzx)) + (DT2 * sumrxz);
+                float lsxx = ((g * (vxx + vzz)) - (f * vzz)) + (DT2 * sumrxx);
+                float lszz = ((g * (vxx + vzz)) - (f * vxx)) + (DT2 * sumrzz);
+                sumrxz = sumrxx = sumrzz = 0;
+                for (int l = 0; l < LVE; l++) {
+                    float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
+                    float cc = 1.0 - (S->eta[l] * 0.5);
+                    long indr = (long)l * NX * NZ + indv;
+                    S->rxz[indr] = b * (S->rxz[indr] * cc - S->eta[l] * (dipkp * (vxz + vzx)));
+                    S->rxx[indr] = b * (S->rxx[indr] * cc -
+                                        S->eta[l] * ((e * (vxx + vzz)) - (d * vzz)));
+                    S->rzz[indr] = b * (S->rzz[indr] * cc -
+                                        S->eta[l] * ((e * (vxx + vzz)) - (d * vxx)));
+                    sumrxz += S->rxz[indr];
+                    sumrxx += S->rxx[indr];
+                    sumrzz += S->rzz[indr];
+                }
+                S->sxz[indv] += lsxz + (DT2 * sumrxz);
+                S->sxx[indv] += lsxx + (DT2 * sumrxx);
+                S->szz[indv] += lszz + (DT2 * sumrzz);
+            }
+            if (tap) {
+                float tp = 1.0f;
+                int any = 0;
+                (void)tp;
+                (void)any;
+                if (!fs && gidz - FDOH < NAB) {
+                    S->sxx[indv] *= S->taper[gidz - FDOH];
+                    S->szz[indv] *= S->taper[gidz - FDOH];
+                    S->sxz[indv] *= S->taper[gidz - FDOH];
+                }
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    S->sxx[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                    S->szz[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                    S->sxz[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                }
+                if (gidx - FDOH < NAB) {
+                    S->sxx[indv] *= S->taper[gidx - FDOH];
+                    S->szz[indv] *= S->taper[gidx - FDOH];
+                    S->sxz[indv] *= S->taper[gidx - FDOH];
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    S->sxx[indv] *= S->taper[NX - FDOH - gidx - 1];
+                    S->szz[indv] *= S->taper[NX - FDOH - gidx - 1];
+                    S->sxz[indv] *= S->taper[NX - FDOH - gidx - 1];
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2D acoustic: src/update_v2D_acc.cl:80-325, src/update_s2D_acc.cl:79-402   */
+/* ------------------------------------------------------------------------ */
+static void update_v2D_acc(sim *S) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NX = S->NXp;
+    const int cpml = S->c->abs_type == 1, tap = S->c->abs_type == 2,
+              fs = S->c->freesurf;
+    const float *hc = S->hc;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+            long indp = (long)(gidx - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+            long indv = (long)gidx * NZ + gidz;
+            float p_x = Dp(S->p, indv, NZ, hc, FDOH);
+            float p_z = Dp(S->p, indv, 1, hc, FDOH);
+            if (cpml) {
+                int i, k, ind;
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    i = gidx - FDOH;
+                    k = gidz - NZ + NAB + FDOH + NAB;
+                    ind = 2 * NAB - 1 - k;
+                    float *pp = &S->psi_p_z[(long)i * (2 * NAB) + k];
+                    *pp = S->b_z_half[ind] * *pp + S->a_z_half[ind] * p_z;
+                    p_z = p_z / S->K_z_half[ind] + *pp;
+                } else if (!fs && gidz - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *pp = &S->psi_p_z[(long)i * (2 * NAB) + k];
+                    *pp = S->b_z_half[k] * *pp + S->a_z_half[k] * p_z;
+                    p_z = p_z / S->K_z_half[k] + *pp;
+                }
+                if (gidx - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *pp = &S->psi_p_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *pp = S->b_x_half[i] * *pp + S->a_x_half[i] * p_x;
+                    p_x = p_x / S->K_x_half[i] + *pp;
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    i = gidx - NX + NAB + FDOH + NAB;
+                    k = gidz - FDOH;
+                    ind = 2 * NAB - 1 - i;
+                    float *pp = &S->psi_p_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *pp = S->b_x_half[ind] * *pp + S->a_x_half[ind] * p_x;
+                    p_x = p_x / S->K_x_half[ind] + *pp;
+                }
+            }
+            S->vx[indv] += (p_x * S->rip[indp]);
+            S->vz[indv] += (p_z * S->rkp[indp]);
+            if (tap) {
+                /* the acoustic kernel tapers the top edge regardless of the
+                 * free-surface flag (src/update_v2D_acc.cl:297-300) */
+                if (gidz - FDOH < NAB) {
+                    S->vx[indv] *= S->taper[gidz - FDOH];
+                    S->vz[indv] *= S->taper[gidz - FDOH];
+                }
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    S->vx[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                    S->vz[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                }
+                if (gidx - FDOH < NAB) {
+                    S->vx[indv] *= S->taper[gidx - FDOH];
+                    S->vz[indv] *= S->taper[gidx - FDOH];
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    S->vx[indv] *= S->taper[NX - FDOH - gidx - 1];
+                    S->vz[indv] *= S->taper[NX - FDOH - gidx - 1];
+                }
+            }
+        }
+    }
+}
+
+static void update_s2D_acc(sim *S) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NX = S->NXp,
+              LVE = S->L;
+    const int cpml = S->c->abs_type == 1, tap = S->c->abs_type == 2,
+              fs = S->c->freesurf;
+    const float *hc = S->hc, DT2 = S->DT2;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+            long indp = (long)(gidx - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+            long indv = (long)gidx * NZ + gidz;
+            float vxx = Dm(S->vx, indv, NZ, hc, FDOH);
+            float vzz = Dm(S->vz, indv, 1, hc, FDOH);
+            if (cpml) {
+                int i, k, ind;
+                if (gidz > NZ - NAB - FDOH - 1) {
+                    i = gidx - FDOH;
+                    k = gidz - NZ + NAB + FDOH + NAB;
+                    ind = 2 * NAB - 1 - k;
+                    float *pp = &S->psi_vz_z[(long)i * (2 * NAB) + k];
+                    *pp = S->b_z[ind + 1] * *pp + S->a_z[ind + 1] * vzz;
+                    vzz = vzz / S->K_z[ind + 1] + *pp;
+                } else if (!fs && gidz - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *pp = &S->psi_vz_z[(long)i * (2 * NAB) + k];
+                    *pp = S->b_z[k] * *pp + S->a_z[k] * vzz;
+                    vzz = vzz / S->K_z[k] + *pp;
+                }
+                if (gidx - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    k = gidz - FDOH;
+                    float *pp = &S->psi_vx_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *pp = S->b_x[i] * *pp + S->a_x[i] * vxx;
+                    vxx = vxx / S->K_x[i] + *pp;
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    i = gidx - NX + NAB + FDOH + NAB;
+                    k = gidz - FDOH;
+                    ind = 2 * NAB - 1 - i;
+                    float *pp = &S->psi_vx_x[(long)i * (NZ - 2 * FDOH) + k];
+                    *pp = S->b_x[ind + 1] * *pp + S->a_x[ind + 1] * vxx;
+                    vxx = vxx / S->K_x[ind + 1] + *pp;
+                }
+            }
+            if (LVE == 0) {
+                float g = S->M[indp];
+                S->p[indv] += (g * (vxx + vzz));
+            } else {
+                float lM = S->M[indp], ltaup = S->taup[indp];
+                float g = lM * (1.0 + (float)LVE * ltaup);
+                float e = lM * ltaup; /* no 1/DT here, unlike the elastic kernel */
+                float sumrp = 0;
+                for (int l = 0; l < LVE; l++) sumrp += S->rp[(long)l * NX * NZ + indv];
+                float lp = (g * (vxx + vzz)) + (DT2 * sumrp);
+                sumrp = 0;
+                for (int l = 0; l < LVE; l++) {
+                    float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
+                    float cc = 1.0 - (S->eta[l] * 0.5);
+                    long indr = (long)l * NX * NZ + indv;
+                    S->rp[indr] = b * (S->rp[indr] * cc - S->eta[l] * e * (vxx + vzz));
+                    sumrp += S->rp[indr];
+                }
+                S->p[indv] += lp + (DT2 * sumrp);
+            }
+            if (tap) {
+                if (gidz - FDOH < NAB) S->p[indv] *= S->taper[gidz - FDOH];
+                if (gidz > NZ - NAB - FDOH - 1) S->p[indv] *= S->taper[NZ - FDOH - gidz - 1];
+                if (gidx - FDOH < NAB) S->p[indv] *= S->taper[gidx - FDOH];
+                if (gidx > NX - NAB - FDOH - 1) S->p[indv] *= S->taper[NX - FDOH - gidx - 1];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D velocity update: src/update_v3D.cl:34-369                              */
+/* ------------------------------------------------------------------------ */
+static void update_v3D(sim *S) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
+    const int cpml = S->c->abs_type == 1, tap = S->c->abs_type == 2,
+              fs = S->c->freesurf;
+    const float *hc = S->hc;
+    const long sy = NZ, sx = (long)NY * NZ;
+    const long nzi = NZ - 2 * FDOH, nyi = NY - 2 * FDOH;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidy = FDOH; gidy < NY - FDOH; gidy++) {
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - FDOH) * nzi +
+                            (gidz - FDOH);
+                long indv = (long)gidx * sx + (long)gidy * sy + gidz;
+                float szz_z = Dp(S->szz, indv, 1, hc, FDOH);
+                float sxx_x = Dp(S->sxx, indv, sx, hc, FDOH);
+                float sxz_x = Dm(S->sxz, indv, sx, hc, FDOH);
+                float sxz_z = Dm(S->sxz, indv, 1, hc, FDOH);
+                float syz_y = Dm(S->syz, indv, sy, hc, FDOH);
+                float syz_z = Dm(S->syz, indv, 1, hc, FDOH);
+                float syy_y = Dp(S->syy, indv, sy, hc, FDOH);
+                float sxy_x = Dm(S->sxy, indv, sx, hc, FDOH);
+                float sxy_y = Dm(S->sxy, indv, sy, hc, FDOH);
+                if (cpml) {
+                    int i, j, k, indm;
+#define PZ(a) a[(long)i * nyi * (2 * NAB) + (long)j * (2 * NAB) + k]
+#define PY(a) a[(long)i * (2 * NAB) * nzi + (long)j * nzi + k]
+#define PX(a) a[(long)i * nyi * nzi + (long)j * nzi + k]
+                    if (gidz > NZ - NAB - FDOH - 1) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - NZ + NAB + FDOH + NAB;
+                        indm = 2 * NAB - 1 - k;
+                        PZ(S->psi_sxz_z) = S->b_z[indm + 1] * PZ(S->psi_sxz_z) + S->a_z[indm + 1] * sxz_z;
+                        sxz_z = sxz_z / S->K_z[indm + 1] + PZ(S->psi_sxz_z);
+                        PZ(S->psi_syz_z) = S->b_z[indm + 1] * PZ(S->psi_syz_z) + S->a_z[indm + 1] * syz_z;
+                        syz_z = syz_z / S->K_z[indm + 1] + PZ(S->psi_syz_z);
+                        PZ(S->psi_szz_z) = S->b_z_half[indm] * PZ(S->psi_szz_z) + S->a_z_half[indm] * szz_z;
+                        szz_z = szz_z / S->K_z_half[indm] + PZ(S->psi_szz_z);
+                    } else if (!fs && gidz - FDOH < NAB) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        PZ(S->psi_sxz_z) = S->b_z[k] * PZ(S->psi_sxz_z) + S->a_z[k] * sxz_z;
+                        sxz_z = sxz_z / S->K_z[k] + PZ(S->psi_sxz_z);
+                        PZ(S->psi_syz_z) = S->b_z[k] * PZ(S->psi_syz_z) + S->a_z[k] * syz_z;
+                        syz_z = syz_z / S->K_z[k] + PZ(S->psi_syz_z);
+                        PZ(S->psi_szz_z) = S->b_z_half[k] * PZ(S->psi_szz_z) + S->a_z_half[k] * szz_z;
+                        szz_z = szz_z / S->K_z_half[k] + PZ(S->psi_szz_z);
+                    }
+                    if (gidy - FDOH < NAB) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        PY(S->psi_sxy_y) = S->b_y[j] * PY(S->psi_sxy_y) + S->a_y[j] * sxy_y;
+                        sxy_y = sxy_y / S->K_y[j] + PY(S->psi_sxy_y);
+                        PY(S->psi_syy_y) = S->b_y_half[j] * PY(S->psi_syy_y) + S->a_y_half[j] * syy_y;
+                        syy_y = syy_y / S->K_y_half[j] + PY(S->psi_syy_y);
+                        PY(S->psi_syz_y) = S->b_y[j] * PY(S->psi_syz_y) + S->a_y[j] * syz_y;
+                        syz_y = syz_y / S->K_y[j] + PY(S->psi_syz_y);
+                    } else if (gidy > NY - NAB - FDOH - 1) {
+                        i = gidx - FDOH;
+                        j = gidy - NY + NAB + FDOH + NAB;
+                        k = gidz - FDOH;
+                        indm = 2 * NAB - 1 - j;
+                        PY(S->psi_sxy_y) = S->b_y[indm + 1] * PY(S->psi_sxy_y) + S->a_y[indm + 1] * sxy_y;
+                        sxy_y = sxy_y / S->K_y[indm + 1] + PY(S->psi_sxy_y);
+                        PY(S->psi_syy_y) = S->b_y_half[indm] * PY(S->psi_syy_y) + S->a_y_half[indm] * syy_y;
+                        syy_y = syy_y / S->K_y_half[indm] + PY(S->psi_syy_y);
+                        PY(S->psi_syz_y) = S->b_y[indm + 1] * PY(S->psi_syz_y) + S->a_y[indm + 1] * syz_y;
+                        syz_y = syz_y / S->K_y[indm + 1] + PY(S->psi_syz_y);
+                    }
+                    if (gidx - FDOH < NAB) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        PX(S->psi_sxx_x) = S->b_x_half[i] * PX(S->psi_sxx_x) + S->a_x_half[i] * sxx_x;
+                        sxx_x = sxx_x / S->K_x_half[i] + PX(S->psi_sxx_x);
+                        PX(S->psi_sxy_x) = S->b_x[i] * PX(S->psi_sxy_x) + S->a_x[i] * sxy_x;
+                        sxy_x = sxy_x / S->K_x[i] + PX(S->psi_sxy_x);
+                        PX(S->psi_sxz_x) = S->b_x[i] * PX(S->psi_sxz_x) + S->a_x[i] * sxz_x;
+                        sxz_x = sxz_x / S->K_x[i] + PX(S->psi_sxz_x);
+                    }
+                    if (gidx > NX - NAB - FDOH - 1) {
+                        i = gidx - NX + NAB + FDOH + NAB;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        indm = 2 * NAB - 1 - i;
+                        PX(S->psi_sxx_x) = S->b_x_half[indm] * PX(S->psi_sxx_x) + S->a_x_half[indm] * sxx_x;
+                        sxx_x = sxx_x / S->K_x_half[indm] + PX(S->psi_sxx_x);
+                        PX(S->psi_sxy_x) = S->b_x[indm + 1] * PX(S->psi_sxy_x) + S->a_x[indm + 1] * sxy_x;
+                        sxy_x = sxy_x / S->K_x[indm + 1] + PX(S->psi_sxy_x);
+                        PX(S->psi_sxz_x) = S->b_x[indm + 1] * PX(S->psi_sxz_x) + S->a_x[indm + 1] * sxz_x;
+                        sxz_x = sxz_x / S->K_x[indm + 1] + PX(S->psi_sxz_x);
+                    }
+                }
+                S->vx[indv] += ((sxx_x + sxy_y + sxz_z) * S->rip[indp]);
+                S->vy[indv] += ((syy_y + sxy_x + syz_z) * S->rjp[indp]);
+                S->vz[indv] += ((szz_z + sxz_x + syz_y) * S->rkp[indp]);
+                if (tap) {
+                    float *V[3] = {&S->vx[indv], &S->vy[indv], &S->vz[indv]};
+                    for (int q = 0; q < 3; q++) {
+                        if (!fs && gidz - FDOH < NAB) *V[q] *= S->taper[gidz - FDOH];
+                        if (gidz > NZ - NAB - FDOH - 1) *V[q] *= S->taper[NZ - FDOH - gidz - 1];
+                        if (gidy - FDOH < NAB) *V[q] *= S->taper[gidy - FDOH];
+                        if (gidy > NY - NAB - FDOH - 1) *V[q] *= S->taper[NY - FDOH - gidy - 1];
+                        if (gidx - FDOH < NAB) *V[q] *= S->taper[gidx - FDOH];
+                        if (gidx > NX - NAB - FDOH - 1) *V[q] *= S->taper[NX - FDOH - gidx - 1];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D stress update: src/update_s3D.cl:35-489                                */
+/* ------------------------------------------------------------------------ */
+static void update_s3D(sim *S) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NY = S->NYp, NX = S->NXp,
+              LVE = S->L;
+    const int cpml = S->c->abs_type == 1, tap = S->c->abs_type == 2,
+              fs = S->c->freesurf;
+    const float *hc = S->hc, DT = S->DT, DT2 = S->DT2;
+    const long sy = NZ, sx = (long)NY * NZ;
+    const long nzi = NZ - 2 * FDOH, nyi = NY - 2 * FDOH;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidy = FDOH; gidy < NY - FDOH; gidy++) {
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - FDOH) * nzi +
+                            (gidz - FDOH);
+                long indv = (long)gidx * sx + (long)gidy * sy + gidz;
+                float vzx = Dp(S->vz, indv, sx, hc, FDOH);
+                float vzy = Dp(S->vz, indv, sy, hc, FDOH);
+                float vzz = Dm(S->vz, indv, 1, hc, FDOH);
+                float vyx = Dp(S->vy, indv, sx, hc, FDOH);
+                float vyy = Dm(S->vy, indv, sy, hc, FDOH);
+                float vyz = Dp(S->vy, indv, 1, hc, FDOH);
+                float vxx = Dm(S->vx, indv, sx, hc, FDOH);
+                float vxy = Dp(S->vx, indv, sy, hc, FDOH);
+                float vxz = Dp(S->vx, indv, 1, hc, FDOH);
+                if (cpml) {
+                    int i, j, k, indm;
+                    if (gidz > NZ - NAB - FDOH - 1) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - NZ + NAB + FDOH + NAB;
+                        indm = 2 * NAB - 1 - k;
+                        PZ(S->psi_vx_z) = S->b_z_half[indm] * PZ(S->psi_vx_z) + S->a_z_half[indm] * vxz;
+                        vxz = vxz / S->K_z_half[indm] + PZ(S->psi_vx_z);
+                        PZ(S->psi_vy_z) = S->b_z_half[indm] * PZ(S->psi_vy_z) + S->a_z_half[indm] * vyz;
+                        vyz = vyz / S->K_z_half[indm] + PZ(S->psi_vy_z);
+                        PZ(S->psi_vz_z) = S->b_z[indm + 1] * PZ(S->psi_vz_z) + S->a_z[indm + 1] * vzz;
+                        vzz = vzz / S->K_z[indm + 1] + PZ(S->psi_vz_z);
+                    }
+                    if (!fs && gidz - FDOH < NAB) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        PZ(S->psi_vx_z) = S->b_z_half[k] * PZ(S->psi_vx_z) + S->a_z_half[k] * vxz;
+                        vxz = vxz / S->K_z_half[k] + PZ(S->psi_vx_z);
+                        PZ(S->psi_vy_z) = S->b_z_half[k] * PZ(S->psi_vy_z) + S->a_z_half[k] * vyz;
+                        vyz = vyz / S->K_z_half[k] + PZ(S->psi_vy_z);
+                        PZ(S->psi_vz_z) = S->b_z[k] * PZ(S->psi_vz_z) + S->a_z[k] * vzz;
+                        vzz = vzz / S->K_z[k] + PZ(S->psi_vz_z);
+                    }
+                    if (gidy - FDOH < NAB) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        PY(S->psi_vx_y) = S->b_y_half[j] * PY(S->psi_vx_y) + S->a_y_half[j] * vxy;
+                        vxy = vxy / S->K_y_half[j] + PY(S->psi_vx_y);
+                        PY(S->psi_vy_y) = S->b_y[j] * PY(S->psi_vy_y) + S->a_y[j] * vyy;
+                        vyy = vyy / S->K_y[j] + PY(S->psi_vy_y);
+                        PY(S->psi_vz_y) = S->b_y_half[j] * PY(S->psi_vz_y) + S->a_y_half[j] * vzy;
+                        vzy = vzy / S->K_y_half[j] + PY(S->psi_vz_y);
+                    }
+                    if (gidy > NY - NAB - FDOH - 1) {
+                        i = gidx - FDOH;
+                        j = gidy - NY + NAB + FDOH + NAB;
+                        k = gidz - FDOH;
+                        indm = 2 * NAB - 1 - j;
+                        PY(S->psi_vx_y) = S->b_y_half[indm] * PY(S->psi_vx_y) + S->a_y_half[indm] * vxy;
+                        vxy = vxy / S->K_y_half[indm] + PY(S->psi_vx_y);
+                        PY(S->psi_vy_y) = S->b_y[indm + 1] * PY(S->psi_vy_y) + S->a_y[indm + 1] * vyy;
+                        vyy = vyy / S->K_y[indm + 1] + PY(S->psi_vy_y);
+                        PY(S->psi_vz_y) = S->b_y_half[indm] * PY(S->psi_vz_y) + S->a_y_half[indm] * vzy;
+                        vzy = vzy / S->K_y_half[indm] + PY(S->psi_vz_y);
+                    }
+                    if (gidx - FDOH < NAB) {
+                        i = gidx - FDOH;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        PX(S->psi_vx_x) = S->b_x[i] * PX(S->psi_vx_x) + S->a_x[i] * vxx;
+                        vxx = vxx / S->K_x[i] + PX(S->psi_vx_x);
+                        PX(S->psi_vy_x) = S->b_x_half[i] * PX(S->psi_vy_x) + S->a_x_half[i] * vyx;
+                        vyx = vyx / S->K_x_half[i] + PX(S->psi_vy_x);
+                        PX(S->psi_vz_x) = S->b_x_half[i] * PX(S->psi_vz_x) + S->a_x_half[i] * vzx;
+                        vzx = vzx / S->K_x_half[i] + PX(S->psi_vz_x);
+                    }
+                    if (gidx > NX - NAB - FDOH - 1) {
+                        i = gidx - NX + NAB + FDOH + NAB;
+                        j = gidy - FDOH;
+                        k = gidz - FDOH;
+                        indm = 2 * NAB - 1 - i;
+                        PX(S->psi_vx_x) = S->b_x[indm + 1] * PX(S->psi_vx_x) + S->a_x[indm + 1] * vxx;
+                        vxx = vxx / S->K_x[indm + 1] + PX(S->psi_vx_x);
+                        PX(S->psi_vy_x) = S->b_x_half[indm] * PX(S->psi_vy_x) + S->a_x_half[indm] * vyx;
+                        vyx = vyx / S->K_x_half[indm] + PX(S->psi_vy_x);
+                        PX(S->psi_vz_x) = S->b_x_half[indm] * PX(S->psi_vz_x) + S->a_x_half[indm] * vzx;
+                        vzx = vzx / S->K_x_half[indm] + PX(S->psi_vz_x);
+                    }
+                }
+                float vxyyx = vxy + vyx;
+                float vyzzy = vyz + vzy;
+                float vxzzx = vxz + vzx;
+                float vxxyyzz = vxx + vyy + vzz;
+                float vyyzz = vyy + vzz;
+                float vxxzz = vxx + vzz;
+                float vxxyy = vxx + vyy;
+                if (LVE == 0) {
+                    float fipjp = S->muipjp[indp];
+                    float fjpkp = S->mujpkp[indp];
+                    float fipkp = S->muipkp[indp];
+                    float g = S->M[indp];
+                    float f = 2.0 * S->mu[indp];
+                    S->sxy[indv] += (fipjp * vxyyx);
+                    S->syz[indv] += (fjpkp * vyzzy);
+                    S->sxz[indv] += (fipkp * vxzzx);
+                    S->sxx[indv] += ((g * vxxyyzz) - (f * vyyzz));
+                    S->syy[indv] += ((g * vxxyyzz) - (f * vxxzz));
+                    S->szz[indv] += ((g * vxxyyzz) - (f * vxxyy));
+                } else {
+                    float lM = S->M[indp], lmu = S->mu[indp];
+                    float lmuipkp = S->muipkp[indp], lmuipjp = S->muipjp[indp],
+                          lmujpkp = S->mujpkp[indp];
+                    float ltaup = S->taup[indp], ltaus = S->taus[indp];
+                    float ltausipkp = S->tausipkp[indp], ltausipjp = S->tausipjp[indp],
+                          ltausjpkp = S->tausjpkp[indp];
+                    float fipjp = lmuipjp * (1.0 + (float)LVE * ltausipjp);
+                    float fjpkp = lmujpkp * (1.0 + (float)LVE * ltausjpkp);
+                    float fipkp = lmuipkp * (1.0 + (float)LVE * ltausipkp);
+                    float g = lM * (1.0 + (float)LVE * ltaup);
+                    float f = 2.0 * lmu * (1.0 + (float)LVE * ltaus);
+                    float dipjp = lmuipjp * ltausipjp / DT;
+                    float djpkp = lmujpkp * ltausjpkp / DT;
+                    float dipkp = lmuipkp * ltausipkp / DT;
+                    float d = 2.0 * lmu * ltaus / DT;
+                    float e = lM * ltaup / DT;
+                    float sumrxy = 0, sumryz = 0, sumrxz = 0, sumrxx = 0, sumryy = 0,
+                          sumrzz = 0;
+                    const long NV = (long)NX * NY * NZ;
+                    for (int l = 0; l < LVE; l++) {
+                        long indr = l * NV + indv;
+                        sumrxy += S->rxy[indr];
+                        sumryz += S->ryz[indr];
+                        sumrxz += S->rxz[indr];
+                        sumrxx += S->rxx[indr];
+                        sumryy += S->ryy[indr];
+                        sumrzz += S->rzz[indr];
+                    }
+                    float lsxy = (fipjp * vxyyx) + (DT2 * sumrxy);
+                    float lsyz = (fjpkp * vyzzy) + (DT2 * sumryz);
+                    float lsxz = (fipkp * vxzzx) + (DT2 * sumrxz);
+                    float lsxx = ((g * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
+                    float lsyy = ((g * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
+                    float lszz = ((g * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
+                    sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
+                    for (int l = 0; l < LVE; l++) {
+                        float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
+                        float cc = 1.0 - (S->eta[l] * 0.5);
+                        long indr = l * NV + indv;
+                        S->rxy[indr] = b * (S->rxy[indr] * cc - S->eta[l] * (dipjp * vxyyx));
+                        S->ryz[indr] = b * (S->ryz[indr] * cc - S->eta[l] * (djpkp * vyzzy));
+                        S->rxz[indr] = b * (S->rxz[indr] * cc - S->eta[l] * (dipkp * vxzzx));
+                        S->rxx[indr] = b * (S->rxx[indr] * cc -
+                                            S->eta[l] * ((e * vxxyyzz) - (d * vyyzz)));
+                        S->ryy[indr] = b * (S->ryy[indr] * cc -
+                                            S->eta[l] * ((e * vxxyyzz) - (d * vxxzz)));
+                        S->rzz[indr] = b * (S->rzz[indr] * cc -
+                                            S->eta[l] * ((e * vxxyyzz) - (d * vxxyy)));
+                        sumrxy += S->rxy[indr];
+                        sumryz += S->ryz[indr];
+                        sumrxz += S->rxz[indr];
+                        sumrxx += S->rxx[indr];
+                        sumryy += S->ryy[indr];
+                        sumrzz += S->rzz[indr];
+                    }
+                    S->sxy[indv] += lsxy + (DT2 * sumrxy);
+                    S->syz[indv] += lsyz + (DT2 * sumryz);
+                    S->sxz[indv] += lsxz + (DT2 * sumrxz);
+                    S->sxx[indv] += lsxx + (DT2 * sumrxx);
+                    S->syy[indv] += lsyy + (DT2 * sumryy);
+                    S->szz[indv] += lszz + (DT2 * sumrzz);
+                }
+                if (tap) {
+                    float *V[6] = {&S->sxy[indv], &S->syz[indv], &S->sxz[indv],
+                                   &S->sxx[indv], &S->syy[indv], &S->szz[indv]};
+                    for (int q = 0; q < 6; q++) {
+                        if (!fs && gidz - FDOH < NAB) *V[q] *= S->taper[gidz - FDOH];
+                        if (gidz > NZ - NAB - FDOH - 1) *V[q] *= S->taper[NZ - FDOH - gidz - 1];
+                        if (gidy - FDOH < NAB) *V[q] *= S->taper[gidy - FDOH];
+                        if (gidy > NY - NAB - FDOH - 1) *V[q] *= S->taper[NY - FDOH - gidy - 1];
+                        if (gidx - FDOH < NAB) *V[q] *= S->taper[gidx - FDOH];
+                        if (gidx > NX - NAB - FDOH - 1) *V[q] *= S->taper[NX - FDOH - gidx - 1];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Free surface, image method: src/surface2D.cl:30-193                       */
+/* ------------------------------------------------------------------------ */
+static void surface2D(sim *S, int pdir) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NX = S->NXp, LVE = S->L;
+    const float *hc = S->hc, DT = S->DT, DT2 = S->DT2;
+    const int cpml = S->c->abs_type == 1;
+    const int gidz = FDOH;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        long indp = (long)(gidx - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+        long iv = (long)gidx * NZ + gidz;
+        S->szz[iv] = 0.0;
+        for (int l = 0; l < LVE; l++) S->rzz[(long)l * NX * NZ + iv] = 0.0;
+        for (int m = 1; m <= FDOH; m++) {
+            S->szz[iv - m] = -S->szz[iv + m];
+            S->sxz[iv - m] = -S->sxz[iv + m - 1];
+        }
+        float vxx = Dm(S->vx, iv, NZ, hc, FDOH);
+        float vzz = Dm(S->vz, iv, 1, hc, FDOH);
+        if (cpml) {
+            int i, k, ind;
+            if (gidx - FDOH < NAB) {
+                i = gidx - FDOH;
+                k = gidz - FDOH;
+                vxx = vxx / S->K_x[i] + S->psi_vx_x[(long)i * (NZ - 2 * FDOH) + k];
+            }
+            if (gidx > NX - NAB - FDOH - 1) {
+                i = gidx - NX + NAB + FDOH + NAB;
+                k = gidz - FDOH;
+                ind = 2 * NAB - 1 - i;
+                vxx = vxx / S->K_x[ind + 1] + S->psi_vx_x[(long)i * (NZ - 2 * FDOH) + k];
+            }
+        }
+        if (LVE == 0) {
+            float f = S->mu[indp] * 2.0;
+            float g = S->M[indp];
+            float h = -((g - f) * (g - f) * (vxx) / g) - ((g - f) * vzz);
+            S->sxx[iv] += pdir * h;
+        } else {
+            float f = S->mu[indp] * 2.0 * (1.0 + LVE * S->taus[indp]);
+            float g = S->M[indp] * (1.0 + LVE * S->taup[indp]);
+            float h = -((g - f) * (g - f) * (vxx) / g) - ((g - f) * vzz);
+            float sump = 0;
+            for (int l = 0; l < LVE; l++) sump += S->rxx[(long)l * NX * NZ + iv];
+            h += -DT2 * sump;
+            S->sxx[iv] += pdir * h;
+            float d = 2.0 * S->mu[indp] * S->taus[indp] / DT;
+            float e = S->M[indp] * S->taup[indp] / DT;
+            sump = 0;
+            for (int l = 0; l < LVE; l++) {
+                float b = S->eta[l] / (1.0 + (S->eta[l] * 0.5));
+                h = b * (((d - e) * ((f / g) - 1.0) * vxx) - ((d - e) * vzz));
+                S->rxx[(long)l * NX * NZ + iv] += pdir * h;
+                sump += S->rxx[(long)l * NX * NZ + iv];
+            }
+            S->sxx[iv] += pdir * (DT2 * sump);
+        }
+    }
+}
+
+/* Free surface 3D: src/surface3D.cl:39-271 */
+static void surface3D(sim *S, int pdir) {
+    const int FDOH = S->FDOH, NAB = S->NAB, NZ = S->NZp, NY = S->NYp, NX = S->NXp,
+              LVE = S->L;
+    const float *hc = S->hc, DT = S->DT, DT2 = S->DT2;
+    const int cpml = S->c->abs_type == 1;
+    const long sy = NZ, sx = (long)NY * NZ, NV = (long)NX * NY * NZ;
+    const long nzi = NZ - 2 * FDOH, nyi = NY - 2 * FDOH;
+    const int gidz = FDOH;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        for (int gidy = FDOH; gidy < NY - FDOH; gidy++) {
+            long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - FDOH) * nzi +
+                        (gidz - FDOH);
+            long iv = (long)gidx * sx + (long)gidy * sy + gidz;
+            S->szz[iv] = 0.0;
+            for (int l = 0; l < LVE; l++) S->rzz[l * NV + iv] = 0.0;
+            for (int m = 1; m <= FDOH; m++) {
+                S->szz[iv - m] = -S->szz[iv + m];
+                S->sxz[iv - m] = -S->sxz[iv + m - 1];
+                S->syz[iv - m] = -S->syz[iv + m - 1];
+            }
+            float vxx = Dm(S->vx, iv, sx, hc, FDOH);
+            float vyy = Dm(S->vy, iv, sy, hc, FDOH);
+            float vzz = Dm(S->vz, iv, 1, hc, FDOH);
+            if (cpml) {
+                int i, j, k, ind;
+                if (gidy - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    j = gidy - FDOH;
+                    k = gidz - FDOH;
+                    vyy = vyy / S->K_y[j] + PY(S->psi_vy_y);
+                } else if (gidy > NY - NAB - FDOH - 1) {
+                    i = gidx - FDOH;
+                    j = gidy - NY + NAB + FDOH + NAB;
+                    k = gidz - FDOH;
+                    ind = 2 * NAB - 1 - j;
+                    vyy = vyy / S->K_y[ind + 1] + PY(S->psi_vy_y);
+                }
+                if (gidx - FDOH < NAB) {
+                    i = gidx - FDOH;
+                    j = gidy - FDOH;
+                    k = gidz - FDOH;
+                    vxx = vxx / S->K_x[i] + PX(S->psi_vx_x);
+                }
+                if (gidx > NX - NAB - FDOH - 1) {
+                    i = gidx - NX + NAB + FDOH + NAB;
+                    j = gidy - FDOH;
+                    k = gidz - FDOH;
+                    ind = 2 * NAB - 1 - i;
+                    vxx = vxx / S->K_x[ind + 1] + PX(S->psi_vx_x);
+                }
+            }
+            if (LVE == 0) {
+                float f = S->mu[indp] * 2.0f;
+                float g = S->M[indp];
+                float h = -((g - f) * (g - f) * (vxx + vyy) / g) - ((g - f) * vzz);
+                S->sxx[iv] += h;
+                S->syy[iv] += h;
+            } else {
+                float f = S->mu[indp] * 2.0 * (1.0 + LVE * S->taus[indp]);
+                float g = S->M[indp] * (1.0 + LVE * S->taup[indp]);
+                float h = -((g - f) * (g - f) * (vxx + vyy) / g) - ((g - f) * vzz);
+                float sumx = 0, sumy = 0;
+                for (int l = 0; l < LVE; l++) {
+                    sumx += S->rxx[l * NV + iv];
+                    sumy += S->ryy[l * NV + iv];
+                }
+                S->sxx[iv] += pdir * (h - DT2 * sumx);
+                S->syy[iv] += pdir * (h - DT2 * sumy);
+                float d = 2.0 * S->mu[indp] * S->taus[indp] / DT;
+                float e = S->M[indp] * S->taup[indp] / DT;
+                sumx = 0;
+                sumy = 0;
+                for (int l = 0; l < LVE; l++) {
+                    float b = S->eta[l] / (1.0 + (S->eta[l] * 0.5));
+                    h = b * (((d - e) * ((f / g) - 1.0) * (vxx + vyy)) - ((d - e) * vzz));
+                    S->rxx[l * NV + iv] += h;
+                    S->ryy[l * NV + iv] += h;
+                    sumx += S->rxx[l * NV + iv];
+                    sumy += S->ryy[l * NV + iv];
+                }
+                S->sxx[iv] += (DT2 * sumx);
+                S->syy[iv] += (DT2 * sumy);
+            }
+        }
+    }
+}
+#undef PX
+#undef PY
+#undef PZ
+
+/* ------------------------------------------------------------------------ */
+/* Sources: generated kernel of src/automatic_kernels.c:394-629.             */
+/* Each source is dispatched on its own type code (the reference compares    */
+/* against the variable index when several types are mixed, :562-585).      */
+/* ------------------------------------------------------------------------ */
+static long cell_of(const sim *S, const float *pos, int *i_out, int *k_out) {
+    int i = (int)(pos[0] / S->DH) + S->FDOH;
+    int j = (int)(pos[1] / S->DH) + S->FDOH;
+    int k = (int)(pos[2] / S->DH) + S->FDOH;
+    *i_out = i;
+    *k_out = k;
+    if (S->ND == 3) return (long)i * S->NYp * S->NZp + (long)j * S->NZp + k;
+    return (long)i * S->NZp + k;
+}
+
+static int check_positions(const sim *S, int n, const float *pos, int stride,
+                           char *err, int errlen) {
+    for (int s = 0; s < n; s++) {
+        int i = (int)(pos[s * stride + 0] / S->DH) + S->FDOH;
+        int j = (int)(pos[s * stride + 1] / S->DH) + S->FDOH;
+        int k = (int)(pos[s * stride + 2] / S->DH) + S->FDOH;
+        int bad = k < 0 || k >= S->NZp;
+        if (S->ND == 3) bad |= j < 0 || j >= S->NYp;
+        if (bad) {
+            snprintf(err, errlen, "position %d lies outside the padded grid", s);
+            return 1;
+        }
+        (void)i;
+    }
+    return 0;
+}
+
+static void sources(sim *S, int nt, int pdir, int nsrc, const float *src_pos,
+                    const float *src) {
+    const int NT = S->c->NT;
+    for (int s = 0; s < nsrc; s++) {
+        int i, k;
+        long pos = cell_of(S, &src_pos[5 * s], &i, &k);
+        if (i < S->FDOH || i > S->NXp - S->FDOH - 1) continue;
+        if (S->c->freesurf == 1 && pdir == -1 && k < 2 * S->FDOH) continue;
+        int type = (int)src_pos[4 + 5 * s];
+        float amp = (float)pdir * (S->DT * src[s * NT + nt]);
+        if (type == 0 && S->vx) S->vx[pos] += amp;
+        else if (type == 1 && S->vy) S->vy[pos] += amp;
+        else if (type == 2 && S->vz) S->vz[pos] += amp;
+        else if ((type == 3 || type == 100) && S->ND == 22) S->p[pos] += amp;
+        else if (type == 3 || type == 100) {
+            /* generated text: "+=amp/%f" with (float)ND, a double literal */
+            double nd = (double)(S->ND == 3 ? 3 : 2);
+            S->sxx[pos] = (float)((double)S->sxx[pos] + (double)amp / nd);
+            S->szz[pos] = (float)((double)S->szz[pos] + (double)amp / nd);
+            if (S->ND == 3) S->syy[pos] = (float)((double)S->syy[pos] + (double)amp / nd);
+        }
+    }
+}
+
+/* Seismogram sampling: generated kernel of src/automatic_kernels.c:22-219 */
+static float *named_var(sim *S, const char *n) {
+    if (!strcmp(n, "vx")) return S->vx;
+    if (!strcmp(n, "vy")) return S->vy;
+    if (!strcmp(n, "vz")) return S->vz;
+    if (!strcmp(n, "sxx")) return S->sxx;
+    if (!strcmp(n, "syy")) return S->syy;
+    if (!strcmp(n, "szz")) return S->szz;
+    if (!strcmp(n, "sxy")) return S->sxy;
+    if (!strcmp(n, "syz")) return S->syz;
+    if (!strcmp(n, "sxz")) return S->sxz;
+    if (!strcmp(n, "p")) return S->p;
+    return NULL;
+}
+
+static void varsout(sim *S, int nt, int nrec, const float *rec_pos, float *out) {
+    const int NT = S->c->NT, nout = or_nout(S->c);
+    for (int r = 0; r < nrec; r++) {
+        int i, k;
+        long pos = cell_of(S, &rec_pos[8 * r], &i, &k);
+        if (i < S->FDOH || i > S->NXp - S->FDOH - 1) continue;
+        for (int o = 0; o < nout; o++) {
+            const char *name = or_outname(S->c, o);
+            float v;
+            if (!strcmp(name, "p") && S->ND != 22) {
+                /* (sxx+szz[+syy])/ND with "%f"-printed ND, a double literal */
+                float s = S->sxx[pos] + S->szz[pos];
+                if (S->ND == 3) s = s + S->syy[pos];
+                v = (float)((double)s / (double)(S->ND == 3 ? 3 : 2));
+            } else {
+                v = named_var(S, name)[pos];
+            }
+            out[((long)o * nrec + r) * NT + nt] = v;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Forward driver: src/time_stepping.c:747-835 (one shot)                    */
+/* ------------------------------------------------------------------------ */
+static int sim_init(sim *S, const or_csts *c, char *err, int errlen) {
+    memset(S, 0, sizeof(*S));
+    S->c = c;
+    S->ND = c->ND;
+    S->FDOH = c->FDORDER / 2;
+    S->NAB = c->nab;
+    S->L = c->L;
+    if (c->ND != 2 && c->ND != 3 && c->ND != 22) {
+        snprintf(err, errlen, "ND=%d is not supported", c->ND);
+        return 1;
+    }
+    if (c->L < 0 || c->L > OR_MAXL) {
+        snprintf(err, errlen, "L must be in [0, %d]", OR_MAXL);
+        return 1;
+    }
+    if (c->nab > 120) {
+        snprintf(err, errlen, "nab too large");
+        return 1;
+    }
+    int NZ, NY, NX;
+    dims3(c, &NZ, &NY, &NX);
+    S->NZ = NZ;
+    S->NY = NY;
+    S->NX = NX;
+    S->NZp = NZ + 2 * S->FDOH;
+    S->NXp = NX + 2 * S->FDOH;
+    S->NYp = c->ND == 3 ? NY + 2 * S->FDOH : 1;
+    S->nv = (long)S->NZp * S->NYp * S->NXp;
+    S->np = (long)NZ * NY * NX;
+    double hc[7];
+    if (or_holberg(c->FDORDER, c->MAXRELERROR, hc)) {
+        snprintf(err, errlen, "invalid FDORDER/MAXRELERROR");
+        return 1;
+    }
+    or_kernel_hc(c->FDORDER, c->MAXRELERROR, S->hc);
+    S->DT = or_literal9((double)c->dt);
+    S->DT2 = or_literal9((double)c->dt / 2.0);
+    S->DH = or_literal9((double)c->dh);
+    or_eta(c->L, c->FL, c->dt, S->eta);
+    if (c->abs_type == 1) {
+        or_cpml_coeff(c->NPOWER, c->K_MAX_CPML, c->FPML, c->VPPML, c->dh, c->dt,
+                      c->nab, S->K_x, S->b_x, S->a_x, S->K_x_half, S->b_x_half,
+                      S->a_x_half);
+        memcpy(S->K_y, S->K_x, sizeof(S->K_x));
+        memcpy(S->a_y, S->a_x, sizeof(S->a_x));
+        memcpy(S->b_y, S->b_x, sizeof(S->b_x));
+        memcpy(S->K_y_half, S->K_x_half, sizeof(S->K_x));
+        memcpy(S->a_y_half, S->a_x_half, sizeof(S->a_x));
+        memcpy(S->b_y_half, S->b_x_half, sizeof(S->b_x));
+        memcpy(S->K_z, S->K_x, sizeof(S->K_x));
+        memcpy(S->a_z, S->a_x, sizeof(S->a_x));
+        memcpy(S->b_z, S->b_x, sizeof(S->b_x));
+        memcpy(S->K_z_half, S->K_x_half, sizeof(S->K_x));
+        memcpy(S->a_z_half, S->a_x_half, sizeof(S->a_x));
+        memcpy(S->b_z_half, S->b_x_half, sizeof(S->b_x));
+    } else if (c->abs_type == 2) {
+        or_taper(c->nab, c->abpc, S->taper);
+    }
+    long nv = S->nv, L = c->L;
+    S->vx = zalloc(nv);
+    S->vz = zalloc(nv);
+    if (c->ND == 3) S->vy = zalloc(nv);
+    if (c->ND == 22) {
+        S->p = zalloc(nv);
+        if (L) S->rp = zalloc(nv * L);
+    } else {
+        S->sxx = zalloc(nv);
+        S->szz = zalloc(nv);
+        S->sxz = zalloc(nv);
+        if (L) {
+            S->rxx = zalloc(nv * L);
+            S->rzz = zalloc(nv * L);
+            S->rxz = zalloc(nv * L);
+        }
+        if (c->ND == 3) {
+            S->syy = zalloc(nv);
+            S->sxy = zalloc(nv);
+            S->syz = zalloc(nv);
+            if (L) {
+                S->ryy = zalloc(nv * L);
+                S->rxy = zalloc(nv * L);
+                S->ryz = zalloc(nv * L);
+            }
+        }
+    }
+    long np = S->np;
+    S->M = zalloc(np);
+    S->rho = zalloc(np);
+    S->rip = zalloc(np);
+    S->rkp = zalloc(np);
+    if (c->ND == 3) S->rjp = zalloc(np);
+    if (c->ND != 22) {
+        S->mu = zalloc(np);
+        S->muipkp = zalloc(np);
+        if (c->ND == 3) {
+            S->muipjp = zalloc(np);
+            S->mujpkp = zalloc(np);
+        }
+    }
+    if (L) {
+        S->taup = zalloc(np);
+        if (c->ND != 22) {
+            S->taus = zalloc(np);
+            S->tausipkp = zalloc(np);
+            if (c->ND == 3) {
+                S->tausipjp = zalloc(np);
+                S->tausjpkp = zalloc(np);
+            }
+        }
+    }
+    if (c->abs_type == 1) {
+        long nab2 = 2 * c->nab;
+        long nzi = NZ, nyi = (c->ND == 3) ? NY : 1, nxi = NX;
+        long sxs = nab2 * nyi * nzi, sys = nxi * nab2 * nzi, szs = nxi * nyi * nab2;
+        if (c->ND == 22) {
+            S->psi_p_x = zalloc(sxs);
+            S->psi_p_z = zalloc(szs);
+            S->psi_vx_x = zalloc(sxs);
+            S->psi_vz_z = zalloc(szs);
+        } else {
+            S->psi_sxx_x = zalloc(sxs);
+            S->psi_sxz_x = zalloc(sxs);
+            S->psi_szz_z = zalloc(szs);
+            S->psi_sxz_z = zalloc(szs);
+            S->psi_vx_x = zalloc(sxs);
+            S->psi_vz_x = zalloc(sxs);
+            S->psi_vx_z = zalloc(szs);
+            S->psi_vz_z = zalloc(szs);
+            if (c->ND == 3) {
+                S->psi_sxy_x = zalloc(sxs);
+                S->psi_syz_z = zalloc(szs);
+                S->psi_vy_x = zalloc(sxs);
+                S->psi_vy_z = zalloc(szs);
+                S->psi_syy_y = zalloc(sys);
+                S->psi_sxy_y = zalloc(sys);
+                S->psi_syz_y = zalloc(sys);
+                S->psi_vx_y = zalloc(sys);
+                S->psi_vy_y = zalloc(sys);
+                S->psi_vz_y = zalloc(sys);
+            }
+        }
+    }
+    return 0;
+}
+
+static void sim_free(sim *S) {
+    float **ptrs[] = {&S->vx, &S->vy, &S->vz, &S->sxx, &S->syy, &S->szz, &S->sxy,
+                      &S->syz, &S->sxz, &S->p, &S->rxx, &S->ryy, &S->rzz, &S->rxy,
+                      &S->ryz, &S->rxz, &S->rp, &S->M, &S->mu, &S->rho, &S->rip,
+                      &S->rjp, &S->rkp, &S->muipkp, &S->muipjp, &S->mujpkp,
+                      &S->taup, &S->taus, &S->tausipkp, &S->tausipjp, &S->tausjpkp,
+                      &S->psi_sxx_x, &S->psi_sxy_x, &S->psi_sxz_x, &S->psi_sxy_y,
+                      &S->psi_syy_y, &S->psi_syz_y, &S->psi_sxz_z, &S->psi_syz_z,
+                      &S->psi_szz_z, &S->psi_vx_x, &S->psi_vy_x, &S->psi_vz_x,
+                      &S->psi_vx_y, &S->psi_vy_y, &S->psi_vz_y, &S->psi_vx_z,
+                      &S->psi_vy_z, &S->psi_vz_z, &S->psi_p_x, &S->psi_p_z};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) {
+        free(*ptrs[i]);
+        *ptrs[i] = NULL;
+    }
+}
+
+static double g_loop_seconds = 0.0;
+double or_last_loop_seconds(void) { return g_loop_seconds; }
+
+static double wall_seconds(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+int or_forward(const or_csts *c, const float *p0, const float *p1,
+               const float *rho, const float *taup, const float *taus,
+               int nsrc, const float *src_pos, const float *src, int nrec,
+               const float *rec_pos, float *out, float *fields_out, char *err,
+               int errlen) {
+    sim S;
+    char dummy[8];
+    if (!err) {
+        err = dummy;
+        errlen = sizeof(dummy);
+    }
+    if (sim_init(&S, c, err, errlen)) {
+        sim_free(&S);
+        return 1;
+    }
+#ifdef _OPENMP
+    if (c->nthreads > 0) omp_set_num_threads(c->nthreads);
+#endif
+    if (or_nout(c) == 0) {
+        snprintf(err, errlen, "seisout=%d has no outputs for ND=%d", c->seisout, c->ND);
+        sim_free(&S);
+        return 1;
+    }
+    if (c->ND == 22 && c->freesurf) {
+        snprintf(err, errlen, "the free surface is not available for ND=22");
+        sim_free(&S);
+        return 1;
+    }
+    or_transform(c, p0, p1, rho, taup, taus, S.M, S.mu, S.rho, S.rip, S.rjp, S.rkp,
+                 S.muipkp, S.muipjp, S.mujpkp, S.taup, S.taus, S.tausipkp,
+                 S.tausipjp, S.tausjpkp);
+    float dts;
+    if (or_check_stability(c, S.M, S.mu, S.rho, &dts)) {
+        snprintf(err, errlen, "Time step too large, to be stable, set dt<%f", dts);
+        sim_free(&S);
+        return 1;
+    }
+    if (check_positions(&S, nsrc, src_pos, 5, err, errlen) ||
+        check_positions(&S, nrec, rec_pos, 8, err, errlen)) {
+        sim_free(&S);
+        return 1;
+    }
+    int tmax = c->tmax > 0 ? c->tmax : c->NT;
+    memset(out, 0, sizeof(float) * (size_t)or_nout(c) * nrec * c->NT);
+    double t_start = wall_seconds();
+    for (int t = 0; t < tmax; t++) {
+        sources(&S, t, 1, nsrc, src_pos, src);
+        if (c->ND == 3) {
+            update_v3D(&S);
+            update_s3D(&S);
+        } else if (c->ND == 2) {
+            update_v2D(&S);
+            update_s2D(&S);
+        } else {
+            update_v2D_acc(&S);
+            update_s2D_acc(&S);
+        }
+        if (c->freesurf) {
+            if (c->ND == 3) surface3D(&S, 1);
+            else surface2D(&S, 1);
+        }
+        varsout(&S, t, nrec, rec_pos, out);
+    }
+    g_loop_seconds = wall_seconds() - t_start;
+    if (fields_out) {
+        for (int v = 0; v < or_nvars(c); v++)
+            memcpy(&fields_out[v * S.nv], named_var(&S, or_varname(c, v)),
+                   S.nv * sizeof(float));
+    }
+    sim_free(&S);
+    return 0;
+}

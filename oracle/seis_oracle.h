@@ -1,0 +1,89 @@
+/*
+ * seis_oracle.h -- CPU restatement of SeisCL's time-domain FD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against.  Only tests/, the smoke entry point and
+ * the CPU-baseline leg of bench.py may load it; the product library never
+ * links it and never falls back to it.
+ *
+ * Layout conventions follow the reference (src/F.h, src/update_v3D.cl:128-129):
+ *   - N[0]=NZ, N[1]=NX (2D) or N[0]=NZ, N[1]=NY, N[2]=NX (3D), z fastest;
+ *   - wavefields are padded by FDOH on every side:
+ *       indv = x*NYp*NZp + y*NZp + z  (padded sizes, padded coordinates);
+ *   - parameters and gradients are unpadded: indp = x*NY*NZ + y*NZ + z;
+ *   - memory variables: r[l*NXp*NYp*NZp + indv];
+ *   - CPML strips: src/update_v3D.cl:23-31.
+ */
+#ifndef SEIS_ORACLE_H
+#define SEIS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXL 8
+
+typedef struct or_csts {
+    int ND;            /* 2: 2D P-SV elastic, 3: 3D, 22: 2D acoustic           */
+    int N[3];          /* see header comment                                     */
+    int NT;            /* number of time samples                                 */
+    int tmax;          /* forward steps (0 -> NT)                                */
+    int tmin;          /* first adjoint step bound (default 0)                   */
+    float dt, dh;
+    int FDORDER, MAXRELERROR;
+    int freesurf, abs_type, nab;
+    float abpc, VPPML, NPOWER, FPML, K_MAX_CPML;
+    int L;
+    float f0;
+    float FL[OR_MAXL];
+    int seisout;       /* 1 velocities, 2 pressure, 3 stresses, 4 all          */
+    int param_type;    /* 0: vp,vs,rho  1: M,mu,rho  2: Ip,Is,rho  3: complex v */
+    int nthreads;      /* OpenMP threads (0 -> runtime default)                  */
+} or_csts;
+
+/* ---- setup pieces, exported individually so tests can pin them ---------- */
+int  or_holberg(int fdorder, int maxrelerror, double hc[7]);
+void or_kernel_hc(int fdorder, int maxrelerror, float hcf[7]);
+float or_literal9(double v);
+int  or_cpml_coeff(float npower, float kmax, float fpml, float vppml, float dh,
+                   float dt, int nab, float *K, float *b, float *a,
+                   float *K_half, float *b_half, float *a_half);
+void or_taper(int nab, float abpc, float *taper);
+void or_eta(int L, const float *FL, float dt, float *eta);
+/* Parameter transforms of src/assign_modeling_case.c applied in the order of
+ * registration (src/assign_modeling_case.c:1001-1040).  Outputs are the 14
+ * kernel parameters in reference (unpadded) layout; any pointer may be NULL
+ * if the case does not use that parameter. */
+int  or_transform(const or_csts *c, const float *p0, const float *p1,
+                  const float *rho, const float *taup, const float *taus,
+                  float *M, float *mu, float *rhoo, float *rip, float *rjp,
+                  float *rkp, float *muipkp, float *muipjp, float *mujpkp,
+                  float *taupo, float *tauso, float *tausipkp, float *tausipjp,
+                  float *tausjpkp);
+int  or_check_stability(const or_csts *c, const float *M, const float *mu,
+                        const float *rho, float *dtstable);
+
+/* number of output traces per receiver for seisout, and their names */
+int  or_nout(const or_csts *c);
+const char *or_outname(const or_csts *c, int i);
+/* number of wavefield variables (vx..syy/p) and their names */
+int  or_nvars(const or_csts *c);
+const char *or_varname(const or_csts *c, int i);
+
+/* Forward modeling of one shot (all sources with the same srcid fire
+ * together).  out: [nout][nrec][NT].  fields_out (optional): the final
+ * wavefields in padded reference layout, [nvars][Pi(N+FDORDER)].
+ * Returns 0 on success, non-zero with a message in err otherwise. */
+int  or_forward(const or_csts *c, const float *p0, const float *p1,
+                const float *rho, const float *taup, const float *taus,
+                int nsrc, const float *src_pos, const float *src,
+                int nrec, const float *rec_pos, float *out,
+                float *fields_out, char *err, int errlen);
+
+/* wall-clock seconds of the time loop of the last or_forward call */
+double or_last_loop_seconds(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,690 @@
+// C-ABI runtime: device context, buffers, model set-up and the per-shot
+// time loop.  Replaces the reference's Init_OpenCL / clprogram / clbuf layer
+// and the forward half of time_stepping (src/time_stepping.c:553-848).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "seis_internal.hpp"
+
+using namespace seis;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+    g_err = msg;
+    return 1;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+}  // namespace
+
+struct seis_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t bytes = 0;
+    std::map<void *, size_t> allocs;
+
+    int alloc(size_t n, void **p) {
+        *p = nullptr;
+        if (n == 0) return 0;
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipMalloc(p, n));
+        allocs[*p] = n;
+        bytes += n;
+        launch_fill(*p, n, stream);
+        return 0;
+    }
+    int release(void *p) {
+        if (!p) return 0;
+        auto it = allocs.find(p);
+        if (it == allocs.end()) return fail("seis_buf_free: unknown pointer");
+        bytes -= it->second;
+        allocs.erase(it);
+        HIP_TRY(hipFree(p));
+        return 0;
+    }
+};
+
+struct seis_model {
+    seis_ctx *ctx = nullptr;
+    seis_csts c{};
+    int nd_case = 2;
+    Layout g;
+    Setup s;
+    FDArgs<float> a{};
+    std::vector<void *> owned;       // device buffers owned by the model
+    size_t bytes = 0;
+    std::vector<std::string> outn;
+    std::vector<int> out_codes;
+    // shot
+    ShotDev shot;
+    int shot_nsrc_cap = 0, shot_nrec_cap = 0;
+    bool shot_bound = false;
+    // timing
+    bool profiling = false;
+    std::vector<hipEvent_t> evpool;
+    double ms_v = 0, ms_s = 0, ms_loop = 0;
+
+    float *dalloc(long n) {
+        void *p = nullptr;
+        if (n <= 0) return nullptr;
+        if (ctx->alloc((size_t)n * sizeof(float), &p)) return nullptr;
+        owned.push_back(p);
+        bytes += (size_t)n * sizeof(float);
+        return (float *)p;
+    }
+    ~seis_model() {
+        for (void *p : owned) ctx->release(p);
+        void *sp[] = {shot.src_cell, shot.src_type, shot.src_k, shot.src, shot.rec_cell, shot.out};
+        for (void *p : sp) ctx->release(p);
+        for (auto e : evpool) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+// Unpadded reference array (z fastest) -> padded device layout.
+std::vector<float> to_padded(const Layout &g, const std::vector<float> &ref) {
+    std::vector<float> out((size_t)g.nfield, 0.f);
+    for (int x = 0; x < g.NX; ++x)
+        for (int y = 0; y < g.NY; ++y) {
+            const float *src = &ref[((size_t)x * g.NY + y) * g.NZ];
+            std::copy(src, src + g.NZ, &out[g.idx(0, y, x)]);
+        }
+    return out;
+}
+
+int upload(seis_model *m, const float **dst, const std::vector<float> &ref) {
+    float *d = m->dalloc(m->g.nfield);
+    if (!d) return 1;
+    std::vector<float> pad = to_padded(m->g, ref);
+    HIP_TRY(hipMemcpyAsync(d, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice,
+                           m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    *dst = d;
+    return 0;
+}
+
+int out_code(const std::string &n) {
+    static const std::map<std::string, int> k = {
+        {"vx", O_VX},   {"vy", O_VY},   {"vz", O_VZ},   {"sxx", O_SXX}, {"syy", O_SYY},
+        {"szz", O_SZZ}, {"sxz", O_SXZ}, {"sxy", O_SXY}, {"syz", O_SYZ}, {"p", O_P}};
+    return k.at(n);
+}
+
+// Derived per-cell coefficients of the viscoelastic stress update, with the
+// reference's expressions (src/update_s3D.cl:324-340, src/update_s2D.cl:252-259,
+// src/update_s2D_acc.cl:343-346).
+int upload_params(seis_model *m, const RefParams &p) {
+    const seis_csts &c = m->c;
+    const int L = c.L;
+    const float DT = m->s.DT;
+    const size_t n = p.M.size();
+    FDArgs<float> &a = m->a;
+    if (upload(m, &a.rip, p.rip) || upload(m, &a.rkp, p.rkp)) return 1;
+    if (c.ND == 3 && upload(m, &a.rjp, p.rjp)) return 1;
+    if (L == 0) {
+        if (upload(m, &a.M, p.M)) return 1;
+        if (c.ND != 22) {
+            if (upload(m, &a.mu, p.mu) || upload(m, &a.muipkp, p.muipkp)) return 1;
+            if (c.ND == 3 && (upload(m, &a.muipjp, p.muipjp) || upload(m, &a.mujpkp, p.mujpkp)))
+                return 1;
+        }
+        return 0;
+    }
+    std::vector<float> g(n), f, d, e(n);
+    if (c.ND == 22) {
+        for (size_t i = 0; i < n; ++i) {
+            g[i] = p.M[i] * (1.0 + (float)L * p.taup[i]);
+            e[i] = p.M[i] * p.taup[i];
+        }
+        return upload(m, &a.cg, g) || upload(m, &a.ce, e);
+    }
+    f.resize(n);
+    d.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        g[i] = p.M[i] * (1.0 + (float)L * p.taup[i]);
+        f[i] = 2.0 * p.mu[i] * (1.0 + (float)L * p.taus[i]);
+        d[i] = 2.0 * p.mu[i] * p.taus[i] / DT;
+        e[i] = p.M[i] * p.taup[i] / DT;
+    }
+    if (upload(m, &a.cg, g) || upload(m, &a.cf, f) || upload(m, &a.cd, d) || upload(m, &a.ce, e))
+        return 1;
+    auto fij = [&](const std::vector<float> &mu, const std::vector<float> &ts, const float **fo,
+                   const float **dO) {
+        std::vector<float> ff(n), dd(n);
+        for (size_t i = 0; i < n; ++i) {
+            ff[i] = mu[i] * (1.0 + (float)L * ts[i]);
+            dd[i] = mu[i] * ts[i] / DT;
+        }
+        return upload(m, fo, ff) || upload(m, dO, dd);
+    };
+    if (fij(p.muipkp, p.tausipkp, &a.fipkp, &a.dipkp)) return 1;
+    if (c.ND == 3) {
+        if (fij(p.muipjp, p.tausipjp, &a.fipjp, &a.dipjp)) return 1;
+        if (fij(p.mujpkp, p.tausjpkp, &a.fjpkp, &a.djpkp)) return 1;
+    }
+    return 0;
+}
+
+int alloc_fields(seis_model *m) {
+    const seis_csts &c = m->c;
+    FDArgs<float> &a = m->a;
+    const long nf = m->g.nfield, L = c.L;
+    a.vx = m->dalloc(nf);
+    a.vz = m->dalloc(nf);
+    if (c.ND == 3) a.vy = m->dalloc(nf);
+    if (c.ND == 22) {
+        a.p = m->dalloc(nf);
+        if (L) a.rp = m->dalloc(nf * L);
+    } else {
+        a.sxx = m->dalloc(nf);
+        a.szz = m->dalloc(nf);
+        a.sxz = m->dalloc(nf);
+        if (L) {
+            a.rxx = m->dalloc(nf * L);
+            a.rzz = m->dalloc(nf * L);
+            a.rxz = m->dalloc(nf * L);
+        }
+        if (c.ND == 3) {
+            a.syy = m->dalloc(nf);
+            a.sxy = m->dalloc(nf);
+            a.syz = m->dalloc(nf);
+            if (L) {
+                a.ryy = m->dalloc(nf * L);
+                a.rxy = m->dalloc(nf * L);
+                a.ryz = m->dalloc(nf * L);
+            }
+        }
+    }
+    if (c.abs_type == 1) {
+        const long nab2 = 2L * c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
+        const long sxs = nab2 * NY * NZ, sys = NX * nab2 * NZ, szs = NX * NY * nab2;
+        if (c.ND == 22) {
+            a.psi_p_x = m->dalloc(sxs);
+            a.psi_p_z = m->dalloc(szs);
+            a.psi_vx_x = m->dalloc(sxs);
+            a.psi_vz_z = m->dalloc(szs);
+        } else {
+            a.psi_sxx_x = m->dalloc(sxs);
+            a.psi_sxz_x = m->dalloc(sxs);
+            a.psi_szz_z = m->dalloc(szs);
+            a.psi_sxz_z = m->dalloc(szs);
+            a.psi_vx_x = m->dalloc(sxs);
+            a.psi_vz_x = m->dalloc(sxs);
+            a.psi_vx_z = m->dalloc(szs);
+            a.psi_vz_z = m->dalloc(szs);
+            if (c.ND == 3) {
+                a.psi_sxy_x = m->dalloc(sxs);
+                a.psi_syz_z = m->dalloc(szs);
+                a.psi_vy_x = m->dalloc(sxs);
+                a.psi_vy_z = m->dalloc(szs);
+                a.psi_syy_y = m->dalloc(sys);
+                a.psi_sxy_y = m->dalloc(sys);
+                a.psi_syz_y = m->dalloc(sys);
+                a.psi_vx_y = m->dalloc(sys);
+                a.psi_vy_y = m->dalloc(sys);
+                a.psi_vz_y = m->dalloc(sys);
+            }
+        }
+    }
+    return g_err.empty() ? 0 : 1;
+}
+
+// Zero every wavefield, memory variable and CPML memory of the model
+// (vars_init, src/automatic_kernels.c:291-392).
+void zero_state(seis_model *m) {
+    const FDArgs<float> &a = m->a;
+    const size_t nf = (size_t)m->g.nfield * sizeof(float);
+    const size_t L = m->c.L;
+    float *f1[] = {a.vx, a.vy, a.vz, a.sxx, a.syy, a.szz, a.sxy, a.syz, a.sxz, a.p};
+    for (float *p : f1) launch_fill(p, nf, m->ctx->stream);
+    float *fl[] = {a.rxx, a.ryy, a.rzz, a.rxy, a.ryz, a.rxz, a.rp};
+    for (float *p : fl) launch_fill(p, nf * L, m->ctx->stream);
+    if (m->c.abs_type == 1) {
+        const long nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
+        const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
+        float *px[] = {a.psi_sxx_x, a.psi_sxy_x, a.psi_sxz_x, a.psi_vx_x, a.psi_vy_x, a.psi_vz_x, a.psi_p_x};
+        float *py[] = {a.psi_sxy_y, a.psi_syy_y, a.psi_syz_y, a.psi_vx_y, a.psi_vy_y, a.psi_vz_y};
+        float *pz[] = {a.psi_sxz_z, a.psi_syz_z, a.psi_szz_z, a.psi_vx_z, a.psi_vy_z, a.psi_vz_z, a.psi_p_z};
+        for (float *p : px) launch_fill(p, sx, m->ctx->stream);
+        for (float *p : py) launch_fill(p, sy, m->ctx->stream);
+        for (float *p : pz) launch_fill(p, sz, m->ctx->stream);
+    }
+}
+
+float *field_ptr(seis_model *m, const std::string &n) {
+    const FDArgs<float> &a = m->a;
+    if (n == "vx") return a.vx;
+    if (n == "vy") return a.vy;
+    if (n == "vz") return a.vz;
+    if (n == "sxx") return a.sxx;
+    if (n == "syy") return a.syy;
+    if (n == "szz") return a.szz;
+    if (n == "sxy") return a.sxy;
+    if (n == "syz") return a.syz;
+    if (n == "sxz") return a.sxz;
+    if (n == "p") return a.p;
+    return nullptr;
+}
+
+hipEvent_t pool_event(seis_model *m, size_t k) {
+    while (m->evpool.size() <= k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        m->evpool.push_back(e);
+    }
+    return m->evpool[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *seis_last_error(void) { return g_err.c_str(); }
+
+size_t seis_csts_size(void) { return sizeof(seis_csts); }
+
+void seis_csts_default(seis_csts *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->ND = 2;
+    c->dh = 10;
+    c->dt = 0.0008f;
+    c->NT = 875;
+    c->FDORDER = 8;
+    c->MAXRELERROR = 1;
+    c->abs_type = 1;
+    c->nab = 16;
+    c->abpc = 6;
+    c->VPPML = 3500;
+    c->NPOWER = 2;
+    c->FPML = 15;
+    c->K_MAX_CPML = 2;
+    c->f0 = 15;
+    c->FL[0] = 15;
+    c->seisout = 2;
+}
+
+int seis_ctx_create(int device_ordinal, seis_ctx **out) {
+    g_err.clear();
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device_ordinal < 0 || device_ordinal >= n)
+        return fail("seis_ctx_create: no HIP device " + std::to_string(device_ordinal));
+    auto ctx = std::make_unique<seis_ctx>();
+    ctx->device = device_ordinal;
+    HIP_TRY(hipSetDevice(device_ordinal));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+    return 0;
+}
+
+int seis_ctx_destroy(seis_ctx *ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int seis_ctx_sync(seis_ctx *ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+size_t seis_ctx_footprint(const seis_ctx *ctx) { return ctx ? ctx->bytes : 0; }
+
+int seis_buf_create(seis_ctx *ctx, size_t bytes, void **dptr) { return ctx->alloc(bytes, dptr); }
+int seis_buf_free(seis_ctx *ctx, void *dptr) { return ctx->release(dptr); }
+int seis_buf_send(seis_ctx *ctx, void *dptr, const void *host, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+int seis_buf_read(seis_ctx *ctx, void *host, const void *dptr, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, const float *p1,
+                      const float *rho, const float *taup, const float *taus, seis_model **out) {
+    g_err.clear();
+    *out = nullptr;
+    if (!ctx || !cin) return fail("seis_model_create: null argument");
+    const seis_csts c = *cin;
+    if (c.ND != 2 && c.ND != 3 && c.ND != 22) return fail("ND must be 2, 3 or 22");
+    if (c.L < 0 || c.L > 4) return fail("L must be in [0, 4]");
+    if (c.FP16 != 0) return fail("only FP16=0 is available in this build");
+    if (c.NT <= 0) return fail("NT must be positive");
+    const int nd = c.ND == 3 ? 3 : 2;
+    for (int d = 0; d < nd; ++d) {
+        if (c.N[d] <= 0) return fail("grid sizes must be positive");
+        if (c.abs_type > 0 && c.N[d] < 2 * c.nab)
+            return fail("each grid dimension must hold both absorbing layers (N >= 2*nab)");
+    }
+    if (c.abs_type > 0 && (c.nab < 1 || c.nab > kMaxNab)) return fail("nab out of range");
+    if (c.abs_type == 2 && c.nab < 2) return fail("the Cerjan taper needs nab >= 2");
+    if (c.ND == 22 && c.freesurf) return fail("the free surface is not available for ND=22");
+    if (out_names(c).empty()) return fail("seisout has no outputs for this ND");
+    auto m = std::make_unique<seis_model>();
+    m->ctx = ctx;
+    m->c = c;
+    m->nd_case = c.ND;
+    std::string err;
+    if (build_setup(c, m->s, err)) return fail(err);
+    RefParams rp;
+    if (transform_params(c, p0, p1, rho, taup, taus, rp, err)) return fail(err);
+    float dts = 0;
+    if (check_stability(c, rp, &dts)) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "Time step too large, to be stable, set dt<%f", dts);
+        return fail(buf);
+    }
+    const int NZ = c.N[0], NY = c.ND == 3 ? c.N[1] : 1, NX = c.ND == 3 ? c.N[2] : c.N[1];
+    m->g = make_layout(nd, NZ, NY, NX, c.FDORDER / 2, 4);
+    FDArgs<float> &a = m->a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = m->g;
+    for (int k = 0; k < 7; ++k) a.hc[k] = m->s.hc[k];
+    a.DT = m->s.DT;
+    a.DT2 = m->s.DT2;
+    a.L = c.L;
+    for (int l = 0; l < c.L; ++l) {
+        a.eta[l] = m->s.eta[l];
+        a.bl[l] = m->s.bl[l];
+        a.cl[l] = m->s.cl[l];
+    }
+    a.abs_type = c.abs_type;
+    a.freesurf = c.freesurf;
+    a.nab = c.nab;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (upload_params(m.get(), rp)) return 1;
+    if (c.abs_type == 1) {
+        float *d = m->dalloc((long)m->s.cpml.size());
+        if (!d) return 1;
+        HIP_TRY(hipMemcpyAsync(d, m->s.cpml.data(), m->s.cpml.size() * 4, hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.cpml = d;
+    } else if (c.abs_type == 2) {
+        float *d = m->dalloc((long)m->s.taper.size());
+        if (!d) return 1;
+        HIP_TRY(hipMemcpyAsync(d, m->s.taper.data(), m->s.taper.size() * 4, hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.taper = d;
+    }
+    if (alloc_fields(m.get())) return 1;
+    m->outn = out_names(c);
+    for (auto &n : m->outn) m->out_codes.push_back(out_code(n));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    return 0;
+}
+
+int seis_model_destroy(seis_model *m) {
+    if (!m) return 0;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+    return 0;
+}
+
+int seis_model_set_params(seis_model *m, const float *p0, const float *p1, const float *rho,
+                          const float *taup, const float *taus) {
+    g_err.clear();
+    std::string err;
+    RefParams rp;
+    if (transform_params(m->c, p0, p1, rho, taup, taus, rp, err)) return fail(err);
+    float dts = 0;
+    if (check_stability(m->c, rp, &dts)) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "Time step too large, to be stable, set dt<%f", dts);
+        return fail(buf);
+    }
+    // free the previous coefficient arrays and upload the new ones
+    FDArgs<float> &a = m->a;
+    const float **pp[] = {&a.rip, &a.rjp, &a.rkp, &a.M, &a.mu, &a.muipkp, &a.muipjp, &a.mujpkp,
+                          &a.cg, &a.cf, &a.cd, &a.ce, &a.fipkp, &a.fipjp, &a.fjpkp, &a.dipkp,
+                          &a.dipjp, &a.djpkp};
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    for (const float **p : pp) {
+        if (!*p) continue;
+        auto it = std::find(m->owned.begin(), m->owned.end(), (void *)*p);
+        if (it != m->owned.end()) m->owned.erase(it);
+        m->bytes -= (size_t)m->g.nfield * 4;
+        m->ctx->release((void *)*p);
+        *p = nullptr;
+    }
+    return upload_params(m, rp);
+}
+
+size_t seis_model_footprint(const seis_model *m) { return m ? m->bytes : 0; }
+
+int seis_out_count(const seis_model *m) { return (int)m->outn.size(); }
+const char *seis_out_name(const seis_model *m, int i) {
+    if (i < 0 || i >= (int)m->outn.size()) return nullptr;
+    return m->outn[i].c_str();
+}
+
+int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
+                   const float *rec_pos) {
+    g_err.clear();
+    const seis_csts &c = m->c;
+    const Layout &g = m->g;
+    const int H = g.H, NT = c.NT;
+    const float DH = m->s.DH;
+    std::vector<int> scell(nsrc), stype(nsrc), sk(nsrc), rcell(nrec);
+    auto cell = [&](const float *p, int &k, int &out) -> int {
+        const int i = (int)(p[0] / DH) + H, j = (int)(p[1] / DH) + H;
+        k = (int)(p[2] / DH) + H;
+        if (k < 0 || k >= g.NZ + 2 * H || (c.ND == 3 && (j < 0 || j >= g.NY + 2 * H)))
+            return fail("source or receiver outside the padded grid");
+        if (i < H || i > g.NX + H - 1) {
+            out = -1;   // outside this domain in x: skipped, as the reference does
+            return 0;
+        }
+        out = (int)g.idx(k - H, c.ND == 3 ? j - H : 0, i - H);
+        return 0;
+    };
+    for (int s = 0; s < nsrc; ++s) {
+        if (cell(&src_pos[5 * s], sk[s], scell[s])) return 1;
+        stype[s] = (int)src_pos[5 * s + 4];
+        if (c.ND == 22 && stype[s] == 100)
+            return fail("explosive sources (type 100) are not defined for ND=22; use type 3");
+    }
+    for (int r = 0; r < nrec; ++r) {
+        int k;
+        if (cell(&rec_pos[8 * r], k, rcell[r])) return 1;
+    }
+    seis_ctx *ctx = m->ctx;
+    ShotDev &sd = m->shot;
+    const int nout = (int)m->outn.size();
+    if (nsrc > m->shot_nsrc_cap || nrec > m->shot_nrec_cap || !sd.out) {
+        void *olds[] = {sd.src_cell, sd.src_type, sd.src_k, sd.src, sd.rec_cell, sd.out};
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (void *p : olds) ctx->release(p);
+        const int ns = std::max(nsrc, 1), nr = std::max(nrec, 1);
+        void *p;
+        if (ctx->alloc(ns * 4, &p)) return 1;
+        sd.src_cell = (int *)p;
+        if (ctx->alloc(ns * 4, &p)) return 1;
+        sd.src_type = (int *)p;
+        if (ctx->alloc(ns * 4, &p)) return 1;
+        sd.src_k = (int *)p;
+        if (ctx->alloc((size_t)ns * NT * 4, &p)) return 1;
+        sd.src = (float *)p;
+        if (ctx->alloc(nr * 4, &p)) return 1;
+        sd.rec_cell = (int *)p;
+        if (ctx->alloc((size_t)nr * NT * nout * 4, &p)) return 1;
+        sd.out = (float *)p;
+        m->shot_nsrc_cap = ns;
+        m->shot_nrec_cap = nr;
+    }
+    sd.nsrc = nsrc;
+    sd.nrec = nrec;
+    hipStream_t st = ctx->stream;
+    if (nsrc) {
+        HIP_TRY(hipMemcpyAsync(sd.src_cell, scell.data(), nsrc * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(sd.src_type, stype.data(), nsrc * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(sd.src_k, sk.data(), nsrc * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(sd.src, src, (size_t)nsrc * NT * 4, hipMemcpyHostToDevice, st));
+    }
+    if (nrec) HIP_TRY(hipMemcpyAsync(sd.rec_cell, rcell.data(), nrec * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    m->shot_bound = true;
+    return 0;
+}
+
+int seis_launch_init(seis_model *m) {
+    zero_state(m);
+    if (m->shot.out)
+        launch_fill(m->shot.out, (size_t)m->shot.nrec * m->c.NT * m->outn.size() * 4,
+                    m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int seis_launch_sources(seis_model *m, int nt, int pdir) {
+    if (!m->shot_bound) return fail("no shot bound");
+    launch_sources(m->a, m->nd_case, m->shot, m->c.NT, nt, pdir, m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int seis_launch_update_v(seis_model *m) {
+    launch_update_v(m->a, m->nd_case, m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int seis_launch_update_s(seis_model *m) {
+    launch_update_s(m->a, m->nd_case, m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int seis_launch_surface(seis_model *m) {
+    if (!m->c.freesurf) return 0;
+    launch_surface(m->a, m->nd_case, 1, m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int seis_launch_varsout(seis_model *m, int nt) {
+    if (!m->shot_bound) return fail("no shot bound");
+    launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(), nt,
+                   m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int seis_shot_read(seis_model *m, float *out) {
+    const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
+    std::vector<float> tmp((size_t)NT * nout * nrec);
+    if (!tmp.empty())
+        HIP_TRY(hipMemcpyAsync(tmp.data(), m->shot.out, tmp.size() * 4, hipMemcpyDeviceToHost,
+                               m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    for (int o = 0; o < nout; ++o)
+        for (int r = 0; r < nrec; ++r)
+            for (int t = 0; t < NT; ++t)
+                out[((size_t)o * nrec + r) * NT + t] = tmp[((size_t)t * nout + o) * nrec + r];
+    return 0;
+}
+
+int seis_run_steps(seis_model *m, int t0, int nsteps) {
+    g_err.clear();
+    if (!m->shot_bound) return fail("no shot bound");
+    if (t0 < 0 || nsteps < 0 || t0 + nsteps > m->c.NT) return fail("steps outside [0, NT)");
+    hipStream_t st = m->ctx->stream;
+    hipEvent_t e0 = pool_event(m, 0), e1 = pool_event(m, 1);
+    if (!e0 || !e1) return fail("hipEventCreate failed");
+    const bool prof = m->profiling;
+    if (prof && !pool_event(m, 2 + 4 * (size_t)nsteps)) return fail("hipEventCreate failed");
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int k = 0; k < nsteps; ++k) {
+        const int t = t0 + k;
+        // step order of src/time_stepping.c:785-816
+        launch_sources(m->a, m->nd_case, m->shot, m->c.NT, t, 1, st);
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
+        launch_update_v(m->a, m->nd_case, st);
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
+        launch_update_s(m->a, m->nd_case, st);
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
+        if (m->c.freesurf) launch_surface(m->a, m->nd_case, 1, st);
+        launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(),
+                       t, st);
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    m->ms_loop = ms;
+    m->ms_v = m->ms_s = 0;
+    if (prof) {
+        for (int k = 0; k < nsteps; ++k) {
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, m->evpool[2 + 4 * k], m->evpool[3 + 4 * k]));
+            HIP_TRY(hipEventElapsedTime(&b, m->evpool[4 + 4 * k], m->evpool[5 + 4 * k]));
+            m->ms_v += a;
+            m->ms_s += b;
+        }
+    }
+    return 0;
+}
+
+int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
+                      const float *rec_pos, float *out) {
+    g_err.clear();
+    if (seis_shot_bind(m, nsrc, src_pos, src, nrec, rec_pos)) return 1;
+    if (seis_launch_init(m)) return 1;
+    const int tmax = m->c.tmax > 0 ? std::min(m->c.tmax, m->c.NT) : m->c.NT;
+    if (seis_run_steps(m, 0, tmax)) return 1;
+    return out ? seis_shot_read(m, out) : 0;
+}
+
+int seis_model_set_profiling(seis_model *m, int on) {
+    m->profiling = on != 0;
+    return 0;
+}
+
+int seis_read_field(seis_model *m, const char *name, float *host) {
+    g_err.clear();
+    float *d = field_ptr(m, name ? name : "");
+    if (!d) return fail(std::string("no wavefield named ") + (name ? name : "(null)"));
+    std::vector<float> tmp((size_t)m->g.nfield);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), d, tmp.size() * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    const Layout &g = m->g;
+    const int H = g.H;
+    const long NZp = g.NZ + 2 * H, NYp = g.nd == 3 ? g.NY + 2 * H : 1, NXp = g.NX + 2 * H;
+    for (long x = 0; x < NXp; ++x)
+        for (long y = 0; y < NYp; ++y)
+            for (long z = 0; z < NZp; ++z)
+                host[(x * NYp + y) * NZp + z] =
+                    tmp[g.idx((int)z - H, g.nd == 3 ? (int)y - H : 0, (int)x - H)];
+    return 0;
+}
+
+int seis_last_timing(const seis_model *m, double *ms_v, double *ms_s, double *ms_loop) {
+    if (ms_v) *ms_v = m->ms_v;
+    if (ms_s) *ms_s = m->ms_s;
+    if (ms_loop) *ms_loop = m->ms_loop;
+    return 0;
+}
+
+}  // extern "C"

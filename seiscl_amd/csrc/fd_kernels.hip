@@ -1,0 +1,816 @@
+// Staggered-grid velocity / stress kernels for gfx950.
+//
+// One work-item owns one interior cell; a 256-lane block covers 64 cells of
+// a z row (one wave, a contiguous 256-byte line pair of every field) times
+// 4 rows in y (3D) or x (2D).  Blocks are renumbered so that consecutive
+// hardware blocks, which the dispatcher deals round-robin to the 8 XCDs,
+// get tiles that are far apart, and the tiles one XCD runs back to back are
+// neighbours: the halo rows a block reads were fetched into that XCD's L2 by
+// its neighbour.
+//
+// Arithmetic follows the reference kernels term by term (src/update_v3D.cl,
+// src/update_s3D.cl, src/update_v2D.cl, src/update_s2D.cl, the acoustic
+// src/update_{v,s}2D_acc.cl, src/surface{2D,3D}.cl and the generated
+// sources/varsout of src/automatic_kernels.c) with the same operation order;
+// the build disables FMA contraction, so results are bit-identical to a
+// C restatement compiled the same way.
+#include <hip/hip_runtime.h>
+
+#include "seis_internal.hpp"
+
+namespace seis {
+
+namespace {
+
+constexpr int BZ = 64;   // lanes along z
+constexpr int BT = 4;    // rows along y (3D) or x (2D)
+
+// Forward / backward staggered derivatives, src/header_FD.cl:87-265.
+template <int H, typename R>
+__device__ __forceinline__ R Dp(const R *__restrict__ v, long i, long s, const R *hc) {
+    R acc = hc[1] * (v[i + s] - v[i]);
+#pragma unroll
+    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (v[i + k * s] - v[i - (k - 1) * s]);
+    return acc;
+}
+template <int H, typename R>
+__device__ __forceinline__ R Dm(const R *__restrict__ v, long i, long s, const R *hc) {
+    R acc = hc[1] * (v[i] - v[i - s]);
+#pragma unroll
+    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (v[i + (k - 1) * s] - v[i - k * s]);
+    return acc;
+}
+
+// CPML coefficient access: cpml[axis][kind][2*nab]
+enum { CK = 0, CA = 1, CB = 2, CKH = 3, CAH = 4, CBH = 5 };
+template <typename R>
+__device__ __forceinline__ R cp(const FDArgs<R> &a, int axis, int kind, int i) {
+    return a.cpml[(axis * 6 + kind) * 2 * a.nab + i];
+}
+
+// psi <- b psi + a d ;  d <- d / K + psi   (src/update_v3D.cl:210-314)
+template <typename R>
+__device__ __forceinline__ void pml(R &d, R *psi, R b, R aa, R K) {
+    R ps = b * *psi + aa * d;
+    *psi = ps;
+    d = d / K + ps;
+}
+
+// Renumber blocks: hardware block b runs on XCD b % 8; give each XCD a
+// contiguous slab of tiles.
+__device__ __forceinline__ void xcd_tile(unsigned nb, unsigned &tile) {
+    const unsigned b = blockIdx.x;
+    const unsigned per = (nb + 7) / 8;
+    const unsigned xcd = b % 8, slot = b / 8;
+    tile = xcd * per + slot;
+}
+
+// ---------------------------------------------------------------------------
+// 3D velocity: src/update_v3D.cl:34-369
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_update_v3d(FDArgs<R> a, int nbz, int nby, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, by = (tile / nbz) % nby, x = tile / (nbz * nby);
+    const int z = bz * BZ + threadIdx.x, y = by * BT + threadIdx.y;
+    const Layout &g = a.g;
+    if (z >= g.NZ || y >= g.NY) return;
+    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
+    const R *hc = a.hc;
+    R szz_z = Dp<H>(a.szz, i, 1, hc);
+    R sxx_x = Dp<H>(a.sxx, i, sx, hc);
+    R sxz_x = Dm<H>(a.sxz, i, sx, hc);
+    R sxz_z = Dm<H>(a.sxz, i, 1, hc);
+    R syz_y = Dm<H>(a.syz, i, sy, hc);
+    R syz_z = Dm<H>(a.syz, i, 1, hc);
+    R syy_y = Dp<H>(a.syy, i, sy, hc);
+    R sxy_x = Dm<H>(a.sxy, i, sx, hc);
+    R sxy_y = Dm<H>(a.sxy, i, sy, hc);
+    if (a.abs_type == 1) {
+        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const int k = z < NAB ? z : z - NZ + 2 * NAB;
+            const int ki = z < NAB ? k : 2 * NAB - k;      // integer grid
+            const int kh = z < NAB ? k : 2 * NAB - 1 - k;  // half grid
+            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+            pml(sxz_z, a.psi_sxz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
+            pml(syz_z, a.psi_syz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
+            pml(szz_z, a.psi_szz_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
+        }
+        if (y < NAB || y > NY - NAB - 1) {
+            const int j = y < NAB ? y : y - NY + 2 * NAB;
+            const int ji = y < NAB ? j : 2 * NAB - j;
+            const int jh = y < NAB ? j : 2 * NAB - 1 - j;
+            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+            pml(sxy_y, a.psi_sxy_y + ip, cp(a, 1, CB, ji), cp(a, 1, CA, ji), cp(a, 1, CK, ji));
+            pml(syy_y, a.psi_syy_y + ip, cp(a, 1, CBH, jh), cp(a, 1, CAH, jh), cp(a, 1, CKH, jh));
+            pml(syz_y, a.psi_syz_y + ip, cp(a, 1, CB, ji), cp(a, 1, CA, ji), cp(a, 1, CK, ji));
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii;
+            const int iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
+            pml(sxx_x, a.psi_sxx_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
+            pml(sxy_x, a.psi_sxy_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
+            pml(sxz_x, a.psi_sxz_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
+        }
+    }
+    R vx = a.vx[i] + ((sxx_x + sxy_y + sxz_z) * a.rip[i]);
+    R vy = a.vy[i] + ((syy_y + sxy_x + syz_z) * a.rjp[i]);
+    R vz = a.vz[i] + ((szz_z + sxz_x + syz_y) * a.rkp[i]);
+    if (a.abs_type == 2) {
+        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+        const R *t = a.taper;
+        // applied one edge after the other, as the reference does
+        if (!a.freesurf && z < NAB) { vx *= t[z]; vy *= t[z]; vz *= t[z]; }
+        if (z > NZ - NAB - 1) { vx *= t[NZ - z - 1]; vy *= t[NZ - z - 1]; vz *= t[NZ - z - 1]; }
+        if (y < NAB) { vx *= t[y]; vy *= t[y]; vz *= t[y]; }
+        if (y > NY - NAB - 1) { vx *= t[NY - y - 1]; vy *= t[NY - y - 1]; vz *= t[NY - y - 1]; }
+        if (x < NAB) { vx *= t[x]; vy *= t[x]; vz *= t[x]; }
+        if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vy *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
+    }
+    a.vx[i] = vx;
+    a.vy[i] = vy;
+    a.vz[i] = vz;
+}
+
+// ---------------------------------------------------------------------------
+// 3D stress: src/update_s3D.cl:35-489.  For L>0 the per-cell coefficients
+// g, f, d, e, f_ij, d_ij (:301-340) are precomputed on the host with the
+// reference's expressions; they are pure functions of the parameters.
+// ---------------------------------------------------------------------------
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_update_s3d(FDArgs<R> a, int nbz, int nby, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, by = (tile / nbz) % nby, x = tile / (nbz * nby);
+    const int z = bz * BZ + threadIdx.x, y = by * BT + threadIdx.y;
+    const Layout &g = a.g;
+    if (z >= g.NZ || y >= g.NY) return;
+    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
+    const R *hc = a.hc;
+    R vzx = Dp<H>(a.vz, i, sx, hc);
+    R vzy = Dp<H>(a.vz, i, sy, hc);
+    R vzz = Dm<H>(a.vz, i, 1, hc);
+    R vyx = Dp<H>(a.vy, i, sx, hc);
+    R vyy = Dm<H>(a.vy, i, sy, hc);
+    R vyz = Dp<H>(a.vy, i, 1, hc);
+    R vxx = Dm<H>(a.vx, i, sx, hc);
+    R vxy = Dp<H>(a.vx, i, sy, hc);
+    R vxz = Dp<H>(a.vx, i, 1, hc);
+    if (a.abs_type == 1) {
+        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const int k = z < NAB ? z : z - NZ + 2 * NAB;
+            const int ki = z < NAB ? k : 2 * NAB - k;
+            const int kh = z < NAB ? k : 2 * NAB - 1 - k;
+            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+            pml(vxz, a.psi_vx_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
+            pml(vyz, a.psi_vy_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
+            pml(vzz, a.psi_vz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
+        }
+        if (y < NAB || y > NY - NAB - 1) {
+            const int j = y < NAB ? y : y - NY + 2 * NAB;
+            const int ji = y < NAB ? j : 2 * NAB - j;
+            const int jh = y < NAB ? j : 2 * NAB - 1 - j;
+            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+            pml(vxy, a.psi_vx_y + ip, cp(a, 1, CBH, jh), cp(a, 1, CAH, jh), cp(a, 1, CKH, jh));
+            pml(vyy, a.psi_vy_y + ip, cp(a, 1, CB, ji), cp(a, 1, CA, ji), cp(a, 1, CK, ji));
+            pml(vzy, a.psi_vz_y + ip, cp(a, 1, CBH, jh), cp(a, 1, CAH, jh), cp(a, 1, CKH, jh));
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii;
+            const int iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
+            pml(vxx, a.psi_vx_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
+            pml(vyx, a.psi_vy_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
+            pml(vzx, a.psi_vz_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
+        }
+    }
+    const R vxyyx = vxy + vyx;
+    const R vyzzy = vyz + vzy;
+    const R vxzzx = vxz + vzx;
+    const R vxxyyzz = vxx + vyy + vzz;
+    const R vyyzz = vyy + vzz;
+    const R vxxzz = vxx + vzz;
+    const R vxxyy = vxx + vyy;
+    R sxy = a.sxy[i], syz = a.syz[i], sxz = a.sxz[i];
+    R sxx = a.sxx[i], syy = a.syy[i], szz = a.szz[i];
+    if constexpr (L == 0) {
+        const R fipjp = a.muipjp[i], fjpkp = a.mujpkp[i], fipkp = a.muipkp[i];
+        const R gg = a.M[i];
+        const R f = (R)2 * a.mu[i];   // exact in either precision
+        sxy += (fipjp * vxyyx);
+        syz += (fjpkp * vyzzy);
+        sxz += (fipkp * vxzzx);
+        sxx += ((gg * vxxyyzz) - (f * vyyzz));
+        syy += ((gg * vxxyyzz) - (f * vxxzz));
+        szz += ((gg * vxxyyzz) - (f * vxxyy));
+    } else {
+        const R DT2 = a.DT2;
+        const R gg = a.cg[i], f = a.cf[i], d = a.cd[i], e = a.ce[i];
+        const R fipjp = a.fipjp[i], fjpkp = a.fjpkp[i], fipkp = a.fipkp[i];
+        const R dipjp = a.dipjp[i], djpkp = a.djpkp[i], dipkp = a.dipkp[i];
+        const long nf = g.nfield;
+        R rxy[L], ryz[L], rxz[L], rxx[L], ryy[L], rzz[L];
+        R sumrxy = 0, sumryz = 0, sumrxz = 0, sumrxx = 0, sumryy = 0, sumrzz = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const long ir = l * nf + i;
+            rxy[l] = a.rxy[ir]; ryz[l] = a.ryz[ir]; rxz[l] = a.rxz[ir];
+            rxx[l] = a.rxx[ir]; ryy[l] = a.ryy[ir]; rzz[l] = a.rzz[ir];
+            sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
+            sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
+        }
+        const R lsxy = (fipjp * vxyyx) + (DT2 * sumrxy);
+        const R lsyz = (fjpkp * vyzzy) + (DT2 * sumryz);
+        const R lsxz = (fipkp * vxzzx) + (DT2 * sumrxz);
+        const R lsxx = ((gg * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
+        const R lsyy = ((gg * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
+        const R lszz = ((gg * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
+        sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
+            const long ir = l * nf + i;
+            rxy[l] = b * (rxy[l] * c - et * (dipjp * vxyyx));
+            ryz[l] = b * (ryz[l] * c - et * (djpkp * vyzzy));
+            rxz[l] = b * (rxz[l] * c - et * (dipkp * vxzzx));
+            rxx[l] = b * (rxx[l] * c - et * ((e * vxxyyzz) - (d * vyyzz)));
+            ryy[l] = b * (ryy[l] * c - et * ((e * vxxyyzz) - (d * vxxzz)));
+            rzz[l] = b * (rzz[l] * c - et * ((e * vxxyyzz) - (d * vxxyy)));
+            a.rxy[ir] = rxy[l]; a.ryz[ir] = ryz[l]; a.rxz[ir] = rxz[l];
+            a.rxx[ir] = rxx[l]; a.ryy[ir] = ryy[l]; a.rzz[ir] = rzz[l];
+            sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
+            sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
+        }
+        sxy += lsxy + (DT2 * sumrxy);
+        syz += lsyz + (DT2 * sumryz);
+        sxz += lsxz + (DT2 * sumrxz);
+        sxx += lsxx + (DT2 * sumrxx);
+        syy += lsyy + (DT2 * sumryy);
+        szz += lszz + (DT2 * sumrzz);
+    }
+    if (a.abs_type == 2) {
+        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+        const R *t = a.taper;
+        R *s6[6] = {&sxy, &syz, &sxz, &sxx, &syy, &szz};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            R &s = *s6[q];
+            if (!a.freesurf && z < NAB) s *= t[z];
+            if (z > NZ - NAB - 1) s *= t[NZ - z - 1];
+            if (y < NAB) s *= t[y];
+            if (y > NY - NAB - 1) s *= t[NY - y - 1];
+            if (x < NAB) s *= t[x];
+            if (x > NX - NAB - 1) s *= t[NX - x - 1];
+        }
+    }
+    a.sxy[i] = sxy; a.syz[i] = syz; a.sxz[i] = sxz;
+    a.sxx[i] = sxx; a.syy[i] = syy; a.szz[i] = szz;
+}
+
+// ---------------------------------------------------------------------------
+// 2D P-SV: src/update_v2D.cl:32-246, src/update_s2D.cl:30-326
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_update_v2d(FDArgs<R> a, int nbz, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, bx = tile / nbz;
+    const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    const Layout &g = a.g;
+    if (z >= g.NZ || x >= g.NX) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = a.hc;
+    R sxx_x = Dp<H>(a.sxx, i, sx, hc);
+    R szz_z = Dp<H>(a.szz, i, 1, hc);
+    R sxz_z = Dm<H>(a.sxz, i, 1, hc);
+    R sxz_x = Dm<H>(a.sxz, i, sx, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const int k = z > NZ - NAB - 1 ? z - NZ + 2 * NAB : z;
+            const bool bot = z > NZ - NAB - 1;
+            const int ki = bot ? 2 * NAB - k : k, kh = bot ? 2 * NAB - 1 - k : k;
+            const long ip = (long)x * 2 * NAB + k;
+            pml(sxz_z, a.psi_sxz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
+            pml(szz_z, a.psi_szz_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NZ + z;
+            pml(sxx_x, a.psi_sxx_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
+            pml(sxz_x, a.psi_sxz_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
+        }
+    }
+    R vx = a.vx[i] + ((sxx_x + sxz_z) * a.rip[i]);
+    R vz = a.vz[i] + ((szz_z + sxz_x) * a.rkp[i]);
+    if (a.abs_type == 2) {
+        const R *t = a.taper;
+        if (!a.freesurf && z < NAB) { vx *= t[z]; vz *= t[z]; }
+        if (z > NZ - NAB - 1) { vx *= t[NZ - z - 1]; vz *= t[NZ - z - 1]; }
+        if (x < NAB) { vx *= t[x]; vz *= t[x]; }
+        if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
+    }
+    a.vx[i] = vx;
+    a.vz[i] = vz;
+}
+
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_update_s2d(FDArgs<R> a, int nbz, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, bx = tile / nbz;
+    const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    const Layout &g = a.g;
+    if (z >= g.NZ || x >= g.NX) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = a.hc;
+    R vxx = Dm<H>(a.vx, i, sx, hc);
+    R vxz = Dp<H>(a.vx, i, 1, hc);
+    R vzz = Dm<H>(a.vz, i, 1, hc);
+    R vzx = Dp<H>(a.vz, i, sx, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int ki = bot ? 2 * NAB - k : k, kh = bot ? 2 * NAB - 1 - k : k;
+            const long ip = (long)x * 2 * NAB + k;
+            pml(vxz, a.psi_vx_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
+            pml(vzz, a.psi_vz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NZ + z;
+            pml(vxx, a.psi_vx_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
+            pml(vzx, a.psi_vz_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
+        }
+    }
+    R sxz = a.sxz[i], sxx = a.sxx[i], szz = a.szz[i];
+    if constexpr (L == 0) {
+        const R fipkp = a.muipkp[i];
+        const R f = (R)2 * a.mu[i];
+        const R gg = a.M[i];
+        sxz += (fipkp * (vxz + vzx));
+        sxx += (gg * (vxx + vzz)) - (f * vzz);
+        szz += (gg * (vxx + vzz)) - (f * vxx);
+    } else {
+        const R DT2 = a.DT2;
+        const R gg = a.cg[i], f = a.cf[i], d = a.cd[i], e = a.ce[i];
+        const R fipkp = a.fipkp[i], dipkp = a.dipkp[i];
+        const long nf = g.nfield;
+        R rxz[L], rxx[L], rzz[L];
+        R sumrxz = 0, sumrxx = 0, sumrzz = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const long ir = l * nf + i;
+            rxz[l] = a.rxz[ir]; rxx[l] = a.rxx[ir]; rzz[l] = a.rzz[ir];
+            sumrxz += rxz[l]; sumrxx += rxx[l]; sumrzz += rzz[l];
+        }
+        const R lsxz = (fipkp * (vxz + vzx)) + (DT2 * sumrxz);
+        const R lsxx = ((gg * (vxx + vzz)) - (f * vzz)) + (DT2 * sumrxx);
+        const R lszz = ((gg * (vxx + vzz)) - (f * vxx)) + (DT2 * sumrzz);
+        sumrxz = sumrxx = sumrzz = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
+            const long ir = l * nf + i;
+            rxz[l] = b * (rxz[l] * c - et * (dipkp * (vxz + vzx)));
+            rxx[l] = b * (rxx[l] * c - et * ((e * (vxx + vzz)) - (d * vzz)));
+            rzz[l] = b * (rzz[l] * c - et * ((e * (vxx + vzz)) - (d * vxx)));
+            a.rxz[ir] = rxz[l]; a.rxx[ir] = rxx[l]; a.rzz[ir] = rzz[l];
+            sumrxz += rxz[l]; sumrxx += rxx[l]; sumrzz += rzz[l];
+        }
+        sxz += lsxz + (DT2 * sumrxz);
+        sxx += lsxx + (DT2 * sumrxx);
+        szz += lszz + (DT2 * sumrzz);
+    }
+    if (a.abs_type == 2) {
+        const R *t = a.taper;
+        if (!a.freesurf && z < NAB) { sxx *= t[z]; szz *= t[z]; sxz *= t[z]; }
+        if (z > NZ - NAB - 1) { sxx *= t[NZ - z - 1]; szz *= t[NZ - z - 1]; sxz *= t[NZ - z - 1]; }
+        if (x < NAB) { sxx *= t[x]; szz *= t[x]; sxz *= t[x]; }
+        if (x > NX - NAB - 1) { sxx *= t[NX - x - 1]; szz *= t[NX - x - 1]; sxz *= t[NX - x - 1]; }
+    }
+    a.sxz[i] = sxz;
+    a.sxx[i] = sxx;
+    a.szz[i] = szz;
+}
+
+// ---------------------------------------------------------------------------
+// 2D acoustic: src/update_v2D_acc.cl:80-325, src/update_s2D_acc.cl:79-402
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_update_v2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, bx = tile / nbz;
+    const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    const Layout &g = a.g;
+    if (z >= g.NZ || x >= g.NX) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = a.hc;
+    R p_x = Dp<H>(a.p, i, sx, hc);
+    R p_z = Dp<H>(a.p, i, 1, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int kh = bot ? 2 * NAB - 1 - k : k;
+            pml(p_z, a.psi_p_z + (long)x * 2 * NAB + k, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh),
+                cp(a, 2, CKH, kh));
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            pml(p_x, a.psi_p_x + (long)ii * NZ + z, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH),
+                cp(a, 0, CKH, iH));
+        }
+    }
+    R vx = a.vx[i] + (p_x * a.rip[i]);
+    R vz = a.vz[i] + (p_z * a.rkp[i]);
+    if (a.abs_type == 2) {
+        const R *t = a.taper;
+        // the acoustic kernel tapers the top edge even with a free surface
+        if (z < NAB) { vx *= t[z]; vz *= t[z]; }
+        if (z > NZ - NAB - 1) { vx *= t[NZ - z - 1]; vz *= t[NZ - z - 1]; }
+        if (x < NAB) { vx *= t[x]; vz *= t[x]; }
+        if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
+    }
+    a.vx[i] = vx;
+    a.vz[i] = vz;
+}
+
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_update_s2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, bx = tile / nbz;
+    const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    const Layout &g = a.g;
+    if (z >= g.NZ || x >= g.NX) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = a.hc;
+    R vxx = Dm<H>(a.vx, i, sx, hc);
+    R vzz = Dm<H>(a.vz, i, 1, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int ki = bot ? 2 * NAB - k : k;
+            pml(vzz, a.psi_vz_z + (long)x * 2 * NAB + k, cp(a, 2, CB, ki), cp(a, 2, CA, ki),
+                cp(a, 2, CK, ki));
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii;
+            pml(vxx, a.psi_vx_x + (long)ii * NZ + z, cp(a, 0, CB, iI), cp(a, 0, CA, iI),
+                cp(a, 0, CK, iI));
+        }
+    }
+    R p = a.p[i];
+    if constexpr (L == 0) {
+        p += (a.M[i] * (vxx + vzz));
+    } else {
+        const R DT2 = a.DT2, gg = a.cg[i], e = a.ce[i];
+        const long nf = g.nfield;
+        R rp[L];
+        R sumrp = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) { rp[l] = a.rp[l * nf + i]; sumrp += rp[l]; }
+        const R lp = (gg * (vxx + vzz)) + (DT2 * sumrp);
+        sumrp = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            rp[l] = a.bl[l] * (rp[l] * a.cl[l] - a.eta[l] * e * (vxx + vzz));
+            a.rp[l * nf + i] = rp[l];
+            sumrp += rp[l];
+        }
+        p += lp + (DT2 * sumrp);
+    }
+    if (a.abs_type == 2) {
+        const R *t = a.taper;
+        if (z < NAB) p *= t[z];
+        if (z > NZ - NAB - 1) p *= t[NZ - z - 1];
+        if (x < NAB) p *= t[x];
+        if (x > NX - NAB - 1) p *= t[NX - x - 1];
+    }
+    a.p[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// Free surface by the image method (Levander 1988): src/surface2D.cl:30-193,
+// src/surface3D.cl:39-271.  One lane per surface column.
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ void k_surface2d(FDArgs<R> a, int pdir) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const Layout &g = a.g;
+    if (x >= g.NX) return;
+    const long i = g.idx(0, 0, x), sx = g.sx, nf = g.nfield;
+    const int L = a.L, NAB = a.nab, NX = g.NX, NZ = g.NZ;
+    a.szz[i] = 0;
+    for (int l = 0; l < L; ++l) a.rzz[l * nf + i] = 0;
+    for (int m = 1; m <= H; ++m) {
+        a.szz[i - m] = -a.szz[i + m];
+        a.sxz[i - m] = -a.sxz[i + m - 1];
+    }
+    R vxx = Dm<H>(a.vx, i, sx, a.hc);
+    R vzz = Dm<H>(a.vz, i, 1, a.hc);
+    if (a.abs_type == 1) {
+        if (x < NAB) vxx = vxx / cp(a, 0, CK, x) + a.psi_vx_x[(long)x * NZ];
+        if (x > NX - NAB - 1) {
+            const int ii = x - NX + 2 * NAB;
+            vxx = vxx / cp(a, 0, CK, 2 * NAB - ii) + a.psi_vx_x[(long)ii * NZ];
+        }
+    }
+    const R fp = (R)pdir;
+    if (L == 0) {
+        const R f = a.mu[i] * (R)2, gg = a.M[i];
+        const R h = -((gg - f) * (gg - f) * (vxx) / gg) - ((gg - f) * vzz);
+        a.sxx[i] += fp * h;
+    } else {
+        const R f = a.cf[i], gg = a.cg[i], d = a.cd[i], e = a.ce[i], DT2 = a.DT2;
+        R h = -((gg - f) * (gg - f) * (vxx) / gg) - ((gg - f) * vzz);
+        R sump = 0;
+        for (int l = 0; l < L; ++l) sump += a.rxx[l * nf + i];
+        h += -DT2 * sump;
+        a.sxx[i] += fp * h;
+        sump = 0;
+        for (int l = 0; l < L; ++l) {
+            // b = eta/(1+eta/2) and the (f/g - 1.0) factor are doubles in
+            // the reference expression
+            const double b = (double)a.eta[l] / (1.0 + ((double)a.eta[l] * 0.5));
+            const R bb = (R)b;
+            const R hh = (R)((double)bb * ((double)(d - e) * ((double)(f / gg) - 1.0) * (double)vxx -
+                                          (double)((d - e) * vzz)));
+            a.rxx[l * nf + i] += fp * hh;
+            sump += a.rxx[l * nf + i];
+        }
+        a.sxx[i] += fp * (DT2 * sump);
+    }
+}
+
+template <typename R, int H>
+__global__ void k_surface3d(FDArgs<R> a, int pdir) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    const Layout &g = a.g;
+    if (y >= g.NY) return;
+    const long i = g.idx(0, y, x), sx = g.sx, sy = g.sy, nf = g.nfield;
+    const int L = a.L, NAB = a.nab, NX = g.NX, NY = g.NY, NZ = g.NZ;
+    a.szz[i] = 0;
+    for (int l = 0; l < L; ++l) a.rzz[l * nf + i] = 0;
+    for (int m = 1; m <= H; ++m) {
+        a.szz[i - m] = -a.szz[i + m];
+        a.sxz[i - m] = -a.sxz[i + m - 1];
+        a.syz[i - m] = -a.syz[i + m - 1];
+    }
+    R vxx = Dm<H>(a.vx, i, sx, a.hc);
+    R vyy = Dm<H>(a.vy, i, sy, a.hc);
+    R vzz = Dm<H>(a.vz, i, 1, a.hc);
+    if (a.abs_type == 1) {
+        if (y < NAB) vyy = vyy / cp(a, 1, CK, y) + a.psi_vy_y[(long)x * 2 * NAB * NZ + (long)y * NZ];
+        else if (y > NY - NAB - 1) {
+            const int j = y - NY + 2 * NAB;
+            vyy = vyy / cp(a, 1, CK, 2 * NAB - j) + a.psi_vy_y[(long)x * 2 * NAB * NZ + (long)j * NZ];
+        }
+        if (x < NAB) vxx = vxx / cp(a, 0, CK, x) + a.psi_vx_x[(long)x * NY * NZ + (long)y * NZ];
+        if (x > NX - NAB - 1) {
+            const int ii = x - NX + 2 * NAB;
+            vxx = vxx / cp(a, 0, CK, 2 * NAB - ii) + a.psi_vx_x[(long)ii * NY * NZ + (long)y * NZ];
+        }
+    }
+    if (L == 0) {
+        const R f = a.mu[i] * (R)2, gg = a.M[i];
+        const R h = -((gg - f) * (gg - f) * (vxx + vyy) / gg) - ((gg - f) * vzz);
+        a.sxx[i] += h;   // the 3D elastic kernel does not apply pdir
+        a.syy[i] += h;
+    } else {
+        const R fp = (R)pdir;
+        const R f = a.cf[i], gg = a.cg[i], d = a.cd[i], e = a.ce[i], DT2 = a.DT2;
+        const R h = -((gg - f) * (gg - f) * (vxx + vyy) / gg) - ((gg - f) * vzz);
+        R sumx = 0, sumy = 0;
+        for (int l = 0; l < L; ++l) { sumx += a.rxx[l * nf + i]; sumy += a.ryy[l * nf + i]; }
+        a.sxx[i] += fp * (h - DT2 * sumx);
+        a.syy[i] += fp * (h - DT2 * sumy);
+        sumx = 0;
+        sumy = 0;
+        for (int l = 0; l < L; ++l) {
+            const double b = (double)a.eta[l] / (1.0 + ((double)a.eta[l] * 0.5));
+            const R bb = (R)b;
+            const R hh = (R)((double)bb * ((double)(d - e) * ((double)(f / gg) - 1.0) * (double)(vxx + vyy) -
+                                          (double)((d - e) * vzz)));
+            a.rxx[l * nf + i] += hh;
+            a.ryy[l * nf + i] += hh;
+            sumx += a.rxx[l * nf + i];
+            sumy += a.ryy[l * nf + i];
+        }
+        a.sxx[i] += (DT2 * sumx);
+        a.syy[i] += (DT2 * sumy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sources (src/automatic_kernels.c:394-629) and receivers (:22-219).
+// One lane walks all sources in order, so sources sharing a cell add up in
+// a fixed order.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_sources(FDArgs<R> a, int nd_case, ShotDev s, int NT, int nt, int pdir) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int k = 0; k < s.nsrc; ++k) {
+        const int cell = s.src_cell[k];
+        if (cell < 0) continue;
+        if (a.freesurf && pdir == -1 && s.src_k[k] < 2 * a.g.H) continue;
+        const int type = s.src_type[k];
+        const R amp = (R)pdir * ((R)a.DT * (R)s.src[(long)k * NT + nt]);
+        if (type == 0) a.vx[cell] += amp;
+        else if (type == 1) {
+            if (a.vy) a.vy[cell] += amp;
+        } else if (type == 2) a.vz[cell] += amp;
+        else if (type != 3 && type != 100) continue;
+        else if (nd_case == 22) a.p[cell] += amp;
+        else {
+            const double nd = nd_case == 3 ? 3.0 : 2.0;   // "+=amp/%f" (double)
+            a.sxx[cell] = (R)((double)a.sxx[cell] + (double)amp / nd);
+            a.szz[cell] = (R)((double)a.szz[cell] + (double)amp / nd);
+            if (nd_case == 3) a.syy[cell] = (R)((double)a.syy[cell] + (double)amp / nd);
+        }
+    }
+}
+
+struct OutList {
+    int n;
+    int code[10];
+};
+
+template <typename R>
+__global__ void k_varsout(FDArgs<R> a, int nd_case, ShotDev s, OutList o, int nt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.nrec) return;
+    const int cell = s.rec_cell[r];
+    if (cell < 0) return;
+    float *out = s.out + (long)nt * o.n * s.nrec;
+    for (int q = 0; q < o.n; ++q) {
+        R v;
+        switch (o.code[q]) {
+            case O_VX: v = a.vx[cell]; break;
+            case O_VY: v = a.vy[cell]; break;
+            case O_VZ: v = a.vz[cell]; break;
+            case O_SXX: v = a.sxx[cell]; break;
+            case O_SYY: v = a.syy[cell]; break;
+            case O_SZZ: v = a.szz[cell]; break;
+            case O_SXZ: v = a.sxz[cell]; break;
+            case O_SXY: v = a.sxy[cell]; break;
+            case O_SYZ: v = a.syz[cell]; break;
+            default:
+                if (nd_case == 22) v = a.p[cell];
+                else {
+                    R sm = a.sxx[cell] + a.szz[cell];
+                    if (nd_case == 3) sm = sm + a.syy[cell];
+                    v = (R)((double)sm / (nd_case == 3 ? 3.0 : 2.0));
+                }
+        }
+        out[(long)q * s.nrec + r] = (float)v;
+    }
+}
+
+__global__ void k_fill(uint32_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+template <typename R, int H>
+void dispatch_v(const FDArgs<R> &a, int nd_case, hipStream_t st) {
+    const Layout &g = a.g;
+    const int nbz = (g.NZ + BZ - 1) / BZ;
+    dim3 blk(BZ, BT);
+    if (nd_case == 3) {
+        const int nby = (g.NY + BT - 1) / BT;
+        const unsigned nb = (unsigned)nbz * nby * g.NX;
+        hipLaunchKernelGGL((k_update_v3d<R, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nby, nb);
+    } else {
+        const int nbx = (g.NX + BT - 1) / BT;
+        const unsigned nb = (unsigned)nbz * nbx;
+        if (nd_case == 22)
+            hipLaunchKernelGGL((k_update_v2d_acc<R, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+        else
+            hipLaunchKernelGGL((k_update_v2d<R, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+    }
+}
+
+template <typename R, int H, int L>
+void dispatch_s_L(const FDArgs<R> &a, int nd_case, hipStream_t st) {
+    const Layout &g = a.g;
+    const int nbz = (g.NZ + BZ - 1) / BZ;
+    dim3 blk(BZ, BT);
+    if (nd_case == 3) {
+        const int nby = (g.NY + BT - 1) / BT;
+        const unsigned nb = (unsigned)nbz * nby * g.NX;
+        hipLaunchKernelGGL((k_update_s3d<R, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nby, nb);
+    } else {
+        const int nbx = (g.NX + BT - 1) / BT;
+        const unsigned nb = (unsigned)nbz * nbx;
+        if (nd_case == 22)
+            hipLaunchKernelGGL((k_update_s2d_acc<R, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+        else
+            hipLaunchKernelGGL((k_update_s2d<R, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+    }
+}
+
+template <typename R, int H>
+void dispatch_s(const FDArgs<R> &a, int nd_case, hipStream_t st) {
+    switch (a.L) {
+        case 0: dispatch_s_L<R, H, 0>(a, nd_case, st); break;
+        case 1: dispatch_s_L<R, H, 1>(a, nd_case, st); break;
+        case 2: dispatch_s_L<R, H, 2>(a, nd_case, st); break;
+        case 3: dispatch_s_L<R, H, 3>(a, nd_case, st); break;
+        case 4: dispatch_s_L<R, H, 4>(a, nd_case, st); break;
+        default: break;   // rejected at model creation
+    }
+}
+
+template <typename R, int H>
+void dispatch_surface(const FDArgs<R> &a, int nd_case, int pdir, hipStream_t st) {
+    const Layout &g = a.g;
+    if (nd_case == 3)
+        hipLaunchKernelGGL((k_surface3d<R, H>), dim3((g.NY + 63) / 64, g.NX), dim3(64), 0, st, a, pdir);
+    else
+        hipLaunchKernelGGL((k_surface2d<R, H>), dim3((g.NX + 63) / 64), dim3(64), 0, st, a, pdir);
+}
+
+}  // namespace
+
+#define SEIS_SWITCH_H(H, CALL)                   \
+    switch (H) {                                 \
+        case 1: { constexpr int HH = 1; CALL; } break; \
+        case 2: { constexpr int HH = 2; CALL; } break; \
+        case 3: { constexpr int HH = 3; CALL; } break; \
+        case 4: { constexpr int HH = 4; CALL; } break; \
+        case 5: { constexpr int HH = 5; CALL; } break; \
+        case 6: { constexpr int HH = 6; CALL; } break; \
+        default: break;                          \
+    }
+
+template <typename R>
+void launch_update_v(const FDArgs<R> &a, int nd_case, void *stream) {
+    SEIS_SWITCH_H(a.g.H, (dispatch_v<R, HH>(a, nd_case, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_update_s(const FDArgs<R> &a, int nd_case, void *stream) {
+    SEIS_SWITCH_H(a.g.H, (dispatch_s<R, HH>(a, nd_case, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream) {
+    SEIS_SWITCH_H(a.g.H, (dispatch_surface<R, HH>(a, nd_case, pdir, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_sources(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT, int nt,
+                    int pdir, void *stream) {
+    if (s.nsrc <= 0) return;
+    hipLaunchKernelGGL((k_sources<R>), dim3(1), dim3(64), 0, (hipStream_t)stream, a, nd_case, s,
+                       NT, nt, pdir);
+}
+template <typename R>
+void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int *codes,
+                    int nout, int nt, void *stream) {
+    if (s.nrec <= 0) return;
+    OutList o;
+    o.n = nout;
+    for (int q = 0; q < nout && q < 10; ++q) o.code[q] = codes[q];
+    hipLaunchKernelGGL((k_varsout<R>), dim3((s.nrec + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, a, nd_case, s, o, nt);
+}
+
+void launch_fill(void *ptr, size_t bytes, void *stream) {
+    if (!ptr || bytes == 0) return;
+    const size_t n = bytes / 4;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint32_t *)ptr, n);
+}
+
+template void launch_update_v<float>(const FDArgs<float> &, int, void *);
+template void launch_update_s<float>(const FDArgs<float> &, int, void *);
+template void launch_surface<float>(const FDArgs<float> &, int, int, void *);
+template void launch_sources<float>(const FDArgs<float> &, int, const ShotDev &, int, int, int, void *);
+template void launch_varsout<float>(const FDArgs<float> &, int, const ShotDev &, const int *, int, int, void *);
+
+}  // namespace seis

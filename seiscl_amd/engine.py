@@ -1,0 +1,218 @@
+"""Python handles over the C-ABI: a GPU context and a resident model.
+
+Array conventions are SeisCL.py's (SeisCL/SeisCL.py:414-602):
+  - model parameters are numpy arrays shaped [NZ, NX] or [NZ, NY, NX];
+  - src_pos is (5, nsrc), src is (NT, nsrc), rec_pos is (8, nrec);
+  - seismograms come back as (NT, nrec) arrays, one per output variable.
+The C side receives the transposed (x-slowest, z-fastest) buffers, exactly
+what the reference's HDF5 files hold after SeisCL.py's np.transpose.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SeisCsts, check, fptr
+
+# SeisCL.py constructor defaults (SeisCL/SeisCL.py:36-83) for the keys the
+# engine uses.
+CSTS_DEFAULTS = dict(
+    ND=2, dh=10.0, dt=0.0008, NT=875, L=0, f0=15.0, FL=(15.0,), FDORDER=8,
+    MAXRELERROR=1, FP16=0, freesurf=0, abs_type=1, VPPML=3500.0, NPOWER=2.0,
+    FPML=15.0, K_MAX_CPML=2.0, nab=16, abpc=6.0, seisout=2, param_type=0,
+    tmax=0.0, tmin=0.0, gradout=0,
+)
+
+PARAM_NAMES = {0: ("vp", "vs", "rho"), 1: ("M", "mu", "rho"), 2: ("Ip", "Is", "rho")}
+
+
+def make_csts(**kw) -> SeisCsts:
+    """Build a seis_csts from SeisCL.py-style keyword constants."""
+    d = dict(CSTS_DEFAULTS)
+    d.update(kw)
+    c = SeisCsts()
+    _lib.load().seis_csts_default(ctypes.byref(c))
+    N = [int(v) for v in np.atleast_1d(d["N"])]
+    c.ND = int(d["ND"])
+    for i, v in enumerate(N):
+        c.N[i] = v
+    c.NT = int(d["NT"])
+    c.dt = float(d["dt"])
+    c.dh = float(d["dh"])
+    c.FDORDER = int(d["FDORDER"])
+    c.MAXRELERROR = int(d["MAXRELERROR"])
+    c.freesurf = int(d["freesurf"])
+    c.abs_type = int(d["abs_type"])
+    c.nab = int(d["nab"])
+    c.abpc = float(d["abpc"])
+    c.VPPML = float(d["VPPML"])
+    c.NPOWER = float(d["NPOWER"])
+    c.FPML = float(d["FPML"])
+    c.K_MAX_CPML = float(d["K_MAX_CPML"])
+    c.L = int(d["L"])
+    c.f0 = float(d["f0"])
+    fl = np.atleast_1d(np.asarray(d["FL"], dtype=np.float32))
+    for i in range(min(len(fl), _lib.MAXL)):
+        c.FL[i] = float(fl[i])
+    c.seisout = int(d["seisout"])
+    c.param_type = int(d["param_type"])
+    # tmax/tmin are seconds in SeisCL.py; the C side counts steps
+    # (src/read_hdf5.c:523-528)
+    c.tmax = int(np.float32(d["tmax"]) / np.float32(c.dt)) if d["tmax"] else 0
+    c.tmin = int(np.float32(d["tmin"]) / np.float32(c.dt)) if d["tmin"] else 0
+    c.gradout = int(d["gradout"])
+    c.FP16 = int(d["FP16"])
+    return c
+
+
+def to_c_layout(a: np.ndarray) -> np.ndarray:
+    """[NZ, (NY,) NX] numpy array -> flat float32, x slowest, z fastest."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).T).ravel()
+
+
+def from_c_layout(flat: np.ndarray, shape) -> np.ndarray:
+    """Inverse of to_c_layout for a grid of the given [NZ, (NY,) NX] shape."""
+    return np.ascontiguousarray(flat.reshape(tuple(shape)[::-1]).T)
+
+
+def param_arrays(params: dict, param_type: int, L: int, ND: int):
+    """Order the parameter dict as the C-ABI wants it (p0, p1, rho, taup, taus)."""
+    names = PARAM_NAMES.get(param_type)
+    if names is None:
+        names = ("vpR", "vsR", "rho")
+    out = []
+    for k, n in enumerate(names):
+        if ND == 22 and k == 1:
+            out.append(None)
+            continue
+        if param_type == 3 and k < 2:
+            n = ("vpR", "vsR")[k]
+        out.append(to_c_layout(params[n]))
+    if L > 0:
+        tp = "vpI" if param_type == 3 else "taup"
+        ts = "vsI" if param_type == 3 else "taus"
+        out.append(to_c_layout(params[tp]))
+        out.append(to_c_layout(params[ts]) if ND != 22 else None)
+    else:
+        out += [None, None]
+    return out
+
+
+def shot_arrays(src_pos, src, rec_pos):
+    """SeisCL.py (5,ns)/(NT,ns)/(8,nr) arrays -> C row-major float32 buffers."""
+    sp = np.ascontiguousarray(np.asarray(src_pos, dtype=np.float32).T)
+    s = np.ascontiguousarray(np.asarray(src, dtype=np.float32).T)
+    rp = np.ascontiguousarray(np.asarray(rec_pos, dtype=np.float32).T)
+    return sp, s, rp
+
+
+class Context:
+    """One GPU (seis_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.lib.seis_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+
+    def footprint(self) -> int:
+        return int(self.lib.seis_ctx_footprint(self.handle))
+
+    def sync(self):
+        check(self.lib.seis_ctx_sync(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.seis_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model:
+    """A modeling case resident on one GPU (seis_model)."""
+
+    def __init__(self, ctx: Context, params: dict, **csts):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.csts = make_csts(**csts)
+        self.shape = tuple(int(v) for v in np.atleast_1d(csts["N"]))
+        self._keep = param_arrays(params, self.csts.param_type, self.csts.L, self.csts.ND)
+        h = ctypes.c_void_p()
+        check(self.lib.seis_model_create(ctx.handle, ctypes.byref(self.csts),
+                                         *[fptr(a) for a in self._keep], ctypes.byref(h)))
+        self.handle = h
+        n = self.lib.seis_out_count(h)
+        self.out_names = [self.lib.seis_out_name(h, i).decode() for i in range(n)]
+
+    def set_params(self, params: dict):
+        self._keep = param_arrays(params, self.csts.param_type, self.csts.L, self.csts.ND)
+        check(self.lib.seis_model_set_params(self.handle, *[fptr(a) for a in self._keep]))
+
+    def footprint(self) -> int:
+        return int(self.lib.seis_model_footprint(self.handle))
+
+    def set_profiling(self, on: bool):
+        check(self.lib.seis_model_set_profiling(self.handle, int(bool(on))))
+
+    def timing(self):
+        v, s, loop = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self.lib.seis_last_timing(self.handle, ctypes.byref(v), ctypes.byref(s),
+                                        ctypes.byref(loop)))
+        return dict(update_v_ms=v.value, update_s_ms=s.value, loop_ms=loop.value)
+
+    def forward(self, src_pos, src, rec_pos, raw: bool = False):
+        """Model one shot; returns a list of (NT, nrec) seismograms in
+        ``self.out_names`` order (raw=True: the C array [nout][nrec][NT])."""
+        sp, s, rp = shot_arrays(src_pos, src, rec_pos)
+        nsrc, nrec = sp.shape[0], rp.shape[0]
+        NT = self.csts.NT
+        out = np.zeros((len(self.out_names), nrec, NT), dtype=np.float32)
+        check(self.lib.seis_forward_shot(self.handle, nsrc, fptr(sp), fptr(s), nrec,
+                                         fptr(rp), fptr(out)))
+        if raw:
+            return out
+        return [np.ascontiguousarray(o.T) for o in out]
+
+    def bind_shot(self, src_pos, src, rec_pos):
+        """Upload one shot and zero the wavefields (first half of forward)."""
+        self._shot = shot_arrays(src_pos, src, rec_pos)
+        sp, s, rp = self._shot
+        check(self.lib.seis_shot_bind(self.handle, sp.shape[0], fptr(sp), fptr(s), rp.shape[0],
+                                      fptr(rp)))
+        check(self.lib.seis_launch_init(self.handle))
+
+    def run_steps(self, t0: int, nsteps: int):
+        """Advance the bound shot by nsteps time steps starting at step t0."""
+        check(self.lib.seis_run_steps(self.handle, int(t0), int(nsteps)))
+
+    def read_shot(self):
+        nrec = self._shot[2].shape[0]
+        out = np.zeros((len(self.out_names), nrec, self.csts.NT), dtype=np.float32)
+        check(self.lib.seis_shot_read(self.handle, fptr(out)))
+        return [np.ascontiguousarray(o.T) for o in out]
+
+    def read_field(self, name: str) -> np.ndarray:
+        """Wavefield in the reference's padded layout, reshaped [NZp, (NYp,) NXp]."""
+        H = self.csts.FDORDER // 2
+        shp = [n + 2 * H for n in self.shape]
+        flat = np.zeros(int(np.prod(shp)), dtype=np.float32)
+        check(self.lib.seis_read_field(self.handle, name.encode(), fptr(flat)))
+        return from_c_layout(flat, shp)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.seis_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,118 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Each builder returns ``(csts, params, shots)`` where ``csts`` holds
+SeisCL.py-style constants, ``params`` the material arrays ([NZ, NX] or
+[NZ, NY, NX]) and ``shots`` a list of ``(src_pos, src, rec_pos)`` triplets in
+SeisCL.py shapes ((5, ns), (NT, ns), (8, nr)).
+
+Smooth random fields are white noise passed through a Gaussian filter of
+sigma = correlation length / dh cells, normalised to max |.| = 1.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+
+def ricker(f0: float, NT: int, dt: float, tmin: float | None = None) -> np.ndarray:
+    """Ricker wavelet of SeisCL.ricker_wavelet (SeisCL/SeisCL.py:810-834)."""
+    if tmin is None:
+        tmin = -1.5 / f0
+    t = np.linspace(tmin, (NT - 1) * dt + tmin, num=int(NT))
+    return ((1.0 - 2.0 * (np.pi ** 2) * (f0 ** 2) * (t ** 2))
+            * np.exp(-(np.pi ** 2) * (f0 ** 2) * (t ** 2)))
+
+
+def smooth(shape, corr_len: float, dh: float, rng) -> np.ndarray:
+    g = gaussian_filter(rng.standard_normal(shape), sigma=corr_len / dh, mode="nearest")
+    return g / np.abs(g).max()
+
+
+def q_to_tau(Q: float, fmin: float, fmax: float, FL, Qmin: float = 5.0,
+             Qmax: float = 500.0) -> float:
+    """tau giving an average 1/Q over [fmin, fmax] for relaxation frequencies
+    FL: the calibration table of QTAU.Q2tau (SeisCL/Q_tau.py:25-131), i.e.
+    Q(w) = (1/tau + sum w^2 t_l^2/(1+w^2 t_l^2)) / sum w t_l/(1+w^2 t_l^2),
+    t_l = 1/(2 pi FL_l), tabulated on 40 log-spaced tau and interpolated
+    linearly in log-log."""
+    taus = [1.0 / f / 2.0 / np.pi for f in np.atleast_1d(FL)]
+    omega = 2 * np.pi * np.arange(fmin, fmax, 0.1)
+    tau = np.linspace(np.log(1 / Qmax / 2), np.log(1 / Qmin * 2), 40)
+    Qi = np.zeros(len(tau))
+    for ii, lt in enumerate(tau):
+        f1 = sum(omega ** 2 * t ** 2 / (1.0 + omega ** 2 * t ** 2) for t in taus)
+        f2 = sum(omega * t / (1.0 + omega ** 2 * t ** 2) for t in taus)
+        q = (1 / np.exp(lt) + f1) / f2
+        Qi[ii] = np.average(1 / q)
+    return float(np.exp(np.interp(np.log(1.0 / Q), np.log(Qi), tau)))
+
+
+def layered(shape, nlayers: int, vmin: float, vmax: float, rng, axis: int = 0) -> np.ndarray:
+    """1D layered velocity increasing with depth (axis 0 = z), seeded interfaces."""
+    nz = shape[axis]
+    cuts = np.sort(rng.choice(np.arange(1, nz), size=nlayers - 1, replace=False))
+    v1 = np.zeros(nz)
+    vals = np.linspace(vmin, vmax, nlayers)
+    edges = np.concatenate([[0], cuts, [nz]])
+    for k in range(nlayers):
+        v1[edges[k]:edges[k + 1]] = vals[k]
+    sh = [1] * len(shape)
+    sh[axis] = nz
+    return np.broadcast_to(v1.reshape(sh), shape).copy()
+
+
+def config1(NT: int = 2000, seed: int = 1):
+    """Plain 2D isotropic elastic, one shot: 200 x 400, dh 10 m, dt 1 ms,
+    FD4, CPML 20, no free surface, vz Ricker 10 Hz at (50 m, 2000 m),
+    400 receivers at z = 50 m."""
+    rng = np.random.default_rng(seed)
+    N = (200, 400)
+    dh, dt = 10.0, 1e-3
+    vp = 3500 * (1 + 0.05 * smooth(N, 100, dh, rng))
+    vs = 2000 * (1 + 0.05 * smooth(N, 100, dh, rng))
+    rho = 2000 * (1 + 0.05 * smooth(N, 100, dh, rng))
+    csts = dict(N=np.array(N), ND=2, dh=dh, dt=dt, NT=NT, FDORDER=4, MAXRELERROR=1,
+                abs_type=1, nab=20, freesurf=0, L=0, f0=10.0, seisout=1)
+    src_pos = np.array([[2000.0], [0.0], [50.0], [0.0], [2.0]])
+    src = ricker(10.0, NT, dt)[:, None]
+    gx = np.arange(400) * dh
+    rec_pos = np.stack([gx, 0 * gx, 0 * gx + 50.0, 0 * gx, np.arange(400) + 1.0,
+                        0 * gx, 0 * gx, 0 * gx])
+    params = dict(vp=vp.astype(np.float32), vs=vs.astype(np.float32),
+                  rho=rho.astype(np.float32))
+    return csts, params, [(src_pos, src, rec_pos)]
+
+
+def config2(n: int = 256, NT: int = 1500, seed: int = 2, nrec_side: int = 64):
+    """Plain 3D viscoelastic, one shot: n^3, dh 10 m, dt 0.8 ms, FD4, CPML 16,
+    free surface, L=2 (FL 5 and 25 Hz, Q=50), layered vp 1800-4500,
+    vs = vp/1.8, rho 1800-2600, 3% smooth perturbations, 12 Hz Ricker
+    explosion, 64 x 64 surface receivers."""
+    rng = np.random.default_rng(seed)
+    N = (n, n, n)
+    dh, dt = 10.0, 8e-4
+    vp = layered(N, 8, 1800.0, 4500.0, rng)
+    pert = smooth(N, 100, dh, rng)
+    vp = vp * (1 + 0.03 * pert)
+    vs = vp / 1.8
+    rho = 1800 + 800 * (vp - 1800) / 2700
+    FL = [5.0, 25.0]
+    tau = q_to_tau(50.0, 5.0, 25.0, FL)
+    taup = np.full(N, tau, np.float32)
+    taus = np.full(N, tau, np.float32)
+    csts = dict(N=np.array(N), ND=3, dh=dh, dt=dt, NT=NT, FDORDER=4, MAXRELERROR=1,
+                abs_type=1, nab=16, freesurf=1, L=2, FL=np.array(FL), f0=12.0, seisout=1)
+    c = n // 2 * dh
+    src_pos = np.array([[c], [c], [10 * dh], [0.0], [100.0]])
+    src = ricker(12.0, NT, dt)[:, None]
+    k = min(nrec_side, n // 3)
+    off = (n - 3 * (k - 1)) // 2
+    gi = (off + 3 * np.arange(k)) * dh
+    gx, gy = np.meshgrid(gi, gi, indexing="ij")
+    gx, gy = gx.ravel(), gy.ravel()
+    nr = gx.size
+    rec_pos = np.stack([gx, gy, 0 * gx + dh, 0 * gx, np.arange(nr) + 1.0, 0 * gx,
+                        0 * gx, 0 * gx])
+    params = dict(vp=vp.astype(np.float32), vs=vs.astype(np.float32),
+                  rho=rho.astype(np.float32), taup=taup, taus=taus)
+    return csts, params, [(src_pos, src, rec_pos)]

@@ -1,0 +1,349 @@
+"""Pins of the CPU oracle against what the mathematics and the reference's
+own known answers fix.  These run on the CPU (no GPU marker).
+
+Coverage of oracle functions:
+  or_holberg / or_kernel_hc  -> Taylor closed form, Holberg dispersion bound,
+                                 "%9.9f" literal quantisation
+  or_cpml_coeff              -> Komatitsch-Martin profile recovered from (K,a,b)
+  or_taper                   -> Cerjan end values and Gaussian shape
+  or_transform               -> homogeneous closed forms, harmonic/arithmetic
+                                 staggered averages, GSLS phase-velocity
+                                 normalisation, the zero-edge quirk of the
+                                 reference averages
+  or_check_stability         -> CFL closed form
+  or_forward                 -> analytic Green's functions (2D/3D, elastic and
+                                 viscoelastic), reciprocity, exact linearity,
+                                 mirror symmetry, CPML absorption, free-surface
+                                 traction, receiver skipping
+"""
+import numpy as np
+import pytest
+
+from analytic import force_z_2d, force_z_3d
+from cases import rel_l2, small_case
+from seiscl_amd.workloads import ricker
+
+# ---------------------------------------------------------------- FD coefficients
+
+
+def _taylor_staggered(n):
+    """Staggered-grid first-derivative Taylor weights: sum_k c_k (2k-1)^(2m-1)
+    = delta_{m,1}, m = 1..n."""
+    A = np.array([[(2 * k - 1) ** (2 * m - 1) for k in range(1, n + 1)] for m in range(1, n + 1)],
+                 dtype=np.float64)
+    rhs = np.zeros(n)
+    rhs[0] = 1.0
+    return np.linalg.solve(A, rhs)
+
+
+@pytest.mark.parametrize("fd", [2, 4, 6, 8, 10, 12])
+def test_taylor_coefficients_closed_form(oracle_lib, fd):
+    hc = oracle_lib.holberg(fd, 0)
+    ref = _taylor_staggered(fd // 2)
+    np.testing.assert_allclose(hc[1:fd // 2 + 1], ref, rtol=2e-7)
+    assert np.all(hc[fd // 2 + 1:] == 0)
+
+
+@pytest.mark.parametrize("mre,E", [(1, 1e-3), (2, 5e-3), (3, 1e-2), (4, 3e-2)])
+@pytest.mark.parametrize("fd", [4, 6, 8, 10, 12])
+def test_holberg_dispersion_bound(oracle_lib, fd, mre, E):
+    """Holberg weights keep the phase error of the staggered derivative within
+    E up to the table's minimum points per wavelength."""
+    hc = oracle_lib.holberg(fd, mre)
+    ppw, c = hc[0], hc[1:fd // 2 + 1]
+    k = np.arange(1, fd // 2 + 1)
+    th = np.linspace(1e-4, 2 * np.pi / ppw, 4000)
+    D = np.sum(c[None] * 2 * np.sin((2 * k - 1)[None] * th[:, None] / 2), 1) / th
+    assert np.abs(D - 1).max() <= 1.05 * E
+    # and a plausibly wrong table entry would break it
+    cbad = c.copy()
+    cbad[-1] *= 1.5
+    Dbad = np.sum(cbad[None] * 2 * np.sin((2 * k - 1)[None] * th[:, None] / 2), 1) / th
+    assert np.abs(Dbad - 1).max() > 1.05 * E
+
+
+def test_kernel_literals_are_9_decimal_text(oracle_lib):
+    # HC values reach the kernels as "%9.9f" text (src/clprogram.c:323-327)
+    for fd, mre in [(4, 1), (8, 1), (12, 0), (12, 4)]:
+        hc = oracle_lib.holberg(fd, mre)
+        got = oracle_lib.kernel_hc(fd, mre)
+        for k in range(1, 7):
+            assert got[k] == np.float32(float("%9.9f" % hc[k]))
+    # the 9-decimal text loses digits of small Taylor weights: FD12 HC6
+    hc6 = oracle_lib.kernel_hc(12, 0)[6]
+    assert hc6 == np.float32(-0.000021848)
+    assert hc6 != np.float32(-63.0 / 2883584.0)
+    assert oracle_lib.load().or_literal9(0.0008) == np.float32(0.0008)
+
+
+# ---------------------------------------------------------------- CPML / taper
+
+
+@pytest.mark.parametrize("nab,npow,kmax,fpml,vppml,dh,dt", [
+    (20, 2.0, 2.0, 15.0, 3500.0, 10.0, 1e-3),
+    (16, 2.0, 1.0, 12.0, 4500.0, 10.0, 8e-4),
+    (8, 3.0, 4.0, 25.0, 2000.0, 5.0, 5e-4),
+])
+def test_cpml_profile_recovered(oracle_lib, nab, npow, kmax, fpml, vppml, dh, dt):
+    """Invert (K, a, b) back to the damping d, the frequency shift alpha and
+    the normalised depth x, and check them against the published profile
+    d = d0 (x/4 + 3 x^n / 4), K = 1 + (Kmax-1) x^n, alpha = pi fpml (1-x),
+    d0 = -(n+1) vppml ln(R) / (2 nab dh), R = 8e-4 (Komatitsch & Martin 2007)."""
+    c = oracle_lib.cpml(npow, kmax, fpml, vppml, dh, dt, nab)
+    assert c["state"] == 0
+    d0 = -(npow + 1) * vppml * np.log(0.0008) / (2 * nab * dh)
+    for K, a, b, xs in [(c["K"][:nab + 1], c["a"][:nab + 1], c["b"][:nab + 1],
+                         (nab - np.arange(nab + 1)) / nab),
+                        (c["K_half"][:nab], c["a_half"][:nab], c["b_half"][:nab],
+                         (nab - np.arange(nab) - 0.5) / nab)]:
+        K, a, b = K.astype(np.float64), a.astype(np.float64), b.astype(np.float64)
+        s = -np.log(b) / dt                       # d/K + alpha
+        d = np.where(a != 0, a * K ** 2 * s / np.where(b != 1, b - 1, 1), 0.0)
+        alpha = s - d / K
+        np.testing.assert_allclose(d, d0 * (0.25 * xs + 0.75 * xs ** npow), rtol=2e-3, atol=1e-3)
+        np.testing.assert_allclose(alpha, np.pi * fpml * (1 - xs), rtol=2e-3, atol=1e-2)
+        if kmax > 1:
+            np.testing.assert_allclose(K, 1 + (kmax - 1) * xs ** npow, rtol=1e-6)
+    # far side mirrors the near side
+    np.testing.assert_array_equal(c["K"][1:nab], c["K"][2 * nab - 1:nab:-1])
+    np.testing.assert_array_equal(c["b_half"][:nab], c["b_half"][2 * nab - 1:nab - 1:-1])
+    assert c["K"][0] == np.float32(kmax) and c["K"][nab] == 1.0 and c["a"][nab] == 0.0
+
+
+def test_cerjan_taper(oracle_lib):
+    nab, abpc = 30, 6.0
+    t = oracle_lib.taper(nab, abpc).astype(np.float64)
+    assert t[-1] == 1.0
+    assert abs(t[0] - (1 - abpc / 100)) < 1e-6
+    q = np.log(t)
+    n = nab - 1 - np.arange(nab)
+    np.testing.assert_allclose(q, q[0] * (n / n[0]) ** 2, rtol=1e-5, atol=1e-7)
+
+
+# ---------------------------------------------------------------- transforms
+
+
+def test_transform_homogeneous_closed_forms(oracle_lib):
+    N = [12, 10, 14]
+    vp, vs, rho, dt, dh = 3000.0, 1700.0, 2200.0, 1e-3, 10.0
+    p = dict(vp=np.full(N, vp, np.float32), vs=np.full(N, vs, np.float32),
+             rho=np.full(N, rho, np.float32))
+    t = oracle_lib.transform(p, N=N, ND=3, dt=dt, dh=dh)
+    np.testing.assert_allclose(t["M"], vp ** 2 * rho * dt / dh, rtol=1e-6)
+    np.testing.assert_allclose(t["mu"], vs ** 2 * rho * dt / dh, rtol=1e-6)
+    for k in ("rho", "rip", "rjp", "rkp"):
+        np.testing.assert_allclose(t[k], dt / (dh * rho), rtol=1e-6)
+    mu = vs ** 2 * rho * dt / dh
+    # harmonic 4-point averages: interior equals mu; the second edge of each
+    # average stays zero except its corner line (reference quirk, see
+    # ave_harmonic, src/assign_modeling_case.c:225-301)
+    m = t["muipkp"]
+    np.testing.assert_allclose(m[:-1, :, :], mu, rtol=1e-6)
+    assert np.all(m[-1, :, :-1] == 0) and np.all(m[-1, :, -1] != 0)
+    m = t["mujpkp"]
+    np.testing.assert_allclose(m[:-1, :, :], mu, rtol=1e-6)
+    assert np.all(m[-1, :-1, :] == 0)
+    m = t["muipjp"]
+    np.testing.assert_allclose(m[:, :-1, :], mu, rtol=1e-6)
+    assert np.all(m[:, -1, :-1] == 0)
+
+
+def test_transform_staggered_averages(oracle_lib):
+    """Two layers: buoyancy is the arithmetic mean and shear modulus the
+    harmonic mean across the interface (effective staggered media)."""
+    N = [8, 6]
+    vs = np.full(N, 1000.0, np.float32)
+    vs[4:] = 2000.0
+    rho = np.full(N, 2000.0, np.float32)
+    rho[4:] = 2500.0
+    p = dict(vp=vs * 2, vs=vs, rho=rho)
+    dt, dh = 1e-3, 5.0
+    t = oracle_lib.transform(p, N=N, ND=2, dt=dt, dh=dh)
+    mu1, mu2 = 1000.0 ** 2 * 2000 * dt / dh, 2000.0 ** 2 * 2500 * dt / dh
+    b1, b2 = dt / (dh * 2000), dt / (dh * 2500)
+    assert np.isclose(t["rkp"][3, 2], 0.5 * (b1 + b2), rtol=1e-6)
+    assert np.isclose(t["rip"][3, 2], b1, rtol=1e-6)
+    assert np.isclose(t["muipkp"][3, 2], 4 / (2 / mu1 + 2 / mu2), rtol=1e-6)
+
+
+def test_transform_visco_phase_velocity(oracle_lib):
+    """The relaxed modulus makes the GSLS phase velocity at f0 equal vp:
+    Re M(w0) = M_R (1 + tau sum w0^2 t_l^2 / (1 + w0^2 t_l^2)) = vp^2 rho dt/dh."""
+    N = [6, 5]
+    vp, vs, rho, tau, f0 = 3000.0, 1800.0, 2100.0, 0.05, 12.0
+    FL = [5.0, 25.0]
+    dt, dh = 8e-4, 10.0
+    p = dict(vp=np.full(N, vp, np.float32), vs=np.full(N, vs, np.float32),
+             rho=np.full(N, rho, np.float32), taup=np.full(N, tau, np.float32),
+             taus=np.full(N, 2 * tau, np.float32))
+    t = oracle_lib.transform(p, N=N, ND=2, dt=dt, dh=dh, L=2, FL=FL, f0=f0)
+    w0 = 2 * np.pi * f0
+    tl = 1 / (2 * np.pi * np.array(FL))
+    s = np.sum(w0 ** 2 * tl ** 2 / (1 + w0 ** 2 * tl ** 2))
+    np.testing.assert_allclose(t["M"] * (1 + tau * s), vp ** 2 * rho * dt / dh, rtol=1e-5)
+    np.testing.assert_allclose(t["mu"] * (1 + 2 * tau * s), vs ** 2 * rho * dt / dh, rtol=1e-5)
+
+
+def test_stability_closed_form(oracle_lib):
+    N = [20, 20]
+    p = dict(vp=np.full(N, 4000.0, np.float32), vs=np.full(N, 2000.0, np.float32),
+             rho=np.full(N, 2000.0, np.float32))
+    bad, dts = oracle_lib.check_stability(p, N=N, ND=2, dt=1e-3, dh=10.0, FDORDER=4)
+    assert not bad
+    assert np.isclose(dts, 10.0 / (1.184614 * np.sqrt(2.0) * 4000.0), rtol=1e-5)
+    bad, _ = oracle_lib.check_stability(p, N=N, ND=2, dt=2e-3, dh=10.0, FDORDER=4)
+    assert bad
+
+
+# ---------------------------------------------------------------- forward physics
+
+
+def _analytic_setup(ND, N, dh, nab, FD=4, NT=800, dt=0.25e-3, f0=20.0):
+    nbuf = FD * 2
+    sx = N // 2 * dh
+    sz = (nab + nbuf) * dh
+    gz = np.arange(sz + 5 * dh, sz + (N - nab - nbuf) * dh - sz, dh)
+    gx = gz * 0 + sx
+    src_pos = np.array([[sx], [sx if ND == 3 else 0.0], [sz], [0.0], [2.0]])
+    rec_pos = np.stack([gx, gx * 0 + (sx if ND == 3 else 0.0), gz, 0 * gz,
+                        np.arange(gz.size) + 1.0, 0 * gz + 2, 0 * gz, 0 * gz])
+    src = ricker(f0, NT, dt)[:, None]
+    offs = np.stack([gx - sx, gx * 0, gz - sz], 1)
+    return src_pos, src, rec_pos, offs
+
+
+@pytest.mark.parametrize("L,tau", [(0, 0.0), (1, 0.1)])
+def test_analytic_2d(oracle_lib, L, tau):
+    N, dh, nab, NT, dt, f0 = 300, 2.0, 112, 800, 0.25e-3, 20.0
+    vp, vs, rho = 3500.0, 2000.0, 2000.0
+    sp, s, rp, offs = _analytic_setup(2, N, dh, nab, NT=NT, dt=dt, f0=f0)
+    FL = np.array([f0 / (l + 1) for l in range(max(L, 1))])
+    shp = (N, N)
+    p = dict(vp=np.full(shp, vp, np.float32), vs=np.full(shp, vs, np.float32),
+             rho=np.full(shp, rho, np.float32), taup=np.full(shp, tau, np.float32),
+             taus=np.full(shp, tau, np.float32))
+    out = oracle_lib.forward(p, sp, s, rp, N=[N, N], ND=2, dh=dh, dt=dt, NT=NT, FDORDER=4,
+                             abs_type=2, nab=nab, abpc=3.0, L=L, FL=FL, f0=f0, seisout=1)
+    fd = out[-1].astype(np.float64)
+    src2 = ricker(f0, 2 * NT, dt)
+    ana = force_z_2d(vp, vs, rho, tau, tau, f0, FL if L else [], dt, offs[:, [0, 2]], src2)[1][:NT]
+    err = rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana))
+    assert err < 6e-2, err       # tolerance of SeisCL/tests/test_analytics.py:142
+
+
+def test_analytic_3d(oracle_lib):
+    N, dh, nab, NT, dt, f0 = 90, 4.0, 20, 500, 0.4e-3, 25.0
+    vp, vs, rho = 3500.0, 2000.0, 2000.0
+    sp, s, rp, offs = _analytic_setup(3, N, dh, nab, FD=4, NT=NT, dt=dt, f0=f0)
+    shp = (N, N, N)
+    p = dict(vp=np.full(shp, vp, np.float32), vs=np.full(shp, vs, np.float32),
+             rho=np.full(shp, rho, np.float32))
+    out = oracle_lib.forward(p, sp, s, rp, N=[N, N, N], ND=3, dh=dh, dt=dt, NT=NT, FDORDER=4,
+                             abs_type=1, nab=nab, f0=f0, seisout=1, VPPML=vp, FPML=f0)
+    fd = out[-1].astype(np.float64)
+    ana = force_z_3d(vp, vs, rho, 0.0, 0.0, f0, [], dt, offs, ricker(f0, 2 * NT, dt))[2][:NT]
+    err = rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana))
+    assert err < 6e-2, err
+
+
+@pytest.mark.parametrize("ND", [2, 3])
+def test_reciprocity(oracle_lib, ND):
+    """z force at A recorded as vz at B equals z force at B recorded at A
+    (heterogeneous vp/vs, uniform density so the force scaling cancels)."""
+    csts, p, _ = small_case(ND=ND, FDORDER=4, seed=3, NT=150)
+    p["rho"][:] = 2000.0
+    N = csts["N"]
+    dh = csts["dh"]
+    nd = len(N)
+    A = [10 * dh, (N[1] // 2) * dh if nd == 3 else 0.0, 12 * dh]
+    B = [(N[-1] - 14) * dh, (N[1] // 2 - 3) * dh if nd == 3 else 0.0, (N[0] - 13) * dh]
+    s = ricker(15.0, csts["NT"], csts["dt"])[:, None]
+
+    def shot(S, R):
+        sp = np.array([[S[0]], [S[1]], [S[2]], [0], [2]])
+        rp = np.array([[R[0]], [R[1]], [R[2]], [0], [1], [0], [0], [0]])
+        return oracle_lib.forward(p, sp, s, rp, **csts)[nd - 1][:, 0]
+
+    ab, ba = shot(A, B), shot(B, A)
+    assert np.abs(ab).max() > 0
+    assert rel_l2(ab, ba) < 1e-5
+
+
+def test_linearity_exact(oracle_lib):
+    """Scaling the sources by 2 scales every output by 2.  Exact except where
+    subnormal intermediates (the tails ahead of the wavefront) round in
+    absolute terms, so the check is relative L2 < 1e-12."""
+    csts, p, (sp, s, rp) = small_case(ND=3, FDORDER=4, L=2, freesurf=1, seed=5, NT=80)
+    a = oracle_lib.forward(p, sp, s, rp, **csts)
+    b = oracle_lib.forward(p, sp, 2 * s, rp, **csts)
+    for x, y in zip(a, b):
+        assert np.abs(x).max() > 0
+        assert rel_l2(2 * x, y) < 1e-12
+
+
+def test_mirror_symmetry(oracle_lib):
+    """Homogeneous model, explosive source on the symmetry axis: vz is even
+    and vx odd about the source (vx is staggered by half a cell).  No absorbing
+    layer (the reference's integer-grid CPML profile is offset by one cell
+    between the two sides) and the run ends before the boundaries reflect."""
+    N = (60, 61)
+    dh = 10.0
+    p = dict(vp=np.full(N, 3000, np.float32), vs=np.full(N, 1700, np.float32),
+             rho=np.full(N, 2000, np.float32))
+    x0 = 30
+    sp = np.array([[x0 * dh], [0], [30 * dh], [0], [100]])
+    xs = np.arange(5, 56)
+    rp = np.stack([xs * dh, 0 * xs, 0 * xs + 40 * dh, 0 * xs, xs + 1.0, 0 * xs, 0 * xs, 0 * xs])
+    NT = 150
+    out = oracle_lib.forward(p, sp, ricker(15.0, NT, 1e-3)[:, None], rp, N=N, ND=2, dh=dh,
+                             dt=1e-3, NT=NT, FDORDER=8, abs_type=0, seisout=1)
+    vx, vz = out
+    d = xs - x0
+    for k in range(1, 16):
+        i, j = np.where(d == k)[0][0], np.where(d == -k)[0][0]
+        assert rel_l2(vz[:, i], vz[:, j]) < 1e-5
+        # vx at x+1/2: the image of cell x0+k is x0-k-1
+        jj = np.where(d == -k - 1)[0][0]
+        assert rel_l2(vx[:, i], -vx[:, jj]) < 1e-5
+
+
+def test_cpml_absorbs(oracle_lib):
+    N = (80, 80)
+    p = dict(vp=np.full(N, 3000, np.float32), vs=np.full(N, 1700, np.float32),
+             rho=np.full(N, 2000, np.float32))
+    sp = np.array([[400.0], [0], [400.0], [0], [100]])
+    rp = np.array([[400.0], [0], [400.0], [0], [1], [0], [0], [0]])
+    NT = 900
+    kw = dict(N=N, ND=2, dh=10.0, dt=1e-3, NT=NT, FDORDER=4, nab=20, seisout=2, f0=15.0)
+    _, f = oracle_lib.forward(p, sp, ricker(15.0, NT, 1e-3)[:, None], rp, fields=True, **kw)
+    _, f0 = oracle_lib.forward(p, sp, ricker(15.0, NT, 1e-3)[:, None], rp, fields=True,
+                               **dict(kw, NT=120))
+    e_late = sum(float(np.sum(f[k].astype(np.float64) ** 2)) for k in ("vx", "vz"))
+    e_peak = sum(float(np.sum(f0[k].astype(np.float64) ** 2)) for k in ("vx", "vz"))
+    assert e_late < 1e-3 * e_peak
+
+
+def test_free_surface_traction(oracle_lib):
+    csts, p, (sp, s, rp) = small_case(ND=3, FDORDER=6, freesurf=1, L=1, seed=7, NT=60)
+    _, f = oracle_lib.forward(p, sp, s, rp, fields=True, **csts)
+    H = 3
+    assert np.all(f["szz"][H] == 0)
+    np.testing.assert_array_equal(f["szz"][H - 1], -f["szz"][H + 1])
+    np.testing.assert_array_equal(f["sxz"][H - 1], -f["sxz"][H])
+
+
+def test_receiver_outside_x_records_zero(oracle_lib):
+    csts, p, (sp, s, rp) = small_case(ND=2, FDORDER=4, seed=2, NT=60)
+    out = oracle_lib.forward(p, sp, s, rp, **csts)
+    assert np.all(out[0][:, -1] == 0) and np.abs(out[0][:, 0]).max() > 0
+
+
+def test_errors(oracle_lib):
+    csts, p, (sp, s, rp) = small_case(ND=2, FDORDER=4, seed=2, NT=20)
+    with pytest.raises(RuntimeError, match="Time step"):
+        oracle_lib.forward(p, sp, s, rp, **dict(csts, dt=0.1))
+    with pytest.raises(RuntimeError, match="FDORDER"):
+        oracle_lib.forward(p, sp, s, rp, **dict(csts, FDORDER=5))
+    csts, p, (sp, s, rp) = small_case(ND=22, FDORDER=4, seed=2, NT=20)
+    with pytest.raises(RuntimeError, match="free surface"):
+        oracle_lib.forward(p, sp, s, rp, **dict(csts, freesurf=1))
