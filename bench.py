@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid size per axis")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 512 MiB device-to-device copy first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the whole grid of this workload lives on "
@@ -72,6 +74,15 @@ def main():
 
     from seiscl_amd import Context, Model
     from seiscl_amd.workloads import config2
+
+    if args.calib_copy:
+        import torch
+        x = torch.ones(128 * 2**20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 512 MiB read + 512 MiB write")
 
     W, K = args.warmup, args.steps
     log(f"building workload n={args.n} NT={W + K}")

@@ -400,6 +400,9 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     }
     const int NZ = c.N[0], NY = c.ND == 3 ? c.N[1] : 1, NX = c.ND == 3 ? c.N[2] : c.N[1];
     m->g = make_layout(nd, NZ, NY, NX, c.FDORDER / 2, 4);
+    // kernels address every array with 32-bit byte offsets
+    if ((unsigned long long)m->g.nfield * sizeof(float) >= (1ull << 32))
+        return fail("grid too large: one padded field must stay below 4 GiB");
     FDArgs<float> &a = m->a;
     std::memset(&a, 0, sizeof(a));
     a.g = m->g;
@@ -411,10 +414,13 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
         a.eta[l] = m->s.eta[l];
         a.bl[l] = m->s.bl[l];
         a.cl[l] = m->s.cl[l];
+        a.bs[l] = m->s.bs[l];
     }
     a.abs_type = c.abs_type;
     a.freesurf = c.freesurf;
     a.nab = c.nab;
+    // forward modeling applies the 3D free surface inside the stress kernel
+    a.fuse_surface = c.ND == 3 ? 1 : 0;
     HIP_TRY(hipSetDevice(ctx->device));
     if (upload_params(m.get(), rp)) return 1;
     if (c.abs_type == 1) {
@@ -577,7 +583,8 @@ int seis_launch_update_s(seis_model *m) {
     return 0;
 }
 int seis_launch_surface(seis_model *m) {
-    if (!m->c.freesurf) return 0;
+    // with the surface fused into update_s there is nothing left to launch
+    if (!m->c.freesurf || m->a.fuse_surface) return 0;
     launch_surface(m->a, m->nd_case, 1, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -624,7 +631,7 @@ int seis_run_steps(seis_model *m, int t0, int nsteps) {
         if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
         launch_update_s(m->a, m->nd_case, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
-        if (m->c.freesurf) launch_surface(m->a, m->nd_case, 1, st);
+        if (m->c.freesurf && !m->a.fuse_surface) launch_surface(m->a, m->nd_case, 1, st);
         launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(),
                        t, st);
     }

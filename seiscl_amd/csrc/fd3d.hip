@@ -1,0 +1,399 @@
+// 3D velocity and stress updates.
+//
+// One lane owns one interior cell; a block covers 64 cells along z (one
+// wave reads two whole 128-byte lines of every field) times BY rows in y at a
+// single x-plane.  Tiles are numbered z fastest, then x, then the y slab,
+// and each XCD receives a contiguous range of that numbering.  The blocks an
+// XCD holds at any moment therefore cover consecutive x-planes of one y slab:
+// the x-neighbour planes a stencil reads are being read by co-resident
+// blocks and are served by that XCD's L2, and a y slab's halo rows are
+// re-read only once per slab.
+//
+// The free surface (image method, src/surface3D.cl:39-271) is applied in the
+// stress kernel's epilogue by the lanes that own z = 0..H, on the values they
+// just computed; it is skipped when the caller needs the stresses between
+// the stress update and the surface (boundary saving for gradients).
+//
+// Arithmetic: src/update_v3D.cl:34-369 and src/update_s3D.cl:35-489, term by
+// term and in the same order (FMA contraction disabled in the build).
+#include <hip/hip_runtime.h>
+
+#include "fd_common.hpp"
+#include "seis_internal.hpp"
+
+namespace seis {
+namespace {
+
+#ifndef SEIS_BY
+#define SEIS_BY 4           // rows of y per block
+#endif
+constexpr int BY3 = SEIS_BY;
+
+// tile -> (bz, y slab, x) with z fastest, then x, then the y slab
+__device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int &y, int &x) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return false;
+    const int bz = tile % nbz;
+    const unsigned r = tile / nbz;
+    x = r % NX;
+    const int by = r / NX;
+    z = bz * BZ + threadIdx.x;
+    y = by * BY3 + threadIdx.y;
+    return true;
+}
+
+template <typename R, int H>
+__global__ __launch_bounds__(64 * BY3) void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
+    const Layout &g = a.g;
+    int z, y, x;
+    if (!tile3(nbz, g.NX, nb, z, y, x)) return;
+    if (z >= g.NZ || y >= g.NY) return;
+    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const long sx = g.sx, sy = g.sy;
+    const R *hc = a.hc;
+    const unsigned o = (unsigned)(g.idx(z, y, x) * sizeof(R));   // lane byte offset
+    R szz_z = DpO<H>(a.szz, o, 1, hc);
+    R sxx_x = DpO<H>(a.sxx, o, sx, hc);
+    R sxz_x = DmO<H>(a.sxz, o, sx, hc);
+    R sxz_z = DmO<H>(a.sxz, o, 1, hc);
+    R syz_y = DmO<H>(a.syz, o, sy, hc);
+    R syz_z = DmO<H>(a.syz, o, 1, hc);
+    R syy_y = DpO<H>(a.syy, o, sy, hc);
+    R sxy_x = DmO<H>(a.sxy, o, sx, hc);
+    R sxy_y = DmO<H>(a.sxy, o, sy, hc);
+    // CPML strips: every load first (psi and profiles, then the cell's own
+    // values), the psi stores after
+    const bool cpml = a.abs_type == 1;
+    const bool pz = cpml && (z > NZ - NAB - 1 || (!a.freesurf && z < NAB));
+    const bool py = cpml && (y < NAB || y > NY - NAB - 1);
+    const bool px = cpml && (x < NAB || x > NX - NAB - 1);
+    const int k = z < NAB ? z : z - NZ + 2 * NAB;
+    const int ki = z < NAB ? k : 2 * NAB - k, kh = z < NAB ? k : 2 * NAB - 1 - k;
+    const int j = y < NAB ? y : y - NY + 2 * NAB;
+    const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
+    const int ii = x < NAB ? x : x - NX + 2 * NAB;
+    const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+    const long ipz = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+    const long ipy = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+    const long ipx = (long)ii * NY * NZ + (long)y * NZ + z;
+    Pml<R> z0, z1, z2, y0, y1, y2, x0, x1, x2;
+    if (pz) {
+        z0 = pml_load(a, a.psi_sxz_z + ipz, 2, false, ki);
+        z1 = pml_load(a, a.psi_syz_z + ipz, 2, false, ki);
+        z2 = pml_load(a, a.psi_szz_z + ipz, 2, true, kh);
+    }
+    if (py) {
+        y0 = pml_load(a, a.psi_sxy_y + ipy, 1, false, ji);
+        y1 = pml_load(a, a.psi_syy_y + ipy, 1, true, jh);
+        y2 = pml_load(a, a.psi_syz_y + ipy, 1, false, ji);
+    }
+    if (px) {
+        x0 = pml_load(a, a.psi_sxx_x + ipx, 0, true, iH);
+        x1 = pml_load(a, a.psi_sxy_x + ipx, 0, false, iI);
+        x2 = pml_load(a, a.psi_sxz_x + ipx, 0, false, iI);
+    }
+    const R vx0 = ldo(a.vx, o), vy0 = ldo(a.vy, o), vz0 = ldo(a.vz, o);
+    const R rip = ldo(a.rip, o), rjp = ldo(a.rjp, o), rkp = ldo(a.rkp, o);
+    if (pz) {
+        pml_apply(sxz_z, a.psi_sxz_z + ipz, z0);
+        pml_apply(syz_z, a.psi_syz_z + ipz, z1);
+        pml_apply(szz_z, a.psi_szz_z + ipz, z2);
+    }
+    if (py) {
+        pml_apply(sxy_y, a.psi_sxy_y + ipy, y0);
+        pml_apply(syy_y, a.psi_syy_y + ipy, y1);
+        pml_apply(syz_y, a.psi_syz_y + ipy, y2);
+    }
+    if (px) {
+        pml_apply(sxx_x, a.psi_sxx_x + ipx, x0);
+        pml_apply(sxy_x, a.psi_sxy_x + ipx, x1);
+        pml_apply(sxz_x, a.psi_sxz_x + ipx, x2);
+    }
+    R vx = vx0 + ((sxx_x + sxy_y + sxz_z) * rip);
+    R vy = vy0 + ((syy_y + sxy_x + syz_z) * rjp);
+    R vz = vz0 + ((szz_z + sxz_x + syz_y) * rkp);
+    if (a.abs_type == 2) {
+        const R *t = a.taper;
+        if (!a.freesurf && z < NAB) { vx *= t[z]; vy *= t[z]; vz *= t[z]; }
+        if (z > NZ - NAB - 1) { vx *= t[NZ - z - 1]; vy *= t[NZ - z - 1]; vz *= t[NZ - z - 1]; }
+        if (y < NAB) { vx *= t[y]; vy *= t[y]; vz *= t[y]; }
+        if (y > NY - NAB - 1) { vx *= t[NY - y - 1]; vy *= t[NY - y - 1]; vz *= t[NY - y - 1]; }
+        if (x < NAB) { vx *= t[x]; vy *= t[x]; vz *= t[x]; }
+        if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vy *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
+    }
+    sto(a.vx, o, vx);
+    sto(a.vy, o, vy);
+    sto(a.vz, o, vz);
+}
+
+template <typename R, int H, int L, bool FS>
+__global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
+    const Layout &g = a.g;
+    int z, y, x;
+    if (!tile3(nbz, g.NX, nb, z, y, x)) return;
+    if (z >= g.NZ || y >= g.NY) return;
+    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const long sx = g.sx, sy = g.sy, nf = g.nfield;
+    const R *hc = a.hc;
+    const unsigned o = (unsigned)(g.idx(z, y, x) * sizeof(R));   // lane byte offset
+    R vzx = DpO<H>(a.vz, o, sx, hc);
+    R vzy = DpO<H>(a.vz, o, sy, hc);
+    R vzz = DmO<H>(a.vz, o, 1, hc);
+    R vyx = DpO<H>(a.vy, o, sx, hc);
+    R vyy = DmO<H>(a.vy, o, sy, hc);
+    R vyz = DpO<H>(a.vy, o, 1, hc);
+    R vxx = DmO<H>(a.vx, o, sx, hc);
+    R vxy = DpO<H>(a.vx, o, sy, hc);
+    R vxz = DpO<H>(a.vx, o, 1, hc);
+    const bool cpml = a.abs_type == 1;
+    const bool pz = cpml && (z > NZ - NAB - 1 || (!a.freesurf && z < NAB));
+    const bool py = cpml && (y < NAB || y > NY - NAB - 1);
+    const bool px = cpml && (x < NAB || x > NX - NAB - 1);
+    const int k = z < NAB ? z : z - NZ + 2 * NAB;
+    const int ki = z < NAB ? k : 2 * NAB - k, kh = z < NAB ? k : 2 * NAB - 1 - k;
+    const int j = y < NAB ? y : y - NY + 2 * NAB;
+    const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
+    const int ii = x < NAB ? x : x - NX + 2 * NAB;
+    const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+    const long ipz = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+    const long ipy = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+    const long ipx = (long)ii * NY * NZ + (long)y * NZ + z;
+    Pml<R> z0, z1, z2, y0, y1, y2, x0, x1, x2;
+    if (pz) {
+        z0 = pml_load(a, a.psi_vx_z + ipz, 2, true, kh);
+        z1 = pml_load(a, a.psi_vy_z + ipz, 2, true, kh);
+        z2 = pml_load(a, a.psi_vz_z + ipz, 2, false, ki);
+    }
+    if (py) {
+        y0 = pml_load(a, a.psi_vx_y + ipy, 1, true, jh);
+        y1 = pml_load(a, a.psi_vy_y + ipy, 1, false, ji);
+        y2 = pml_load(a, a.psi_vz_y + ipy, 1, true, jh);
+    }
+    if (px) {
+        x0 = pml_load(a, a.psi_vx_x + ipx, 0, false, iI);
+        x1 = pml_load(a, a.psi_vy_x + ipx, 0, true, iH);
+        x2 = pml_load(a, a.psi_vz_x + ipx, 0, true, iH);
+    }
+    // the cell's own values and coefficients, before any store
+    R sxy = ldo(a.sxy, o), syz = ldo(a.syz, o), sxz = ldo(a.sxz, o);
+    R sxx = ldo(a.sxx, o), syy = ldo(a.syy, o), szz = ldo(a.szz, o);
+    R rxy[L > 0 ? L : 1], ryz[L > 0 ? L : 1], rxz[L > 0 ? L : 1];
+    R rxx[L > 0 ? L : 1], ryy[L > 0 ? L : 1], rzz[L > 0 ? L : 1];
+    R gg, f, d = 0, e = 0;
+    R fipjp, fjpkp, fipkp, dipjp = 0, djpkp = 0, dipkp = 0;
+    if constexpr (L == 0) {
+        fipjp = ldo(a.muipjp, o);
+        fjpkp = ldo(a.mujpkp, o);
+        fipkp = ldo(a.muipkp, o);
+        gg = ldo(a.M, o);
+        f = (R)2 * ldo(a.mu, o);
+    } else {
+        gg = ldo(a.cg, o);
+        f = ldo(a.cf, o);
+        d = ldo(a.cd, o);
+        e = ldo(a.ce, o);
+        fipjp = ldo(a.fipjp, o);
+        fjpkp = ldo(a.fjpkp, o);
+        fipkp = ldo(a.fipkp, o);
+        dipjp = ldo(a.dipjp, o);
+        djpkp = ldo(a.djpkp, o);
+        dipkp = ldo(a.dipkp, o);
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const long ln = l * nf;
+            rxy[l] = ldo(a.rxy + ln, o); ryz[l] = ldo(a.ryz + ln, o); rxz[l] = ldo(a.rxz + ln, o);
+            rxx[l] = ldo(a.rxx + ln, o); ryy[l] = ldo(a.ryy + ln, o); rzz[l] = ldo(a.rzz + ln, o);
+        }
+    }
+    if (pz) {
+        pml_apply(vxz, a.psi_vx_z + ipz, z0);
+        pml_apply(vyz, a.psi_vy_z + ipz, z1);
+        pml_apply(vzz, a.psi_vz_z + ipz, z2);
+    }
+    if (py) {
+        pml_apply(vxy, a.psi_vx_y + ipy, y0);
+        pml_apply(vyy, a.psi_vy_y + ipy, y1);
+        pml_apply(vzy, a.psi_vz_y + ipy, y2);
+    }
+    if (px) {
+        pml_apply(vxx, a.psi_vx_x + ipx, x0);
+        pml_apply(vyx, a.psi_vy_x + ipx, x1);
+        pml_apply(vzx, a.psi_vz_x + ipx, x2);
+    }
+    const R vxyyx = vxy + vyx;
+    const R vyzzy = vyz + vzy;
+    const R vxzzx = vxz + vzx;
+    const R vxxyyzz = vxx + vyy + vzz;
+    const R vyyzz = vyy + vzz;
+    const R vxxzz = vxx + vzz;
+    const R vxxyy = vxx + vyy;
+    if constexpr (L == 0) {
+        sxy += (fipjp * vxyyx);
+        syz += (fjpkp * vyzzy);
+        sxz += (fipkp * vxzzx);
+        sxx += ((gg * vxxyyzz) - (f * vyyzz));
+        syy += ((gg * vxxyyzz) - (f * vxxzz));
+        szz += ((gg * vxxyyzz) - (f * vxxyy));
+    } else {
+        const R DT2 = a.DT2;
+        R sumrxy = 0, sumryz = 0, sumrxz = 0, sumrxx = 0, sumryy = 0, sumrzz = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
+            sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
+        }
+        const R lsxy = (fipjp * vxyyx) + (DT2 * sumrxy);
+        const R lsyz = (fjpkp * vyzzy) + (DT2 * sumryz);
+        const R lsxz = (fipkp * vxzzx) + (DT2 * sumrxz);
+        const R lsxx = ((gg * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
+        const R lsyy = ((gg * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
+        const R lszz = ((gg * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
+        sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
+            rxy[l] = b * (rxy[l] * c - et * (dipjp * vxyyx));
+            ryz[l] = b * (ryz[l] * c - et * (djpkp * vyzzy));
+            rxz[l] = b * (rxz[l] * c - et * (dipkp * vxzzx));
+            rxx[l] = b * (rxx[l] * c - et * ((e * vxxyyzz) - (d * vyyzz)));
+            ryy[l] = b * (ryy[l] * c - et * ((e * vxxyyzz) - (d * vxxzz)));
+            rzz[l] = b * (rzz[l] * c - et * ((e * vxxyyzz) - (d * vxxyy)));
+            sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
+            sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
+        }
+        sxy += lsxy + (DT2 * sumrxy);
+        syz += lsyz + (DT2 * sumryz);
+        sxz += lsxz + (DT2 * sumrxz);
+        sxx += lsxx + (DT2 * sumrxx);
+        syy += lsyy + (DT2 * sumryy);
+        szz += lszz + (DT2 * sumrzz);
+    }
+    if (a.abs_type == 2) {
+        const R *t = a.taper;
+        R *s6[6] = {&sxy, &syz, &sxz, &sxx, &syy, &szz};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            R &s = *s6[q];
+            if (!a.freesurf && z < NAB) s *= t[z];
+            if (z > NZ - NAB - 1) s *= t[NZ - z - 1];
+            if (y < NAB) s *= t[y];
+            if (y > NY - NAB - 1) s *= t[NY - y - 1];
+            if (x < NAB) s *= t[x];
+            if (x > NX - NAB - 1) s *= t[NX - x - 1];
+        }
+    }
+    if constexpr (FS) {
+        if (z == 0) {
+            // image method at the surface row (src/surface3D.cl:84-252);
+            // vxx, vyy (CPML-corrected) and vzz are those of this update
+            szz = 0;
+            if constexpr (L == 0) {
+                const R h = -((gg - f) * (gg - f) * (vxx + vyy) / gg) - ((gg - f) * vzz);
+                sxx += h;
+                syy += h;
+            } else {
+                const R DT2 = a.DT2;
+                const R h = -((gg - f) * (gg - f) * (vxx + vyy) / gg) - ((gg - f) * vzz);
+                R sumx = 0, sumy = 0;
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    rzz[l] = 0;
+                    sumx += rxx[l];
+                    sumy += ryy[l];
+                }
+                sxx += (R)1 * (h - DT2 * sumx);
+                syy += (R)1 * (h - DT2 * sumy);
+                sumx = 0;
+                sumy = 0;
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    const R hh = (R)((double)a.bs[l] *
+                                     ((double)(d - e) * ((double)(f / gg) - 1.0) *
+                                          (double)(vxx + vyy) -
+                                      (double)((d - e) * vzz)));
+                    rxx[l] += hh;
+                    ryy[l] += hh;
+                    sumx += rxx[l];
+                    sumy += ryy[l];
+                }
+                sxx += (DT2 * sumx);
+                syy += (DT2 * sumy);
+            }
+        }
+    }
+    sto(a.sxy, o, sxy); sto(a.syz, o, syz); sto(a.sxz, o, sxz);
+    sto(a.sxx, o, sxx); sto(a.syy, o, syy); sto(a.szz, o, szz);
+    if constexpr (L > 0) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const long ln = l * nf;
+            sto(a.rxy + ln, o, rxy[l]); sto(a.ryz + ln, o, ryz[l]); sto(a.rxz + ln, o, rxz[l]);
+            sto(a.rxx + ln, o, rxx[l]); sto(a.ryy + ln, o, ryy[l]); sto(a.rzz + ln, o, rzz[l]);
+        }
+    }
+    if constexpr (FS) {
+        // mirror images above the surface: szz(-m) = -szz(m),
+        // sxz(-m) = -sxz(m-1), syz(-m) = -syz(m-1), m = 1..H
+        if (z >= 1 && z <= H) sto(a.szz, o - 2 * z * (unsigned)sizeof(R), -szz);
+        if (z <= H - 1) {
+            sto(a.sxz, o - (2 * z + 1) * (unsigned)sizeof(R), -sxz);
+            sto(a.syz, o - (2 * z + 1) * (unsigned)sizeof(R), -syz);
+        }
+    }
+}
+
+template <typename R>
+void grid3(const Layout &g, int &nbz, unsigned &nb) {
+    nbz = (g.NZ + BZ - 1) / BZ;
+    const int nby = (g.NY + BY3 - 1) / BY3;
+    nb = (unsigned)nbz * g.NX * nby;
+}
+
+template <typename R, int H>
+void run_v3d(const FDArgs<R> &a, hipStream_t st) {
+    int nbz;
+    unsigned nb;
+    grid3<R>(a.g, nbz, nb);
+    hipLaunchKernelGGL((k_v3d<R, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BY3), 0, st, a, nbz, nb);
+}
+
+template <typename R, int H, int L>
+void run_s3d(const FDArgs<R> &a, hipStream_t st) {
+    int nbz;
+    unsigned nb;
+    grid3<R>(a.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, BY3);
+    if (a.freesurf && a.fuse_surface)
+        hipLaunchKernelGGL((k_s3d<R, H, L, true>), grid, blk, 0, st, a, nbz, nb);
+    else
+        hipLaunchKernelGGL((k_s3d<R, H, L, false>), grid, blk, 0, st, a, nbz, nb);
+}
+
+template <typename R, int H>
+void run_s3d_L(const FDArgs<R> &a, hipStream_t st) {
+    switch (a.L) {
+        case 0: run_s3d<R, H, 0>(a, st); break;
+        case 1: run_s3d<R, H, 1>(a, st); break;
+        case 2: run_s3d<R, H, 2>(a, st); break;
+        case 3: run_s3d<R, H, 3>(a, st); break;
+        case 4: run_s3d<R, H, 4>(a, st); break;
+        default: break;
+    }
+}
+
+}  // namespace
+
+template <typename R>
+void launch_v3d(const FDArgs<R> &a, void *stream) {
+    SEIS_SWITCH_H(a.g.H, (run_v3d<R, HH>(a, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_s3d(const FDArgs<R> &a, void *stream) {
+    SEIS_SWITCH_H(a.g.H, (run_s3d_L<R, HH>(a, (hipStream_t)stream)));
+}
+
+template void launch_v3d<float>(const FDArgs<float> &, void *);
+template void launch_s3d<float>(const FDArgs<float> &, void *);
+
+}  // namespace seis

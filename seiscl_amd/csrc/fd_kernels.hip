@@ -16,264 +16,12 @@
 // C restatement compiled the same way.
 #include <hip/hip_runtime.h>
 
+#include "fd_common.hpp"
 #include "seis_internal.hpp"
 
 namespace seis {
 
 namespace {
-
-constexpr int BZ = 64;   // lanes along z
-constexpr int BT = 4;    // rows along y (3D) or x (2D)
-
-// Forward / backward staggered derivatives, src/header_FD.cl:87-265.
-template <int H, typename R>
-__device__ __forceinline__ R Dp(const R *__restrict__ v, long i, long s, const R *hc) {
-    R acc = hc[1] * (v[i + s] - v[i]);
-#pragma unroll
-    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (v[i + k * s] - v[i - (k - 1) * s]);
-    return acc;
-}
-template <int H, typename R>
-__device__ __forceinline__ R Dm(const R *__restrict__ v, long i, long s, const R *hc) {
-    R acc = hc[1] * (v[i] - v[i - s]);
-#pragma unroll
-    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (v[i + (k - 1) * s] - v[i - k * s]);
-    return acc;
-}
-
-// CPML coefficient access: cpml[axis][kind][2*nab]
-enum { CK = 0, CA = 1, CB = 2, CKH = 3, CAH = 4, CBH = 5 };
-template <typename R>
-__device__ __forceinline__ R cp(const FDArgs<R> &a, int axis, int kind, int i) {
-    return a.cpml[(axis * 6 + kind) * 2 * a.nab + i];
-}
-
-// psi <- b psi + a d ;  d <- d / K + psi   (src/update_v3D.cl:210-314)
-template <typename R>
-__device__ __forceinline__ void pml(R &d, R *psi, R b, R aa, R K) {
-    R ps = b * *psi + aa * d;
-    *psi = ps;
-    d = d / K + ps;
-}
-
-// Renumber blocks: hardware block b runs on XCD b % 8; give each XCD a
-// contiguous slab of tiles.
-__device__ __forceinline__ void xcd_tile(unsigned nb, unsigned &tile) {
-    const unsigned b = blockIdx.x;
-    const unsigned per = (nb + 7) / 8;
-    const unsigned xcd = b % 8, slot = b / 8;
-    tile = xcd * per + slot;
-}
-
-// ---------------------------------------------------------------------------
-// 3D velocity: src/update_v3D.cl:34-369
-// ---------------------------------------------------------------------------
-template <typename R, int H>
-__global__ __launch_bounds__(256) void k_update_v3d(FDArgs<R> a, int nbz, int nby, unsigned nb) {
-    unsigned tile;
-    xcd_tile(nb, tile);
-    if (tile >= nb) return;
-    const int bz = tile % nbz, by = (tile / nbz) % nby, x = tile / (nbz * nby);
-    const int z = bz * BZ + threadIdx.x, y = by * BT + threadIdx.y;
-    const Layout &g = a.g;
-    if (z >= g.NZ || y >= g.NY) return;
-    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
-    const R *hc = a.hc;
-    R szz_z = Dp<H>(a.szz, i, 1, hc);
-    R sxx_x = Dp<H>(a.sxx, i, sx, hc);
-    R sxz_x = Dm<H>(a.sxz, i, sx, hc);
-    R sxz_z = Dm<H>(a.sxz, i, 1, hc);
-    R syz_y = Dm<H>(a.syz, i, sy, hc);
-    R syz_z = Dm<H>(a.syz, i, 1, hc);
-    R syy_y = Dp<H>(a.syy, i, sy, hc);
-    R sxy_x = Dm<H>(a.sxy, i, sx, hc);
-    R sxy_y = Dm<H>(a.sxy, i, sy, hc);
-    if (a.abs_type == 1) {
-        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
-        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
-            const int k = z < NAB ? z : z - NZ + 2 * NAB;
-            const int ki = z < NAB ? k : 2 * NAB - k;      // integer grid
-            const int kh = z < NAB ? k : 2 * NAB - 1 - k;  // half grid
-            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
-            pml(sxz_z, a.psi_sxz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
-            pml(syz_z, a.psi_syz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
-            pml(szz_z, a.psi_szz_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
-        }
-        if (y < NAB || y > NY - NAB - 1) {
-            const int j = y < NAB ? y : y - NY + 2 * NAB;
-            const int ji = y < NAB ? j : 2 * NAB - j;
-            const int jh = y < NAB ? j : 2 * NAB - 1 - j;
-            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
-            pml(sxy_y, a.psi_sxy_y + ip, cp(a, 1, CB, ji), cp(a, 1, CA, ji), cp(a, 1, CK, ji));
-            pml(syy_y, a.psi_syy_y + ip, cp(a, 1, CBH, jh), cp(a, 1, CAH, jh), cp(a, 1, CKH, jh));
-            pml(syz_y, a.psi_syz_y + ip, cp(a, 1, CB, ji), cp(a, 1, CA, ji), cp(a, 1, CK, ji));
-        }
-        if (x < NAB || x > NX - NAB - 1) {
-            const int ii = x < NAB ? x : x - NX + 2 * NAB;
-            const int iI = x < NAB ? ii : 2 * NAB - ii;
-            const int iH = x < NAB ? ii : 2 * NAB - 1 - ii;
-            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
-            pml(sxx_x, a.psi_sxx_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
-            pml(sxy_x, a.psi_sxy_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
-            pml(sxz_x, a.psi_sxz_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
-        }
-    }
-    R vx = a.vx[i] + ((sxx_x + sxy_y + sxz_z) * a.rip[i]);
-    R vy = a.vy[i] + ((syy_y + sxy_x + syz_z) * a.rjp[i]);
-    R vz = a.vz[i] + ((szz_z + sxz_x + syz_y) * a.rkp[i]);
-    if (a.abs_type == 2) {
-        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
-        const R *t = a.taper;
-        // applied one edge after the other, as the reference does
-        if (!a.freesurf && z < NAB) { vx *= t[z]; vy *= t[z]; vz *= t[z]; }
-        if (z > NZ - NAB - 1) { vx *= t[NZ - z - 1]; vy *= t[NZ - z - 1]; vz *= t[NZ - z - 1]; }
-        if (y < NAB) { vx *= t[y]; vy *= t[y]; vz *= t[y]; }
-        if (y > NY - NAB - 1) { vx *= t[NY - y - 1]; vy *= t[NY - y - 1]; vz *= t[NY - y - 1]; }
-        if (x < NAB) { vx *= t[x]; vy *= t[x]; vz *= t[x]; }
-        if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vy *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
-    }
-    a.vx[i] = vx;
-    a.vy[i] = vy;
-    a.vz[i] = vz;
-}
-
-// ---------------------------------------------------------------------------
-// 3D stress: src/update_s3D.cl:35-489.  For L>0 the per-cell coefficients
-// g, f, d, e, f_ij, d_ij (:301-340) are precomputed on the host with the
-// reference's expressions; they are pure functions of the parameters.
-// ---------------------------------------------------------------------------
-template <typename R, int H, int L>
-__global__ __launch_bounds__(256) void k_update_s3d(FDArgs<R> a, int nbz, int nby, unsigned nb) {
-    unsigned tile;
-    xcd_tile(nb, tile);
-    if (tile >= nb) return;
-    const int bz = tile % nbz, by = (tile / nbz) % nby, x = tile / (nbz * nby);
-    const int z = bz * BZ + threadIdx.x, y = by * BT + threadIdx.y;
-    const Layout &g = a.g;
-    if (z >= g.NZ || y >= g.NY) return;
-    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
-    const R *hc = a.hc;
-    R vzx = Dp<H>(a.vz, i, sx, hc);
-    R vzy = Dp<H>(a.vz, i, sy, hc);
-    R vzz = Dm<H>(a.vz, i, 1, hc);
-    R vyx = Dp<H>(a.vy, i, sx, hc);
-    R vyy = Dm<H>(a.vy, i, sy, hc);
-    R vyz = Dp<H>(a.vy, i, 1, hc);
-    R vxx = Dm<H>(a.vx, i, sx, hc);
-    R vxy = Dp<H>(a.vx, i, sy, hc);
-    R vxz = Dp<H>(a.vx, i, 1, hc);
-    if (a.abs_type == 1) {
-        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
-        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
-            const int k = z < NAB ? z : z - NZ + 2 * NAB;
-            const int ki = z < NAB ? k : 2 * NAB - k;
-            const int kh = z < NAB ? k : 2 * NAB - 1 - k;
-            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
-            pml(vxz, a.psi_vx_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
-            pml(vyz, a.psi_vy_z + ip, cp(a, 2, CBH, kh), cp(a, 2, CAH, kh), cp(a, 2, CKH, kh));
-            pml(vzz, a.psi_vz_z + ip, cp(a, 2, CB, ki), cp(a, 2, CA, ki), cp(a, 2, CK, ki));
-        }
-        if (y < NAB || y > NY - NAB - 1) {
-            const int j = y < NAB ? y : y - NY + 2 * NAB;
-            const int ji = y < NAB ? j : 2 * NAB - j;
-            const int jh = y < NAB ? j : 2 * NAB - 1 - j;
-            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
-            pml(vxy, a.psi_vx_y + ip, cp(a, 1, CBH, jh), cp(a, 1, CAH, jh), cp(a, 1, CKH, jh));
-            pml(vyy, a.psi_vy_y + ip, cp(a, 1, CB, ji), cp(a, 1, CA, ji), cp(a, 1, CK, ji));
-            pml(vzy, a.psi_vz_y + ip, cp(a, 1, CBH, jh), cp(a, 1, CAH, jh), cp(a, 1, CKH, jh));
-        }
-        if (x < NAB || x > NX - NAB - 1) {
-            const int ii = x < NAB ? x : x - NX + 2 * NAB;
-            const int iI = x < NAB ? ii : 2 * NAB - ii;
-            const int iH = x < NAB ? ii : 2 * NAB - 1 - ii;
-            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
-            pml(vxx, a.psi_vx_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
-            pml(vyx, a.psi_vy_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
-            pml(vzx, a.psi_vz_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
-        }
-    }
-    const R vxyyx = vxy + vyx;
-    const R vyzzy = vyz + vzy;
-    const R vxzzx = vxz + vzx;
-    const R vxxyyzz = vxx + vyy + vzz;
-    const R vyyzz = vyy + vzz;
-    const R vxxzz = vxx + vzz;
-    const R vxxyy = vxx + vyy;
-    R sxy = a.sxy[i], syz = a.syz[i], sxz = a.sxz[i];
-    R sxx = a.sxx[i], syy = a.syy[i], szz = a.szz[i];
-    if constexpr (L == 0) {
-        const R fipjp = a.muipjp[i], fjpkp = a.mujpkp[i], fipkp = a.muipkp[i];
-        const R gg = a.M[i];
-        const R f = (R)2 * a.mu[i];   // exact in either precision
-        sxy += (fipjp * vxyyx);
-        syz += (fjpkp * vyzzy);
-        sxz += (fipkp * vxzzx);
-        sxx += ((gg * vxxyyzz) - (f * vyyzz));
-        syy += ((gg * vxxyyzz) - (f * vxxzz));
-        szz += ((gg * vxxyyzz) - (f * vxxyy));
-    } else {
-        const R DT2 = a.DT2;
-        const R gg = a.cg[i], f = a.cf[i], d = a.cd[i], e = a.ce[i];
-        const R fipjp = a.fipjp[i], fjpkp = a.fjpkp[i], fipkp = a.fipkp[i];
-        const R dipjp = a.dipjp[i], djpkp = a.djpkp[i], dipkp = a.dipkp[i];
-        const long nf = g.nfield;
-        R rxy[L], ryz[L], rxz[L], rxx[L], ryy[L], rzz[L];
-        R sumrxy = 0, sumryz = 0, sumrxz = 0, sumrxx = 0, sumryy = 0, sumrzz = 0;
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-            const long ir = l * nf + i;
-            rxy[l] = a.rxy[ir]; ryz[l] = a.ryz[ir]; rxz[l] = a.rxz[ir];
-            rxx[l] = a.rxx[ir]; ryy[l] = a.ryy[ir]; rzz[l] = a.rzz[ir];
-            sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
-            sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
-        }
-        const R lsxy = (fipjp * vxyyx) + (DT2 * sumrxy);
-        const R lsyz = (fjpkp * vyzzy) + (DT2 * sumryz);
-        const R lsxz = (fipkp * vxzzx) + (DT2 * sumrxz);
-        const R lsxx = ((gg * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
-        const R lsyy = ((gg * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
-        const R lszz = ((gg * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
-        sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-            const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
-            const long ir = l * nf + i;
-            rxy[l] = b * (rxy[l] * c - et * (dipjp * vxyyx));
-            ryz[l] = b * (ryz[l] * c - et * (djpkp * vyzzy));
-            rxz[l] = b * (rxz[l] * c - et * (dipkp * vxzzx));
-            rxx[l] = b * (rxx[l] * c - et * ((e * vxxyyzz) - (d * vyyzz)));
-            ryy[l] = b * (ryy[l] * c - et * ((e * vxxyyzz) - (d * vxxzz)));
-            rzz[l] = b * (rzz[l] * c - et * ((e * vxxyyzz) - (d * vxxyy)));
-            a.rxy[ir] = rxy[l]; a.ryz[ir] = ryz[l]; a.rxz[ir] = rxz[l];
-            a.rxx[ir] = rxx[l]; a.ryy[ir] = ryy[l]; a.rzz[ir] = rzz[l];
-            sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
-            sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
-        }
-        sxy += lsxy + (DT2 * sumrxy);
-        syz += lsyz + (DT2 * sumryz);
-        sxz += lsxz + (DT2 * sumrxz);
-        sxx += lsxx + (DT2 * sumrxx);
-        syy += lsyy + (DT2 * sumryy);
-        szz += lszz + (DT2 * sumrzz);
-    }
-    if (a.abs_type == 2) {
-        const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
-        const R *t = a.taper;
-        R *s6[6] = {&sxy, &syz, &sxz, &sxx, &syy, &szz};
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            R &s = *s6[q];
-            if (!a.freesurf && z < NAB) s *= t[z];
-            if (z > NZ - NAB - 1) s *= t[NZ - z - 1];
-            if (y < NAB) s *= t[y];
-            if (y > NY - NAB - 1) s *= t[NY - y - 1];
-            if (x < NAB) s *= t[x];
-            if (x > NX - NAB - 1) s *= t[NX - x - 1];
-        }
-    }
-    a.sxy[i] = sxy; a.syz[i] = syz; a.sxz[i] = sxz;
-    a.sxx[i] = sxx; a.syy[i] = syy; a.szz[i] = szz;
-}
 
 // ---------------------------------------------------------------------------
 // 2D P-SV: src/update_v2D.cl:32-246, src/update_s2D.cl:30-326
@@ -702,11 +450,7 @@ void dispatch_v(const FDArgs<R> &a, int nd_case, hipStream_t st) {
     const Layout &g = a.g;
     const int nbz = (g.NZ + BZ - 1) / BZ;
     dim3 blk(BZ, BT);
-    if (nd_case == 3) {
-        const int nby = (g.NY + BT - 1) / BT;
-        const unsigned nb = (unsigned)nbz * nby * g.NX;
-        hipLaunchKernelGGL((k_update_v3d<R, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nby, nb);
-    } else {
+    {
         const int nbx = (g.NX + BT - 1) / BT;
         const unsigned nb = (unsigned)nbz * nbx;
         if (nd_case == 22)
@@ -721,11 +465,7 @@ void dispatch_s_L(const FDArgs<R> &a, int nd_case, hipStream_t st) {
     const Layout &g = a.g;
     const int nbz = (g.NZ + BZ - 1) / BZ;
     dim3 blk(BZ, BT);
-    if (nd_case == 3) {
-        const int nby = (g.NY + BT - 1) / BT;
-        const unsigned nb = (unsigned)nbz * nby * g.NX;
-        hipLaunchKernelGGL((k_update_s3d<R, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nby, nb);
-    } else {
+    {
         const int nbx = (g.NX + BT - 1) / BT;
         const unsigned nb = (unsigned)nbz * nbx;
         if (nd_case == 22)
@@ -758,23 +498,14 @@ void dispatch_surface(const FDArgs<R> &a, int nd_case, int pdir, hipStream_t st)
 
 }  // namespace
 
-#define SEIS_SWITCH_H(H, CALL)                   \
-    switch (H) {                                 \
-        case 1: { constexpr int HH = 1; CALL; } break; \
-        case 2: { constexpr int HH = 2; CALL; } break; \
-        case 3: { constexpr int HH = 3; CALL; } break; \
-        case 4: { constexpr int HH = 4; CALL; } break; \
-        case 5: { constexpr int HH = 5; CALL; } break; \
-        case 6: { constexpr int HH = 6; CALL; } break; \
-        default: break;                          \
-    }
-
 template <typename R>
 void launch_update_v(const FDArgs<R> &a, int nd_case, void *stream) {
+    if (nd_case == 3) return launch_v3d<R>(a, stream);
     SEIS_SWITCH_H(a.g.H, (dispatch_v<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
 void launch_update_s(const FDArgs<R> &a, int nd_case, void *stream) {
+    if (nd_case == 3) return launch_s3d<R>(a, stream);
     SEIS_SWITCH_H(a.g.H, (dispatch_s<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>

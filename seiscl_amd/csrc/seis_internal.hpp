@@ -48,7 +48,7 @@ Layout make_layout(int nd, int NZ, int NY, int NX, int H, int elem);
 struct Setup {
     float hc[7] = {0};
     float DT = 0, DT2 = 0, DH = 0;
-    float eta[kMaxL] = {0}, bl[kMaxL] = {0}, cl[kMaxL] = {0};
+    float eta[kMaxL] = {0}, bl[kMaxL] = {0}, cl[kMaxL] = {0}, bs[kMaxL] = {0};
     // CPML: [axis x,y,z][K, a, b, K_half, a_half, b_half][2*nab]
     std::vector<float> cpml;
     std::vector<float> taper;
@@ -81,6 +81,8 @@ struct FDArgs {
     int L;
     R eta[kMaxL], bl[kMaxL], cl[kMaxL];
     int abs_type, freesurf, nab;
+    int fuse_surface;                    // apply the free surface inside update_s
+    R bs[kMaxL];                         // eta/(1+eta/2), surface memory update
     // wavefields
     R *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *syz, *sxz, *p;
     // memory variables: L consecutive fields each
@@ -126,6 +128,10 @@ template <typename R>
 void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s,
                     const int *out_codes, int nout, int nt, void *stream);
 void launch_fill(void *ptr, size_t bytes, void *stream);
+template <typename R>
+void launch_v3d(const FDArgs<R> &a, void *stream);
+template <typename R>
+void launch_s3d(const FDArgs<R> &a, void *stream);
 
 // Output codes used by the receiver kernel.
 enum OutCode { O_VX, O_VY, O_VZ, O_SXX, O_SYY, O_SZZ, O_SXZ, O_SXY, O_SYZ, O_P };

@@ -223,6 +223,7 @@ int build_setup(const seis_csts &c, Setup &s, std::string &err) {
         s.eta[l] = (2.0 * kPi * c.FL[l]) * c.dt;             // :536-548
         s.bl[l] = 1.0 / (1.0 + (s.eta[l] * 0.5));            // src/update_s3D.cl:406
         s.cl[l] = 1.0 - (s.eta[l] * 0.5);                    // src/update_s3D.cl:407
+        s.bs[l] = s.eta[l] / (1.0 + (s.eta[l] * 0.5));       // src/surface3D.cl:239
     }
     const int nab = c.nab;
     if (c.abs_type == 1) {
