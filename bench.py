@@ -14,6 +14,8 @@ roofline: the stress update (the dominant kernel), algorithmic bytes per
 launch = 196 B per cell (3 velocities read, 6 stresses and 12 memory
 variables read-modify-write, 10 coefficient arrays read) / its average
 duration from HIP events on the compute stream.
+roofline.traffic: HBM bytes per stress-update launch from rocprofv3 counters
+(FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 512 MiB copy and fill).
 cpu_baseline: the C oracle on the host cores, same grid, a bounded number of
 steps.
 """
@@ -34,6 +36,20 @@ METRIC = "Gcell-updates/s & % HBM roofline, 3D elastic; shots/hr + p99 latency b
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_CELL_S = 196       # stress update, 3D visco L=2 FP32
 BYTES_PER_CELL_V = 60        # velocity update, 3D FP32
+# HBM bytes per stress-update launch at 256^3, from the FETCH_SIZE / WRITE_SIZE
+# rocprofv3 passes summarised in this file (the command is written there)
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "update_s_v_visco3d_256_hbm_counters.md")
+
+
+def measured_traffic(n):
+    """HBM bytes per k_s3d launch from the committed counter summary (256^3 only)."""
+    if n != 256 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch k_s3d:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def log(*a):
@@ -50,7 +66,7 @@ def cpu_baseline(csts, params, shot, steps, threads):
     loop = oracle.last_loop_seconds()
     n = int(np.prod(csts["N"]))
     return dict(value=n * steps / loop / 1e9, unit="Gcell/s", cores=threads, kind="port",
-                sample=(f"{steps} time steps of the same {tuple(csts['N'])} grid "
+                sample=(f"{steps} time steps of the same {tuple(int(v) for v in csts['N'])} grid "
                         f"(loop {loop:.1f} s, {wall:.1f} s with set-up), "
                         "C oracle with OpenMP"))
 
@@ -61,7 +77,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256, help="grid size per axis")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 512 MiB device-to-device copy first (HBM counter calibration)")
@@ -134,7 +150,7 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": "update_s (3D viscoelastic stress update)",
             "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(args.n),
         },
     }
     m.close()

@@ -43,8 +43,13 @@ __device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int 
     return true;
 }
 
+// Waves per SIMD the velocity update is compiled for: FD orders 2 and 4 fit
+// 80 VGPRs (6 waves instead of 4) without spilling; higher orders need more.
+constexpr int v3d_waves(int H) { return H <= 2 ? 6 : H <= 4 ? 4 : 2; }
+
 template <typename R, int H>
-__global__ __launch_bounds__(64 * BY3) void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
+__global__ __launch_bounds__(64 * BY3) __attribute__((amdgpu_waves_per_eu(v3d_waves(H), 8)))
+void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
     const Layout &g = a.g;
     int z, y, x;
     if (!tile3(nbz, g.NX, nb, z, y, x)) return;

@@ -231,19 +231,28 @@ def test_analytic_2d(oracle_lib, L, tau):
     assert err < 6e-2, err       # tolerance of SeisCL/tests/test_analytics.py:142
 
 
-def test_analytic_3d(oracle_lib):
+@pytest.mark.parametrize("L,tau", [(0, 0.0), (2, 0.3)])
+def test_analytic_3d(oracle_lib, L, tau):
     N, dh, nab, NT, dt, f0 = 90, 4.0, 20, 500, 0.4e-3, 25.0
     vp, vs, rho = 3500.0, 2000.0, 2000.0
     sp, s, rp, offs = _analytic_setup(3, N, dh, nab, FD=4, NT=NT, dt=dt, f0=f0)
+    FL = np.array([f0 / (l + 1) for l in range(max(L, 1))])
     shp = (N, N, N)
     p = dict(vp=np.full(shp, vp, np.float32), vs=np.full(shp, vs, np.float32),
-             rho=np.full(shp, rho, np.float32))
+             rho=np.full(shp, rho, np.float32), taup=np.full(shp, tau, np.float32),
+             taus=np.full(shp, tau, np.float32))
     out = oracle_lib.forward(p, sp, s, rp, N=[N, N, N], ND=3, dh=dh, dt=dt, NT=NT, FDORDER=4,
-                             abs_type=1, nab=nab, f0=f0, seisout=1, VPPML=vp, FPML=f0)
+                             abs_type=1, nab=nab, f0=f0, seisout=1, VPPML=vp, FPML=f0,
+                             L=L, FL=FL)
     fd = out[-1].astype(np.float64)
-    ana = force_z_3d(vp, vs, rho, 0.0, 0.0, f0, [], dt, offs, ricker(f0, 2 * NT, dt))[2][:NT]
+    ana = force_z_3d(vp, vs, rho, tau, tau, f0, FL if L else [], dt, offs,
+                     ricker(f0, 2 * NT, dt))[2][:NT]
     err = rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana))
     assert err < 6e-2, err
+    if L:
+        # the attenuation is visible: the elastic solution misses by more
+        ela = force_z_3d(vp, vs, rho, 0.0, 0.0, f0, [], dt, offs, ricker(f0, 2 * NT, dt))[2][:NT]
+        assert rel_l2(fd / np.linalg.norm(fd), ela / np.linalg.norm(ela)) > 2 * err
 
 
 @pytest.mark.parametrize("ND", [2, 3])
