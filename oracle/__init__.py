@@ -70,6 +70,14 @@ def load():
         lib.or_nout.argtypes = [ctypes.POINTER(OrCsts)]
         lib.or_outname.restype = ctypes.c_char_p
         lib.or_outname.argtypes = [ctypes.POINTER(OrCsts), ctypes.c_int]
+        lib.or_gradient.restype = ctypes.c_int
+        lib.or_gradient.argtypes = ([ctypes.POINTER(OrCsts)] + [_FP] * 5 +
+                                    [ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
+                                     ctypes.c_int, _FP, _FP, _FP,
+                                     ctypes.POINTER(ctypes.c_double), _FP,
+                                     ctypes.c_char_p, ctypes.c_int])
+        lib.or_shell_check.restype = ctypes.c_int
+        lib.or_shell_check.argtypes = [ctypes.POINTER(OrCsts), ctypes.POINTER(ctypes.c_long)]
         lib.or_nvars.argtypes = [ctypes.POINTER(OrCsts)]
         lib.or_varname.restype = ctypes.c_char_p
         lib.or_varname.argtypes = [ctypes.POINTER(OrCsts), ctypes.c_int]
@@ -231,3 +239,74 @@ def check_stability(params: dict, **csts):
     rho = to_c(t["rho"])
     bad = lib.or_check_stability(ctypes.byref(c), fp(M), fp(mu), fp(rho), ctypes.byref(dts))
     return bool(bad), float(dts.value)
+
+
+FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE = 1, 2, 4
+FIX_ALL = 7
+
+
+def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FIX_ALL,
+             nthreads: int = 0, fields: bool = False, **csts):
+    """Oracle gradient of one shot by backpropagation.
+
+    din: observed seismograms (list of (NT, nrec) in out_names order), or
+    res: adjoint sources in the same layout (used as given, then scaled).
+    Returns dict(out=[(NT,nrec)...], grad={p0name, p1name, 'rho'} arrays in
+    model layout, res=[scaled adjoint sources], rms=(rms, rmsnorm)) and,
+    with fields=True, the backpropagated forward wavefields at the end."""
+    lib = load()
+    c = make_csts(nthreads=nthreads, **csts)
+    p0, p1, rho, taup, taus = _params(params, c)
+    sp = np.ascontiguousarray(np.asarray(src_pos, np.float32).T)
+    s = np.ascontiguousarray(np.asarray(src, np.float32).T)
+    rp = np.ascontiguousarray(np.asarray(rec_pos, np.float32).T)
+    nout = lib.or_nout(ctypes.byref(c))
+    nrec = rp.shape[0]
+
+    def pack(lst):
+        if lst is None:
+            return None
+        a = np.stack([np.asarray(x, np.float32).T for x in lst])
+        return np.ascontiguousarray(a.reshape(nout, nrec, c.NT))
+
+    d = pack(din)
+    r = pack(res)
+    out = np.zeros((nout, nrec, c.NT), np.float32)
+    res_out = np.zeros((nout, nrec, c.NT), np.float32)
+    shape = [int(v) for v in np.atleast_1d(csts["N"])]
+    n = int(np.prod(shape))
+    grad = np.zeros(3 * n, np.float32)
+    rms = (ctypes.c_double * 2)()
+    H = c.FDORDER // 2
+    pshape = [v + 2 * H for v in shape]
+    nv = int(np.prod(pshape))
+    nvars = lib.or_nvars(ctypes.byref(c))
+    fo = np.zeros(nvars * nv, np.float32) if fields else None
+    err = ctypes.create_string_buffer(256)
+    st = lib.or_gradient(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
+                         sp.shape[0], fp(sp), fp(s), nrec, fp(rp), fp(d), fp(r),
+                         int(gradfix), fp(out), fp(res_out), fp(grad), rms, fp(fo),
+                         err, 256)
+    if st:
+        raise RuntimeError("oracle: " + err.value.decode())
+    names = {0: ("vp", "vs"), 1: ("M", "mu"), 2: ("Ip", "Is"), 3: ("vp", "vs")}[c.param_type]
+    g = {}
+    for q, name in enumerate((names[0], names[1], "rho")):
+        g[name] = np.ascontiguousarray(grad[q * n:(q + 1) * n].reshape(shape[::-1]).T)
+    result = dict(out=[np.ascontiguousarray(o.T) for o in out], grad=g,
+                  res=[np.ascontiguousarray(o.T) for o in res_out], rms=(rms[0], rms[1]))
+    if fields:
+        fd = {}
+        for v, name in enumerate(var_names(c)):
+            fd[name] = np.ascontiguousarray(fo[v * nv:(v + 1) * nv].reshape(pshape[::-1]).T)
+        result["fields"] = fd
+    return result
+
+
+def shell_check(**csts):
+    """(inconsistencies, NBND) of the backpropagation boundary shell."""
+    lib = load()
+    c = make_csts(**csts)
+    n = ctypes.c_long()
+    bad = lib.or_shell_check(ctypes.byref(c), ctypes.byref(n))
+    return int(bad), int(n.value)

@@ -510,6 +510,10 @@ typedef struct {
     float *psi_vx_x, *psi_vy_x, *psi_vz_x, *psi_vx_y, *psi_vy_y, *psi_vz_y,
         *psi_vx_z, *psi_vy_z, *psi_vz_z;
     float *psi_p_x, *psi_p_z;
+    /* adjoint runs: the increment each update adds before the taper, per
+     * cell (2D v: vx vz; 2D s: sxx szz sxz; 3D v: vx vy vz; 3D s: sxx syy
+     * szz sxy syz sxz); NULL in forward runs */
+    float *inc[6];
 } sim;
 
 static float *zalloc(long n) { return n > 0 ? calloc(n, sizeof(float)) : NULL; }
@@ -589,8 +593,14 @@ static void update_v2D(sim *S) {
                     sxz_x = sxz_x / S->K_x[ind + 1] + *pxz;
                 }
             }
-            S->vx[indv] += ((sxx_x + sxz_z) * S->rip[indp]);
-            S->vz[indv] += ((szz_z + sxz_x) * S->rkp[indp]);
+            float lvx = ((sxx_x + sxz_z) * S->rip[indp]);
+            float lvz = ((szz_z + sxz_x) * S->rkp[indp]);
+            S->vx[indv] += lvx;
+            S->vz[indv] += lvz;
+            if (S->inc[0]) {
+                S->inc[0][indv] = lvx;
+                S->inc[1][indv] = lvz;
+            }
             if (tap) {
                 if (!fs && gidz - FDOH < NAB) {
                     S->vx[indv] *= S->taper[gidz - FDOH];
@@ -679,9 +689,17 @@ static void update_s2D(sim *S) {
                 float fipkp = S->muipkp[indp];
                 float f = 2.0 * S->mu[indp];
                 float g = S->M[indp];
-                S->sxz[indv] += (fipkp * (vxz + vzx));
-                S->sxx[indv] += (g * (vxx + vzz)) - (f * vzz);
-                S->szz[indv] += (g * (vxx + vzz)) - (f * vxx);
+                float lsxz = (fipkp * (vxz + vzx));
+                float lsxx = (g * (vxx + vzz)) - (f * vzz);
+                float lszz = (g * (vxx + vzz)) - (f * vxx);
+                S->sxz[indv] += lsxz;
+                S->sxx[indv] += lsxx;
+                S->szz[indv] += lszz;
+                if (S->inc[0]) {
+                    S->inc[0][indv] = lsxx;
+                    S->inc[1][indv] = lszz;
+                    S->inc[2][indv] = lsxz;
+                }
             } else {
                 float lM = S->M[indp], lmu = S->mu[indp], lmuipkp = S->muipkp[indp];
                 float ltaup = S->taup[indp], ltaus = S->taus[indp],
@@ -997,9 +1015,17 @@ static void update_v3D(sim *S) {
                         sxz_x = sxz_x / S->K_x[indm + 1] + PX(S->psi_sxz_x);
                     }
                 }
-                S->vx[indv] += ((sxx_x + sxy_y + sxz_z) * S->rip[indp]);
-                S->vy[indv] += ((syy_y + sxy_x + syz_z) * S->rjp[indp]);
-                S->vz[indv] += ((szz_z + sxz_x + syz_y) * S->rkp[indp]);
+                float lvx = ((sxx_x + sxy_y + sxz_z) * S->rip[indp]);
+                float lvy = ((syy_y + sxy_x + syz_z) * S->rjp[indp]);
+                float lvz = ((szz_z + sxz_x + syz_y) * S->rkp[indp]);
+                S->vx[indv] += lvx;
+                S->vy[indv] += lvy;
+                S->vz[indv] += lvz;
+                if (S->inc[0]) {
+                    S->inc[0][indv] = lvx;
+                    S->inc[1][indv] = lvy;
+                    S->inc[2][indv] = lvz;
+                }
                 if (tap) {
                     float *V[3] = {&S->vx[indv], &S->vy[indv], &S->vz[indv]};
                     for (int q = 0; q < 3; q++) {
@@ -1128,12 +1154,24 @@ static void update_s3D(sim *S) {
                     float fipkp = S->muipkp[indp];
                     float g = S->M[indp];
                     float f = 2.0 * S->mu[indp];
-                    S->sxy[indv] += (fipjp * vxyyx);
-                    S->syz[indv] += (fjpkp * vyzzy);
-                    S->sxz[indv] += (fipkp * vxzzx);
-                    S->sxx[indv] += ((g * vxxyyzz) - (f * vyyzz));
-                    S->syy[indv] += ((g * vxxyyzz) - (f * vxxzz));
-                    S->szz[indv] += ((g * vxxyyzz) - (f * vxxyy));
+                    float lsxy = (fipjp * vxyyx), lsyz = (fjpkp * vyzzy), lsxz = (fipkp * vxzzx);
+                    float lsxx = ((g * vxxyyzz) - (f * vyyzz));
+                    float lsyy = ((g * vxxyyzz) - (f * vxxzz));
+                    float lszz = ((g * vxxyyzz) - (f * vxxyy));
+                    S->sxy[indv] += lsxy;
+                    S->syz[indv] += lsyz;
+                    S->sxz[indv] += lsxz;
+                    S->sxx[indv] += lsxx;
+                    S->syy[indv] += lsyy;
+                    S->szz[indv] += lszz;
+                    if (S->inc[0]) {
+                        S->inc[0][indv] = lsxx;
+                        S->inc[1][indv] = lsyy;
+                        S->inc[2][indv] = lszz;
+                        S->inc[3][indv] = lsxy;
+                        S->inc[4][indv] = lsyz;
+                        S->inc[5][indv] = lsxz;
+                    }
                 } else {
                     float lM = S->M[indp], lmu = S->mu[indp];
                     float lmuipkp = S->muipkp[indp], lmuipjp = S->muipjp[indp],
@@ -1679,6 +1717,12 @@ int or_forward(const or_csts *c, const float *p0, const float *p1,
         return 1;
     }
     int tmax = c->tmax > 0 ? c->tmax : c->NT;
+    if (tmax > c->NT || c->tmin < 0 || c->tmin >= tmax) {
+        snprintf(err, errlen, "time window [tmin=%d, tmax=%d) outside [0, NT=%d]", c->tmin,
+                 tmax, c->NT);
+        sim_free(&S);
+        return 1;
+    }
     memset(out, 0, sizeof(float) * (size_t)or_nout(c) * nrec * c->NT);
     double t_start = wall_seconds();
     for (int t = 0; t < tmax; t++) {
@@ -1705,6 +1749,638 @@ int or_forward(const or_csts *c, const float *p0, const float *p1,
             memcpy(&fields_out[v * S.nv], named_var(&S, or_varname(c, v)),
                    S.nv * sizeof(float));
     }
+    sim_free(&S);
+    return 0;
+}
+
+/* ======================================================================== */
+/* Gradient by backpropagation (BACK_PROP_TYPE=1), elastic 2D and 3D.        */
+/* ======================================================================== */
+
+/* Boundary shell saved during the forward run and re-injected while the    */
+/* forward wavefield is run backwards: an FDOH-thick shell on the inner face */
+/* of the absorbing layer.  One device, FP32 storage.                        */
+/* savebnd: src/savebnd2D.cl:43-223, src/savebnd3D.cl:38-284;               */
+/* inject_ind: src/header_injectbnd.cl:25-232 (3D), :238-398 (2D);          */
+/* NBND: src/Init_OpenCL.c:988-1038.  Coordinates are padded.               */
+typedef struct {
+    int NXbnd, NYbnd, NZbnd, lbnd, lbnds;
+    long n;
+} shell_t;
+
+static void shell_init(const sim *S, shell_t *B) {
+    const int F = S->FDOH, NAB = S->NAB;
+    B->NXbnd = S->NXp - 2 * F - 2 * NAB;
+    B->NYbnd = S->ND == 3 ? S->NYp - 2 * F - 2 * NAB : 0;
+    B->NZbnd = S->c->freesurf ? S->NZp - 2 * F - NAB : S->NZp - 2 * F - 2 * NAB;
+    B->lbnd = F + NAB;
+    B->lbnds = S->c->freesurf ? F : F + NAB;
+    if (S->ND == 3)
+        B->n = (long)B->NYbnd * B->NZbnd * 2 * F + (long)B->NZbnd * (B->NXbnd - 2 * F) * 2 * F +
+               (long)(B->NYbnd - 2 * F) * (B->NXbnd - 2 * F) * 2 * F;
+    else
+        B->n = (long)B->NZbnd * F * 2 + (long)(B->NXbnd - 2 * F) * 2 * F;
+}
+
+/* padded cell of shell entry gid (the savebnd enumeration) */
+static long shell_cell(const sim *S, const shell_t *B, long gid) {
+    const int F = S->FDOH;
+    const long NXb = B->NXbnd, NYb = B->NYbnd, NZb = B->NZbnd;
+    long i, j = 0, k, g;
+    if (S->ND == 3) {
+        const long fb = NYb * NZb * F, lr = NZb * (NXb - 2 * F) * F,
+                   du = (NYb - 2 * F) * (NXb - 2 * F) * F;
+        if (gid < fb) {                                  /* front */
+            g = gid;
+            i = g / (NYb * NZb) + B->lbnd;
+            j = (g / NZb) % NYb + B->lbnd;
+            k = g % NZb + B->lbnds;
+        } else if (gid < 2 * fb) {                       /* back */
+            g = gid - fb;
+            i = g / (NYb * NZb) + NXb + S->NAB;
+            j = (g / NZb) % NYb + B->lbnd;
+            k = g % NZb + B->lbnds;
+        } else if (gid < 2 * fb + lr) {                  /* left */
+            g = gid - 2 * fb;
+            i = g / (NZb * F) + B->lbnd + F;
+            j = (g / NZb) % F + B->lbnd;
+            k = g % NZb + B->lbnds;
+        } else if (gid < 2 * fb + 2 * lr) {              /* right */
+            g = gid - 2 * fb - lr;
+            i = g / (NZb * F) + B->lbnd + F;
+            j = (g / NZb) % F + NYb + S->NAB;
+            k = g % NZb + B->lbnds;
+        } else if (gid < 2 * fb + 2 * lr + du) {         /* down */
+            g = gid - 2 * fb - 2 * lr;
+            i = g / (F * (NYb - 2 * F)) + B->lbnd + F;
+            j = (g / F) % (NYb - 2 * F) + B->lbnd + F;
+            k = g % F + NZb + B->lbnds - F;
+        } else {                                         /* up */
+            g = gid - 2 * fb - 2 * lr - du;
+            i = g / (F * (NYb - 2 * F)) + B->lbnd + F;
+            j = (g / F) % (NYb - 2 * F) + B->lbnd + F;
+            k = g % F + B->lbnds;
+        }
+        return i * S->NYp * S->NZp + j * S->NZp + k;
+    }
+    const long fb = NZb * F, du = (NXb - 2 * F) * F;
+    if (gid < fb) {                                      /* front */
+        i = gid / NZb + B->lbnd;
+        k = gid % NZb + B->lbnds;
+    } else if (gid < 2 * fb) {                           /* back */
+        g = gid - fb;
+        i = g / NZb + NXb + S->NAB;
+        k = g % NZb + B->lbnds;
+    } else if (gid < 2 * fb + du) {                      /* bottom */
+        g = gid - 2 * fb;
+        i = g % (NXb - 2 * F) + B->lbnd + F;
+        k = g / (NXb - 2 * F) + NZb + B->lbnds - F;
+    } else {                                             /* up */
+        g = gid - 2 * fb - du;
+        i = g % (NXb - 2 * F) + B->lbnd + F;
+        k = g / (NXb - 2 * F) + B->lbnds;
+    }
+    return i * S->NZp + k;
+}
+
+/* shell entry of padded cell (k, j, i), or -1 (inject_ind) */
+static long inject_ind(const sim *S, const shell_t *B, int k, int j, int i) {
+    const int F = S->FDOH;
+    const long NXb = B->NXbnd, NYb = B->NYbnd, NZb = B->NZbnd;
+    i -= B->lbnd;
+    k -= B->lbnds;
+    if (S->ND == 3) {
+        j -= B->lbnd;
+        if ((k > F - 1 && k < NZb - F) && (j > F - 1 && j < NYb - F) && (i > F - 1 && i < NXb - F))
+            return -1;
+        if (k < 0 || k > NZb - 1 || j < 0 || j > NYb - 1 || i < 0 || i > NXb - 1) return -1;
+        if (i < F) return (long)i * NYb * NZb + (long)j * NZb + k;
+        if (i > NXb - 1 - F) {
+            i = i - NXb + F;
+            return NYb * NZb * F + (long)i * NYb * NZb + (long)j * NZb + k;
+        }
+        if (j < F) {
+            i = i - F;
+            return NYb * NZb * F * 2 + (long)i * F * NZb + (long)j * NZb + k;
+        }
+        if (j > NYb - 1 - F) {
+            i = i - F;
+            j = j - NYb + F;
+            return NYb * NZb * F * 2 + (NXb - 2 * F) * NZb * F + (long)i * F * NZb + (long)j * NZb + k;
+        }
+        if (k < F) {
+            i = i - F;
+            j = j - F;
+            return NYb * NZb * F * 2 + (NXb - 2 * F) * NZb * F * 2 + (NXb - 2 * F) * (NYb - 2 * F) * F +
+                   (long)i * (NYb - 2 * F) * F + (long)j * F + k;
+        }
+        i = i - F;
+        j = j - F;
+        k = k - NZb + F;
+        return NYb * NZb * F * 2 + (NXb - 2 * F) * NZb * F * 2 + (long)i * (NYb - 2 * F) * F +
+               (long)j * F + k;
+    }
+    if ((k > F - 1 && k < NZb - F) && (i > F - 1 && i < NXb - F)) return -1;
+    if (k < 0 || k > NZb - 1 || i < 0 || i > NXb - 1) return -1;
+    if (i < F) return (long)i * NZb + k;
+    if (i > NXb - 1 - F) {
+        i = i - NXb + F;
+        return NZb * F + (long)i * NZb + k;
+    }
+    if (k < F) {
+        i = i - F;
+        return NZb * F * 2 + (NXb - 2 * F) * F + i + (long)k * (NXb - 2 * F);
+    }
+    i = i - F;
+    k = k - NZb + F;
+    return NZb * F * 2 + i + (long)k * (NXb - 2 * F);
+}
+
+/* forward variables kept on the shell, in the order of the boundary buffer */
+static int shell_vars(const sim *S, float **v) {
+    if (S->ND == 3) {
+        float *w[9] = {S->vx, S->vy, S->vz, S->sxx, S->syy, S->szz, S->sxy, S->syz, S->sxz};
+        memcpy(v, w, sizeof(w));
+        return 9;
+    }
+    float *w[5] = {S->vx, S->vz, S->sxx, S->szz, S->sxz};
+    memcpy(v, w, sizeof(w));
+    return 5;
+}
+
+int or_shell_check(const or_csts *c, long *nbnd) {
+    sim S;
+    char err[128];
+    if (sim_init(&S, c, err, sizeof(err))) {
+        sim_free(&S);
+        return -1;
+    }
+    shell_t B;
+    shell_init(&S, &B);
+    *nbnd = B.n;
+    long bad = 0, inside = 0;
+    char *seen = calloc(B.n > 0 ? B.n : 1, 1);
+    for (long g = 0; g < B.n; g++) {
+        long cell = shell_cell(&S, &B, g);
+        int i = (int)(cell / ((long)S.NYp * S.NZp));
+        int j = (int)((cell / S.NZp) % S.NYp);
+        int k = (int)(cell % S.NZp);
+        if (inject_ind(&S, &B, k, j, i) != g) bad++;
+    }
+    for (int i = 0; i < S.NXp; i++)
+        for (int j = 0; j < S.NYp; j++)
+            for (int k = 0; k < S.NZp; k++) {
+                long m = inject_ind(&S, &B, k, j, i);
+                if (m < 0) continue;
+                inside++;
+                if (m >= B.n || seen[m]) bad++;
+                else seen[m] = 1;
+            }
+    free(seen);
+    sim_free(&S);
+    if (inside != B.n) bad++;
+    return (int)bad;
+}
+
+static void save_shell(sim *S, const shell_t *B, float *dst) {
+    float *v[9];
+    const int nv = shell_vars(S, v);
+    for (int q = 0; q < nv; q++)
+        for (long g = 0; g < B->n; g++) dst[q * B->n + g] = v[q][shell_cell(S, B, g)];
+}
+
+static void inject_shell(sim *S, const shell_t *B, const float *src, int first, int count) {
+    float *v[9];
+    shell_vars(S, v);
+    for (int q = first; q < first + count; q++)
+        for (long g = 0; g < B->n; g++) v[q][shell_cell(S, B, g)] = src[q * B->n + g];
+}
+
+/* Backpropagation of the forward stresses and velocities: the forward       */
+/* increment without CPML, subtracted (src/update_adjs2D.cl:205-250,        */
+/* src/update_adjv2D.cl:196-210, src/update_adjs3D.cl:309-324,               */
+/* src/update_adjv3D.cl:296-305).  Elastic only.                             */
+static void backprop_s(sim *S) {
+    const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
+    const float *hc = S->hc;
+    const long sy = NZ, sx = (long)NY * NZ;
+    const long nzi = NZ - 2 * FDOH, nyi = NY - 2 * FDOH;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
+        for (int gidy = (S->ND == 3 ? FDOH : 0); gidy < (S->ND == 3 ? NY - FDOH : 1); gidy++)
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - (S->ND == 3 ? FDOH : 0)) * nzi + (gidz - FDOH);
+                long indv = (long)gidx * sx + (long)gidy * sy + gidz;
+                if (S->ND == 2) {
+                    indp = (long)(gidx - FDOH) * nzi + (gidz - FDOH);
+                    indv = (long)gidx * NZ + gidz;
+                    float vxx = Dm(S->vx, indv, NZ, hc, FDOH);
+                    float vxz = Dp(S->vx, indv, 1, hc, FDOH);
+                    float vzz = Dm(S->vz, indv, 1, hc, FDOH);
+                    float vzx = Dp(S->vz, indv, NZ, hc, FDOH);
+                    float fipkp = S->muipkp[indp];
+                    float f = 2.0 * S->mu[indp];
+                    float g = S->M[indp];
+                    S->sxz[indv] -= (fipkp * (vxz + vzx));
+                    S->sxx[indv] -= (g * (vxx + vzz)) - (f * vzz);
+                    S->szz[indv] -= (g * (vxx + vzz)) - (f * vxx);
+                    continue;
+                }
+                float vxx = Dm(S->vx, indv, sx, hc, FDOH);
+                float vxy = Dp(S->vx, indv, sy, hc, FDOH);
+                float vxz = Dp(S->vx, indv, 1, hc, FDOH);
+                float vyx = Dp(S->vy, indv, sx, hc, FDOH);
+                float vyy = Dm(S->vy, indv, sy, hc, FDOH);
+                float vyz = Dp(S->vy, indv, 1, hc, FDOH);
+                float vzx = Dp(S->vz, indv, sx, hc, FDOH);
+                float vzy = Dp(S->vz, indv, sy, hc, FDOH);
+                float vzz = Dm(S->vz, indv, 1, hc, FDOH);
+                float fipjp = S->muipjp[indp], fjpkp = S->mujpkp[indp], fipkp = S->muipkp[indp];
+                float g = S->M[indp];
+                float f = 2.0 * S->mu[indp];
+                float vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+                float vxxyyzz = vxx + vyy + vzz, vyyzz = vyy + vzz, vxxzz = vxx + vzz,
+                      vxxyy = vxx + vyy;
+                S->sxy[indv] -= (fipjp * vxyyx);
+                S->syz[indv] -= (fjpkp * vyzzy);
+                S->sxz[indv] -= (fipkp * vxzzx);
+                S->sxx[indv] -= ((g * vxxyyzz) - (f * vyyzz));
+                S->syy[indv] -= ((g * vxxyyzz) - (f * vxxzz));
+                S->szz[indv] -= ((g * vxxyyzz) - (f * vxxyy));
+            }
+}
+
+static void backprop_v(sim *S) {
+    const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
+    const float *hc = S->hc;
+    const long sy = NZ, sx = (long)NY * NZ;
+    const long nzi = NZ - 2 * FDOH, nyi = NY - 2 * FDOH;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
+        for (int gidy = (S->ND == 3 ? FDOH : 0); gidy < (S->ND == 3 ? NY - FDOH : 1); gidy++)
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                if (S->ND == 2) {
+                    long indp = (long)(gidx - FDOH) * nzi + (gidz - FDOH);
+                    long indv = (long)gidx * NZ + gidz;
+                    float sxx_x = Dp(S->sxx, indv, NZ, hc, FDOH);
+                    float szz_z = Dp(S->szz, indv, 1, hc, FDOH);
+                    float sxz_z = Dm(S->sxz, indv, 1, hc, FDOH);
+                    float sxz_x = Dm(S->sxz, indv, NZ, hc, FDOH);
+                    float lvx = ((sxx_x + sxz_z) * S->rip[indp]);
+                    float lvz = ((szz_z + sxz_x) * S->rkp[indp]);
+                    S->vx[indv] -= lvx;
+                    S->vz[indv] -= lvz;
+                    continue;
+                }
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - FDOH) * nzi + (gidz - FDOH);
+                long indv = (long)gidx * sx + (long)gidy * sy + gidz;
+                float szz_z = Dp(S->szz, indv, 1, hc, FDOH);
+                float sxx_x = Dp(S->sxx, indv, sx, hc, FDOH);
+                float sxz_x = Dm(S->sxz, indv, sx, hc, FDOH);
+                float sxz_z = Dm(S->sxz, indv, 1, hc, FDOH);
+                float syz_y = Dm(S->syz, indv, sy, hc, FDOH);
+                float syz_z = Dm(S->syz, indv, 1, hc, FDOH);
+                float syy_y = Dp(S->syy, indv, sy, hc, FDOH);
+                float sxy_x = Dm(S->sxy, indv, sx, hc, FDOH);
+                float sxy_y = Dm(S->sxy, indv, sy, hc, FDOH);
+                float lvx = ((sxx_x + sxy_y + sxz_z) * S->rip[indp]);
+                float lvy = ((syy_y + sxy_x + syz_z) * S->rjp[indp]);
+                float lvz = ((szz_z + sxz_x + syz_y) * S->rkp[indp]);
+                S->vx[indv] -= lvx;
+                S->vy[indv] -= lvy;
+                S->vz[indv] -= lvz;
+            }
+}
+
+/* Gradient increments of one adjoint step, accumulated in double.          */
+/* rho: src/update_adjv2D.cl:326, src/update_adjv3D.cl:481.                  */
+/* M, mu: src/update_adjs2D.cl:441-451, src/update_adjs3D.cl:679-689.        */
+static void grad_rho(const sim *S, const sim *A, double *grho) {
+    const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
+    const long nzi = NZ - 2 * FDOH, nyi = S->ND == 3 ? NY - 2 * FDOH : 1;
+    const int y0 = S->ND == 3 ? FDOH : 0, y1 = S->ND == 3 ? NY - FDOH : 1;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
+        for (int gidy = y0; gidy < y1; gidy++)
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - y0) * nzi + (gidz - FDOH);
+                long indv = (long)gidx * NY * NZ + (long)gidy * NZ + gidz;
+                float d;
+                if (S->ND == 3)
+                    d = -S->vx[indv] * A->inc[0][indv] - S->vy[indv] * A->inc[1][indv] -
+                        S->vz[indv] * A->inc[2][indv];
+                else
+                    d = -S->vx[indv] * A->inc[0][indv] - S->vz[indv] * A->inc[1][indv];
+                grho[indp] += d;
+            }
+}
+
+static void grad_Mmu(const sim *S, const sim *A, double *gM, double *gmu) {
+    const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
+    const long nzi = NZ - 2 * FDOH, nyi = S->ND == 3 ? NY - 2 * FDOH : 1;
+    const int y0 = S->ND == 3 ? FDOH : 0, y1 = S->ND == 3 ? NY - FDOH : 1;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
+        for (int gidy = y0; gidy < y1; gidy++)
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - y0) * nzi + (gidz - FDOH);
+                long indv = (long)gidx * NY * NZ + (long)gidy * NZ + gidz;
+                float lM = S->M[indp], lmu = S->mu[indp];
+                if (S->ND == 2) {
+                    float lsxx = A->inc[0][indv], lszz = A->inc[1][indv], lsxz = A->inc[2][indv];
+                    float sxx = S->sxx[indv], szz = S->szz[indv], sxz = S->sxz[indv];
+                    float c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
+                    float c3 = 1.0 / (lmu * lmu);
+                    float c5 = 0.25 * c3;
+                    float dM = c1 * (sxx + szz) * (lsxx + lszz);
+                    gM[indp] += -dM;
+                    gmu[indp] += -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+                    continue;
+                }
+                float lsxx = A->inc[0][indv], lsyy = A->inc[1][indv], lszz = A->inc[2][indv];
+                float lsxy = A->inc[3][indv], lsyz = A->inc[4][indv], lsxz = A->inc[5][indv];
+                float sxx = S->sxx[indv], syy = S->syy[indv], szz = S->szz[indv];
+                float sxy = S->sxy[indv], syz = S->syz[indv], sxz = S->sxz[indv];
+                float c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
+                float c3 = 1.0 / lmu / lmu;
+                float c5 = 1.0 / 6.0 * c3;
+                float dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
+                gM[indp] += -dM;
+                gmu[indp] += -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
+                             c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
+                                   lszz * (2.0 * szz - sxx - syy));
+            }
+}
+
+/* Adjoint source injection: generated kernel of                           */
+/* src/automatic_kernels.c:631-836; p spreads res/ND over the normal        */
+/* stresses (integer literal ND, float division).                           */
+static void residuals_inject(sim *A, int nt, int nrec, const float *rec_pos, const float *res) {
+    const int NT = A->c->NT, nout = or_nout(A->c);
+    for (int r = 0; r < nrec; r++) {
+        int i, k;
+        long pos = cell_of(A, &rec_pos[8 * r], &i, &k);
+        if (i < A->FDOH || i > A->NXp - A->FDOH - 1) continue;
+        for (int o = 0; o < nout; o++) {
+            const char *name = or_outname(A->c, o);
+            float v = res[((long)o * nrec + r) * NT + nt];
+            if (!strcmp(name, "p") && A->ND != 22) {
+                float q = v / (A->ND == 3 ? 3 : 2);
+                A->sxx[pos] += q;
+                A->szz[pos] += q;
+                if (A->ND == 3) A->syy[pos] += q;
+            } else {
+                named_var(A, name)[pos] += v;
+            }
+        }
+    }
+}
+
+/* L2 residuals (src/residuals.c:58-450, no filter, no rms scaling):        */
+/* res = out - in for t < tmax; rms += (in - out)^2, rmsnorm += in^2.       */
+static void residuals_l2(const or_csts *c, int nrec, const float *out, const float *din,
+                         float *res, double *rms) {
+    const int NT = c->NT, nout = or_nout(c), tmax = c->tmax > 0 ? c->tmax : c->NT;
+    for (int g = 0; g < nrec; g++)
+        for (int t = 0; t < tmax; t++)
+            for (int o = 0; o < nout; o++) {
+                long q = ((long)o * nrec + g) * NT + t;
+                rms[0] += pow(-out[q] + din[q], 2);
+                rms[1] += pow(din[q], 2);
+                res[q] = 1.0f * (out[q] - din[q]);
+            }
+}
+
+/* res_scale (src/residuals.c:452-570), FP32 storage.  With at_receiver=0  */
+/* the velocity scaling reads the buoyancy at (z=x, y=x, x=x) of the       */
+/* receiver's x index, as the reference does (:487-491); at_receiver=1     */
+/* reads it at the receiver's own cell.                                      */
+static void res_scale(const sim *S, int nrec, const float *rec_pos, float *res, int at_receiver) {
+    const or_csts *c = S->c;
+    const int NT = c->NT, nout = or_nout(c), tmax = c->tmax > 0 ? c->tmax : c->NT;
+    for (int o = 0; o < nout; o++) {
+        const char *name = or_outname(c, o);
+        const float *par = NULL;
+        int isv = 0;
+        if (!strcmp(name, "vx")) { par = S->rip; isv = 1; }
+        else if (!strcmp(name, "vy")) { par = S->rjp; isv = 1; }
+        else if (!strcmp(name, "vz")) { par = S->rkp; isv = 1; }
+        else if (!strcmp(name, "p")) { par = S->M; }
+        else continue;
+        if (!isv && S->ND == 22) continue;
+        for (int g = 0; g < nrec; g++) {
+            int x, y, z;
+            if (isv && !at_receiver) {
+                x = y = z = (int)(rec_pos[0 + 8 * g] / c->dh);
+            } else {
+                x = (int)(rec_pos[0 + 8 * g] / c->dh);
+                y = (int)(rec_pos[1 + 8 * g] / c->dh);
+                z = (int)(rec_pos[2 + 8 * g] / c->dh);
+            }
+            long pos = S->ND == 3 ? (long)x * S->NZ * S->NY + (long)y * S->NZ + z
+                                  : (long)x * S->NZ + z;
+            if (pos < 0 || pos >= S->np) continue;
+            float *r = &res[((long)o * nrec + g) * NT];
+            if (isv) {
+                float parscal = 1.0 / par[pos] * c->dh / c->dt * powf(2, 0);
+                for (int t = 0; t < tmax; t++) r[t] *= 1.0 / parscal * c->dt;
+            } else {
+                float parscal = -2.0 * (par[pos] - S->mu[pos]) * c->dh / c->dt * powf(2, -0);
+                for (int t = 0; t < tmax; t++) r[t] *= parscal * c->dt;
+            }
+        }
+    }
+}
+
+/* transf_grad (src/calc_grad.c:975-1110), FP32 storage, par_scale 0, in  */
+/* double; with rho_fix the direct density term is scaled by dh^2/dt^2     */
+/* (finite differences of the misfit confirm the scaled form).              */
+/* double from the double accumulators.  Outputs follow param_type: 0      */
+/* vp, vs, rho; 1 M, mu, rho; 2 Ip, Is, rho.                                 */
+static void transf_grad(const sim *S, const double *gM, const double *gmu, const double *grho,
+                        float *out, int rho_fix) {
+    const or_csts *c = S->c;
+    const long n = S->np;
+    for (long i = 0; i < n; i++) {
+        double rho = 1.0 / S->rho[i] * c->dt / c->dh;
+        double M = (double)S->M[i] * c->dh / c->dt;
+        double mu = (double)S->mu[i] * c->dh / c->dt;
+        double gr = grho[i] / c->dt, gm = gM[i] / c->dt, gu = gmu[i] / c->dt;
+        /* the density kernel accumulates v . dv_adjoint, whose scale differs */
+        /* from dJ/drho by dh^2/dt^2; the reference omits that factor         */
+        if (rho_fix) gr *= (double)c->dh * c->dh / ((double)c->dt * c->dt);
+        if (c->param_type == 0 || c->param_type == 2) {
+            gr = gr + M / rho * gm;
+            if (mu > 0) gr = gr + mu / rho * gu;
+            if (c->param_type == 0) {
+                gm = 2.0 * sqrt(rho * M) * gm;
+                gu = 2.0 * sqrt(rho * mu) * gu;
+            } else {
+                gm = 2.0 * sqrt(M / rho) * gm;
+                gu = 2.0 * sqrt(mu / rho) * gu;
+            }
+        }
+        out[i] = (float)gm;
+        out[n + i] = (float)gu;
+        out[2 * n + i] = (float)gr;
+    }
+}
+
+static int adj_alloc(sim *A, const sim *S, char *err, int errlen) {
+    /* the adjoint run shares the model and the absorbing profiles; it has   */
+    /* its own wavefields and its own zeroed CPML memory                      */
+    if (sim_init(A, S->c, err, errlen)) return 1;
+    free(A->M); free(A->mu); free(A->rho); free(A->rip); free(A->rjp); free(A->rkp);
+    free(A->muipkp); free(A->muipjp); free(A->mujpkp);
+    A->M = S->M; A->mu = S->mu; A->rho = S->rho; A->rip = S->rip; A->rjp = S->rjp;
+    A->rkp = S->rkp; A->muipkp = S->muipkp; A->muipjp = S->muipjp; A->mujpkp = S->mujpkp;
+    int ninc = S->ND == 3 ? 6 : 3;
+    for (int q = 0; q < ninc; q++) A->inc[q] = zalloc(S->nv);
+    return 0;
+}
+
+static void adj_free(sim *A) {
+    A->M = A->mu = A->rho = A->rip = A->rjp = A->rkp = A->muipkp = A->muipjp = A->mujpkp = NULL;
+    for (int q = 0; q < 6; q++) {
+        free(A->inc[q]);
+        A->inc[q] = NULL;
+    }
+    sim_free(A);
+}
+
+int or_gradient(const or_csts *c, const float *p0, const float *p1, const float *rho,
+                const float *taup, const float *taus, int nsrc, const float *src_pos,
+                const float *src, int nrec, const float *rec_pos, const float *din,
+                const float *res_in, int gradfix, float *out, float *res_out,
+                float *grad, double *rms, float *fields_out, char *err, int errlen) {
+    sim S, A;
+    char dummy[8];
+    if (!err) {
+        err = dummy;
+        errlen = sizeof(dummy);
+    }
+    if (c->ND == 22 || c->L > 0) {
+        snprintf(err, errlen, "gradients need the elastic case (ND 2 or 3, L=0)");
+        return 1;
+    }
+    if (c->freesurf) {
+        snprintf(err, errlen, "gradients with a free surface are not supported");
+        return 1;
+    }
+    if (sim_init(&S, c, err, errlen)) {
+        sim_free(&S);
+        return 1;
+    }
+#ifdef _OPENMP
+    if (c->nthreads > 0) omp_set_num_threads(c->nthreads);
+#endif
+    or_transform(c, p0, p1, rho, taup, taus, S.M, S.mu, S.rho, S.rip, S.rjp, S.rkp,
+                 S.muipkp, S.muipjp, S.mujpkp, S.taup, S.taus, S.tausipkp,
+                 S.tausipjp, S.tausjpkp);
+    float dts;
+    if (or_check_stability(c, S.M, S.mu, S.rho, &dts)) {
+        snprintf(err, errlen, "Time step too large, to be stable, set dt<%f", dts);
+        sim_free(&S);
+        return 1;
+    }
+    if (check_positions(&S, nsrc, src_pos, 5, err, errlen) ||
+        check_positions(&S, nrec, rec_pos, 8, err, errlen)) {
+        sim_free(&S);
+        return 1;
+    }
+    shell_t B;
+    shell_init(&S, &B);
+    const int tmax = c->tmax > 0 ? c->tmax : c->NT, tmin = c->tmin;
+    if (tmax > c->NT || tmin < 0 || tmin >= tmax) {
+        snprintf(err, errlen, "time window [tmin=%d, tmax=%d) outside [0, NT=%d]", tmin, tmax,
+                 c->NT);
+        sim_free(&S);
+        return 1;
+    }
+    const int nvb = S.ND == 3 ? 9 : 5, nout = or_nout(c);
+    float *bnd = malloc(sizeof(float) * (size_t)tmax * nvb * B.n);
+    memset(out, 0, sizeof(float) * (size_t)nout * nrec * c->NT);
+    /* forward with boundary saving: src/time_stepping.c:755-835 */
+    for (int t = 0; t < tmax; t++) {
+        sources(&S, t, 1, nsrc, src_pos, src);
+        if (S.ND == 3) {
+            update_v3D(&S);
+            update_s3D(&S);
+        } else {
+            update_v2D(&S);
+            update_s2D(&S);
+        }
+        save_shell(&S, &B, bnd + (size_t)t * nvb * B.n);
+        varsout(&S, t, nrec, rec_pos, out);
+    }
+    /* residuals and their scaling: src/time_stepping.c:838-848 */
+    size_t nres = (size_t)nout * nrec * c->NT;
+    float *res = calloc(nres, sizeof(float));
+    double rr[2] = {0, 0};
+    if (res_in) memcpy(res, res_in, nres * sizeof(float));
+    else residuals_l2(c, nrec, out, din, res, rr);
+    res_scale(&S, nrec, rec_pos, res, gradfix & OR_FIX_RES_SCALE);
+    if (res_out) memcpy(res_out, res, nres * sizeof(float));
+    if (rms) {
+        rms[0] = rr[0];
+        rms[1] = rr[1];
+    }
+    /* adjoint: src/time_stepping.c:869-940 */
+    if (adj_alloc(&A, &S, err, errlen)) {
+        free(bnd);
+        free(res);
+        sim_free(&S);
+        return 1;
+    }
+    if (!(gradfix & OR_FIX_ADJ_CPML)) {
+        /* the reference's adjoint kernels update the forward CPML memory,   */
+        /* which still holds the end of the forward run                      */
+        float **fw[] = {&S.psi_sxx_x, &S.psi_sxy_x, &S.psi_sxz_x, &S.psi_sxy_y, &S.psi_syy_y,
+                        &S.psi_syz_y, &S.psi_sxz_z, &S.psi_syz_z, &S.psi_szz_z, &S.psi_vx_x,
+                        &S.psi_vy_x, &S.psi_vz_x, &S.psi_vx_y, &S.psi_vy_y, &S.psi_vz_y,
+                        &S.psi_vx_z, &S.psi_vy_z, &S.psi_vz_z};
+        float **aw[] = {&A.psi_sxx_x, &A.psi_sxy_x, &A.psi_sxz_x, &A.psi_sxy_y, &A.psi_syy_y,
+                        &A.psi_syz_y, &A.psi_sxz_z, &A.psi_syz_z, &A.psi_szz_z, &A.psi_vx_x,
+                        &A.psi_vy_x, &A.psi_vz_x, &A.psi_vx_y, &A.psi_vy_y, &A.psi_vz_y,
+                        &A.psi_vx_z, &A.psi_vy_z, &A.psi_vz_z};
+        long nab2 = 2 * S.NAB, nyi = S.ND == 3 ? S.NY : 1;
+        long sizes[3] = {nab2 * nyi * S.NZ, (long)S.NX * nab2 * S.NZ, (long)S.NX * nyi * nab2};
+        const int axis[18] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 1, 1, 1, 2, 2, 2};
+        for (int q = 0; q < 18; q++)
+            if (*fw[q] && *aw[q]) memcpy(*aw[q], *fw[q], sizeof(float) * sizes[axis[q]]);
+    }
+    double *gM = calloc(S.np, sizeof(double)), *gmu = calloc(S.np, sizeof(double)),
+           *grho = calloc(S.np, sizeof(double));
+    for (int t = tmax - 1; t > tmin; t--) {
+        const float *bprev = bnd + (size_t)(t - 1) * nvb * B.n;
+        residuals_inject(&A, t, nrec, rec_pos, res);
+        /* update_adjs: backpropagate the stresses, re-inject their shell,  */
+        /* update the adjoint stresses, accumulate the M and mu gradients    */
+        backprop_s(&S);
+        inject_shell(&S, &B, bprev, S.ND == 3 ? 3 : 2, S.ND == 3 ? 6 : 3);
+        if (S.ND == 3) update_s3D(&A);
+        else update_s2D(&A);
+        grad_Mmu(&S, &A, gM, gmu);
+        /* update_adjv: the same for the velocities and the density */
+        backprop_v(&S);
+        inject_shell(&S, &B, bprev, 0, S.ND == 3 ? 3 : 2);
+        if (S.ND == 3) update_v3D(&A);
+        else update_v2D(&A);
+        grad_rho(&S, &A, grho);
+        sources(&S, t, -1, nsrc, src_pos, src);
+    }
+    transf_grad(&S, gM, gmu, grho, grad, gradfix & OR_FIX_RHO_SCALE);
+    if (fields_out) {
+        for (int v = 0; v < or_nvars(c); v++)
+            memcpy(&fields_out[v * S.nv], named_var(&S, or_varname(c, v)),
+                   S.nv * sizeof(float));
+    }
+    free(gM);
+    free(gmu);
+    free(grho);
+    free(bnd);
+    free(res);
+    adj_free(&A);
     sim_free(&S);
     return 0;
 }

@@ -22,6 +22,9 @@ extern "C" {
 #endif
 
 #define OR_MAXL 8
+#define OR_FIX_RES_SCALE 1
+#define OR_FIX_ADJ_CPML 2
+#define OR_FIX_RHO_SCALE 4
 
 typedef struct or_csts {
     int ND;            /* 2: 2D P-SV elastic, 3: 3D, 22: 2D acoustic           */
@@ -82,6 +85,34 @@ int  or_forward(const or_csts *c, const float *p0, const float *p1,
 
 /* wall-clock seconds of the time loop of the last or_forward call */
 double or_last_loop_seconds(void);
+
+/* Backpropagation boundary shell: returns the number of inconsistencies
+ * between the save enumeration and the inject lookup (0 when they are
+ * inverse maps covering the same cells) and the shell size in *nbnd. */
+int  or_shell_check(const or_csts *c, long *nbnd);
+
+/* Gradient of one shot by backpropagation with boundary saving
+ * (src/time_stepping.c:747-1002, BACK_PROP_TYPE=1), elastic L=0 without a
+ * free surface.  Adjoint sources: res_in [nout][nrec][NT] when given
+ * (inputres), else out - din; both are scaled as res_scale does.
+ * gradfix selects corrections of reference defects (0 = as the reference):
+ *   OR_FIX_RES_SCALE  res_scale reads the buoyancy at the receiver's own
+ *                     cell (the reference uses x for all three indices);
+ *   OR_FIX_ADJ_CPML   the adjoint run starts from zeroed CPML memory (the
+ *                     reference reuses the forward run's);
+ *   OR_FIX_RHO_SCALE  the direct density term of the gradient is scaled by
+ *                     dh^2/dt^2.
+ * grad: [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), gradients
+ * accumulated in double.  res_out (optional): the scaled adjoint sources;
+ * rms (optional): [rms, rmsnorm]; fields_out (optional): the backpropagated
+ * forward wavefields at the end, padded. */
+int  or_gradient(const or_csts *c, const float *p0, const float *p1,
+                 const float *rho, const float *taup, const float *taus,
+                 int nsrc, const float *src_pos, const float *src,
+                 int nrec, const float *rec_pos, const float *din,
+                 const float *res_in, int gradfix, float *out,
+                 float *res_out, float *grad, double *rms, float *fields_out,
+                 char *err, int errlen);
 
 #ifdef __cplusplus
 }
