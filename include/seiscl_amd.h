@@ -54,6 +54,14 @@ typedef struct seis_csts {
     int tmin;           /* first adjoint step bound (src/time_stepping.c:869)     */
     int gradout;        /* 1: keep what the gradient needs                        */
     int FP16;           /* 0: FP32; 2: FP16 storage, FP32 compute; 64: FP64      */
+    int gradfix;        /* gradient corrections of reference defects (bit set):   */
+                        /*  1 res_scale reads the buoyancy at the receiver's own  */
+                        /*    cell (src/residuals.c:487-491 uses x three times);  */
+                        /*  2 the adjoint run has its own zeroed CPML memory (the */
+                        /*    reference's adjoint kernels bind the forward one);  */
+                        /*  4 the direct density term is scaled by dh^2/dt^2      */
+                        /*    (src/calc_grad.c:1005-1013 omits it).               */
+                        /* seis_csts_default sets 7; 0 reproduces the reference.  */
 } seis_csts;
 
 typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
@@ -130,6 +138,44 @@ int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos,
 /* padded layout, Pi(N+FDORDER) floats (used by movout, src/time_stepping.c: */
 /* 272-350, and by parity tests).                                              */
 int seis_read_field(seis_model *m, const char *name, float *host);
+
+/* ---- gradient by backpropagation (BACK_PROP_TYPE=1), elastic ND 2/3,      */
+/* L=0, no free surface.  The forward run saves an FDOH-thick boundary shell */
+/* of every wavefield at every step in device memory (save_bnd,              */
+/* src/time_stepping.c:352-395, kept resident instead of the reference's     */
+/* pinned host copy); the adjoint run re-injects it while running the        */
+/* forward wavefield backwards (inject_bnd, :397-424) and accumulates the    */
+/* gradient in double (update_adjs / update_adjv, src/update_adj*.cl).       */
+/* Zero the gradient accumulators (kernel_gradinit, src/automatic_kernels.c: */
+/* 840-930; src/time_stepping.c:725-729).                                     */
+int seis_grad_zero(seis_model *m);
+/* Bind a shot and run it forward with boundary saving; out as             */
+/* seis_forward_shot (may be NULL).                                          */
+int seis_forward_save(seis_model *m, int nsrc, const float *src_pos,
+                      const float *src, int nrec, const float *rec_pos,
+                      float *out);
+/* L2 residuals out - din for t < tmax, [nout][nrec][NT] (var_res_raw,      */
+/* src/residuals.c:58-450, without filters or rms scaling); rms = sum       */
+/* (din-out)^2, rmsnorm = sum din^2.  Host only.                             */
+int seis_residuals(const seis_model *m, int nrec, const float *out,
+                   const float *din, float *res, double *rms, double *rmsnorm);
+/* Scale adjoint sources in place (res_scale, src/residuals.c:452-570).      */
+int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos,
+                   float *res);
+/* Adjoint run of the shot bound by seis_forward_save with scaled adjoint   */
+/* sources res [nout][nrec][NT]: steps tmax-1 .. tmin+1 (src/time_stepping.c:*/
+/* 869-940); adds this shot's gradient to the accumulators.                 */
+int seis_adjoint(seis_model *m, const float *res);
+/* Gradients [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), in the */
+/* parameter layout (transf_grad, src/calc_grad.c:975-1110).                 */
+int seis_grad_read(seis_model *m, float *grad);
+/* One shot of gradient: forward with boundary saving, residuals against    */
+/* din (or the given adjoint sources res_in, the inputres mode), res_scale, */
+/* adjoint.  out: forward seismograms (may be NULL); rms: [rms, rmsnorm].   */
+int seis_gradient_shot(seis_model *m, int nsrc, const float *src_pos,
+                       const float *src, int nrec, const float *rec_pos,
+                       const float *din, const float *res_in, float *out,
+                       double *rms);
 
 /* Record HIP events around every update_v / update_s launch of the next    */
 /* forward runs (costs a few microseconds per step).                          */

@@ -44,6 +44,7 @@ class SeisCsts(ctypes.Structure):
         ("tmin", ctypes.c_int),
         ("gradout", ctypes.c_int),
         ("FP16", ctypes.c_int),
+        ("gradfix", ctypes.c_int),
     ]
 
 
@@ -81,6 +82,15 @@ SIGNATURES = [
     ("seis_run_steps", _I, [_P, _I, _I]),
     ("seis_forward_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
     ("seis_read_field", _I, [_P, ctypes.c_char_p, _FP]),
+    ("seis_grad_zero", _I, [_P]),
+    ("seis_forward_save", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
+    ("seis_residuals", _I, [_P, _I, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double)]),
+    ("seis_res_scale", _I, [_P, _I, _FP, _FP]),
+    ("seis_adjoint", _I, [_P, _FP]),
+    ("seis_grad_read", _I, [_P, _FP]),
+    ("seis_gradient_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP, _FP, _FP,
+                                ctypes.POINTER(ctypes.c_double)]),
     ("seis_model_set_profiling", _I, [_P, _I]),
     ("seis_last_timing", _I, [_P, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_double),

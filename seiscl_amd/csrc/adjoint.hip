@@ -1,0 +1,509 @@
+// Adjoint step kernels of the gradient by backpropagation (elastic, 2D/3D).
+//
+// Each kernel does, for the cell its lane owns and in the reference's order
+// (src/update_adjs2D.cl, src/update_adjv2D.cl, src/update_adjs3D.cl,
+// src/update_adjv3D.cl):
+//   1. backpropagate the forward field: subtract this step's increment,
+//      computed without CPML from the other forward field;
+//   2. overwrite it with the saved boundary value when the cell lies on the
+//      boundary shell (inject_ind);
+//   3. update the adjoint field exactly as the forward kernels update theirs
+//      (same stencils, CPML on the adjoint run's own memory, taper);
+//   4. accumulate the gradient, in double, from the backpropagated forward
+//      field and the adjoint increment.
+// Every access of a lane is to its own cell, except stencil reads of the
+// other field, which no kernel writes, so the steps need no barriers.
+//
+// Also here: boundary saving (savebnd) and adjoint-source injection
+// (residuals, src/automatic_kernels.c:631-836).
+#include <hip/hip_runtime.h>
+
+#include "fd_common.hpp"
+#include "seis_internal.hpp"
+
+namespace seis {
+namespace {
+
+constexpr int ABY = 4;   // rows of y (3D) or x (2D) per block
+
+// CPML correction of one adjoint derivative on the near/far strip of an
+// axis, with the profile index of the reference kernels (full grid: k or
+// 2*NAB-k; half grid: k or 2*NAB-1-k).
+template <typename R>
+__device__ __forceinline__ void pml_full(const FDArgs<R> &a, int axis, R &d, R *psi, int ki) {
+    pml(d, psi, cp(a, axis, CB, ki), cp(a, axis, CA, ki), cp(a, axis, CK, ki));
+}
+template <typename R>
+__device__ __forceinline__ void pml_half(const FDArgs<R> &a, int axis, R &d, R *psi, int kh) {
+    pml(d, psi, cp(a, axis, CBH, kh), cp(a, axis, CAH, kh), cp(a, axis, CKH, kh));
+}
+
+// taper factor product of the cell (abs_type 2), applied factor by factor
+template <typename R>
+__device__ __forceinline__ void taper_apply(const FDArgs<R> &a, int z, int y, int x, R *v, int n) {
+    const R *t = a.taper;
+    const int NAB = a.nab, NZ = a.g.NZ, NY = a.g.NY, NX = a.g.NX;
+    if (!a.freesurf && z < NAB)
+        for (int q = 0; q < n; ++q) v[q] *= t[z];
+    if (z > NZ - NAB - 1)
+        for (int q = 0; q < n; ++q) v[q] *= t[NZ - z - 1];
+    if (a.g.nd == 3) {
+        if (y < NAB)
+            for (int q = 0; q < n; ++q) v[q] *= t[y];
+        if (y > NY - NAB - 1)
+            for (int q = 0; q < n; ++q) v[q] *= t[NY - y - 1];
+    }
+    if (x < NAB)
+        for (int q = 0; q < n; ++q) v[q] *= t[x];
+    if (x > NX - NAB - 1)
+        for (int q = 0; q < n; ++q) v[q] *= t[NX - x - 1];
+}
+
+// lane -> cell, z fastest; 2D uses y = 0
+__device__ __forceinline__ bool adj_cell(const Layout &g, int nbz, unsigned nb, int &z, int &y,
+                                         int &x) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return false;
+    const int bz = tile % nbz;
+    const unsigned r = tile / nbz;
+    z = bz * BZ + threadIdx.x;
+    if (g.nd == 3) {
+        const int nby = (g.NY + ABY - 1) / ABY;
+        y = (r % nby) * ABY + threadIdx.y;
+        x = r / nby;
+        return z < g.NZ && y < g.NY && x < g.NX;
+    }
+    y = 0;
+    x = r * ABY + threadIdx.y;
+    return z < g.NZ && x < g.NX;
+}
+
+// ---------------------------------------------------------------------------
+// 2D
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_adj_s2d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &f = A.f, &a = A.r;
+    const Layout &g = f.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = f.hc;
+    const R fipkp = f.muipkp[i], lmu = f.mu[i], lM = f.M[i];
+    const R ff = 2.0 * lmu, gg = lM;
+    // 1-2. backpropagate the forward stresses, re-inject the shell
+    R sxz = f.sxz[i], sxx = f.sxx[i], szz = f.szz[i];
+    {
+        const R vxx = Dm<H>(f.vx, i, sx, hc), vxz = Dp<H>(f.vx, i, 1, hc);
+        const R vzz = Dm<H>(f.vz, i, 1, hc), vzx = Dp<H>(f.vz, i, sx, hc);
+        sxz -= (fipkp * (vxz + vzx));
+        sxx -= (gg * (vxx + vzz)) - (ff * vzz);
+        szz -= (gg * (vxx + vzz)) - (ff * vxx);
+        const long m = A.sh.inject(z + H, 0, x + H);
+        if (m >= 0) {
+            const long n = A.sh.n;
+            sxx = A.bnd[2 * n + m];
+            szz = A.bnd[3 * n + m];
+            sxz = A.bnd[4 * n + m];
+        }
+        f.sxx[i] = sxx;
+        f.szz[i] = szz;
+        f.sxz[i] = sxz;
+    }
+    // 3. adjoint stress update
+    R vxx = Dm<H>(a.vx, i, sx, hc), vxz = Dp<H>(a.vx, i, 1, hc);
+    R vzz = Dm<H>(a.vz, i, 1, hc), vzx = Dp<H>(a.vz, i, sx, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const long ip = (long)x * 2 * NAB + k;
+            pml_half(a, 2, vxz, a.psi_vx_z + ip, bot ? 2 * NAB - 1 - k : k);
+            pml_full(a, 2, vzz, a.psi_vz_z + ip, bot ? 2 * NAB - k : k);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const long ip = (long)ii * NZ + z;
+            pml_full(a, 0, vxx, a.psi_vx_x + ip, x < NAB ? ii : 2 * NAB - ii);
+            pml_half(a, 0, vzx, a.psi_vz_x + ip, x < NAB ? ii : 2 * NAB - 1 - ii);
+        }
+    }
+    const R lsxz = (fipkp * (vxz + vzx));
+    const R lsxx = (gg * (vxx + vzz)) - (ff * vzz);
+    const R lszz = (gg * (vxx + vzz)) - (ff * vxx);
+    R s3[3] = {a.sxx[i] + lsxx, a.szz[i] + lszz, a.sxz[i] + lsxz};
+    if (a.abs_type == 2) taper_apply(a, z, 0, x, s3, 3);
+    a.sxx[i] = s3[0];
+    a.szz[i] = s3[1];
+    a.sxz[i] = s3[2];
+    // 4. M and mu gradients (src/update_adjs2D.cl:441-451)
+    const R c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
+    const R c3 = 1.0 / (lmu * lmu);
+    const R c5 = 0.25 * c3;
+    const R dM = c1 * (sxx + szz) * (lsxx + lszz);
+    A.gM[i] += -dM;
+    A.gmu[i] += -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+}
+
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_adj_v2d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &f = A.f, &a = A.r;
+    const Layout &g = f.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = f.hc;
+    const R rip = f.rip[i], rkp = f.rkp[i];
+    R vx = f.vx[i], vz = f.vz[i];
+    {
+        const R sxx_x = Dp<H>(f.sxx, i, sx, hc), szz_z = Dp<H>(f.szz, i, 1, hc);
+        const R sxz_z = Dm<H>(f.sxz, i, 1, hc), sxz_x = Dm<H>(f.sxz, i, sx, hc);
+        vx -= ((sxx_x + sxz_z) * rip);
+        vz -= ((szz_z + sxz_x) * rkp);
+        const long m = A.sh.inject(z + H, 0, x + H);
+        if (m >= 0) {
+            vx = A.bnd[m];
+            vz = A.bnd[A.sh.n + m];
+        }
+        f.vx[i] = vx;
+        f.vz[i] = vz;
+    }
+    R sxx_x = Dp<H>(a.sxx, i, sx, hc), szz_z = Dp<H>(a.szz, i, 1, hc);
+    R sxz_z = Dm<H>(a.sxz, i, 1, hc), sxz_x = Dm<H>(a.sxz, i, sx, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const long ip = (long)x * 2 * NAB + k;
+            pml_full(a, 2, sxz_z, a.psi_sxz_z + ip, bot ? 2 * NAB - k : k);
+            pml_half(a, 2, szz_z, a.psi_szz_z + ip, bot ? 2 * NAB - 1 - k : k);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const long ip = (long)ii * NZ + z;
+            pml_half(a, 0, sxx_x, a.psi_sxx_x + ip, x < NAB ? ii : 2 * NAB - 1 - ii);
+            pml_full(a, 0, sxz_x, a.psi_sxz_x + ip, x < NAB ? ii : 2 * NAB - ii);
+        }
+    }
+    const R lvx = ((sxx_x + sxz_z) * rip);
+    const R lvz = ((szz_z + sxz_x) * rkp);
+    R v2[2] = {a.vx[i] + lvx, a.vz[i] + lvz};
+    if (a.abs_type == 2) taper_apply(a, z, 0, x, v2, 2);
+    a.vx[i] = v2[0];
+    a.vz[i] = v2[1];
+    // density gradient (src/update_adjv2D.cl:326)
+    const R d = -vx * lvx - vz * lvz;
+    A.grho[i] += d;
+}
+
+// ---------------------------------------------------------------------------
+// 3D
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_adj_s3d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &f = A.f, &a = A.r;
+    const Layout &g = f.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
+    const R *hc = f.hc;
+    const R fipjp = f.muipjp[i], fjpkp = f.mujpkp[i], fipkp = f.muipkp[i];
+    const R lM = f.M[i], lmu = f.mu[i];
+    const R gg = lM, ff = 2.0 * lmu;
+    R sxx = f.sxx[i], syy = f.syy[i], szz = f.szz[i];
+    R sxy = f.sxy[i], syz = f.syz[i], sxz = f.sxz[i];
+    {
+        const R vxx = Dm<H>(f.vx, i, sx, hc), vxy = Dp<H>(f.vx, i, sy, hc), vxz = Dp<H>(f.vx, i, 1, hc);
+        const R vyx = Dp<H>(f.vy, i, sx, hc), vyy = Dm<H>(f.vy, i, sy, hc), vyz = Dp<H>(f.vy, i, 1, hc);
+        const R vzx = Dp<H>(f.vz, i, sx, hc), vzy = Dp<H>(f.vz, i, sy, hc), vzz = Dm<H>(f.vz, i, 1, hc);
+        const R vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+        const R vxxyyzz = vxx + vyy + vzz, vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
+        sxy -= (fipjp * vxyyx);
+        syz -= (fjpkp * vyzzy);
+        sxz -= (fipkp * vxzzx);
+        sxx -= ((gg * vxxyyzz) - (ff * vyyzz));
+        syy -= ((gg * vxxyyzz) - (ff * vxxzz));
+        szz -= ((gg * vxxyyzz) - (ff * vxxyy));
+        const long m = A.sh.inject(z + H, y + H, x + H);
+        if (m >= 0) {
+            const long n = A.sh.n;
+            sxx = A.bnd[3 * n + m];
+            syy = A.bnd[4 * n + m];
+            szz = A.bnd[5 * n + m];
+            sxy = A.bnd[6 * n + m];
+            syz = A.bnd[7 * n + m];
+            sxz = A.bnd[8 * n + m];
+        }
+        f.sxx[i] = sxx;
+        f.syy[i] = syy;
+        f.szz[i] = szz;
+        f.sxy[i] = sxy;
+        f.syz[i] = syz;
+        f.sxz[i] = sxz;
+    }
+    R vxx = Dm<H>(a.vx, i, sx, hc), vxy = Dp<H>(a.vx, i, sy, hc), vxz = Dp<H>(a.vx, i, 1, hc);
+    R vyx = Dp<H>(a.vy, i, sx, hc), vyy = Dm<H>(a.vy, i, sy, hc), vyz = Dp<H>(a.vy, i, 1, hc);
+    R vzx = Dp<H>(a.vz, i, sx, hc), vzy = Dp<H>(a.vz, i, sy, hc), vzz = Dm<H>(a.vz, i, 1, hc);
+    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int ki = bot ? 2 * NAB - k : k, kh = bot ? 2 * NAB - 1 - k : k;
+            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+            pml_half(a, 2, vxz, a.psi_vx_z + ip, kh);
+            pml_half(a, 2, vyz, a.psi_vy_z + ip, kh);
+            pml_full(a, 2, vzz, a.psi_vz_z + ip, ki);
+        }
+        if (y < NAB || y > NY - NAB - 1) {
+            const int j = y < NAB ? y : y - NY + 2 * NAB;
+            const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
+            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+            pml_half(a, 1, vxy, a.psi_vx_y + ip, jh);
+            pml_full(a, 1, vyy, a.psi_vy_y + ip, ji);
+            pml_half(a, 1, vzy, a.psi_vz_y + ip, jh);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
+            pml_full(a, 0, vxx, a.psi_vx_x + ip, iI);
+            pml_half(a, 0, vyx, a.psi_vy_x + ip, iH);
+            pml_half(a, 0, vzx, a.psi_vz_x + ip, iH);
+        }
+    }
+    const R vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+    const R vxxyyzz = vxx + vyy + vzz, vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
+    const R lsxy = (fipjp * vxyyx), lsyz = (fjpkp * vyzzy), lsxz = (fipkp * vxzzx);
+    const R lsxx = ((gg * vxxyyzz) - (ff * vyyzz));
+    const R lsyy = ((gg * vxxyyzz) - (ff * vxxzz));
+    const R lszz = ((gg * vxxyyzz) - (ff * vxxyy));
+    R s6[6] = {a.sxy[i] + lsxy, a.syz[i] + lsyz, a.sxz[i] + lsxz,
+               a.sxx[i] + lsxx, a.syy[i] + lsyy, a.szz[i] + lszz};
+    if (a.abs_type == 2) taper_apply(a, z, y, x, s6, 6);
+    a.sxy[i] = s6[0];
+    a.syz[i] = s6[1];
+    a.sxz[i] = s6[2];
+    a.sxx[i] = s6[3];
+    a.syy[i] = s6[4];
+    a.szz[i] = s6[5];
+    // M and mu gradients (src/update_adjs3D.cl:679-689)
+    const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
+    const R c3 = 1.0 / lmu / lmu;
+    const R c5 = 1.0 / 6.0 * c3;
+    const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
+    A.gM[i] += -dM;
+    A.gmu[i] += -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
+                c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
+                      lszz * (2.0 * szz - sxx - syy));
+}
+
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &f = A.f, &a = A.r;
+    const Layout &g = f.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
+    const R *hc = f.hc;
+    const R rip = f.rip[i], rjp = f.rjp[i], rkp = f.rkp[i];
+    R vx = f.vx[i], vy = f.vy[i], vz = f.vz[i];
+    {
+        const R szz_z = Dp<H>(f.szz, i, 1, hc), sxx_x = Dp<H>(f.sxx, i, sx, hc);
+        const R sxz_x = Dm<H>(f.sxz, i, sx, hc), sxz_z = Dm<H>(f.sxz, i, 1, hc);
+        const R syz_y = Dm<H>(f.syz, i, sy, hc), syz_z = Dm<H>(f.syz, i, 1, hc);
+        const R syy_y = Dp<H>(f.syy, i, sy, hc), sxy_x = Dm<H>(f.sxy, i, sx, hc);
+        const R sxy_y = Dm<H>(f.sxy, i, sy, hc);
+        vx -= ((sxx_x + sxy_y + sxz_z) * rip);
+        vy -= ((syy_y + sxy_x + syz_z) * rjp);
+        vz -= ((szz_z + sxz_x + syz_y) * rkp);
+        const long m = A.sh.inject(z + H, y + H, x + H);
+        if (m >= 0) {
+            const long n = A.sh.n;
+            vx = A.bnd[m];
+            vy = A.bnd[n + m];
+            vz = A.bnd[2 * n + m];
+        }
+        f.vx[i] = vx;
+        f.vy[i] = vy;
+        f.vz[i] = vz;
+    }
+    R szz_z = Dp<H>(a.szz, i, 1, hc), sxx_x = Dp<H>(a.sxx, i, sx, hc);
+    R sxz_x = Dm<H>(a.sxz, i, sx, hc), sxz_z = Dm<H>(a.sxz, i, 1, hc);
+    R syz_y = Dm<H>(a.syz, i, sy, hc), syz_z = Dm<H>(a.syz, i, 1, hc);
+    R syy_y = Dp<H>(a.syy, i, sy, hc), sxy_x = Dm<H>(a.sxy, i, sx, hc);
+    R sxy_y = Dm<H>(a.sxy, i, sy, hc);
+    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int ki = bot ? 2 * NAB - k : k, kh = bot ? 2 * NAB - 1 - k : k;
+            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+            pml_full(a, 2, sxz_z, a.psi_sxz_z + ip, ki);
+            pml_full(a, 2, syz_z, a.psi_syz_z + ip, ki);
+            pml_half(a, 2, szz_z, a.psi_szz_z + ip, kh);
+        }
+        if (y < NAB || y > NY - NAB - 1) {
+            const int j = y < NAB ? y : y - NY + 2 * NAB;
+            const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
+            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+            pml_full(a, 1, sxy_y, a.psi_sxy_y + ip, ji);
+            pml_half(a, 1, syy_y, a.psi_syy_y + ip, jh);
+            pml_full(a, 1, syz_y, a.psi_syz_y + ip, ji);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
+            pml_half(a, 0, sxx_x, a.psi_sxx_x + ip, iH);
+            pml_full(a, 0, sxy_x, a.psi_sxy_x + ip, iI);
+            pml_full(a, 0, sxz_x, a.psi_sxz_x + ip, iI);
+        }
+    }
+    const R lvx = ((sxx_x + sxy_y + sxz_z) * rip);
+    const R lvy = ((syy_y + sxy_x + syz_z) * rjp);
+    const R lvz = ((szz_z + sxz_x + syz_y) * rkp);
+    R v3[3] = {a.vx[i] + lvx, a.vy[i] + lvy, a.vz[i] + lvz};
+    if (a.abs_type == 2) taper_apply(a, z, y, x, v3, 3);
+    a.vx[i] = v3[0];
+    a.vy[i] = v3[1];
+    a.vz[i] = v3[2];
+    // density gradient (src/update_adjv3D.cl:481)
+    const R d = -vx * lvx - vy * lvy - vz * lvz;
+    A.grho[i] += d;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary saving and adjoint sources
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_savebnd(FDArgs<R> f, Shell sh, R *bnd) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= sh.n) return;
+    int k, j, i;
+    sh.cell(gid, k, j, i);
+    const int H = f.g.H;
+    const long c = f.g.idx(k - H, f.g.nd == 3 ? j - H : 0, i - H);
+    const long n = sh.n;
+    if (f.g.nd == 3) {
+        const R *v[9] = {f.vx, f.vy, f.vz, f.sxx, f.syy, f.szz, f.sxy, f.syz, f.sxz};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) bnd[q * n + gid] = v[q][c];
+    } else {
+        const R *v[5] = {f.vx, f.vz, f.sxx, f.szz, f.sxz};
+#pragma unroll
+        for (int q = 0; q < 5; ++q) bnd[q * n + gid] = v[q][c];
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void add_residual(const FDArgs<R> &a, int nd_case, int code, int cell,
+                                             float v) {
+    switch (code) {
+        case O_VX: a.vx[cell] += v; break;
+        case O_VY: a.vy[cell] += v; break;
+        case O_VZ: a.vz[cell] += v; break;
+        case O_SXX: a.sxx[cell] += v; break;
+        case O_SYY: a.syy[cell] += v; break;
+        case O_SZZ: a.szz[cell] += v; break;
+        case O_SXZ: a.sxz[cell] += v; break;
+        case O_SXY: a.sxy[cell] += v; break;
+        case O_SYZ: a.syz[cell] += v; break;
+        default: {
+            // p spreads over the normal stresses ("out/%d", an int literal)
+            const R q = v / (nd_case == 3 ? 3 : 2);
+            a.sxx[cell] += q;
+            a.szz[cell] += q;
+            if (nd_case == 3) a.syy[cell] += q;
+        }
+    }
+}
+
+struct CodeList {
+    int n;
+    int code[10];
+};
+
+// serial: one lane walks every receiver in order (receivers may share cells)
+template <typename R>
+__global__ void k_residuals(FDArgs<R> a, int nd_case, ShotDev s, CodeList o, const float *res,
+                            int NT, int nt, int serial) {
+    const int r0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r1 = serial ? s.nrec : r0 + 1;
+    if (serial && r0 != 0) return;
+    for (int r = r0; r < r1 && r < s.nrec; ++r) {
+        const int cell = s.rec_cell[r];
+        if (cell < 0) continue;
+        for (int q = 0; q < o.n; ++q)
+            add_residual(a, nd_case, o.code[q], cell, res[((long)q * s.nrec + r) * NT + nt]);
+    }
+}
+
+template <typename R>
+void grid_adj(const Layout &g, int &nbz, unsigned &nb) {
+    nbz = (g.NZ + BZ - 1) / BZ;
+    if (g.nd == 3)
+        nb = (unsigned)nbz * ((g.NY + ABY - 1) / ABY) * g.NX;
+    else
+        nb = (unsigned)nbz * ((g.NX + ABY - 1) / ABY);
+}
+
+template <typename R, int H>
+void run_adj_s(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
+    int nbz;
+    unsigned nb;
+    grid_adj<R>(a.f.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, ABY);
+    if (nd_case == 3) hipLaunchKernelGGL((k_adj_s3d<R, H>), grid, blk, 0, st, a, nbz, nb);
+    else hipLaunchKernelGGL((k_adj_s2d<R, H>), grid, blk, 0, st, a, nbz, nb);
+}
+template <typename R, int H>
+void run_adj_v(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
+    int nbz;
+    unsigned nb;
+    grid_adj<R>(a.f.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, ABY);
+    if (nd_case == 3) hipLaunchKernelGGL((k_adj_v3d<R, H>), grid, blk, 0, st, a, nbz, nb);
+    else hipLaunchKernelGGL((k_adj_v2d<R, H>), grid, blk, 0, st, a, nbz, nb);
+}
+
+}  // namespace
+
+template <typename R>
+void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream) {
+    SEIS_SWITCH_H(a.f.g.H, (run_adj_s<R, HH>(a, nd_case, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream) {
+    SEIS_SWITCH_H(a.f.g.H, (run_adj_v<R, HH>(a, nd_case, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream) {
+    (void)nd_case;
+    if (sh.n <= 0) return;
+    const unsigned blocks = (unsigned)((sh.n + 255) / 256);
+    hipLaunchKernelGGL((k_savebnd<R>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, f, sh, bnd);
+}
+template <typename R>
+void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const int *codes,
+                      int nout, const float *res, int NT, int nt, int serial, void *stream) {
+    if (s.nrec <= 0) return;
+    CodeList o;
+    o.n = nout;
+    for (int q = 0; q < nout && q < 10; ++q) o.code[q] = codes[q];
+    const unsigned blocks = serial ? 1u : (unsigned)((s.nrec + 255) / 256);
+    hipLaunchKernelGGL((k_residuals<R>), dim3(blocks), dim3(serial ? 64 : 256), 0,
+                       (hipStream_t)stream, r, nd_case, s, o, res, NT, nt, serial);
+}
+
+template void launch_adj_s<float>(const AdjArgs<float> &, int, void *);
+template void launch_adj_v<float>(const AdjArgs<float> &, int, void *);
+template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *);
+template void launch_residuals<float>(const FDArgs<float> &, int, const ShotDev &, const int *,
+                                      int, const float *, int, int, int, void *);
+
+}  // namespace seis

@@ -79,6 +79,23 @@ struct seis_model {
     bool profiling = false;
     std::vector<hipEvent_t> evpool;
     double ms_v = 0, ms_s = 0, ms_loop = 0;
+    // host copy of the transformed parameters (residual scaling, gradients)
+    RefParams rp;
+    std::vector<int> rcell_host;     // receiver cells of the bound shot
+    // gradient state, allocated on first use
+    struct Grad {
+        bool ready = false;
+        Shell sh;
+        int nvb = 0;
+        float *bnd = nullptr;       // [tmax][nvb][n]
+        long bnd_steps = 0;
+        FDArgs<float> r{};          // adjoint wavefields (same model)
+        double *gM = nullptr, *gmu = nullptr, *grho = nullptr;   // padded layout
+        float *res = nullptr;       // [nout][nrec][NT] scaled adjoint sources
+        size_t res_cap = 0;
+        std::vector<float *> own_psi;
+        bool saved = false;         // a forward run with boundaries is bound
+    } grad;
 
     float *dalloc(long n) {
         void *p = nullptr;
@@ -90,7 +107,8 @@ struct seis_model {
     }
     ~seis_model() {
         for (void *p : owned) ctx->release(p);
-        void *sp[] = {shot.src_cell, shot.src_type, shot.src_k, shot.src, shot.rec_cell, shot.out};
+        void *sp[] = {shot.src_cell, shot.src_type, shot.src_k, shot.src, shot.rec_cell, shot.out,
+                      grad.bnd, grad.res};
         for (void *p : sp) ctx->release(p);
         for (auto e : evpool) (void)hipEventDestroy(e);
     }
@@ -292,6 +310,102 @@ hipEvent_t pool_event(seis_model *m, size_t k) {
     return m->evpool[k];
 }
 
+// Gradient state: adjoint wavefields, their CPML memory (or the forward
+// run's, as the reference does, without FIX_ADJ_CPML), double accumulators.
+int grad_setup(seis_model *m) {
+    auto &G = m->grad;
+    if (G.ready) return 0;
+    const seis_csts &c = m->c;
+    if (c.ND == 22 || c.L > 0)
+        return fail("gradients need the elastic case (ND 2 or 3, L=0)");
+    if (c.freesurf) return fail("gradients with a free surface are not available in this build");
+    G.sh = make_shell(m->g, c.abs_type > 0 ? c.nab : 0, c.freesurf);
+    if (G.sh.NXbnd < 2 * G.sh.F || G.sh.NZbnd < 2 * G.sh.F ||
+        (c.ND == 3 && G.sh.NYbnd < 2 * G.sh.F))
+        return fail("the grid inside the absorbing layers is too small for the boundary shell");
+    G.nvb = c.ND == 3 ? 9 : 5;
+    FDArgs<float> &r = G.r;
+    r = m->a;
+    const long nf = m->g.nfield;
+    float **fl[] = {&r.vx, &r.vz, &r.sxx, &r.szz, &r.sxz, &r.vy, &r.syy, &r.sxy, &r.syz};
+    const int nfl = c.ND == 3 ? 9 : 5;
+    for (int q = 0; q < nfl; ++q)
+        if (!(*fl[q] = m->dalloc(nf))) return 1;
+    if (c.abs_type == 1 && (c.gradfix & FIX_ADJ_CPML)) {
+        const long nab2 = 2L * c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
+        const long sxs = nab2 * NY * NZ, sys = NX * nab2 * NZ, szs = NX * NY * nab2;
+        float **px[] = {&r.psi_sxx_x, &r.psi_sxy_x, &r.psi_sxz_x, &r.psi_vx_x, &r.psi_vy_x,
+                        &r.psi_vz_x};
+        float **py[] = {&r.psi_sxy_y, &r.psi_syy_y, &r.psi_syz_y, &r.psi_vx_y, &r.psi_vy_y,
+                        &r.psi_vz_y};
+        float **pz[] = {&r.psi_sxz_z, &r.psi_syz_z, &r.psi_szz_z, &r.psi_vx_z, &r.psi_vy_z,
+                        &r.psi_vz_z};
+        for (float **p : px)
+            if (*p && !(*p = m->dalloc(sxs))) return 1;
+        for (float **p : py)
+            if (*p && !(*p = m->dalloc(sys))) return 1;
+        for (float **p : pz)
+            if (*p && !(*p = m->dalloc(szs))) return 1;
+    }
+    double **gd[] = {&G.gM, &G.gmu, &G.grho};
+    for (double **p : gd)
+        if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
+    G.ready = true;
+    return 0;
+}
+
+// zero the adjoint wavefields and the adjoint run's own CPML memory
+void grad_zero_adjoint(seis_model *m) {
+    const FDArgs<float> &r = m->grad.r, &a = m->a;
+    const size_t nf = (size_t)m->g.nfield * 4;
+    float *fl[] = {r.vx, r.vy, r.vz, r.sxx, r.syy, r.szz, r.sxy, r.syz, r.sxz};
+    for (float *p : fl) launch_fill(p, nf, m->ctx->stream);
+    if (m->c.abs_type == 1 && (m->c.gradfix & FIX_ADJ_CPML)) {
+        const long nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
+        const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
+        float *px[] = {r.psi_sxx_x, r.psi_sxy_x, r.psi_sxz_x, r.psi_vx_x, r.psi_vy_x, r.psi_vz_x};
+        float *py[] = {r.psi_sxy_y, r.psi_syy_y, r.psi_syz_y, r.psi_vx_y, r.psi_vy_y, r.psi_vz_y};
+        float *pz[] = {r.psi_sxz_z, r.psi_syz_z, r.psi_szz_z, r.psi_vx_z, r.psi_vy_z, r.psi_vz_z};
+        for (float *p : px)
+            if (p) launch_fill(p, sx, m->ctx->stream);
+        for (float *p : py)
+            if (p) launch_fill(p, sy, m->ctx->stream);
+        for (float *p : pz)
+            if (p) launch_fill(p, sz, m->ctx->stream);
+    }
+    (void)a;
+}
+
+// The model's current arguments with the adjoint run's wavefields and CPML
+// memory swapped in (parameters may have changed since grad_setup).
+FDArgs<float> adjoint_args(const seis_model *m) {
+    FDArgs<float> r = m->a;
+    const FDArgs<float> &q = m->grad.r;
+    r.vx = q.vx; r.vy = q.vy; r.vz = q.vz;
+    r.sxx = q.sxx; r.syy = q.syy; r.szz = q.szz;
+    r.sxy = q.sxy; r.syz = q.syz; r.sxz = q.sxz;
+    r.psi_sxx_x = q.psi_sxx_x; r.psi_sxy_x = q.psi_sxy_x; r.psi_sxz_x = q.psi_sxz_x;
+    r.psi_sxy_y = q.psi_sxy_y; r.psi_syy_y = q.psi_syy_y; r.psi_syz_y = q.psi_syz_y;
+    r.psi_sxz_z = q.psi_sxz_z; r.psi_syz_z = q.psi_syz_z; r.psi_szz_z = q.psi_szz_z;
+    r.psi_vx_x = q.psi_vx_x; r.psi_vy_x = q.psi_vy_x; r.psi_vz_x = q.psi_vz_x;
+    r.psi_vx_y = q.psi_vx_y; r.psi_vy_y = q.psi_vy_y; r.psi_vz_y = q.psi_vz_y;
+    r.psi_vx_z = q.psi_vx_z; r.psi_vy_z = q.psi_vy_z; r.psi_vz_z = q.psi_vz_z;
+    return r;
+}
+
+int tmax_of(const seis_model *m) {
+    return m->c.tmax > 0 ? std::min(m->c.tmax, m->c.NT) : m->c.NT;
+}
+
+// receivers that share a cell must add their adjoint sources in order
+int receivers_share_cells(const seis_model *m, const std::vector<int> &cells) {
+    std::vector<int> v;
+    for (int c : cells)
+        if (c >= 0) v.push_back(c);
+    std::sort(v.begin(), v.end());
+    return std::adjacent_find(v.begin(), v.end()) != v.end();
+}
+
 }  // namespace
 
 extern "C" {
@@ -318,6 +432,7 @@ void seis_csts_default(seis_csts *c) {
     c->f0 = 15;
     c->FL[0] = 15;
     c->seisout = 2;
+    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE;
 }
 
 int seis_ctx_create(int device_ordinal, seis_ctx **out) {
@@ -437,6 +552,7 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
         a.taper = d;
     }
     if (alloc_fields(m.get())) return 1;
+    m->rp = std::move(rp);
     m->outn = out_names(c);
     for (auto &n : m->outn) m->out_codes.push_back(out_code(n));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -477,7 +593,9 @@ int seis_model_set_params(seis_model *m, const float *p0, const float *p1, const
         m->ctx->release((void *)*p);
         *p = nullptr;
     }
-    return upload_params(m, rp);
+    if (upload_params(m, rp)) return 1;
+    m->rp = std::move(rp);
+    return 0;
 }
 
 size_t seis_model_footprint(const seis_model *m) { return m ? m->bytes : 0; }
@@ -553,7 +671,9 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
     }
     if (nrec) HIP_TRY(hipMemcpyAsync(sd.rec_cell, rcell.data(), nrec * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
+    m->rcell_host = rcell;
     m->shot_bound = true;
+    m->grad.saved = false;
     return 0;
 }
 
@@ -684,6 +804,161 @@ int seis_read_field(seis_model *m, const char *name, float *host) {
             for (long z = 0; z < NZp; ++z)
                 host[(x * NYp + y) * NZp + z] =
                     tmp[g.idx((int)z - H, g.nd == 3 ? (int)y - H : 0, (int)x - H)];
+    return 0;
+}
+
+// ---- gradient by backpropagation ------------------------------------------
+
+int seis_grad_zero(seis_model *m) {
+    g_err.clear();
+    if (grad_setup(m)) return 1;
+    const size_t nb = (size_t)m->g.nfield * 8;
+    launch_fill(m->grad.gM, nb, m->ctx->stream);
+    launch_fill(m->grad.gmu, nb, m->ctx->stream);
+    launch_fill(m->grad.grho, nb, m->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return 0;
+}
+
+int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
+                      const float *rec_pos, float *out) {
+    g_err.clear();
+    if (grad_setup(m)) return 1;
+    auto &G = m->grad;
+    const int tmax = tmax_of(m);
+    const long need = (long)tmax;
+    if (G.bnd_steps < need) {
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        if (G.bnd) {
+            m->bytes -= (size_t)G.bnd_steps * G.nvb * G.sh.n * 4;
+            m->ctx->release(G.bnd);
+            G.bnd = nullptr;
+            G.bnd_steps = 0;
+        }
+        void *p = nullptr;
+        const size_t bytes = (size_t)need * G.nvb * G.sh.n * 4;
+        if (m->ctx->alloc(bytes, &p)) return 1;
+        G.bnd = (float *)p;
+        G.bnd_steps = need;
+        m->bytes += bytes;
+    }
+    if (seis_shot_bind(m, nsrc, src_pos, src, nrec, rec_pos)) return 1;
+    if (seis_launch_init(m)) return 1;
+    hipStream_t st = m->ctx->stream;
+    const size_t per = (size_t)G.nvb * G.sh.n;
+    // forward step order with boundary saving after update_s
+    // (src/time_stepping.c:785-816)
+    for (int t = 0; t < tmax; ++t) {
+        launch_sources(m->a, m->nd_case, m->shot, m->c.NT, t, 1, st);
+        launch_update_v(m->a, m->nd_case, st);
+        launch_update_s(m->a, m->nd_case, st);
+        launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (size_t)t * per, st);
+        launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(),
+                       t, st);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    G.saved = true;
+    return out ? seis_shot_read(m, out) : 0;
+}
+
+int seis_residuals(const seis_model *m, int nrec, const float *out, const float *din, float *res,
+                   double *rms, double *rmsnorm) {
+    const double r = residuals_l2(m->c, (int)m->outn.size(), nrec, out, din, res, rmsnorm);
+    if (rms) *rms = r;
+    return 0;
+}
+
+int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos, float *res) {
+    res_scale(m->c, m->rp, m->outn, nrec, rec_pos, res, m->c.gradfix & FIX_RES_SCALE);
+    return 0;
+}
+
+int seis_adjoint(seis_model *m, const float *res) {
+    g_err.clear();
+    auto &G = m->grad;
+    if (!G.saved) return fail("seis_adjoint: no forward run with saved boundaries is bound");
+    const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
+    const int tmax = tmax_of(m), tmin = m->c.tmin;
+    if (tmin < 0 || tmin >= tmax) return fail("tmin outside [0, tmax)");
+    hipStream_t st = m->ctx->stream;
+    const size_t nres = (size_t)nout * nrec * NT;
+    if (nres > G.res_cap) {
+        HIP_TRY(hipStreamSynchronize(st));
+        m->ctx->release(G.res);
+        G.res = nullptr;
+        void *p = nullptr;
+        if (m->ctx->alloc(nres * 4, &p)) return 1;
+        G.res = (float *)p;
+        G.res_cap = nres;
+    }
+    if (nres) HIP_TRY(hipMemcpyAsync(G.res, res, nres * 4, hipMemcpyHostToDevice, st));
+    grad_zero_adjoint(m);
+    AdjArgs<float> A;
+    A.f = m->a;
+    A.r = adjoint_args(m);
+    A.gM = G.gM;
+    A.gmu = G.gmu;
+    A.grho = G.grho;
+    A.sh = G.sh;
+    const int serial = receivers_share_cells(m, m->rcell_host);
+    const size_t per = (size_t)G.nvb * G.sh.n;
+    // adjoint loop of src/time_stepping.c:869-940
+    for (int t = tmax - 1; t > tmin; --t) {
+        launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t, serial,
+                         st);
+        A.bnd = G.bnd + (size_t)(t - 1) * per;
+        launch_adj_s(A, m->nd_case, st);
+        launch_adj_v(A, m->nd_case, st);
+        launch_sources(m->a, m->nd_case, m->shot, NT, t, -1, st);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    G.saved = false;   // the forward fields are now at the start of the run
+    return 0;
+}
+
+int seis_grad_read(seis_model *m, float *grad) {
+    g_err.clear();
+    auto &G = m->grad;
+    if (!G.ready) return fail("seis_grad_read: no gradient accumulated");
+    const Layout &g = m->g;
+    const size_t nf = (size_t)g.nfield, n = (size_t)g.NZ * g.NY * g.NX;
+    std::vector<double> tmp(nf), acc[3];
+    double *src[3] = {G.gM, G.gmu, G.grho};
+    for (int q = 0; q < 3; ++q) {
+        HIP_TRY(hipMemcpyAsync(tmp.data(), src[q], nf * 8, hipMemcpyDeviceToHost, m->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        acc[q].resize(n);
+        for (int x = 0; x < g.NX; ++x)
+            for (int y = 0; y < g.NY; ++y)
+                for (int z = 0; z < g.NZ; ++z)
+                    acc[q][((size_t)x * g.NY + y) * g.NZ + z] = tmp[g.idx(z, y, x)];
+    }
+    transf_grad(m->c, m->rp, acc[0].data(), acc[1].data(), acc[2].data(), grad,
+                m->c.gradfix & FIX_RHO_SCALE);
+    return 0;
+}
+
+int seis_gradient_shot(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
+                       const float *rec_pos, const float *din, const float *res_in, float *out,
+                       double *rms) {
+    g_err.clear();
+    if (!din && !res_in) return fail("seis_gradient_shot: give observed data or residuals");
+    const int NT = m->c.NT, nout = (int)m->outn.size();
+    std::vector<float> o((size_t)nout * nrec * NT), res(o.size(), 0.f);
+    if (seis_forward_save(m, nsrc, src_pos, src, nrec, rec_pos, o.data())) return 1;
+    double rr[2] = {0, 0};
+    if (res_in) std::copy(res_in, res_in + res.size(), res.begin());
+    else rr[0] = residuals_l2(m->c, nout, nrec, o.data(), din, res.data(), &rr[1]);
+    res_scale(m->c, m->rp, m->outn, nrec, rec_pos, res.data(), m->c.gradfix & FIX_RES_SCALE);
+    if (seis_adjoint(m, res.data())) return 1;
+    if (out) std::copy(o.begin(), o.end(), out);
+    if (rms) {
+        rms[0] = rr[0];
+        rms[1] = rr[1];
+    }
     return 0;
 }
 

@@ -136,4 +136,144 @@ void launch_s3d(const FDArgs<R> &a, void *stream);
 // Output codes used by the receiver kernel.
 enum OutCode { O_VX, O_VY, O_VZ, O_SXX, O_SYY, O_SZZ, O_SXZ, O_SXY, O_SYZ, O_P };
 
+// ---------------------------------------------------------------------------
+// Gradient by backpropagation (BACK_PROP_TYPE=1).
+// ---------------------------------------------------------------------------
+// Corrections of reference defects, selected by seis_csts.gradfix.
+enum GradFix { FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4 };
+
+// Boundary shell saved during the forward run and re-injected while the
+// forward wavefield runs backwards: FDOH cells thick, on the inner face of
+// the absorbing layer, enumerated front, back, [left, right,] down, up as
+// savebnd does (src/savebnd2D.cl:43-223, src/savebnd3D.cl:38-284); the
+// inverse lookup is inject_ind (src/header_injectbnd.cl:25-232, :238-398).
+// Coordinates (k, j, i) are the reference's padded ones (interior + FDOH).
+struct Shell {
+    int nd = 2, F = 0, NAB = 0;
+    int NXbnd = 0, NYbnd = 0, NZbnd = 0, lbnd = 0, lbnds = 0;
+    long n = 0;
+
+    SEIS_HD void cell(long gid, int &k, int &j, int &i) const {
+        const long NXb = NXbnd, NYb = NYbnd, NZb = NZbnd;
+        long g;
+        j = 0;
+        if (nd == 3) {
+            const long fb = NYb * NZb * F, lr = NZb * (NXb - 2 * F) * F,
+                       du = (NYb - 2 * F) * (NXb - 2 * F) * F;
+            if (gid < fb) {
+                i = (int)(gid / (NYb * NZb)) + lbnd;
+                j = (int)((gid / NZb) % NYb) + lbnd;
+                k = (int)(gid % NZb) + lbnds;
+            } else if (gid < 2 * fb) {
+                g = gid - fb;
+                i = (int)(g / (NYb * NZb) + NXb + NAB);
+                j = (int)((g / NZb) % NYb) + lbnd;
+                k = (int)(g % NZb) + lbnds;
+            } else if (gid < 2 * fb + lr) {
+                g = gid - 2 * fb;
+                i = (int)(g / (NZb * F)) + lbnd + F;
+                j = (int)((g / NZb) % F) + lbnd;
+                k = (int)(g % NZb) + lbnds;
+            } else if (gid < 2 * fb + 2 * lr) {
+                g = gid - 2 * fb - lr;
+                i = (int)(g / (NZb * F)) + lbnd + F;
+                j = (int)((g / NZb) % F + NYb + NAB);
+                k = (int)(g % NZb) + lbnds;
+            } else if (gid < 2 * fb + 2 * lr + du) {
+                g = gid - 2 * fb - 2 * lr;
+                i = (int)(g / (F * (NYb - 2 * F))) + lbnd + F;
+                j = (int)((g / F) % (NYb - 2 * F)) + lbnd + F;
+                k = (int)(g % F + NZb) + lbnds - F;
+            } else {
+                g = gid - 2 * fb - 2 * lr - du;
+                i = (int)(g / (F * (NYb - 2 * F))) + lbnd + F;
+                j = (int)((g / F) % (NYb - 2 * F)) + lbnd + F;
+                k = (int)(g % F) + lbnds;
+            }
+            return;
+        }
+        const long fb = NZb * F, du = (NXb - 2 * F) * F;
+        if (gid < fb) {
+            i = (int)(gid / NZb) + lbnd;
+            k = (int)(gid % NZb) + lbnds;
+        } else if (gid < 2 * fb) {
+            g = gid - fb;
+            i = (int)(g / NZb + NXb + NAB);
+            k = (int)(g % NZb) + lbnds;
+        } else if (gid < 2 * fb + du) {
+            g = gid - 2 * fb;
+            i = (int)(g % (NXb - 2 * F)) + lbnd + F;
+            k = (int)(g / (NXb - 2 * F) + NZb) + lbnds - F;
+        } else {
+            g = gid - 2 * fb - du;
+            i = (int)(g % (NXb - 2 * F)) + lbnd + F;
+            k = (int)(g / (NXb - 2 * F)) + lbnds;
+        }
+    }
+
+    // shell entry of padded cell (k, j, i), or -1
+    SEIS_HD long inject(int k, int j, int i) const {
+        const long NXb = NXbnd, NYb = NYbnd, NZb = NZbnd;
+        i -= lbnd;
+        k -= lbnds;
+        if (nd == 3) {
+            j -= lbnd;
+            if ((k > F - 1 && k < NZb - F) && (j > F - 1 && j < NYb - F) &&
+                (i > F - 1 && i < NXb - F))
+                return -1;
+            if (k < 0 || k > NZb - 1 || j < 0 || j > NYb - 1 || i < 0 || i > NXb - 1) return -1;
+            if (i < F) return (long)i * NYb * NZb + (long)j * NZb + k;
+            if (i > NXb - 1 - F) return NYb * NZb * F + (long)(i - NXb + F) * NYb * NZb + (long)j * NZb + k;
+            if (j < F) return NYb * NZb * F * 2 + (long)(i - F) * F * NZb + (long)j * NZb + k;
+            if (j > NYb - 1 - F)
+                return NYb * NZb * F * 2 + (NXb - 2 * F) * NZb * F + (long)(i - F) * F * NZb +
+                       (long)(j - NYb + F) * NZb + k;
+            if (k < F)
+                return NYb * NZb * F * 2 + (NXb - 2 * F) * NZb * F * 2 +
+                       (NXb - 2 * F) * (NYb - 2 * F) * F + (long)(i - F) * (NYb - 2 * F) * F +
+                       (long)(j - F) * F + k;
+            return NYb * NZb * F * 2 + (NXb - 2 * F) * NZb * F * 2 +
+                   (long)(i - F) * (NYb - 2 * F) * F + (long)(j - F) * F + (k - NZb + F);
+        }
+        if ((k > F - 1 && k < NZb - F) && (i > F - 1 && i < NXb - F)) return -1;
+        if (k < 0 || k > NZb - 1 || i < 0 || i > NXb - 1) return -1;
+        if (i < F) return (long)i * NZb + k;
+        if (i > NXb - 1 - F) return NZb * F + (long)(i - NXb + F) * NZb + k;
+        if (k < F) return NZb * F * 2 + (NXb - 2 * F) * F + (i - F) + (long)k * (NXb - 2 * F);
+        return NZb * F * 2 + (i - F) + (long)(k - NZb + F) * (NXb - 2 * F);
+    }
+};
+Shell make_shell(const Layout &g, int nab, int freesurf);
+
+// Arguments of the adjoint kernels: the forward run being backpropagated
+// (f), the adjoint run (r: its own wavefields and CPML memory, the same
+// model), the gradient accumulators and the boundary of the previous step.
+template <typename R>
+struct AdjArgs {
+    FDArgs<R> f, r;
+    double *gM, *gmu, *grho;     // padded layout, one double per cell
+    const R *bnd;                // [nvars][n] shell values of step t-1
+    Shell sh;
+};
+
+template <typename R>
+void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream);
+template <typename R>
+void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream);
+// bnd[q][g] = field_q(shell cell g), q over the shell variables
+template <typename R>
+void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream);
+// adjoint sources of step nt into the adjoint wavefields; res [nout][nrec][NT]
+template <typename R>
+void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const int *out_codes,
+                      int nout, const float *res, int NT, int nt, int serial, void *stream);
+
+// Host side of the gradient (src/residuals.c, src/calc_grad.c:975-1110).
+double residuals_l2(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
+                    float *res, double *rmsnorm);
+void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
+               int nrec, const float *rec_pos, float *res, int at_receiver);
+void transf_grad(const seis_csts &c, const RefParams &p, const double *gM, const double *gmu,
+                 const double *grho, float *out, int rho_fix);
+
 }  // namespace seis

@@ -22,7 +22,7 @@ CSTS_DEFAULTS = dict(
     ND=2, dh=10.0, dt=0.0008, NT=875, L=0, f0=15.0, FL=(15.0,), FDORDER=8,
     MAXRELERROR=1, FP16=0, freesurf=0, abs_type=1, VPPML=3500.0, NPOWER=2.0,
     FPML=15.0, K_MAX_CPML=2.0, nab=16, abpc=6.0, seisout=2, param_type=0,
-    tmax=0.0, tmin=0.0, gradout=0,
+    tmax=0.0, tmin=0.0, gradout=0, gradfix=7,
 )
 
 PARAM_NAMES = {0: ("vp", "vs", "rho"), 1: ("M", "mu", "rho"), 2: ("Ip", "Is", "rho")}
@@ -64,6 +64,7 @@ def make_csts(**kw) -> SeisCsts:
     c.tmin = int(np.float32(d["tmin"]) / np.float32(c.dt)) if d["tmin"] else 0
     c.gradout = int(d["gradout"])
     c.FP16 = int(d["FP16"])
+    c.gradfix = int(d["gradfix"])
     return c
 
 
@@ -205,6 +206,57 @@ class Model:
         flat = np.zeros(int(np.prod(shp)), dtype=np.float32)
         check(self.lib.seis_read_field(self.handle, name.encode(), fptr(flat)))
         return from_c_layout(flat, shp)
+
+    # ---- gradient by backpropagation --------------------------------------
+    def _pack(self, lst, nrec):
+        """list of (NT, nrec) arrays in out_names order -> [nout][nrec][NT]."""
+        a = np.stack([np.asarray(x, dtype=np.float32).T for x in lst])
+        return np.ascontiguousarray(a.reshape(len(self.out_names), nrec, self.csts.NT))
+
+    def grad_zero(self):
+        check(self.lib.seis_grad_zero(self.handle))
+
+    def forward_save(self, src_pos, src, rec_pos):
+        """Forward run that keeps the boundary shell of every step for the
+        adjoint run; returns the seismograms like forward()."""
+        self._shot = shot_arrays(src_pos, src, rec_pos)
+        sp, s, rp = self._shot
+        out = np.zeros((len(self.out_names), rp.shape[0], self.csts.NT), dtype=np.float32)
+        check(self.lib.seis_forward_save(self.handle, sp.shape[0], fptr(sp), fptr(s),
+                                         rp.shape[0], fptr(rp), fptr(out)))
+        return [np.ascontiguousarray(o.T) for o in out]
+
+    def adjoint(self, res, scale: bool = True):
+        """Adjoint run of the shot of the last forward_save with adjoint
+        sources res (list of (NT, nrec)); res_scale is applied when scale."""
+        rp = self._shot[2]
+        r = self._pack(res, rp.shape[0])
+        if scale:
+            check(self.lib.seis_res_scale(self.handle, rp.shape[0], fptr(rp), fptr(r)))
+        check(self.lib.seis_adjoint(self.handle, fptr(r)))
+
+    def grad_read(self) -> dict:
+        """Accumulated gradients {p0, p1, rho} in the model layout."""
+        n = int(np.prod(self.shape))
+        g = np.zeros(3 * n, dtype=np.float32)
+        check(self.lib.seis_grad_read(self.handle, fptr(g)))
+        names = PARAM_NAMES.get(self.csts.param_type, ("vp", "vs", "rho"))
+        return {nm: from_c_layout(g[q * n:(q + 1) * n], self.shape) for q, nm in enumerate(names)}
+
+    def gradient(self, src_pos, src, rec_pos, din=None, res=None):
+        """One shot of gradient (added to the accumulators): forward with
+        boundary saving, residuals against din (or adjoint sources res),
+        res_scale, adjoint.  Returns (seismograms, (rms, rmsnorm))."""
+        sp, s, rp = shot_arrays(src_pos, src, rec_pos)
+        self._shot = (sp, s, rp)
+        nrec = rp.shape[0]
+        d = self._pack(din, nrec) if din is not None else None
+        r = self._pack(res, nrec) if res is not None else None
+        out = np.zeros((len(self.out_names), nrec, self.csts.NT), dtype=np.float32)
+        rms = (ctypes.c_double * 2)()
+        check(self.lib.seis_gradient_shot(self.handle, sp.shape[0], fptr(sp), fptr(s), nrec,
+                                          fptr(rp), fptr(d), fptr(r), fptr(out), rms))
+        return [np.ascontiguousarray(o.T) for o in out], (rms[0], rms[1])
 
     def close(self):
         if getattr(self, "handle", None):

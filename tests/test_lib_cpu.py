@@ -62,3 +62,17 @@ def test_context_fails_cleanly_without_device():
     h = ctypes.c_void_p()
     assert lib.seis_ctx_create(0, ctypes.byref(h)) != 0
     assert lib.seis_last_error()
+
+
+def test_boundary_shell_maps_host(tmp_path):
+    """The product's boundary shell (save enumeration / inject lookup) is a
+    pair of inverse maps over in-grid cells, checked on the host."""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(here, "..", "seiscl_amd", "csrc")
+    exe = str(tmp_path / "shell_check")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-fopenmp", "-I", csrc, "-o", exe,
+                    os.path.join(here, "shell_check.cpp"), os.path.join(csrc, "grad_host.cpp"),
+                    os.path.join(csrc, "setup.cpp")], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
