@@ -76,3 +76,11 @@ def test_boundary_shell_maps_host(tmp_path):
                     os.path.join(csrc, "setup.cpp")], check=True)
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_autograd_module_needs_the_gpu():
+    """The autograd wrapper imports without a GPU and fails loudly on use."""
+    from seiscl_amd import Context
+    from seiscl_amd.autograd import Propagator, seismic  # noqa: F401
+    with pytest.raises(Exception):
+        Context(0)
