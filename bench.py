@@ -18,6 +18,9 @@ roofline.traffic: HBM bytes per stress-update launch from rocprofv3 counters
 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 512 MiB copy and fill).
 cpu_baseline: the C oracle on the host cores, same grid, a bounded number of
 steps.
+config.fp16_storage / config.fp64: the same shot run with FP16 wavefield and
+memory-variable storage (FP32 arithmetic) and in FP64; seismogram errors are
+relative L2 norms against the FP64 run (skip with --no-variants).
 """
 from __future__ import annotations
 
@@ -36,6 +39,7 @@ METRIC = "Gcell-updates/s & % HBM roofline, 3D elastic; shots/hr + p99 latency b
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_CELL_S = 196       # stress update, 3D visco L=2 FP32
 BYTES_PER_CELL_V = 60        # velocity update, 3D FP32
+BYTES_PER_CELL_FP16 = 154    # whole step, FP16 fields and memory, FP32 coefficients
 # HBM bytes per stress-update launch at 256^3, from the FETCH_SIZE / WRITE_SIZE
 # rocprofv3 passes summarised in this file (the command is written there)
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -79,6 +83,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid size per axis")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP16-storage and FP64 runs of the same shot")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 512 MiB device-to-device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -104,28 +110,60 @@ def main():
     log(f"building workload n={args.n} NT={W + K}")
     csts, params, shots = config2(n=args.n, NT=W + K)
     ctx = Context(0)
-    t0 = time.time()
-    m = Model(ctx, params, **csts)
-    log(f"model resident in {time.time() - t0:.1f} s, {m.footprint() / 2**30:.2f} GiB")
-    m.bind_shot(*shots[0])
-    m.run_steps(0, W)                       # warm-up, untimed
-    m.set_profiling(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    m.run_steps(W, K)                       # returns after the stream drained
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    tm = m.timing()
-    seis = m.read_shot()
-    if not all(np.isfinite(x).all() for x in seis):
-        raise RuntimeError("non-finite seismograms")
     ncell = int(np.prod(csts["N"]))
-    value = ncell * K / wall / 1e9
-    avg_s = tm["update_s_ms"] / K
-    avg_v = tm["update_v_ms"] / K
+
+    def run(fp16):
+        t0 = time.time()
+        m = Model(ctx, params, **dict(csts, FP16=fp16))
+        log(f"FP16={fp16}: model resident in {time.time() - t0:.1f} s, "
+            f"{m.footprint() / 2**30:.2f} GiB")
+        m.bind_shot(*shots[0])
+        m.run_steps(0, W)                       # warm-up, untimed
+        m.set_profiling(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        m.run_steps(W, K)                       # returns after the stream drained
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        tm = m.timing()
+        seis = m.read_shot()
+        m.close()
+        if not all(np.isfinite(x).all() for x in seis):
+            raise RuntimeError(f"non-finite seismograms (FP16={fp16})")
+        avg_s, avg_v = tm["update_s_ms"] / K, tm["update_v_ms"] / K
+        log(f"FP16={fp16}: timed {K} steps: wall {wall:.3f} s, loop(events) "
+            f"{tm['loop_ms'] / 1e3:.3f} s, update_s {avg_s:.3f} ms, update_v {avg_v:.3f} ms")
+        return dict(wall=wall, avg_s=avg_s, avg_v=avg_v, seis=seis,
+                    gcell=ncell * K / wall / 1e9)
+
+    main_run = run(0)
+    wall, avg_s, avg_v = main_run["wall"], main_run["avg_s"], main_run["avg_v"]
+    value = main_run["gcell"]
     achieved = BYTES_PER_CELL_S * ncell / (avg_s / 1e3) / 1e9
-    log(f"timed {K} steps: wall {wall:.3f} s, loop(events) {tm['loop_ms'] / 1e3:.3f} s, "
-        f"update_s {avg_s:.3f} ms, update_v {avg_v:.3f} ms per step")
+    variants = {}
+    if not args.no_variants:
+        # the same shot with FP16 wavefield storage and in FP64; seismogram
+        # error (relative L2 over all traces) against the FP64 run
+        r16, r64 = run(2), run(64)
+
+        def err(a, b):
+            a = np.concatenate([x.ravel() for x in a]).astype(np.float64)
+            b = np.concatenate([x.ravel() for x in b]).astype(np.float64)
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+        variants = {
+            "fp32_rel_l2_vs_fp64": err(main_run["seis"], r64["seis"]),
+            "fp16_storage": {"gcell_s": round(r16["gcell"], 4),
+                             "ms_per_step": round(r16["wall"] / K * 1e3, 5),
+                             "update_s_ms": round(r16["avg_s"], 5),
+                             "update_v_ms": round(r16["avg_v"], 5),
+                             "bytes_per_cell": BYTES_PER_CELL_FP16,
+                             "rel_l2_vs_fp64": err(r16["seis"], r64["seis"])},
+            "fp64": {"gcell_s": round(r64["gcell"], 4),
+                     "ms_per_step": round(r64["wall"] / K * 1e3, 5)},
+        }
+        log(f"accuracy vs FP64: FP32 {variants['fp32_rel_l2_vs_fp64']:.2e}, "
+            f"FP16 storage {variants['fp16_storage']['rel_l2_vs_fp64']:.2e}")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -146,6 +184,7 @@ def main():
             "fd_order": 4, "L": 2, "precision": "fp32",
             "update_v_ms": round(avg_v, 5), "update_s_ms": round(avg_s, 5),
             "update_v_gbs": round(BYTES_PER_CELL_V * ncell / (avg_v / 1e3) / 1e9, 1),
+            **variants,
         },
         "roofline": {
             "bound": "hbm", "kernel": "update_s (3D viscoelastic stress update)",
@@ -153,7 +192,6 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(args.n),
         },
     }
-    m.close()
     ctx.close()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

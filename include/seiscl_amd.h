@@ -66,6 +66,7 @@ typedef struct seis_csts {
 
 typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
 typedef struct seis_model seis_model;   /* one modeling case resident on a GPU   */
+typedef struct seis_batch seis_batch;   /* modeling jobs stepped together        */
 
 /* ---- errors ------------------------------------------------------------- */
 const char *seis_last_error(void);
@@ -176,6 +177,26 @@ int seis_gradient_shot(seis_model *m, int nsrc, const float *src_pos,
                        const float *src, int nrec, const float *rec_pos,
                        const float *din, const float *res_in, float *out,
                        double *rms);
+
+/* ---- batched serving (no counterpart in the reference, which runs one    */
+/* shot at a time, src/time_stepping.c:747): up to max_slots modeling jobs, */
+/* each a model with a bound shot, advance together, one launch per kernel  */
+/* class and time step for all of them; a job's results are identical      */
+/* whether it runs alone or in a batch.  2D, FP32, L=0, no free surface.     */
+int seis_batch_create(seis_ctx *ctx, int max_slots, seis_batch **batch);
+int seis_batch_destroy(seis_batch *b);
+/* Add a job (zeroes its wavefields; it starts at step 0); *slot receives   */
+/* its slot.  Fails when the batch is full.                                   */
+int seis_batch_add(seis_batch *b, seis_model *m, int *slot);
+/* Free a slot (read its seismograms first with seis_shot_read).             */
+int seis_batch_remove(seis_batch *b, int slot);
+/* Advance every job by nsteps steps (jobs at their last step stay idle).    */
+/* Asynchronous: seis_shot_read or seis_ctx_sync wait for it.                */
+int seis_batch_step(seis_batch *b, int nsteps);
+/* Jobs still stepping and the fewest steps any of them has left.           */
+int seis_batch_status(const seis_batch *b, int *nactive, int *min_left);
+/* Steps a slot still has to run (0: done), -1 for a free slot.             */
+int seis_batch_slot_left(const seis_batch *b, int slot);
 
 /* Record HIP events around every update_v / update_s launch of the next    */
 /* forward runs (costs a few microseconds per step).                          */

@@ -91,6 +91,13 @@ SIGNATURES = [
     ("seis_grad_read", _I, [_P, _FP]),
     ("seis_gradient_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP, _FP, _FP,
                                 ctypes.POINTER(ctypes.c_double)]),
+    ("seis_batch_create", _I, [_P, _I, ctypes.POINTER(_P)]),
+    ("seis_batch_destroy", _I, [_P]),
+    ("seis_batch_add", _I, [_P, _P, ctypes.POINTER(_I)]),
+    ("seis_batch_remove", _I, [_P, _I]),
+    ("seis_batch_step", _I, [_P, _I]),
+    ("seis_batch_status", _I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    ("seis_batch_slot_left", _I, [_P, _I]),
     ("seis_model_set_profiling", _I, [_P, _I]),
     ("seis_last_timing", _I, [_P, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_double),

@@ -1,6 +1,7 @@
 // C-ABI runtime: device context, buffers, model set-up and the per-shot
 // time loop.  Replaces the reference's Init_OpenCL / clprogram / clbuf layer
 // and the forward half of time_stepping (src/time_stepping.c:553-848).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +61,12 @@ struct seis_ctx {
     }
 };
 
+// Precision modes (seis_csts.FP16): FP32; FP32 storage with the scaled
+// units of FP16 (a test of the scaling, results equal FP32 up to
+// subnormals); FP16 storage of wavefields and memory variables with FP32
+// arithmetic; FP64 storage and arithmetic.
+enum Prec { P_FP32 = 0, P_SCALED = 1, P_FP16 = 2, P_FP64 = 64 };
+
 struct seis_model {
     seis_ctx *ctx = nullptr;
     seis_csts c{};
@@ -67,6 +74,13 @@ struct seis_model {
     Layout g;
     Setup s;
     FDArgs<float> a{};
+    // precision: storage bytes of wavefields (selem) and of coefficients (celem)
+    int prec = P_FP32, selem = 4, celem = 4;
+    FDArgs<double> ad{};             // the same buffers, FP64 mode
+    // scaled units: stored v = av * v, stored stresses = as * sigma, with
+    // av / as = q fixed by the model and av chosen per shot from the source
+    float q = 1, av = 1, as = 1;
+    std::vector<float> unscale;      // per output: 1/av or 1/as
     std::vector<void *> owned;       // device buffers owned by the model
     size_t bytes = 0;
     std::vector<std::string> outn;
@@ -105,6 +119,10 @@ struct seis_model {
         bytes += (size_t)n * sizeof(float);
         return (float *)p;
     }
+    // n wavefield / memory-variable elements
+    float *salloc(long n) { return dalloc((n * selem + 3) / 4); }
+    // n coefficient or CPML-memory elements
+    float *calloc_(long n) { return dalloc(n * (celem / 4)); }
     ~seis_model() {
         for (void *p : owned) ctx->release(p);
         void *sp[] = {shot.src_cell, shot.src_type, shot.src_k, shot.src, shot.rec_cell, shot.out,
@@ -112,6 +130,26 @@ struct seis_model {
         for (void *p : sp) ctx->release(p);
         for (auto e : evpool) (void)hipEventDestroy(e);
     }
+};
+
+// A batch of modeling jobs advanced together, one launch per kernel class and
+// step for all of them (serving).  Slots hold models with a bound shot.
+struct seis_batch {
+    seis_ctx *ctx = nullptr;
+    int max_slots = 0;
+    std::vector<seis_model *> model;     // nullptr: free slot
+    std::vector<int> t;                  // host copy of each slot's next step
+    std::vector<int> tmax;
+    BatchSlot *dslots = nullptr;         // device slot table
+    int *dids = nullptr;                 // device class tables
+    unsigned *dtile0 = nullptr;
+    std::vector<BatchClass> classes;
+    int maxrec = 0;
+    bool dirty = true;
+    // host sources of asynchronous copies (kept alive until the next sync)
+    std::vector<BatchSlot> host_slots;
+    std::vector<int> host_ids, host_active;
+    std::vector<unsigned> host_tile0;
 };
 
 namespace {
@@ -127,15 +165,29 @@ std::vector<float> to_padded(const Layout &g, const std::vector<float> &ref) {
     return out;
 }
 
-int upload(seis_model *m, const float **dst, const std::vector<float> &ref) {
-    float *d = m->dalloc(m->g.nfield);
+// Host float array -> device coefficient array (float, or double in FP64
+// mode), optionally multiplied by a power of two (exact).
+int upload_raw(seis_model *m, const float **dst, const std::vector<float> &host, float mul = 1) {
+    float *d = m->calloc_((long)host.size());
     if (!d) return 1;
-    std::vector<float> pad = to_padded(m->g, ref);
-    HIP_TRY(hipMemcpyAsync(d, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice,
-                           m->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    if (m->celem == 8) {
+        std::vector<double> h(host.size());
+        for (size_t i = 0; i < h.size(); ++i) h[i] = (double)host[i] * mul;
+        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, m->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    } else {
+        std::vector<float> h(host);
+        if (mul != 1)
+            for (float &v : h) v *= mul;
+        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, m->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    }
     *dst = d;
     return 0;
+}
+
+int upload(seis_model *m, const float **dst, const std::vector<float> &ref, float mul = 1) {
+    return upload_raw(m, dst, to_padded(m->g, ref), mul);
 }
 
 int out_code(const std::string &n) {
@@ -154,13 +206,16 @@ int upload_params(seis_model *m, const RefParams &p) {
     const float DT = m->s.DT;
     const size_t n = p.M.size();
     FDArgs<float> &a = m->a;
-    if (upload(m, &a.rip, p.rip) || upload(m, &a.rkp, p.rkp)) return 1;
-    if (c.ND == 3 && upload(m, &a.rjp, p.rjp)) return 1;
+    // scaled units: buoyancies times q, stress-update coefficients over q
+    const float qv = m->q, qs = 1.0f / m->q;
+    if (upload(m, &a.rip, p.rip, qv) || upload(m, &a.rkp, p.rkp, qv)) return 1;
+    if (c.ND == 3 && upload(m, &a.rjp, p.rjp, qv)) return 1;
     if (L == 0) {
-        if (upload(m, &a.M, p.M)) return 1;
+        if (upload(m, &a.M, p.M, qs)) return 1;
         if (c.ND != 22) {
-            if (upload(m, &a.mu, p.mu) || upload(m, &a.muipkp, p.muipkp)) return 1;
-            if (c.ND == 3 && (upload(m, &a.muipjp, p.muipjp) || upload(m, &a.mujpkp, p.mujpkp)))
+            if (upload(m, &a.mu, p.mu, qs) || upload(m, &a.muipkp, p.muipkp, qs)) return 1;
+            if (c.ND == 3 &&
+                (upload(m, &a.muipjp, p.muipjp, qs) || upload(m, &a.mujpkp, p.mujpkp, qs)))
                 return 1;
         }
         return 0;
@@ -171,7 +226,7 @@ int upload_params(seis_model *m, const RefParams &p) {
             g[i] = p.M[i] * (1.0 + (float)L * p.taup[i]);
             e[i] = p.M[i] * p.taup[i];
         }
-        return upload(m, &a.cg, g) || upload(m, &a.ce, e);
+        return upload(m, &a.cg, g, qs) || upload(m, &a.ce, e, qs);
     }
     f.resize(n);
     d.resize(n);
@@ -181,7 +236,8 @@ int upload_params(seis_model *m, const RefParams &p) {
         d[i] = 2.0 * p.mu[i] * p.taus[i] / DT;
         e[i] = p.M[i] * p.taup[i] / DT;
     }
-    if (upload(m, &a.cg, g) || upload(m, &a.cf, f) || upload(m, &a.cd, d) || upload(m, &a.ce, e))
+    if (upload(m, &a.cg, g, qs) || upload(m, &a.cf, f, qs) || upload(m, &a.cd, d, qs) ||
+        upload(m, &a.ce, e, qs))
         return 1;
     auto fij = [&](const std::vector<float> &mu, const std::vector<float> &ts, const float **fo,
                    const float **dO) {
@@ -190,7 +246,7 @@ int upload_params(seis_model *m, const RefParams &p) {
             ff[i] = mu[i] * (1.0 + (float)L * ts[i]);
             dd[i] = mu[i] * ts[i] / DT;
         }
-        return upload(m, fo, ff) || upload(m, dO, dd);
+        return upload(m, fo, ff, qs) || upload(m, dO, dd, qs);
     };
     if (fij(p.muipkp, p.tausipkp, &a.fipkp, &a.dipkp)) return 1;
     if (c.ND == 3) {
@@ -204,29 +260,29 @@ int alloc_fields(seis_model *m) {
     const seis_csts &c = m->c;
     FDArgs<float> &a = m->a;
     const long nf = m->g.nfield, L = c.L;
-    a.vx = m->dalloc(nf);
-    a.vz = m->dalloc(nf);
-    if (c.ND == 3) a.vy = m->dalloc(nf);
+    a.vx = m->salloc(nf);
+    a.vz = m->salloc(nf);
+    if (c.ND == 3) a.vy = m->salloc(nf);
     if (c.ND == 22) {
-        a.p = m->dalloc(nf);
-        if (L) a.rp = m->dalloc(nf * L);
+        a.p = m->salloc(nf);
+        if (L) a.rp = m->salloc(nf * L);
     } else {
-        a.sxx = m->dalloc(nf);
-        a.szz = m->dalloc(nf);
-        a.sxz = m->dalloc(nf);
+        a.sxx = m->salloc(nf);
+        a.szz = m->salloc(nf);
+        a.sxz = m->salloc(nf);
         if (L) {
-            a.rxx = m->dalloc(nf * L);
-            a.rzz = m->dalloc(nf * L);
-            a.rxz = m->dalloc(nf * L);
+            a.rxx = m->salloc(nf * L);
+            a.rzz = m->salloc(nf * L);
+            a.rxz = m->salloc(nf * L);
         }
         if (c.ND == 3) {
-            a.syy = m->dalloc(nf);
-            a.sxy = m->dalloc(nf);
-            a.syz = m->dalloc(nf);
+            a.syy = m->salloc(nf);
+            a.sxy = m->salloc(nf);
+            a.syz = m->salloc(nf);
             if (L) {
-                a.ryy = m->dalloc(nf * L);
-                a.rxy = m->dalloc(nf * L);
-                a.ryz = m->dalloc(nf * L);
+                a.ryy = m->salloc(nf * L);
+                a.rxy = m->salloc(nf * L);
+                a.ryz = m->salloc(nf * L);
             }
         }
     }
@@ -234,30 +290,30 @@ int alloc_fields(seis_model *m) {
         const long nab2 = 2L * c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
         const long sxs = nab2 * NY * NZ, sys = NX * nab2 * NZ, szs = NX * NY * nab2;
         if (c.ND == 22) {
-            a.psi_p_x = m->dalloc(sxs);
-            a.psi_p_z = m->dalloc(szs);
-            a.psi_vx_x = m->dalloc(sxs);
-            a.psi_vz_z = m->dalloc(szs);
+            a.psi_p_x = m->calloc_(sxs);
+            a.psi_p_z = m->calloc_(szs);
+            a.psi_vx_x = m->calloc_(sxs);
+            a.psi_vz_z = m->calloc_(szs);
         } else {
-            a.psi_sxx_x = m->dalloc(sxs);
-            a.psi_sxz_x = m->dalloc(sxs);
-            a.psi_szz_z = m->dalloc(szs);
-            a.psi_sxz_z = m->dalloc(szs);
-            a.psi_vx_x = m->dalloc(sxs);
-            a.psi_vz_x = m->dalloc(sxs);
-            a.psi_vx_z = m->dalloc(szs);
-            a.psi_vz_z = m->dalloc(szs);
+            a.psi_sxx_x = m->calloc_(sxs);
+            a.psi_sxz_x = m->calloc_(sxs);
+            a.psi_szz_z = m->calloc_(szs);
+            a.psi_sxz_z = m->calloc_(szs);
+            a.psi_vx_x = m->calloc_(sxs);
+            a.psi_vz_x = m->calloc_(sxs);
+            a.psi_vx_z = m->calloc_(szs);
+            a.psi_vz_z = m->calloc_(szs);
             if (c.ND == 3) {
-                a.psi_sxy_x = m->dalloc(sxs);
-                a.psi_syz_z = m->dalloc(szs);
-                a.psi_vy_x = m->dalloc(sxs);
-                a.psi_vy_z = m->dalloc(szs);
-                a.psi_syy_y = m->dalloc(sys);
-                a.psi_sxy_y = m->dalloc(sys);
-                a.psi_syz_y = m->dalloc(sys);
-                a.psi_vx_y = m->dalloc(sys);
-                a.psi_vy_y = m->dalloc(sys);
-                a.psi_vz_y = m->dalloc(sys);
+                a.psi_sxy_x = m->calloc_(sxs);
+                a.psi_syz_z = m->calloc_(szs);
+                a.psi_vy_x = m->calloc_(sxs);
+                a.psi_vy_z = m->calloc_(szs);
+                a.psi_syy_y = m->calloc_(sys);
+                a.psi_sxy_y = m->calloc_(sys);
+                a.psi_syz_y = m->calloc_(sys);
+                a.psi_vx_y = m->calloc_(sys);
+                a.psi_vy_y = m->calloc_(sys);
+                a.psi_vz_y = m->calloc_(sys);
             }
         }
     }
@@ -268,7 +324,7 @@ int alloc_fields(seis_model *m) {
 // (vars_init, src/automatic_kernels.c:291-392).
 void zero_state(seis_model *m) {
     const FDArgs<float> &a = m->a;
-    const size_t nf = (size_t)m->g.nfield * sizeof(float);
+    const size_t nf = ((size_t)m->g.nfield * m->selem + 3) / 4 * 4;
     const size_t L = m->c.L;
     float *f1[] = {a.vx, a.vy, a.vz, a.sxx, a.syy, a.szz, a.sxy, a.syz, a.sxz, a.p};
     for (float *p : f1) launch_fill(p, nf, m->ctx->stream);
@@ -276,7 +332,8 @@ void zero_state(seis_model *m) {
     for (float *p : fl) launch_fill(p, nf * L, m->ctx->stream);
     if (m->c.abs_type == 1) {
         const long nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
-        const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
+        const size_t ce = m->celem;
+        const size_t sx = nab2 * NY * NZ * ce, sy = NX * nab2 * NZ * ce, sz = NX * NY * nab2 * ce;
         float *px[] = {a.psi_sxx_x, a.psi_sxy_x, a.psi_sxz_x, a.psi_vx_x, a.psi_vy_x, a.psi_vz_x, a.psi_p_x};
         float *py[] = {a.psi_sxy_y, a.psi_syy_y, a.psi_syz_y, a.psi_vx_y, a.psi_vy_y, a.psi_vz_y};
         float *pz[] = {a.psi_sxz_z, a.psi_syz_z, a.psi_szz_z, a.psi_vx_z, a.psi_vy_z, a.psi_vz_z, a.psi_p_z};
@@ -310,6 +367,89 @@ hipEvent_t pool_event(seis_model *m, size_t k) {
     return m->evpool[k];
 }
 
+// IEEE binary16 -> float (host side of FP16 storage)
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31, ex = (h >> 10) & 0x1f, man = h & 0x3ff;
+    float f;
+    if (ex == 0) f = std::ldexp((float)man, -24);                    // zero, subnormal
+    else if (ex == 31) f = man ? NAN : INFINITY;
+    else f = std::ldexp((float)(man | 0x400), (int)ex - 25);
+    uint32_t bits;
+    std::memcpy(&bits, &f, 4);
+    bits |= sign;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+// The FP64 view of the model's buffers: the same device arrays (allocated
+// with 8-byte elements) and the scalars promoted exactly.
+FDArgs<double> as_double(const FDArgs<float> &a) {
+    FDArgs<double> d;
+    std::memset(&d, 0, sizeof(d));
+    d.g = a.g;
+    for (int k = 0; k < 7; ++k) d.hc[k] = a.hc[k];
+    d.DT = a.DT;
+    d.DT2 = a.DT2;
+    d.L = a.L;
+    for (int l = 0; l < kMaxL; ++l) {
+        d.eta[l] = a.eta[l];
+        d.bl[l] = a.bl[l];
+        d.cl[l] = a.cl[l];
+        d.bs[l] = a.bs[l];
+    }
+    d.abs_type = a.abs_type;
+    d.freesurf = a.freesurf;
+    d.nab = a.nab;
+    d.fuse_surface = a.fuse_surface;
+#define SEIS_AS_D(f) d.f = (decltype(d.f))(a.f)
+    SEIS_AS_D(vx); SEIS_AS_D(vy); SEIS_AS_D(vz); SEIS_AS_D(sxx); SEIS_AS_D(syy); SEIS_AS_D(szz);
+    SEIS_AS_D(sxy); SEIS_AS_D(syz); SEIS_AS_D(sxz); SEIS_AS_D(p);
+    SEIS_AS_D(rxx); SEIS_AS_D(ryy); SEIS_AS_D(rzz); SEIS_AS_D(rxy); SEIS_AS_D(ryz); SEIS_AS_D(rxz);
+    SEIS_AS_D(rp);
+    SEIS_AS_D(rip); SEIS_AS_D(rjp); SEIS_AS_D(rkp); SEIS_AS_D(M); SEIS_AS_D(mu);
+    SEIS_AS_D(muipkp); SEIS_AS_D(muipjp); SEIS_AS_D(mujpkp); SEIS_AS_D(cg); SEIS_AS_D(cf);
+    SEIS_AS_D(cd); SEIS_AS_D(ce); SEIS_AS_D(fipkp); SEIS_AS_D(fipjp); SEIS_AS_D(fjpkp);
+    SEIS_AS_D(dipkp); SEIS_AS_D(dipjp); SEIS_AS_D(djpkp); SEIS_AS_D(cpml); SEIS_AS_D(taper);
+    SEIS_AS_D(psi_sxx_x); SEIS_AS_D(psi_sxy_x); SEIS_AS_D(psi_sxz_x); SEIS_AS_D(psi_sxy_y);
+    SEIS_AS_D(psi_syy_y); SEIS_AS_D(psi_syz_y); SEIS_AS_D(psi_sxz_z); SEIS_AS_D(psi_syz_z);
+    SEIS_AS_D(psi_szz_z); SEIS_AS_D(psi_vx_x); SEIS_AS_D(psi_vy_x); SEIS_AS_D(psi_vz_x);
+    SEIS_AS_D(psi_vx_y); SEIS_AS_D(psi_vy_y); SEIS_AS_D(psi_vz_y); SEIS_AS_D(psi_vx_z);
+    SEIS_AS_D(psi_vy_z); SEIS_AS_D(psi_vz_z); SEIS_AS_D(psi_p_x); SEIS_AS_D(psi_p_z);
+#undef SEIS_AS_D
+    return d;
+}
+
+// One launch of each step kernel in the model's precision.
+void step_sources(seis_model *m, int t, int pdir, hipStream_t st) {
+    if (m->prec == P_FP16)
+        launch_sources<float, __half>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st);
+    else if (m->prec == P_FP64)
+        launch_sources<double, double>(m->ad, m->nd_case, m->shot, m->c.NT, t, pdir, st);
+    else
+        launch_sources<float, float>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st);
+}
+void step_v(seis_model *m, hipStream_t st) {
+    if (m->prec == P_FP16) launch_v3d<float, __half>(m->a, st);
+    else if (m->prec == P_FP64) launch_v3d<double, double>(m->ad, st);
+    else launch_update_v(m->a, m->nd_case, st);
+}
+void step_s(seis_model *m, hipStream_t st) {
+    if (m->prec == P_FP16) launch_s3d<float, __half>(m->a, st);
+    else if (m->prec == P_FP64) launch_s3d<double, double>(m->ad, st);
+    else launch_update_s(m->a, m->nd_case, st);
+}
+void step_varsout(seis_model *m, int t, hipStream_t st) {
+    const int n = (int)m->out_codes.size();
+    const float *u = m->unscale.empty() ? nullptr : m->unscale.data();
+    if (m->prec == P_FP16)
+        launch_varsout<float, __half>(m->a, m->nd_case, m->shot, m->out_codes.data(), u, n, t, st);
+    else if (m->prec == P_FP64)
+        launch_varsout<double, double>(m->ad, m->nd_case, m->shot, m->out_codes.data(), u, n, t,
+                                       st);
+    else
+        launch_varsout<float, float>(m->a, m->nd_case, m->shot, m->out_codes.data(), u, n, t, st);
+}
+
 // Gradient state: adjoint wavefields, their CPML memory (or the forward
 // run's, as the reference does, without FIX_ADJ_CPML), double accumulators.
 int grad_setup(seis_model *m) {
@@ -319,6 +459,7 @@ int grad_setup(seis_model *m) {
     if (c.ND == 22 || c.L > 0)
         return fail("gradients need the elastic case (ND 2 or 3, L=0)");
     if (c.freesurf) return fail("gradients with a free surface are not available in this build");
+    if (m->prec != P_FP32) return fail("gradients are available with FP32 wavefields (FP16=0)");
     G.sh = make_shell(m->g, c.abs_type > 0 ? c.nab : 0, c.freesurf);
     if (G.sh.NXbnd < 2 * G.sh.F || G.sh.NZbnd < 2 * G.sh.F ||
         (c.ND == 3 && G.sh.NYbnd < 2 * G.sh.F))
@@ -487,7 +628,10 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     const seis_csts c = *cin;
     if (c.ND != 2 && c.ND != 3 && c.ND != 22) return fail("ND must be 2, 3 or 22");
     if (c.L < 0 || c.L > 4) return fail("L must be in [0, 4]");
-    if (c.FP16 != 0) return fail("only FP16=0 is available in this build");
+    if (c.FP16 != P_FP32 && c.FP16 != P_SCALED && c.FP16 != P_FP16 && c.FP16 != P_FP64)
+        return fail("FP16 must be 0 (FP32), 1 (FP32, scaled units), 2 (FP16 storage) or 64 (FP64)");
+    if (c.FP16 != P_FP32 && c.ND != 3)
+        return fail("FP16 storage and FP64 are available for ND=3 in this build");
     if (c.NT <= 0) return fail("NT must be positive");
     const int nd = c.ND == 3 ? 3 : 2;
     for (int d = 0; d < nd; ++d) {
@@ -514,10 +658,21 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
         return fail(buf);
     }
     const int NZ = c.N[0], NY = c.ND == 3 ? c.N[1] : 1, NX = c.ND == 3 ? c.N[2] : c.N[1];
-    m->g = make_layout(nd, NZ, NY, NX, c.FDORDER / 2, 4);
+    m->prec = c.FP16;
+    m->selem = c.FP16 == P_FP16 ? 2 : c.FP16 == P_FP64 ? 8 : 4;
+    m->celem = c.FP16 == P_FP64 ? 8 : 4;
+    m->g = make_layout(nd, NZ, NY, NX, c.FDORDER / 2, m->selem);
     // kernels address every array with 32-bit byte offsets
-    if ((unsigned long long)m->g.nfield * sizeof(float) >= (1ull << 32))
+    if ((unsigned long long)m->g.nfield * std::max(m->selem, m->celem) >= (1ull << 32))
         return fail("grid too large: one padded field must stay below 4 GiB");
+    if (c.FP16 == P_SCALED || c.FP16 == P_FP16) {
+        // q = av/as ~ the largest impedance rho*vp = sqrt(M/buoyancy), as a
+        // power of two so that rescaling the coefficients is exact
+        double zmax = 0;
+        for (size_t i = 0; i < rp.M.size(); ++i)
+            if (rp.rip[i] > 0) zmax = std::max(zmax, std::sqrt((double)rp.M[i] / rp.rip[i]));
+        m->q = zmax > 0 ? std::ldexp(1.0f, (int)std::lround(std::log2(zmax))) : 1.0f;
+    }
     FDArgs<float> &a = m->a;
     std::memset(&a, 0, sizeof(a));
     a.g = m->g;
@@ -539,19 +694,12 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     HIP_TRY(hipSetDevice(ctx->device));
     if (upload_params(m.get(), rp)) return 1;
     if (c.abs_type == 1) {
-        float *d = m->dalloc((long)m->s.cpml.size());
-        if (!d) return 1;
-        HIP_TRY(hipMemcpyAsync(d, m->s.cpml.data(), m->s.cpml.size() * 4, hipMemcpyHostToDevice,
-                               ctx->stream));
-        a.cpml = d;
+        if (upload_raw(m.get(), &a.cpml, m->s.cpml)) return 1;
     } else if (c.abs_type == 2) {
-        float *d = m->dalloc((long)m->s.taper.size());
-        if (!d) return 1;
-        HIP_TRY(hipMemcpyAsync(d, m->s.taper.data(), m->s.taper.size() * 4, hipMemcpyHostToDevice,
-                               ctx->stream));
-        a.taper = d;
+        if (upload_raw(m.get(), &a.taper, m->s.taper)) return 1;
     }
     if (alloc_fields(m.get())) return 1;
+    if (m->prec == P_FP64) m->ad = as_double(m->a);
     m->rp = std::move(rp);
     m->outn = out_names(c);
     for (auto &n : m->outn) m->out_codes.push_back(out_code(n));
@@ -667,7 +815,41 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
         HIP_TRY(hipMemcpyAsync(sd.src_cell, scell.data(), nsrc * 4, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(sd.src_type, stype.data(), nsrc * 4, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(sd.src_k, sk.data(), nsrc * 4, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(sd.src, src, (size_t)nsrc * NT * 4, hipMemcpyHostToDevice, st));
+        if (m->prec == P_SCALED || m->prec == P_FP16) {
+            // scaled units: the largest per-step injection (DT * |src|) of the
+            // sources feeding velocities (or, without any, stresses) is stored
+            // as about 2^8; powers of two keep the scaling exact
+            double af = 0, ae = 0;
+            for (int k = 0; k < nsrc; ++k)
+                for (int t = 0; t < NT; ++t) {
+                    const double v = std::fabs((double)src[(size_t)k * NT + t]) * c.dt;
+                    if (stype[k] <= 2) af = std::max(af, v);
+                    else ae = std::max(ae, v);
+                }
+            auto pow2 = [](double x) { return std::ldexp(1.0f, (int)std::lround(std::log2(x))); };
+            if (af > 0) {
+                m->av = pow2(256.0 / af);
+                m->as = m->av / m->q;
+            } else if (ae > 0) {
+                m->as = pow2(256.0 / ae);
+                m->av = m->as * m->q;
+            } else {
+                m->av = m->q;
+                m->as = 1;
+            }
+            std::vector<float> ss((size_t)nsrc * NT);
+            for (int k = 0; k < nsrc; ++k) {
+                const float f = stype[k] <= 2 ? m->av : m->as;
+                for (int t = 0; t < NT; ++t) ss[(size_t)k * NT + t] = src[(size_t)k * NT + t] * f;
+            }
+            HIP_TRY(hipMemcpyAsync(sd.src, ss.data(), ss.size() * 4, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            m->unscale.clear();
+            for (int code : m->out_codes)
+                m->unscale.push_back(code <= O_VZ ? 1.0f / m->av : 1.0f / m->as);
+        } else {
+            HIP_TRY(hipMemcpyAsync(sd.src, src, (size_t)nsrc * NT * 4, hipMemcpyHostToDevice, st));
+        }
     }
     if (nrec) HIP_TRY(hipMemcpyAsync(sd.rec_cell, rcell.data(), nrec * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -688,17 +870,17 @@ int seis_launch_init(seis_model *m) {
 
 int seis_launch_sources(seis_model *m, int nt, int pdir) {
     if (!m->shot_bound) return fail("no shot bound");
-    launch_sources(m->a, m->nd_case, m->shot, m->c.NT, nt, pdir, m->ctx->stream);
+    step_sources(m, nt, pdir, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 int seis_launch_update_v(seis_model *m) {
-    launch_update_v(m->a, m->nd_case, m->ctx->stream);
+    step_v(m, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 int seis_launch_update_s(seis_model *m) {
-    launch_update_s(m->a, m->nd_case, m->ctx->stream);
+    step_s(m, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -711,8 +893,7 @@ int seis_launch_surface(seis_model *m) {
 }
 int seis_launch_varsout(seis_model *m, int nt) {
     if (!m->shot_bound) return fail("no shot bound");
-    launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(), nt,
-                   m->ctx->stream);
+    step_varsout(m, nt, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -744,16 +925,15 @@ int seis_run_steps(seis_model *m, int t0, int nsteps) {
     for (int k = 0; k < nsteps; ++k) {
         const int t = t0 + k;
         // step order of src/time_stepping.c:785-816
-        launch_sources(m->a, m->nd_case, m->shot, m->c.NT, t, 1, st);
+        step_sources(m, t, 1, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
-        launch_update_v(m->a, m->nd_case, st);
+        step_v(m, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
         if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
-        launch_update_s(m->a, m->nd_case, st);
+        step_s(m, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
         if (m->c.freesurf && !m->a.fuse_surface) launch_surface(m->a, m->nd_case, 1, st);
-        launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(),
-                       t, st);
+        step_varsout(m, t, st);
     }
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipGetLastError());
@@ -791,11 +971,22 @@ int seis_model_set_profiling(seis_model *m, int on) {
 
 int seis_read_field(seis_model *m, const char *name, float *host) {
     g_err.clear();
-    float *d = field_ptr(m, name ? name : "");
+    const std::string nm = name ? name : "";
+    float *d = field_ptr(m, nm);
     if (!d) return fail(std::string("no wavefield named ") + (name ? name : "(null)"));
-    std::vector<float> tmp((size_t)m->g.nfield);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), d, tmp.size() * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    const size_t nf = (size_t)m->g.nfield;
+    std::vector<unsigned char> raw(nf * m->selem);
+    HIP_TRY(hipMemcpyAsync(raw.data(), d, raw.size(), hipMemcpyDeviceToHost, m->ctx->stream));
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    std::vector<float> tmp(nf);
+    const float u = (m->prec == P_SCALED || m->prec == P_FP16)
+                        ? (nm[0] == 'v' ? 1.0f / m->av : 1.0f / m->as)
+                        : 1.0f;
+    for (size_t i = 0; i < nf; ++i) {
+        if (m->selem == 8) tmp[i] = (float)((const double *)raw.data())[i];
+        else if (m->selem == 2) tmp[i] = half_to_float(((const uint16_t *)raw.data())[i]) * u;
+        else tmp[i] = ((const float *)raw.data())[i] * u;
+    }
     const Layout &g = m->g;
     const int H = g.H;
     const long NZp = g.NZ + 2 * H, NYp = g.nd == 3 ? g.NY + 2 * H : 1, NXp = g.NX + 2 * H;
@@ -850,12 +1041,11 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
     // forward step order with boundary saving after update_s
     // (src/time_stepping.c:785-816)
     for (int t = 0; t < tmax; ++t) {
-        launch_sources(m->a, m->nd_case, m->shot, m->c.NT, t, 1, st);
-        launch_update_v(m->a, m->nd_case, st);
-        launch_update_s(m->a, m->nd_case, st);
+        step_sources(m, t, 1, st);
+        step_v(m, st);
+        step_s(m, st);
         launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (size_t)t * per, st);
-        launch_varsout(m->a, m->nd_case, m->shot, m->out_codes.data(), (int)m->out_codes.size(),
-                       t, st);
+        step_varsout(m, t, st);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
@@ -911,7 +1101,7 @@ int seis_adjoint(seis_model *m, const float *res) {
         A.bnd = G.bnd + (size_t)(t - 1) * per;
         launch_adj_s(A, m->nd_case, st);
         launch_adj_v(A, m->nd_case, st);
-        launch_sources(m->a, m->nd_case, m->shot, NT, t, -1, st);
+        step_sources(m, t, -1, st);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
@@ -959,6 +1149,169 @@ int seis_gradient_shot(seis_model *m, int nsrc, const float *src_pos, const floa
         rms[0] = rr[0];
         rms[1] = rr[1];
     }
+    return 0;
+}
+
+// ---- batched serving -------------------------------------------------------
+
+int seis_batch_create(seis_ctx *ctx, int max_slots, seis_batch **out) {
+    g_err.clear();
+    *out = nullptr;
+    if (!ctx || max_slots <= 0 || max_slots > 4096) return fail("seis_batch_create: bad arguments");
+    auto b = std::make_unique<seis_batch>();
+    b->ctx = ctx;
+    b->max_slots = max_slots;
+    b->model.assign(max_slots, nullptr);
+    b->host_slots.resize(max_slots);
+    b->host_active.assign(max_slots, 0);
+    b->t.assign(max_slots, 0);
+    b->tmax.assign(max_slots, 0);
+    void *p = nullptr;
+    if (ctx->alloc(sizeof(BatchSlot) * max_slots, &p)) return 1;
+    b->dslots = (BatchSlot *)p;
+    if (ctx->alloc(sizeof(int) * max_slots * 4, &p)) return 1;
+    b->dids = (int *)p;
+    if (ctx->alloc(sizeof(unsigned) * (max_slots + 1) * 4, &p)) return 1;
+    b->dtile0 = (unsigned *)p;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *out = b.release();
+    return 0;
+}
+
+int seis_batch_destroy(seis_batch *b) {
+    if (!b) return 0;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->ctx->release(b->dslots);
+    b->ctx->release(b->dids);
+    b->ctx->release(b->dtile0);
+    delete b;
+    return 0;
+}
+
+int seis_batch_add(seis_batch *b, seis_model *m, int *slot) {
+    g_err.clear();
+    if (!m->shot_bound) return fail("seis_batch_add: bind a shot first");
+    if (m->ctx != b->ctx) return fail("seis_batch_add: the model lives in another context");
+    const seis_csts &c = m->c;
+    if (c.ND == 3 || c.L != 0 || c.freesurf || m->prec != P_FP32)
+        return fail("seis_batch_add: batches take 2D FP32 jobs with L=0 and no free surface");
+    int k = -1;
+    for (int i = 0; i < b->max_slots; ++i)
+        if (!b->model[i]) {
+            k = i;
+            break;
+        }
+    if (k < 0) return fail("seis_batch_add: the batch is full");
+    BatchSlot sl;
+    std::memset(&sl, 0, sizeof(sl));
+    sl.a = m->a;
+    sl.shot = m->shot;
+    sl.active = 1;
+    sl.nd_case = m->nd_case;
+    sl.NT = c.NT;
+    sl.tmax = tmax_of(m);
+    sl.t = 0;
+    sl.nout = (int)m->out_codes.size();
+    for (int q = 0; q < sl.nout && q < 10; ++q) sl.codes[q] = m->out_codes[q];
+    // the job starts from zeroed wavefields and seismograms
+    if (seis_launch_init(m)) return 1;
+    b->host_slots[k] = sl;
+    HIP_TRY(hipMemcpyAsync(b->dslots + k, &b->host_slots[k], sizeof(sl), hipMemcpyHostToDevice,
+                           b->ctx->stream));
+    b->model[k] = m;
+    b->t[k] = 0;
+    b->tmax[k] = sl.tmax;
+    b->dirty = true;
+    *slot = k;
+    return 0;
+}
+
+int seis_batch_remove(seis_batch *b, int slot) {
+    g_err.clear();
+    if (slot < 0 || slot >= b->max_slots || !b->model[slot]) return fail("seis_batch_remove: bad slot");
+    b->host_active[slot] = 0;
+    HIP_TRY(hipMemcpyAsync((char *)(b->dslots + slot) + offsetof(BatchSlot, active),
+                           &b->host_active[slot], sizeof(int), hipMemcpyHostToDevice,
+                           b->ctx->stream));
+    b->model[slot] = nullptr;
+    b->dirty = true;
+    return 0;
+}
+
+// slots still stepping, and the fewest steps any of them has left
+int seis_batch_status(const seis_batch *b, int *nactive, int *min_left) {
+    int n = 0, ml = 0;
+    for (int i = 0; i < b->max_slots; ++i)
+        if (b->model[i] && b->t[i] < b->tmax[i]) {
+            const int left = b->tmax[i] - b->t[i];
+            ml = n == 0 ? left : std::min(ml, left);
+            ++n;
+        }
+    if (nactive) *nactive = n;
+    if (min_left) *min_left = ml;
+    return 0;
+}
+
+int seis_batch_slot_left(const seis_batch *b, int slot) {
+    if (slot < 0 || slot >= b->max_slots || !b->model[slot]) return -1;
+    return b->tmax[slot] - b->t[slot];
+}
+
+int seis_batch_step(seis_batch *b, int nsteps) {
+    g_err.clear();
+    hipStream_t st = b->ctx->stream;
+    if (b->dirty) {
+        // rebuild the class tables: slots grouped by (acoustic/elastic, FD order)
+        std::map<std::pair<int, int>, std::vector<int>> groups;
+        int maxrec = 0;
+        for (int i = 0; i < b->max_slots; ++i) {
+            seis_model *m = b->model[i];
+            if (!m) continue;
+            groups[{m->nd_case, m->g.H}].push_back(i);
+            maxrec = std::max(maxrec, m->shot.nrec);
+        }
+        // the previous tables may still be read by queued copies
+        HIP_TRY(hipStreamSynchronize(st));
+        std::vector<int> &ids = b->host_ids;
+        std::vector<unsigned> &tile0 = b->host_tile0;
+        ids.clear();
+        tile0.clear();
+        b->classes.clear();
+        for (auto &kv : groups) {
+            BatchClass c;
+            c.nd_case = kv.first.first;
+            c.H = kv.first.second;
+            c.n = (int)kv.second.size();
+            const size_t off_ids = ids.size(), off_t = tile0.size();
+            unsigned acc = 0;
+            for (int i : kv.second) {
+                const Layout &g = b->model[i]->g;
+                ids.push_back(i);
+                tile0.push_back(acc);
+                acc += (unsigned)((g.NZ + 63) / 64) * (unsigned)((g.NX + 3) / 4);
+            }
+            tile0.push_back(acc);
+            c.ntiles = acc;
+            c.ids = b->dids + off_ids;
+            c.tile0 = b->dtile0 + off_t;
+            b->classes.push_back(c);
+        }
+        if (!ids.empty())
+            HIP_TRY(hipMemcpyAsync(b->dids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
+        if (!tile0.empty())
+            HIP_TRY(hipMemcpyAsync(b->dtile0, tile0.data(), tile0.size() * 4, hipMemcpyHostToDevice,
+                                   st));
+        b->maxrec = maxrec;
+        b->dirty = false;
+    }
+    int hi = 0;
+    for (int i = 0; i < b->max_slots; ++i)
+        if (b->model[i]) hi = i + 1;
+    for (int k = 0; k < nsteps; ++k)
+        launch_batch_tick(b->dslots, hi, b->classes.data(), (int)b->classes.size(), b->maxrec, st);
+    for (int i = 0; i < b->max_slots; ++i)
+        if (b->model[i]) b->t[i] = std::min(b->tmax[i], b->t[i] + nsteps);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

@@ -45,11 +45,13 @@ __device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int 
 
 // Waves per SIMD the velocity update is compiled for: FD orders 2 and 4 fit
 // 80 VGPRs (6 waves instead of 4) without spilling; higher orders need more.
-constexpr int v3d_waves(int H) { return H <= 2 ? 6 : H <= 4 ? 4 : 2; }
+// (FP64 arithmetic needs twice the registers: no constraint there.)
+constexpr int v3d_waves(int H, int rbytes) { return rbytes == 8 ? 2 : H <= 2 ? 6 : H <= 4 ? 4 : 2; }
 
-template <typename R, int H>
-__global__ __launch_bounds__(64 * BY3) __attribute__((amdgpu_waves_per_eu(v3d_waves(H), 8)))
+template <typename R, typename S, int H>
+__global__ __launch_bounds__(64 * BY3) __attribute__((amdgpu_waves_per_eu(v3d_waves(H, sizeof(R)), 8)))
 void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
+    using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
     if (!tile3(nbz, g.NX, nb, z, y, x)) return;
@@ -57,16 +59,21 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy;
     const R *hc = a.hc;
-    const unsigned o = (unsigned)(g.idx(z, y, x) * sizeof(R));   // lane byte offset
-    R szz_z = DpO<H>(a.szz, o, 1, hc);
-    R sxx_x = DpO<H>(a.sxx, o, sx, hc);
-    R sxz_x = DmO<H>(a.sxz, o, sx, hc);
-    R sxz_z = DmO<H>(a.sxz, o, 1, hc);
-    R syz_y = DmO<H>(a.syz, o, sy, hc);
-    R syz_z = DmO<H>(a.syz, o, 1, hc);
-    R syy_y = DpO<H>(a.syy, o, sy, hc);
-    R sxy_x = DmO<H>(a.sxy, o, sx, hc);
-    R sxy_y = DmO<H>(a.sxy, o, sy, hc);
+    const long cell = g.idx(z, y, x);
+    const unsigned o = (unsigned)(cell * sizeof(R));    // lane byte offset, coefficients
+    const unsigned os = (unsigned)(cell * sizeof(S));   // lane byte offset, wavefields
+    const S *szzp = (const S *)a.szz, *sxxp = (const S *)a.sxx, *sxzp = (const S *)a.sxz;
+    const S *syzp = (const S *)a.syz, *syyp = (const S *)a.syy, *sxyp = (const S *)a.sxy;
+    S *vxp = (S *)a.vx, *vyp = (S *)a.vy, *vzp = (S *)a.vz;
+    R szz_z = DpS<H>(szzp, os, 1, hc);
+    R sxx_x = DpS<H>(sxxp, os, sx, hc);
+    R sxz_x = DmS<H>(sxzp, os, sx, hc);
+    R sxz_z = DmS<H>(sxzp, os, 1, hc);
+    R syz_y = DmS<H>(syzp, os, sy, hc);
+    R syz_z = DmS<H>(syzp, os, 1, hc);
+    R syy_y = DpS<H>(syyp, os, sy, hc);
+    R sxy_x = DmS<H>(sxyp, os, sx, hc);
+    R sxy_y = DmS<H>(sxyp, os, sy, hc);
     // CPML strips: every load first (psi and profiles, then the cell's own
     // values), the psi stores after
     const bool cpml = a.abs_type == 1;
@@ -98,7 +105,7 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
         x1 = pml_load(a, a.psi_sxy_x + ipx, 0, false, iI);
         x2 = pml_load(a, a.psi_sxz_x + ipx, 0, false, iI);
     }
-    const R vx0 = ldo(a.vx, o), vy0 = ldo(a.vy, o), vz0 = ldo(a.vz, o);
+    const R vx0 = T::ld(vxp, os), vy0 = T::ld(vyp, os), vz0 = T::ld(vzp, os);
     const R rip = ldo(a.rip, o), rjp = ldo(a.rjp, o), rkp = ldo(a.rkp, o);
     if (pz) {
         pml_apply(sxz_z, a.psi_sxz_z + ipz, z0);
@@ -127,13 +134,14 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
         if (x < NAB) { vx *= t[x]; vy *= t[x]; vz *= t[x]; }
         if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vy *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
     }
-    sto(a.vx, o, vx);
-    sto(a.vy, o, vy);
-    sto(a.vz, o, vz);
+    T::st(vxp, os, vx);
+    T::st(vyp, os, vy);
+    T::st(vzp, os, vz);
 }
 
-template <typename R, int H, int L, bool FS>
+template <typename R, typename S, int H, int L, bool FS>
 __global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
+    using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
     if (!tile3(nbz, g.NX, nb, z, y, x)) return;
@@ -141,16 +149,23 @@ __global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy, nf = g.nfield;
     const R *hc = a.hc;
-    const unsigned o = (unsigned)(g.idx(z, y, x) * sizeof(R));   // lane byte offset
-    R vzx = DpO<H>(a.vz, o, sx, hc);
-    R vzy = DpO<H>(a.vz, o, sy, hc);
-    R vzz = DmO<H>(a.vz, o, 1, hc);
-    R vyx = DpO<H>(a.vy, o, sx, hc);
-    R vyy = DmO<H>(a.vy, o, sy, hc);
-    R vyz = DpO<H>(a.vy, o, 1, hc);
-    R vxx = DmO<H>(a.vx, o, sx, hc);
-    R vxy = DpO<H>(a.vx, o, sy, hc);
-    R vxz = DpO<H>(a.vx, o, 1, hc);
+    const long cell = g.idx(z, y, x);
+    const unsigned o = (unsigned)(cell * sizeof(R));    // lane byte offset, coefficients
+    const unsigned os = (unsigned)(cell * sizeof(S));   // lane byte offset, wavefields
+    const S *vxp = (const S *)a.vx, *vyp = (const S *)a.vy, *vzp = (const S *)a.vz;
+    S *sxxp = (S *)a.sxx, *syyp = (S *)a.syy, *szzp = (S *)a.szz;
+    S *sxyp = (S *)a.sxy, *syzp = (S *)a.syz, *sxzp = (S *)a.sxz;
+    S *rxxp = (S *)a.rxx, *ryyp = (S *)a.ryy, *rzzp = (S *)a.rzz;
+    S *rxyp = (S *)a.rxy, *ryzp = (S *)a.ryz, *rxzp = (S *)a.rxz;
+    R vzx = DpS<H>(vzp, os, sx, hc);
+    R vzy = DpS<H>(vzp, os, sy, hc);
+    R vzz = DmS<H>(vzp, os, 1, hc);
+    R vyx = DpS<H>(vyp, os, sx, hc);
+    R vyy = DmS<H>(vyp, os, sy, hc);
+    R vyz = DpS<H>(vyp, os, 1, hc);
+    R vxx = DmS<H>(vxp, os, sx, hc);
+    R vxy = DpS<H>(vxp, os, sy, hc);
+    R vxz = DpS<H>(vxp, os, 1, hc);
     const bool cpml = a.abs_type == 1;
     const bool pz = cpml && (z > NZ - NAB - 1 || (!a.freesurf && z < NAB));
     const bool py = cpml && (y < NAB || y > NY - NAB - 1);
@@ -181,8 +196,8 @@ __global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned
         x2 = pml_load(a, a.psi_vz_x + ipx, 0, true, iH);
     }
     // the cell's own values and coefficients, before any store
-    R sxy = ldo(a.sxy, o), syz = ldo(a.syz, o), sxz = ldo(a.sxz, o);
-    R sxx = ldo(a.sxx, o), syy = ldo(a.syy, o), szz = ldo(a.szz, o);
+    R sxy = T::ld(sxyp, os), syz = T::ld(syzp, os), sxz = T::ld(sxzp, os);
+    R sxx = T::ld(sxxp, os), syy = T::ld(syyp, os), szz = T::ld(szzp, os);
     R rxy[L > 0 ? L : 1], ryz[L > 0 ? L : 1], rxz[L > 0 ? L : 1];
     R rxx[L > 0 ? L : 1], ryy[L > 0 ? L : 1], rzz[L > 0 ? L : 1];
     R gg, f, d = 0, e = 0;
@@ -207,8 +222,8 @@ __global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             const long ln = l * nf;
-            rxy[l] = ldo(a.rxy + ln, o); ryz[l] = ldo(a.ryz + ln, o); rxz[l] = ldo(a.rxz + ln, o);
-            rxx[l] = ldo(a.rxx + ln, o); ryy[l] = ldo(a.ryy + ln, o); rzz[l] = ldo(a.rzz + ln, o);
+            rxy[l] = T::ld(rxyp + ln, os); ryz[l] = T::ld(ryzp + ln, os); rxz[l] = T::ld(rxzp + ln, os);
+            rxx[l] = T::ld(rxxp + ln, os); ryy[l] = T::ld(ryyp + ln, os); rzz[l] = T::ld(rzzp + ln, os);
         }
     }
     if (pz) {
@@ -327,23 +342,23 @@ __global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned
             }
         }
     }
-    sto(a.sxy, o, sxy); sto(a.syz, o, syz); sto(a.sxz, o, sxz);
-    sto(a.sxx, o, sxx); sto(a.syy, o, syy); sto(a.szz, o, szz);
+    T::st(sxyp, os, sxy); T::st(syzp, os, syz); T::st(sxzp, os, sxz);
+    T::st(sxxp, os, sxx); T::st(syyp, os, syy); T::st(szzp, os, szz);
     if constexpr (L > 0) {
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             const long ln = l * nf;
-            sto(a.rxy + ln, o, rxy[l]); sto(a.ryz + ln, o, ryz[l]); sto(a.rxz + ln, o, rxz[l]);
-            sto(a.rxx + ln, o, rxx[l]); sto(a.ryy + ln, o, ryy[l]); sto(a.rzz + ln, o, rzz[l]);
+            T::st(rxyp + ln, os, rxy[l]); T::st(ryzp + ln, os, ryz[l]); T::st(rxzp + ln, os, rxz[l]);
+            T::st(rxxp + ln, os, rxx[l]); T::st(ryyp + ln, os, ryy[l]); T::st(rzzp + ln, os, rzz[l]);
         }
     }
     if constexpr (FS) {
         // mirror images above the surface: szz(-m) = -szz(m),
         // sxz(-m) = -sxz(m-1), syz(-m) = -syz(m-1), m = 1..H
-        if (z >= 1 && z <= H) sto(a.szz, o - 2 * z * (unsigned)sizeof(R), -szz);
+        if (z >= 1 && z <= H) T::st(szzp, os - 2 * z * (unsigned)sizeof(S), -szz);
         if (z <= H - 1) {
-            sto(a.sxz, o - (2 * z + 1) * (unsigned)sizeof(R), -sxz);
-            sto(a.syz, o - (2 * z + 1) * (unsigned)sizeof(R), -syz);
+            T::st(sxzp, os - (2 * z + 1) * (unsigned)sizeof(S), -sxz);
+            T::st(syzp, os - (2 * z + 1) * (unsigned)sizeof(S), -syz);
         }
     }
 }
@@ -355,50 +370,54 @@ void grid3(const Layout &g, int &nbz, unsigned &nb) {
     nb = (unsigned)nbz * g.NX * nby;
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 void run_v3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
     grid3<R>(a.g, nbz, nb);
-    hipLaunchKernelGGL((k_v3d<R, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BY3), 0, st, a, nbz, nb);
+    hipLaunchKernelGGL((k_v3d<R, S, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BY3), 0, st, a, nbz, nb);
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 void run_s3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
     grid3<R>(a.g, nbz, nb);
     const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, BY3);
     if (a.freesurf && a.fuse_surface)
-        hipLaunchKernelGGL((k_s3d<R, H, L, true>), grid, blk, 0, st, a, nbz, nb);
+        hipLaunchKernelGGL((k_s3d<R, S, H, L, true>), grid, blk, 0, st, a, nbz, nb);
     else
-        hipLaunchKernelGGL((k_s3d<R, H, L, false>), grid, blk, 0, st, a, nbz, nb);
+        hipLaunchKernelGGL((k_s3d<R, S, H, L, false>), grid, blk, 0, st, a, nbz, nb);
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 void run_s3d_L(const FDArgs<R> &a, hipStream_t st) {
     switch (a.L) {
-        case 0: run_s3d<R, H, 0>(a, st); break;
-        case 1: run_s3d<R, H, 1>(a, st); break;
-        case 2: run_s3d<R, H, 2>(a, st); break;
-        case 3: run_s3d<R, H, 3>(a, st); break;
-        case 4: run_s3d<R, H, 4>(a, st); break;
+        case 0: run_s3d<R, S, H, 0>(a, st); break;
+        case 1: run_s3d<R, S, H, 1>(a, st); break;
+        case 2: run_s3d<R, S, H, 2>(a, st); break;
+        case 3: run_s3d<R, S, H, 3>(a, st); break;
+        case 4: run_s3d<R, S, H, 4>(a, st); break;
         default: break;
     }
 }
 
 }  // namespace
 
-template <typename R>
+template <typename R, typename S>
 void launch_v3d(const FDArgs<R> &a, void *stream) {
-    SEIS_SWITCH_H(a.g.H, (run_v3d<R, HH>(a, (hipStream_t)stream)));
+    SEIS_SWITCH_H(a.g.H, (run_v3d<R, S, HH>(a, (hipStream_t)stream)));
 }
-template <typename R>
+template <typename R, typename S>
 void launch_s3d(const FDArgs<R> &a, void *stream) {
-    SEIS_SWITCH_H(a.g.H, (run_s3d_L<R, HH>(a, (hipStream_t)stream)));
+    SEIS_SWITCH_H(a.g.H, (run_s3d_L<R, S, HH>(a, (hipStream_t)stream)));
 }
 
-template void launch_v3d<float>(const FDArgs<float> &, void *);
-template void launch_s3d<float>(const FDArgs<float> &, void *);
+template void launch_v3d<float, float>(const FDArgs<float> &, void *);
+template void launch_s3d<float, float>(const FDArgs<float> &, void *);
+template void launch_v3d<float, __half>(const FDArgs<float> &, void *);
+template void launch_s3d<float, __half>(const FDArgs<float> &, void *);
+template void launch_v3d<double, double>(const FDArgs<double> &, void *);
+template void launch_s3d<double, double>(const FDArgs<double> &, void *);
 
 }  // namespace seis

@@ -1,5 +1,6 @@
 // Device helpers shared by the finite-difference kernels.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "seis_internal.hpp"
@@ -122,6 +123,41 @@ __device__ __forceinline__ R DmO(const R *v, unsigned o, long s, const R *hc) {
     R acc = hc[1] * (ldo(v, o) - ldo(v - s, o));
 #pragma unroll
     for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (ldo(v + (k - 1) * s, o) - ldo(v - k * s, o));
+    return acc;
+}
+
+// Wavefield storage narrower than the compute type (FP16 fields with FP32
+// arithmetic): loads widen exactly, stores round to nearest even.
+template <typename R, typename S>
+struct Store {
+    static __device__ __forceinline__ R ld(const S *base, unsigned boff) {
+        return (R)ldo(base, boff);
+    }
+    static __device__ __forceinline__ void st(S *base, unsigned boff, R v) { sto(base, boff, (S)v); }
+};
+template <>
+struct Store<float, __half> {
+    static __device__ __forceinline__ float ld(const __half *base, unsigned boff) {
+        return __half2float(ldo(base, boff));
+    }
+    static __device__ __forceinline__ void st(__half *base, unsigned boff, float v) {
+        sto(base, boff, __float2half_rn(v));
+    }
+};
+template <int H, typename R, typename S>
+__device__ __forceinline__ R DpS(const S *v, unsigned o, long s, const R *hc) {
+    using T = Store<R, S>;
+    R acc = hc[1] * (T::ld(v + s, o) - T::ld(v, o));
+#pragma unroll
+    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (T::ld(v + k * s, o) - T::ld(v - (k - 1) * s, o));
+    return acc;
+}
+template <int H, typename R, typename S>
+__device__ __forceinline__ R DmS(const S *v, unsigned o, long s, const R *hc) {
+    using T = Store<R, S>;
+    R acc = hc[1] * (T::ld(v, o) - T::ld(v - s, o));
+#pragma unroll
+    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (T::ld(v + (k - 1) * s, o) - T::ld(v - k * s, o));
     return acc;
 }
 

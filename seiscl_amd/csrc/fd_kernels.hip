@@ -27,14 +27,8 @@ namespace {
 // 2D P-SV: src/update_v2D.cl:32-246, src/update_s2D.cl:30-326
 // ---------------------------------------------------------------------------
 template <typename R, int H>
-__global__ __launch_bounds__(256) void k_update_v2d(FDArgs<R> a, int nbz, unsigned nb) {
-    unsigned tile;
-    xcd_tile(nb, tile);
-    if (tile >= nb) return;
-    const int bz = tile % nbz, bx = tile / nbz;
-    const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+__device__ __forceinline__ void cell_v2d(const FDArgs<R> &a, int z, int x) {
     const Layout &g = a.g;
-    if (z >= g.NZ || x >= g.NX) return;
     const long i = g.idx(z, 0, x), sx = g.sx;
     const R *hc = a.hc;
     R sxx_x = Dp<H>(a.sxx, i, sx, hc);
@@ -72,15 +66,20 @@ __global__ __launch_bounds__(256) void k_update_v2d(FDArgs<R> a, int nbz, unsign
     a.vz[i] = vz;
 }
 
-template <typename R, int H, int L>
-__global__ __launch_bounds__(256) void k_update_s2d(FDArgs<R> a, int nbz, unsigned nb) {
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_update_v2d(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
     if (tile >= nb) return;
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    if (z >= a.g.NZ || x >= a.g.NX) return;
+    cell_v2d<R, H>(a, z, x);
+}
+
+template <typename R, int H, int L>
+__device__ __forceinline__ void cell_s2d(const FDArgs<R> &a, int z, int x) {
     const Layout &g = a.g;
-    if (z >= g.NZ || x >= g.NX) return;
     const long i = g.idx(z, 0, x), sx = g.sx;
     const R *hc = a.hc;
     R vxx = Dm<H>(a.vx, i, sx, hc);
@@ -156,18 +155,23 @@ __global__ __launch_bounds__(256) void k_update_s2d(FDArgs<R> a, int nbz, unsign
     a.szz[i] = szz;
 }
 
-// ---------------------------------------------------------------------------
-// 2D acoustic: src/update_v2D_acc.cl:80-325, src/update_s2D_acc.cl:79-402
-// ---------------------------------------------------------------------------
-template <typename R, int H>
-__global__ __launch_bounds__(256) void k_update_v2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_update_s2d(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
     if (tile >= nb) return;
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    if (z >= a.g.NZ || x >= a.g.NX) return;
+    cell_s2d<R, H, L>(a, z, x);
+}
+
+// ---------------------------------------------------------------------------
+// 2D acoustic: src/update_v2D_acc.cl:80-325, src/update_s2D_acc.cl:79-402
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__device__ __forceinline__ void cell_v2d_acc(const FDArgs<R> &a, int z, int x) {
     const Layout &g = a.g;
-    if (z >= g.NZ || x >= g.NX) return;
     const long i = g.idx(z, 0, x), sx = g.sx;
     const R *hc = a.hc;
     R p_x = Dp<H>(a.p, i, sx, hc);
@@ -202,15 +206,20 @@ __global__ __launch_bounds__(256) void k_update_v2d_acc(FDArgs<R> a, int nbz, un
     a.vz[i] = vz;
 }
 
-template <typename R, int H, int L>
-__global__ __launch_bounds__(256) void k_update_s2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_update_v2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
     if (tile >= nb) return;
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    if (z >= a.g.NZ || x >= a.g.NX) return;
+    cell_v2d_acc<R, H>(a, z, x);
+}
+
+template <typename R, int H, int L>
+__device__ __forceinline__ void cell_s2d_acc(const FDArgs<R> &a, int z, int x) {
     const Layout &g = a.g;
-    if (z >= g.NZ || x >= g.NX) return;
     const long i = g.idx(z, 0, x), sx = g.sx;
     const R *hc = a.hc;
     R vxx = Dm<H>(a.vx, i, sx, hc);
@@ -259,6 +268,17 @@ __global__ __launch_bounds__(256) void k_update_s2d_acc(FDArgs<R> a, int nbz, un
         if (x > NX - NAB - 1) p *= t[NX - x - 1];
     }
     a.p[i] = p;
+}
+
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_update_s2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
+    unsigned tile;
+    xcd_tile(nb, tile);
+    if (tile >= nb) return;
+    const int bz = tile % nbz, bx = tile / nbz;
+    const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
+    if (z >= a.g.NZ || x >= a.g.NX) return;
+    cell_s2d_acc<R, H, L>(a, z, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -379,64 +399,151 @@ __global__ void k_surface3d(FDArgs<R> a, int pdir) {
 // One lane walks all sources in order, so sources sharing a cell add up in
 // a fixed order.
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void k_sources(FDArgs<R> a, int nd_case, ShotDev s, int NT, int nt, int pdir) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <typename R, typename S>
+__device__ __forceinline__ R getS(const R *p, long i) {
+    return Store<R, S>::ld((const S *)p, (unsigned)(i * sizeof(S)));
+}
+template <typename R, typename S>
+__device__ __forceinline__ void putS(R *p, long i, R v) {
+    Store<R, S>::st((S *)p, (unsigned)(i * sizeof(S)), v);
+}
+
+// src holds each source already multiplied by the storage scale of the
+// field it feeds (1 unless wavefields are stored scaled).
+template <typename R, typename S>
+__device__ void sources_body(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT, int nt,
+                             int pdir) {
     for (int k = 0; k < s.nsrc; ++k) {
         const int cell = s.src_cell[k];
         if (cell < 0) continue;
         if (a.freesurf && pdir == -1 && s.src_k[k] < 2 * a.g.H) continue;
         const int type = s.src_type[k];
         const R amp = (R)pdir * ((R)a.DT * (R)s.src[(long)k * NT + nt]);
-        if (type == 0) a.vx[cell] += amp;
-        else if (type == 1) {
-            if (a.vy) a.vy[cell] += amp;
-        } else if (type == 2) a.vz[cell] += amp;
+        R *f = nullptr;
+        if (type == 0) f = a.vx;
+        else if (type == 1) f = a.vy;
+        else if (type == 2) f = a.vz;
         else if (type != 3 && type != 100) continue;
-        else if (nd_case == 22) a.p[cell] += amp;
-        else {
-            const double nd = nd_case == 3 ? 3.0 : 2.0;   // "+=amp/%f" (double)
-            a.sxx[cell] = (R)((double)a.sxx[cell] + (double)amp / nd);
-            a.szz[cell] = (R)((double)a.szz[cell] + (double)amp / nd);
-            if (nd_case == 3) a.syy[cell] = (R)((double)a.syy[cell] + (double)amp / nd);
+        else if (nd_case == 22) f = a.p;
+        if (f) {
+            putS<R, S>(f, cell, getS<R, S>(f, cell) + amp);
+            continue;
         }
+        const double nd = nd_case == 3 ? 3.0 : 2.0;   // "+=amp/%f" (double)
+        R *st[3] = {a.sxx, a.szz, nd_case == 3 ? a.syy : nullptr};
+        for (int q = 0; q < 3; ++q)
+            if (st[q]) putS<R, S>(st[q], cell, (R)((double)getS<R, S>(st[q], cell) + (double)amp / nd));
     }
+}
+
+template <typename R, typename S>
+__global__ void k_sources(FDArgs<R> a, int nd_case, ShotDev s, int NT, int nt, int pdir) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sources_body<R, S>(a, nd_case, s, NT, nt, pdir);
 }
 
 struct OutList {
     int n;
     int code[10];
+    float unscale[10];   // inverse storage scale of each output
 };
 
-template <typename R>
-__global__ void k_varsout(FDArgs<R> a, int nd_case, ShotDev s, OutList o, int nt) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= s.nrec) return;
+template <typename R, typename S>
+__device__ void varsout_body(const FDArgs<R> &a, int nd_case, const ShotDev &s, const OutList &o,
+                             int nt, int r) {
     const int cell = s.rec_cell[r];
     if (cell < 0) return;
     float *out = s.out + (long)nt * o.n * s.nrec;
     for (int q = 0; q < o.n; ++q) {
         R v;
         switch (o.code[q]) {
-            case O_VX: v = a.vx[cell]; break;
-            case O_VY: v = a.vy[cell]; break;
-            case O_VZ: v = a.vz[cell]; break;
-            case O_SXX: v = a.sxx[cell]; break;
-            case O_SYY: v = a.syy[cell]; break;
-            case O_SZZ: v = a.szz[cell]; break;
-            case O_SXZ: v = a.sxz[cell]; break;
-            case O_SXY: v = a.sxy[cell]; break;
-            case O_SYZ: v = a.syz[cell]; break;
+            case O_VX: v = getS<R, S>(a.vx, cell); break;
+            case O_VY: v = getS<R, S>(a.vy, cell); break;
+            case O_VZ: v = getS<R, S>(a.vz, cell); break;
+            case O_SXX: v = getS<R, S>(a.sxx, cell); break;
+            case O_SYY: v = getS<R, S>(a.syy, cell); break;
+            case O_SZZ: v = getS<R, S>(a.szz, cell); break;
+            case O_SXZ: v = getS<R, S>(a.sxz, cell); break;
+            case O_SXY: v = getS<R, S>(a.sxy, cell); break;
+            case O_SYZ: v = getS<R, S>(a.syz, cell); break;
             default:
-                if (nd_case == 22) v = a.p[cell];
+                if (nd_case == 22) v = getS<R, S>(a.p, cell);
                 else {
-                    R sm = a.sxx[cell] + a.szz[cell];
-                    if (nd_case == 3) sm = sm + a.syy[cell];
+                    R sm = getS<R, S>(a.sxx, cell) + getS<R, S>(a.szz, cell);
+                    if (nd_case == 3) sm = sm + getS<R, S>(a.syy, cell);
                     v = (R)((double)sm / (nd_case == 3 ? 3.0 : 2.0));
                 }
         }
-        out[(long)q * s.nrec + r] = (float)v;
+        out[(long)q * s.nrec + r] = (float)v * o.unscale[q];
     }
+}
+
+template <typename R, typename S>
+__global__ void k_varsout(FDArgs<R> a, int nd_case, ShotDev s, OutList o, int nt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.nrec) return;
+    varsout_body<R, S>(a, nd_case, s, o, nt, r);
+}
+
+// ---------------------------------------------------------------------------
+// Batched time steps (serving): every launch advances all active slots of a
+// batch, each slot a whole modeling job at its own time step.  The per-cell
+// arithmetic is the single-shot kernels' (the same device functions), so a
+// slot's results do not depend on what else is in the batch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int batch_find(const unsigned *tile0, int n, unsigned tile) {
+    int lo = 0, hi = n;           // largest k with tile0[k] <= tile
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tile0[mid] <= tile) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// KIND 0: elastic velocity, 1: elastic stress, 2: acoustic velocity, 3: acoustic pressure
+template <int H, int KIND>
+__global__ __launch_bounds__(256) void k_batch_update(const BatchSlot *slots, BatchClass c) {
+    unsigned tile;
+    xcd_tile(c.ntiles, tile);
+    if (tile >= c.ntiles) return;
+    const int k = batch_find(c.tile0, c.n, tile);
+    const BatchSlot &S = slots[c.ids[k]];
+    if (S.t >= S.tmax) return;
+    const unsigned lt = tile - c.tile0[k];
+    const int nbz = (S.a.g.NZ + BZ - 1) / BZ;
+    const int z = (lt % nbz) * BZ + threadIdx.x, x = (lt / nbz) * BT + threadIdx.y;
+    if (z >= S.a.g.NZ || x >= S.a.g.NX) return;
+    if constexpr (KIND == 0) cell_v2d<float, H>(S.a, z, x);
+    else if constexpr (KIND == 1) cell_s2d<float, H, 0>(S.a, z, x);
+    else if constexpr (KIND == 2) cell_v2d_acc<float, H>(S.a, z, x);
+    else cell_s2d_acc<float, H, 0>(S.a, z, x);
+}
+
+__global__ void k_batch_sources(const BatchSlot *slots, int n) {
+    if (threadIdx.x != 0 || (int)blockIdx.x >= n) return;
+    const BatchSlot &S = slots[blockIdx.x];
+    if (!S.active || S.t >= S.tmax) return;
+    sources_body<float, float>(S.a, S.nd_case, S.shot, S.NT, S.t, 1);
+}
+
+__global__ void k_batch_varsout(const BatchSlot *slots, int n) {
+    if ((int)blockIdx.y >= n) return;
+    const BatchSlot &S = slots[blockIdx.y];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!S.active || S.t >= S.tmax || r >= S.shot.nrec) return;
+    OutList o;
+    o.n = S.nout;
+    for (int q = 0; q < S.nout; ++q) {
+        o.code[q] = S.codes[q];
+        o.unscale[q] = 1.0f;
+    }
+    varsout_body<float, float>(S.a, S.nd_case, S.shot, o, S.t, r);
+}
+
+__global__ void k_batch_tick(BatchSlot *slots, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && slots[k].active && slots[k].t < slots[k].tmax) slots[k].t += 1;
 }
 
 __global__ void k_fill(uint32_t *p, size_t n) {
@@ -500,34 +607,69 @@ void dispatch_surface(const FDArgs<R> &a, int nd_case, int pdir, hipStream_t st)
 
 template <typename R>
 void launch_update_v(const FDArgs<R> &a, int nd_case, void *stream) {
-    if (nd_case == 3) return launch_v3d<R>(a, stream);
+    if (nd_case == 3) return launch_v3d<R, R>(a, stream);
     SEIS_SWITCH_H(a.g.H, (dispatch_v<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
 void launch_update_s(const FDArgs<R> &a, int nd_case, void *stream) {
-    if (nd_case == 3) return launch_s3d<R>(a, stream);
+    if (nd_case == 3) return launch_s3d<R, R>(a, stream);
     SEIS_SWITCH_H(a.g.H, (dispatch_s<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
 void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream) {
     SEIS_SWITCH_H(a.g.H, (dispatch_surface<R, HH>(a, nd_case, pdir, (hipStream_t)stream)));
 }
-template <typename R>
+template <typename R, typename S>
 void launch_sources(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT, int nt,
                     int pdir, void *stream) {
     if (s.nsrc <= 0) return;
-    hipLaunchKernelGGL((k_sources<R>), dim3(1), dim3(64), 0, (hipStream_t)stream, a, nd_case, s,
-                       NT, nt, pdir);
+    hipLaunchKernelGGL((k_sources<R, S>), dim3(1), dim3(64), 0, (hipStream_t)stream, a, nd_case,
+                       s, NT, nt, pdir);
 }
-template <typename R>
+template <typename R, typename S>
 void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int *codes,
-                    int nout, int nt, void *stream) {
+                    const float *unscale, int nout, int nt, void *stream) {
     if (s.nrec <= 0) return;
     OutList o;
     o.n = nout;
-    for (int q = 0; q < nout && q < 10; ++q) o.code[q] = codes[q];
-    hipLaunchKernelGGL((k_varsout<R>), dim3((s.nrec + 255) / 256), dim3(256), 0,
+    for (int q = 0; q < nout && q < 10; ++q) {
+        o.code[q] = codes[q];
+        o.unscale[q] = unscale ? unscale[q] : 1.0f;
+    }
+    hipLaunchKernelGGL((k_varsout<R, S>), dim3((s.nrec + 255) / 256), dim3(256), 0,
                        (hipStream_t)stream, a, nd_case, s, o, nt);
+}
+
+namespace {
+template <int H>
+void batch_update_launch(int kind, dim3 grid, dim3 blk, hipStream_t st, const BatchSlot *dslots,
+                         const BatchClass &c) {
+    switch (kind) {
+        case 0: hipLaunchKernelGGL((k_batch_update<H, 0>), grid, blk, 0, st, dslots, c); break;
+        case 1: hipLaunchKernelGGL((k_batch_update<H, 1>), grid, blk, 0, st, dslots, c); break;
+        case 2: hipLaunchKernelGGL((k_batch_update<H, 2>), grid, blk, 0, st, dslots, c); break;
+        default: hipLaunchKernelGGL((k_batch_update<H, 3>), grid, blk, 0, st, dslots, c); break;
+    }
+}
+}  // namespace
+
+void launch_batch_tick(BatchSlot *dslots, int nslots, const BatchClass *classes, int nclass,
+                       int maxrec, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (nslots <= 0) return;
+    hipLaunchKernelGGL(k_batch_sources, dim3(nslots), dim3(64), 0, st, dslots, nslots);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int ci = 0; ci < nclass; ++ci) {
+            const BatchClass &c = classes[ci];
+            if (c.n == 0 || c.ntiles == 0) continue;
+            const dim3 grid(((c.ntiles + 7) / 8) * 8), blk(BZ, BT);
+            const int kind = (c.nd_case == 22 ? 2 : 0) + pass;
+            SEIS_SWITCH_H(c.H, (batch_update_launch<HH>(kind, grid, blk, st, dslots, c)));
+        }
+    if (maxrec > 0)
+        hipLaunchKernelGGL(k_batch_varsout, dim3((maxrec + 255) / 256, nslots), dim3(256), 0, st,
+                           (const BatchSlot *)dslots, nslots);
+    hipLaunchKernelGGL(k_batch_tick, dim3((nslots + 63) / 64), dim3(64), 0, st, dslots, nslots);
 }
 
 void launch_fill(void *ptr, size_t bytes, void *stream) {
@@ -541,7 +683,17 @@ void launch_fill(void *ptr, size_t bytes, void *stream) {
 template void launch_update_v<float>(const FDArgs<float> &, int, void *);
 template void launch_update_s<float>(const FDArgs<float> &, int, void *);
 template void launch_surface<float>(const FDArgs<float> &, int, int, void *);
-template void launch_sources<float>(const FDArgs<float> &, int, const ShotDev &, int, int, int, void *);
-template void launch_varsout<float>(const FDArgs<float> &, int, const ShotDev &, const int *, int, int, void *);
+template void launch_sources<float, float>(const FDArgs<float> &, int, const ShotDev &, int, int,
+                                          int, void *);
+template void launch_sources<float, __half>(const FDArgs<float> &, int, const ShotDev &, int, int,
+                                           int, void *);
+template void launch_sources<double, double>(const FDArgs<double> &, int, const ShotDev &, int,
+                                            int, int, void *);
+template void launch_varsout<float, float>(const FDArgs<float> &, int, const ShotDev &,
+                                          const int *, const float *, int, int, void *);
+template void launch_varsout<float, __half>(const FDArgs<float> &, int, const ShotDev &,
+                                           const int *, const float *, int, int, void *);
+template void launch_varsout<double, double>(const FDArgs<double> &, int, const ShotDev &,
+                                            const int *, const float *, int, int, void *);
 
 }  // namespace seis

@@ -121,20 +121,43 @@ template <typename R>
 void launch_update_s(const FDArgs<R> &a, int nd_case, void *stream);
 template <typename R>
 void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream);
-template <typename R>
+// S: storage type of wavefields and memory variables (float, __half or
+// double); R: compute and coefficient type.
+template <typename R, typename S>
 void launch_sources(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT,
                     int nt, int pdir, void *stream);
-template <typename R>
+template <typename R, typename S>
 void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s,
-                    const int *out_codes, int nout, int nt, void *stream);
+                    const int *out_codes, const float *unscale, int nout, int nt,
+                    void *stream);
 void launch_fill(void *ptr, size_t bytes, void *stream);
-template <typename R>
+template <typename R, typename S>
 void launch_v3d(const FDArgs<R> &a, void *stream);
-template <typename R>
+template <typename R, typename S>
 void launch_s3d(const FDArgs<R> &a, void *stream);
 
 // Output codes used by the receiver kernel.
 enum OutCode { O_VX, O_VY, O_VZ, O_SXX, O_SYY, O_SZZ, O_SXZ, O_SXY, O_SYZ, O_P };
+
+// ---------------------------------------------------------------------------
+// Batched serving: one slot per modeling job (2D, L=0, FP32, no free surface).
+// ---------------------------------------------------------------------------
+struct BatchSlot {
+    FDArgs<float> a;
+    ShotDev shot;
+    int active, nd_case, NT, tmax, t;   // t: the next step (advanced on the device)
+    int nout, codes[10];
+};
+// the slots of one kernel class (acoustic or elastic, FD order), with the
+// prefix sum of their tile counts
+struct BatchClass {
+    int nd_case, H, n;
+    unsigned ntiles;
+    const int *ids;          // [n] slot indices (device)
+    const unsigned *tile0;   // [n+1] first tile of each slot (device)
+};
+void launch_batch_tick(BatchSlot *dslots, int nslots, const BatchClass *classes, int nclass,
+                       int maxrec, void *stream);
 
 // ---------------------------------------------------------------------------
 // Gradient by backpropagation (BACK_PROP_TYPE=1).

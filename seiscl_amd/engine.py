@@ -268,3 +268,48 @@ class Model:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Modeling jobs stepped together (seis_batch): one launch per kernel
+    class and time step advances every job in the batch."""
+
+    def __init__(self, ctx: Context, max_slots: int = 64):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        check(self.lib.seis_batch_create(ctx.handle, int(max_slots), ctypes.byref(h)))
+        self.handle = h
+        self.max_slots = int(max_slots)
+
+    def add(self, model: Model) -> int:
+        """Add a model whose shot is bound (Model.bind_shot); returns its slot."""
+        slot = ctypes.c_int()
+        check(self.lib.seis_batch_add(self.handle, model.handle, ctypes.byref(slot)))
+        return slot.value
+
+    def remove(self, slot: int):
+        check(self.lib.seis_batch_remove(self.handle, int(slot)))
+
+    def step(self, nsteps: int):
+        check(self.lib.seis_batch_step(self.handle, int(nsteps)))
+
+    def status(self):
+        """(jobs still stepping, fewest steps any of them has left)"""
+        n, m = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.seis_batch_status(self.handle, ctypes.byref(n), ctypes.byref(m)))
+        return n.value, m.value
+
+    def left(self, slot: int) -> int:
+        return int(self.lib.seis_batch_slot_left(self.handle, int(slot)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.seis_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

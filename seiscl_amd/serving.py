@@ -1,0 +1,140 @@
+"""Serving: continuous batching of modeling requests on one GPU.
+
+A request is one 2D modeling job (model, shot, constants).  Requests arrive
+over time; the server admits them into a batch (seis_batch) while slots and
+the HBM budget allow, advances every admitted job together -- one launch per
+kernel class and time step for the whole batch -- and returns each job's
+seismograms as soon as it has run its last step, freeing its slot for the
+next arrival.  The reference has no counterpart: it models one shot after
+another (src/time_stepping.c:747), which is the plain path timed here for
+comparison (one model, one launch sequence per shot, first come first served).
+
+Latency is arrival -> result on the host; throughput is shots per hour over
+the run.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import Batch, Context, Model
+
+
+@dataclass
+class Request:
+    rid: int
+    csts: dict
+    params: dict
+    shot: tuple
+    arrival: float = 0.0          # seconds after the start of the run
+    start: float = -1.0           # admitted
+    finish: float = -1.0          # result on the host
+    out: list = field(default_factory=list)
+
+    def cell_updates(self) -> int:
+        return int(np.prod(self.csts["N"])) * int(self.csts["NT"])
+
+
+def estimate_bytes(csts: dict) -> int:
+    """HBM a request needs (wavefields, coefficients, CPML memory, shot
+    buffers), from the sizes alone (the model's own footprint is known only
+    once it exists)."""
+    NZ, NX = (int(v) for v in csts["N"])
+    H = int(csts["FDORDER"]) // 2
+    cells = (NZ + 2 * H + 64) * (NX + 2 * H)
+    nf = 3 if csts["ND"] == 22 else 5
+    npar = 3 if csts["ND"] == 22 else 5
+    nab = int(csts.get("nab", 16))
+    psi = 8 * 2 * nab * (NZ + NX)
+    return 4 * (cells * (nf + npar) + psi) + 8 * int(csts["NT"]) * (NX // 2 + 8)
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 40) -> np.ndarray:
+    """Arrival times of a Poisson process of the given rate (per second)."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n)) - 0.0
+
+
+def _wait_until(t0: float, t: float):
+    dt = t - (time.perf_counter() - t0)
+    if dt > 0:
+        time.sleep(dt)
+
+
+def serve_plain(ctx: Context, requests):
+    """One job at a time in arrival order (the reference's execution model)."""
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda q: q.arrival):
+        _wait_until(t0, r.arrival)
+        r.start = time.perf_counter() - t0
+        m = Model(ctx, r.params, **r.csts)
+        r.out = m.forward(*r.shot)
+        m.close()
+        r.finish = time.perf_counter() - t0
+    return requests
+
+
+def serve_batched(ctx: Context, requests, max_slots: int = 32, hbm_budget: float = 64e9,
+                  chunk: int = 64):
+    """Continuous batching: admit arrivals while slots and the HBM budget
+    allow, step the batch, hand back finished jobs at once."""
+    batch = Batch(ctx, max_slots)
+    pending = deque(sorted(requests, key=lambda q: q.arrival))
+    waiting = deque()
+    active = {}                     # slot -> (request, model, bytes)
+    used = 0.0
+    t0 = time.perf_counter()
+    try:
+        while pending or waiting or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            while waiting and len(active) < max_slots:
+                need = estimate_bytes(waiting[0].csts)
+                if active and used + need > hbm_budget:
+                    break
+                r = waiting.popleft()
+                r.start = time.perf_counter() - t0
+                m = Model(ctx, r.params, **r.csts)
+                m.bind_shot(*r.shot)
+                active[batch.add(m)] = (r, m, need)
+                used += need
+            if not active:
+                if pending:
+                    _wait_until(t0, pending[0].arrival)
+                continue
+            _, min_left = batch.status()
+            batch.step(max(1, min(min_left, chunk)))
+            for slot in [s for s in active if batch.left(s) == 0]:
+                r, m, need = active.pop(slot)
+                r.out = m.read_shot()          # waits for the batch's queued steps
+                r.finish = time.perf_counter() - t0
+                batch.remove(slot)
+                m.close()
+                used -= need
+    finally:
+        for r, m, _ in active.values():
+            m.close()
+        batch.close()
+    return requests
+
+
+def summarize(requests) -> dict:
+    lat = np.array([r.finish - r.arrival for r in requests])
+    span = max(r.finish for r in requests) - min(r.arrival for r in requests)
+    cu = sum(r.cell_updates() for r in requests)
+    return dict(shots_per_hour=len(requests) / span * 3600.0, span_s=span,
+                p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)),
+                gcell_s=cu / span / 1e9)
+
+
+def make_requests(workload, arrivals=None):
+    reqs = []
+    for i, (csts, params, shot) in enumerate(workload):
+        reqs.append(Request(i, csts, params, shot,
+                            arrival=float(arrivals[i]) if arrivals is not None else 0.0))
+    return reqs

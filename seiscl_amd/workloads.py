@@ -116,3 +116,45 @@ def config2(n: int = 256, NT: int = 1500, seed: int = 2, nrec_side: int = 64):
     params = dict(vp=vp.astype(np.float32), vs=vs.astype(np.float32),
                   rho=rho.astype(np.float32), taup=taup, taus=taus)
     return csts, params, [(src_pos, src, rec_pos)]
+
+
+def layered_model(shape, rng, nlayers: int = 6, vmin: float = 1800.0, vmax: float = 4000.0,
+                  elastic: bool = True):
+    """Seeded layered model: vp from vmin to vmax with depth, vs = vp/1.8,
+    rho = 1800 + 800 (vp - 1800) / 2700."""
+    vp = layered(shape, nlayers, vmin, vmax, rng)
+    rho = 1800 + 800 * (vp - 1800) / 2700
+    p = dict(vp=vp.astype(np.float32), rho=rho.astype(np.float32))
+    if elastic:
+        p["vs"] = (vp / 1.8).astype(np.float32)
+    return p
+
+
+def config4_request(rng, nab: int = 16, NZ=None, NX=None, NT=None):
+    """One serving request: 2D acoustic (ND 22) or elastic (ND 2) modeling,
+    NZ ~ U{100..600} even, NX ~ U{200..1500}, NT ~ U{1000..5000}, FD 4 or
+    8, dh 10 m, dt 1 ms, Ricker 5-20 Hz, layered model, one source (acoustic
+    type 3, elastic explosion) and receivers every 2 cells at z = nab+2."""
+    NZ = NZ or 2 * int(rng.integers(50, 301))
+    NX = NX or int(rng.integers(200, 1501))
+    NT = NT or int(rng.integers(1000, 5001))
+    fd = int(rng.choice([4, 8]))
+    ND = int(rng.choice([22, 2]))
+    f0 = float(rng.uniform(5.0, 20.0))
+    dh, dt = 10.0, 1e-3
+    params = layered_model((NZ, NX), rng, elastic=ND == 2)
+    csts = dict(N=np.array([NZ, NX]), ND=ND, dh=dh, dt=dt, NT=NT, FDORDER=fd, MAXRELERROR=1,
+                abs_type=1, nab=nab, freesurf=0, L=0, f0=f0, seisout=2 if ND == 22 else 1)
+    sx = float(rng.integers(nab + 5, NX - nab - 5)) * dh
+    src_pos = np.array([[sx], [0.0], [(nab + 2) * dh], [0.0], [3.0 if ND == 22 else 100.0]])
+    src = ricker(f0, NT, dt)[:, None]
+    gx = np.arange(nab, NX - nab, 2) * dh
+    rec_pos = np.stack([gx, 0 * gx, 0 * gx + (nab + 2) * dh, 0 * gx, np.arange(gx.size) + 1.0,
+                        0 * gx, 0 * gx, 0 * gx])
+    return csts, params, (src_pos, src.astype(np.float32), rec_pos)
+
+
+def config4(n: int = 256, seed: int = 4, **kw):
+    """The serving workload: n seeded requests (config4_request)."""
+    rng = np.random.default_rng(seed)
+    return [config4_request(rng, **kw) for _ in range(n)]

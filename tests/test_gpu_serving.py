@@ -1,0 +1,74 @@
+"""Batched serving on the GPU: every job's result is identical whether it
+runs alone (the plain per-shot path, itself pinned against the oracle by
+the forward parity tests) or inside a batch with other jobs of different
+sizes, lengths, physics and FD orders; jobs join and leave the batch while
+others are running."""
+import numpy as np
+import pytest
+
+from cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, seed=9):
+    from seiscl_amd.workloads import config4_request
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n):
+        NZ = int(rng.choice([60, 84, 100]))
+        NX = int(rng.choice([70, 96, 130]))
+        NT = int(rng.choice([90, 140, 200]))
+        reqs.append(config4_request(rng, nab=10, NZ=NZ, NX=NX, NT=NT))
+    return reqs
+
+
+def test_batch_equals_alone(oracle_lib, gpu_ctx):
+    from seiscl_amd import Batch, Model
+    reqs = _requests(6)
+    kinds = {(c["ND"], c["FDORDER"]) for c, _, _ in reqs}
+    print(f"[serving] kinds={sorted(kinds)}", flush=True)
+    alone = []
+    for csts, params, shot in reqs:
+        m = Model(gpu_ctx, params, **csts)
+        alone.append(m.forward(*shot))
+        m.close()
+    # one request against the oracle as well
+    ref = oracle_lib.forward(reqs[0][1], *reqs[0][2], **reqs[0][0])
+    for r, o in zip(ref, alone[0]):
+        assert rel_l2(o, r) <= 1e-6
+    batch = Batch(gpu_ctx, 8)
+    models, slots = [], []
+    for csts, params, shot in reqs:
+        m = Model(gpu_ctx, params, **csts)
+        m.bind_shot(*shot)
+        slots.append(batch.add(m))
+        models.append(m)
+    while batch.status()[0]:
+        batch.step(max(1, min(batch.status()[1], 37)))
+    for m, a in zip(models, alone):
+        for x, y in zip(m.read_shot(), a):
+            np.testing.assert_array_equal(x, y)
+    for s, m in zip(slots, models):
+        batch.remove(s)
+        m.close()
+    batch.close()
+
+
+def test_continuous_batching_staggered_arrivals(gpu_ctx):
+    from seiscl_amd import Model
+    from seiscl_amd.serving import make_requests, serve_batched, summarize
+    work = _requests(10, seed=3)
+    alone = []
+    for csts, params, shot in work:
+        m = Model(gpu_ctx, params, **csts)
+        alone.append(m.forward(*shot))
+        m.close()
+    arrivals = np.linspace(0.0, 0.05, len(work))
+    reqs = serve_batched(gpu_ctx, make_requests(work, arrivals), max_slots=4, chunk=25)
+    for r, a in zip(sorted(reqs, key=lambda q: q.rid), alone):
+        assert r.finish >= r.arrival
+        for x, y in zip(r.out, a):
+            np.testing.assert_array_equal(x, y)
+    s = summarize(reqs)
+    assert s["shots_per_hour"] > 0 and s["p99_latency_s"] >= s["p50_latency_s"]
