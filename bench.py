@@ -75,8 +75,57 @@ def cpu_baseline(csts, params, shot, steps, threads):
                         "C oracle with OpenMP"))
 
 
+def serving_main(args):
+    """Serving workload: seeded 2D acoustic/elastic requests of uneven sizes
+    and lengths, Poisson arrivals at 0.5x, 1x and 2x the plain path's
+    measured capacity; continuous batching against one shot at a time."""
+    from seiscl_amd import Context
+    from seiscl_amd.serving import (make_requests, poisson_arrivals, serve_batched,
+                                    serve_plain, summarize)
+    from seiscl_amd.workloads import config4
+    work = config4(args.requests)
+    ctx = Context(0)
+    # warm-up: one small job of each kind compiles nothing but touches every
+    # kernel once
+    serve_plain(ctx, make_requests(work[:4]))
+    cap_plain = summarize(serve_plain(ctx, make_requests(work)))
+    cap_batch = summarize(serve_batched(ctx, make_requests(work), max_slots=args.slots))
+    log(f"capacity: plain {cap_plain['shots_per_hour']:.0f} shots/h, "
+        f"batched {cap_batch['shots_per_hour']:.0f} shots/h")
+    rate0 = cap_plain["shots_per_hour"] / 3600.0
+    loads = {}
+    for f in (0.5, 1.0, 2.0):
+        arr = poisson_arrivals(len(work), f * rate0)
+        p = summarize(serve_plain(ctx, make_requests(work, arr)))
+        b = summarize(serve_batched(ctx, make_requests(work, arr), max_slots=args.slots))
+        loads[f"{f}x"] = {"plain": p, "batched": b}
+        log(f"load {f}x: plain {p['shots_per_hour']:.0f}/h p99 {p['p99_latency_s']:.2f} s; "
+            f"batched {b['shots_per_hour']:.0f}/h p99 {b['p99_latency_s']:.2f} s")
+    ctx.close()
+    value = cap_batch["shots_per_hour"]
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "shots/h", "n_gpus": 1,
+        "steps": len(work), "warmup": 4,
+        "ms_per_step": round(cap_batch["span_s"] / len(work) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"serving: {len(work)} seeded 2D acoustic/elastic modeling requests, "
+                         "NZ 100-600, NX 200-1500, NT 1000-5000, FD 4/8, Poisson arrivals"),
+            "max_slots": args.slots,
+            "capacity_plain": cap_plain, "capacity_batched": cap_batch,
+            "speedup_capacity": cap_batch["shots_per_hour"] / cap_plain["shots_per_hour"],
+            "loads": loads,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["forward3d", "serving"], default="forward3d")
+    ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
+    ap.add_argument("--slots", type=int, default=48, help="serving: batch slots")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--warmup", type=int, default=10)
@@ -93,6 +142,9 @@ def main():
               f"one MI355X (requested --gpus {args.gpus}); multi-GPU runs are not part "
               "of this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "serving":
+        return serving_main(args)
 
     from seiscl_amd import Context, Model
     from seiscl_amd.workloads import config2

@@ -60,8 +60,10 @@ typedef struct seis_csts {
                         /*  2 the adjoint run has its own zeroed CPML memory (the */
                         /*    reference's adjoint kernels bind the forward one);  */
                         /*  4 the direct density term is scaled by dh^2/dt^2      */
-                        /*    (src/calc_grad.c:1005-1013 omits it).               */
-                        /* seis_csts_default sets 7; 0 reproduces the reference.  */
+                        /*    (src/calc_grad.c:1005-1013 omits it);               */
+                        /*  8 the 3D adjoint free surface reads syy (the          */
+                        /*    reference reads sxx twice, src/surface3D_adj.cl:69).*/
+                        /* seis_csts_default sets 15; 0 reproduces the reference. */
 } seis_csts;
 
 typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
@@ -141,7 +143,9 @@ int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos,
 int seis_read_field(seis_model *m, const char *name, float *host);
 
 /* ---- gradient by backpropagation (BACK_PROP_TYPE=1), elastic ND 2/3,      */
-/* L=0, no free surface.  The forward run saves an FDOH-thick boundary shell */
+/* L=0, FP32, with or without a free surface (adjoint surface:               */
+/* src/surface2D_adj.cl, src/surface3D_adj.cl).  The forward run saves an    */
+/* FDOH-thick boundary shell */
 /* of every wavefield at every step in device memory (save_bnd,              */
 /* src/time_stepping.c:352-395, kept resident instead of the reference's     */
 /* pinned host copy); the adjoint run re-injects it while running the        */

@@ -241,8 +241,8 @@ def check_stability(params: dict, **csts):
     return bool(bad), float(dts.value)
 
 
-FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE = 1, 2, 4
-FIX_ALL = 7
+FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE, FIX_SURF_SYY = 1, 2, 4, 8
+FIX_ALL = 15
 
 
 def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FIX_ALL,

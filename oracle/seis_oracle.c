@@ -2112,6 +2112,94 @@ static void grad_Mmu(const sim *S, const sim *A, double *gM, double *gmu) {
             }
 }
 
+/* Adjoint of the image-method free surface, applied to the adjoint fields */
+/* at the start of each adjoint step (src/surface2D_adj.cl:35-148,          */
+/* src/surface3D_adj.cl:29-180), elastic.  The reference scatters into the  */
+/* neighbouring columns' velocities; run column by column in order, as     */
+/* here, that is deterministic.  The 3D kernel reads sxxr where syyr is     */
+/* meant (src/surface3D_adj.cl:69); syy_fix reads syyr.                      */
+static void surface_adj2D(sim *A) {
+    const int FDOH = A->FDOH, NZ = A->NZp, NX = A->NXp;
+    const float *hc = &A->hc[1];          /* {HC1..HC6} */
+    const int gidz = FDOH;
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++) {
+        long indp = (long)(gidx - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+        long iv = (long)gidx * NZ + gidz;
+        float lszz = A->szz[iv], lsxx = A->sxx[iv];
+        float lmu = A->mu[indp], lM = A->M[indp];
+        {
+            float temp1 = lsxx, temp2 = lszz;
+            float a = (float)1.0 / ((float)4.0 * lM * lmu - (float)4.0 * lmu * lmu);
+            lsxx = -a * (lM * temp1 + (-lM + (float)2.0 * lmu) * temp2);
+            lszz = -a * (lM * temp2 + (-lM + (float)2.0 * lmu) * temp1);
+        }
+        lszz = 0.0;
+        float hx = -((lM - lmu * (float)2.0) * (lM - lmu * (float)2.0) / lM);
+        float hz = -(lM - lmu * (float)2.0);
+        for (int m = 0; m < FDOH; m++) {
+            if ((gidx - FDOH + m) < (NX - 2 * FDOH)) {
+                long ip = (long)(gidx + m - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+                A->vx[iv + (long)m * NZ] += (hc[m] * hx * lsxx * A->rip[ip]);
+            }
+            if ((gidx - (m + 1)) > (FDOH - 1)) {
+                long ip = (long)(gidx - (m + 1) - FDOH) * (NZ - 2 * FDOH) + (gidz - FDOH);
+                A->vx[iv - (long)(m + 1) * NZ] += (-hc[m] * hx * lsxx * A->rip[ip]);
+            }
+            A->vz[iv + m] += (hc[m] * hz * lsxx * A->rkp[indp + m]);
+        }
+        A->szz[iv] = -(lM * lszz + (lM - (float)2.0 * lmu) * lsxx);
+        A->sxx[iv] = -(lM * lsxx + (lM - (float)2.0 * lmu) * lszz);
+    }
+}
+
+static void surface_adj3D(sim *A, int syy_fix) {
+    const int FDOH = A->FDOH, NZ = A->NZp, NY = A->NYp, NX = A->NXp;
+    const float *hc = &A->hc[1];
+    const long nzi = NZ - 2 * FDOH, nyi = NY - 2 * FDOH;
+    const int gidz = FDOH;
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
+        for (int gidy = FDOH; gidy < NY - FDOH; gidy++) {
+#define IV(z, y, x) ((long)(x) * NY * NZ + (long)(y) * NZ + (z))
+#define IP(z, y, x) ((long)((x) - FDOH) * nyi * nzi + (long)((y) - FDOH) * nzi + ((z) - FDOH))
+            long iv = IV(gidz, gidy, gidx);
+            float lszz = -A->szz[iv];
+            float lsxx = -A->sxx[iv];
+            float lsyy = syy_fix ? -A->syy[iv] : -A->sxx[iv];
+            float lmu = A->mu[IP(gidz, gidy, gidx)], lM = A->M[IP(gidz, gidy, gidx)];
+            {
+                float temp1 = lsxx, temp2 = lsyy, temp3 = lszz;
+                float a = (float)1.0 / ((float)6.0 * lM * lmu - (float)8.0 * lmu * lmu);
+                lsxx = a * ((float)2.0 * (lM - lmu) * temp1 + (-lM + (float)2.0 * lmu) * (temp2 + temp3));
+                lsyy = a * ((float)2.0 * (lM - lmu) * temp2 + (-lM + (float)2.0 * lmu) * (temp1 + temp3));
+                lszz = a * ((float)2.0 * (lM - lmu) * temp3 + (-lM + (float)2.0 * lmu) * (temp1 + temp2));
+            }
+            lszz = 0.0;
+            float h = -((lM - lmu * (float)2.0) * (lM - lmu * (float)2.0) / lM);
+            float hz = -(lM - lmu * (float)2.0);
+            for (int m = 0; m < FDOH; m++) {
+                if ((gidx - FDOH + m) < (NX - 2 * FDOH))
+                    A->vx[IV(gidz, gidy, gidx + m)] +=
+                        (hc[m] * h * (lsxx + lsyy) * A->rip[IP(gidz, gidy, gidx + m)]);
+                if ((gidx - (m + 1)) > (FDOH - 1))
+                    A->vx[IV(gidz, gidy, gidx - (m + 1))] +=
+                        (-hc[m] * h * (lsxx + lsyy) * A->rip[IP(gidz, gidy, gidx - (m + 1))]);
+                if ((gidy - FDOH + m) < (NY - 2 * FDOH))
+                    A->vy[IV(gidz, gidy + m, gidx)] +=
+                        (hc[m] * h * (lsxx + lsyy) * A->rjp[IP(gidz, gidy + m, gidx)]);
+                if ((gidy - (m + 1)) > (FDOH - 1))
+                    A->vy[IV(gidz, gidy - (m + 1), gidx)] +=
+                        (-hc[m] * h * (lsxx + lsyy) * A->rjp[IP(gidz, gidy - (m + 1), gidx)]);
+                A->vz[IV(gidz + m, gidy, gidx)] +=
+                    (hc[m] * hz * (lsxx + lsyy) * A->rkp[IP(gidz + m, gidy, gidx)]);
+            }
+            A->sxx[iv] = -(lM * lsxx + (lM - (float)2.0 * lmu) * (lsyy + lszz));
+            A->syy[iv] = -(lM * lsyy + (lM - (float)2.0 * lmu) * (lsxx + lszz));
+            A->szz[iv] = -(lM * lszz + (lM - (float)2.0 * lmu) * (lsxx + lsyy));
+#undef IV
+#undef IP
+        }
+}
+
 /* Adjoint source injection: generated kernel of                           */
 /* src/automatic_kernels.c:631-836; p spreads res/ND over the normal        */
 /* stresses (integer literal ND, float division).                           */
@@ -2263,10 +2351,6 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         snprintf(err, errlen, "gradients need the elastic case (ND 2 or 3, L=0)");
         return 1;
     }
-    if (c->freesurf) {
-        snprintf(err, errlen, "gradients with a free surface are not supported");
-        return 1;
-    }
     if (sim_init(&S, c, err, errlen)) {
         sim_free(&S);
         return 1;
@@ -2311,6 +2395,10 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
             update_s2D(&S);
         }
         save_shell(&S, &B, bnd + (size_t)t * nvb * B.n);
+        if (c->freesurf) {
+            if (S.ND == 3) surface3D(&S, 1);
+            else surface2D(&S, 1);
+        }
         varsout(&S, t, nrec, rec_pos, out);
     }
     /* residuals and their scaling: src/time_stepping.c:838-848 */
@@ -2353,6 +2441,10 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
            *grho = calloc(S.np, sizeof(double));
     for (int t = tmax - 1; t > tmin; t--) {
         const float *bprev = bnd + (size_t)(t - 1) * nvb * B.n;
+        if (c->freesurf) {
+            if (S.ND == 3) surface_adj3D(&A, gradfix & OR_FIX_SURF_SYY);
+            else surface_adj2D(&A);
+        }
         residuals_inject(&A, t, nrec, rec_pos, res);
         /* update_adjs: backpropagate the stresses, re-inject their shell,  */
         /* update the adjoint stresses, accumulate the M and mu gradients    */

@@ -25,6 +25,7 @@ extern "C" {
 #define OR_FIX_RES_SCALE 1
 #define OR_FIX_ADJ_CPML 2
 #define OR_FIX_RHO_SCALE 4
+#define OR_FIX_SURF_SYY 8
 
 typedef struct or_csts {
     int ND;            /* 2: 2D P-SV elastic, 3: 3D, 22: 2D acoustic           */
@@ -92,8 +93,7 @@ double or_last_loop_seconds(void);
 int  or_shell_check(const or_csts *c, long *nbnd);
 
 /* Gradient of one shot by backpropagation with boundary saving
- * (src/time_stepping.c:747-1002, BACK_PROP_TYPE=1), elastic L=0 without a
- * free surface.  Adjoint sources: res_in [nout][nrec][NT] when given
+ * (src/time_stepping.c:747-1002, BACK_PROP_TYPE=1), elastic L=0.  Adjoint sources: res_in [nout][nrec][NT] when given
  * (inputres), else out - din; both are scaled as res_scale does.
  * gradfix selects corrections of reference defects (0 = as the reference):
  *   OR_FIX_RES_SCALE  res_scale reads the buoyancy at the receiver's own
@@ -101,7 +101,9 @@ int  or_shell_check(const or_csts *c, long *nbnd);
  *   OR_FIX_ADJ_CPML   the adjoint run starts from zeroed CPML memory (the
  *                     reference reuses the forward run's);
  *   OR_FIX_RHO_SCALE  the direct density term of the gradient is scaled by
- *                     dh^2/dt^2.
+ *                     dh^2/dt^2;
+ *   OR_FIX_SURF_SYY   the 3D adjoint free surface reads syy (the reference
+ *                     reads sxx twice).
  * grad: [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), gradients
  * accumulated in double.  res_out (optional): the scaled adjoint sources;
  * rms (optional): [rms, rmsnorm]; fields_out (optional): the backpropagated

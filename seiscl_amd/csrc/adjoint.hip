@@ -443,6 +443,93 @@ __global__ void k_residuals(FDArgs<R> a, int nd_case, ShotDev s, CodeList o, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the image-method free surface (src/surface2D_adj.cl:35-148,
+// src/surface3D_adj.cl:29-180), elastic.  The reference scatters from each
+// surface column into its neighbours' velocities; here a first pass
+// transforms every column's surface stresses (keeping what the scatter
+// needs) and a second gathers, for each column, its neighbours'
+// contributions in increasing neighbour order -- the order of a
+// column-by-column run, and free of write races.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_surf_adj_a(FDArgs<R> a, R *tmp, int nd, int syy_fix) {
+    const Layout &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = nd == 3 ? blockIdx.y : 0;
+    if (x >= g.NX) return;
+    const long i = g.idx(0, y, x);
+    const long col = (long)y * g.NX + x, nc = (long)g.NX * g.NY;
+    const R lmu = a.mu[i], lM = a.M[i];
+    if (nd == 2) {
+        R lsxx = a.sxx[i], lszz = a.szz[i];
+        const R temp1 = lsxx, temp2 = lszz;
+        const R aa = (R)1.0 / ((R)4.0 * lM * lmu - (R)4.0 * lmu * lmu);
+        lsxx = -aa * (lM * temp1 + (-lM + (R)2.0 * lmu) * temp2);
+        lszz = -aa * (lM * temp2 + (-lM + (R)2.0 * lmu) * temp1);
+        lszz = 0.0;
+        tmp[col] = -((lM - lmu * (R)2.0) * (lM - lmu * (R)2.0) / lM);   // hx
+        tmp[nc + col] = -(lM - lmu * (R)2.0);                          // hz
+        tmp[2 * nc + col] = lsxx;
+        a.szz[i] = -(lM * lszz + (lM - (R)2.0 * lmu) * lsxx);
+        a.sxx[i] = -(lM * lsxx + (lM - (R)2.0 * lmu) * lszz);
+        return;
+    }
+    R lszz = -a.szz[i], lsxx = -a.sxx[i], lsyy = syy_fix ? -a.syy[i] : -a.sxx[i];
+    const R temp1 = lsxx, temp2 = lsyy, temp3 = lszz;
+    const R aa = (R)1.0 / ((R)6.0 * lM * lmu - (R)8.0 * lmu * lmu);
+    lsxx = aa * ((R)2.0 * (lM - lmu) * temp1 + (-lM + (R)2.0 * lmu) * (temp2 + temp3));
+    lsyy = aa * ((R)2.0 * (lM - lmu) * temp2 + (-lM + (R)2.0 * lmu) * (temp1 + temp3));
+    lszz = aa * ((R)2.0 * (lM - lmu) * temp3 + (-lM + (R)2.0 * lmu) * (temp1 + temp2));
+    lszz = 0.0;
+    tmp[col] = -((lM - lmu * (R)2.0) * (lM - lmu * (R)2.0) / lM);       // h
+    tmp[nc + col] = -(lM - lmu * (R)2.0);                              // hz
+    tmp[2 * nc + col] = lsxx + lsyy;
+    a.sxx[i] = -(lM * lsxx + (lM - (R)2.0 * lmu) * (lsyy + lszz));
+    a.syy[i] = -(lM * lsyy + (lM - (R)2.0 * lmu) * (lsxx + lszz));
+    a.szz[i] = -(lM * lszz + (lM - (R)2.0 * lmu) * (lsxx + lsyy));
+}
+
+template <typename R, int H>
+__global__ void k_surf_adj_b(FDArgs<R> a, const R *tmp, int nd) {
+    const Layout &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = nd == 3 ? blockIdx.y : 0;
+    if (x >= g.NX) return;
+    const R *hc = a.hc + 1;                 // {HC1..HC6}
+    const long nc = (long)g.NX * g.NY;
+    const R *th = tmp, *thz = tmp + nc, *ts = tmp + 2 * nc;
+    const long i = g.idx(0, y, x);
+    // x neighbours (x' = x-H+1 .. x+H), then y neighbours in 3D
+    {
+        R v = a.vx[i];
+        const R rip = a.rip[i];
+#pragma unroll
+        for (int d = -(H - 1); d <= H; ++d) {
+            const int xs = x + d;
+            if (xs < 0 || xs >= g.NX) continue;
+            const long c = (long)y * g.NX + xs;
+            if (d <= 0) v += (hc[-d] * th[c] * ts[c] * rip);
+            else v += (-hc[d - 1] * th[c] * ts[c] * rip);
+        }
+        a.vx[i] = v;
+    }
+    if (nd == 3) {
+        R v = a.vy[i];
+        const R rjp = a.rjp[i];
+#pragma unroll
+        for (int d = -(H - 1); d <= H; ++d) {
+            const int ys = y + d;
+            if (ys < 0 || ys >= g.NY) continue;
+            const long c = (long)ys * g.NX + x;
+            if (d <= 0) v += (hc[-d] * th[c] * ts[c] * rjp);
+            else v += (-hc[d - 1] * th[c] * ts[c] * rjp);
+        }
+        a.vy[i] = v;
+    }
+    const long c = (long)y * g.NX + x;
+#pragma unroll
+    for (int m = 0; m < H; ++m) a.vz[i + m] += (hc[m] * thz[c] * ts[c] * a.rkp[i + m]);
+}
+
 template <typename R>
 void grid_adj(const Layout &g, int &nbz, unsigned &nb) {
     nbz = (g.NZ + BZ - 1) / BZ;
@@ -471,7 +558,20 @@ void run_adj_v(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
     else hipLaunchKernelGGL((k_adj_v2d<R, H>), grid, blk, 0, st, a, nbz, nb);
 }
 
+template <typename R, int H>
+void run_surf_adj(const FDArgs<R> &r, R *tmp, int nd, int syy_fix, hipStream_t st) {
+    const dim3 grid((r.g.NX + 63) / 64, nd == 3 ? r.g.NY : 1);
+    hipLaunchKernelGGL((k_surf_adj_a<R>), grid, dim3(64), 0, st, r, tmp, nd, syy_fix);
+    hipLaunchKernelGGL((k_surf_adj_b<R, H>), grid, dim3(64), 0, st, r, (const R *)tmp, nd);
+}
+
 }  // namespace
+
+template <typename R>
+void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, void *stream) {
+    const int nd = nd_case == 3 ? 3 : 2;
+    SEIS_SWITCH_H(r.g.H, (run_surf_adj<R, HH>(r, tmp, nd, syy_fix, (hipStream_t)stream)));
+}
 
 template <typename R>
 void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream) {
@@ -501,6 +601,7 @@ void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const i
 }
 
 template void launch_adj_s<float>(const AdjArgs<float> &, int, void *);
+template void launch_surface_adj<float>(const FDArgs<float> &, float *, int, int, void *);
 template void launch_adj_v<float>(const AdjArgs<float> &, int, void *);
 template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *);
 template void launch_residuals<float>(const FDArgs<float> &, int, const ShotDev &, const int *,

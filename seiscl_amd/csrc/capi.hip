@@ -106,6 +106,7 @@ struct seis_model {
         FDArgs<float> r{};          // adjoint wavefields (same model)
         double *gM = nullptr, *gmu = nullptr, *grho = nullptr;   // padded layout
         float *res = nullptr;       // [nout][nrec][NT] scaled adjoint sources
+        float *stmp = nullptr;      // adjoint free surface: 3 * NX * NY
         size_t res_cap = 0;
         std::vector<float *> own_psi;
         bool saved = false;         // a forward run with boundaries is bound
@@ -458,7 +459,6 @@ int grad_setup(seis_model *m) {
     const seis_csts &c = m->c;
     if (c.ND == 22 || c.L > 0)
         return fail("gradients need the elastic case (ND 2 or 3, L=0)");
-    if (c.freesurf) return fail("gradients with a free surface are not available in this build");
     if (m->prec != P_FP32) return fail("gradients are available with FP32 wavefields (FP16=0)");
     G.sh = make_shell(m->g, c.abs_type > 0 ? c.nab : 0, c.freesurf);
     if (G.sh.NXbnd < 2 * G.sh.F || G.sh.NZbnd < 2 * G.sh.F ||
@@ -488,6 +488,7 @@ int grad_setup(seis_model *m) {
         for (float **p : pz)
             if (*p && !(*p = m->dalloc(szs))) return 1;
     }
+    if (c.freesurf && !(G.stmp = m->dalloc(3L * m->g.NX * m->g.NY))) return 1;
     double **gd[] = {&G.gM, &G.gmu, &G.grho};
     for (double **p : gd)
         if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
@@ -573,7 +574,7 @@ void seis_csts_default(seis_csts *c) {
     c->f0 = 15;
     c->FL[0] = 15;
     c->seisout = 2;
-    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE;
+    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY;
 }
 
 int seis_ctx_create(int device_ordinal, seis_ctx **out) {
@@ -1040,13 +1041,19 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
     const size_t per = (size_t)G.nvb * G.sh.n;
     // forward step order with boundary saving after update_s
     // (src/time_stepping.c:785-816)
+    // the boundary is saved before the free surface is applied, so the
+    // surface runs as its own kernel here
+    const int fused = m->a.fuse_surface;
+    m->a.fuse_surface = 0;
     for (int t = 0; t < tmax; ++t) {
         step_sources(m, t, 1, st);
         step_v(m, st);
         step_s(m, st);
         launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (size_t)t * per, st);
+        if (m->c.freesurf) launch_surface(m->a, m->nd_case, 1, st);
         step_varsout(m, t, st);
     }
+    m->a.fuse_surface = fused;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     G.saved = true;
@@ -1096,6 +1103,8 @@ int seis_adjoint(seis_model *m, const float *res) {
     const size_t per = (size_t)G.nvb * G.sh.n;
     // adjoint loop of src/time_stepping.c:869-940
     for (int t = tmax - 1; t > tmin; --t) {
+        if (m->c.freesurf)
+            launch_surface_adj(A.r, G.stmp, m->nd_case, m->c.gradfix & FIX_SURF_SYY, st);
         launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t, serial,
                          st);
         A.bnd = G.bnd + (size_t)(t - 1) * per;

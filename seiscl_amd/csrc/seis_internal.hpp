@@ -163,7 +163,7 @@ void launch_batch_tick(BatchSlot *dslots, int nslots, const BatchClass *classes,
 // Gradient by backpropagation (BACK_PROP_TYPE=1).
 // ---------------------------------------------------------------------------
 // Corrections of reference defects, selected by seis_csts.gradfix.
-enum GradFix { FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4 };
+enum GradFix { FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8 };
 
 // Boundary shell saved during the forward run and re-injected while the
 // forward wavefield runs backwards: FDOH cells thick, on the inner face of
@@ -283,6 +283,9 @@ template <typename R>
 void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream);
 template <typename R>
 void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream);
+// adjoint free surface on the adjoint wavefields r; tmp: 3 * NX * NY reals
+template <typename R>
+void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, void *stream);
 // bnd[q][g] = field_q(shell cell g), q over the shell variables
 template <typename R>
 void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream);

@@ -21,6 +21,10 @@ CASES = (
     + [dict(ND=2, FDORDER=4, seisout=2), dict(ND=2, FDORDER=8, seisout=1, MAXRELERROR=0)]
     + [dict(ND=3, FDORDER=o, abs_type=a) for o in (2, 4, 8) for a in (1, 2)]
     + [dict(ND=3, FDORDER=4, gradfix=0), dict(ND=3, FDORDER=4, seisout=2)]
+    # free surface: adjoint image method, 3D with and without the syy fix
+    + [dict(ND=2, FDORDER=o, freesurf=1) for o in (4, 8)]
+    + [dict(ND=2, FDORDER=4, freesurf=1, abs_type=2)]
+    + [dict(ND=3, FDORDER=4, freesurf=1, gradfix=f) for f in (15, 7)]
 )
 
 
@@ -30,7 +34,7 @@ def _ids(c):
 
 def _setup(case):
     case = dict(case)
-    gradfix = case.pop("gradfix", 7)
+    gradfix = case.pop("gradfix", 15)
     nt = 160 if case.get("ND") == 3 else 220
     csts, params, shot = small_case(NT=nt, seed=11, **case)
     csts["gradfix"] = gradfix
