@@ -84,3 +84,23 @@ def test_autograd_module_needs_the_gpu():
     from seiscl_amd.autograd import Propagator, seismic  # noqa: F401
     with pytest.raises(Exception):
         Context(0)
+
+
+def test_seiscl_front_end_acquisition():
+    """The SeisCL-compatible class selects shots, receivers and output names
+    as SeisCL.py does (no GPU needed before execute)."""
+    from seiscl_amd.seiscl import SeisCL
+    seis = SeisCL(N=np.array([60, 120]), ND=2, nab=10, seisout=1, NT=300, dt=1e-3)
+    seis.surface_acquisition_2d(dg=4, ds=20)
+    assert seis.params == ["vp", "vs", "rho"]
+    assert seis.to_load_names == ["vx", "vz"]
+    nsrc = seis.src_pos_all.shape[1]
+    assert nsrc == len(range(12, 120 - 12, 20))
+    seis.prepare_data([1, 2])
+    assert set(seis.src_pos[3].astype(int)) == {1, 2}
+    assert set(seis.rec_pos[3].astype(int)) == {1, 2}
+    ids = np.where(np.isin(seis.rec_pos_all[3].astype(int), [1, 2]))[0]
+    np.testing.assert_array_equal(seis.rec_pos[4], ids + 1)
+    assert seis.src.shape == (300, 2)
+    seis.param_type, seis.L = 0, 1
+    assert seis.params == ["vp", "vs", "rho", "taup", "taus"]
