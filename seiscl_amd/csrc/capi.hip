@@ -431,13 +431,18 @@ void step_sources(seis_model *m, int t, int pdir, hipStream_t st) {
 }
 void step_v(seis_model *m, hipStream_t st) {
     if (m->prec == P_FP16) launch_v3d<float, __half>(m->a, st);
-    else if (m->prec == P_FP64) launch_v3d<double, double>(m->ad, st);
+    else if (m->prec == P_FP64) launch_update_v(m->ad, m->nd_case, st);
     else launch_update_v(m->a, m->nd_case, st);
 }
 void step_s(seis_model *m, hipStream_t st) {
     if (m->prec == P_FP16) launch_s3d<float, __half>(m->a, st);
-    else if (m->prec == P_FP64) launch_s3d<double, double>(m->ad, st);
+    else if (m->prec == P_FP64) launch_update_s(m->ad, m->nd_case, st);
     else launch_update_s(m->a, m->nd_case, st);
+}
+// the free surface as its own kernel (2D, and 3D runs that save boundaries)
+void step_surface(seis_model *m, hipStream_t st) {
+    if (m->prec == P_FP64) launch_surface(m->ad, m->nd_case, 1, st);
+    else launch_surface(m->a, m->nd_case, 1, st);
 }
 void step_varsout(seis_model *m, int t, hipStream_t st) {
     const int n = (int)m->out_codes.size();
@@ -631,8 +636,8 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     if (c.L < 0 || c.L > 4) return fail("L must be in [0, 4]");
     if (c.FP16 != P_FP32 && c.FP16 != P_SCALED && c.FP16 != P_FP16 && c.FP16 != P_FP64)
         return fail("FP16 must be 0 (FP32), 1 (FP32, scaled units), 2 (FP16 storage) or 64 (FP64)");
-    if (c.FP16 != P_FP32 && c.ND != 3)
-        return fail("FP16 storage and FP64 are available for ND=3 in this build");
+    if ((c.FP16 == P_FP16 || c.FP16 == P_SCALED) && c.ND != 3)
+        return fail("FP16 storage is available for ND=3 in this build");
     if (c.NT <= 0) return fail("NT must be positive");
     const int nd = c.ND == 3 ? 3 : 2;
     for (int d = 0; d < nd; ++d) {
@@ -738,11 +743,12 @@ int seis_model_set_params(seis_model *m, const float *p0, const float *p1, const
         if (!*p) continue;
         auto it = std::find(m->owned.begin(), m->owned.end(), (void *)*p);
         if (it != m->owned.end()) m->owned.erase(it);
-        m->bytes -= (size_t)m->g.nfield * 4;
+        m->bytes -= (size_t)m->g.nfield * m->celem;
         m->ctx->release((void *)*p);
         *p = nullptr;
     }
     if (upload_params(m, rp)) return 1;
+    if (m->prec == P_FP64) m->ad = as_double(m->a);
     m->rp = std::move(rp);
     return 0;
 }
@@ -888,7 +894,7 @@ int seis_launch_update_s(seis_model *m) {
 int seis_launch_surface(seis_model *m) {
     // with the surface fused into update_s there is nothing left to launch
     if (!m->c.freesurf || m->a.fuse_surface) return 0;
-    launch_surface(m->a, m->nd_case, 1, m->ctx->stream);
+    step_surface(m, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -933,7 +939,7 @@ int seis_run_steps(seis_model *m, int t0, int nsteps) {
         if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
         step_s(m, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
-        if (m->c.freesurf && !m->a.fuse_surface) launch_surface(m->a, m->nd_case, 1, st);
+        if (m->c.freesurf && !m->a.fuse_surface) step_surface(m, st);
         step_varsout(m, t, st);
     }
     HIP_TRY(hipEventRecord(e1, st));

@@ -683,6 +683,9 @@ void launch_fill(void *ptr, size_t bytes, void *stream) {
 template void launch_update_v<float>(const FDArgs<float> &, int, void *);
 template void launch_update_s<float>(const FDArgs<float> &, int, void *);
 template void launch_surface<float>(const FDArgs<float> &, int, int, void *);
+template void launch_update_v<double>(const FDArgs<double> &, int, void *);
+template void launch_update_s<double>(const FDArgs<double> &, int, void *);
+template void launch_surface<double>(const FDArgs<double> &, int, int, void *);
 template void launch_sources<float, float>(const FDArgs<float> &, int, const ShotDev &, int, int,
                                           int, void *);
 template void launch_sources<float, __half>(const FDArgs<float> &, int, const ShotDev &, int, int,

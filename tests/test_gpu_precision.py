@@ -58,3 +58,17 @@ def test_fp32_and_fp16_against_fp64(gpu_ctx, case):
         assert rel_l2(b, c) <= 1e-2
         assert rel_l2(b, c) > rel_l2(a, c)      # FP16 storage is really in use
     assert np.isfinite(f16["vz"]).all()
+
+
+@pytest.mark.parametrize("case", [dict(ND=2, FDORDER=4, freesurf=1), dict(ND=2, FDORDER=8, L=2),
+                                  dict(ND=22, FDORDER=4, abs_type=2), dict(ND=22, FDORDER=8, L=1)],
+                         ids=_ids)
+def test_fp64_2d(gpu_ctx, case):
+    """2D runs in FP64 (used for finite-difference checks of gradients):
+    the FP32 path agrees with them to 1e-5."""
+    csts, params, shot = small_case(NT=150, **case)
+    o64, _ = _run(gpu_ctx, csts, params, shot, 64)
+    o32, _ = _run(gpu_ctx, csts, params, shot, 0)
+    for a, c in zip(o32, o64):
+        assert np.abs(c).max() > 0
+        assert 0 < rel_l2(a, c) <= 1e-5
