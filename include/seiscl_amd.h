@@ -84,6 +84,10 @@ size_t seis_ctx_footprint(const seis_ctx *ctx);     /* device bytes allocated   
 
 /* ---- buffers: replace clbuf_create / clbuf_send / clbuf_read              */
 /* (src/clbuf.c:23-281).  Transfers are ordered on the compute stream.        */
+/* Freed device memory stays with the context and is reused by later         */
+/* allocations (no device-wide synchronisation while jobs come and go); it   */
+/* returns to the system at seis_ctx_destroy.  seis_ctx_footprint counts the */
+/* bytes in use.                                                              */
 int seis_buf_create(seis_ctx *ctx, size_t bytes, void **dptr);
 int seis_buf_free(seis_ctx *ctx, void *dptr);
 int seis_buf_send(seis_ctx *ctx, void *dptr, const void *host, size_t bytes);

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,27 +37,55 @@ int fail(const std::string &msg) {
 
 struct seis_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
-    size_t bytes = 0;
-    std::map<void *, size_t> allocs;
+    hipStream_t stream = nullptr;   // compute
+    hipStream_t xfer = nullptr;     // model set-up uploads and result copies
+    size_t bytes = 0;               // bytes handed out and not released
+    std::mutex mu;                  // models may be set up on another host thread
+    struct Block {
+        size_t cap;
+        hipEvent_t freed;           // compute-stream point after the last use
+    };
+    std::map<void *, size_t> allocs;          // live block -> requested bytes
+    std::map<void *, size_t> caps;            // every block -> capacity
+    std::multimap<size_t, std::pair<void *, hipEvent_t>> cache;   // released blocks
 
-    int alloc(size_t n, void **p) {
+    // Device memory, zero-filled on fill_stream (the compute stream by
+    // default).  Released blocks are kept and reused for requests of up to
+    // their size and at least half of it: no hipMalloc / hipFree (which
+    // synchronises the device) while jobs come and go.
+    int alloc(size_t n, void **p, hipStream_t fill_stream = nullptr) {
         *p = nullptr;
         if (n == 0) return 0;
+        hipStream_t fs = fill_stream ? fill_stream : stream;
+        std::lock_guard<std::mutex> lock(mu);
         HIP_TRY(hipSetDevice(device));
-        HIP_TRY(hipMalloc(p, n));
+        auto it = cache.lower_bound(n);
+        if (it != cache.end() && it->first <= 2 * n) {
+            *p = it->second.first;
+            // the block's previous user may still be queued on the compute stream
+            HIP_TRY(hipStreamWaitEvent(fs, it->second.second, 0));
+            (void)hipEventDestroy(it->second.second);
+            cache.erase(it);
+        } else {
+            HIP_TRY(hipMalloc(p, n));
+            caps[*p] = n;
+        }
         allocs[*p] = n;
         bytes += n;
-        launch_fill(*p, n, stream);
+        launch_fill(*p, n, fs);
         return 0;
     }
     int release(void *p) {
         if (!p) return 0;
+        std::lock_guard<std::mutex> lock(mu);
         auto it = allocs.find(p);
         if (it == allocs.end()) return fail("seis_buf_free: unknown pointer");
         bytes -= it->second;
         allocs.erase(it);
-        HIP_TRY(hipFree(p));
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(e, stream));
+        cache.emplace(caps[p], std::make_pair(p, e));
         return 0;
     }
 };
@@ -95,6 +124,8 @@ struct seis_model {
     double ms_v = 0, ms_s = 0, ms_loop = 0;
     // host copy of the transformed parameters (residual scaling, gradients)
     RefParams rp;
+    bool setting_up = false;         // buffers zero-filled on the transfer stream
+    hipEvent_t done_ev = nullptr;    // batch: compute-stream point after the last step
     std::vector<int> rcell_host;     // receiver cells of the bound shot
     // gradient state, allocated on first use
     struct Grad {
@@ -115,7 +146,8 @@ struct seis_model {
     float *dalloc(long n) {
         void *p = nullptr;
         if (n <= 0) return nullptr;
-        if (ctx->alloc((size_t)n * sizeof(float), &p)) return nullptr;
+        if (ctx->alloc((size_t)n * sizeof(float), &p, setting_up ? ctx->xfer : nullptr))
+            return nullptr;
         owned.push_back(p);
         bytes += (size_t)n * sizeof(float);
         return (float *)p;
@@ -147,10 +179,18 @@ struct seis_batch {
     std::vector<BatchClass> classes;
     int maxrec = 0;
     bool dirty = true;
-    // host sources of asynchronous copies (kept alive until the next sync)
+    // host sources of asynchronous copies
     std::vector<BatchSlot> host_slots;
-    std::vector<int> host_ids, host_active;
-    std::vector<unsigned> host_tile0;
+    std::vector<int> host_active;
+    // class tables, double buffered: a rebuild writes the buffer the queued
+    // launches are not reading, after the launches that last read it ended
+    std::vector<int> host_ids[2];
+    std::vector<unsigned> host_tile0[2];
+    hipEvent_t table_ev[2] = {nullptr, nullptr};
+    int tbuf = 0;
+    // completion points of the steps, reused round robin
+    std::vector<hipEvent_t> step_ev;
+    size_t step_ev_next = 0;
 };
 
 namespace {
@@ -174,14 +214,16 @@ int upload_raw(seis_model *m, const float **dst, const std::vector<float> &host,
     if (m->celem == 8) {
         std::vector<double> h(host.size());
         for (size_t i = 0; i < h.size(); ++i) h[i] = (double)host[i] * mul;
-        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, m->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        hipStream_t us = m->setting_up ? m->ctx->xfer : m->ctx->stream;
+        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, us));
+        HIP_TRY(hipStreamSynchronize(us));
     } else {
         std::vector<float> h(host);
         if (mul != 1)
             for (float &v : h) v *= mul;
-        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, m->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        hipStream_t us = m->setting_up ? m->ctx->xfer : m->ctx->stream;
+        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, us));
+        HIP_TRY(hipStreamSynchronize(us));
     }
     *dst = d;
     return 0;
@@ -592,6 +634,7 @@ int seis_ctx_create(int device_ordinal, seis_ctx **out) {
     ctx->device = device_ordinal;
     HIP_TRY(hipSetDevice(device_ordinal));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->xfer, hipStreamNonBlocking));
     *out = ctx.release();
     return 0;
 }
@@ -600,8 +643,11 @@ int seis_ctx_destroy(seis_ctx *ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
+    (void)hipStreamSynchronize(ctx->xfer);
+    for (auto &kv : ctx->caps) (void)hipFree(kv.first);
+    for (auto &kv : ctx->cache) (void)hipEventDestroy(kv.second.second);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->xfer);
     delete ctx;
     return 0;
 }
@@ -652,6 +698,9 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     auto m = std::make_unique<seis_model>();
     m->ctx = ctx;
     m->c = c;
+    // set-up traffic goes to the transfer stream, so a model can be built on
+    // another host thread while the compute stream runs other jobs
+    m->setting_up = true;
     m->nd_case = c.ND;
     std::string err;
     if (build_setup(c, m->s, err)) return fail(err);
@@ -709,14 +758,16 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     m->rp = std::move(rp);
     m->outn = out_names(c);
     for (auto &n : m->outn) m->out_codes.push_back(out_code(n));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->xfer));
+    m->setting_up = false;
     *out = m.release();
     return 0;
 }
 
 int seis_model_destroy(seis_model *m) {
     if (!m) return 0;
-    (void)hipStreamSynchronize(m->ctx->stream);
+    // buffers return to the context's cache behind a compute-stream event
+    (void)hipStreamSynchronize(m->ctx->xfer);
     delete m;
     return 0;
 }
@@ -796,28 +847,28 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
     const int nout = (int)m->outn.size();
     if (nsrc > m->shot_nsrc_cap || nrec > m->shot_nrec_cap || !sd.out) {
         void *olds[] = {sd.src_cell, sd.src_type, sd.src_k, sd.src, sd.rec_cell, sd.out};
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (void *p : olds) ctx->release(p);
         const int ns = std::max(nsrc, 1), nr = std::max(nrec, 1);
+        hipStream_t xf = ctx->xfer;
         void *p;
-        if (ctx->alloc(ns * 4, &p)) return 1;
+        if (ctx->alloc(ns * 4, &p, xf)) return 1;
         sd.src_cell = (int *)p;
-        if (ctx->alloc(ns * 4, &p)) return 1;
+        if (ctx->alloc(ns * 4, &p, xf)) return 1;
         sd.src_type = (int *)p;
-        if (ctx->alloc(ns * 4, &p)) return 1;
+        if (ctx->alloc(ns * 4, &p, xf)) return 1;
         sd.src_k = (int *)p;
-        if (ctx->alloc((size_t)ns * NT * 4, &p)) return 1;
+        if (ctx->alloc((size_t)ns * NT * 4, &p, xf)) return 1;
         sd.src = (float *)p;
-        if (ctx->alloc(nr * 4, &p)) return 1;
+        if (ctx->alloc(nr * 4, &p, xf)) return 1;
         sd.rec_cell = (int *)p;
-        if (ctx->alloc((size_t)nr * NT * nout * 4, &p)) return 1;
+        if (ctx->alloc((size_t)nr * NT * nout * 4, &p, xf)) return 1;
         sd.out = (float *)p;
         m->shot_nsrc_cap = ns;
         m->shot_nrec_cap = nr;
     }
     sd.nsrc = nsrc;
     sd.nrec = nrec;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = ctx->xfer;   // the host waits for these copies below
     if (nsrc) {
         HIP_TRY(hipMemcpyAsync(sd.src_cell, scell.data(), nsrc * 4, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(sd.src_type, stype.data(), nsrc * 4, hipMemcpyHostToDevice, st));
@@ -908,10 +959,16 @@ int seis_launch_varsout(seis_model *m, int nt) {
 int seis_shot_read(seis_model *m, float *out) {
     const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
     std::vector<float> tmp((size_t)NT * nout * nrec);
+    // a batch job is read on the transfer stream once its last step is
+    // done, while the batch keeps running on the compute stream
+    hipStream_t st = m->ctx->stream;
+    if (m->done_ev) {
+        st = m->ctx->xfer;
+        HIP_TRY(hipStreamWaitEvent(st, m->done_ev, 0));
+    }
     if (!tmp.empty())
-        HIP_TRY(hipMemcpyAsync(tmp.data(), m->shot.out, tmp.size() * 4, hipMemcpyDeviceToHost,
-                               m->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        HIP_TRY(hipMemcpyAsync(tmp.data(), m->shot.out, tmp.size() * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     for (int o = 0; o < nout; ++o)
         for (int r = 0; r < nrec; ++r)
             for (int t = 0; t < NT; ++t)
@@ -1184,10 +1241,13 @@ int seis_batch_create(seis_ctx *ctx, int max_slots, seis_batch **out) {
     void *p = nullptr;
     if (ctx->alloc(sizeof(BatchSlot) * max_slots, &p)) return 1;
     b->dslots = (BatchSlot *)p;
-    if (ctx->alloc(sizeof(int) * max_slots * 4, &p)) return 1;
+    if (ctx->alloc(sizeof(int) * max_slots * 2, &p)) return 1;
     b->dids = (int *)p;
-    if (ctx->alloc(sizeof(unsigned) * (max_slots + 1) * 4, &p)) return 1;
+    if (ctx->alloc(sizeof(unsigned) * (max_slots + 8) * 2, &p)) return 1;
     b->dtile0 = (unsigned *)p;
+    for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreateWithFlags(&b->table_ev[k], hipEventDisableTiming));
+    b->step_ev.resize(64);
+    for (auto &e : b->step_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     *out = b.release();
     return 0;
@@ -1196,6 +1256,10 @@ int seis_batch_create(seis_ctx *ctx, int max_slots, seis_batch **out) {
 int seis_batch_destroy(seis_batch *b) {
     if (!b) return 0;
     (void)hipStreamSynchronize(b->ctx->stream);
+    for (int i = 0; i < b->max_slots; ++i)
+        if (b->model[i]) b->model[i]->done_ev = nullptr;
+    for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->table_ev[k]);
+    for (auto e : b->step_ev) (void)hipEventDestroy(e);
     b->ctx->release(b->dslots);
     b->ctx->release(b->dids);
     b->ctx->release(b->dtile0);
@@ -1234,6 +1298,7 @@ int seis_batch_add(seis_batch *b, seis_model *m, int *slot) {
     HIP_TRY(hipMemcpyAsync(b->dslots + k, &b->host_slots[k], sizeof(sl), hipMemcpyHostToDevice,
                            b->ctx->stream));
     b->model[k] = m;
+    m->done_ev = nullptr;
     b->t[k] = 0;
     b->tmax[k] = sl.tmax;
     b->dirty = true;
@@ -1248,6 +1313,7 @@ int seis_batch_remove(seis_batch *b, int slot) {
     HIP_TRY(hipMemcpyAsync((char *)(b->dslots + slot) + offsetof(BatchSlot, active),
                            &b->host_active[slot], sizeof(int), hipMemcpyHostToDevice,
                            b->ctx->stream));
+    b->model[slot]->done_ev = nullptr;
     b->model[slot] = nullptr;
     b->dirty = true;
     return 0;
@@ -1285,10 +1351,13 @@ int seis_batch_step(seis_batch *b, int nsteps) {
             groups[{m->nd_case, m->g.H}].push_back(i);
             maxrec = std::max(maxrec, m->shot.nrec);
         }
-        // the previous tables may still be read by queued copies
-        HIP_TRY(hipStreamSynchronize(st));
-        std::vector<int> &ids = b->host_ids;
-        std::vector<unsigned> &tile0 = b->host_tile0;
+        const int tb = b->tbuf ^ 1;
+        HIP_TRY(hipEventSynchronize(b->table_ev[tb]));   // last launches reading this buffer
+        b->tbuf = tb;
+        int *dids = b->dids + (size_t)tb * b->max_slots;
+        unsigned *dtile0 = b->dtile0 + (size_t)tb * (b->max_slots + 8);
+        std::vector<int> &ids = b->host_ids[tb];
+        std::vector<unsigned> &tile0 = b->host_tile0[tb];
         ids.clear();
         tile0.clear();
         b->classes.clear();
@@ -1307,14 +1376,14 @@ int seis_batch_step(seis_batch *b, int nsteps) {
             }
             tile0.push_back(acc);
             c.ntiles = acc;
-            c.ids = b->dids + off_ids;
-            c.tile0 = b->dtile0 + off_t;
+            c.ids = dids + off_ids;
+            c.tile0 = dtile0 + off_t;
             b->classes.push_back(c);
         }
         if (!ids.empty())
-            HIP_TRY(hipMemcpyAsync(b->dids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(dids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
         if (!tile0.empty())
-            HIP_TRY(hipMemcpyAsync(b->dtile0, tile0.data(), tile0.size() * 4, hipMemcpyHostToDevice,
+            HIP_TRY(hipMemcpyAsync(dtile0, tile0.data(), tile0.size() * 4, hipMemcpyHostToDevice,
                                    st));
         b->maxrec = maxrec;
         b->dirty = false;
@@ -1324,8 +1393,14 @@ int seis_batch_step(seis_batch *b, int nsteps) {
         if (b->model[i]) hi = i + 1;
     for (int k = 0; k < nsteps; ++k)
         launch_batch_tick(b->dslots, hi, b->classes.data(), (int)b->classes.size(), b->maxrec, st);
+    HIP_TRY(hipEventRecord(b->table_ev[b->tbuf], st));
+    hipEvent_t ev = b->step_ev[b->step_ev_next++ % b->step_ev.size()];
+    HIP_TRY(hipEventRecord(ev, st));
     for (int i = 0; i < b->max_slots; ++i)
-        if (b->model[i]) b->t[i] = std::min(b->tmax[i], b->t[i] + nsteps);
+        if (b->model[i] && b->t[i] < b->tmax[i]) {
+            b->t[i] = std::min(b->tmax[i], b->t[i] + nsteps);
+            if (b->t[i] == b->tmax[i]) b->model[i]->done_ev = ev;
+        }
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -181,13 +181,14 @@ class Model:
             return out
         return [np.ascontiguousarray(o.T) for o in out]
 
-    def bind_shot(self, src_pos, src, rec_pos):
-        """Upload one shot and zero the wavefields (first half of forward)."""
+    def bind_shot(self, src_pos, src, rec_pos, init: bool = True):
+        """Upload one shot and (init) zero the wavefields (first half of forward)."""
         self._shot = shot_arrays(src_pos, src, rec_pos)
         sp, s, rp = self._shot
         check(self.lib.seis_shot_bind(self.handle, sp.shape[0], fptr(sp), fptr(s), rp.shape[0],
                                       fptr(rp)))
-        check(self.lib.seis_launch_init(self.handle))
+        if init:
+            check(self.lib.seis_launch_init(self.handle))
 
     def run_steps(self, t0: int, nsteps: int):
         """Advance the bound shot by nsteps time steps starting at step t0."""

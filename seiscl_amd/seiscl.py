@@ -8,19 +8,38 @@ names, acquisition helpers and the call sequence of an FWI script:
     seis.execute()
     data = seis.read_data(); grad = seis.read_grad(); rms = seis.read_rms()
 
-The reference writes HDF5 files and runs the SeisCL_MPI program in a
-subprocess (SeisCL.py:414-505); here ``execute`` runs the shots on the GPU
-through the C-ABI and keeps the outputs in memory, so the read_* methods
-return what the reference reads back from its output files (same names,
-shapes and trace order).  File paths (workdir, file) are accepted and
-ignored.  Shots are the distinct srcids of the selected sources; receivers
-belong to the shot whose srcid they carry (SeisCL.py:765-808).
+Two ways to run:
+
+- in process (default): ``execute`` runs the shots on the GPU through the
+  C-ABI and keeps the outputs in memory, so the read_* methods return what
+  the reference reads back from its output files (same names, shapes and
+  trace order); file paths are accepted and ignored.  Shots are the
+  distinct srcids of the selected sources; receivers belong to the shot
+  whose srcid they carry (SeisCL.py:765-808).
+- ``files=True``: the reference's workflow (SeisCL.py:414-505, 507-763).
+  set_forward / set_backward / write_data write the HDF5 constants, model,
+  residual and data files; ``execute`` runs the SeisCL_MPI program on them
+  (a child process, or its body in this process with ``spawn=False``); the
+  read_* methods read the output files.
 """
 from __future__ import annotations
 
+import os
+import subprocess
+
 import numpy as np
 
+from . import h5mat
 from .engine import Context, Model
+
+# keys of the constants file (SeisCL/SeisCL.py:23-30)
+CSTS_KEYS = ['N', 'ND', 'dh', 'dt', 'NT', 'freesurf', 'FDORDER', 'MAXRELERROR', 'L', 'f0',
+             'FL', 'src_pos', 'rec_pos', 'src', 'abs_type', 'VPPML', 'NPOWER', 'FPML',
+             'K_MAX_CPML', 'nab', 'abpc', 'pref_device_type', 'no_use_GPUs', 'MPI_NPROC_SHOT',
+             'nmax_dev', 'back_prop_type', 'param_type', 'gradfreqs', 'tmax', 'tmin',
+             'scalerms', 'scalermsnorm', 'scaleshot', 'fmin', 'fmax', 'gradout', 'Hout',
+             'gradsrcout', 'seisout', 'resout', 'rmsout', 'movout', 'restype', 'inputres',
+             'FP16']
 
 
 class SeisCLError(Exception):
@@ -46,7 +65,8 @@ class SeisCL:
                  offmin=-float("Inf"), offmax=float("Inf"), inputres=0, restype=0,
                  scalerms=0, scalermsnorm=0, scaleshot=0, seisout=2, resout=0, rmsout=0,
                  movout=0, file="SeisCL", workdir="./seiscl", device: int = 0,
-                 gradfix: int = 15):
+                 gradfix: int = 15, files: bool = False, spawn: bool = True,
+                 progname: str | None = None):
         self.N, self.ND, self.dh, self.dt, self.NT = N, ND, dh, dt, NT
         self.L, self.f0, self.FL = L, f0, FL
         self.FDORDER, self.MAXRELERROR, self.FP16 = FDORDER, MAXRELERROR, FP16
@@ -68,6 +88,8 @@ class SeisCL:
         self.seisout, self.resout, self.rmsout, self.movout = seisout, resout, rmsout, movout
         self.file, self.workdir = file, workdir
         self.device, self.gradfix = device, gradfix
+        self.files, self.spawn = files, spawn
+        self.progname = h5mat.BIN_PATH if progname is None else progname
         self.wavelet_generator = self.ricker_wavelet
         self._to_load_names = None
         self._params = None
@@ -76,6 +98,26 @@ class SeisCL:
         self._res_in = None
         self._results = {}
         self._ctx = None
+
+    # ---- file names (SeisCL.py:355-371) ------------------------------------
+    @property
+    def file(self):
+        return self._file
+
+    @file.setter
+    def file(self, file):
+        self._file = file
+        self.file_model = file + "_model.mat"
+        self.file_csts = file + "_csts.mat"
+        self.file_dout = file + "_dout.mat"
+        self.file_gout = file + "_gout.mat"
+        self.file_rms = file + "_rms.mat"
+        self.file_movout = file + "_movie.mat"
+        self.file_din = file + "_din.mat"
+        self.file_res = file + "_res.mat"
+
+    def _path(self, workdir, name):
+        return os.path.join(self.workdir if workdir is None else workdir, name)
 
     # ---- names and shapes (SeisCL.py:341-412) -----------------------------
     @property
@@ -187,6 +229,9 @@ class SeisCL:
         self.N = np.array(params[self.params[0]].shape, dtype=int)
         self._params = {k: np.asarray(v, np.float32) for k, v in params.items()}
         self._jobids = jobids
+        if self.files:
+            self.write_csts(workdir)
+            self.write_model(params, workdir)
 
     def set_backward(self, residuals=None, workdir=None):
         """Adjoint sources given per output variable, (NT, ntraces) each, in
@@ -198,11 +243,47 @@ class SeisCL:
             self._res_in = None
             self.inputres = 0
         self.gradout = 1
+        if self.files:
+            if residuals is not None:
+                h5mat.savemat(self._path(workdir, self.file_res),
+                              {w + "res": r for w, r in zip(self.to_load_names, self._res_in)})
+            self.write_csts(workdir)
 
     def write_data(self, data, workdir=None, filename=None):
         """Observed data for the residuals: {name: (NT, ntraces)} with the
         keys of to_load_names, in the trace order of the prepared receivers."""
         self._din = {k: np.asarray(v, np.float32) for k, v in data.items()}
+        if self.files:
+            data = dict(data)
+            for k in ("src_pos", "rec_pos", "src"):
+                if k not in data:
+                    data[k] = getattr(self, k)
+            h5mat.savemat(self._path(workdir, self.file_din if filename is None else filename),
+                          data)
+
+    # ---- files (SeisCL.py:682-763) ------------------------------------------
+    def write_csts(self, workdir=None, filename=None):
+        vals = {}
+        for k in CSTS_KEYS:
+            v = getattr(self, k)
+            if v is None:
+                raise SeisCLError(f"{k} is not set: call set_forward first")
+            vals[k] = v
+        h5mat.savemat(self._path(workdir, self.file_csts if filename is None else filename),
+                      vals)
+
+    def write_model(self, params, workdir=None):
+        for p in self.params:
+            if p not in params:
+                raise SeisCLError(f"Parameter with {p} not defined\n")
+        h5mat.savemat(self._path(workdir, self.file_model),
+                      {k: np.asarray(v).astype(np.float32) for k, v in params.items()})
+
+    def callcmd(self, workdir=None):
+        """The program and its arguments: base name and data file."""
+        workdir = os.path.abspath(self.workdir if workdir is None else workdir)
+        return [self.progname, os.path.join(workdir, self.file),
+                os.path.join(workdir, self.file_din)]
 
     def _engine_csts(self):
         c = {k: getattr(self, k) for k in self._ENGINE_KEYS}
@@ -210,6 +291,18 @@ class SeisCL:
         return c
 
     def execute(self, workdir=None):
+        if self.files:
+            cmd = self.callcmd(workdir)
+            if not self.spawn:
+                if h5mat.cli_main(cmd[1:]) != 0:
+                    raise SeisCLError("SeisCL_MPI failed")
+                return ""
+            p = subprocess.run(cmd, capture_output=True)
+            if p.stderr:
+                raise SeisCLError(p.stderr.decode())
+            if p.returncode:
+                raise SeisCLError(f"SeisCL_MPI exited with {p.returncode}")
+            return p.stdout.decode()
         if self._params is None:
             raise SeisCLError("call set_forward first")
         if self._ctx is None:
@@ -261,12 +354,41 @@ class SeisCL:
 
     # ---- outputs (SeisCL.py:507-656) ---------------------------------------
     def read_data(self, workdir=None, filename=None, residuals=False):
+        if self.files:
+            path = self._path(workdir, self.file_dout if filename is None else filename)
+            suf = "res" if residuals else "out"
+            try:
+                have = set(h5mat.names(path))
+            except Exception as e:
+                raise SeisCLError("Could not read data") from e
+            out = [np.transpose(h5mat.read(path, w + suf)) for w in self.to_load_names
+                   if w + suf in have]
+            if not out:
+                raise SeisCLError("Could not read data: variables not found")
+            return out
         key = "res" if residuals else "out"
         if key not in self._results:
             raise SeisCLError("Could not read data")
         return [self._results[key][n] for n in self.to_load_names]
 
+    def _crop(self, o):
+        if self.cropgrad:
+            if self.freesurf == 0:
+                o[:self.nab, :] = 0
+            o[-self.nab:, :] = 0
+            o[:, :self.nab] = 0
+            o[:, -self.nab:] = 0
+        return o
+
     def read_grad(self, workdir=None, param_names=None, filename=None):
+        if self.files:
+            path = self._path(workdir, self.file_gout if filename is None else filename)
+            param_names = self.params if param_names is None else param_names
+            try:
+                return [self._crop(np.transpose(h5mat.read(path, "grad" + n)).copy())
+                        for n in param_names]
+            except Exception as e:
+                raise SeisCLError("Could not read grad") from e
         if "grad" not in self._results:
             raise SeisCLError("Could not read grad")
         param_names = self.params if param_names is None else param_names
@@ -274,17 +396,17 @@ class SeisCL:
         for name in param_names:
             if name not in self._results["grad"]:
                 raise SeisCLError(f"no gradient for {name}")
-            o = np.array(self._results["grad"][name])
-            if self.cropgrad:
-                if self.freesurf == 0:
-                    o[:self.nab, :] = 0
-                o[-self.nab:, :] = 0
-                o[:, :self.nab] = 0
-                o[:, -self.nab:] = 0
-            out.append(o)
+            out.append(self._crop(np.array(self._results["grad"][name])))
         return out
 
     def read_rms(self, workdir=None, filename=None):
+        if self.files:
+            path = self._path(workdir, self.file_rms if filename is None else filename)
+            try:
+                rms, norm = h5mat.read(path, "rms")[0], h5mat.read(path, "rms_norm")[0]
+            except Exception as e:
+                raise SeisCLError("Forward modeling failed, could not read rms\n") from e
+            return rms / norm, norm
         if "rms" not in self._results or self._results["rms"][1] == 0:
             raise SeisCLError("Forward modeling failed, could not read rms\n")
         rms, norm = self._results["rms"]

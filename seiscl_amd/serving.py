@@ -78,44 +78,88 @@ def serve_plain(ctx: Context, requests):
 
 
 def serve_batched(ctx: Context, requests, max_slots: int = 32, hbm_budget: float = 64e9,
-                  chunk: int = 64):
-    """Continuous batching: admit arrivals while slots and the HBM budget
-    allow, step the batch, hand back finished jobs at once."""
+                  chunk: int = 64, prefetch: int = 4):
+    """Continuous batching.  A set-up thread builds each admitted request's
+    model (host transforms, uploads on the context's transfer stream) while
+    the GPU steps the batch; the main loop moves ready models into free
+    slots, queues the next steps, and only then copies back the jobs that
+    finished in the previous steps, so the GPU always has work queued."""
+    import queue
+    import threading
+
     batch = Batch(ctx, max_slots)
     pending = deque(sorted(requests, key=lambda q: q.arrival))
-    waiting = deque()
-    active = {}                     # slot -> (request, model, bytes)
+    todo, ready = queue.Queue(), queue.Queue()
+    errors = []
+
+    def setup():
+        while True:
+            r = todo.get()
+            if r is None:
+                return
+            try:
+                m = Model(ctx, r.params, **r.csts)
+                m.bind_shot(*r.shot, init=False)
+                ready.put((r, m))
+            except Exception as e:          # reported by the main loop
+                errors.append(e)
+                ready.put((r, None))
+
+    worker = threading.Thread(target=setup, daemon=True)
+    worker.start()
+    active = {}                 # slot -> (request, model, bytes)
+    done_prev = []              # slots that ran their last step in the previous round
+    inflight = 0                # requests handed to the set-up thread
     used = 0.0
     t0 = time.perf_counter()
     try:
-        while pending or waiting or active:
+        while pending or inflight or active:
             now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
-            while waiting and len(active) < max_slots:
-                need = estimate_bytes(waiting[0].csts)
-                if active and used + need > hbm_budget:
+            while (pending and pending[0].arrival <= now
+                   and len(active) + inflight < max_slots + prefetch):
+                need = estimate_bytes(pending[0].csts)
+                if (active or inflight) and used + need > hbm_budget:
                     break
-                r = waiting.popleft()
-                r.start = time.perf_counter() - t0
-                m = Model(ctx, r.params, **r.csts)
-                m.bind_shot(*r.shot)
-                active[batch.add(m)] = (r, m, need)
+                r = pending.popleft()
+                r.start = now
                 used += need
-            if not active:
-                if pending:
-                    _wait_until(t0, pending[0].arrival)
-                continue
-            _, min_left = batch.status()
-            batch.step(max(1, min(min_left, chunk)))
-            for slot in [s for s in active if batch.left(s) == 0]:
+                inflight += 1
+                todo.put(r)
+            while len(active) < max_slots:
+                try:
+                    r, m = ready.get_nowait()
+                except queue.Empty:
+                    break
+                inflight -= 1
+                if m is None:
+                    raise errors[0]
+                active[batch.add(m)] = (r, m, estimate_bytes(r.csts))
+            nstep, min_left = batch.status()
+            if nstep:
+                batch.step(max(1, min(min_left, chunk)))
+            # copy back what finished before the steps just queued
+            for slot in done_prev:
                 r, m, need = active.pop(slot)
-                r.out = m.read_shot()          # waits for the batch's queued steps
+                r.out = m.read_shot()
                 r.finish = time.perf_counter() - t0
                 batch.remove(slot)
                 m.close()
                 used -= need
+            done_prev = [s for s in active if batch.left(s) == 0]
+            if not nstep and not done_prev:
+                # nothing on the GPU: wait for a model or the next arrival
+                if inflight:
+                    r, m = ready.get()
+                    ready.put((r, m))
+                elif pending:
+                    _wait_until(t0, pending[0].arrival)
     finally:
+        todo.put(None)
+        worker.join()
+        while not ready.empty():
+            _, m = ready.get()
+            if m is not None:
+                m.close()
         for r, m, _ in active.values():
             m.close()
         batch.close()

@@ -1,0 +1,169 @@
+"""The file-level boundary on the CPU: HDF5 (.mat) files in SeisCL's
+conventions and the SeisCL_MPI program's input checks, which all run before
+any GPU is touched.
+
+File layout pins (src/writehdf5.c:24-96, 166-188; SeisCL/SeisCL.py:658-763):
+the 512-byte user block with the MATLAB 7.3 header, arrays stored
+transposed, float32 data with a MATLAB_class attribute, scalars readable
+as int or float whatever type they were written with.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from seiscl_amd import h5mat
+from seiscl_amd.seiscl import CSTS_KEYS, SeisCL
+from seiscl_amd.workloads import layered_model
+
+
+def test_h5_round_trip(tmp_path):
+    p = str(tmp_path / "a.mat")
+    a = np.arange(24, dtype=np.float32).reshape(2, 3, 4)
+    h5mat.savemat(p, {"a": a, "n": 7, "x": 0.25, "i": np.array([3, 4]), "e": np.empty((1, 0))})
+    assert sorted(h5mat.names(p)) == ["a", "e", "i", "n", "x"]
+    # stored transposed: the C reader sees the Python array's axes reversed
+    assert h5mat.shape(p, "a") == (4, 3, 2)
+    np.testing.assert_array_equal(h5mat.read(p, "a"), a.T)
+    np.testing.assert_array_equal(h5mat.loadmat(p, ["a"])["a"], a)
+    # scalars convert to the reader's type (readscalar, src/read_hdf5.c:243-252)
+    assert h5mat.shape(p, "n") == ()
+    assert h5mat.read(p, "n", np.int32) == 7
+    assert h5mat.read(p, "x", np.float32) == 0.25
+    assert h5mat.read(p, "n", np.float32) == 7.0
+    np.testing.assert_array_equal(h5mat.read(p, "i", np.int32), [3, 4])
+    assert h5mat.shape(p, "e") == (0, 1)
+    # overwrite in place with the same shape; another shape is refused
+    h5mat.write(p, "a", np.ones((4, 3, 2), np.float32))
+    assert h5mat.read(p, "a").sum() == 24
+    with pytest.raises(Exception, match="another shape"):
+        h5mat.write(p, "a", np.ones((2, 2), np.float32))
+    with pytest.raises(Exception, match="not defined"):
+        h5mat.read(p, "missing")
+
+
+def test_h5_user_block(tmp_path):
+    p = str(tmp_path / "b.mat")
+    h5mat.create(p)
+    head = open(p, "rb").read(512)
+    assert head.startswith(b"MATLAB 7.3 MAT-file")
+    assert head[116:124] == b"\0" * 8
+    assert head[124:128] == b"\x00\x02IM"
+    # the HDF5 superblock follows the 512-byte user block
+    assert open(p, "rb").read()[512:520] == b"\x89HDF\r\n\x1a\n"
+
+
+def _seis(tmp_path, **kw):
+    N = np.array([60, 90])
+    args = dict(N=N, ND=2, dh=10.0, dt=1e-3, NT=120, FDORDER=4, nab=10, seisout=1, f0=15.0,
+                files=True, spawn=False, workdir=str(tmp_path))
+    args.update(kw)
+    seis = SeisCL(**args)
+    seis.surface_acquisition_2d(dg=3, ds=25)
+    model = layered_model(tuple(N), np.random.default_rng(2), nlayers=3)
+    return seis, model
+
+
+def test_files_written_like_seiscl_py(tmp_path):
+    seis, model = _seis(tmp_path)
+    seis.set_forward([0, 1], model, withgrad=False)
+    csts = os.path.join(str(tmp_path), "SeisCL_csts.mat")
+    assert set(h5mat.names(csts)) == set(CSTS_KEYS)
+    ns, ng = seis.src_pos.shape[1], seis.rec_pos.shape[1]
+    assert h5mat.shape(csts, "src_pos") == (ns, 5)
+    assert h5mat.shape(csts, "rec_pos") == (ng, 8)
+    assert h5mat.shape(csts, "src") == (ns, 120)
+    np.testing.assert_array_equal(h5mat.read(csts, "N", np.int32), [60, 90])
+    mod = os.path.join(str(tmp_path), "SeisCL_model.mat")
+    assert h5mat.shape(mod, "vp") == (90, 60)
+    np.testing.assert_array_equal(h5mat.read(mod, "vs"), model["vs"].T)
+    # receiver ids are the 1-based positions in rec_pos_all (SeisCL.py:808)
+    assert h5mat.read(csts, "rec_pos")[:, 4].tolist() == list(range(1, ng + 1))
+
+
+def _fails(capfd, args, match):
+    st = h5mat.cli_main(args)
+    err = capfd.readouterr().err
+    assert st != 0
+    assert match in err, err
+    return err
+
+
+def test_cli_input_errors(tmp_path, capfd):
+    base = os.path.join(str(tmp_path), "SeisCL")
+    _fails(capfd, [base], "Could not open the input file csts")
+    seis, model = _seis(tmp_path)
+    seis.set_forward([0, 1], model, withgrad=False)
+    csts = base + "_csts.mat"
+    # features outside this engine are refused, not ignored
+    for key, val, msg in [("movout", 1, "movie output"), ("Hout", 1, "Hessian"),
+                          ("gradsrcout", 1, "source gradient"), ("FP16", 3, "half arithmetic"),
+                          ("scalerms", 1, "plain L2"), ("back_prop_type", 2, "back_prop_type=2"),
+                          ("fmax", 20.0, "filtering")]:
+        old = getattr(seis, key)
+        setattr(seis, key, val)
+        if key in ("back_prop_type", "fmax"):
+            seis.gradout = 1
+        seis.write_csts()
+        _fails(capfd, [base], msg)
+        setattr(seis, key, old)
+        seis.gradout = 0
+    # tmax beyond the time window, tmax < tmin (seconds in the file)
+    seis.tmax = 1.0
+    seis.write_csts()
+    _fails(capfd, [base], "time window")
+    seis.tmax, seis.tmin = 0.05, 0.08
+    seis.write_csts()
+    _fails(capfd, [base], "tmax<tmin")
+    seis.tmax, seis.tmin = 0, 0
+    seis.write_csts()
+    # a required constant missing
+    c = h5mat.loadmat(csts, dtype=np.float64)
+    del c["nab"]
+    h5mat.savemat(csts, {k: v for k, v in c.items()})
+    _fails(capfd, [base], "Variable /nab is not defined")
+
+
+def test_cli_srcrec_and_model_errors(tmp_path, capfd):
+    base = os.path.join(str(tmp_path), "SeisCL")
+    seis, model = _seis(tmp_path)
+    seis.set_forward([0, 1], model, withgrad=False)
+    csts = base + "_csts.mat"
+    sp = h5mat.read(csts, "src_pos")
+    bad = sp[::-1].copy()
+    h5mat.write(csts, "src_pos", bad)
+    _fails(capfd, [base], "sorted in ascending order")
+    h5mat.write(csts, "src_pos", sp)
+    # model of the wrong size
+    h5mat.savemat(base + "_model.mat", {k: v[:, :-2] for k, v in model.items()})
+    _fails(capfd, [base], "Dimension 0 of variable /vp should be 90, got 88")
+    # gradient without observed data
+    seis.set_forward([0, 1], model, withgrad=True)
+    _fails(capfd, [base, base + "_nodata.mat"], "Could not open the input file for data_in")
+    # observed data that do not form one output set
+    seis.write_data({"vx": np.zeros((120, seis.rec_pos.shape[1]), np.float32)})
+    _fails(capfd, [base, base + "_din.mat"], "one output set")
+    # inputres and resout together
+    seis.resout = 1
+    seis.set_backward([np.zeros((120, seis.rec_pos.shape[1]), np.float32)] * 2)
+    _fails(capfd, [base], "Cannot input and output residuals simultaneously")
+
+
+def test_program_runs_as_a_child(tmp_path):
+    """The SeisCL_MPI executable itself: arguments and exit status."""
+    try:
+        import torch
+        if torch.cuda.is_initialized():
+            pytest.skip("this process has initialised a GPU: no child processes")
+    except ImportError:
+        pass
+    base = os.path.join(str(tmp_path), "SeisCL")
+    p = subprocess.run([h5mat.BIN_PATH, base], capture_output=True, timeout=60)
+    assert p.returncode == 1
+    assert b"Could not open the input file csts" in p.stderr
+    assert b"Input files for SeisCL" in p.stdout
+    p = subprocess.run([h5mat.BIN_PATH, "-x"], capture_output=True, timeout=60)
+    assert p.returncode == 1 and b"Unknown option" in p.stderr
+    p = subprocess.run([h5mat.BIN_PATH, "a", "b", "c"], capture_output=True, timeout=60)
+    assert p.returncode == 1 and b"Too many default argument" in p.stderr

@@ -1,0 +1,153 @@
+"""The SeisCL_MPI file boundary on the GPU (its body runs in this process
+through seis_cli_main, so no program is started from a process holding the
+GPU).  Files are written and read with SeisCL.py's conventions; results are
+checked shot by shot against the oracle:
+
+- forward seismograms (<var>out), 2D and 3D;
+- gradient files (grad<name>) and the rms file for a two-shot FWI round with
+  observed data;
+- the inputres mode (adjoint sources from the residual file);
+- resout: the scaled adjoint sources the reference writes as <var>res.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cases import rel_l2, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _fwi_case(tmp_path, **kw):
+    from seiscl_amd.seiscl import SeisCL
+    from seiscl_amd.workloads import layered_model
+    N = np.array([60, 90])
+    args = dict(N=N, ND=2, dh=10.0, dt=1e-3, NT=220, FDORDER=4, nab=10, seisout=1, f0=15.0,
+                files=True, spawn=False, workdir=str(tmp_path))
+    args.update(kw)
+    seis = SeisCL(**args)
+    seis.surface_acquisition_2d(dg=3, ds=25)
+    rng = np.random.default_rng(2)
+    model = layered_model(tuple(N), rng, nlayers=4)
+    true = {k: (v * (1 + 0.03 * rng.standard_normal(v.shape))).astype(np.float32)
+            for k, v in model.items()}
+    csts = {k: args[k] for k in ("N", "ND", "dh", "dt", "NT", "FDORDER", "nab", "seisout", "f0")}
+    return seis, model, true, csts
+
+
+def _shots(seis):
+    for sid in sorted(set(seis.src_pos[3].astype(int))):
+        sc = seis.src_pos[3].astype(int) == sid
+        rc = seis.rec_pos[3].astype(int) == sid
+        yield rc, (seis.src_pos[:, sc], seis.src[:, sc], seis.rec_pos[:, rc])
+
+
+def _cropped(g, nab):
+    g = g.astype(np.float64)
+    g[:nab, :] = 0
+    g[-nab:, :] = 0
+    g[:, :nab] = 0
+    g[:, -nab:] = 0
+    return g
+
+
+def test_forward_and_gradient_files(oracle_lib, tmp_path):
+    seis, model, true, csts = _fwi_case(tmp_path)
+    print("[cli] forward files", flush=True)
+    seis.set_forward([0, 1], true, withgrad=False)
+    seis.execute()
+    dobs = seis.read_data()
+    assert [d.shape for d in dobs] == [(220, seis.rec_pos.shape[1])] * 2
+    refs = []
+    for rc, shot in _shots(seis):
+        ref = oracle_lib.forward(true, *shot, **csts)
+        for d, r in zip(dobs, ref):
+            assert rel_l2(d[:, rc], r) <= 1e-6
+        refs.append((rc, shot, ref))
+    # data file with every trace, rows by receiver id (read_seis)
+    seis.write_data({"vx": dobs[0], "vz": dobs[1]})
+    print("[cli] gradient files", flush=True)
+    seis.rmsout = 1
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.execute()
+    gout = seis.read_grad()
+    rms, norm = seis.read_rms()
+    tot = {k: 0 for k in seis.params}
+    r_sum = rn_sum = 0.0
+    for rc, shot, ref in refs:
+        g = oracle_lib.gradient(model, *shot, din=ref, **csts)
+        for k in tot:
+            tot[k] = tot[k] + g["grad"][k].astype(np.float64)
+        r_sum += g["rms"][0]
+        rn_sum += g["rms"][1]
+    for name, g in zip(seis.params, gout):
+        assert rel_l2(g, _cropped(tot[name], seis.nab)) <= 1e-5, name
+    # the rms file holds sqrt of the sums (src/writehdf5.c:236-245)
+    assert abs(norm - np.sqrt(rn_sum)) <= 1e-5 * np.sqrt(rn_sum)
+    assert abs(rms - np.sqrt(r_sum) / np.sqrt(rn_sum)) <= 1e-5 * rms
+    # dout carries src_pos and rec_pos as given
+    dout = os.path.join(str(tmp_path), "SeisCL_dout.mat")
+    from seiscl_amd import h5mat
+    np.testing.assert_array_equal(h5mat.read(dout, "rec_pos"), seis.rec_pos.T.astype(np.float32))
+
+
+def test_inputres_files(oracle_lib, tmp_path):
+    seis, model, _, csts = _fwi_case(tmp_path)
+    seis.prepare_data([0, 1])
+    rng = np.random.default_rng(5)
+    res = [rng.standard_normal((220, seis.rec_pos.shape[1])).astype(np.float32) for _ in range(2)]
+    print("[cli] inputres files", flush=True)
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.set_backward(res)
+    seis.execute()
+    gout = seis.read_grad()
+    tot = {k: 0 for k in seis.params}
+    for rc, shot in _shots(seis):
+        g = oracle_lib.gradient(model, *shot, res=[r[:, rc] for r in res], **csts)
+        for k in tot:
+            tot[k] = tot[k] + g["grad"][k].astype(np.float64)
+    for name, g in zip(seis.params, gout):
+        assert rel_l2(g, _cropped(tot[name], seis.nab)) <= 1e-5, name
+
+
+def test_resout_files(oracle_lib, tmp_path):
+    seis, model, true, csts = _fwi_case(tmp_path, resout=1, rmsout=1)
+    seis.prepare_data([0, 1])
+    dobs = []
+    for rc, shot in _shots(seis):
+        dobs.append((rc, oracle_lib.forward(true, *shot, **csts)))
+    full = [np.zeros((220, seis.rec_pos.shape[1]), np.float32) for _ in range(2)]
+    for rc, d in dobs:
+        for f, x in zip(full, d):
+            f[:, rc] = x
+    print("[cli] resout files", flush=True)
+    seis.set_forward([0, 1], model, withgrad=False)
+    seis.write_data({"vx": full[0], "vz": full[1]})
+    seis.execute()
+    res = seis.read_data(residuals=True)
+    for (rc, d), (_, shot) in zip(dobs, _shots(seis)):
+        ref = oracle_lib.gradient(model, *shot, din=d, **csts)["res"]
+        for a, r in zip(res, ref):
+            assert np.abs(r).max() > 0
+            assert rel_l2(a[:, rc], r) <= 1e-6
+
+
+@pytest.mark.parametrize("seisout", [1, 2])
+def test_forward_files_3d(oracle_lib, tmp_path, seisout):
+    from seiscl_amd.seiscl import SeisCL
+    csts, params, (sp, s, rp) = small_case(ND=3, FDORDER=4, L=2, freesurf=1, NT=80,
+                                           seisout=seisout)
+    keys = ("N", "ND", "dh", "dt", "NT", "FDORDER", "MAXRELERROR", "abs_type", "nab",
+            "freesurf", "L", "FL", "f0", "seisout", "abpc")
+    seis = SeisCL(files=True, spawn=False, workdir=str(tmp_path),
+                  **{k: csts[k] for k in keys})
+    seis.src_pos_all, seis.src_all, seis.rec_pos_all = sp, s, rp.copy()
+    print(f"[cli] 3D forward seisout={seisout}", flush=True)
+    seis.set_forward([0], params, withgrad=False)
+    seis.execute()
+    out = seis.read_data()
+    ref = oracle_lib.forward(params, sp, s, seis.rec_pos, **csts)
+    assert len(out) == len(ref)
+    for o, r in zip(out, ref):
+        assert rel_l2(o, r) <= 1e-6

@@ -25,6 +25,8 @@ CASES = (
     + [dict(ND=2, FDORDER=o, freesurf=1) for o in (4, 8)]
     + [dict(ND=2, FDORDER=4, freesurf=1, abs_type=2)]
     + [dict(ND=3, FDORDER=4, freesurf=1, gradfix=f) for f in (15, 7)]
+    # tall 3D grid: several z tiles of 64 cells
+    + [dict(ND=3, FDORDER=4, abs_type=a, N=(150, 28, 30)) for a in (1, 2)]
 )
 
 
@@ -69,7 +71,7 @@ def test_gradient_matches_oracle(oracle_lib, gpu_ctx, case):
         assert rel_l2(g[k], ref["grad"][k]) <= TOL, k
 
 
-@pytest.mark.parametrize("case", [CASES[2], CASES[-3]], ids=_ids)
+@pytest.mark.parametrize("case", [CASES[2], CASES[-5], CASES[-1]], ids=_ids)
 def test_gradient_bitwise(oracle_lib, gpu_ctx, case):
     ref, out, rms, g = _run_both(oracle_lib, gpu_ctx, case)
     for r, o in zip(ref["out"], out):

@@ -16,8 +16,10 @@ from cases import rel_l2, small_case
 
 pytestmark = pytest.mark.gpu
 
-CASES = [dict(ND=3, FDORDER=o, abs_type=a, freesurf=f, L=l)
-         for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
+CASES = ([dict(ND=3, FDORDER=o, abs_type=a, freesurf=f, L=l)
+          for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
+         # several z tiles tall
+         + [dict(ND=3, FDORDER=4, abs_type=1, freesurf=1, L=2, N=(150, 28, 30))])
 
 
 def _ids(c):

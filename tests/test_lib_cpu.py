@@ -13,8 +13,8 @@ from seiscl_amd.engine import from_c_layout, make_csts, to_c_layout
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    txt = open(os.path.join(ROOT, "include", "seiscl_amd.h")).read()
+def header_functions(header="seiscl_amd.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(seis_\w+)\s*\(", txt)))
 
@@ -27,6 +27,16 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     bound = {s[0] for s in _lib.SIGNATURES}
     assert set(names) <= bound, set(names) - bound
+
+
+def test_cli_library_exports_every_declared_symbol():
+    from seiscl_amd import h5mat
+    lib = h5mat.load()
+    names = header_functions("seiscl_cli.h")
+    assert "seis_cli_main" in names and len(names) >= 7
+    for n in names:
+        assert hasattr(lib, n), n
+    assert os.access(h5mat.BIN_PATH, os.X_OK)
 
 
 def test_csts_struct_matches_c():
