@@ -165,8 +165,8 @@ struct seis_model {
     }
 };
 
-// A batch of modeling jobs advanced together, one launch per kernel class and
-// step for all of them (serving).  Slots hold models with a bound shot.
+// A batch of modeling jobs advanced together, one launch per pass and step
+// for all of them (serving).  Slots hold models with a bound shot.
 struct seis_batch {
     seis_ctx *ctx = nullptr;
     int max_slots = 0;
@@ -174,9 +174,10 @@ struct seis_batch {
     std::vector<int> t;                  // host copy of each slot's next step
     std::vector<int> tmax;
     BatchSlot *dslots = nullptr;         // device slot table
-    int *dids = nullptr;                 // device class tables
+    int *dids = nullptr;                 // device slot table in class order
     unsigned *dtile0 = nullptr;
-    std::vector<BatchClass> classes;
+    BatchTable table{};
+    int step = 0;                        // the next batch step to launch
     int maxrec = 0;
     bool dirty = true;
     // host sources of asynchronous copies
@@ -195,42 +196,43 @@ struct seis_batch {
 
 namespace {
 
-// Unpadded reference array (z fastest) -> padded device layout.
-std::vector<float> to_padded(const Layout &g, const std::vector<float> &ref) {
-    std::vector<float> out((size_t)g.nfield, 0.f);
-    for (int x = 0; x < g.NX; ++x)
-        for (int y = 0; y < g.NY; ++y) {
-            const float *src = &ref[((size_t)x * g.NY + y) * g.NZ];
-            std::copy(src, src + g.NZ, &out[g.idx(0, y, x)]);
-        }
-    return out;
-}
-
-// Host float array -> device coefficient array (float, or double in FP64
-// mode), optionally multiplied by a power of two (exact).
-int upload_raw(seis_model *m, const float **dst, const std::vector<float> &host, float mul = 1) {
+// Small host float array (profiles) -> device array of the coefficient type.
+int upload_raw(seis_model *m, const float **dst, const std::vector<float> &host) {
     float *d = m->calloc_((long)host.size());
     if (!d) return 1;
+    hipStream_t us = m->setting_up ? m->ctx->xfer : m->ctx->stream;
     if (m->celem == 8) {
-        std::vector<double> h(host.size());
-        for (size_t i = 0; i < h.size(); ++i) h[i] = (double)host[i] * mul;
-        hipStream_t us = m->setting_up ? m->ctx->xfer : m->ctx->stream;
+        std::vector<double> h(host.begin(), host.end());
         HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, us));
         HIP_TRY(hipStreamSynchronize(us));
     } else {
-        std::vector<float> h(host);
-        if (mul != 1)
-            for (float &v : h) v *= mul;
-        hipStream_t us = m->setting_up ? m->ctx->xfer : m->ctx->stream;
-        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, us));
+        HIP_TRY(hipMemcpyAsync(d, host.data(), host.size() * 4, hipMemcpyHostToDevice, us));
         HIP_TRY(hipStreamSynchronize(us));
     }
     *dst = d;
     return 0;
 }
 
+// Unpadded reference array (z fastest) -> device coefficient array in the
+// padded layout (float, or double in FP64 mode), optionally multiplied by a
+// power of two (exact).  The host expansion runs on all host threads.
 int upload(seis_model *m, const float **dst, const std::vector<float> &ref, float mul = 1) {
-    return upload_raw(m, dst, to_padded(m->g, ref), mul);
+    float *d = m->calloc_(m->g.nfield);
+    if (!d) return 1;
+    hipStream_t us = m->setting_up ? m->ctx->xfer : m->ctx->stream;
+    if (m->celem == 8) {
+        std::vector<double> h;
+        to_padded(m->g, ref.data(), mul, h);
+        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, us));
+        HIP_TRY(hipStreamSynchronize(us));
+    } else {
+        std::vector<float> h;
+        to_padded(m->g, ref.data(), mul, h);
+        HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, us));
+        HIP_TRY(hipStreamSynchronize(us));
+    }
+    *dst = d;
+    return 0;
 }
 
 int out_code(const std::string &n) {
@@ -247,7 +249,6 @@ int upload_params(seis_model *m, const RefParams &p) {
     const seis_csts &c = m->c;
     const int L = c.L;
     const float DT = m->s.DT;
-    const size_t n = p.M.size();
     FDArgs<float> &a = m->a;
     // scaled units: buoyancies times q, stress-update coefficients over q
     const float qv = m->q, qs = 1.0f / m->q;
@@ -263,32 +264,16 @@ int upload_params(seis_model *m, const RefParams &p) {
         }
         return 0;
     }
-    std::vector<float> g(n), f, d, e(n);
-    if (c.ND == 22) {
-        for (size_t i = 0; i < n; ++i) {
-            g[i] = p.M[i] * (1.0 + (float)L * p.taup[i]);
-            e[i] = p.M[i] * p.taup[i];
-        }
-        return upload(m, &a.cg, g, qs) || upload(m, &a.ce, e, qs);
-    }
-    f.resize(n);
-    d.resize(n);
-    for (size_t i = 0; i < n; ++i) {
-        g[i] = p.M[i] * (1.0 + (float)L * p.taup[i]);
-        f[i] = 2.0 * p.mu[i] * (1.0 + (float)L * p.taus[i]);
-        d[i] = 2.0 * p.mu[i] * p.taus[i] / DT;
-        e[i] = p.M[i] * p.taup[i] / DT;
-    }
+    std::vector<float> g, f, d, e;
+    visco_coeffs(c, p, DT, g, f, d, e);
+    if (c.ND == 22) return upload(m, &a.cg, g, qs) || upload(m, &a.ce, e, qs);
     if (upload(m, &a.cg, g, qs) || upload(m, &a.cf, f, qs) || upload(m, &a.cd, d, qs) ||
         upload(m, &a.ce, e, qs))
         return 1;
     auto fij = [&](const std::vector<float> &mu, const std::vector<float> &ts, const float **fo,
                    const float **dO) {
-        std::vector<float> ff(n), dd(n);
-        for (size_t i = 0; i < n; ++i) {
-            ff[i] = mu[i] * (1.0 + (float)L * ts[i]);
-            dd[i] = mu[i] * ts[i] / DT;
-        }
+        std::vector<float> ff, dd;
+        visco_pair(mu, ts, L, DT, ff, dd);
         return upload(m, fo, ff, qs) || upload(m, dO, dd, qs);
     };
     if (fij(p.muipkp, p.tausipkp, &a.fipkp, &a.dipkp)) return 1;
@@ -1289,7 +1274,7 @@ int seis_batch_add(seis_batch *b, seis_model *m, int *slot) {
     sl.nd_case = m->nd_case;
     sl.NT = c.NT;
     sl.tmax = tmax_of(m);
-    sl.t = 0;
+    sl.start = b->step;
     sl.nout = (int)m->out_codes.size();
     for (int q = 0; q < sl.nout && q < 10; ++q) sl.codes[q] = m->out_codes[q];
     // the job starts from zeroed wavefields and seismograms
@@ -1360,39 +1345,31 @@ int seis_batch_step(seis_batch *b, int nsteps) {
         std::vector<unsigned> &tile0 = b->host_tile0[tb];
         ids.clear();
         tile0.clear();
-        b->classes.clear();
-        for (auto &kv : groups) {
-            BatchClass c;
-            c.nd_case = kv.first.first;
-            c.H = kv.first.second;
-            c.n = (int)kv.second.size();
-            const size_t off_ids = ids.size(), off_t = tile0.size();
-            unsigned acc = 0;
+        unsigned acc = 0;
+        for (auto &kv : groups)
             for (int i : kv.second) {
                 const Layout &g = b->model[i]->g;
                 ids.push_back(i);
                 tile0.push_back(acc);
-                acc += (unsigned)((g.NZ + 63) / 64) * (unsigned)((g.NX + 3) / 4);
+                const unsigned tiles = (unsigned)((g.NZ + 63) / 64) * (unsigned)((g.NX + 3) / 4);
+                acc += (tiles + kBatchTiles - 1) / kBatchTiles;
             }
-            tile0.push_back(acc);
-            c.ntiles = acc;
-            c.ids = dids + off_ids;
-            c.tile0 = dtile0 + off_t;
-            b->classes.push_back(c);
-        }
+        tile0.push_back(acc);
+        b->table.n = (int)ids.size();
+        b->table.nblocks = acc;
+        b->table.ids = dids;
+        b->table.block0 = dtile0;
         if (!ids.empty())
             HIP_TRY(hipMemcpyAsync(dids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
-        if (!tile0.empty())
-            HIP_TRY(hipMemcpyAsync(dtile0, tile0.data(), tile0.size() * 4, hipMemcpyHostToDevice,
-                                   st));
+        HIP_TRY(hipMemcpyAsync(dtile0, tile0.data(), tile0.size() * 4, hipMemcpyHostToDevice, st));
         b->maxrec = maxrec;
         b->dirty = false;
     }
     int hi = 0;
     for (int i = 0; i < b->max_slots; ++i)
         if (b->model[i]) hi = i + 1;
-    for (int k = 0; k < nsteps; ++k)
-        launch_batch_tick(b->dslots, hi, b->classes.data(), (int)b->classes.size(), b->maxrec, st);
+    launch_batch_steps(b->dslots, hi, b->table, b->maxrec, b->step, nsteps, st);
+    b->step += nsteps;
     HIP_TRY(hipEventRecord(b->table_ev[b->tbuf], st));
     hipEvent_t ev = b->step_ev[b->step_ev_next++ % b->step_ev.size()];
     HIP_TRY(hipEventRecord(ev, st));

@@ -503,47 +503,72 @@ __device__ __forceinline__ int batch_find(const unsigned *tile0, int n, unsigned
 
 // KIND 0: elastic velocity, 1: elastic stress, 2: acoustic velocity, 3: acoustic pressure
 template <int H, int KIND>
-__global__ __launch_bounds__(256) void k_batch_update(const BatchSlot *slots, BatchClass c) {
-    unsigned tile;
-    xcd_tile(c.ntiles, tile);
-    if (tile >= c.ntiles) return;
-    const int k = batch_find(c.tile0, c.n, tile);
-    const BatchSlot &S = slots[c.ids[k]];
-    if (S.t >= S.tmax) return;
-    const unsigned lt = tile - c.tile0[k];
-    const int nbz = (S.a.g.NZ + BZ - 1) / BZ;
-    const int z = (lt % nbz) * BZ + threadIdx.x, x = (lt / nbz) * BT + threadIdx.y;
-    if (z >= S.a.g.NZ || x >= S.a.g.NX) return;
-    if constexpr (KIND == 0) cell_v2d<float, H>(S.a, z, x);
-    else if constexpr (KIND == 1) cell_s2d<float, H, 0>(S.a, z, x);
-    else if constexpr (KIND == 2) cell_v2d_acc<float, H>(S.a, z, x);
-    else cell_s2d_acc<float, H, 0>(S.a, z, x);
+__device__ __forceinline__ void batch_cell(const FDArgs<float> &a, int z, int x) {
+    if constexpr (KIND == 0) cell_v2d<float, H>(a, z, x);
+    else if constexpr (KIND == 1) cell_s2d<float, H, 0>(a, z, x);
+    else if constexpr (KIND == 2) cell_v2d_acc<float, H>(a, z, x);
+    else cell_s2d_acc<float, H, 0>(a, z, x);
 }
 
-__global__ void k_batch_sources(const BatchSlot *slots, int n) {
+// The tiles [lt0, lt0 + kBatchTiles) of one slot, in its z-fastest tile order.
+template <int H, int KIND>
+__device__ __forceinline__ void batch_tiles(const FDArgs<float> &a, unsigned lt0) {
+    const int NZ = a.g.NZ, NX = a.g.NX;
+    const unsigned nbz = (NZ + BZ - 1) / BZ, nt = nbz * ((NX + BT - 1) / BT);
+#pragma unroll 1
+    for (int i = 0; i < kBatchTiles; ++i) {
+        const unsigned lt = lt0 + i;
+        if (lt >= nt) break;
+        const int z = (lt % nbz) * BZ + threadIdx.x, x = (lt / nbz) * BT + threadIdx.y;
+        if (z < NZ && x < NX) batch_cell<H, KIND>(a, z, x);
+    }
+}
+
+// One pass of one batch step over every slot: PASS 0 the velocities, 1 the
+// stresses (pressure).  A block finds its slot by binary search in the block
+// prefix and updates kBatchTiles tiles of it, which spreads the cost of the
+// search and of reading the slot over more cells.  The slot's kind and FD
+// order select the cell update (uniform per block); slots are grouped by kind
+// and order, so neighbouring blocks run the same code.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_batch_pass(const BatchSlot *__restrict__ slots,
+                                                    BatchTable tab, int step) {
+    unsigned blk;
+    xcd_tile(tab.nblocks, blk);
+    if (blk >= tab.nblocks) return;
+    const int k = batch_find(tab.block0, tab.n, blk);
+    const BatchSlot &S = slots[tab.ids[k]];
+    const int t = step - S.start;
+    if (!S.active || t < 0 || t >= S.tmax) return;
+    const unsigned lt0 = (blk - tab.block0[k]) * kBatchTiles;
+    if (S.nd_case == 22) {
+        SEIS_SWITCH_H(S.a.g.H, (batch_tiles<HH, PASS + 2>(S.a, lt0)));
+    } else {
+        SEIS_SWITCH_H(S.a.g.H, (batch_tiles<HH, PASS>(S.a, lt0)));
+    }
+}
+
+__global__ void k_batch_sources(const BatchSlot *slots, int n, int step) {
     if (threadIdx.x != 0 || (int)blockIdx.x >= n) return;
     const BatchSlot &S = slots[blockIdx.x];
-    if (!S.active || S.t >= S.tmax) return;
-    sources_body<float, float>(S.a, S.nd_case, S.shot, S.NT, S.t, 1);
+    const int t = step - S.start;
+    if (!S.active || t < 0 || t >= S.tmax) return;
+    sources_body<float, float>(S.a, S.nd_case, S.shot, S.NT, t, 1);
 }
 
-__global__ void k_batch_varsout(const BatchSlot *slots, int n) {
+__global__ void k_batch_varsout(const BatchSlot *slots, int n, int step) {
     if ((int)blockIdx.y >= n) return;
     const BatchSlot &S = slots[blockIdx.y];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!S.active || S.t >= S.tmax || r >= S.shot.nrec) return;
+    const int t = step - S.start;
+    if (!S.active || t < 0 || t >= S.tmax || r >= S.shot.nrec) return;
     OutList o;
     o.n = S.nout;
     for (int q = 0; q < S.nout; ++q) {
         o.code[q] = S.codes[q];
         o.unscale[q] = 1.0f;
     }
-    varsout_body<float, float>(S.a, S.nd_case, S.shot, o, S.t, r);
-}
-
-__global__ void k_batch_tick(BatchSlot *slots, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && slots[k].active && slots[k].t < slots[k].tmax) slots[k].t += 1;
+    varsout_body<float, float>(S.a, S.nd_case, S.shot, o, t, r);
 }
 
 __global__ void k_fill(uint32_t *p, size_t n) {
@@ -640,36 +665,22 @@ void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int
                        (hipStream_t)stream, a, nd_case, s, o, nt);
 }
 
-namespace {
-template <int H>
-void batch_update_launch(int kind, dim3 grid, dim3 blk, hipStream_t st, const BatchSlot *dslots,
-                         const BatchClass &c) {
-    switch (kind) {
-        case 0: hipLaunchKernelGGL((k_batch_update<H, 0>), grid, blk, 0, st, dslots, c); break;
-        case 1: hipLaunchKernelGGL((k_batch_update<H, 1>), grid, blk, 0, st, dslots, c); break;
-        case 2: hipLaunchKernelGGL((k_batch_update<H, 2>), grid, blk, 0, st, dslots, c); break;
-        default: hipLaunchKernelGGL((k_batch_update<H, 3>), grid, blk, 0, st, dslots, c); break;
-    }
-}
-}  // namespace
-
-void launch_batch_tick(BatchSlot *dslots, int nslots, const BatchClass *classes, int nclass,
-                       int maxrec, void *stream) {
+void launch_batch_steps(const BatchSlot *dslots, int nslots, const BatchTable &tab, int maxrec,
+                        int step0, int nsteps, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     if (nslots <= 0) return;
-    hipLaunchKernelGGL(k_batch_sources, dim3(nslots), dim3(64), 0, st, dslots, nslots);
-    for (int pass = 0; pass < 2; ++pass)
-        for (int ci = 0; ci < nclass; ++ci) {
-            const BatchClass &c = classes[ci];
-            if (c.n == 0 || c.ntiles == 0) continue;
-            const dim3 grid(((c.ntiles + 7) / 8) * 8), blk(BZ, BT);
-            const int kind = (c.nd_case == 22 ? 2 : 0) + pass;
-            SEIS_SWITCH_H(c.H, (batch_update_launch<HH>(kind, grid, blk, st, dslots, c)));
+    const dim3 grid(((tab.nblocks + 7) / 8) * 8), blk(BZ, BT);
+    for (int k = 0; k < nsteps; ++k) {
+        const int step = step0 + k;
+        hipLaunchKernelGGL(k_batch_sources, dim3(nslots), dim3(64), 0, st, dslots, nslots, step);
+        if (tab.nblocks) {
+            hipLaunchKernelGGL(k_batch_pass<0>, grid, blk, 0, st, dslots, tab, step);
+            hipLaunchKernelGGL(k_batch_pass<1>, grid, blk, 0, st, dslots, tab, step);
         }
-    if (maxrec > 0)
-        hipLaunchKernelGGL(k_batch_varsout, dim3((maxrec + 255) / 256, nslots), dim3(256), 0, st,
-                           (const BatchSlot *)dslots, nslots);
-    hipLaunchKernelGGL(k_batch_tick, dim3((nslots + 63) / 64), dim3(64), 0, st, dslots, nslots);
+        if (maxrec > 0)
+            hipLaunchKernelGGL(k_batch_varsout, dim3((maxrec + 255) / 256, nslots), dim3(256), 0,
+                               st, dslots, nslots, step);
+    }
 }
 
 void launch_fill(void *ptr, size_t bytes, void *stream) {

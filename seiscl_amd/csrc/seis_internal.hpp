@@ -66,6 +66,18 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
                      const float *rho, const float *taup, const float *taus,
                      RefParams &out, std::string &err);
 int check_stability(const seis_csts &c, const RefParams &p, float *dtstable);
+// Unpadded reference array (z fastest) -> padded layout, times mul (a power
+// of two: exact), converted to T.
+template <typename T>
+void to_padded(const Layout &g, const float *ref, float mul, std::vector<T> &out);
+// Derived viscoelastic coefficients of the stress update with the
+// reference's expressions (src/update_s3D.cl:324-340, src/update_s2D.cl:
+// 252-259, src/update_s2D_acc.cl:343-346): g, f, d, e (f, d for ND 2/3).
+void visco_coeffs(const seis_csts &c, const RefParams &p, float DT, std::vector<float> &g,
+                  std::vector<float> &f, std::vector<float> &d, std::vector<float> &e);
+// The staggered shear pair: f_ij = mu_ij (1 + L tau_ij), d_ij = mu_ij tau_ij / DT.
+void visco_pair(const std::vector<float> &mu, const std::vector<float> &ts, int L, float DT,
+                std::vector<float> &ff, std::vector<float> &dd);
 
 // Output names of seisout (SeisCL.py to_load_names order).
 std::vector<std::string> out_names(const seis_csts &c);
@@ -145,19 +157,25 @@ enum OutCode { O_VX, O_VY, O_VZ, O_SXX, O_SYY, O_SZZ, O_SXZ, O_SXY, O_SYZ, O_P }
 struct BatchSlot {
     FDArgs<float> a;
     ShotDev shot;
-    int active, nd_case, NT, tmax, t;   // t: the next step (advanced on the device)
+    int active, nd_case, NT, tmax;
+    int start;          // the batch step that runs the job's step 0: step t of
+                        // the job is batch step start + t
     int nout, codes[10];
 };
-// the slots of one kernel class (acoustic or elastic, FD order), with the
-// prefix sum of their tile counts
-struct BatchClass {
-    int nd_case, H, n;
-    unsigned ntiles;
+// every slot of a batch grouped by kernel class (acoustic or elastic, FD
+// order), with the prefix sum of their block counts; a block of a batch pass
+// updates kBatchTiles consecutive 64x4 tiles of one slot
+constexpr int kBatchTiles = 4;
+struct BatchTable {
+    int n;
+    unsigned nblocks;
     const int *ids;          // [n] slot indices (device)
-    const unsigned *tile0;   // [n+1] first tile of each slot (device)
+    const unsigned *block0;  // [n+1] first block of each slot (device)
 };
-void launch_batch_tick(BatchSlot *dslots, int nslots, const BatchClass *classes, int nclass,
-                       int maxrec, void *stream);
+// Batch steps step0 .. step0+nsteps-1: per step one source launch, one
+// velocity and one stress launch for all slots, one receiver launch.
+void launch_batch_steps(const BatchSlot *dslots, int nslots, const BatchTable &tab, int maxrec,
+                        int step0, int nsteps, void *stream);
 
 // ---------------------------------------------------------------------------
 // Gradient by backpropagation (BACK_PROP_TYPE=1).

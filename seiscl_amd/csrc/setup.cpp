@@ -128,6 +128,12 @@ void dims(const seis_csts &c, int &NZ, int &NY, int &NX) {
     }
 }
 
+// Host loops use threads only for grids of at least this many cells: the
+// 2D jobs of a serving batch are set up on a thread beside the loop that
+// feeds the GPU, and idle OpenMP workers spinning after a parallel region
+// would take cores from it.
+constexpr long kParCells = 1L << 21;
+
 // The reference's powf(x, 2): compilers emit the correctly rounded x*x.
 inline float sq(float x) { return x * x; }
 
@@ -140,6 +146,7 @@ inline long lin(int z, int y, int x, int NZ, int NY) {
 void avg2(const std::vector<float> &in, std::vector<float> &out, int NZ, int NY,
           int NX, int dz, int dy, int dx) {
     out.assign(in.size(), 0.f);
+#pragma omp parallel for schedule(static) if ((long)in.size() >= kParCells)
     for (int x = 0; x < NX - dx; ++x)
         for (int y = 0; y < NY - dy; ++y)
             for (int z = 0; z < NZ - dz; ++z)
@@ -162,6 +169,7 @@ void avg4(const std::vector<float> &in, std::vector<float> &out, int NZ, int NY,
           int NX, const int d[2][3]) {
     out.assign(in.size(), 0.f);
     const int ez = d[0][0] + d[1][0], ey = d[0][1] + d[1][1], ex = d[0][2] + d[1][2];
+#pragma omp parallel for schedule(static) if ((long)in.size() >= kParCells)
     for (int x = 0; x < NX - ex; ++x)
         for (int y = 0; y < NY - ey; ++y)
             for (int z = 0; z < NZ - ez; ++z) {
@@ -271,8 +279,15 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
         o.taup.assign(taup_in, taup_in + n);
         if (elastic) o.taus.assign(taus_in, taus_in + n);
     }
+    if (c.param_type < 0 || c.param_type > 3) {
+        err = "param_type must be 0, 1, 2 or 3";
+        return 1;
+    }
+    // Every loop below computes each cell on its own, so the threads change
+    // nothing in the results.
     // M (src/assign_modeling_case.c:380-449)
     o.M.resize(n);
+#pragma omp parallel for schedule(static) if (n >= kParCells)
     for (long i = 0; i < n; ++i) {
         const float v = p0[i], r = rho_in[i];
         switch (c.param_type) {
@@ -285,10 +300,10 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
                 o.M[i] = sq(vr) * r * c.dt / c.dh;
                 o.taup[i] = tp;
             } break;
-            default: err = "param_type must be 0, 1, 2 or 3"; return 1;
         }
     }
     if (L > 0)
+#pragma omp parallel for schedule(static) if (n >= kParCells)
         for (long i = 0; i < n; ++i) {
             float s = 0.0;
             for (int l = 0; l < L; ++l)
@@ -298,6 +313,7 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
     // mu (src/assign_modeling_case.c:305-379)
     if (elastic) {
         o.mu.resize(n);
+#pragma omp parallel for schedule(static) if (n >= kParCells)
         for (long i = 0; i < n; ++i) {
             const float v = p1[i], r = rho_in[i];
             switch (c.param_type) {
@@ -319,11 +335,13 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
             float s = 0.0;
             for (int l = 0; l < L; ++l)
                 s += ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));
+#pragma omp parallel for schedule(static) if (n >= kParCells)
             for (long i = 0; i < n; ++i) o.mu[i] = o.mu[i] / (1.0 + s * o.taus[i]);
         }
     }
     // buoyancy dt/(dh rho) (src/assign_modeling_case.c:451-462)
     o.rho.resize(n);
+#pragma omp parallel for schedule(static) if (n >= kParCells)
     for (long i = 0; i < n; ++i) o.rho[i] = 1.0 / rho_in[i] * c.dt / c.dh * 1.0;
     avg2(o.rho, o.rip, NZ, NY, NX, 0, 0, 1);
     if (c.ND == 3) avg2(o.rho, o.rjp, NZ, NY, NX, 0, 1, 0);
@@ -349,15 +367,24 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
 }
 
 int check_stability(const seis_csts &c, const RefParams &p, float *dtstable) {
-    float vpmax = 0, vsmax = 0;
-    for (size_t i = 0; i < p.mu.size(); ++i) {
-        float v = std::sqrt((double)(p.mu[i] * p.rho[i])) / c.dt * c.dh;
-        if (vsmax < v) vsmax = v;
-    }
-    for (size_t i = 0; i < p.M.size(); ++i) {
-        float v = std::sqrt((double)(p.M[i] * p.rho[i])) / c.dt * c.dh;
-        if (vpmax < v) vpmax = v;
-    }
+    // maxima per thread with the serial loop's comparison (NaNs never win),
+    // then over threads: the same value as one pass
+    auto vmax_of = [&](const std::vector<float> &m) {
+        float best = 0;
+#pragma omp parallel if ((long)m.size() >= kParCells)
+        {
+            float b = 0;
+#pragma omp for schedule(static) nowait
+            for (long i = 0; i < (long)m.size(); ++i) {
+                float v = std::sqrt((double)(m[i] * p.rho[i])) / c.dt * c.dh;
+                if (b < v) b = v;
+            }
+#pragma omp critical
+            if (best < b) best = b;
+        }
+        return best;
+    };
+    const float vsmax = vmax_of(p.mu), vpmax = vmax_of(p.M);
     const float vmax = vpmax == 0 ? vsmax : vpmax;
     float gamma;
     switch (c.FDORDER) {
@@ -372,6 +399,64 @@ int check_stability(const seis_csts &c, const RefParams &p, float *dtstable) {
                           : c.dh / (gamma * std::sqrt(2.0) * vmax);
     if (dtstable) *dtstable = dts;
     return c.dt > dts;
+}
+
+template <typename T>
+void to_padded(const Layout &g, const float *ref, float mul, std::vector<T> &out) {
+    out.resize((size_t)g.nfield);
+    T *o = out.data();
+    const long nf = g.nfield;
+    const bool par = nf >= kParCells;
+#pragma omp parallel for schedule(static) if (par)
+    for (long i = 0; i < nf; ++i) o[i] = 0;
+#pragma omp parallel for schedule(static) if (par)
+    for (int x = 0; x < g.NX; ++x)
+        for (int y = 0; y < g.NY; ++y) {
+            const float *src = ref + ((size_t)x * g.NY + y) * g.NZ;
+            T *dst = o + g.idx(0, y, x);
+            if (mul == 1)
+                for (int z = 0; z < g.NZ; ++z) dst[z] = (T)src[z];
+            else
+                for (int z = 0; z < g.NZ; ++z) dst[z] = (T)src[z] * (T)mul;
+        }
+}
+template void to_padded<float>(const Layout &, const float *, float, std::vector<float> &);
+template void to_padded<double>(const Layout &, const float *, float, std::vector<double> &);
+
+void visco_coeffs(const seis_csts &c, const RefParams &p, float DT, std::vector<float> &g,
+                  std::vector<float> &f, std::vector<float> &d, std::vector<float> &e) {
+    const long n = (long)p.M.size();
+    const int L = c.L;
+    const bool elastic = c.ND != 22;
+    g.resize(n);
+    e.resize(n);
+    if (elastic) {
+        f.resize(n);
+        d.resize(n);
+    }
+#pragma omp parallel for schedule(static) if (n >= kParCells)
+    for (long i = 0; i < n; ++i) {
+        g[i] = p.M[i] * (1.0 + (float)L * p.taup[i]);
+        if (elastic) {
+            f[i] = 2.0 * p.mu[i] * (1.0 + (float)L * p.taus[i]);
+            d[i] = 2.0 * p.mu[i] * p.taus[i] / DT;
+            e[i] = p.M[i] * p.taup[i] / DT;
+        } else {
+            e[i] = p.M[i] * p.taup[i];
+        }
+    }
+}
+
+void visco_pair(const std::vector<float> &mu, const std::vector<float> &ts, int L, float DT,
+                std::vector<float> &ff, std::vector<float> &dd) {
+    const long n = (long)mu.size();
+    ff.resize(n);
+    dd.resize(n);
+#pragma omp parallel for schedule(static) if (n >= kParCells)
+    for (long i = 0; i < n; ++i) {
+        ff[i] = mu[i] * (1.0 + (float)L * ts[i]);
+        dd[i] = mu[i] * ts[i] / DT;
+    }
 }
 
 std::vector<std::string> out_names(const seis_csts &c) {
