@@ -241,8 +241,9 @@ def check_stability(params: dict, **csts):
     return bool(bad), float(dts.value)
 
 
-FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE, FIX_SURF_SYY = 1, 2, 4, 8
-FIX_ALL = 15
+FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE, FIX_SURF_SYY, FIX_VISCO_F0 = 1, 2, 4, 8, 16
+FIX_ALL = 31
+HISTORY = 32    # elastic runs also take the forward state from the stored history
 
 
 def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FIX_ALL,

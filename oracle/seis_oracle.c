@@ -720,6 +720,13 @@ static void update_s2D(sim *S) {
                 float lsxz = (fipkp * (vxz + vzx)) + (DT2 * sumrxz);
                 float lsxx = ((g * (vxx + vzz)) - (f * vzz)) + (DT2 * sumrxx);
                 float lszz = ((g * (vxx + vzz)) - (f * vxx)) + (DT2 * sumrzz);
+                if (S->inc[0]) {
+                    /* the partial increments, which the reference's viscoelastic */
+                    /* gradient reads (src/update_adjs2D.cl:362-364, 441-451)     */
+                    S->inc[0][indv] = lsxx;
+                    S->inc[1][indv] = lszz;
+                    S->inc[2][indv] = lsxz;
+                }
                 sumrxz = sumrxx = sumrzz = 0;
                 for (int l = 0; l < LVE; l++) {
                     float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
@@ -1207,6 +1214,15 @@ static void update_s3D(sim *S) {
                     float lsxx = ((g * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
                     float lsyy = ((g * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
                     float lszz = ((g * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
+                    if (S->inc[0]) {
+                        /* partial increments (src/update_adjs3D.cl:570-575, 679-689) */
+                        S->inc[0][indv] = lsxx;
+                        S->inc[1][indv] = lsyy;
+                        S->inc[2][indv] = lszz;
+                        S->inc[3][indv] = lsxy;
+                        S->inc[4][indv] = lsyz;
+                        S->inc[5][indv] = lsxz;
+                    }
                     sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
                     for (int l = 0; l < LVE; l++) {
                         float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
@@ -2075,7 +2091,11 @@ static void grad_rho(const sim *S, const sim *A, double *grho) {
             }
 }
 
-static void grad_Mmu(const sim *S, const sim *A, double *gM, double *gmu) {
+/* lMs, lmus: the moduli of the stress-to-strain conversion (the relaxed   */
+/* ones in the reference; those at f0 for viscoelastic runs with           */
+/* OR_FIX_VISCO_F0).                                                       */
+static void grad_Mmu(const sim *S, const sim *A, const float *lMs, const float *lmus,
+                     double *gM, double *gmu) {
     const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
     const long nzi = NZ - 2 * FDOH, nyi = S->ND == 3 ? NY - 2 * FDOH : 1;
     const int y0 = S->ND == 3 ? FDOH : 0, y1 = S->ND == 3 ? NY - FDOH : 1;
@@ -2085,7 +2105,7 @@ static void grad_Mmu(const sim *S, const sim *A, double *gM, double *gmu) {
             for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
                 long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - y0) * nzi + (gidz - FDOH);
                 long indv = (long)gidx * NY * NZ + (long)gidy * NZ + gidz;
-                float lM = S->M[indp], lmu = S->mu[indp];
+                float lM = lMs[indp], lmu = lmus[indp];
                 if (S->ND == 2) {
                     float lsxx = A->inc[0][indv], lszz = A->inc[1][indv], lsxz = A->inc[2][indv];
                     float sxx = S->sxx[indv], szz = S->szz[indv], sxz = S->sxz[indv];
@@ -2285,14 +2305,14 @@ static void res_scale(const sim *S, int nrec, const float *rec_pos, float *res, 
 /* (finite differences of the misfit confirm the scaled form).              */
 /* double from the double accumulators.  Outputs follow param_type: 0      */
 /* vp, vs, rho; 1 M, mu, rho; 2 Ip, Is, rho.                                 */
-static void transf_grad(const sim *S, const double *gM, const double *gmu, const double *grho,
-                        float *out, int rho_fix) {
+static void transf_grad(const sim *S, const float *Ms, const float *mus, const double *gM,
+                        const double *gmu, const double *grho, float *out, int rho_fix) {
     const or_csts *c = S->c;
     const long n = S->np;
     for (long i = 0; i < n; i++) {
         double rho = 1.0 / S->rho[i] * c->dt / c->dh;
-        double M = (double)S->M[i] * c->dh / c->dt;
-        double mu = (double)S->mu[i] * c->dh / c->dt;
+        double M = (double)Ms[i] * c->dh / c->dt;
+        double mu = (double)mus[i] * c->dh / c->dt;
         double gr = grho[i] / c->dt, gm = gM[i] / c->dt, gu = gmu[i] / c->dt;
         /* the density kernel accumulates v . dv_adjoint, whose scale differs */
         /* from dJ/drho by dh^2/dt^2; the reference omits that factor         */
@@ -2320,8 +2340,11 @@ static int adj_alloc(sim *A, const sim *S, char *err, int errlen) {
     if (sim_init(A, S->c, err, errlen)) return 1;
     free(A->M); free(A->mu); free(A->rho); free(A->rip); free(A->rjp); free(A->rkp);
     free(A->muipkp); free(A->muipjp); free(A->mujpkp);
+    free(A->taup); free(A->taus); free(A->tausipkp); free(A->tausipjp); free(A->tausjpkp);
     A->M = S->M; A->mu = S->mu; A->rho = S->rho; A->rip = S->rip; A->rjp = S->rjp;
     A->rkp = S->rkp; A->muipkp = S->muipkp; A->muipjp = S->muipjp; A->mujpkp = S->mujpkp;
+    A->taup = S->taup; A->taus = S->taus; A->tausipkp = S->tausipkp;
+    A->tausipjp = S->tausipjp; A->tausjpkp = S->tausjpkp;
     int ninc = S->ND == 3 ? 6 : 3;
     for (int q = 0; q < ninc; q++) A->inc[q] = zalloc(S->nv);
     return 0;
@@ -2329,6 +2352,7 @@ static int adj_alloc(sim *A, const sim *S, char *err, int errlen) {
 
 static void adj_free(sim *A) {
     A->M = A->mu = A->rho = A->rip = A->rjp = A->rkp = A->muipkp = A->muipjp = A->mujpkp = NULL;
+    A->taup = A->taus = A->tausipkp = A->tausipjp = A->tausjpkp = NULL;
     for (int q = 0; q < 6; q++) {
         free(A->inc[q]);
         A->inc[q] = NULL;
@@ -2347,8 +2371,8 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         err = dummy;
         errlen = sizeof(dummy);
     }
-    if (c->ND == 22 || c->L > 0) {
-        snprintf(err, errlen, "gradients need the elastic case (ND 2 or 3, L=0)");
+    if (c->ND == 22) {
+        snprintf(err, errlen, "gradients need the elastic or viscoelastic case (ND 2 or 3)");
         return 1;
     }
     if (sim_init(&S, c, err, errlen)) {
@@ -2382,11 +2406,43 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         return 1;
     }
     const int nvb = S.ND == 3 ? 9 : 5, nout = or_nout(c);
-    float *bnd = malloc(sizeof(float) * (size_t)tmax * nvb * B.n);
+    /* Viscoelastic runs cannot be reconstructed backwards (the memory-     */
+    /* variable recursion decays forwards and grows backwards, which makes  */
+    /* the reference's backpropagation unstable): the adjoint reads the     */
+    /* forward state each step needs from a stored history instead, here    */
+    /* the post-source velocities and stresses of every step t in           */
+    /* (tmin, tmax).  What the elastic backpropagation reconstructs at step  */
+    /* t is exactly this state: the stresses and velocities after sources(t) */
+    /* and before update_v(t).                                                */
+    const int visco = c->L > 0 || (gradfix & OR_HISTORY);
+    float *hist = NULL;
+    const size_t nh = (size_t)(tmax - tmin - 1) * nvb;
+    float *fld[9];
+    int nfld = 0;
+    if (S.ND == 3) {
+        float *f3[9] = {S.vx, S.vy, S.vz, S.sxx, S.syy, S.szz, S.sxy, S.syz, S.sxz};
+        for (int q = 0; q < 9; q++) fld[nfld++] = f3[q];
+    } else {
+        float *f2[5] = {S.vx, S.vz, S.sxx, S.szz, S.sxz};
+        for (int q = 0; q < 5; q++) fld[nfld++] = f2[q];
+    }
+    float *bnd = visco ? NULL : malloc(sizeof(float) * (size_t)tmax * nvb * B.n);
+    if (visco) {
+        hist = malloc(sizeof(float) * (nh ? nh : 1) * S.nv);
+        if (!hist) {
+            snprintf(err, errlen, "out of host memory for the forward history");
+            sim_free(&S);
+            return 1;
+        }
+    }
     memset(out, 0, sizeof(float) * (size_t)nout * nrec * c->NT);
     /* forward with boundary saving: src/time_stepping.c:755-835 */
     for (int t = 0; t < tmax; t++) {
         sources(&S, t, 1, nsrc, src_pos, src);
+        if (visco && t > tmin)
+            for (int q = 0; q < nfld; q++)
+                memcpy(hist + ((size_t)(t - tmin - 1) * nvb + q) * S.nv, fld[q],
+                       sizeof(float) * S.nv);
         if (S.ND == 3) {
             update_v3D(&S);
             update_s3D(&S);
@@ -2394,7 +2450,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
             update_v2D(&S);
             update_s2D(&S);
         }
-        save_shell(&S, &B, bnd + (size_t)t * nvb * B.n);
+        if (!visco) save_shell(&S, &B, bnd + (size_t)t * nvb * B.n);
         if (c->freesurf) {
             if (S.ND == 3) surface3D(&S, 1);
             else surface2D(&S, 1);
@@ -2416,6 +2472,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     /* adjoint: src/time_stepping.c:869-940 */
     if (adj_alloc(&A, &S, err, errlen)) {
         free(bnd);
+        free(hist);
         free(res);
         sim_free(&S);
         return 1;
@@ -2437,22 +2494,61 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         for (int q = 0; q < 18; q++)
             if (*fw[q] && *aw[q]) memcpy(*aw[q], *fw[q], sizeof(float) * sizes[axis[q]]);
     }
+    /* Moduli of the gradient formulas.  The reference uses the relaxed     */
+    /* moduli M, mu for the stress-to-strain conversion and the chain rule   */
+    /* (src/update_adjs2D.cl:441-451, src/calc_grad.c:1020-1056); for a       */
+    /* viscoelastic medium the stresses relate to the strains through the    */
+    /* moduli at the reference frequency f0, larger by the relaxation factor */
+    /* 1+s, which inflates the reference's gradient by about (1+s)^(3/2).    */
+    /* With OR_FIX_VISCO_F0 the formulas use the moduli at f0: what the      */
+    /* transform gives without the relaxation division (that is, with L=0).  */
+    const float *Mg = S.M, *mug = S.mu;
+    float *Mf = NULL, *muf = NULL;
+    if (c->L > 0 && (gradfix & OR_FIX_VISCO_F0)) {
+        or_csts c0 = *c;
+        c0.L = 0;
+        float *tmp[14];
+        for (int q = 0; q < 14; q++) tmp[q] = zalloc(S.np);
+        Mf = tmp[0];
+        muf = tmp[1];
+        or_transform(&c0, p0, p1, rho, taup, taus, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4],
+                     tmp[5], tmp[6], tmp[7], tmp[8], tmp[9], tmp[10], tmp[11], tmp[12], tmp[13]);
+        for (int q = 2; q < 14; q++) free(tmp[q]);
+        Mg = Mf;
+        mug = muf;
+    }
     double *gM = calloc(S.np, sizeof(double)), *gmu = calloc(S.np, sizeof(double)),
            *grho = calloc(S.np, sizeof(double));
     for (int t = tmax - 1; t > tmin; t--) {
-        const float *bprev = bnd + (size_t)(t - 1) * nvb * B.n;
         if (c->freesurf) {
             if (S.ND == 3) surface_adj3D(&A, gradfix & OR_FIX_SURF_SYY);
             else surface_adj2D(&A);
         }
         residuals_inject(&A, t, nrec, rec_pos, res);
+        if (visco) {
+            /* the forward state of step t from the history; the adjoint      */
+            /* stress update with memory variables is the forward one run on  */
+            /* the adjoint fields (src/update_adjs2D.cl:367-408,               */
+            /* src/update_adjs3D.cl:549-605)                                   */
+            for (int q = 0; q < nfld; q++)
+                memcpy(fld[q], hist + ((size_t)(t - tmin - 1) * nvb + q) * S.nv,
+                       sizeof(float) * S.nv);
+            if (S.ND == 3) update_s3D(&A);
+            else update_s2D(&A);
+            grad_Mmu(&S, &A, Mg, mug, gM, gmu);
+            if (S.ND == 3) update_v3D(&A);
+            else update_v2D(&A);
+            grad_rho(&S, &A, grho);
+            continue;
+        }
+        const float *bprev = bnd + (size_t)(t - 1) * nvb * B.n;
         /* update_adjs: backpropagate the stresses, re-inject their shell,  */
         /* update the adjoint stresses, accumulate the M and mu gradients    */
         backprop_s(&S);
         inject_shell(&S, &B, bprev, S.ND == 3 ? 3 : 2, S.ND == 3 ? 6 : 3);
         if (S.ND == 3) update_s3D(&A);
         else update_s2D(&A);
-        grad_Mmu(&S, &A, gM, gmu);
+        grad_Mmu(&S, &A, Mg, mug, gM, gmu);
         /* update_adjv: the same for the velocities and the density */
         backprop_v(&S);
         inject_shell(&S, &B, bprev, 0, S.ND == 3 ? 3 : 2);
@@ -2461,7 +2557,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         grad_rho(&S, &A, grho);
         sources(&S, t, -1, nsrc, src_pos, src);
     }
-    transf_grad(&S, gM, gmu, grho, grad, gradfix & OR_FIX_RHO_SCALE);
+    transf_grad(&S, Mg, mug, gM, gmu, grho, grad, gradfix & OR_FIX_RHO_SCALE);
     if (fields_out) {
         for (int v = 0; v < or_nvars(c); v++)
             memcpy(&fields_out[v * S.nv], named_var(&S, or_varname(c, v)),
@@ -2470,7 +2566,10 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     free(gM);
     free(gmu);
     free(grho);
+    free(Mf);
+    free(muf);
     free(bnd);
+    free(hist);
     free(res);
     adj_free(&A);
     sim_free(&S);

@@ -26,6 +26,10 @@ extern "C" {
 #define OR_FIX_ADJ_CPML 2
 #define OR_FIX_RHO_SCALE 4
 #define OR_FIX_SURF_SYY 8
+#define OR_FIX_VISCO_F0 16
+/* checker only: elastic runs also read the forward state from the stored
+ * history (to pin that path against backpropagation) */
+#define OR_HISTORY 32
 
 typedef struct or_csts {
     int ND;            /* 2: 2D P-SV elastic, 3: 3D, 22: 2D acoustic           */
@@ -92,9 +96,16 @@ double or_last_loop_seconds(void);
  * inverse maps covering the same cells) and the shell size in *nbnd. */
 int  or_shell_check(const or_csts *c, long *nbnd);
 
-/* Gradient of one shot by backpropagation with boundary saving
- * (src/time_stepping.c:747-1002, BACK_PROP_TYPE=1), elastic L=0.  Adjoint sources: res_in [nout][nrec][NT] when given
- * (inputres), else out - din; both are scaled as res_scale does.
+/* Gradient of one shot (src/time_stepping.c:747-1002, BACK_PROP_TYPE=1),
+ * ND 2 or 3.  Elastic runs reconstruct the forward wavefield by
+ * backpropagation with boundary saving, as the reference does; viscoelastic
+ * runs (L>0) read it from a stored history of post-source states (the
+ * reference's viscoelastic backpropagation is unstable) and update the
+ * adjoint memory variables with the forward recursion (src/update_adjs2D.cl:
+ * 367-408, src/update_adjs3D.cl:549-605, with the memory variables of each
+ * l indexed by l and written to the adjoint arrays).  Adjoint sources:
+ * res_in [nout][nrec][NT] when given (inputres), else out - din; both are
+ * scaled as res_scale does.
  * gradfix selects corrections of reference defects (0 = as the reference):
  *   OR_FIX_RES_SCALE  res_scale reads the buoyancy at the receiver's own
  *                     cell (the reference uses x for all three indices);
@@ -103,7 +114,11 @@ int  or_shell_check(const or_csts *c, long *nbnd);
  *   OR_FIX_RHO_SCALE  the direct density term of the gradient is scaled by
  *                     dh^2/dt^2;
  *   OR_FIX_SURF_SYY   the 3D adjoint free surface reads syy (the reference
- *                     reads sxx twice).
+ *                     reads sxx twice);
+ *   OR_FIX_VISCO_F0   viscoelastic runs: the gradient formulas use the
+ *                     moduli at the reference frequency f0 (the reference
+ *                     uses the relaxed moduli, which inflates the gradient
+ *                     by about (1+s)^(3/2), s the relaxation factor).
  * grad: [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), gradients
  * accumulated in double.  res_out (optional): the scaled adjoint sources;
  * rms (optional): [rms, rmsnorm]; fields_out (optional): the backpropagated
