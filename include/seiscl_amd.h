@@ -62,8 +62,12 @@ typedef struct seis_csts {
                         /*  4 the direct density term is scaled by dh^2/dt^2      */
                         /*    (src/calc_grad.c:1005-1013 omits it);               */
                         /*  8 the 3D adjoint free surface reads syy (the          */
-                        /*    reference reads sxx twice, src/surface3D_adj.cl:69).*/
-                        /* seis_csts_default sets 15; 0 reproduces the reference. */
+                        /*    reference reads sxx twice, src/surface3D_adj.cl:69);*/
+                        /* 16 viscoelastic gradients use the moduli at f0 for the */
+                        /*    stress-to-strain conversion and the chain rule (the */
+                        /*    reference uses the relaxed ones, src/update_adjs2D. */
+                        /*    cl:441-451, src/calc_grad.c:1020-1056).             */
+                        /* seis_csts_default sets 31; 0 reproduces the reference. */
 } seis_csts;
 
 typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
@@ -146,15 +150,18 @@ int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos,
 /* 272-350, and by parity tests).                                              */
 int seis_read_field(seis_model *m, const char *name, float *host);
 
-/* ---- gradient by backpropagation (BACK_PROP_TYPE=1), elastic ND 2/3,      */
-/* L=0, FP32, with or without a free surface (adjoint surface:               */
-/* src/surface2D_adj.cl, src/surface3D_adj.cl).  The forward run saves an    */
-/* FDOH-thick boundary shell */
-/* of every wavefield at every step in device memory (save_bnd,              */
-/* src/time_stepping.c:352-395, kept resident instead of the reference's     */
-/* pinned host copy); the adjoint run re-injects it while running the        */
-/* forward wavefield backwards (inject_bnd, :397-424) and accumulates the    */
-/* gradient in double (update_adjs / update_adjv, src/update_adj*.cl).       */
+/* ---- gradient by backpropagation (BACK_PROP_TYPE=1), ND 2/3, FP32, with */
+/* or without a free surface (adjoint surface: src/surface2D_adj.cl,         */
+/* src/surface3D_adj.cl).  Elastic (L=0): the forward run saves an           */
+/* FDOH-thick boundary shell of every wavefield at every step in device      */
+/* memory (save_bnd, src/time_stepping.c:352-395, kept resident instead of   */
+/* the reference's pinned host copy); the adjoint run re-injects it while    */
+/* running the forward wavefield backwards (inject_bnd, :397-424).           */
+/* Viscoelastic (L=1..4): running the memory variables backwards is          */
+/* unstable, so the forward run keeps full-state checkpoints every K steps   */
+/* in device memory and the adjoint recomputes each segment forward, keeping */
+/* the post-source states it reads.  Both accumulate the gradient in double  */
+/* (update_adjs / update_adjv, src/update_adj*.cl).                          */
 /* Zero the gradient accumulators (kernel_gradinit, src/automatic_kernels.c: */
 /* 840-930; src/time_stepping.c:725-729).                                     */
 int seis_grad_zero(seis_model *m);

@@ -378,6 +378,279 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
 }
 
 // ---------------------------------------------------------------------------
+// Viscoelastic adjoint steps (L > 0).  The forward state of the step comes
+// from the recomputed history (A.st), not from backpropagation, which is
+// unstable for the memory-variable recursion.  The adjoint stress update is
+// the forward viscoelastic recursion run on the adjoint fields and their own
+// memory variables, each l at its own slab (src/update_adjs2D.cl:367-408,
+// src/update_adjs3D.cl:549-605 index and write them otherwise); the gradient
+// reads the partial increments, as the reference's does.
+// ---------------------------------------------------------------------------
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_adjh_s2d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &a = A.r;
+    const Layout &g = a.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = a.hc;
+    R vxx = Dm<H>(a.vx, i, sx, hc), vxz = Dp<H>(a.vx, i, 1, hc);
+    R vzz = Dm<H>(a.vz, i, 1, hc), vzx = Dp<H>(a.vz, i, sx, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const long ip = (long)x * 2 * NAB + k;
+            pml_half(a, 2, vxz, a.psi_vx_z + ip, bot ? 2 * NAB - 1 - k : k);
+            pml_full(a, 2, vzz, a.psi_vz_z + ip, bot ? 2 * NAB - k : k);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const long ip = (long)ii * NZ + z;
+            pml_full(a, 0, vxx, a.psi_vx_x + ip, x < NAB ? ii : 2 * NAB - ii);
+            pml_half(a, 0, vzx, a.psi_vz_x + ip, x < NAB ? ii : 2 * NAB - 1 - ii);
+        }
+    }
+    const R DT2 = a.DT2;
+    const R gg = a.cg[i], f = a.cf[i], d = a.cd[i], e = a.ce[i];
+    const R fipkp = a.fipkp[i], dipkp = a.dipkp[i];
+    const long nf = g.nfield;
+    R rxz[L], rxx[L], rzz[L];
+    R sumrxz = 0, sumrxx = 0, sumrzz = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const long ir = l * nf + i;
+        rxz[l] = a.rxz[ir]; rxx[l] = a.rxx[ir]; rzz[l] = a.rzz[ir];
+        sumrxz += rxz[l]; sumrxx += rxx[l]; sumrzz += rzz[l];
+    }
+    const R lsxz = (fipkp * (vxz + vzx)) + (DT2 * sumrxz);
+    const R lsxx = ((gg * (vxx + vzz)) - (f * vzz)) + (DT2 * sumrxx);
+    const R lszz = ((gg * (vxx + vzz)) - (f * vxx)) + (DT2 * sumrzz);
+    sumrxz = sumrxx = sumrzz = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
+        const long ir = l * nf + i;
+        rxz[l] = b * (rxz[l] * c - et * (dipkp * (vxz + vzx)));
+        rxx[l] = b * (rxx[l] * c - et * ((e * (vxx + vzz)) - (d * vzz)));
+        rzz[l] = b * (rzz[l] * c - et * ((e * (vxx + vzz)) - (d * vxx)));
+        a.rxz[ir] = rxz[l]; a.rxx[ir] = rxx[l]; a.rzz[ir] = rzz[l];
+        sumrxz += rxz[l]; sumrxx += rxx[l]; sumrzz += rzz[l];
+    }
+    R s3[3] = {a.sxx[i] + (lsxx + (DT2 * sumrxx)), a.szz[i] + (lszz + (DT2 * sumrzz)),
+               a.sxz[i] + (lsxz + (DT2 * sumrxz))};
+    if (a.abs_type == 2) taper_apply(a, z, 0, x, s3, 3);
+    a.sxx[i] = s3[0];
+    a.szz[i] = s3[1];
+    a.sxz[i] = s3[2];
+    // M and mu gradients (src/update_adjs2D.cl:441-451) from the forward state
+    const R sxx = A.st[2][i], szz = A.st[3][i], sxz = A.st[4][i];
+    const R lM = A.Ms[i], lmu = A.mus[i];
+    const R c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
+    const R c3 = 1.0 / (lmu * lmu);
+    const R c5 = 0.25 * c3;
+    const R dM = c1 * (sxx + szz) * (lsxx + lszz);
+    A.gM[i] += -dM;
+    A.gmu[i] += -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+}
+
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_adjh_v2d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &a = A.r;
+    const Layout &g = a.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, 0, x), sx = g.sx;
+    const R *hc = a.hc;
+    const R rip = a.rip[i], rkp = a.rkp[i];
+    R sxx_x = Dp<H>(a.sxx, i, sx, hc), szz_z = Dp<H>(a.szz, i, 1, hc);
+    R sxz_z = Dm<H>(a.sxz, i, 1, hc), sxz_x = Dm<H>(a.sxz, i, sx, hc);
+    const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const long ip = (long)x * 2 * NAB + k;
+            pml_full(a, 2, sxz_z, a.psi_sxz_z + ip, bot ? 2 * NAB - k : k);
+            pml_half(a, 2, szz_z, a.psi_szz_z + ip, bot ? 2 * NAB - 1 - k : k);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const long ip = (long)ii * NZ + z;
+            pml_half(a, 0, sxx_x, a.psi_sxx_x + ip, x < NAB ? ii : 2 * NAB - 1 - ii);
+            pml_full(a, 0, sxz_x, a.psi_sxz_x + ip, x < NAB ? ii : 2 * NAB - ii);
+        }
+    }
+    const R lvx = ((sxx_x + sxz_z) * rip);
+    const R lvz = ((szz_z + sxz_x) * rkp);
+    R v2[2] = {a.vx[i] + lvx, a.vz[i] + lvz};
+    if (a.abs_type == 2) taper_apply(a, z, 0, x, v2, 2);
+    a.vx[i] = v2[0];
+    a.vz[i] = v2[1];
+    // density gradient (src/update_adjv2D.cl:326) from the forward state
+    const R d = -A.st[0][i] * lvx - A.st[1][i] * lvz;
+    A.grho[i] += d;
+}
+
+template <typename R, int H, int L>
+__global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &a = A.r;
+    const Layout &g = a.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
+    const R *hc = a.hc;
+    R vxx = Dm<H>(a.vx, i, sx, hc), vxy = Dp<H>(a.vx, i, sy, hc), vxz = Dp<H>(a.vx, i, 1, hc);
+    R vyx = Dp<H>(a.vy, i, sx, hc), vyy = Dm<H>(a.vy, i, sy, hc), vyz = Dp<H>(a.vy, i, 1, hc);
+    R vzx = Dp<H>(a.vz, i, sx, hc), vzy = Dp<H>(a.vz, i, sy, hc), vzz = Dm<H>(a.vz, i, 1, hc);
+    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int ki = bot ? 2 * NAB - k : k, kh = bot ? 2 * NAB - 1 - k : k;
+            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+            pml_half(a, 2, vxz, a.psi_vx_z + ip, kh);
+            pml_half(a, 2, vyz, a.psi_vy_z + ip, kh);
+            pml_full(a, 2, vzz, a.psi_vz_z + ip, ki);
+        }
+        if (y < NAB || y > NY - NAB - 1) {
+            const int j = y < NAB ? y : y - NY + 2 * NAB;
+            const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
+            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+            pml_half(a, 1, vxy, a.psi_vx_y + ip, jh);
+            pml_full(a, 1, vyy, a.psi_vy_y + ip, ji);
+            pml_half(a, 1, vzy, a.psi_vz_y + ip, jh);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
+            pml_full(a, 0, vxx, a.psi_vx_x + ip, iI);
+            pml_half(a, 0, vyx, a.psi_vy_x + ip, iH);
+            pml_half(a, 0, vzx, a.psi_vz_x + ip, iH);
+        }
+    }
+    const R vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+    const R vxxyyzz = vxx + vyy + vzz, vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
+    const R DT2 = a.DT2;
+    const R fipjp = a.fipjp[i], fjpkp = a.fjpkp[i], fipkp = a.fipkp[i];
+    const R gg = a.cg[i], f = a.cf[i], d = a.cd[i], e = a.ce[i];
+    const R dipjp = a.dipjp[i], djpkp = a.djpkp[i], dipkp = a.dipkp[i];
+    const long nf = g.nfield;
+    R rxy[L], ryz[L], rxz[L], rxx[L], ryy[L], rzz[L];
+    R sumrxy = 0, sumryz = 0, sumrxz = 0, sumrxx = 0, sumryy = 0, sumrzz = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const long ir = l * nf + i;
+        rxy[l] = a.rxy[ir]; ryz[l] = a.ryz[ir]; rxz[l] = a.rxz[ir];
+        rxx[l] = a.rxx[ir]; ryy[l] = a.ryy[ir]; rzz[l] = a.rzz[ir];
+        sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
+        sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
+    }
+    const R lsxy = (fipjp * vxyyx) + (DT2 * sumrxy);
+    const R lsyz = (fjpkp * vyzzy) + (DT2 * sumryz);
+    const R lsxz = (fipkp * vxzzx) + (DT2 * sumrxz);
+    const R lsxx = ((gg * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
+    const R lsyy = ((gg * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
+    const R lszz = ((gg * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
+    sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
+        const long ir = l * nf + i;
+        rxy[l] = b * (rxy[l] * c - et * (dipjp * vxyyx));
+        ryz[l] = b * (ryz[l] * c - et * (djpkp * vyzzy));
+        rxz[l] = b * (rxz[l] * c - et * (dipkp * vxzzx));
+        rxx[l] = b * (rxx[l] * c - et * ((e * vxxyyzz) - (d * vyyzz)));
+        ryy[l] = b * (ryy[l] * c - et * ((e * vxxyyzz) - (d * vxxzz)));
+        rzz[l] = b * (rzz[l] * c - et * ((e * vxxyyzz) - (d * vxxyy)));
+        a.rxy[ir] = rxy[l]; a.ryz[ir] = ryz[l]; a.rxz[ir] = rxz[l];
+        a.rxx[ir] = rxx[l]; a.ryy[ir] = ryy[l]; a.rzz[ir] = rzz[l];
+        sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
+        sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
+    }
+    R s6[6] = {a.sxy[i] + (lsxy + (DT2 * sumrxy)), a.syz[i] + (lsyz + (DT2 * sumryz)),
+               a.sxz[i] + (lsxz + (DT2 * sumrxz)), a.sxx[i] + (lsxx + (DT2 * sumrxx)),
+               a.syy[i] + (lsyy + (DT2 * sumryy)), a.szz[i] + (lszz + (DT2 * sumrzz))};
+    if (a.abs_type == 2) taper_apply(a, z, y, x, s6, 6);
+    a.sxy[i] = s6[0];
+    a.syz[i] = s6[1];
+    a.sxz[i] = s6[2];
+    a.sxx[i] = s6[3];
+    a.syy[i] = s6[4];
+    a.szz[i] = s6[5];
+    // M and mu gradients (src/update_adjs3D.cl:679-689) from the forward state
+    const R sxx = A.st[3][i], syy = A.st[4][i], szz = A.st[5][i];
+    const R sxy = A.st[6][i], syz = A.st[7][i], sxz = A.st[8][i];
+    const R lM = A.Ms[i], lmu = A.mus[i];
+    const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
+    const R c3 = 1.0 / lmu / lmu;
+    const R c5 = 1.0 / 6.0 * c3;
+    const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
+    A.gM[i] += -dM;
+    A.gmu[i] += -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
+                c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
+                      lszz * (2.0 * szz - sxx - syy));
+}
+
+template <typename R, int H>
+__global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigned nb) {
+    const FDArgs<R> &a = A.r;
+    const Layout &g = a.g;
+    int z, y, x;
+    if (!adj_cell(g, nbz, nb, z, y, x)) return;
+    const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
+    const R *hc = a.hc;
+    const R rip = a.rip[i], rjp = a.rjp[i], rkp = a.rkp[i];
+    R szz_z = Dp<H>(a.szz, i, 1, hc), sxx_x = Dp<H>(a.sxx, i, sx, hc);
+    R sxz_x = Dm<H>(a.sxz, i, sx, hc), sxz_z = Dm<H>(a.sxz, i, 1, hc);
+    R syz_y = Dm<H>(a.syz, i, sy, hc), syz_z = Dm<H>(a.syz, i, 1, hc);
+    R syy_y = Dp<H>(a.syy, i, sy, hc), sxy_x = Dm<H>(a.sxy, i, sx, hc);
+    R sxy_y = Dm<H>(a.sxy, i, sy, hc);
+    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
+    if (a.abs_type == 1) {
+        if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
+            const bool bot = z > NZ - NAB - 1;
+            const int k = bot ? z - NZ + 2 * NAB : z;
+            const int ki = bot ? 2 * NAB - k : k, kh = bot ? 2 * NAB - 1 - k : k;
+            const long ip = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
+            pml_full(a, 2, sxz_z, a.psi_sxz_z + ip, ki);
+            pml_full(a, 2, syz_z, a.psi_syz_z + ip, ki);
+            pml_half(a, 2, szz_z, a.psi_szz_z + ip, kh);
+        }
+        if (y < NAB || y > NY - NAB - 1) {
+            const int j = y < NAB ? y : y - NY + 2 * NAB;
+            const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
+            const long ip = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
+            pml_full(a, 1, sxy_y, a.psi_sxy_y + ip, ji);
+            pml_half(a, 1, syy_y, a.psi_syy_y + ip, jh);
+            pml_full(a, 1, syz_y, a.psi_syz_y + ip, ji);
+        }
+        if (x < NAB || x > NX - NAB - 1) {
+            const int ii = x < NAB ? x : x - NX + 2 * NAB;
+            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
+            const long ip = (long)ii * NY * NZ + (long)y * NZ + z;
+            pml_half(a, 0, sxx_x, a.psi_sxx_x + ip, iH);
+            pml_full(a, 0, sxy_x, a.psi_sxy_x + ip, iI);
+            pml_full(a, 0, sxz_x, a.psi_sxz_x + ip, iI);
+        }
+    }
+    const R lvx = ((sxx_x + sxy_y + sxz_z) * rip);
+    const R lvy = ((syy_y + sxy_x + syz_z) * rjp);
+    const R lvz = ((szz_z + sxz_x + syz_y) * rkp);
+    R v3[3] = {a.vx[i] + lvx, a.vy[i] + lvy, a.vz[i] + lvz};
+    if (a.abs_type == 2) taper_apply(a, z, y, x, v3, 3);
+    a.vx[i] = v3[0];
+    a.vy[i] = v3[1];
+    a.vz[i] = v3[2];
+    // density gradient (src/update_adjv3D.cl:481) from the forward state
+    const R d = -A.st[0][i] * lvx - A.st[1][i] * lvy - A.st[2][i] * lvz;
+    A.grho[i] += d;
+}
+
+// ---------------------------------------------------------------------------
 // Boundary saving and adjoint sources
 // ---------------------------------------------------------------------------
 template <typename R>
@@ -573,6 +846,35 @@ void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, vo
     SEIS_SWITCH_H(r.g.H, (run_surf_adj<R, HH>(r, tmp, nd, syy_fix, (hipStream_t)stream)));
 }
 
+template <typename R, int H, int L>
+void run_adjh_s(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
+    int nbz;
+    unsigned nb;
+    grid_adj<R>(a.f.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, ABY);
+    if (nd_case == 3) hipLaunchKernelGGL((k_adjh_s3d<R, H, L>), grid, blk, 0, st, a, nbz, nb);
+    else hipLaunchKernelGGL((k_adjh_s2d<R, H, L>), grid, blk, 0, st, a, nbz, nb);
+}
+template <typename R, int H>
+void run_adjh_v(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
+    int nbz;
+    unsigned nb;
+    grid_adj<R>(a.f.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, ABY);
+    if (nd_case == 3) hipLaunchKernelGGL((k_adjh_v3d<R, H>), grid, blk, 0, st, a, nbz, nb);
+    else hipLaunchKernelGGL((k_adjh_v2d<R, H>), grid, blk, 0, st, a, nbz, nb);
+}
+template <typename R, int H>
+void run_adjh_s_L(const AdjArgs<R> &a, int nd_case, int L, hipStream_t st) {
+    switch (L) {
+        case 1: run_adjh_s<R, H, 1>(a, nd_case, st); break;
+        case 2: run_adjh_s<R, H, 2>(a, nd_case, st); break;
+        case 3: run_adjh_s<R, H, 3>(a, nd_case, st); break;
+        case 4: run_adjh_s<R, H, 4>(a, nd_case, st); break;
+        default: break;
+    }
+}
+
 template <typename R>
 void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream) {
     SEIS_SWITCH_H(a.f.g.H, (run_adj_s<R, HH>(a, nd_case, (hipStream_t)stream)));
@@ -580,6 +882,14 @@ void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream) {
 template <typename R>
 void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream) {
     SEIS_SWITCH_H(a.f.g.H, (run_adj_v<R, HH>(a, nd_case, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, void *stream) {
+    SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, HH>(a, nd_case, L, (hipStream_t)stream)));
+}
+template <typename R>
+void launch_adjh_v(const AdjArgs<R> &a, int nd_case, void *stream) {
+    SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
 void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream) {
@@ -603,6 +913,8 @@ void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const i
 template void launch_adj_s<float>(const AdjArgs<float> &, int, void *);
 template void launch_surface_adj<float>(const FDArgs<float> &, float *, int, int, void *);
 template void launch_adj_v<float>(const AdjArgs<float> &, int, void *);
+template void launch_adjh_s<float>(const AdjArgs<float> &, int, int, void *);
+template void launch_adjh_v<float>(const AdjArgs<float> &, int, void *);
 template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *);
 template void launch_residuals<float>(const FDArgs<float> &, int, const ShotDev &, const int *,
                                       int, const float *, int, int, int, void *);

@@ -141,6 +141,14 @@ struct seis_model {
         size_t res_cap = 0;
         std::vector<float *> own_psi;
         bool saved = false;         // a forward run with boundaries is bound
+        // viscoelastic runs: full-state checkpoints every K steps and the
+        // recomputed post-source history of one segment of K steps
+        bool visco = false;
+        int K = 0, nseg = 0;
+        float *ckpt = nullptr;      // [nseg][state]
+        float *hist = nullptr;      // [K][nst][nfield]
+        size_t ckpt_bytes = 0, hist_bytes = 0;
+        float *Ms = nullptr, *mus = nullptr;   // moduli of the gradient formulas
     } grad;
 
     float *dalloc(long n) {
@@ -159,7 +167,7 @@ struct seis_model {
     ~seis_model() {
         for (void *p : owned) ctx->release(p);
         void *sp[] = {shot.src_cell, shot.src_type, shot.src_k, shot.src, shot.rec_cell, shot.out,
-                      grad.bnd, grad.res};
+                      grad.bnd, grad.res, grad.ckpt, grad.hist};
         for (void *p : sp) ctx->release(p);
         for (auto e : evpool) (void)hipEventDestroy(e);
     }
@@ -267,6 +275,9 @@ int upload_params(seis_model *m, const RefParams &p) {
     std::vector<float> g, f, d, e;
     visco_coeffs(c, p, DT, g, f, d, e);
     if (c.ND == 22) return upload(m, &a.cg, g, qs) || upload(m, &a.ce, e, qs);
+    // the adjoint free surface reads the relaxed moduli (src/surface2D_adj.cl:
+    // 67-68, src/surface3D_adj.cl), which the viscoelastic update does not
+    if (c.freesurf && (upload(m, &a.M, p.M, qs) || upload(m, &a.mu, p.mu, qs))) return 1;
     if (upload(m, &a.cg, g, qs) || upload(m, &a.cf, f, qs) || upload(m, &a.cd, d, qs) ||
         upload(m, &a.ce, e, qs))
         return 1;
@@ -489,13 +500,16 @@ int grad_setup(seis_model *m) {
     auto &G = m->grad;
     if (G.ready) return 0;
     const seis_csts &c = m->c;
-    if (c.ND == 22 || c.L > 0)
-        return fail("gradients need the elastic case (ND 2 or 3, L=0)");
+    if (c.ND == 22)
+        return fail("gradients need the elastic or viscoelastic case (ND 2 or 3)");
     if (m->prec != P_FP32) return fail("gradients are available with FP32 wavefields (FP16=0)");
-    G.sh = make_shell(m->g, c.abs_type > 0 ? c.nab : 0, c.freesurf);
-    if (G.sh.NXbnd < 2 * G.sh.F || G.sh.NZbnd < 2 * G.sh.F ||
-        (c.ND == 3 && G.sh.NYbnd < 2 * G.sh.F))
-        return fail("the grid inside the absorbing layers is too small for the boundary shell");
+    G.visco = c.L > 0;
+    if (!G.visco) {
+        G.sh = make_shell(m->g, c.abs_type > 0 ? c.nab : 0, c.freesurf);
+        if (G.sh.NXbnd < 2 * G.sh.F || G.sh.NZbnd < 2 * G.sh.F ||
+            (c.ND == 3 && G.sh.NYbnd < 2 * G.sh.F))
+            return fail("the grid inside the absorbing layers is too small for the boundary shell");
+    }
     G.nvb = c.ND == 3 ? 9 : 5;
     FDArgs<float> &r = G.r;
     r = m->a;
@@ -504,7 +518,17 @@ int grad_setup(seis_model *m) {
     const int nfl = c.ND == 3 ? 9 : 5;
     for (int q = 0; q < nfl; ++q)
         if (!(*fl[q] = m->dalloc(nf))) return 1;
-    if (c.abs_type == 1 && (c.gradfix & FIX_ADJ_CPML)) {
+    if (G.visco) {
+        // the adjoint run's own memory variables, L slabs per component
+        float **mv[] = {&r.rxx, &r.rzz, &r.rxz, &r.ryy, &r.rxy, &r.ryz};
+        for (int q = 0; q < (c.ND == 3 ? 6 : 3); ++q)
+            if (!(*mv[q] = m->dalloc(nf * c.L))) return 1;
+        if (!(G.Ms = m->dalloc(nf)) || !(G.mus = m->dalloc(nf))) return 1;
+    }
+    // own adjoint CPML memory: zeroed (FIX_ADJ_CPML), or, for viscoelastic
+    // runs whose forward memory is overwritten by the recomputation, a copy
+    // of the forward run's final memory (what the reference's kernels bind)
+    if (c.abs_type == 1 && ((c.gradfix & FIX_ADJ_CPML) || G.visco)) {
         const long nab2 = 2L * c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
         const long sxs = nab2 * NY * NZ, sys = NX * nab2 * NZ, szs = NX * NY * nab2;
         float **px[] = {&r.psi_sxx_x, &r.psi_sxy_x, &r.psi_sxz_x, &r.psi_vx_x, &r.psi_vy_x,
@@ -534,20 +558,31 @@ void grad_zero_adjoint(seis_model *m) {
     const size_t nf = (size_t)m->g.nfield * 4;
     float *fl[] = {r.vx, r.vy, r.vz, r.sxx, r.syy, r.szz, r.sxy, r.syz, r.sxz};
     for (float *p : fl) launch_fill(p, nf, m->ctx->stream);
-    if (m->c.abs_type == 1 && (m->c.gradfix & FIX_ADJ_CPML)) {
+    float *mv[] = {r.rxx, r.ryy, r.rzz, r.rxy, r.ryz, r.rxz};
+    if (m->grad.visco)
+        for (float *p : mv)
+            if (p) launch_fill(p, nf * m->c.L, m->ctx->stream);
+    const bool fix = m->c.gradfix & FIX_ADJ_CPML;
+    if (m->c.abs_type == 1 && (fix || m->grad.visco)) {
         const long nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
         const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
         float *px[] = {r.psi_sxx_x, r.psi_sxy_x, r.psi_sxz_x, r.psi_vx_x, r.psi_vy_x, r.psi_vz_x};
         float *py[] = {r.psi_sxy_y, r.psi_syy_y, r.psi_syz_y, r.psi_vx_y, r.psi_vy_y, r.psi_vz_y};
         float *pz[] = {r.psi_sxz_z, r.psi_syz_z, r.psi_szz_z, r.psi_vx_z, r.psi_vy_z, r.psi_vz_z};
-        for (float *p : px)
-            if (p) launch_fill(p, sx, m->ctx->stream);
-        for (float *p : py)
-            if (p) launch_fill(p, sy, m->ctx->stream);
-        for (float *p : pz)
-            if (p) launch_fill(p, sz, m->ctx->stream);
+        float *fx[] = {a.psi_sxx_x, a.psi_sxy_x, a.psi_sxz_x, a.psi_vx_x, a.psi_vy_x, a.psi_vz_x};
+        float *fy[] = {a.psi_sxy_y, a.psi_syy_y, a.psi_syz_y, a.psi_vx_y, a.psi_vy_y, a.psi_vz_y};
+        float *fz[] = {a.psi_sxz_z, a.psi_syz_z, a.psi_szz_z, a.psi_vx_z, a.psi_vy_z, a.psi_vz_z};
+        auto put = [&](float *dst, float *src, size_t bytes) {
+            if (!dst) return;
+            if (fix) launch_fill(dst, bytes, m->ctx->stream);
+            else (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, m->ctx->stream);
+        };
+        for (int q = 0; q < 6; ++q) {
+            put(px[q], fx[q], sx);
+            put(py[q], fy[q], sy);
+            put(pz[q], fz[q], sz);
+        }
     }
-    (void)a;
 }
 
 // The model's current arguments with the adjoint run's wavefields and CPML
@@ -564,7 +599,42 @@ FDArgs<float> adjoint_args(const seis_model *m) {
     r.psi_vx_x = q.psi_vx_x; r.psi_vy_x = q.psi_vy_x; r.psi_vz_x = q.psi_vz_x;
     r.psi_vx_y = q.psi_vx_y; r.psi_vy_y = q.psi_vy_y; r.psi_vz_y = q.psi_vz_y;
     r.psi_vx_z = q.psi_vx_z; r.psi_vy_z = q.psi_vy_z; r.psi_vz_z = q.psi_vz_z;
+    r.rxx = q.rxx; r.ryy = q.ryy; r.rzz = q.rzz; r.rxy = q.rxy; r.ryz = q.ryz; r.rxz = q.rxz;
     return r;
+}
+
+// The forward run's state (wavefields, memory variables, CPML memory) as
+// (device pointer, bytes) pairs: what a checkpoint holds.
+std::vector<std::pair<float *, size_t>> forward_state(const seis_model *m) {
+    const FDArgs<float> &a = m->a;
+    const size_t nf = (size_t)m->g.nfield * 4, L = m->c.L;
+    std::vector<std::pair<float *, size_t>> st;
+    float *fl[] = {a.vx, a.vy, a.vz, a.sxx, a.syy, a.szz, a.sxy, a.syz, a.sxz};
+    for (float *p : fl)
+        if (p) st.push_back({p, nf});
+    float *mv[] = {a.rxx, a.ryy, a.rzz, a.rxy, a.ryz, a.rxz};
+    for (float *p : mv)
+        if (p) st.push_back({p, nf * L});
+    if (m->c.abs_type == 1) {
+        const size_t nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
+        const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
+        float *px[] = {a.psi_sxx_x, a.psi_sxy_x, a.psi_sxz_x, a.psi_vx_x, a.psi_vy_x, a.psi_vz_x};
+        float *py[] = {a.psi_sxy_y, a.psi_syy_y, a.psi_syz_y, a.psi_vx_y, a.psi_vy_y, a.psi_vz_y};
+        float *pz[] = {a.psi_sxz_z, a.psi_syz_z, a.psi_szz_z, a.psi_vx_z, a.psi_vy_z, a.psi_vz_z};
+        for (int q = 0; q < 6; ++q) {
+            if (px[q]) st.push_back({px[q], sx});
+            if (py[q]) st.push_back({py[q], sy});
+            if (pz[q]) st.push_back({pz[q], sz});
+        }
+    }
+    return st;
+}
+
+// the forward fields a viscoelastic adjoint step reads, in AdjArgs::st order
+std::vector<float *> history_fields(const seis_model *m) {
+    const FDArgs<float> &a = m->a;
+    if (m->c.ND == 3) return {a.vx, a.vy, a.vz, a.sxx, a.syy, a.szz, a.sxy, a.syz, a.sxz};
+    return {a.vx, a.vz, a.sxx, a.szz, a.sxz};
 }
 
 int tmax_of(const seis_model *m) {
@@ -606,7 +676,7 @@ void seis_csts_default(seis_csts *c) {
     c->f0 = 15;
     c->FL[0] = 15;
     c->seisout = 2;
-    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY;
+    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY | FIX_VISCO_F0;
 }
 
 int seis_ctx_create(int device_ordinal, seis_ctx **out) {
@@ -1061,11 +1131,145 @@ int seis_grad_zero(seis_model *m) {
     return 0;
 }
 
+namespace {
+
+// (Re)allocate a gradient work buffer of at least `bytes`.
+int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes) {
+    if (*have >= bytes) return 0;
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    if (*buf) {
+        m->bytes -= *have;
+        m->ctx->release(*buf);
+        *buf = nullptr;
+        *have = 0;
+    }
+    void *p = nullptr;
+    if (m->ctx->alloc(bytes, &p)) return 1;
+    *buf = (float *)p;
+    *have = bytes;
+    m->bytes += bytes;
+    return 0;
+}
+
+// Viscoelastic forward run for the gradient: checkpoints of the full state
+// every K steps, K chosen to balance the checkpoints against the history of
+// one segment (about sqrt(tmax * state / history-per-step) steps).
+int forward_save_visco(seis_model *m, float *out) {
+    auto &G = m->grad;
+    const seis_csts &c = m->c;
+    const int tmax = tmax_of(m);
+    hipStream_t st = m->ctx->stream;
+    // moduli of the gradient formulas: at f0 (FIX_VISCO_F0) or relaxed
+    const bool f0 = c.gradfix & FIX_VISCO_F0;
+    const std::vector<float> &Ms = f0 ? m->rp.Mf0 : m->rp.M, &mus = f0 ? m->rp.muf0 : m->rp.mu;
+    std::vector<float> h;
+    to_padded(m->g, Ms.data(), 1.0f, h);
+    HIP_TRY(hipMemcpyAsync(G.Ms, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    to_padded(m->g, mus.data(), 1.0f, h);
+    HIP_TRY(hipMemcpyAsync(G.mus, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const auto state = forward_state(m);
+    size_t sbytes = 0;
+    for (auto &e : state) sbytes += e.second;
+    const size_t hbytes = (size_t)G.nvb * m->g.nfield * 4;
+    int K = (int)std::lround(std::sqrt((double)tmax * sbytes / hbytes));
+    K = std::max(1, std::min(K, tmax));
+    const int nseg = (tmax + K - 1) / K;
+    const size_t need = (size_t)nseg * sbytes + (size_t)K * hbytes;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    if (need > G.ckpt_bytes + G.hist_bytes + freeb / 10 * 9) {
+        char buf[200];
+        std::snprintf(buf, sizeof(buf),
+                      "viscoelastic gradient: %d checkpoints and %d history steps need %.1f GB, "
+                      "%.1f GB are free", nseg, K, need / 1e9, freeb / 1e9);
+        return fail(buf);
+    }
+    if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, (size_t)nseg * sbytes)) return 1;
+    if (grad_buffer(m, &G.hist, &G.hist_bytes, (size_t)K * hbytes)) return 1;
+    G.K = K;
+    G.nseg = nseg;
+    for (int t = 0; t < tmax; ++t) {
+        if (t % K == 0) {
+            char *dst = (char *)G.ckpt + (size_t)(t / K) * sbytes;
+            for (auto &e : state) {
+                HIP_TRY(hipMemcpyAsync(dst, e.first, e.second, hipMemcpyDeviceToDevice, st));
+                dst += e.second;
+            }
+        }
+        step_sources(m, t, 1, st);
+        step_v(m, st);
+        step_s(m, st);
+        if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+        step_varsout(m, t, st);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    G.saved = true;
+    return out ? seis_shot_read(m, out) : 0;
+}
+
+// Viscoelastic adjoint: segments from the last to the first; each restores
+// its checkpoint, recomputes its steps while keeping every post-source
+// state, then runs its adjoint steps backwards over them.
+int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial) {
+    auto &G = m->grad;
+    const seis_csts &c = m->c;
+    const int tmax = tmax_of(m), tmin = c.tmin, NT = c.NT, nout = (int)m->outn.size();
+    hipStream_t st = m->ctx->stream;
+    const auto state = forward_state(m);
+    size_t sbytes = 0;
+    for (auto &e : state) sbytes += e.second;
+    const auto hf = history_fields(m);
+    const size_t nf = (size_t)m->g.nfield;
+    A.Ms = G.Ms;
+    A.mus = G.mus;
+    for (int seg = G.nseg - 1; seg >= 0; --seg) {
+        const int t0 = seg * G.K, t1 = std::min(t0 + G.K, tmax);
+        if (t1 - 1 <= tmin) break;
+        const char *src = (const char *)G.ckpt + (size_t)seg * sbytes;
+        for (auto &e : state) {
+            HIP_TRY(hipMemcpyAsync(e.first, src, e.second, hipMemcpyDeviceToDevice, st));
+            src += e.second;
+        }
+        for (int t = t0; t < t1; ++t) {
+            step_sources(m, t, 1, st);
+            float *dst = G.hist + (size_t)(t - t0) * G.nvb * nf;
+            for (size_t q = 0; q < hf.size(); ++q)
+                HIP_TRY(hipMemcpyAsync(dst + q * nf, hf[q], nf * 4, hipMemcpyDeviceToDevice, st));
+            step_v(m, st);
+            step_s(m, st);
+            if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+        }
+        for (int t = t1 - 1; t >= std::max(t0, tmin + 1); --t) {
+            if (c.freesurf)
+                launch_surface_adj(A.r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
+            launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t,
+                             serial, st);
+            const float *h = G.hist + (size_t)(t - t0) * G.nvb * nf;
+            for (size_t q = 0; q < hf.size(); ++q) A.st[q] = h + q * nf;
+            launch_adjh_s(A, m->nd_case, c.L, st);
+            launch_adjh_v(A, m->nd_case, st);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace
+
 int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
                       const float *rec_pos, float *out) {
     g_err.clear();
     if (grad_setup(m)) return 1;
     auto &G = m->grad;
+    if (G.visco) {
+        if (seis_shot_bind(m, nsrc, src_pos, src, nrec, rec_pos)) return 1;
+        if (seis_launch_init(m)) return 1;
+        return forward_save_visco(m, out);
+    }
     const int tmax = tmax_of(m);
     const long need = (long)tmax;
     if (G.bnd_steps < need) {
@@ -1147,7 +1351,14 @@ int seis_adjoint(seis_model *m, const float *res) {
     A.gmu = G.gmu;
     A.grho = G.grho;
     A.sh = G.sh;
+    if (m->c.freesurf && (!A.r.M || !A.r.mu || !G.stmp))
+        return fail("seis_adjoint: the adjoint free surface has no moduli on the device");
     const int serial = receivers_share_cells(m, m->rcell_host);
+    if (G.visco) {
+        const int r = adjoint_visco(m, A, serial);
+        G.saved = false;
+        return r;
+    }
     const size_t per = (size_t)G.nvb * G.sh.n;
     // adjoint loop of src/time_stepping.c:869-940
     for (int t = tmax - 1; t > tmin; --t) {
@@ -1183,8 +1394,9 @@ int seis_grad_read(seis_model *m, float *grad) {
                 for (int z = 0; z < g.NZ; ++z)
                     acc[q][((size_t)x * g.NY + y) * g.NZ + z] = tmp[g.idx(z, y, x)];
     }
-    transf_grad(m->c, m->rp, acc[0].data(), acc[1].data(), acc[2].data(), grad,
-                m->c.gradfix & FIX_RHO_SCALE);
+    const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
+    transf_grad(m->c, m->rp, f0 ? m->rp.Mf0 : m->rp.M, f0 ? m->rp.muf0 : m->rp.mu,
+                acc[0].data(), acc[1].data(), acc[2].data(), grad, m->c.gradfix & FIX_RHO_SCALE);
     return 0;
 }
 

@@ -98,14 +98,17 @@ void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::st
 // order: vp|M|Ip, vs|mu|Is, rho.  With rho_fix the direct density term is
 // scaled by dh^2/dt^2, the factor between the density kernel's
 // v . dv_adjoint and the derivative of the misfit.
-void transf_grad(const seis_csts &c, const RefParams &p, const double *gM, const double *gmu,
+// Ms, mus: the moduli of the chain rule (the relaxed ones, or those at f0 for
+// viscoelastic runs with FIX_VISCO_F0).
+void transf_grad(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,
+                 const std::vector<float> &mus, const double *gM, const double *gmu,
                  const double *grho, float *out, int rho_fix) {
-    const size_t n = p.M.size();
-    const bool elastic = !p.mu.empty();
+    const size_t n = Ms.size();
+    const bool elastic = !mus.empty();
     for (size_t i = 0; i < n; ++i) {
         const double rho = 1.0 / p.rho[i] * c.dt / c.dh;
-        const double M = (double)p.M[i] * c.dh / c.dt;
-        const double mu = elastic ? (double)p.mu[i] * c.dh / c.dt : 0.0;
+        const double M = (double)Ms[i] * c.dh / c.dt;
+        const double mu = elastic ? (double)mus[i] * c.dh / c.dt : 0.0;
         double gr = grho[i] / c.dt, gm = gM[i] / c.dt, gu = elastic ? gmu[i] / c.dt : 0.0;
         if (rho_fix) gr *= (double)c.dh * c.dh / ((double)c.dt * c.dt);
         if (c.param_type == 0 || c.param_type == 2) {

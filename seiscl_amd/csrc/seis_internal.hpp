@@ -61,6 +61,9 @@ int build_setup(const seis_csts &c, Setup &s, std::string &err);
 struct RefParams {
     std::vector<float> M, mu, rho, rip, rjp, rkp, muipkp, muipjp, mujpkp;
     std::vector<float> taup, taus, tausipkp, tausipjp, tausjpkp;
+    // viscoelastic cases: the moduli at the reference frequency f0, before
+    // the relaxation division (what the transform gives with L=0)
+    std::vector<float> Mf0, muf0;
 };
 int transform_params(const seis_csts &c, const float *p0, const float *p1,
                      const float *rho, const float *taup, const float *taus,
@@ -181,7 +184,9 @@ void launch_batch_steps(const BatchSlot *dslots, int nslots, const BatchTable &t
 // Gradient by backpropagation (BACK_PROP_TYPE=1).
 // ---------------------------------------------------------------------------
 // Corrections of reference defects, selected by seis_csts.gradfix.
-enum GradFix { FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8 };
+enum GradFix {
+    FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8, FIX_VISCO_F0 = 16
+};
 
 // Boundary shell saved during the forward run and re-injected while the
 // forward wavefield runs backwards: FDOH cells thick, on the inner face of
@@ -295,12 +300,22 @@ struct AdjArgs {
     double *gM, *gmu, *grho;     // padded layout, one double per cell
     const R *bnd;                // [nvars][n] shell values of step t-1
     Shell sh;
+    // viscoelastic runs: the forward state of the step from the recomputed
+    // history (2D: vx vz sxx szz sxz; 3D: vx vy vz sxx syy szz sxy syz sxz)
+    // and the moduli of the gradient formulas (padded layout)
+    const R *st[9];
+    const R *Ms, *mus;
 };
 
 template <typename R>
 void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream);
 template <typename R>
 void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream);
+// viscoelastic adjoint steps reading the forward state from A.st (L = 1..4)
+template <typename R>
+void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, void *stream);
+template <typename R>
+void launch_adjh_v(const AdjArgs<R> &a, int nd_case, void *stream);
 // adjoint free surface on the adjoint wavefields r; tmp: 3 * NX * NY reals
 template <typename R>
 void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, void *stream);
@@ -317,7 +332,8 @@ double residuals_l2(const seis_csts &c, int nout, int nrec, const float *out, co
                     float *res, double *rmsnorm);
 void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
                int nrec, const float *rec_pos, float *res, int at_receiver);
-void transf_grad(const seis_csts &c, const RefParams &p, const double *gM, const double *gmu,
+void transf_grad(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,
+                 const std::vector<float> &mus, const double *gM, const double *gmu,
                  const double *grho, float *out, int rho_fix);
 
 }  // namespace seis

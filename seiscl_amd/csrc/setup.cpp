@@ -302,6 +302,7 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
             } break;
         }
     }
+    if (L > 0) o.Mf0 = o.M;
     if (L > 0)
 #pragma omp parallel for schedule(static) if (n >= kParCells)
         for (long i = 0; i < n; ++i) {
@@ -332,6 +333,7 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
             }
         }
         if (L > 0) {
+            o.muf0 = o.mu;
             float s = 0.0;
             for (int l = 0; l < L; ++l)
                 s += ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));

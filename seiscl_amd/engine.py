@@ -22,7 +22,7 @@ CSTS_DEFAULTS = dict(
     ND=2, dh=10.0, dt=0.0008, NT=875, L=0, f0=15.0, FL=(15.0,), FDORDER=8,
     MAXRELERROR=1, FP16=0, freesurf=0, abs_type=1, VPPML=3500.0, NPOWER=2.0,
     FPML=15.0, K_MAX_CPML=2.0, nab=16, abpc=6.0, seisout=2, param_type=0,
-    tmax=0.0, tmin=0.0, gradout=0, gradfix=15,
+    tmax=0.0, tmin=0.0, gradout=0, gradfix=31,
 )
 
 PARAM_NAMES = {0: ("vp", "vs", "rho"), 1: ("M", "mu", "rho"), 2: ("Ip", "Is", "rho")}

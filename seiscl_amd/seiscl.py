@@ -65,7 +65,7 @@ class SeisCL:
                  offmin=-float("Inf"), offmax=float("Inf"), inputres=0, restype=0,
                  scalerms=0, scalermsnorm=0, scaleshot=0, seisout=2, resout=0, rmsout=0,
                  movout=0, file="SeisCL", workdir="./seiscl", device: int = 0,
-                 gradfix: int = 15, files: bool = False, spawn: bool = True,
+                 gradfix: int = 31, files: bool = False, spawn: bool = True,
                  progname: str | None = None):
         self.N, self.ND, self.dh, self.dt, self.NT = N, ND, dh, dt, NT
         self.L, self.f0, self.FL = L, f0, FL

@@ -1,4 +1,6 @@
-"""GPU gradient by backpropagation against the CPU oracle on the same inputs.
+"""GPU gradient against the CPU oracle on the same inputs: elastic runs by
+backpropagation with boundary saving, viscoelastic runs from checkpoints
+and recomputed segments (the oracle keeps the whole forward history).
 
 The adjoint kernels follow the oracle's expressions with FMA contraction
 disabled, and both accumulate the per-step gradient increments in double in
@@ -28,6 +30,15 @@ CASES = (
     # tall 3D grid: several z tiles of 64 cells
     + [dict(ND=3, FDORDER=4, abs_type=a, N=(150, 28, 30)) for a in (1, 2)]
 )
+# viscoelastic: forward state from checkpoints and recomputed segments
+VISCO = (
+    [dict(ND=2, FDORDER=4, L=l, abs_type=a) for l in (1, 2, 3) for a in (1, 2)]
+    + [dict(ND=2, FDORDER=8, L=2), dict(ND=2, FDORDER=4, L=2, freesurf=1)]
+    + [dict(ND=2, FDORDER=4, L=2, gradfix=f) for f in (0, 15)]
+    + [dict(ND=3, FDORDER=4, L=2, abs_type=a) for a in (1, 2)]
+    + [dict(ND=3, FDORDER=4, L=1, freesurf=1), dict(ND=3, FDORDER=8, L=2, gradfix=15)]
+)
+CASES = CASES + VISCO
 
 
 def _ids(c):
@@ -36,7 +47,7 @@ def _ids(c):
 
 def _setup(case):
     case = dict(case)
-    gradfix = case.pop("gradfix", 15)
+    gradfix = case.pop("gradfix", 31)
     nt = 160 if case.get("ND") == 3 else 220
     csts, params, shot = small_case(NT=nt, seed=11, **case)
     csts["gradfix"] = gradfix
@@ -71,7 +82,11 @@ def test_gradient_matches_oracle(oracle_lib, gpu_ctx, case):
         assert rel_l2(g[k], ref["grad"][k]) <= TOL, k
 
 
-@pytest.mark.parametrize("case", [CASES[2], CASES[-5], CASES[-1]], ids=_ids)
+@pytest.mark.parametrize("case", [dict(ND=2, FDORDER=4, abs_type=1),
+                                  dict(ND=2, FDORDER=4, freesurf=1, abs_type=2),
+                                  dict(ND=3, FDORDER=4, abs_type=2, N=(150, 28, 30)),
+                                  dict(ND=2, FDORDER=4, L=2, abs_type=1),
+                                  dict(ND=3, FDORDER=4, L=1, freesurf=1)], ids=_ids)
 def test_gradient_bitwise(oracle_lib, gpu_ctx, case):
     ref, out, rms, g = _run_both(oracle_lib, gpu_ctx, case)
     for r, o in zip(ref["out"], out):
