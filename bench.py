@@ -121,14 +121,162 @@ def serving_main(args):
     print(json.dumps(out), flush=True)
 
 
+BYTES_ADJ_S_2D = 108   # adjoint stress kernel, 2D elastic FP32, per cell
+BYTES_ADJ_V_2D = 80    # adjoint velocity kernel, 2D elastic FP32, per cell
+
+
+def gradient_main(args):
+    """2D elastic FWI gradient (300 x 800, FD8, NT 3000, 16 shots) through the
+    torch.autograd.Function: forward with boundary saving for every shot,
+    backward = adjoint runs (and recomputed forward runs for all shots but
+    the last).  A step is one whole gradient (forward + backward)."""
+    import torch
+    from seiscl_amd import Context, Model
+    from seiscl_amd.autograd import Propagator, seismic
+    from seiscl_amd.workloads import config3
+    csts, true, start, shots = config3(NT=args.nt or 3000)
+    NT, ncell = int(csts["NT"]), int(np.prod(csts["N"]))
+    ctx = Context(0)
+    t0 = time.time()
+    mt = Model(ctx, true, **csts)
+    din = [mt.forward(*sh) for sh in shots]
+    mt.close()
+    log(f"observed data of {len(shots)} shots in {time.time() - t0:.1f} s")
+    obs = [torch.from_numpy(x) for d in din for x in d]
+    prop = Propagator(ctx, start, **csts)
+    params = [torch.tensor(start[n], requires_grad=True) for n in prop.param_names]
+
+    def one():
+        for p in params:
+            p.grad = None
+        ctx.sync()
+        ta = time.perf_counter()
+        outs = seismic(prop, shots, *params)
+        tb = time.perf_counter()
+        loss = sum(0.5 * ((o.double() - d.double()) ** 2).sum() for o, d in zip(outs, obs))
+        tc = time.perf_counter()
+        loss.backward()
+        ctx.sync()
+        td = time.perf_counter()
+        return tb - ta, td - tc, (tc - tb), float(loss.detach())
+
+    for _ in range(args.warmup):
+        one()
+    prop.model.set_profiling(True)
+    runs = [one() for _ in range(args.steps)]
+    tm = prop.model.timing()              # the last adjoint loop (shot 0)
+    grad = {n: p.grad.numpy().astype(np.float64) for n, p in zip(prop.param_names, params)}
+    prop.close()
+    fwd = sum(r[0] for r in runs) / len(runs)
+    bwd = sum(r[1] for r in runs) / len(runs)
+    misfit = sum(r[2] for r in runs) / len(runs)
+    total = fwd + bwd + misfit
+    cells = len(shots) * NT * ncell
+    nadj = NT - 2
+    adj_s, adj_v = tm["update_s_ms"] / nadj, tm["update_v_ms"] / nadj
+    ach = BYTES_ADJ_S_2D * ncell / (adj_s / 1e3) / 1e9
+    log(f"gradient: forward {fwd:.2f} s, backward {bwd:.2f} s, misfit {misfit:.2f} s; "
+        f"adjoint kernels {adj_s * 1e3:.1f} / {adj_v * 1e3:.1f} us per step")
+    cfg = {
+        "workload": ("2D elastic FWI gradient through torch.autograd, 300x800, FD8, NT "
+                     f"{NT}, 16 shots, CPML 16, boundary-saving recomputation"),
+        "grid": [int(v) for v in csts["N"]], "fd_order": 8, "shots": len(shots), "NT": NT,
+        "forward_s": round(fwd, 4), "backward_s": round(bwd, 4), "misfit_s": round(misfit, 4),
+        "forward_gcell_s": round(cells / fwd / 1e9, 3),
+        "backward_gcell_s": round(cells / bwd / 1e9, 3),
+        "adj_s_us": round(adj_s * 1e3, 3), "adj_v_us": round(adj_v * 1e3, 3),
+        "adj_v_gbs": round(BYTES_ADJ_V_2D * ncell / (adj_v / 1e3) / 1e9, 1),
+        "misfit": runs[-1][3],
+    }
+    if args.fd_cells > 0:
+        cfg["finite_differences"] = fd_check(ctx, csts, start, shots, din, grad, args.fd_cells)
+    out = {
+        "metric": METRIC, "value": round(cells / total / 1e9, 4), "unit": "Gcell/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": cfg,
+        "roofline": {"bound": "hbm", "kernel": "adjoint stress update (2D elastic, FD8)",
+                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        sp, s, rp = shots[0]
+        nt = min(NT, args.cpu_steps)
+        c = dict(csts, NT=nt)
+        t0 = time.time()
+        oracle.gradient(start, sp, s[:nt], rp, din=[d[:nt] for d in din[0]], nthreads=threads,
+                        **c)
+        wall = time.time() - t0
+        out["cpu_baseline"] = dict(
+            value=ncell * nt / wall / 1e9, unit="Gcell/s", cores=threads, kind="port",
+            sample=(f"one shot's gradient (forward with boundary saving + adjoint), {nt} steps "
+                    f"of the same grid in {wall:.1f} s, C oracle with OpenMP"))
+    print(json.dumps(out), flush=True)
+
+
+def fd_check(ctx, csts, start, shots, din, grad, ncells):
+    """Central differences (h = 1e-3 of the value) of J = 1/2 sum (out - d)^2
+    from FP64 runs, on ncells random interior cells per parameter."""
+    from seiscl_amd import Model
+    NT = int(csts["NT"])
+    m64 = Model(ctx, start, **dict(csts, FP16=64))
+
+    def J():
+        s = 0.0
+        for (sp, sr, rp), d in zip(shots, din):
+            m64.bind_shot(sp, sr, rp)
+            m64.run_steps(0, NT)
+            s += 0.5 * sum(float(np.sum((o - x.astype(np.float64)) ** 2))
+                           for o, x in zip(m64.read_shot(f64=True), d))
+        return s
+
+    rng = np.random.default_rng(300)
+    nab = int(csts["nab"])
+    res = {}
+    for name in ("vp", "vs", "rho"):
+        fd, gr = [], []
+        # random interior cells the wavefield reaches (|g| >= 1% of its max)
+        inner = np.zeros(grad[name].shape, bool)
+        inner[nab + 10:-nab - 10, nab + 10:-nab - 10] = True
+        a = np.abs(grad[name]) * inner
+        cand = np.argwhere(a >= 1e-2 * a.max())
+        for k in rng.choice(len(cand), size=ncells, replace=False):
+            iz, ix = (int(v) for v in cand[k])
+            v0 = float(start[name][iz, ix])
+            vals = []
+            for sgn in (1, -1):
+                p = {k: v.copy() for k, v in start.items()}
+                p[name][iz, ix] = np.float32(v0 * (1 + sgn * 1e-3))
+                m64.set_params(p)
+                vals.append((J(), float(p[name][iz, ix])))
+            fd.append((vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1]))
+            gr.append(float(grad[name][iz, ix]))
+        fd, gr = np.array(fd), np.array(gr)
+        res[name] = {"cells": ncells,
+                     "rel_l2": float(np.linalg.norm(gr - fd) / np.linalg.norm(fd)),
+                     "median_ratio": float(np.median(gr / fd))}
+        log(f"finite differences {name}: {res[name]}")
+    m64.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["forward3d", "serving"], default="forward3d")
+    ap.add_argument("--workload", choices=["forward3d", "serving", "gradient2d"],
+                    default="forward3d")
+    ap.add_argument("--nt", type=int, default=0, help="gradient2d: time steps (default 3000)")
+    ap.add_argument("--fd-cells", type=int, default=0,
+                    help="gradient2d: finite-difference check on this many cells per parameter")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
     ap.add_argument("--slots", type=int, default=48, help="serving: batch slots")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="forward3d: time steps (1500); gradient2d: whole gradients (2)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="forward3d: time steps (10); gradient2d: whole gradients (1)")
     ap.add_argument("--n", type=int, default=256, help="grid size per axis")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
@@ -145,6 +293,13 @@ def main():
 
     if args.workload == "serving":
         return serving_main(args)
+    grad = args.workload == "gradient2d"
+    if args.steps is None:
+        args.steps = 2 if grad else 1500
+    if args.warmup is None:
+        args.warmup = 1 if grad else 10
+    if grad:
+        return gradient_main(args)
 
     from seiscl_amd import Context, Model
     from seiscl_amd.workloads import config2

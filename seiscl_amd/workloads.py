@@ -130,6 +130,42 @@ def layered_model(shape, rng, nlayers: int = 6, vmin: float = 1800.0, vmax: floa
     return p
 
 
+def config3(N=(300, 800), NT: int = 3000, nshots: int = 16, seed: int = 3, f0: float = 8.0):
+    """2D elastic FWI gradient: 300 x 800, dh 10 m, dt 1 ms, NT 3000, FD8,
+    CPML 16.  True model: 6 seeded layers vp 1800-4000, vs = vp/1.8,
+    rho = 1800 + 800 (vp - 1800) / 2700, plus 3 Gaussian anomalies (+-10%,
+    radius 150 m); starting model: the true model smoothed with a 200 m
+    Gaussian.  16 explosive shots evenly spaced along x and receivers every
+    2 cells, both at z = nab + 5 cells; Ricker f0 (8 Hz: 5 cells per shortest
+    S wavelength at 2.5 f0).  Returns (csts, true params, start params, shots)."""
+    rng = np.random.default_rng(seed)
+    NZ, NX = N
+    dh, dt, nab = 10.0, 1e-3, 16
+    vp = layered(N, 6, 1800.0, 4000.0, rng)
+    zz, xx = np.meshgrid(np.arange(NZ) * dh, np.arange(NX) * dh, indexing="ij")
+    for _ in range(3):
+        cz = rng.uniform(0.3, 0.8) * NZ * dh
+        cx = rng.uniform(0.2, 0.8) * NX * dh
+        amp = rng.choice([-0.1, 0.1])
+        vp = vp * (1 + amp * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / 150.0 ** 2))
+    true = dict(vp=vp.astype(np.float32), vs=(vp / 1.8).astype(np.float32),
+                rho=(1800 + 800 * (vp - 1800) / 2700).astype(np.float32))
+    start = {k: gaussian_filter(v.astype(np.float64), sigma=200.0 / dh,
+                                mode="nearest").astype(np.float32) for k, v in true.items()}
+    csts = dict(N=np.array(N), ND=2, dh=dh, dt=dt, NT=NT, FDORDER=8, MAXRELERROR=1,
+                abs_type=1, nab=nab, freesurf=0, L=0, f0=f0, seisout=1)
+    zs = (nab + 5) * dh
+    src = ricker(f0, NT, dt)[:, None].astype(np.float32)
+    gx = np.arange(nab, NX - nab, 2) * dh
+    shots = []
+    for k, x in enumerate(np.linspace(nab + 5, NX - nab - 5, nshots).round() * dh):
+        sp = np.array([[x], [0.0], [zs], [float(k)], [100.0]])
+        rp = np.stack([gx, 0 * gx, 0 * gx + zs, 0 * gx + k, k * gx.size + np.arange(gx.size) + 1.0,
+                       0 * gx, 0 * gx, 0 * gx])
+        shots.append((sp, src, rp))
+    return csts, true, start, shots
+
+
 def config4_request(rng, nab: int = 16, NZ=None, NX=None, NT=None):
     """One serving request: 2D acoustic (ND 22) or elastic (ND 2) modeling,
     NZ ~ U{100..600} even, NX ~ U{200..1500}, NT ~ U{1000..5000}, FD 4 or

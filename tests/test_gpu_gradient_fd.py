@@ -1,0 +1,127 @@
+"""2D elastic FWI gradient of the 300 x 800 FD8 configuration through the
+torch.autograd.Function, checked against finite differences on random cells
+and against the oracle's gradient.
+
+Case: ``workloads.config3`` (seeded layered true model with three Gaussian
+anomalies; starting model = true model smoothed with a 200 m Gaussian),
+300 x 800 cells, FD8, CPML 16, dh 10 m, dt 1 ms, explosive sources and
+receivers at z = nab + 5 cells.  Here two of the 16 shots keep the test
+short; bench.py --workload gradient2d --fd-cells 20 runs the full 16-shot
+gradient and the same finite-difference check.  The cells are drawn at
+random (seed 300) among the interior cells the wavefield reaches, those
+whose gradient is at least 1% of the largest: elsewhere both the gradient
+and the difference quotient vanish.
+
+Finite differences: central, h = 1e-3 of the cell's value, of the misfit
+J = 1/2 sum (out - d)^2 from FP64 runs of the same kernels (receivers
+recording in double), so the difference quotient carries no FP32 rounding.
+The gradient (FP32 fields, double accumulation) is the reference's
+correlation formula, not the exact derivative of the discrete scheme:
+vp enters one cell's modulus, so it matches cell by cell; vs and rho also
+enter staggered averages over neighbouring cells (harmonic mu, arithmetic
+buoyancy), which the formula attributes to the cell itself.  Tolerances,
+as relative L2 error over the checked cells: vp 3%, vs 10%, rho 30%
+(a CPU run of the oracle on a 100 x 200 version gave about 0.5%, 5% and
+20%).
+"""
+import numpy as np
+import pytest
+
+from cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NT = 3000
+TOL = {"vp": 0.03, "vs": 0.10, "rho": 0.30}
+NCELLS = {"vp": 20, "vs": 10, "rho": 10}
+
+
+def _case(nt=NT):
+    from seiscl_amd.workloads import config3
+    csts, true, start, shots = config3(NT=nt, nshots=16)
+    return csts, true, start, [shots[4], shots[11]]
+
+
+def _observed(ctx, true, csts, shots):
+    from seiscl_amd import Model
+    m = Model(ctx, true, **csts)
+    din = [m.forward(*sh) for sh in shots]
+    m.close()
+    return din
+
+
+def _misfit64(m64, shots, din):
+    J = 0.0
+    for (sp, s, rp), d in zip(shots, din):
+        m64.bind_shot(sp, s, rp)
+        m64.run_steps(0, NT)
+        out = m64.read_shot(f64=True)
+        J += 0.5 * sum(float(np.sum((o - x.astype(np.float64)) ** 2)) for o, x in zip(out, d))
+    return J
+
+
+def _cells(g, margin, n, rng):
+    """n random interior cells (margin cells from every edge) where |g| is at
+    least 1% of its largest value."""
+    inner = np.zeros(g.shape, bool)
+    inner[margin:-margin, margin:-margin] = True
+    a = np.abs(g) * inner
+    cand = np.argwhere(a >= 1e-2 * a.max())
+    pick = rng.choice(len(cand), size=n, replace=False)
+    return [tuple(int(v) for v in cand[k]) for k in pick]
+
+
+def test_config3_gradient_matches_oracle(oracle_lib, gpu_ctx):
+    from seiscl_amd import Model
+    csts, true, start, shots = _case(500)     # 500 steps: the oracle takes ~20 s
+    sh = shots[0]
+    din = _observed(gpu_ctx, true, csts, [sh])[0]
+    ref = oracle_lib.gradient(start, *sh, din=din, nthreads=16, **csts)
+    m = Model(gpu_ctx, start, **csts)
+    m.grad_zero()
+    m.gradient(*sh, din=din)
+    g = m.grad_read()
+    m.close()
+    for k in ("vp", "vs", "rho"):
+        assert np.abs(ref["grad"][k]).max() > 0
+        assert rel_l2(g[k], ref["grad"][k]) <= 1e-5, k
+
+
+def test_config3_gradient_finite_differences(gpu_ctx):
+    import torch
+    from seiscl_amd import Model
+    from seiscl_amd.autograd import Propagator, seismic
+    csts, true, start, shots = _case()
+    din = _observed(gpu_ctx, true, csts, shots)
+    prop = Propagator(gpu_ctx, start, **csts)
+    params = [torch.tensor(start[n], requires_grad=True) for n in prop.param_names]
+    outs = seismic(prop, shots, *params)
+    dins = [torch.from_numpy(x) for d in din for x in d]
+    loss = sum(0.5 * ((o.double() - d.double()) ** 2).sum() for o, d in zip(outs, dins))
+    loss.backward()
+    grad = {n: p.grad.numpy().astype(np.float64) for n, p in zip(prop.param_names, params)}
+    prop.close()
+
+    m64 = Model(gpu_ctx, start, **dict(csts, FP16=64))
+    J0 = _misfit64(m64, shots, din)
+    assert abs(J0 - float(loss.detach())) <= 1e-4 * J0
+    rng = np.random.default_rng(300)
+    nab = int(csts["nab"])
+    for name in ("vp", "vs", "rho"):
+        fd, gr = [], []
+        for iz, ix in _cells(grad[name], nab + 10, NCELLS[name], rng):
+            v0 = float(start[name][iz, ix])
+            J = []
+            for sgn in (1, -1):
+                p = {k: v.copy() for k, v in start.items()}
+                p[name][iz, ix] = np.float32(v0 * (1 + sgn * 1e-3))
+                m64.set_params(p)
+                J.append((_misfit64(m64, shots, din), float(p[name][iz, ix])))
+            fd.append((J[0][0] - J[1][0]) / (J[0][1] - J[1][1]))
+            gr.append(grad[name][iz, ix])
+            print(f"{name} ({iz},{ix}): fd {fd[-1]:.5e} grad {gr[-1]:.5e}", flush=True)
+        fd, gr = np.array(fd), np.array(gr)
+        err = float(np.linalg.norm(gr - fd) / np.linalg.norm(fd))
+        print(f"{name}: relative L2 error over {len(fd)} cells {err:.4f}", flush=True)
+        assert err <= TOL[name], (name, err)
+    m64.close()
