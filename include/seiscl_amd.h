@@ -133,6 +133,10 @@ int seis_launch_varsout(seis_model *m, int nt);
 /* D2H of the bound shot's seismograms, [nout][nrec][NT] (reduce_seis,       */
 /* src/time_stepping.c:25-90).                                                */
 int seis_shot_read(seis_model *m, float *out);
+/* The same in double, for models running in FP64 (FP16=64), whose receivers */
+/* record in double.  No counterpart in the reference, which has no FP64     */
+/* mode; finite-difference checks of gradients need the digits.             */
+int seis_shot_read_f64(seis_model *m, double *out);
 
 /* Run forward steps t0 .. t0+nsteps-1 of the bound shot (sources, update_v, */
 /* update_s, free surface, varsout), waiting for completion; the device time */
@@ -150,21 +154,32 @@ int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos,
 /* 272-350, and by parity tests).                                              */
 int seis_read_field(seis_model *m, const char *name, float *host);
 
-/* ---- gradient by backpropagation (BACK_PROP_TYPE=1), ND 2/3, FP32, with */
-/* or without a free surface (adjoint surface: src/surface2D_adj.cl,         */
-/* src/surface3D_adj.cl).  Elastic (L=0): the forward run saves an           */
+/* ---- gradient by backpropagation (BACK_PROP_TYPE=1), ND 2/3, with or    */
+/* without a free surface (adjoint surface: src/surface2D_adj.cl,            */
+/* src/surface3D_adj.cl).  Elastic (L=0, FP32): the forward run saves an     */
 /* FDOH-thick boundary shell of every wavefield at every step in device      */
 /* memory (save_bnd, src/time_stepping.c:352-395, kept resident instead of   */
 /* the reference's pinned host copy); the adjoint run re-injects it while    */
 /* running the forward wavefield backwards (inject_bnd, :397-424).           */
-/* Viscoelastic (L=1..4): running the memory variables backwards is          */
-/* unstable, so the forward run keeps full-state checkpoints every K steps   */
-/* in device memory and the adjoint recomputes each segment forward, keeping */
-/* the post-source states it reads.  Both accumulate the gradient in double  */
-/* (update_adjs / update_adjv, src/update_adj*.cl).                          */
+/* Viscoelastic (L=1..4; FP32, and in 3D also FP16 storage or scaled units): */
+/* running the memory variables backwards is unstable, so the forward run    */
+/* keeps full-state checkpoints in device memory and the adjoint recomputes  */
+/* the forward run segment by segment (one or two checkpoint levels, chosen  */
+/* to fit a memory budget), keeping the post-source states it reads.  Both   */
+/* accumulate the gradient in double (update_adjs / update_adjv,             */
+/* src/update_adj*.cl).                                                      */
 /* Zero the gradient accumulators (kernel_gradinit, src/automatic_kernels.c: */
 /* 840-930; src/time_stepping.c:725-729).                                     */
 int seis_grad_zero(seis_model *m);
+/* Device bytes the viscoelastic gradient may hold in checkpoints and the   */
+/* recomputed history (0, the default: 90% of the free device memory at    */
+/* seis_forward_save).  No counterpart in the reference, whose viscoelastic */
+/* adjoint backpropagates; seis_grad_plan reports the plan chosen: level-1 */
+/* spacing K1, level-2 spacing K2, level-1 checkpoints n1, sub-segments per */
+/* level-1 segment n2 (1: one level) and the bytes it holds.               */
+int seis_grad_set_memory(seis_model *m, size_t bytes);
+int seis_grad_plan(const seis_model *m, int *K1, int *K2, int *n1, int *n2,
+                   size_t *bytes);
 /* Bind a shot and run it forward with boundary saving; out as             */
 /* seis_forward_shot (may be NULL).                                          */
 int seis_forward_save(seis_model *m, int nsrc, const float *src_pos,
@@ -217,8 +232,11 @@ int seis_batch_slot_left(const seis_batch *b, int slot);
 /* forward runs (costs a few microseconds per step).                          */
 int seis_model_set_profiling(seis_model *m, int on);
 
-/* Last timed forward run: device milliseconds spent in the update kernels   */
-/* and in the whole step loop (HIP events on the compute stream).             */
+/* Last timed loop: device milliseconds spent in the update kernels and in  */
+/* the whole step loop (HIP events on the compute stream).  After            */
+/* seis_run_steps: the forward update_v / update_s; after seis_adjoint of an */
+/* elastic model: its fused adjoint velocity / stress kernels.  The kernel   */
+/* times need seis_model_set_profiling(m, 1) before the loop.                */
 int seis_last_timing(const seis_model *m, double *ms_update_v,
                      double *ms_update_s, double *ms_loop);
 

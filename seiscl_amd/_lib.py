@@ -79,6 +79,9 @@ SIGNATURES = [
     ("seis_launch_surface", _I, [_P]),
     ("seis_launch_varsout", _I, [_P, _I]),
     ("seis_shot_read", _I, [_P, _FP]),
+    ("seis_shot_read_f64", _I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    ("seis_grad_set_memory", _I, [_P, ctypes.c_size_t]),
+    ("seis_grad_plan", _I, [_P] + [ctypes.POINTER(_I)] * 4 + [ctypes.POINTER(ctypes.c_size_t)]),
     ("seis_run_steps", _I, [_P, _I, _I]),
     ("seis_forward_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
     ("seis_read_field", _I, [_P, ctypes.c_char_p, _FP]),

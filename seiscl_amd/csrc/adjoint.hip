@@ -18,6 +18,8 @@
 // (residuals, src/automatic_kernels.c:631-836).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fd_common.hpp"
 #include "seis_internal.hpp"
 
@@ -386,7 +388,14 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
 // src/update_adjs3D.cl:549-605 index and write them otherwise); the gradient
 // reads the partial increments, as the reference's does.
 // ---------------------------------------------------------------------------
-template <typename R, int H, int L>
+// the forward state q of the step at cell i, stored as S
+template <typename R, typename S>
+__device__ __forceinline__ R hst(const AdjArgs<R> &A, int q, long i) {
+    if constexpr (std::is_same<S, __half>::value) return (R)__half2float(((const S *)A.st[q])[i]);
+    else return (R)((const S *)A.st[q])[i];
+}
+
+template <typename R, typename S, int H, int L>
 __global__ __launch_bounds__(256) void k_adjh_s2d(AdjArgs<R> A, int nbz, unsigned nb) {
     const FDArgs<R> &a = A.r;
     const Layout &g = a.g;
@@ -445,17 +454,18 @@ __global__ __launch_bounds__(256) void k_adjh_s2d(AdjArgs<R> A, int nbz, unsigne
     a.szz[i] = s3[1];
     a.sxz[i] = s3[2];
     // M and mu gradients (src/update_adjs2D.cl:441-451) from the forward state
-    const R sxx = A.st[2][i], szz = A.st[3][i], sxz = A.st[4][i];
+    const R sxx = hst<R, S>(A, 2, i), szz = hst<R, S>(A, 3, i), sxz = hst<R, S>(A, 4, i);
     const R lM = A.Ms[i], lmu = A.mus[i];
     const R c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
     const R c3 = 1.0 / (lmu * lmu);
     const R c5 = 0.25 * c3;
     const R dM = c1 * (sxx + szz) * (lsxx + lszz);
-    A.gM[i] += -dM;
-    A.gmu[i] += -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+    const R dmu = -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+    A.gM[i] += (double)-dM * A.gscale_s;
+    A.gmu[i] += (double)dmu * A.gscale_s;
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 __global__ __launch_bounds__(256) void k_adjh_v2d(AdjArgs<R> A, int nbz, unsigned nb) {
     const FDArgs<R> &a = A.r;
     const Layout &g = a.g;
@@ -489,11 +499,11 @@ __global__ __launch_bounds__(256) void k_adjh_v2d(AdjArgs<R> A, int nbz, unsigne
     a.vx[i] = v2[0];
     a.vz[i] = v2[1];
     // density gradient (src/update_adjv2D.cl:326) from the forward state
-    const R d = -A.st[0][i] * lvx - A.st[1][i] * lvz;
-    A.grho[i] += d;
+    const R d = -hst<R, S>(A, 0, i) * lvx - hst<R, S>(A, 1, i) * lvz;
+    A.grho[i] += (double)d * A.gscale_v;
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigned nb) {
     const FDArgs<R> &a = A.r;
     const Layout &g = a.g;
@@ -582,20 +592,23 @@ __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigne
     a.syy[i] = s6[4];
     a.szz[i] = s6[5];
     // M and mu gradients (src/update_adjs3D.cl:679-689) from the forward state
-    const R sxx = A.st[3][i], syy = A.st[4][i], szz = A.st[5][i];
-    const R sxy = A.st[6][i], syz = A.st[7][i], sxz = A.st[8][i];
+    const R sxx = hst<R, S>(A, 3, i), syy = hst<R, S>(A, 4, i), szz = hst<R, S>(A, 5, i);
+    const R sxy = hst<R, S>(A, 6, i), syz = hst<R, S>(A, 7, i), sxz = hst<R, S>(A, 8, i);
     const R lM = A.Ms[i], lmu = A.mus[i];
     const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
     const R c3 = 1.0 / lmu / lmu;
     const R c5 = 1.0 / 6.0 * c3;
     const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
-    A.gM[i] += -dM;
-    A.gmu[i] += -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
-                c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
-                      lszz * (2.0 * szz - sxx - syy));
+    // evaluated in double from the 4.0 / 3 term on, as the reference's
+    // unsuffixed literals make it
+    const double dmu = -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
+                       c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
+                             lszz * (2.0 * szz - sxx - syy));
+    A.gM[i] += (double)-dM * A.gscale_s;
+    A.gmu[i] += dmu * A.gscale_s;
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 __global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigned nb) {
     const FDArgs<R> &a = A.r;
     const Layout &g = a.g;
@@ -646,8 +659,8 @@ __global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigne
     a.vy[i] = v3[1];
     a.vz[i] = v3[2];
     // density gradient (src/update_adjv3D.cl:481) from the forward state
-    const R d = -A.st[0][i] * lvx - A.st[1][i] * lvy - A.st[2][i] * lvz;
-    A.grho[i] += d;
+    const R d = -hst<R, S>(A, 0, i) * lvx - hst<R, S>(A, 1, i) * lvy - hst<R, S>(A, 2, i) * lvz;
+    A.grho[i] += (double)d * A.gscale_v;
 }
 
 // ---------------------------------------------------------------------------
@@ -846,31 +859,31 @@ void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, vo
     SEIS_SWITCH_H(r.g.H, (run_surf_adj<R, HH>(r, tmp, nd, syy_fix, (hipStream_t)stream)));
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 void run_adjh_s(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
     int nbz;
     unsigned nb;
     grid_adj<R>(a.f.g, nbz, nb);
     const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, ABY);
-    if (nd_case == 3) hipLaunchKernelGGL((k_adjh_s3d<R, H, L>), grid, blk, 0, st, a, nbz, nb);
-    else hipLaunchKernelGGL((k_adjh_s2d<R, H, L>), grid, blk, 0, st, a, nbz, nb);
+    if (nd_case == 3) hipLaunchKernelGGL((k_adjh_s3d<R, S, H, L>), grid, blk, 0, st, a, nbz, nb);
+    else hipLaunchKernelGGL((k_adjh_s2d<R, S, H, L>), grid, blk, 0, st, a, nbz, nb);
 }
-template <typename R, int H>
+template <typename R, typename S, int H>
 void run_adjh_v(const AdjArgs<R> &a, int nd_case, hipStream_t st) {
     int nbz;
     unsigned nb;
     grid_adj<R>(a.f.g, nbz, nb);
     const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, ABY);
-    if (nd_case == 3) hipLaunchKernelGGL((k_adjh_v3d<R, H>), grid, blk, 0, st, a, nbz, nb);
-    else hipLaunchKernelGGL((k_adjh_v2d<R, H>), grid, blk, 0, st, a, nbz, nb);
+    if (nd_case == 3) hipLaunchKernelGGL((k_adjh_v3d<R, S, H>), grid, blk, 0, st, a, nbz, nb);
+    else hipLaunchKernelGGL((k_adjh_v2d<R, S, H>), grid, blk, 0, st, a, nbz, nb);
 }
-template <typename R, int H>
+template <typename R, typename S, int H>
 void run_adjh_s_L(const AdjArgs<R> &a, int nd_case, int L, hipStream_t st) {
     switch (L) {
-        case 1: run_adjh_s<R, H, 1>(a, nd_case, st); break;
-        case 2: run_adjh_s<R, H, 2>(a, nd_case, st); break;
-        case 3: run_adjh_s<R, H, 3>(a, nd_case, st); break;
-        case 4: run_adjh_s<R, H, 4>(a, nd_case, st); break;
+        case 1: run_adjh_s<R, S, H, 1>(a, nd_case, st); break;
+        case 2: run_adjh_s<R, S, H, 2>(a, nd_case, st); break;
+        case 3: run_adjh_s<R, S, H, 3>(a, nd_case, st); break;
+        case 4: run_adjh_s<R, S, H, 4>(a, nd_case, st); break;
         default: break;
     }
 }
@@ -884,12 +897,20 @@ void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream) {
     SEIS_SWITCH_H(a.f.g.H, (run_adj_v<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
-void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, void *stream) {
-    SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, HH>(a, nd_case, L, (hipStream_t)stream)));
+void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, bool half, void *stream) {
+    if (half) {
+        SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, __half, HH>(a, nd_case, L, (hipStream_t)stream)));
+    } else {
+        SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, R, HH>(a, nd_case, L, (hipStream_t)stream)));
+    }
 }
 template <typename R>
-void launch_adjh_v(const AdjArgs<R> &a, int nd_case, void *stream) {
-    SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, HH>(a, nd_case, (hipStream_t)stream)));
+void launch_adjh_v(const AdjArgs<R> &a, int nd_case, bool half, void *stream) {
+    if (half) {
+        SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, __half, HH>(a, nd_case, (hipStream_t)stream)));
+    } else {
+        SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, R, HH>(a, nd_case, (hipStream_t)stream)));
+    }
 }
 template <typename R>
 void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream) {
@@ -913,8 +934,8 @@ void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const i
 template void launch_adj_s<float>(const AdjArgs<float> &, int, void *);
 template void launch_surface_adj<float>(const FDArgs<float> &, float *, int, int, void *);
 template void launch_adj_v<float>(const AdjArgs<float> &, int, void *);
-template void launch_adjh_s<float>(const AdjArgs<float> &, int, int, void *);
-template void launch_adjh_v<float>(const AdjArgs<float> &, int, void *);
+template void launch_adjh_s<float>(const AdjArgs<float> &, int, int, bool, void *);
+template void launch_adjh_v<float>(const AdjArgs<float> &, int, bool, void *);
 template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *);
 template void launch_residuals<float>(const FDArgs<float> &, int, const ShotDev &, const int *,
                                       int, const float *, int, int, int, void *);

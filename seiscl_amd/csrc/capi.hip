@@ -144,10 +144,12 @@ struct seis_model {
         // viscoelastic runs: full-state checkpoints every K steps and the
         // recomputed post-source history of one segment of K steps
         bool visco = false;
-        int K = 0, nseg = 0;
-        float *ckpt = nullptr;      // [nseg][state]
-        float *hist = nullptr;      // [K][nst][nfield]
+        int plan[4] = {0, 0, 0, 0};  // K1, K2, n1, n2 of the checkpoint plan
+        float *ckpt = nullptr;      // [n1 + n2 - 1][state]
+        float *hist = nullptr;      // [K2][nvb][nfield] of the storage type
         size_t ckpt_bytes = 0, hist_bytes = 0;
+        size_t mem_limit = 0;       // budget of ckpt + hist; 0: 90% of free memory
+        size_t plan_bytes = 0;      // checkpoints + history of the plan
         float *Ms = nullptr, *mus = nullptr;   // moduli of the gradient formulas
     } grad;
 
@@ -502,7 +504,13 @@ int grad_setup(seis_model *m) {
     const seis_csts &c = m->c;
     if (c.ND == 22)
         return fail("gradients need the elastic or viscoelastic case (ND 2 or 3)");
-    if (m->prec != P_FP32) return fail("gradients are available with FP32 wavefields (FP16=0)");
+    if (m->prec == P_FP64) return fail("gradients are not available in FP64 runs (FP16=64)");
+    if (m->prec != P_FP32 && c.L == 0)
+        return fail("elastic gradients need FP32 wavefields (FP16=0); viscoelastic ones also "
+                    "run with FP16 storage (FP16=2) or scaled units (FP16=1)");
+    if (m->prec != P_FP32 && c.abs_type == 1 && !(c.gradfix & FIX_ADJ_CPML))
+        return fail("gradients with scaled wavefields (FP16=1 or 2) need the adjoint run's own "
+                    "CPML memory (gradfix bit 2)");
     G.visco = c.L > 0;
     if (!G.visco) {
         G.sh = make_shell(m->g, c.abs_type > 0 ? c.nab : 0, c.freesurf);
@@ -607,7 +615,7 @@ FDArgs<float> adjoint_args(const seis_model *m) {
 // (device pointer, bytes) pairs: what a checkpoint holds.
 std::vector<std::pair<float *, size_t>> forward_state(const seis_model *m) {
     const FDArgs<float> &a = m->a;
-    const size_t nf = (size_t)m->g.nfield * 4, L = m->c.L;
+    const size_t nf = (size_t)m->g.nfield * m->selem, L = m->c.L, ce = m->celem;
     std::vector<std::pair<float *, size_t>> st;
     float *fl[] = {a.vx, a.vy, a.vz, a.sxx, a.syy, a.szz, a.sxy, a.syz, a.sxz};
     for (float *p : fl)
@@ -617,7 +625,7 @@ std::vector<std::pair<float *, size_t>> forward_state(const seis_model *m) {
         if (p) st.push_back({p, nf * L});
     if (m->c.abs_type == 1) {
         const size_t nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
-        const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
+        const size_t sx = nab2 * NY * NZ * ce, sy = NX * nab2 * NZ * ce, sz = NX * NY * nab2 * ce;
         float *px[] = {a.psi_sxx_x, a.psi_sxy_x, a.psi_sxz_x, a.psi_vx_x, a.psi_vy_x, a.psi_vz_x};
         float *py[] = {a.psi_sxy_y, a.psi_syy_y, a.psi_syz_y, a.psi_vx_y, a.psi_vy_y, a.psi_vz_y};
         float *pz[] = {a.psi_sxz_z, a.psi_syz_z, a.psi_szz_z, a.psi_vx_z, a.psi_vy_z, a.psi_vz_z};
@@ -916,7 +924,8 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
         sd.src = (float *)p;
         if (ctx->alloc(nr * 4, &p, xf)) return 1;
         sd.rec_cell = (int *)p;
-        if (ctx->alloc((size_t)nr * NT * nout * 4, &p, xf)) return 1;
+        // FP64 runs record their seismograms in double
+        if (ctx->alloc((size_t)nr * NT * nout * (m->prec == P_FP64 ? 8 : 4), &p, xf)) return 1;
         sd.out = (float *)p;
         m->shot_nsrc_cap = ns;
         m->shot_nrec_cap = nr;
@@ -975,7 +984,8 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
 int seis_launch_init(seis_model *m) {
     zero_state(m);
     if (m->shot.out)
-        launch_fill(m->shot.out, (size_t)m->shot.nrec * m->c.NT * m->outn.size() * 4,
+        launch_fill(m->shot.out,
+                    (size_t)m->shot.nrec * m->c.NT * m->outn.size() * (m->prec == P_FP64 ? 8 : 4),
                     m->ctx->stream);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1011,9 +1021,13 @@ int seis_launch_varsout(seis_model *m, int nt) {
     return 0;
 }
 
-int seis_shot_read(seis_model *m, float *out) {
+namespace {
+// The bound shot's seismograms as [nout][nrec][NT] into outf (float) or
+// outd (double) (device: [NT][nout][nrec] of float, or of double in FP64 runs).
+int shot_read(seis_model *m, float *outf, double *outd) {
     const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
-    std::vector<float> tmp((size_t)NT * nout * nrec);
+    const bool dbl = m->prec == P_FP64;
+    std::vector<char> raw((size_t)NT * nout * nrec * (dbl ? 8 : 4));
     // a batch job is read on the transfer stream once its last step is
     // done, while the batch keeps running on the compute stream
     hipStream_t st = m->ctx->stream;
@@ -1021,14 +1035,29 @@ int seis_shot_read(seis_model *m, float *out) {
         st = m->ctx->xfer;
         HIP_TRY(hipStreamWaitEvent(st, m->done_ev, 0));
     }
-    if (!tmp.empty())
-        HIP_TRY(hipMemcpyAsync(tmp.data(), m->shot.out, tmp.size() * 4, hipMemcpyDeviceToHost, st));
+    if (!raw.empty())
+        HIP_TRY(hipMemcpyAsync(raw.data(), m->shot.out, raw.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const float *tf = (const float *)raw.data();
+    const double *td = (const double *)raw.data();
     for (int o = 0; o < nout; ++o)
         for (int r = 0; r < nrec; ++r)
-            for (int t = 0; t < NT; ++t)
-                out[((size_t)o * nrec + r) * NT + t] = tmp[((size_t)t * nout + o) * nrec + r];
+            for (int t = 0; t < NT; ++t) {
+                const size_t k = ((size_t)t * nout + o) * nrec + r;
+                const size_t j = ((size_t)o * nrec + r) * NT + t;
+                if (outd) outd[j] = dbl ? td[k] : (double)tf[k];
+                else outf[j] = dbl ? (float)td[k] : tf[k];
+            }
     return 0;
+}
+}  // namespace
+
+int seis_shot_read(seis_model *m, float *out) { return shot_read(m, out, nullptr); }
+
+int seis_shot_read_f64(seis_model *m, double *out) {
+    g_err.clear();
+    if (m->prec != P_FP64) return fail("seis_shot_read_f64: the model does not run in FP64 (FP16=64)");
+    return shot_read(m, nullptr, out);
 }
 
 int seis_run_steps(seis_model *m, int t0, int nsteps) {
@@ -1119,6 +1148,21 @@ int seis_read_field(seis_model *m, const char *name, float *host) {
 
 // ---- gradient by backpropagation ------------------------------------------
 
+int seis_grad_set_memory(seis_model *m, size_t bytes) {
+    m->grad.mem_limit = bytes;
+    return 0;
+}
+
+int seis_grad_plan(const seis_model *m, int *K1, int *K2, int *n1, int *n2, size_t *bytes) {
+    const int *p = m->grad.plan;
+    if (K1) *K1 = p[0];
+    if (K2) *K2 = p[1];
+    if (n1) *n1 = p[2];
+    if (n2) *n2 = p[3];
+    if (bytes) *bytes = m->grad.plan_bytes;
+    return 0;
+}
+
 int seis_grad_zero(seis_model *m) {
     g_err.clear();
     if (grad_setup(m)) return 1;
@@ -1133,9 +1177,10 @@ int seis_grad_zero(seis_model *m) {
 
 namespace {
 
-// (Re)allocate a gradient work buffer of at least `bytes`.
-int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes) {
-    if (*have >= bytes) return 0;
+// (Re)allocate a gradient work buffer of at least `bytes` (exact: of exactly
+// `bytes`, freeing it when 0).
+int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes, bool exact) {
+    if (exact ? *have == bytes : *have >= bytes) return 0;
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     if (*buf) {
         m->bytes -= *have;
@@ -1143,6 +1188,7 @@ int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes) {
         *buf = nullptr;
         *have = 0;
     }
+    if (!bytes) return 0;
     void *p = nullptr;
     if (m->ctx->alloc(bytes, &p)) return 1;
     *buf = (float *)p;
@@ -1151,9 +1197,75 @@ int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes) {
     return 0;
 }
 
-// Viscoelastic forward run for the gradient: checkpoints of the full state
-// every K steps, K chosen to balance the checkpoints against the history of
-// one segment (about sqrt(tmax * state / history-per-step) steps).
+// Checkpoint plan of a viscoelastic gradient over T steps: level-1
+// checkpoints of the full state (S bytes) every K1 steps, taken by the
+// forward run; the adjoint recomputes each level-1 segment once, keeping
+// level-2 checkpoints every K2 steps (n2 - 1 of them: the last sub-segment
+// starts from the state the recomputation ends on), then recomputes each
+// sub-segment keeping its K2 post-source states (H bytes each) and runs the
+// adjoint steps backwards over them.  One level (K1 = K2, n2 = 1) costs one
+// recomputation of the forward run, two levels cost two; the plan is the
+// one-level plan of least memory if it fits the budget B, else the two-level
+// plan of least memory.
+struct CkptPlan {
+    int K1 = 0, K2 = 0, n1 = 0, n2 = 0;
+    size_t bytes = 0;
+};
+bool plan_checkpoints(int T, size_t S, size_t H, size_t B, CkptPlan &best) {
+    best = CkptPlan{};
+    bool found = false;
+    for (int K = 1; K <= T; ++K) {
+        const int n1 = (T + K - 1) / K;
+        const size_t b = (size_t)n1 * S + (size_t)K * H;
+        if (b <= B && (!found || b < best.bytes)) {
+            best = CkptPlan{K, K, n1, 1, b};
+            found = true;
+        }
+    }
+    if (found) return true;
+    for (int K2 = 1; K2 < T; ++K2)
+        for (int n2 = 2; (long)n2 * K2 < 2L * T; ++n2) {
+            const int K1 = n2 * K2, n1 = (T + K1 - 1) / K1;
+            const size_t b = (size_t)n1 * S + (size_t)(n2 - 1) * S + (size_t)K2 * H;
+            if (b <= B && (!found || b < best.bytes)) {
+                best = CkptPlan{K1, K2, n1, n2, b};
+                found = true;
+            }
+        }
+    return found;
+}
+
+size_t state_bytes(const std::vector<std::pair<float *, size_t>> &st) {
+    size_t b = 0;
+    for (auto &e : st) b += e.second;
+    return b;
+}
+int state_save(const std::vector<std::pair<float *, size_t>> &st, char *dst, hipStream_t s) {
+    for (auto &e : st) {
+        HIP_TRY(hipMemcpyAsync(dst, e.first, e.second, hipMemcpyDeviceToDevice, s));
+        dst += e.second;
+    }
+    return 0;
+}
+int state_load(const std::vector<std::pair<float *, size_t>> &st, const char *src,
+               hipStream_t s) {
+    for (auto &e : st) {
+        HIP_TRY(hipMemcpyAsync(e.first, src, e.second, hipMemcpyDeviceToDevice, s));
+        src += e.second;
+    }
+    return 0;
+}
+
+// One forward step without receivers (the recomputations).
+void step_forward(seis_model *m, int t, hipStream_t st) {
+    step_sources(m, t, 1, st);
+    step_v(m, st);
+    step_s(m, st);
+    if (m->c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+}
+
+// Viscoelastic forward run for the gradient: the level-1 checkpoints of the
+// plan, then the seismograms.
 int forward_save_visco(seis_model *m, float *out) {
     auto &G = m->grad;
     const seis_csts &c = m->c;
@@ -1170,34 +1282,36 @@ int forward_save_visco(seis_model *m, float *out) {
     HIP_TRY(hipMemcpyAsync(G.mus, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     const auto state = forward_state(m);
-    size_t sbytes = 0;
-    for (auto &e : state) sbytes += e.second;
-    const size_t hbytes = (size_t)G.nvb * m->g.nfield * 4;
-    int K = (int)std::lround(std::sqrt((double)tmax * sbytes / hbytes));
-    K = std::max(1, std::min(K, tmax));
-    const int nseg = (tmax + K - 1) / K;
-    const size_t need = (size_t)nseg * sbytes + (size_t)K * hbytes;
+    const size_t sbytes = state_bytes(state);
+    const size_t hbytes = (size_t)G.nvb * m->g.nfield * m->selem;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    if (need > G.ckpt_bytes + G.hist_bytes + freeb / 10 * 9) {
-        char buf[200];
+    const size_t held = G.ckpt_bytes + G.hist_bytes;
+    const size_t budget = G.mem_limit ? G.mem_limit : held + freeb / 10 * 9;
+    CkptPlan P;
+    if (!plan_checkpoints(tmax, sbytes, hbytes, budget, P)) {
+        char buf[240];
         std::snprintf(buf, sizeof(buf),
-                      "viscoelastic gradient: %d checkpoints and %d history steps need %.1f GB, "
-                      "%.1f GB are free", nseg, K, need / 1e9, freeb / 1e9);
+                      "viscoelastic gradient: no checkpoint plan for %d steps fits %.2f GB "
+                      "(state %.2f GB, history step %.2f GB)", tmax, budget / 1e9, sbytes / 1e9,
+                      hbytes / 1e9);
         return fail(buf);
     }
-    if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, (size_t)nseg * sbytes)) return 1;
-    if (grad_buffer(m, &G.hist, &G.hist_bytes, (size_t)K * hbytes)) return 1;
-    G.K = K;
-    G.nseg = nseg;
+    const size_t ckb = (size_t)(P.n1 + P.n2 - 1) * sbytes, hib = (size_t)P.K2 * hbytes;
+    if (G.ckpt_bytes + G.hist_bytes > budget) {   // a larger earlier plan: give it back
+        if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, 0, true)) return 1;
+        if (grad_buffer(m, &G.hist, &G.hist_bytes, 0, true)) return 1;
+    }
+    if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, ckb, false)) return 1;
+    if (grad_buffer(m, &G.hist, &G.hist_bytes, hib, false)) return 1;
+    G.plan[0] = P.K1;
+    G.plan[1] = P.K2;
+    G.plan[2] = P.n1;
+    G.plan[3] = P.n2;
+    G.plan_bytes = P.bytes;
     for (int t = 0; t < tmax; ++t) {
-        if (t % K == 0) {
-            char *dst = (char *)G.ckpt + (size_t)(t / K) * sbytes;
-            for (auto &e : state) {
-                HIP_TRY(hipMemcpyAsync(dst, e.first, e.second, hipMemcpyDeviceToDevice, st));
-                dst += e.second;
-            }
-        }
+        if (t % P.K1 == 0 && state_save(state, (char *)G.ckpt + (size_t)(t / P.K1) * sbytes, st))
+            return 1;
         step_sources(m, t, 1, st);
         step_v(m, st);
         step_s(m, st);
@@ -1210,47 +1324,59 @@ int forward_save_visco(seis_model *m, float *out) {
     return out ? seis_shot_read(m, out) : 0;
 }
 
-// Viscoelastic adjoint: segments from the last to the first; each restores
-// its checkpoint, recomputes its steps while keeping every post-source
-// state, then runs its adjoint steps backwards over them.
+// Viscoelastic adjoint over the plan of forward_save_visco: level-1
+// segments from the last to the first; each is recomputed once to lay down
+// its level-2 checkpoints, then each sub-segment is recomputed keeping
+// every post-source state and its adjoint steps run backwards over them.
 int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial) {
     auto &G = m->grad;
     const seis_csts &c = m->c;
     const int tmax = tmax_of(m), tmin = c.tmin, NT = c.NT, nout = (int)m->outn.size();
+    const int K1 = G.plan[0], K2 = G.plan[1], n1 = G.plan[2];
     hipStream_t st = m->ctx->stream;
     const auto state = forward_state(m);
-    size_t sbytes = 0;
-    for (auto &e : state) sbytes += e.second;
+    const size_t sbytes = state_bytes(state);
+    const char *l1 = (const char *)G.ckpt, *l2 = l1 + (size_t)n1 * sbytes;
     const auto hf = history_fields(m);
-    const size_t nf = (size_t)m->g.nfield;
+    const size_t fb = (size_t)m->g.nfield * m->selem;     // bytes of one stored field
+    const bool half = m->prec == P_FP16;
     A.Ms = G.Ms;
     A.mus = G.mus;
-    for (int seg = G.nseg - 1; seg >= 0; --seg) {
-        const int t0 = seg * G.K, t1 = std::min(t0 + G.K, tmax);
+    for (int s1 = n1 - 1; s1 >= 0; --s1) {
+        const int t0 = s1 * K1, t1 = std::min(t0 + K1, tmax);
         if (t1 - 1 <= tmin) break;
-        const char *src = (const char *)G.ckpt + (size_t)seg * sbytes;
-        for (auto &e : state) {
-            HIP_TRY(hipMemcpyAsync(e.first, src, e.second, hipMemcpyDeviceToDevice, st));
-            src += e.second;
+        if (state_load(state, l1 + (size_t)s1 * sbytes, st)) return 1;
+        const int nsub = (t1 - t0 + K2 - 1) / K2;
+        // level 2: checkpoints at the sub-segment starts but the last
+        for (int t = t0; t < t0 + (nsub - 1) * K2; ++t) {
+            if ((t - t0) % K2 == 0 &&
+                state_save(state, (char *)l2 + (size_t)((t - t0) / K2) * sbytes, st))
+                return 1;
+            step_forward(m, t, st);
         }
-        for (int t = t0; t < t1; ++t) {
-            step_sources(m, t, 1, st);
-            float *dst = G.hist + (size_t)(t - t0) * G.nvb * nf;
-            for (size_t q = 0; q < hf.size(); ++q)
-                HIP_TRY(hipMemcpyAsync(dst + q * nf, hf[q], nf * 4, hipMemcpyDeviceToDevice, st));
-            step_v(m, st);
-            step_s(m, st);
-            if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
-        }
-        for (int t = t1 - 1; t >= std::max(t0, tmin + 1); --t) {
-            if (c.freesurf)
-                launch_surface_adj(A.r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
-            launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t,
-                             serial, st);
-            const float *h = G.hist + (size_t)(t - t0) * G.nvb * nf;
-            for (size_t q = 0; q < hf.size(); ++q) A.st[q] = h + q * nf;
-            launch_adjh_s(A, m->nd_case, c.L, st);
-            launch_adjh_v(A, m->nd_case, st);
+        for (int s2 = nsub - 1; s2 >= 0; --s2) {
+            const int u0 = t0 + s2 * K2, u1 = std::min(u0 + K2, t1);
+            if (u1 - 1 <= tmin) break;
+            if (s2 < nsub - 1 && state_load(state, l2 + (size_t)s2 * sbytes, st)) return 1;
+            for (int t = u0; t < u1; ++t) {
+                step_sources(m, t, 1, st);
+                char *dst = (char *)G.hist + (size_t)(t - u0) * G.nvb * fb;
+                for (size_t q = 0; q < hf.size(); ++q)
+                    HIP_TRY(hipMemcpyAsync(dst + q * fb, hf[q], fb, hipMemcpyDeviceToDevice, st));
+                step_v(m, st);
+                step_s(m, st);
+                if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+            }
+            for (int t = u1 - 1; t >= std::max(u0, tmin + 1); --t) {
+                if (c.freesurf)
+                    launch_surface_adj(A.r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
+                launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT,
+                                 t, serial, st);
+                const char *h = (const char *)G.hist + (size_t)(t - u0) * G.nvb * fb;
+                for (size_t q = 0; q < hf.size(); ++q) A.st[q] = h + q * fb;
+                launch_adjh_s(A, m->nd_case, c.L, half, st);
+                launch_adjh_v(A, m->nd_case, half, st);
+            }
         }
     }
     HIP_TRY(hipGetLastError());
@@ -1342,7 +1468,21 @@ int seis_adjoint(seis_model *m, const float *res) {
         G.res = (float *)p;
         G.res_cap = nres;
     }
+    // scaled wavefields (FP16=1, 2): the adjoint run keeps its stresses in
+    // physical units and its velocities times q, the units the scaled
+    // coefficients imply, so velocity residuals are injected times q
+    const bool scaled = m->prec == P_SCALED || m->prec == P_FP16;
+    std::vector<float> rs;
+    if (scaled && nres) {
+        rs.assign(res, res + nres);
+        for (int o = 0; o < nout; ++o)
+            if (m->out_codes[o] <= O_VZ)
+                for (size_t k = (size_t)o * nrec * NT; k < (size_t)(o + 1) * nrec * NT; ++k)
+                    rs[k] *= m->q;
+        res = rs.data();
+    }
     if (nres) HIP_TRY(hipMemcpyAsync(G.res, res, nres * 4, hipMemcpyHostToDevice, st));
+    if (!rs.empty()) HIP_TRY(hipStreamSynchronize(st));
     grad_zero_adjoint(m);
     AdjArgs<float> A;
     A.f = m->a;
@@ -1351,6 +1491,12 @@ int seis_adjoint(seis_model *m, const float *res) {
     A.gmu = G.gmu;
     A.grho = G.grho;
     A.sh = G.sh;
+    if (scaled) {
+        // forward stresses are stored times as, adjoint ones as is;
+        // forward velocities times av, adjoint ones times q
+        A.gscale_s = 1.0 / m->as;
+        A.gscale_v = 1.0 / ((double)m->av * m->q);
+    }
     if (m->c.freesurf && (!A.r.M || !A.r.mu || !G.stmp))
         return fail("seis_adjoint: the adjoint free surface has no moduli on the device");
     const int serial = receivers_share_cells(m, m->rcell_host);
@@ -1360,20 +1506,44 @@ int seis_adjoint(seis_model *m, const float *res) {
         return r;
     }
     const size_t per = (size_t)G.nvb * G.sh.n;
+    // with profiling on, events around the two fused kernels of every step
+    const bool prof = m->profiling;
+    const size_t nst = (size_t)(tmax - 1 - tmin);
+    hipEvent_t e0 = pool_event(m, 0), e1 = pool_event(m, 1);
+    if (!e0 || !e1 || (prof && !pool_event(m, 1 + 4 * nst))) return fail("hipEventCreate failed");
+    HIP_TRY(hipEventRecord(e0, st));
+    size_t k = 0;
     // adjoint loop of src/time_stepping.c:869-940
-    for (int t = tmax - 1; t > tmin; --t) {
+    for (int t = tmax - 1; t > tmin; --t, ++k) {
         if (m->c.freesurf)
             launch_surface_adj(A.r, G.stmp, m->nd_case, m->c.gradfix & FIX_SURF_SYY, st);
         launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t, serial,
                          st);
         A.bnd = G.bnd + (size_t)(t - 1) * per;
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
         launch_adj_s(A, m->nd_case, st);
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
         launch_adj_v(A, m->nd_case, st);
+        if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
         step_sources(m, t, -1, st);
     }
+    HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventSynchronize(e1));
     G.saved = false;   // the forward fields are now at the start of the run
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    m->ms_loop = ms;
+    m->ms_v = m->ms_s = 0;
+    if (prof)
+        for (size_t q = 0; q < nst; ++q) {
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, m->evpool[2 + 4 * q], m->evpool[3 + 4 * q]));
+            HIP_TRY(hipEventElapsedTime(&b, m->evpool[4 + 4 * q], m->evpool[5 + 4 * q]));
+            m->ms_s += a;
+            m->ms_v += b;
+        }
     return 0;
 }
 

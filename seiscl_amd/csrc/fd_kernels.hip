@@ -474,7 +474,10 @@ __device__ void varsout_body(const FDArgs<R> &a, int nd_case, const ShotDev &s, 
                     v = (R)((double)sm / (nd_case == 3 ? 3.0 : 2.0));
                 }
         }
-        out[(long)q * s.nrec + r] = (float)v * o.unscale[q];
+        if constexpr (sizeof(R) == 8)
+            ((double *)s.out)[((long)nt * o.n + q) * s.nrec + r] = v * (R)o.unscale[q];
+        else
+            out[(long)q * s.nrec + r] = (float)v * o.unscale[q];
     }
 }
 

@@ -301,21 +301,26 @@ struct AdjArgs {
     const R *bnd;                // [nvars][n] shell values of step t-1
     Shell sh;
     // viscoelastic runs: the forward state of the step from the recomputed
-    // history (2D: vx vz sxx szz sxz; 3D: vx vy vz sxx syy szz sxy syz sxz)
-    // and the moduli of the gradient formulas (padded layout)
-    const R *st[9];
+    // history (2D: vx vz sxx szz sxz; 3D: vx vy vz sxx syy szz sxy syz sxz),
+    // in the model's storage type and units, and the moduli of the gradient
+    // formulas (padded layout)
+    const void *st[9];
     const R *Ms, *mus;
+    // factors taking the accumulated M/mu and rho increments back to
+    // physical units (1 unless the wavefields are stored scaled)
+    double gscale_s = 1, gscale_v = 1;
 };
 
 template <typename R>
 void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream);
 template <typename R>
 void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream);
-// viscoelastic adjoint steps reading the forward state from A.st (L = 1..4)
+// viscoelastic adjoint steps reading the forward state from A.st (L = 1..4),
+// stored as float, or as half when half is set
 template <typename R>
-void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, void *stream);
+void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, bool half, void *stream);
 template <typename R>
-void launch_adjh_v(const AdjArgs<R> &a, int nd_case, void *stream);
+void launch_adjh_v(const AdjArgs<R> &a, int nd_case, bool half, void *stream);
 // adjoint free surface on the adjoint wavefields r; tmp: 3 * NX * NY reals
 template <typename R>
 void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, void *stream);

@@ -194,10 +194,17 @@ class Model:
         """Advance the bound shot by nsteps time steps starting at step t0."""
         check(self.lib.seis_run_steps(self.handle, int(t0), int(nsteps)))
 
-    def read_shot(self):
+    def read_shot(self, f64: bool = False):
+        """Seismograms of the bound shot, (NT, nrec) per output; f64: in
+        double, for models running in FP64 (FP16=64)."""
         nrec = self._shot[2].shape[0]
-        out = np.zeros((len(self.out_names), nrec, self.csts.NT), dtype=np.float32)
-        check(self.lib.seis_shot_read(self.handle, fptr(out)))
+        dt = np.float64 if f64 else np.float32
+        out = np.zeros((len(self.out_names), nrec, self.csts.NT), dtype=dt)
+        if f64:
+            check(self.lib.seis_shot_read_f64(
+                self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        else:
+            check(self.lib.seis_shot_read(self.handle, fptr(out)))
         return [np.ascontiguousarray(o.T) for o in out]
 
     def read_field(self, name: str) -> np.ndarray:
@@ -216,6 +223,21 @@ class Model:
 
     def grad_zero(self):
         check(self.lib.seis_grad_zero(self.handle))
+
+    def set_grad_memory(self, nbytes: int):
+        """Budget of device bytes for the viscoelastic gradient's checkpoints
+        and recomputed history (0: 90% of the free device memory)."""
+        check(self.lib.seis_grad_set_memory(self.handle, int(nbytes)))
+
+    def grad_plan(self) -> dict:
+        """Checkpoint plan of the last viscoelastic forward_save."""
+        v = [ctypes.c_int() for _ in range(4)]
+        b = ctypes.c_size_t()
+        check(self.lib.seis_grad_plan(self.handle, *[ctypes.byref(x) for x in v],
+                                      ctypes.byref(b)))
+        d = dict(zip(("K1", "K2", "n1", "n2"), (x.value for x in v)))
+        d["bytes"] = b.value
+        return d
 
     def forward_save(self, src_pos, src, rec_pos):
         """Forward run that keeps the boundary shell of every step for the

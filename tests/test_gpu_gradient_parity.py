@@ -138,3 +138,92 @@ def test_gradient_accumulates_over_shots(oracle_lib, gpu_ctx):
     for k in g:
         tot = refs[0][1][k].astype(np.float64) + refs[1][1][k]
         assert rel_l2(g[k], tot) <= 1e-5, k
+
+
+# ---- viscoelastic gradients with scaled or FP16 wavefield storage (3D) and
+# ---- the checkpoint plans
+
+VISCO_SCALED = [dict(ND=3, FDORDER=4, L=2, abs_type=1),
+                dict(ND=3, FDORDER=4, L=2, abs_type=1, freesurf=1),
+                dict(ND=3, FDORDER=4, L=1, abs_type=2, freesurf=1)]
+
+
+def _gpu_grad(gpu_ctx, case, FP16=0, mem=0):
+    from seiscl_amd import Model
+    csts, params, (sp, s, rp), true = _setup(case)
+    print(f"[grad visco] case={_ids(case)} FP16={FP16} mem={mem}", flush=True)
+    m = Model(gpu_ctx, true, **csts)
+    din = m.forward(sp, s, rp)
+    m.close()
+    m = Model(gpu_ctx, params, **dict(csts, FP16=FP16))
+    m.set_grad_memory(mem)
+    m.grad_zero()
+    out, _ = m.gradient(sp, s, rp, din=din)
+    g = m.grad_read()
+    plan = m.grad_plan()
+    m.close()
+    return out, g, plan
+
+
+@pytest.mark.parametrize("case", VISCO_SCALED, ids=_ids)
+def test_visco_gradient_scaled_units(gpu_ctx, case):
+    """FP16=1 stores FP32 wavefields in the scaled units of FP16 storage
+    (velocities times av, stresses times as, coefficients rescaled by
+    q = av/as, adjoint velocities times q).  Every factor is a power of two,
+    so the runs differ only where FP32 values ahead of the wavefront are
+    subnormal in one scaling and not in the other: seismograms within 1e-6
+    and gradients within 1e-5 relative L2 of the FP32 run (a wrong unit
+    factor would be off by O(1))."""
+    o32, g32, _ = _gpu_grad(gpu_ctx, case, 0)
+    o1, g1, _ = _gpu_grad(gpu_ctx, case, 1)
+    for a, b in zip(o1, o32):
+        assert rel_l2(a, b) <= 1e-6
+    for k in g32:
+        assert np.abs(g32[k]).max() > 0, k
+        err = rel_l2(g1[k], g32[k])
+        print(f"[grad visco] scaled units {k}: rel L2 {err:.2e}", flush=True)
+        assert err <= 1e-5, k
+
+
+@pytest.mark.parametrize("case", VISCO_SCALED, ids=_ids)
+def test_visco_gradient_fp16_storage(gpu_ctx, case):
+    """FP16 storage of the forward wavefields and memory variables (FP32
+    arithmetic, FP32 adjoint fields, double accumulation): gradients within
+    1e-2 relative L2 of the FP32 run."""
+    o32, g32, _ = _gpu_grad(gpu_ctx, case, 0)
+    o16, g16, _ = _gpu_grad(gpu_ctx, case, 2)
+    for a, b in zip(o16, o32):
+        print(f"[grad visco] FP16 storage seismogram: rel L2 {rel_l2(a, b):.2e}", flush=True)
+    for k in g32:
+        err = rel_l2(g16[k], g32[k])
+        print(f"[grad visco] FP16 storage {k}: rel L2 {err:.2e}", flush=True)
+        assert err <= 2e-2, k
+
+
+@pytest.mark.parametrize("case", [dict(ND=2, FDORDER=4, L=2, abs_type=1),
+                                  dict(ND=3, FDORDER=4, L=2, abs_type=1, freesurf=1)], ids=_ids)
+def test_visco_checkpoint_plans(gpu_ctx, case):
+    """The default plan is the one-level plan of least memory.  A budget one
+    byte smaller leaves no one-level plan, and the two-level plan (level-1
+    segments recomputed once more to lay down level-2 checkpoints) takes
+    over.  The recomputed states are the same, so the gradients equal the
+    default plan's bit for bit."""
+    _, g0, p0 = _gpu_grad(gpu_ctx, case)
+    assert p0["n2"] == 1 and p0["bytes"] > 0, p0
+    _, g2, p2 = _gpu_grad(gpu_ctx, case, mem=p0["bytes"] - 1)
+    print(f"[grad visco] plans {p0} {p2}", flush=True)
+    assert p2["n2"] > 1 and p2["bytes"] < p0["bytes"], p2
+    for k in g0:
+        assert np.abs(g0[k]).max() > 0, k
+        np.testing.assert_array_equal(g2[k], g0[k])
+
+
+def test_visco_checkpoint_budget_error(gpu_ctx):
+    from seiscl_amd import Model
+    from seiscl_amd._lib import SeisError
+    csts, params, (sp, s, rp), _ = _setup(dict(ND=2, FDORDER=4, L=1))
+    m = Model(gpu_ctx, params, **csts)
+    m.set_grad_memory(1024)
+    with pytest.raises(SeisError, match="no checkpoint plan"):
+        m.forward_save(sp, s, rp)
+    m.close()
