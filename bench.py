@@ -217,6 +217,98 @@ def gradient_main(args):
     print(json.dumps(out), flush=True)
 
 
+# viscoelastic adjoint kernels, 3D L=2 FD4, per cell: adjoint velocities read,
+# adjoint stresses and memory variables read-modify-write, 10 coefficient
+# arrays, the forward stresses of the step (FP16: 2 B, FP32: 4 B each), the
+# gradient moduli, M and mu accumulators (double) read-modify-write
+BYTES_ADJH_S_3D = {2: 248, 0: 260, 1: 260}
+BYTES_ADJH_V_3D = {2: 82, 0: 88, 1: 88}
+
+
+def gradient3d_main(args):
+    """3D viscoelastic FWI gradient with FP16 wavefield storage: n^3 (512),
+    NT 2500, L=2, FD4, CPML 20, free surface, 8 shots.  Per shot: forward run
+    laying down level-1 checkpoints, then the adjoint (recomputed segments
+    and adjoint steps); the adjoint sources are the shot's own seismograms
+    (gradient of 1/2 |u|^2, the inputres mode), so no observed data need
+    modeling.  Gradients accumulate in double over the shots."""
+    from seiscl_amd import Context, Model
+    from seiscl_amd.workloads import config5
+    n = args.n or 512
+    NT = args.nt or 2500
+    t0 = time.time()
+    csts, params, shots = config5(n=n, NT=NT, nshots=args.shots)
+    log(f"workload n={n} NT={NT} shots={len(shots)} built in {time.time() - t0:.1f} s")
+    ctx = Context(0)
+    ncell = n ** 3
+
+    def run(fp16, shots_run):
+        t0 = time.time()
+        m = Model(ctx, params, **dict(csts, FP16=fp16))
+        log(f"FP16={fp16}: model resident in {time.time() - t0:.1f} s")
+        m.set_profiling(True)
+        m.grad_zero()
+        rows = []
+        for k, sh in enumerate(shots_run):
+            ctx.sync()
+            ta = time.perf_counter()
+            out = m.forward_save(*sh)
+            tb = time.perf_counter()
+            m.adjoint(out)
+            ctx.sync()
+            tc = time.perf_counter()
+            tm = m.timing()
+            rows.append((tb - ta, tc - tb, tm))
+            log(f"FP16={fp16} shot {k}: forward {tb - ta:.2f} s, adjoint {tc - tb:.2f} s, "
+                f"plan {m.grad_plan()}, footprint {m.footprint() / 2**30:.1f} GiB")
+        g = m.grad_read()
+        plan, fp = m.grad_plan(), m.footprint()
+        m.close()
+        return rows, g, plan, fp
+
+    rows, g16, plan, fp = run(args.fp16, shots)
+    fwd = sum(r[0] for r in rows)
+    adj = sum(r[1] for r in rows)
+    nadj = NT - 1
+    adj_s = sum(r[2]["update_s_ms"] for r in rows) / (len(rows) * nadj)
+    adj_v = sum(r[2]["update_v_ms"] for r in rows) / (len(rows) * nadj)
+    b_s, b_v = BYTES_ADJH_S_3D[args.fp16], BYTES_ADJH_V_3D[args.fp16]
+    ach = b_s * ncell / (adj_s / 1e3) / 1e9
+    cfg = {
+        "workload": (f"3D viscoelastic FWI gradient, {n}^3, L=2, FD4, CPML 20, free surface, "
+                     f"NT {NT}, {len(shots)} shots, FP16={args.fp16} storage, FP64 "
+                     "accumulation, checkpoints + recomputation"),
+        "grid": [n, n, n], "NT": NT, "shots": len(shots), "storage": args.fp16,
+        "forward_s": round(fwd, 3), "adjoint_s": round(adj, 3),
+        "forward_gcell_s": round(len(shots) * NT * ncell / fwd / 1e9, 3),
+        "checkpoint_plan": plan, "footprint_gib": round(fp / 2**30, 2),
+        "adj_s_ms": round(adj_s, 4), "adj_v_ms": round(adj_v, 4),
+        "adj_v_gbs": round(b_v * ncell / (adj_v / 1e3) / 1e9, 1),
+    }
+    if args.check_fp32:
+        # the first shot again in FP32 storage: gradient error of the storage
+        _, g32, p32, _ = run(0, shots[:1])
+        _, g1, _, _ = run(args.fp16, shots[:1])
+        cfg["fp32_plan"] = p32
+        cfg["rel_l2_vs_fp32"] = {k: float(np.linalg.norm(g1[k].astype(np.float64) - g32[k])
+                                          / np.linalg.norm(g32[k].astype(np.float64)))
+                                 for k in g32}
+        log(f"gradient error against FP32 storage (shot 0): {cfg['rel_l2_vs_fp32']}")
+    total = fwd + adj
+    out = {
+        "metric": METRIC, "value": round(len(shots) * NT * ncell / total / 1e9, 4),
+        "unit": "Gcell/s", "n_gpus": 1, "steps": len(shots), "warmup": 0,
+        "ms_per_step": round(total / len(shots) * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "kernel": "viscoelastic adjoint stress update (3D, L=2)",
+                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def fd_check(ctx, csts, start, shots, din, grad, ncells):
     """Central differences (h = 1e-3 of the value) of J = 1/2 sum (out - d)^2
     from FP64 runs, on ncells random interior cells per parameter."""
@@ -265,8 +357,13 @@ def fd_check(ctx, csts, start, shots, din, grad, ncells):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["forward3d", "serving", "gradient2d"],
+    ap.add_argument("--workload", choices=["forward3d", "serving", "gradient2d", "gradient3d"],
                     default="forward3d")
+    ap.add_argument("--shots", type=int, default=8, help="gradient3d: shots")
+    ap.add_argument("--fp16", type=int, default=2,
+                    help="gradient3d: wavefield storage (2 FP16, 1 FP32 scaled, 0 FP32)")
+    ap.add_argument("--check-fp32", action="store_true",
+                    help="gradient3d: rerun shot 0 with FP32 storage and report the error")
     ap.add_argument("--nt", type=int, default=0, help="gradient2d: time steps (default 3000)")
     ap.add_argument("--fd-cells", type=int, default=0,
                     help="gradient2d: finite-difference check on this many cells per parameter")
@@ -277,7 +374,8 @@ def main():
                     help="forward3d: time steps (1500); gradient2d: whole gradients (2)")
     ap.add_argument("--warmup", type=int, default=None,
                     help="forward3d: time steps (10); gradient2d: whole gradients (1)")
-    ap.add_argument("--n", type=int, default=256, help="grid size per axis")
+    ap.add_argument("--n", type=int, default=0,
+                    help="grid size per axis (forward3d 256, gradient3d 512)")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
@@ -300,6 +398,9 @@ def main():
         args.warmup = 1 if grad else 10
     if grad:
         return gradient_main(args)
+    if args.workload == "gradient3d":
+        return gradient3d_main(args)
+    args.n = args.n or 256
 
     from seiscl_amd import Context, Model
     from seiscl_amd.workloads import config2

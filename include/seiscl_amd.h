@@ -174,11 +174,13 @@ int seis_grad_zero(seis_model *m);
 /* Device bytes the viscoelastic gradient may hold in checkpoints and the   */
 /* recomputed history (0, the default: 90% of the free device memory at    */
 /* seis_forward_save).  No counterpart in the reference, whose viscoelastic */
-/* adjoint backpropagates; seis_grad_plan reports the plan chosen: level-1 */
-/* spacing K1, level-2 spacing K2, level-1 checkpoints n1, sub-segments per */
-/* level-1 segment n2 (1: one level) and the bytes it holds.               */
+/* adjoint backpropagates.  seis_grad_plan reports the plan chosen: levels  */
+/* (1..4; each level costs one recomputation of the forward run), K[4] the  */
+/* spacing of each level's checkpoints (K[levels-1]: steps of recomputed    */
+/* history), n the ratio of consecutive spacings, n0 the checkpoints of the */
+/* forward run, and the bytes the plan holds.                               */
 int seis_grad_set_memory(seis_model *m, size_t bytes);
-int seis_grad_plan(const seis_model *m, int *K1, int *K2, int *n1, int *n2,
+int seis_grad_plan(const seis_model *m, int *levels, int *K, int *n, int *n0,
                    size_t *bytes);
 /* Bind a shot and run it forward with boundary saving; out as             */
 /* seis_forward_shot (may be NULL).                                          */
@@ -234,8 +236,9 @@ int seis_model_set_profiling(seis_model *m, int on);
 
 /* Last timed loop: device milliseconds spent in the update kernels and in  */
 /* the whole step loop (HIP events on the compute stream).  After            */
-/* seis_run_steps: the forward update_v / update_s; after seis_adjoint of an */
-/* elastic model: its fused adjoint velocity / stress kernels.  The kernel   */
+/* seis_run_steps: the forward update_v / update_s; after seis_adjoint: its  */
+/* adjoint velocity / stress kernels (the whole adjoint loop includes the    */
+/* recomputations of a viscoelastic model).  The kernel                      */
 /* times need seis_model_set_profiling(m, 1) before the loop.                */
 int seis_last_timing(const seis_model *m, double *ms_update_v,
                      double *ms_update_s, double *ms_loop);

@@ -144,12 +144,11 @@ struct seis_model {
         // viscoelastic runs: full-state checkpoints every K steps and the
         // recomputed post-source history of one segment of K steps
         bool visco = false;
-        int plan[4] = {0, 0, 0, 0};  // K1, K2, n1, n2 of the checkpoint plan
+        CkptPlan plan;              // checkpoint plan of the last forward run
         float *ckpt = nullptr;      // [n1 + n2 - 1][state]
         float *hist = nullptr;      // [K2][nvb][nfield] of the storage type
         size_t ckpt_bytes = 0, hist_bytes = 0;
         size_t mem_limit = 0;       // budget of ckpt + hist; 0: 90% of free memory
-        size_t plan_bytes = 0;      // checkpoints + history of the plan
         float *Ms = nullptr, *mus = nullptr;   // moduli of the gradient formulas
     } grad;
 
@@ -1153,13 +1152,14 @@ int seis_grad_set_memory(seis_model *m, size_t bytes) {
     return 0;
 }
 
-int seis_grad_plan(const seis_model *m, int *K1, int *K2, int *n1, int *n2, size_t *bytes) {
-    const int *p = m->grad.plan;
-    if (K1) *K1 = p[0];
-    if (K2) *K2 = p[1];
-    if (n1) *n1 = p[2];
-    if (n2) *n2 = p[3];
-    if (bytes) *bytes = m->grad.plan_bytes;
+int seis_grad_plan(const seis_model *m, int *levels, int *K, int *n, int *n0, size_t *bytes) {
+    const CkptPlan &p = m->grad.plan;
+    if (levels) *levels = p.L;
+    if (K)
+        for (int l = 0; l < kMaxLevels; ++l) K[l] = l < p.L ? p.K[l] : 0;
+    if (n) *n = p.n;
+    if (n0) *n0 = p.n0;
+    if (bytes) *bytes = p.bytes;
     return 0;
 }
 
@@ -1197,42 +1197,44 @@ int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes, bool exa
     return 0;
 }
 
-// Checkpoint plan of a viscoelastic gradient over T steps: level-1
-// checkpoints of the full state (S bytes) every K1 steps, taken by the
-// forward run; the adjoint recomputes each level-1 segment once, keeping
-// level-2 checkpoints every K2 steps (n2 - 1 of them: the last sub-segment
-// starts from the state the recomputation ends on), then recomputes each
-// sub-segment keeping its K2 post-source states (H bytes each) and runs the
-// adjoint steps backwards over them.  One level (K1 = K2, n2 = 1) costs one
-// recomputation of the forward run, two levels cost two; the plan is the
-// one-level plan of least memory if it fits the budget B, else the two-level
-// plan of least memory.
-struct CkptPlan {
-    int K1 = 0, K2 = 0, n1 = 0, n2 = 0;
-    size_t bytes = 0;
-};
+// Checkpoint plan of a viscoelastic gradient over T steps, with L levels.
+// The forward run keeps n0 = ceil(T / K[0]) checkpoints of the full state
+// (S bytes each), one every K[0] steps.  The adjoint treats the level-0
+// segments from the last to the first: it restores the segment's
+// checkpoint and hands it to level 1, which recomputes it once to keep a
+// checkpoint every K[1] = K[0] / n steps (n - 1 of them: the last
+// sub-segment starts from the state the recomputation ends on), and so on
+// down to level L-1, whose segments of K[L-1] steps are recomputed keeping
+// every post-source state (H bytes each) for the adjoint steps that run
+// backwards over them.  Memory: n0 S + (L-1)(n-1) S + K[L-1] H; work: one
+// forward run plus L recomputations of it.  The plan has the fewest levels
+// that fit the budget B, and of those the least memory.
 bool plan_checkpoints(int T, size_t S, size_t H, size_t B, CkptPlan &best) {
     best = CkptPlan{};
-    bool found = false;
-    for (int K = 1; K <= T; ++K) {
-        const int n1 = (T + K - 1) / K;
-        const size_t b = (size_t)n1 * S + (size_t)K * H;
-        if (b <= B && (!found || b < best.bytes)) {
-            best = CkptPlan{K, K, n1, 1, b};
-            found = true;
-        }
-    }
-    if (found) return true;
-    for (int K2 = 1; K2 < T; ++K2)
-        for (int n2 = 2; (long)n2 * K2 < 2L * T; ++n2) {
-            const int K1 = n2 * K2, n1 = (T + K1 - 1) / K1;
-            const size_t b = (size_t)n1 * S + (size_t)(n2 - 1) * S + (size_t)K2 * H;
-            if (b <= B && (!found || b < best.bytes)) {
-                best = CkptPlan{K1, K2, n1, n2, b};
-                found = true;
+    for (int L = 1; L <= kMaxLevels; ++L) {
+        bool found = false;
+        for (int Kl = 1; Kl <= T; ++Kl)
+            for (int n = L == 1 ? 1 : 2;; ++n) {
+                long K0 = Kl;
+                for (int l = 1; l < L; ++l) K0 *= n;
+                if (L > 1 && K0 >= 2L * T) break;
+                const int n0 = (int)((T + K0 - 1) / K0);
+                const size_t b = (size_t)n0 * S + (size_t)(L - 1) * (n - 1) * S + (size_t)Kl * H;
+                if (b <= B && (!found || b < best.bytes)) {
+                    best = CkptPlan{};
+                    best.L = L;
+                    best.n = n;
+                    best.n0 = n0;
+                    best.bytes = b;
+                    long k = K0;
+                    for (int l = 0; l < L; ++l, k /= n) best.K[l] = (int)k;
+                    found = true;
+                }
+                if (L == 1) break;
             }
-        }
-    return found;
+        if (found) return true;
+    }
+    return false;
 }
 
 size_t state_bytes(const std::vector<std::pair<float *, size_t>> &st) {
@@ -1297,20 +1299,18 @@ int forward_save_visco(seis_model *m, float *out) {
                       hbytes / 1e9);
         return fail(buf);
     }
-    const size_t ckb = (size_t)(P.n1 + P.n2 - 1) * sbytes, hib = (size_t)P.K2 * hbytes;
+    const size_t ckb = ((size_t)P.n0 + (size_t)(P.L - 1) * (P.n - 1)) * sbytes;
+    const size_t hib = (size_t)P.K[P.L - 1] * hbytes;
     if (G.ckpt_bytes + G.hist_bytes > budget) {   // a larger earlier plan: give it back
         if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, 0, true)) return 1;
         if (grad_buffer(m, &G.hist, &G.hist_bytes, 0, true)) return 1;
     }
     if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, ckb, false)) return 1;
     if (grad_buffer(m, &G.hist, &G.hist_bytes, hib, false)) return 1;
-    G.plan[0] = P.K1;
-    G.plan[1] = P.K2;
-    G.plan[2] = P.n1;
-    G.plan[3] = P.n2;
-    G.plan_bytes = P.bytes;
+    G.plan = P;
     for (int t = 0; t < tmax; ++t) {
-        if (t % P.K1 == 0 && state_save(state, (char *)G.ckpt + (size_t)(t / P.K1) * sbytes, st))
+        if (t % P.K[0] == 0 &&
+            state_save(state, (char *)G.ckpt + (size_t)(t / P.K[0]) * sbytes, st))
             return 1;
         step_sources(m, t, 1, st);
         step_v(m, st);
@@ -1324,63 +1324,119 @@ int forward_save_visco(seis_model *m, float *out) {
     return out ? seis_shot_read(m, out) : 0;
 }
 
-// Viscoelastic adjoint over the plan of forward_save_visco: level-1
-// segments from the last to the first; each is recomputed once to lay down
-// its level-2 checkpoints, then each sub-segment is recomputed keeping
-// every post-source state and its adjoint steps run backwards over them.
-int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial) {
-    auto &G = m->grad;
-    const seis_csts &c = m->c;
-    const int tmax = tmax_of(m), tmin = c.tmin, NT = c.NT, nout = (int)m->outn.size();
-    const int K1 = G.plan[0], K2 = G.plan[1], n1 = G.plan[2];
-    hipStream_t st = m->ctx->stream;
-    const auto state = forward_state(m);
-    const size_t sbytes = state_bytes(state);
-    const char *l1 = (const char *)G.ckpt, *l2 = l1 + (size_t)n1 * sbytes;
-    const auto hf = history_fields(m);
-    const size_t fb = (size_t)m->g.nfield * m->selem;     // bytes of one stored field
-    const bool half = m->prec == P_FP16;
-    A.Ms = G.Ms;
-    A.mus = G.mus;
-    for (int s1 = n1 - 1; s1 >= 0; --s1) {
-        const int t0 = s1 * K1, t1 = std::min(t0 + K1, tmax);
-        if (t1 - 1 <= tmin) break;
-        if (state_load(state, l1 + (size_t)s1 * sbytes, st)) return 1;
-        const int nsub = (t1 - t0 + K2 - 1) / K2;
-        // level 2: checkpoints at the sub-segment starts but the last
-        for (int t = t0; t < t0 + (nsub - 1) * K2; ++t) {
-            if ((t - t0) % K2 == 0 &&
-                state_save(state, (char *)l2 + (size_t)((t - t0) / K2) * sbytes, st))
+// Viscoelastic adjoint over the plan of forward_save_visco.
+struct ViscoAdj {
+    seis_model *m;
+    AdjArgs<float> *A;
+    std::vector<std::pair<float *, size_t>> state;
+    std::vector<float *> hf;    // forward fields the adjoint step reads
+    size_t sbytes, fb;          // bytes of a checkpoint, of one stored field
+    int tmin, serial;
+    bool half, prof;
+    size_t k = 0;               // adjoint steps done (event slots)
+
+    // level-l segment [t0, t1), whose starting state is current
+    int segment(int l, int t0, int t1) {
+        auto &G = m->grad;
+        const CkptPlan &P = G.plan;
+        hipStream_t st = m->ctx->stream;
+        if (l == P.L - 1) return innermost(t0, t1);
+        const int K = P.K[l + 1], nsub = (t1 - t0 + K - 1) / K;
+        char *slots = (char *)G.ckpt + ((size_t)P.n0 + (size_t)l * (P.n - 1)) * sbytes;
+        for (int t = t0; t < t0 + (nsub - 1) * K; ++t) {
+            if ((t - t0) % K == 0 && state_save(state, slots + (size_t)((t - t0) / K) * sbytes, st))
                 return 1;
             step_forward(m, t, st);
         }
-        for (int s2 = nsub - 1; s2 >= 0; --s2) {
-            const int u0 = t0 + s2 * K2, u1 = std::min(u0 + K2, t1);
+        for (int s = nsub - 1; s >= 0; --s) {
+            const int u0 = t0 + s * K, u1 = std::min(u0 + K, t1);
             if (u1 - 1 <= tmin) break;
-            if (s2 < nsub - 1 && state_load(state, l2 + (size_t)s2 * sbytes, st)) return 1;
-            for (int t = u0; t < u1; ++t) {
-                step_sources(m, t, 1, st);
-                char *dst = (char *)G.hist + (size_t)(t - u0) * G.nvb * fb;
-                for (size_t q = 0; q < hf.size(); ++q)
-                    HIP_TRY(hipMemcpyAsync(dst + q * fb, hf[q], fb, hipMemcpyDeviceToDevice, st));
-                step_v(m, st);
-                step_s(m, st);
-                if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
-            }
-            for (int t = u1 - 1; t >= std::max(u0, tmin + 1); --t) {
-                if (c.freesurf)
-                    launch_surface_adj(A.r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
-                launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT,
-                                 t, serial, st);
-                const char *h = (const char *)G.hist + (size_t)(t - u0) * G.nvb * fb;
-                for (size_t q = 0; q < hf.size(); ++q) A.st[q] = h + q * fb;
-                launch_adjh_s(A, m->nd_case, c.L, half, st);
-                launch_adjh_v(A, m->nd_case, half, st);
-            }
+            if (s < nsub - 1 && state_load(state, slots + (size_t)s * sbytes, st)) return 1;
+            if (segment(l + 1, u0, u1)) return 1;
         }
+        return 0;
     }
+
+    // recompute [t0, t1) keeping every post-source state, then the adjoint
+    // steps t1-1 .. max(t0, tmin+1) over them
+    int innermost(int t0, int t1) {
+        auto &G = m->grad;
+        const seis_csts &c = m->c;
+        hipStream_t st = m->ctx->stream;
+        const int NT = c.NT, nout = (int)m->outn.size();
+        for (int t = t0; t < t1; ++t) {
+            step_sources(m, t, 1, st);
+            char *dst = (char *)G.hist + (size_t)(t - t0) * G.nvb * fb;
+            for (size_t q = 0; q < hf.size(); ++q)
+                HIP_TRY(hipMemcpyAsync(dst + q * fb, hf[q], fb, hipMemcpyDeviceToDevice, st));
+            step_v(m, st);
+            step_s(m, st);
+            if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+        }
+        for (int t = t1 - 1; t >= std::max(t0, tmin + 1); --t) {
+            if (c.freesurf)
+                launch_surface_adj(A->r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
+            launch_residuals(A->r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t,
+                             serial, st);
+            const char *h = (const char *)G.hist + (size_t)(t - t0) * G.nvb * fb;
+            for (size_t q = 0; q < hf.size(); ++q) A->st[q] = h + q * fb;
+            if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
+            launch_adjh_s(*A, m->nd_case, c.L, half, st);
+            if (prof) HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
+            if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
+            launch_adjh_v(*A, m->nd_case, half, st);
+            if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
+            ++k;
+        }
+        return 0;
+    }
+};
+
+int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial) {
+    auto &G = m->grad;
+    const CkptPlan &P = G.plan;
+    const int tmax = tmax_of(m), tmin = m->c.tmin;
+    hipStream_t st = m->ctx->stream;
+    ViscoAdj V;
+    V.m = m;
+    V.A = &A;
+    V.state = forward_state(m);
+    V.hf = history_fields(m);
+    V.sbytes = state_bytes(V.state);
+    V.fb = (size_t)m->g.nfield * m->selem;
+    V.tmin = tmin;
+    V.serial = serial;
+    V.half = m->prec == P_FP16;
+    V.prof = m->profiling;
+    A.Ms = G.Ms;
+    A.mus = G.mus;
+    // with profiling on, events around the two adjoint kernels of every step
+    const size_t nst = (size_t)std::max(0, tmax - 1 - tmin);
+    hipEvent_t e0 = pool_event(m, 0), e1 = pool_event(m, 1);
+    if (!e0 || !e1 || (V.prof && !pool_event(m, 1 + 4 * nst)))
+        return fail("hipEventCreate failed");
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int s0 = P.n0 - 1; s0 >= 0; --s0) {
+        const int t0 = s0 * P.K[0], t1 = std::min(t0 + P.K[0], tmax);
+        if (t1 - 1 <= tmin) break;
+        if (state_load(V.state, (const char *)G.ckpt + (size_t)s0 * V.sbytes, st)) return 1;
+        if (V.segment(0, t0, t1)) return 1;
+    }
+    HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    m->ms_loop = ms;
+    m->ms_v = m->ms_s = 0;
+    if (V.prof)
+        for (size_t q = 0; q < V.k; ++q) {
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, m->evpool[2 + 4 * q], m->evpool[3 + 4 * q]));
+            HIP_TRY(hipEventElapsedTime(&b, m->evpool[4 + 4 * q], m->evpool[5 + 4 * q]));
+            m->ms_s += a;
+            m->ms_v += b;
+        }
     return 0;
 }
 

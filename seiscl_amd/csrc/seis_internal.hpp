@@ -183,6 +183,16 @@ void launch_batch_steps(const BatchSlot *dslots, int nslots, const BatchTable &t
 // ---------------------------------------------------------------------------
 // Gradient by backpropagation (BACK_PROP_TYPE=1).
 // ---------------------------------------------------------------------------
+// Checkpoint plan of a viscoelastic gradient (see plan_checkpoints): L
+// levels, spacing K[l] of the level-l checkpoints (K[L-1]: steps of
+// recomputed history), ratio n = K[l] / K[l+1], n0 level-0 checkpoints.
+constexpr int kMaxLevels = 4;
+struct CkptPlan {
+    int L = 0, n = 1, n0 = 0;
+    int K[kMaxLevels] = {0, 0, 0, 0};
+    size_t bytes = 0;
+};
+
 // Corrections of reference defects, selected by seis_csts.gradfix.
 enum GradFix {
     FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8, FIX_VISCO_F0 = 16

@@ -231,13 +231,13 @@ class Model:
 
     def grad_plan(self) -> dict:
         """Checkpoint plan of the last viscoelastic forward_save."""
-        v = [ctypes.c_int() for _ in range(4)]
+        lv, n, n0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        K = (ctypes.c_int * 4)()
         b = ctypes.c_size_t()
-        check(self.lib.seis_grad_plan(self.handle, *[ctypes.byref(x) for x in v],
-                                      ctypes.byref(b)))
-        d = dict(zip(("K1", "K2", "n1", "n2"), (x.value for x in v)))
-        d["bytes"] = b.value
-        return d
+        check(self.lib.seis_grad_plan(self.handle, ctypes.byref(lv), K, ctypes.byref(n),
+                                      ctypes.byref(n0), ctypes.byref(b)))
+        return dict(levels=lv.value, K=list(K)[:lv.value], n=n.value, n0=n0.value,
+                    bytes=b.value)
 
     def forward_save(self, src_pos, src, rec_pos):
         """Forward run that keeps the boundary shell of every step for the

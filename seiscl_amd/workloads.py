@@ -118,6 +118,50 @@ def config2(n: int = 256, NT: int = 1500, seed: int = 2, nrec_side: int = 64):
     return csts, params, [(src_pos, src, rec_pos)]
 
 
+def config5(n: int = 512, NT: int = 2500, nshots: int = 8, seed: int = 5,
+            nrec_side: int = 64):
+    """3D viscoelastic FWI gradient batch: n^3 (512), dh 10 m, dt 0.8 ms,
+    NT 2500, FD4, CPML 20, free surface, L=2 (FL 5 and 25 Hz, Q=50), model
+    as config2 (layered vp 1800-4500, vs = vp/1.8, rho 1800-2600, 3% smooth
+    perturbations at 100 m, drawn in float32 at this size), 8 explosive
+    12 Hz Ricker shots on a 4 x 2 grid at z = 10 cells, 64 x 64 surface
+    receivers.  FP16 wavefield storage is the caller's choice (csts FP16)."""
+    rng = np.random.default_rng(seed)
+    N = (n, n, n)
+    dh, dt = 10.0, 8e-4
+    vp = layered(N, 8, 1800.0, 4500.0, rng).astype(np.float32)
+    noise = rng.standard_normal(N, dtype=np.float32)
+    pert = gaussian_filter(noise, sigma=100.0 / dh, mode="nearest")
+    del noise
+    pert /= np.abs(pert).max()
+    vp *= 1 + np.float32(0.03) * pert
+    del pert
+    FL = [5.0, 25.0]
+    tau = np.float32(q_to_tau(50.0, 5.0, 25.0, FL))
+    params = dict(vp=vp, vs=vp / np.float32(1.8),
+                  rho=np.float32(1800) + np.float32(800 / 2700) * (vp - np.float32(1800)),
+                  taup=np.full(N, tau, np.float32), taus=np.full(N, tau, np.float32))
+    csts = dict(N=np.array(N), ND=3, dh=dh, dt=dt, NT=NT, FDORDER=4, MAXRELERROR=1,
+                abs_type=1, nab=20, freesurf=1, L=2, FL=np.array(FL), f0=12.0, seisout=1)
+    k = min(nrec_side, n // 3)
+    off = (n - 3 * (k - 1)) // 2
+    gi = (off + 3 * np.arange(k)) * dh
+    gx, gy = np.meshgrid(gi, gi, indexing="ij")
+    gx, gy = gx.ravel(), gy.ravel()
+    src = ricker(12.0, NT, dt)[:, None].astype(np.float32)
+    shots = []
+    xs = np.linspace(0.2, 0.8, 4) * n * dh
+    ys = np.linspace(0.35, 0.65, 2) * n * dh
+    for j, (x, y) in enumerate((x, y) for y in ys for x in xs):
+        if j >= nshots:
+            break
+        sp = np.array([[x], [y], [10 * dh], [float(j)], [100.0]])
+        rp = np.stack([gx, gy, 0 * gx + dh, 0 * gx + j, j * gx.size + np.arange(gx.size) + 1.0,
+                       0 * gx, 0 * gx, 0 * gx])
+        shots.append((sp, src, rp))
+    return csts, params, shots
+
+
 def layered_model(shape, rng, nlayers: int = 6, vmin: float = 1800.0, vmax: float = 4000.0,
                   elastic: bool = True):
     """Seeded layered model: vp from vmin to vmax with depth, vs = vp/1.8,

@@ -204,18 +204,21 @@ def test_visco_gradient_fp16_storage(gpu_ctx, case):
                                   dict(ND=3, FDORDER=4, L=2, abs_type=1, freesurf=1)], ids=_ids)
 def test_visco_checkpoint_plans(gpu_ctx, case):
     """The default plan is the one-level plan of least memory.  A budget one
-    byte smaller leaves no one-level plan, and the two-level plan (level-1
-    segments recomputed once more to lay down level-2 checkpoints) takes
-    over.  The recomputed states are the same, so the gradients equal the
-    default plan's bit for bit."""
+    byte smaller leaves no one-level plan and a two-level one takes over
+    (its segments recomputed once more to lay down a second level of
+    checkpoints); one byte below that, three levels.  The recomputed states
+    are the same, so the gradients equal the default plan's bit for bit."""
     _, g0, p0 = _gpu_grad(gpu_ctx, case)
-    assert p0["n2"] == 1 and p0["bytes"] > 0, p0
-    _, g2, p2 = _gpu_grad(gpu_ctx, case, mem=p0["bytes"] - 1)
-    print(f"[grad visco] plans {p0} {p2}", flush=True)
-    assert p2["n2"] > 1 and p2["bytes"] < p0["bytes"], p2
-    for k in g0:
-        assert np.abs(g0[k]).max() > 0, k
-        np.testing.assert_array_equal(g2[k], g0[k])
+    assert p0["levels"] == 1 and p0["bytes"] > 0, p0
+    plans = [p0]
+    for levels in (2, 3):
+        _, g, p = _gpu_grad(gpu_ctx, case, mem=plans[-1]["bytes"] - 1)
+        print(f"[grad visco] plan {p}", flush=True)
+        assert p["levels"] == levels and p["bytes"] < plans[-1]["bytes"], p
+        for k in g0:
+            assert np.abs(g0[k]).max() > 0, k
+            np.testing.assert_array_equal(g[k], g0[k])
+        plans.append(p)
 
 
 def test_visco_checkpoint_budget_error(gpu_ctx):
