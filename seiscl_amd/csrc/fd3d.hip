@@ -24,12 +24,20 @@
 namespace seis {
 namespace {
 
-#ifndef SEIS_BY
-#define SEIS_BY 4           // rows of y per block
+// rows of y per block of the velocity and of the stress update: the stress
+// update reads three y-derivatives and runs at 4 waves per SIMD either way,
+// so 16 rows (one 1024-lane block per CU) cut its y-halo re-reads; the
+// velocity update keeps 4 rows, the block size that lets it run 6 waves
+#ifndef SEIS_BY_V
+#define SEIS_BY_V 4
 #endif
-constexpr int BY3 = SEIS_BY;
+#ifndef SEIS_BY_S
+#define SEIS_BY_S 16
+#endif
+constexpr int BYV = SEIS_BY_V, BYS = SEIS_BY_S;
 
 // tile -> (bz, y slab, x) with z fastest, then x, then the y slab
+template <int BY>
 __device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int &y, int &x) {
     unsigned tile;
     xcd_tile(nb, tile);
@@ -39,7 +47,7 @@ __device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int 
     x = r % NX;
     const int by = r / NX;
     z = bz * BZ + threadIdx.x;
-    y = by * BY3 + threadIdx.y;
+    y = by * BY + threadIdx.y;
     return true;
 }
 
@@ -49,12 +57,12 @@ __device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int 
 constexpr int v3d_waves(int H, int rbytes) { return rbytes == 8 ? 2 : H <= 2 ? 6 : H <= 4 ? 4 : 2; }
 
 template <typename R, typename S, int H>
-__global__ __launch_bounds__(64 * BY3) __attribute__((amdgpu_waves_per_eu(v3d_waves(H, sizeof(R)), 8)))
+__global__ __launch_bounds__(64 * BYV) __attribute__((amdgpu_waves_per_eu(v3d_waves(H, sizeof(R)), 8)))
 void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3(nbz, g.NX, nb, z, y, x)) return;
+    if (!tile3<BYV>(nbz, g.NX, nb, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy;
@@ -140,11 +148,11 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
 }
 
 template <typename R, typename S, int H, int L, bool FS>
-__global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
+__global__ __launch_bounds__(64 * BYS) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3(nbz, g.NX, nb, z, y, x)) return;
+    if (!tile3<BYS>(nbz, g.NX, nb, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy, nf = g.nfield;
@@ -363,10 +371,10 @@ __global__ __launch_bounds__(64 * BY3) void k_s3d(FDArgs<R> a, int nbz, unsigned
     }
 }
 
-template <typename R>
+template <int BY>
 void grid3(const Layout &g, int &nbz, unsigned &nb) {
     nbz = (g.NZ + BZ - 1) / BZ;
-    const int nby = (g.NY + BY3 - 1) / BY3;
+    const int nby = (g.NY + BY - 1) / BY;
     nb = (unsigned)nbz * g.NX * nby;
 }
 
@@ -374,16 +382,16 @@ template <typename R, typename S, int H>
 void run_v3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
-    grid3<R>(a.g, nbz, nb);
-    hipLaunchKernelGGL((k_v3d<R, S, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BY3), 0, st, a, nbz, nb);
+    grid3<BYV>(a.g, nbz, nb);
+    hipLaunchKernelGGL((k_v3d<R, S, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BYV), 0, st, a, nbz, nb);
 }
 
 template <typename R, typename S, int H, int L>
 void run_s3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
-    grid3<R>(a.g, nbz, nb);
-    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, BY3);
+    grid3<BYS>(a.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, BYS);
     if (a.freesurf && a.fuse_surface)
         hipLaunchKernelGGL((k_s3d<R, S, H, L, true>), grid, blk, 0, st, a, nbz, nb);
     else
