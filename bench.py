@@ -112,7 +112,7 @@ def serving_main(args):
         "config": {
             "workload": (f"serving: {len(work)} seeded 2D acoustic/elastic modeling requests, "
                          "NZ 100-600, NX 200-1500, NT 1000-5000, FD 4/8, Poisson arrivals"),
-            "max_slots": args.slots,
+            "max_slots": args.slots, "cache_budget_mb": 192,
             "capacity_plain": cap_plain, "capacity_batched": cap_batch,
             "speedup_capacity": cap_batch["shots_per_hour"] / cap_plain["shots_per_hour"],
             "loads": loads,

@@ -168,7 +168,10 @@ struct BatchSlot {
 // every slot of a batch grouped by kernel class (acoustic or elastic, FD
 // order), with the prefix sum of their block counts; a block of a batch pass
 // updates kBatchTiles consecutive 64x4 tiles of one slot
-constexpr int kBatchTiles = 4;
+#ifndef SEIS_BATCH_TILES
+#define SEIS_BATCH_TILES 4
+#endif
+constexpr int kBatchTiles = SEIS_BATCH_TILES;
 struct BatchTable {
     int n;
     unsigned nblocks;

@@ -38,10 +38,9 @@ class Request:
         return int(np.prod(self.csts["N"])) * int(self.csts["NT"])
 
 
-def estimate_bytes(csts: dict) -> int:
-    """HBM a request needs (wavefields, coefficients, CPML memory, shot
-    buffers), from the sizes alone (the model's own footprint is known only
-    once it exists)."""
+def step_bytes(csts: dict) -> int:
+    """Bytes one time step of a request touches (wavefields, coefficients,
+    CPML memory): its share of the last-level cache while it is active."""
     NZ, NX = (int(v) for v in csts["N"])
     H = int(csts["FDORDER"]) // 2
     cells = (NZ + 2 * H + 64) * (NX + 2 * H)
@@ -49,7 +48,15 @@ def estimate_bytes(csts: dict) -> int:
     npar = 3 if csts["ND"] == 22 else 5
     nab = int(csts.get("nab", 16))
     psi = 8 * 2 * nab * (NZ + NX)
-    return 4 * (cells * (nf + npar) + psi) + 8 * int(csts["NT"]) * (NX // 2 + 8)
+    return 4 * (cells * (nf + npar) + psi)
+
+
+def estimate_bytes(csts: dict) -> int:
+    """HBM a request needs (wavefields, coefficients, CPML memory, shot
+    buffers), from the sizes alone (the model's own footprint is known only
+    once it exists)."""
+    NX = int(csts["N"][1])
+    return step_bytes(csts) + 8 * int(csts["NT"]) * (NX // 2 + 8)
 
 
 def poisson_arrivals(n: int, rate: float, seed: int = 40) -> np.ndarray:
@@ -77,13 +84,21 @@ def serve_plain(ctx: Context, requests):
     return requests
 
 
-def serve_batched(ctx: Context, requests, max_slots: int = 32, hbm_budget: float = 64e9,
-                  chunk: int = 64, prefetch: int = 4):
+def serve_batched(ctx: Context, requests, max_slots: int = 48, hbm_budget: float = 64e9,
+                  chunk: int = 64, prefetch: int = 4, cache_budget: float = 192e6,
+                  min_slots: int = 8):
     """Continuous batching.  A set-up thread builds each admitted request's
     model (host transforms, uploads on the context's transfer stream) while
     the GPU steps the batch; the main loop moves ready models into free
     slots, queues the next steps, and only then copies back the jobs that
-    finished in the previous steps, so the GPU always has work queued."""
+    finished in the previous steps, so the GPU always has work queued.
+
+    Admission to set-up is bounded by HBM (hbm_budget); activation in the
+    batch by the bytes a step touches: ready models join while the active
+    ones' step_bytes stay within cache_budget (three quarters of MI355X's
+    256 MB last-level cache, so that a batch step runs from that cache
+    rather than from HBM), but never fewer than min_slots are active while
+    models wait, nor more than max_slots."""
     import queue
     import threading
 
@@ -109,8 +124,10 @@ def serve_batched(ctx: Context, requests, max_slots: int = 32, hbm_budget: float
     worker.start()
     active = {}                 # slot -> (request, model, bytes)
     done_prev = []              # slots that ran their last step in the previous round
-    inflight = 0                # requests handed to the set-up thread
-    used = 0.0
+    inflight = 0                # requests handed to the set-up thread, not yet active
+    used = 0.0                  # HBM of admitted requests
+    hot = 0.0                   # step bytes of the active requests
+    held = None                 # a ready model waiting for cache room
     t0 = time.perf_counter()
     try:
         while pending or inflight or active:
@@ -126,13 +143,19 @@ def serve_batched(ctx: Context, requests, max_slots: int = 32, hbm_budget: float
                 inflight += 1
                 todo.put(r)
             while len(active) < max_slots:
-                try:
-                    r, m = ready.get_nowait()
-                except queue.Empty:
-                    break
-                inflight -= 1
+                if held is None:
+                    try:
+                        held = ready.get_nowait()
+                    except queue.Empty:
+                        break
+                r, m = held
                 if m is None:
                     raise errors[0]
+                if len(active) >= max(1, min_slots) and hot + step_bytes(r.csts) > cache_budget:
+                    break
+                held = None
+                inflight -= 1
+                hot += step_bytes(r.csts)
                 active[batch.add(m)] = (r, m, estimate_bytes(r.csts))
             nstep, min_left = batch.status()
             if nstep:
@@ -145,17 +168,19 @@ def serve_batched(ctx: Context, requests, max_slots: int = 32, hbm_budget: float
                 batch.remove(slot)
                 m.close()
                 used -= need
+                hot -= step_bytes(r.csts)
             done_prev = [s for s in active if batch.left(s) == 0]
             if not nstep and not done_prev:
                 # nothing on the GPU: wait for a model or the next arrival
-                if inflight:
-                    r, m = ready.get()
-                    ready.put((r, m))
+                if held is None and inflight:
+                    held = ready.get()
                 elif pending:
                     _wait_until(t0, pending[0].arrival)
     finally:
         todo.put(None)
         worker.join()
+        if held is not None and held[1] is not None:
+            held[1].close()
         while not ready.empty():
             _, m = ready.get()
             if m is not None:
