@@ -24,17 +24,21 @@
 namespace seis {
 namespace {
 
-// rows of y per block of the velocity and of the stress update: the stress
-// update reads three y-derivatives and runs at 4 waves per SIMD either way,
-// so 16 rows (one 1024-lane block per CU) cut its y-halo re-reads; the
-// velocity update keeps 4 rows, the block size that lets it run 6 waves
+// rows of y per block of the velocity and of the stress update.  The FP32
+// stress update reads three y-derivatives and runs at 4 waves per SIMD
+// either way, so 16 rows (one 1024-lane block per CU) cut its y-halo
+// re-reads; with FP16 storage or FP64 arithmetic 4 rows measure faster (in
+// FP64 a 1024-lane block would cap the kernel at 128 VGPRs and spill).  The
+// velocity update keeps 4 rows, the block size that lets it run 6 waves.
 #ifndef SEIS_BY_V
 #define SEIS_BY_V 4
 #endif
 #ifndef SEIS_BY_S
 #define SEIS_BY_S 16
 #endif
-constexpr int BYV = SEIS_BY_V, BYS = SEIS_BY_S;
+constexpr int BYV = SEIS_BY_V;
+template <typename R, typename S>
+constexpr int bys() { return sizeof(R) == 4 && sizeof(S) == 4 ? SEIS_BY_S : 4; }
 
 // tile -> (bz, y slab, x) with z fastest, then x, then the y slab
 template <int BY>
@@ -148,11 +152,11 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
 }
 
 template <typename R, typename S, int H, int L, bool FS>
-__global__ __launch_bounds__(64 * BYS) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
+__global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3<BYS>(nbz, g.NX, nb, z, y, x)) return;
+    if (!tile3<bys<R, S>()>(nbz, g.NX, nb, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy, nf = g.nfield;
@@ -390,8 +394,8 @@ template <typename R, typename S, int H, int L>
 void run_s3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
-    grid3<BYS>(a.g, nbz, nb);
-    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, BYS);
+    grid3<bys<R, S>()>(a.g, nbz, nb);
+    const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, bys<R, S>());
     if (a.freesurf && a.fuse_surface)
         hipLaunchKernelGGL((k_s3d<R, S, H, L, true>), grid, blk, 0, st, a, nbz, nb);
     else
