@@ -121,6 +121,59 @@ def serving_main(args):
     print(json.dumps(out), flush=True)
 
 
+def forward2d_main(args):
+    """2D elastic forward modeling, one shot: 200 x 400, dh 10 m, dt 1 ms,
+    NT 2000, FD4, CPML 20, vz Ricker 10 Hz, 400 receivers (the reference's
+    CPU-runnable configuration).  A step is one time step; the whole shot is
+    timed with the fields resident, and the oracle models the whole shot on
+    the host cores."""
+    from seiscl_amd import Context, Model
+    from seiscl_amd.workloads import config1
+    csts, params, shots = config1()
+    NT, ncell = int(csts["NT"]), int(np.prod(csts["N"]))
+    ctx = Context(0)
+    m = Model(ctx, params, **csts)
+    m.bind_shot(*shots[0])
+    m.run_steps(0, NT)                       # warm-up: one whole shot
+    m.bind_shot(*shots[0])
+    m.set_profiling(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    m.run_steps(0, NT)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    tm = m.timing()
+    m.close()
+    avg_v, avg_s = tm["update_v_ms"] / NT, tm["update_s_ms"] / NT
+    ach = 44 * ncell / (avg_s / 1e3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(ncell * NT / wall / 1e9, 4), "unit": "Gcell/s",
+        "n_gpus": 1, "steps": NT, "warmup": NT, "ms_per_step": round(wall / NT * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "2D elastic forward modeling, one shot, 200x400, FD4, CPML 20, "
+                               "NT 2000, 400 receivers", "grid": [200, 400], "NT": NT,
+                   "shot_s": round(wall, 4), "update_v_us": round(avg_v * 1e3, 3),
+                   "update_s_us": round(avg_s * 1e3, 3)},
+        "roofline": {"bound": "hbm", "kernel": "update_s (2D elastic stress update)",
+                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        oracle.forward(params, *shots[0], nthreads=threads, **csts)
+        wall_c = time.time() - t0
+        loop = oracle.last_loop_seconds()
+        out["cpu_baseline"] = dict(
+            value=ncell * NT / loop / 1e9, unit="Gcell/s", cores=threads, kind="port",
+            sample=(f"the whole shot ({NT} steps, loop {loop:.2f} s, {wall_c:.2f} s with "
+                    "set-up), C oracle with OpenMP"))
+    print(json.dumps(out), flush=True)
+
+
 BYTES_ADJ_S_2D = 108   # adjoint stress kernel, 2D elastic FP32, per cell
 BYTES_ADJ_V_2D = 80    # adjoint velocity kernel, 2D elastic FP32, per cell
 
@@ -357,7 +410,8 @@ def fd_check(ctx, csts, start, shots, din, grad, ncells):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["forward3d", "serving", "gradient2d", "gradient3d"],
+    ap.add_argument("--workload",
+                    choices=["forward3d", "forward2d", "serving", "gradient2d", "gradient3d"],
                     default="forward3d")
     ap.add_argument("--shots", type=int, default=8, help="gradient3d: shots")
     ap.add_argument("--fp16", type=int, default=2,
@@ -400,6 +454,8 @@ def main():
         return gradient_main(args)
     if args.workload == "gradient3d":
         return gradient3d_main(args)
+    if args.workload == "forward2d":
+        return forward2d_main(args)
     args.n = args.n or 256
 
     from seiscl_amd import Context, Model

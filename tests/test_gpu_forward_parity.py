@@ -74,3 +74,22 @@ def test_forward_bitwise(oracle_lib, gpu_ctx, case):
         np.testing.assert_array_equal(o, r)
     for k in fields:
         np.testing.assert_array_equal(gf[k], fields[k])
+
+
+def test_config1_full_size_bitwise(oracle_lib, gpu_ctx):
+    """The 2D elastic configuration at its full size (200 x 400, dh 10 m,
+    dt 1 ms, NT 2000, FD4, CPML 20, vz Ricker 10 Hz at (50 m, 2000 m), 400
+    receivers at z = 50 m, seeded +-5% smooth perturbations): seismograms
+    equal the oracle's bit for bit."""
+    from seiscl_amd import Model
+    from seiscl_amd.workloads import config1
+    csts, params, shots = config1()
+    print("[forward parity] config1 full size", flush=True)
+    ref = oracle_lib.forward(params, *shots[0], nthreads=16, **csts)
+    m = Model(gpu_ctx, params, **csts)
+    out = m.forward(*shots[0])
+    m.close()
+    assert len(out) == 2
+    for o, r in zip(out, ref):
+        assert np.abs(r).max() > 0
+        np.testing.assert_array_equal(o, r)
