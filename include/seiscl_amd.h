@@ -66,8 +66,22 @@ typedef struct seis_csts {
                         /* 16 viscoelastic gradients use the moduli at f0 for the */
                         /*    stress-to-strain conversion and the chain rule (the */
                         /*    reference uses the relaxed ones, src/update_adjs2D. */
-                        /*    cl:441-451, src/calc_grad.c:1020-1056).             */
-                        /* seis_csts_default sets 31; 0 reproduces the reference. */
+                        /*    cl:441-451, src/calc_grad.c:1020-1056);             */
+                        /* 64 normalised-misfit adjoint sources (scalermsnorm)    */
+                        /*    scale the trace by o.d, the derivative of           */
+                        /*    1/2 |o/|o| - d/|d||^2 (src/residuals.c:331-337 use  */
+                        /*    1 - o.d).                                           */
+                        /* seis_csts_default sets 95; 0 reproduces the reference. */
+    /* misfit options of var_res_raw (src/residuals.c:58-450), read by       */
+    /* seis_residuals and seis_gradient_shot:                                */
+    int restype;        /* 0 L2 residuals; 1 the data themselves (rtm_res,      */
+                        /*   src/residuals.c:24-55)                              */
+    int scalerms;       /* traces scaled by their rms, misfit normalised        */
+    int scaleshot;      /* traces scaled by the rms of the whole shot           */
+    int scalermsnorm;   /* normalised misfit without per-trace scaling          */
+    float fmin, fmax;   /* Butterworth band-pass (order 6, zero phase) of data  */
+                        /*   and seismograms before the misfit, 0: off          */
+                        /*   (src/butterworth.c:24-86)                          */
 } seis_csts;
 
 typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
@@ -187,11 +201,18 @@ int seis_grad_plan(const seis_model *m, int *levels, int *K, int *n, int *n0,
 int seis_forward_save(seis_model *m, int nsrc, const float *src_pos,
                       const float *src, int nrec, const float *rec_pos,
                       float *out);
-/* L2 residuals out - din for t < tmax, [nout][nrec][NT] (var_res_raw,      */
-/* src/residuals.c:58-450, without filters or rms scaling); rms = sum       */
-/* (din-out)^2, rmsnorm = sum din^2.  Host only.                             */
+/* Residuals (adjoint sources before res_scale) of a shot, [nout][nrec][NT]: */
+/* var_res_raw (src/residuals.c:58-450) with the model's misfit options --    */
+/* Butterworth band-pass (fmin, fmax), rms scalings (scalerms, scaleshot,     */
+/* scalermsnorm) -- or the data themselves with restype 1 (rtm_res,           */
+/* src/residuals.c:24-55); rms = sum (din-out)^2, rmsnorm = sum din^2 of the  */
+/* filtered and scaled traces.  Host only.                                    */
 int seis_residuals(const seis_model *m, int nrec, const float *out,
                    const float *din, float *res, double *rms, double *rmsnorm);
+/* The same for constants c and nout outputs, without a model (host only, no */
+/* device needed).                                                            */
+int seis_misfit(const seis_csts *c, int nout, int nrec, const float *out,
+                const float *din, float *res, double *rms, double *rmsnorm);
 /* Scale adjoint sources in place (res_scale, src/residuals.c:452-570).      */
 int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos,
                    float *res);

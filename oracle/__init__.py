@@ -21,6 +21,7 @@ DEFAULTS = dict(
     ND=2, dh=10.0, dt=0.0008, NT=875, L=0, f0=15.0, FL=(15.0,), FDORDER=8,
     MAXRELERROR=1, freesurf=0, abs_type=1, VPPML=3500.0, NPOWER=2.0, FPML=15.0,
     K_MAX_CPML=2.0, nab=16, abpc=6.0, seisout=2, param_type=0, tmax=0.0, tmin=0.0,
+    restype=0, scalerms=0, scaleshot=0, scalermsnorm=0, fmin=0.0, fmax=0.0,
 )
 
 
@@ -35,6 +36,8 @@ class OrCsts(ctypes.Structure):
         ("FPML", ctypes.c_float), ("K_MAX_CPML", ctypes.c_float),
         ("L", ctypes.c_int), ("f0", ctypes.c_float), ("FL", ctypes.c_float * MAXL),
         ("seisout", ctypes.c_int), ("param_type", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("restype", ctypes.c_int), ("scalerms", ctypes.c_int), ("scaleshot", ctypes.c_int),
+        ("scalermsnorm", ctypes.c_int), ("fmin", ctypes.c_float), ("fmax", ctypes.c_float),
     ]
 
 
@@ -105,15 +108,39 @@ def make_csts(nthreads=0, **kw) -> OrCsts:
     c.tmax = int(np.float32(d["tmax"]) / np.float32(c.dt)) if d["tmax"] else 0
     c.tmin = int(np.float32(d["tmin"]) / np.float32(c.dt)) if d["tmin"] else 0
     for k in ("FDORDER", "MAXRELERROR", "freesurf", "abs_type", "nab", "L", "seisout",
-              "param_type"):
+              "param_type", "restype", "scalerms", "scaleshot", "scalermsnorm"):
         setattr(c, k, int(d[k]))
-    for k in ("abpc", "VPPML", "NPOWER", "FPML", "K_MAX_CPML", "f0"):
+    for k in ("abpc", "VPPML", "NPOWER", "FPML", "K_MAX_CPML", "f0", "fmin", "fmax"):
         setattr(c, k, float(d[k]))
     fl = np.atleast_1d(np.asarray(d["FL"], dtype=np.float32))
     for i in range(min(len(fl), MAXL)):
         c.FL[i] = float(fl[i])
     c.nthreads = int(nthreads)
     return c
+
+
+def residuals(out, din, gradfix=None, **csts):
+    """Oracle residuals of one shot (var_res_raw with the misfit options, or
+    rtm_res): lists of (NT, nrec) arrays in, (res list, (rms, rmsnorm)) out."""
+    lib = load()
+    c = make_csts(**csts)
+    o = np.ascontiguousarray(np.stack([np.asarray(x, np.float32).T for x in out]))
+    d = np.ascontiguousarray(np.stack([np.asarray(x, np.float32).T for x in din]))
+    r = np.zeros_like(o)
+    rms = (ctypes.c_double * 2)()
+    lib.or_residuals(ctypes.byref(c), o.shape[0], o.shape[1], fp(o), fp(d), fp(r), rms,
+                     int(FIX_ALL if gradfix is None else gradfix))
+    return [np.ascontiguousarray(x.T) for x in r], (rms[0], rms[1])
+
+
+def butterworth(traces, fmin, fmax, dt, order=6, tmax=None):
+    """Oracle band-pass of (NT, ntrace) traces (a filtered copy)."""
+    lib = load()
+    a = np.ascontiguousarray(np.asarray(traces, np.float32).T).copy()
+    NT, ntr = a.shape[1], a.shape[0]
+    lib.or_butterworth(fp(a), ctypes.c_float(fmin), ctypes.c_float(fmax), ctypes.c_float(dt),
+                       NT, int(tmax or NT), ntr, int(order))
+    return np.ascontiguousarray(a.T)
 
 
 def to_c(a):
@@ -242,7 +269,8 @@ def check_stability(params: dict, **csts):
 
 
 FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE, FIX_SURF_SYY, FIX_VISCO_F0 = 1, 2, 4, 8, 16
-FIX_ALL = 31
+FIX_ALL = 95
+FIX_RMSNORM = 64
 HISTORY = 32    # elastic runs also take the forward state from the stored history
 
 

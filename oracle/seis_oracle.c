@@ -2244,19 +2244,145 @@ static void residuals_inject(sim *A, int nt, int nrec, const float *rec_pos, con
     }
 }
 
-/* L2 residuals (src/residuals.c:58-450, no filter, no rms scaling):        */
-/* res = out - in for t < tmax; rms += (in - out)^2, rmsnorm += in^2.       */
-static void residuals_l2(const or_csts *c, int nrec, const float *out, const float *din,
-                         float *res, double *rms) {
-    const int NT = c->NT, nout = or_nout(c), tmax = c->tmax > 0 ? c->tmax : c->NT;
+/* Butterworth band-pass (src/butterworth.c:24-86): each trace (t < tmax)  */
+/* is extended to nfft = the smallest 2,3,5-smooth length >= tmax by        */
+/* repeating its start, transformed, multiplied by                           */
+/* H(f) = (1 - sqrt(1/(1+(f/fcl)^2n))) sqrt(1/(1+(f/fch)^2n)) and            */
+/* transformed back.  Restated with a direct DFT in double (O(nfft^2)).     */
+void or_butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax, int ntrace,
+                    int order) {
+    int nfft = tmax < 1 ? 1 : tmax;
+    for (;; nfft++) {
+        int k = nfft;
+        while (k % 2 == 0) k /= 2;
+        while (k % 3 == 0) k /= 3;
+        while (k % 5 == 0) k /= 5;
+        if (k == 1) break;
+    }
+    float *H = malloc(sizeof(float) * (nfft / 2 + 1));
+    for (int t = 0; t < nfft / 2 + 1; t++) {
+        float freq = t / (dt * nfft);
+        H[t] = 1.;
+        if (fcl > 0) H[t] *= 1. - sqrt(1. / (1. + pow(freq / fcl, 2 * order)));
+        if (fch > 0) H[t] *= sqrt(1. / (1. + pow(freq / fch, 2 * order)));
+    }
+    double *cs = malloc(sizeof(double) * nfft), *sn = malloc(sizeof(double) * nfft);
+    for (int k = 0; k < nfft; k++) {
+        cs[k] = cos(2.0 * M_PI * k / nfft);
+        sn[k] = sin(2.0 * M_PI * k / nfft);
+    }
+    double *x = malloc(sizeof(double) * nfft), *Xr = malloc(sizeof(double) * nfft),
+           *Xi = malloc(sizeof(double) * nfft);
+    for (int n = 0; n < ntrace; n++) {
+        float *d = data + (long)n * NT;
+        for (int t = 0; t < nfft; t++) x[t] = t < tmax ? d[t] : d[t - tmax];
+        for (int k = 0; k < nfft; k++) {
+            double re = 0, im = 0;
+            for (int t = 0; t < nfft; t++) {
+                int q = (int)(((long)k * t) % nfft);
+                re += x[t] * cs[q];
+                im -= x[t] * sn[q];
+            }
+            double h = H[k <= nfft / 2 ? k : nfft - k];
+            Xr[k] = re * h;
+            Xi[k] = im * h;
+        }
+        for (int t = 0; t < tmax; t++) {
+            double re = 0;
+            for (int k = 0; k < nfft; k++) {
+                int q = (int)(((long)k * t) % nfft);
+                re += Xr[k] * cs[q] - Xi[k] * sn[q];
+            }
+            d[t] = (float)(re / nfft);
+        }
+    }
+    free(H); free(cs); free(sn); free(x); free(Xr); free(Xi);
+}
+
+/* var_res_raw (src/residuals.c:58-450, back_prop_type 1) and rtm_res       */
+/* (:24-55): band-pass (fmin/fmax), per-trace rms scalings over [tmin,     */
+/* tmax) in the reference's float accumulators, then residuals and rms for */
+/* t < tmax.  Arrays [nout][nrec][NT].                                       */
+void or_residuals(const or_csts *c, int nout, int nrec, const float *out, const float *din,
+                  float *res, double *rms, int gradfix) {
+    const int NT = c->NT, tmax = c->tmax > 0 ? c->tmax : c->NT, tmin = c->tmin;
+    const long n = (long)nout * nrec * NT;
+    if (c->restype == 1) {
+        for (long q = 0; q < n; q++)
+            if (q % NT < tmax) res[q] = din[q];
+        return;
+    }
+    float *vo = malloc(sizeof(float) * (n ? n : 1)), *vi = malloc(sizeof(float) * (n ? n : 1));
+    memcpy(vo, out, sizeof(float) * n);
+    memcpy(vi, din, sizeof(float) * n);
+    if (c->fmax > 0 || c->fmin > 0)
+        for (int o = 0; o < nout; o++) {
+            or_butterworth(vo + (long)o * nrec * NT, c->fmin, c->fmax, c->dt, NT, tmax, nrec, 6);
+            or_butterworth(vi + (long)o * nrec * NT, c->fmin, c->fmax, c->dt, NT, tmax, nrec, 6);
+        }
+#define AT(v, o, g, t) v[((long)(o) * nrec + (g)) * NT + (t)]
+    const int scaled = c->scalerms || c->scaleshot || c->scalermsnorm;
+    const int norm = c->scalerms || c->scalermsnorm;
+    float *rs = malloc(sizeof(float) * (nrec + 1)), *rs0 = malloc(sizeof(float) * (nrec + 1)),
+          *rn = malloc(sizeof(float) * (nrec + 1));
+    float ws = 0;
+    for (int g = 0; g < nrec; g++) rs[g] = rs0[g] = 1.0f, rn[g] = 0.0f;
+    if (scaled) {
+        for (int g = 0; g < nrec; g++) {
+            rs[g] = 0;
+            rs0[g] = 0;
+            for (int o = 0; o < nout; o++)
+                for (int t = tmin; t < tmax; t++) {
+                    rs[g] += pow(AT(vo, o, g, t), 2);
+                    rs0[g] += pow(AT(vi, o, g, t), 2);
+                }
+            rs[g] = sqrt(1. / rs[g]);
+            rs0[g] = sqrt(1. / rs0[g]);
+        }
+        if (c->scaleshot) {
+            ws = 0;
+            for (int g = 0; g < nrec; g++) ws += 1 / pow(rs0[g], 2);
+            ws = 1 / sqrt(ws);
+            for (int g = 0; g < nrec; g++) rs[g] = rs0[g] = ws;
+        }
+        if (c->scalerms) {
+            ws = 0;
+            for (int g = 0; g < nrec; g++) ws += 1 / pow(rs0[g], 2);
+            ws = 1 / sqrt(ws);
+            for (int g = 0; g < nrec; g++) {
+                rs[g] *= ws / rs0[g];
+                rs0[g] = ws;
+            }
+        }
+        for (int g = 0; g < nrec; g++)
+            for (int o = 0; o < nout; o++)
+                for (int t = tmin; t < tmax; t++) {
+                    AT(vo, o, g, t) *= rs[g];
+                    AT(vi, o, g, t) *= rs0[g];
+                }
+    }
+    if (norm)
+        for (int g = 0; g < nrec; g++) {
+            rn[g] = 0;
+            for (int o = 0; o < nout; o++)
+                for (int t = tmin; t < tmax; t++)
+                    rn[g] += AT(vo, o, g, t) * (AT(vo, o, g, t) - AT(vi, o, g, t));
+            if (c->scalerms) rn[g] *= pow(ws / rs0[g] / ws, 2);
+            rn[g] = 1.0 - rn[g];
+        }
+    const int fixnorm = (gradfix & OR_FIX_RMSNORM) && c->scalermsnorm && !c->scalerms;
     for (int g = 0; g < nrec; g++)
         for (int t = 0; t < tmax; t++)
             for (int o = 0; o < nout; o++) {
-                long q = ((long)o * nrec + g) * NT + t;
-                rms[0] += pow(-out[q] + din[q], 2);
-                rms[1] += pow(din[q], 2);
-                res[q] = 1.0f * (out[q] - din[q]);
+                const float a = AT(vo, o, g, t), b = AT(vi, o, g, t);
+                rms[0] += pow(-a + b, 2);
+                rms[1] += pow(b, 2);
+                float *r = &res[((long)o * nrec + g) * NT + t];
+                if (fixnorm) *r = rs[g] * (rn[g] * a - b);
+                else *r = norm ? rs[g] * ((1.0 - rn[g]) * a - b) : rs[g] * (a - b);
             }
+#undef AT
+    free(vo); free(vi); free(rs); free(rs0); free(rn);
 }
 
 /* res_scale (src/residuals.c:452-570), FP32 storage.  With at_receiver=0  */
@@ -2462,7 +2588,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     float *res = calloc(nres, sizeof(float));
     double rr[2] = {0, 0};
     if (res_in) memcpy(res, res_in, nres * sizeof(float));
-    else residuals_l2(c, nrec, out, din, res, rr);
+    else or_residuals(c, nout, nrec, out, din, res, rr, gradfix);
     res_scale(&S, nrec, rec_pos, res, gradfix & OR_FIX_RES_SCALE);
     if (res_out) memcpy(res_out, res, nres * sizeof(float));
     if (rms) {

@@ -30,6 +30,7 @@ extern "C" {
 /* checker only: elastic runs also read the forward state from the stored
  * history (to pin that path against backpropagation) */
 #define OR_HISTORY 32
+#define OR_FIX_RMSNORM 64
 
 typedef struct or_csts {
     int ND;            /* 2: 2D P-SV elastic, 3: 3D, 22: 2D acoustic           */
@@ -47,6 +48,10 @@ typedef struct or_csts {
     int seisout;       /* 1 velocities, 2 pressure, 3 stresses, 4 all          */
     int param_type;    /* 0: vp,vs,rho  1: M,mu,rho  2: Ip,Is,rho  3: complex v */
     int nthreads;      /* OpenMP threads (0 -> runtime default)                  */
+    /* misfit options (src/residuals.c:24-450)                                    */
+    int restype;       /* 1: residuals = data (rtm_res)                          */
+    int scalerms, scaleshot, scalermsnorm;
+    float fmin, fmax;  /* Butterworth band-pass of order 6, 0: off              */
 } or_csts;
 
 /* ---- setup pieces, exported individually so tests can pin them ---------- */
@@ -70,6 +75,17 @@ int  or_transform(const or_csts *c, const float *p0, const float *p1,
                   float *tausjpkp);
 int  or_check_stability(const or_csts *c, const float *M, const float *mu,
                         const float *rho, float *dtstable);
+
+/* Residuals of a shot, [nout][nrec][NT] (var_res_raw with its misfit       */
+/* options, or rtm_res): rms[0] = sum (in - out)^2, rms[1] = sum in^2.      */
+/* gradfix & OR_FIX_RMSNORM: normalised-misfit (scalermsnorm) sources      */
+/* weigh the trace by o.d, the derivative of 1/2 |o/|o| - d/|d||^2; the     */
+/* reference weighs it by 1 - o.d (src/residuals.c:331-337).                */
+void or_residuals(const or_csts *c, int nout, int nrec, const float *out, const float *din,
+                  float *res, double *rms, int gradfix);
+/* The band-pass alone, in place on ntrace traces data[n*NT + t], t < tmax.  */
+void or_butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax,
+                    int ntrace, int order);
 
 /* number of output traces per receiver for seisout, and their names */
 int  or_nout(const or_csts *c);

@@ -45,6 +45,12 @@ class SeisCsts(ctypes.Structure):
         ("gradout", ctypes.c_int),
         ("FP16", ctypes.c_int),
         ("gradfix", ctypes.c_int),
+        ("restype", ctypes.c_int),
+        ("scalerms", ctypes.c_int),
+        ("scaleshot", ctypes.c_int),
+        ("scalermsnorm", ctypes.c_int),
+        ("fmin", ctypes.c_float),
+        ("fmax", ctypes.c_float),
     ]
 
 
@@ -89,6 +95,8 @@ SIGNATURES = [
     ("seis_forward_save", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
     ("seis_residuals", _I, [_P, _I, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_double)]),
+    ("seis_misfit", _I, [ctypes.POINTER(SeisCsts), _I, _I, _FP, _FP, _FP,
+                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     ("seis_res_scale", _I, [_P, _I, _FP, _FP]),
     ("seis_adjoint", _I, [_P, _FP]),
     ("seis_grad_read", _I, [_P, _FP]),

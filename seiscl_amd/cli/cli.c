@@ -153,6 +153,13 @@ static int read_csts(hid_t f, job_t *j) {
     if (c->tmin > c->tmax) return err("tmax<tmin");
     if (c->tmax > c->NT) return err("tmax exceeds the NT*dt time window");
     c->gradout = j->gradout;
+    /* misfit options of var_res_raw / rtm_res (src/residuals.c:24-450) */
+    c->restype = j->restype;
+    c->scalerms = j->scalerms;
+    c->scaleshot = j->scaleshot;
+    c->scalermsnorm = j->scalermsnorm;
+    c->fmin = j->fmin;
+    c->fmax = j->fmax;
     /* precision: the reference's FP16=1 is vectorised FP32 (same results as
      * 0); 2 stores fields in half with FP32 arithmetic; 64 selects FP64 */
     switch (j->fp16_file) {
@@ -170,11 +177,8 @@ static int read_csts(hid_t f, job_t *j) {
     if (j->Hout) return err("Hessian output (Hout) is not available");
     if (j->gradsrcout) return err("source gradient output (gradsrcout) is not available");
     if (j->halfpar) return err("half-precision parameters (halfpar) are not available");
-    if (j->scalerms || j->scaleshot || j->scalermsnorm || j->restype)
-        return err("only the plain L2 misfit is available (scalerms, scaleshot, "
-                   "scalermsnorm and restype must be 0)");
-    if ((j->fmin > 0 || j->fmax > 0) && (j->gradout || j->rmsout || j->resout))
-        return err("residual filtering (fmin/fmax) is not available");
+    if (j->restype != 0 && j->restype != 1)
+        return err("restype=%d is not available (0: L2 residuals, 1: RTM)", j->restype);
     if (j->inputres && j->resout) return err("Cannot input and output residuals simultaneously");
     if (j->inputres && !j->gradout)
         return err("inputres without gradout (checkpoint files) is not available");

@@ -683,7 +683,8 @@ void seis_csts_default(seis_csts *c) {
     c->f0 = 15;
     c->FL[0] = 15;
     c->seisout = 2;
-    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY | FIX_VISCO_F0;
+    c->gradfix =
+        FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY | FIX_VISCO_F0 | FIX_RMSNORM;
 }
 
 int seis_ctx_create(int device_ordinal, seis_ctx **out) {
@@ -1494,9 +1495,18 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
     return out ? seis_shot_read(m, out) : 0;
 }
 
+int seis_misfit(const seis_csts *c, int nout, int nrec, const float *out, const float *din,
+                float *res, double *rms, double *rmsnorm) {
+    g_err.clear();
+    if (!c || nout < 0 || nrec < 0 || c->NT <= 0) return fail("seis_misfit: bad sizes");
+    const double r = residuals(*c, nout, nrec, out, din, res, rmsnorm);
+    if (rms) *rms = r;
+    return 0;
+}
+
 int seis_residuals(const seis_model *m, int nrec, const float *out, const float *din, float *res,
                    double *rms, double *rmsnorm) {
-    const double r = residuals_l2(m->c, (int)m->outn.size(), nrec, out, din, res, rmsnorm);
+    const double r = residuals(m->c, (int)m->outn.size(), nrec, out, din, res, rmsnorm);
     if (rms) *rms = r;
     return 0;
 }
@@ -1636,7 +1646,7 @@ int seis_gradient_shot(seis_model *m, int nsrc, const float *src_pos, const floa
     if (seis_forward_save(m, nsrc, src_pos, src, nrec, rec_pos, o.data())) return 1;
     double rr[2] = {0, 0};
     if (res_in) std::copy(res_in, res_in + res.size(), res.begin());
-    else rr[0] = residuals_l2(m->c, nout, nrec, o.data(), din, res.data(), &rr[1]);
+    else rr[0] = residuals(m->c, nout, nrec, o.data(), din, res.data(), &rr[1]);
     res_scale(m->c, m->rp, m->outn, nrec, rec_pos, res.data(), m->c.gradfix & FIX_RES_SCALE);
     if (seis_adjoint(m, res.data())) return 1;
     if (out) std::copy(o.begin(), o.end(), out);

@@ -2,8 +2,11 @@
 // residuals, their scaling and the final parameter transform of the
 // gradient.  Arithmetic follows the reference's C host (mixed float/double
 // exactly as written there).
+#include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
+#include <vector>
 
 #include "seis_internal.hpp"
 
@@ -32,22 +35,174 @@ Shell make_shell(const Layout &g, int nab, int freesurf) {
     return s;
 }
 
-// var_res_raw, L2 without filtering or rms scaling (src/residuals.c:58-450):
-// res = 1.0f * (out - in) for t < tmax; returns rms = sum (in - out)^2 and
-// rmsnorm = sum in^2 (pow in double).  Arrays are [nout][nrec][NT].
-double residuals_l2(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
-                    float *res, double *rmsnorm) {
-    const int NT = c.NT, tmax = c.tmax > 0 ? c.tmax : c.NT;
-    double rms = 0, rn = 0;
+// ---- misfit (var_res_raw and rtm_res, src/residuals.c:24-450) ------------
+
+// Smallest n >= m whose prime factors are 2, 3 and 5 (the FFT length the
+// reference's filter uses).
+int fft_fast_size(int m) {
+    for (int n = std::max(m, 1);; ++n) {
+        int k = n;
+        for (int p : {2, 3, 5})
+            while (k % p == 0) k /= p;
+        if (k == 1) return n;
+    }
+}
+
+namespace {
+
+using cplx = std::complex<double>;
+
+// Mixed-radix decimation-in-time DFT of x[0], x[s], ..., x[(n-1)s] into
+// out[0..n-1], for n with factors 2, 3, 5.  w holds exp(sign 2 pi i k / N)
+// for the top-level length N; this level's length n = N / ws.
+void fft_rec(const cplx *x, long s, int n, cplx *out, const std::vector<cplx> &w, long ws,
+             std::vector<cplx> &tmp) {
+    if (n == 1) {
+        out[0] = x[0];
+        return;
+    }
+    const int p = n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : 5;
+    const int m = n / p;
+    const long N = (long)w.size();
+    for (int r = 0; r < p; ++r) fft_rec(x + r * s, s * p, m, out + r * m, w, ws * p, tmp);
+    // X[k + q m] = sum_r W_n^{r k} W_p^{r q} Y_r[k]
+    cplx y[5];
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r) y[r] = out[r * m + k] * w[(long)r * k * ws % N];
+        for (int q = 0; q < p; ++q) {
+            cplx acc = 0;
+            for (int r = 0; r < p; ++r) acc += y[r] * w[(long)r * q * m * ws % N];
+            tmp[q * m + k] = acc;
+        }
+    }
+    for (int k = 0; k < n; ++k) out[k] = tmp[k];
+}
+
+void fft(std::vector<cplx> &a, int sign) {
+    const int n = (int)a.size();
+    std::vector<cplx> w(n), out(n), tmp(n);
+    for (int k = 0; k < n; ++k) w[k] = std::polar(1.0, sign * 2.0 * M_PI * k / n);
+    fft_rec(a.data(), 1, n, out.data(), w, 1, tmp);
+    a.swap(out);
+}
+
+}  // namespace
+
+// butterworth (src/butterworth.c:24-86): zero-phase band-pass of order
+// `order` applied in the frequency domain to each of ntrace traces of
+// data[n*NT + t], t < tmax, over nfft = fft_fast_size(tmax) samples whose
+// tail repeats the start of the trace.
+void butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax, int ntrace,
+                 int order) {
+    const int nfft = fft_fast_size(tmax);
+    std::vector<float> H(nfft / 2 + 1, 1.0f);
+    for (int t = 0; t < nfft / 2 + 1; ++t) {
+        const float freq = t / (dt * nfft);
+        if (fcl > 0) H[t] *= 1. - std::sqrt(1. / (1. + std::pow(freq / fcl, 2 * order)));
+        if (fch > 0) H[t] *= std::sqrt(1. / (1. + std::pow(freq / fch, 2 * order)));
+    }
+    std::vector<cplx> a(nfft);
+    for (int n = 0; n < ntrace; ++n) {
+        float *d = data + (size_t)n * NT;
+        for (int t = 0; t < nfft; ++t) a[t] = cplx(t < tmax ? d[t] : d[t - tmax], 0.0);
+        fft(a, -1);
+        for (int k = 0; k < nfft; ++k) a[k] *= (double)H[k <= nfft / 2 ? k : nfft - k];
+        fft(a, 1);
+        for (int t = 0; t < tmax; ++t) d[t] = (float)(a[t].real() / nfft);
+    }
+}
+
+// var_res_raw (src/residuals.c:58-450) with back_prop_type 1, or rtm_res
+// (:24-55) with restype 1.  out and din are [nout][nrec][NT]; the data are
+// first band-passed (fmin/fmax), then scaled by trace rms (scalerms,
+// scaleshot, scalermsnorm) over [tmin, tmax); res and the returned
+// rms = sum (in - out)^2, rmsnorm = sum in^2 follow for t < tmax, with the
+// reference's float accumulators and double pow.
+double residuals(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
+                 float *res, double *rmsnorm) {
+    const int NT = c.NT, tmax = c.tmax > 0 ? c.tmax : c.NT, tmin = c.tmin;
+    const size_t n = (size_t)nout * nrec * NT;
+    if (rmsnorm) *rmsnorm = 0;
+    if (c.restype == 1) {
+        for (int o = 0; o < nout; ++o)
+            for (int g = 0; g < nrec; ++g)
+                for (int t = 0; t < tmax; ++t) {
+                    const size_t q = ((size_t)o * nrec + g) * NT + t;
+                    res[q] = din[q];
+                }
+        return 0;
+    }
+    std::vector<float> vo(out, out + n), vi(din, din + n);
+    if (c.fmax > 0 || c.fmin > 0)
+        for (int o = 0; o < nout; ++o) {
+            butterworth(vo.data() + (size_t)o * nrec * NT, c.fmin, c.fmax, c.dt, NT, tmax, nrec, 6);
+            butterworth(vi.data() + (size_t)o * nrec * NT, c.fmin, c.fmax, c.dt, NT, tmax, nrec, 6);
+        }
+    auto at = [&](std::vector<float> &v, int o, int g, int t) -> float & {
+        return v[((size_t)o * nrec + g) * NT + t];
+    };
+    const bool scaled = c.scalerms || c.scaleshot || c.scalermsnorm;
+    const bool norm = c.scalerms || c.scalermsnorm;
+    std::vector<float> rs(nrec, 1.0f), rs0(nrec, 1.0f), rn(nrec, 0.0f);
+    float ws = 0;
+    if (scaled) {
+        for (int g = 0; g < nrec; ++g) {
+            float a = 0, b = 0;
+            for (int o = 0; o < nout; ++o)
+                for (int t = tmin; t < tmax; ++t) {
+                    a += std::pow(at(vo, o, g, t), 2);
+                    b += std::pow(at(vi, o, g, t), 2);
+                }
+            rs[g] = std::sqrt(1. / a);
+            rs0[g] = std::sqrt(1. / b);
+        }
+        if (c.scaleshot) {
+            ws = 0;
+            for (int g = 0; g < nrec; ++g) ws += 1 / std::pow(rs0[g], 2);
+            ws = 1 / std::sqrt((double)ws);   // C's double sqrt
+            for (int g = 0; g < nrec; ++g) rs[g] = rs0[g] = ws;
+        }
+        if (c.scalerms) {
+            ws = 0;
+            for (int g = 0; g < nrec; ++g) ws += 1 / std::pow(rs0[g], 2);
+            ws = 1 / std::sqrt((double)ws);   // C's double sqrt
+            for (int g = 0; g < nrec; ++g) {
+                rs[g] *= ws / rs0[g];
+                rs0[g] = ws;
+            }
+        }
+        for (int g = 0; g < nrec; ++g)
+            for (int o = 0; o < nout; ++o)
+                for (int t = tmin; t < tmax; ++t) {
+                    at(vo, o, g, t) *= rs[g];
+                    at(vi, o, g, t) *= rs0[g];
+                }
+    }
+    if (norm)
+        for (int g = 0; g < nrec; ++g) {
+            float a = 0;
+            for (int o = 0; o < nout; ++o)
+                for (int t = tmin; t < tmax; ++t)
+                    a += at(vo, o, g, t) * (at(vo, o, g, t) - at(vi, o, g, t));
+            if (c.scalerms) a *= std::pow(ws / rs0[g] / ws, 2);
+            rn[g] = 1.0 - a;
+        }
+    // normalised misfit: the reference weighs the trace by 1 - o.d
+    // (rn = o.d for unit traces); the derivative of 1/2 |o/|o| - d/|d||^2
+    // weighs it by o.d (FIX_RMSNORM)
+    const bool fixnorm = (c.gradfix & FIX_RMSNORM) && c.scalermsnorm && !c.scalerms;
+    double rms = 0, rnorm = 0;
     for (int g = 0; g < nrec; ++g)
         for (int t = 0; t < tmax; ++t)
             for (int o = 0; o < nout; ++o) {
-                const size_t q = ((size_t)o * nrec + g) * NT + t;
-                rms += std::pow((double)(-out[q] + din[q]), 2);
-                rn += std::pow((double)din[q], 2);
-                res[q] = 1.0f * (out[q] - din[q]);
+                const float vout = at(vo, o, g, t), vin = at(vi, o, g, t);
+                rms += std::pow(-vout + vin, 2);
+                rnorm += std::pow(vin, 2);
+                float &r = res[((size_t)o * nrec + g) * NT + t];
+                if (fixnorm) r = rs[g] * (rn[g] * vout - vin);
+                else r = norm ? rs[g] * ((1.0 - rn[g]) * vout - vin) : rs[g] * (vout - vin);
             }
-    if (rmsnorm) *rmsnorm = rn;
+    if (rmsnorm) *rmsnorm = rnorm;
     return rms;
 }
 

@@ -198,7 +198,8 @@ struct CkptPlan {
 
 // Corrections of reference defects, selected by seis_csts.gradfix.
 enum GradFix {
-    FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8, FIX_VISCO_F0 = 16
+    FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8, FIX_VISCO_F0 = 16,
+    FIX_RMSNORM = 64
 };
 
 // Boundary shell saved during the forward run and re-injected while the
@@ -346,8 +347,11 @@ void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const i
                       int nout, const float *res, int NT, int nt, int serial, void *stream);
 
 // Host side of the gradient (src/residuals.c, src/calc_grad.c:975-1110).
-double residuals_l2(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
-                    float *res, double *rmsnorm);
+double residuals(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
+                 float *res, double *rmsnorm);
+void butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax, int ntrace,
+                 int order);
+int fft_fast_size(int m);
 void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
                int nrec, const float *rec_pos, float *res, int at_receiver);
 void transf_grad(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,

@@ -22,7 +22,8 @@ CSTS_DEFAULTS = dict(
     ND=2, dh=10.0, dt=0.0008, NT=875, L=0, f0=15.0, FL=(15.0,), FDORDER=8,
     MAXRELERROR=1, FP16=0, freesurf=0, abs_type=1, VPPML=3500.0, NPOWER=2.0,
     FPML=15.0, K_MAX_CPML=2.0, nab=16, abpc=6.0, seisout=2, param_type=0,
-    tmax=0.0, tmin=0.0, gradout=0, gradfix=31,
+    tmax=0.0, tmin=0.0, gradout=0, gradfix=95, restype=0, scalerms=0, scaleshot=0,
+    scalermsnorm=0, fmin=0.0, fmax=0.0,
 )
 
 PARAM_NAMES = {0: ("vp", "vs", "rho"), 1: ("M", "mu", "rho"), 2: ("Ip", "Is", "rho")}
@@ -65,7 +66,32 @@ def make_csts(**kw) -> SeisCsts:
     c.gradout = int(d["gradout"])
     c.FP16 = int(d["FP16"])
     c.gradfix = int(d["gradfix"])
+    c.restype = int(d["restype"])
+    c.scalerms = int(d["scalerms"])
+    c.scaleshot = int(d["scaleshot"])
+    c.scalermsnorm = int(d["scalermsnorm"])
+    c.fmin = float(d["fmin"])
+    c.fmax = float(d["fmax"])
     return c
+
+
+def _pack_traces(lst) -> np.ndarray:
+    """list of (NT, nrec) arrays -> C array [nout][nrec][NT]."""
+    return np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.float32).T for x in lst]))
+
+
+def misfit(out, din, **csts):
+    """Residuals of a shot (var_res_raw / rtm_res with the misfit options in
+    csts: restype, scalerms, scaleshot, scalermsnorm, fmin, fmax): host
+    code of the library, no device needed.  out, din: lists of (NT, nrec)
+    arrays, one per output; returns (res list, (rms, rmsnorm))."""
+    c = make_csts(**csts)
+    o, d = _pack_traces(out), _pack_traces(din)
+    r = np.zeros_like(o)
+    rms, rn = ctypes.c_double(), ctypes.c_double()
+    check(_lib.load().seis_misfit(ctypes.byref(c), o.shape[0], o.shape[1], fptr(o), fptr(d),
+                                  fptr(r), ctypes.byref(rms), ctypes.byref(rn)))
+    return [np.ascontiguousarray(x.T) for x in r], (rms.value, rn.value)
 
 
 def to_c_layout(a: np.ndarray) -> np.ndarray:
@@ -257,6 +283,24 @@ class Model:
         if scale:
             check(self.lib.seis_res_scale(self.handle, rp.shape[0], fptr(rp), fptr(r)))
         check(self.lib.seis_adjoint(self.handle, fptr(r)))
+
+    def res_scale(self, res, rec_pos):
+        """res_scale (src/residuals.c:452-570) of adjoint sources res (list of
+        (NT, nrec)) for receivers rec_pos (8, nrec); returns the scaled list."""
+        rp = np.ascontiguousarray(np.asarray(rec_pos, np.float32).T)
+        r = self._pack(res, rp.shape[0])
+        check(self.lib.seis_res_scale(self.handle, rp.shape[0], fptr(rp), fptr(r)))
+        return [np.ascontiguousarray(x.T) for x in r]
+
+    def residuals(self, out, din):
+        """Residuals of the bound shot's outputs against din with the model's
+        misfit options (seis_residuals); returns (res list, (rms, rmsnorm))."""
+        o, d = _pack_traces(out), _pack_traces(din)
+        r = np.zeros_like(o)
+        rms, rn = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.seis_residuals(self.handle, o.shape[1], fptr(o), fptr(d), fptr(r),
+                                      ctypes.byref(rms), ctypes.byref(rn)))
+        return [np.ascontiguousarray(x.T) for x in r], (rms.value, rn.value)
 
     def grad_read(self) -> dict:
         """Accumulated gradients {p0, p1, rho} in the model layout."""

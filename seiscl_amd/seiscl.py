@@ -51,7 +51,8 @@ class SeisCL:
     # parallelisation and file keys)
     _ENGINE_KEYS = ("ND", "N", "dh", "dt", "NT", "L", "f0", "FL", "FDORDER", "MAXRELERROR",
                     "FP16", "freesurf", "abs_type", "VPPML", "NPOWER", "FPML", "K_MAX_CPML",
-                    "nab", "abpc", "seisout", "param_type", "tmax", "tmin")
+                    "nab", "abpc", "seisout", "param_type", "tmax", "tmin", "restype",
+                    "scalerms", "scaleshot", "scalermsnorm", "fmin", "fmax")
 
     def __init__(self, N=None, ND=2, dh=10, dt=0.0008, NT=875, L=0, f0=15,
                  FL=np.array(15), FDORDER=8, MAXRELERROR=1, FP16=0,
@@ -65,7 +66,7 @@ class SeisCL:
                  offmin=-float("Inf"), offmax=float("Inf"), inputres=0, restype=0,
                  scalerms=0, scalermsnorm=0, scaleshot=0, seisout=2, resout=0, rmsout=0,
                  movout=0, file="SeisCL", workdir="./seiscl", device: int = 0,
-                 gradfix: int = 31, files: bool = False, spawn: bool = True,
+                 gradfix: int = 95, files: bool = False, spawn: bool = True,
                  progname: str | None = None):
         self.N, self.ND, self.dh, self.dt, self.NT = N, ND, dh, dt, NT
         self.L, self.f0, self.FL = L, f0, FL
@@ -317,32 +318,36 @@ class SeisCL:
         res = {n: np.zeros((int(self.NT), ntr), np.float32) for n in names}
         rms = rmsnorm = 0.0
         grad = self.gradout == 1
+        misfit = not self.inputres and (grad or self.rmsout or self.resout)
         if grad:
             if self.back_prop_type != 1:
                 m.close()
                 raise SeisCLError("only back_prop_type=1 (backpropagation) is available")
             m.grad_zero()
+        if misfit and (self._din is None or any(n not in self._din for n in names)):
+            m.close()
+            raise SeisCLError(f"observed data must hold every output of {names}: "
+                              "call write_data first")
         for sid in sorted(set(self.src_pos[3, :].astype(int))):
             scol = self.src_pos[3, :].astype(int) == sid
             rcol = np.where(self.rec_pos[3, :].astype(int) == sid)[0]
             shot = (self.src_pos[:, scol], self.src[:, scol], self.rec_pos[:, rcol])
-            if not grad:
-                o = m.forward(*shot)
-            elif self.inputres:
+            if grad and self.inputres:
                 o, _ = m.gradient(*shot, res=[r[:, rcol] for r in self._res_in])
             else:
-                if self._din is None:
-                    m.close()
-                    raise SeisCLError("no observed data: call write_data first")
-                if any(n not in self._din for n in names):
-                    m.close()
-                    raise SeisCLError(f"observed data must hold every output of {names}")
+                o = m.forward_save(*shot) if grad else m.forward(*shot)
+            if misfit:
+                # var_res_raw with the misfit options, then res_scale: the
+                # adjoint sources, which the reference also writes as <var>res
                 din = [self._din[n][:, rcol] for n in names]
-                o, (r1, r2) = m.gradient(*shot, din=din)
+                r, (r1, r2) = m.residuals(o, din)
                 rms += r1
                 rmsnorm += r2
-                for n, a, b in zip(names, o, din):
-                    res[n][:, rcol] = a - b
+                r = m.res_scale(r, shot[2])
+                if grad:
+                    m.adjoint(r, scale=False)
+                for n, a in zip(names, r):
+                    res[n][:, rcol] = a
             for n, a in zip(names, o):
                 out[n][:, rcol] = a
         self._results = {"out": out, "res": res, "rms": (rms, rmsnorm)}

@@ -99,11 +99,10 @@ def test_cli_input_errors(tmp_path, capfd):
     # features outside this engine are refused, not ignored
     for key, val, msg in [("movout", 1, "movie output"), ("Hout", 1, "Hessian"),
                           ("gradsrcout", 1, "source gradient"), ("FP16", 3, "half arithmetic"),
-                          ("scalerms", 1, "plain L2"), ("back_prop_type", 2, "back_prop_type=2"),
-                          ("fmax", 20.0, "filtering")]:
+                          ("restype", 2, "restype=2"), ("back_prop_type", 2, "back_prop_type=2")]:
         old = getattr(seis, key)
         setattr(seis, key, val)
-        if key in ("back_prop_type", "fmax"):
+        if key in ("back_prop_type",):
             seis.gradout = 1
         seis.write_csts()
         _fails(capfd, [base], msg)

@@ -111,8 +111,15 @@ def test_inputres_files(oracle_lib, tmp_path):
         assert rel_l2(g, _cropped(tot[name], seis.nab)) <= 1e-5, name
 
 
-def test_resout_files(oracle_lib, tmp_path):
-    seis, model, true, csts = _fwi_case(tmp_path, resout=1, rmsout=1)
+MISFITS = [dict(), dict(scalermsnorm=1), dict(scaleshot=1, fmin=4.0, fmax=30.0),
+           dict(scalerms=1), dict(restype=1)]
+
+
+@pytest.mark.parametrize("misfit", MISFITS,
+                         ids=lambda o: "-".join(f"{k}{v}" for k, v in o.items()) or "L2")
+def test_resout_files(oracle_lib, tmp_path, misfit):
+    seis, model, true, csts = _fwi_case(tmp_path, resout=1, rmsout=1, **misfit)
+    csts.update(misfit)
     seis.prepare_data([0, 1])
     dobs = []
     for rc, shot in _shots(seis):
@@ -121,7 +128,7 @@ def test_resout_files(oracle_lib, tmp_path):
     for rc, d in dobs:
         for f, x in zip(full, d):
             f[:, rc] = x
-    print("[cli] resout files", flush=True)
+    print(f"[cli] resout files {misfit}", flush=True)
     seis.set_forward([0, 1], model, withgrad=False)
     seis.write_data({"vx": full[0], "vz": full[1]})
     seis.execute()
