@@ -45,6 +45,11 @@ typedef struct {
     float *din, *res, *out, *outres;   /* [nout][allng][NT] */
     double rms, rmsnorm;
     float *grad;                         /* [3][Pi N] */
+    /* movie: the output variables that are wavefields, their snapshots
+     * [nmov][ns][NT/movout][Pi N] (z fastest) */
+    int nmov;
+    const char *movs[10];
+    float *mov;
 } job_t;
 
 static int err(const char *fmt, ...) {
@@ -173,7 +178,8 @@ static int read_csts(hid_t f, job_t *j) {
     if (j->gradout && j->back_prop_type != 1)
         return err("back_prop_type=%d is not available: gradients are computed by "
                    "backpropagation (back_prop_type=1)", j->back_prop_type);
-    if (j->movout > 0) return err("movie output (movout) is not available");
+    if (j->movout > 0 && j->gradout)
+        return err("movie output (movout) is available for forward runs (gradout=0)");
     if (j->Hout) return err("Hessian output (Hout) is not available");
     if (j->gradsrcout) return err("source gradient output (gradsrcout) is not available");
     if (j->halfpar) return err("half-precision parameters (halfpar) are not available");
@@ -397,6 +403,21 @@ static int write_outputs(const files_t *fs, job_t *j) {
     const seis_csts *c = &j->c;
     char name[64];
     int s = 0;
+    if (j->movout > 0 && j->nmov) {
+        /* mov<var>: [ns][NT/movout][NX][(NY)][NZ] (src/writehdf5.c:295-310) */
+        hid_t f = seis_h5_create(fs->movie);
+        if (f < 0) return err("Could not create %s", fs->movie);
+        const int nd = c->ND == 3 ? 3 : 2;
+        long long d[5] = {j->ns, c->NT / j->movout, 0, 0, 0};
+        for (int i = 0; i < nd; ++i) d[2 + nd - 1 - i] = c->N[i];
+        const long long per = (long long)j->ns * d[1] * n_cells(c);
+        for (int v = 0; v < j->nmov && !s; ++v) {
+            snprintf(name, sizeof(name), "mov%s", j->movs[v]);
+            s = write_f32(f, name, j->mov + v * per, nd + 2, d);
+        }
+        seis_h5_close(f);
+        if (s) return s;
+    }
     if (c->seisout || j->resout) {
         hid_t f = seis_h5_create(fs->dout);
         if (f < 0) return err("Could not open the input/output file %s", fs->dout);
@@ -439,6 +460,63 @@ static int write_outputs(const files_t *fs, job_t *j) {
         seis_h5_close(f);
     }
     return s;
+}
+
+/* ---- movie (movout, src/time_stepping.c:272-350) ---------------------------- */
+
+/* The output variables that are wavefields (not the elastic pressure, which
+ * the reference derives from the stresses and leaves out of the movie). */
+static void movie_vars(job_t *j) {
+    j->nmov = 0;
+    for (int k = 0; k < j->nout; ++k)
+        if (strcmp(j->outs[k], "p") != 0 || j->c.ND == 22) j->movs[j->nmov++] = j->outs[k];
+}
+
+/* Copy the interior of each movie variable into frame `frame` of shot sh. */
+static int movie_frame(seis_model *m, job_t *j, int sh, int frame, float *pad) {
+    const seis_csts *c = &j->c;
+    const int H = c->FDORDER / 2, nd = c->ND == 3 ? 3 : 2;
+    const int NZ = c->N[0], NY = nd == 3 ? c->N[1] : 1, NX = nd == 3 ? c->N[2] : c->N[1];
+    const long long n = n_cells(c), nf = c->NT / j->movout;
+    const long long pz = NZ + 2 * H, py = nd == 3 ? NY + 2 * H : 1, yo = nd == 3 ? H : 0;
+    for (int v = 0; v < j->nmov; ++v) {
+        if (seis_read_field(m, j->movs[v], pad)) return err("%s", seis_last_error());
+        float *dst = j->mov + (((long long)v * j->ns + sh) * nf + frame) * n;
+        for (int x = 0; x < NX; ++x)
+            for (int y = 0; y < NY; ++y)
+                for (int z = 0; z < NZ; ++z)
+                    dst[((long long)x * NY + y) * NZ + z] =
+                        pad[((x + H) * py + (y + yo)) * pz + z + H];
+    }
+    return 0;
+}
+
+/* One forward shot with snapshots after every step t with (t+1) % movout == 0. */
+static int forward_movie(seis_model *m, job_t *j, int sh, int nsrc, const float *sp,
+                         const float *src, int nrec, const float *rp, float *out) {
+    const seis_csts *c = &j->c;
+    const int H = c->FDORDER / 2, nd = c->ND == 3 ? 3 : 2;
+    long long np = 1;
+    for (int d = 0; d < nd; ++d) np *= c->N[d] + 2 * H;
+    float *pad = (float *)malloc(sizeof(float) * np);
+    if (!pad) return err("out of host memory");
+    int s = seis_shot_bind(m, nsrc, sp, src, nrec, rp) || seis_launch_init(m);
+    const int tmax = c->tmax > 0 ? c->tmax : c->NT;
+    int t = 0;
+    for (int f = 0; !s && f < c->NT / j->movout; ++f) {
+        const int t1 = (f + 1) * j->movout;          /* steps done at frame f */
+        if (t1 > tmax) break;
+        if (seis_run_steps(m, t, t1 - t)) s = 1;
+        t = t1;
+        if (!s && movie_frame(m, j, sh, f, pad)) {
+            free(pad);
+            return 1;
+        }
+    }
+    if (!s && t < tmax && seis_run_steps(m, t, tmax - t)) s = 1;
+    if (!s && seis_shot_read(m, out)) s = 1;
+    free(pad);
+    return s ? err("%s", seis_last_error()) : 0;
 }
 
 /* ---- the run -------------------------------------------------------------- */
@@ -502,6 +580,11 @@ static int run(job_t *j) {
                 j->rms += rr[0];
                 j->rmsnorm += rr[1];
             }
+        } else if (j->movout > 0) {
+            if (forward_movie(m, j, sh, nsrc, sp, src, nrec, rp, so)) {
+                s = 1;
+                break;
+            }
         } else if (seis_forward_shot(m, nsrc, sp, src, nrec, rp, so)) {
             s = err("%s", seis_last_error());
             break;
@@ -562,6 +645,7 @@ static void job_free(job_t *j) {
     free(j->out);
     free(j->outres);
     free(j->grad);
+    free(j->mov);
 }
 
 static int file_names(const char *base, const char *din, files_t *fs) {
@@ -658,6 +742,11 @@ int seis_cli_main(int argc, char **argv) {
     } else if (!s) {
         j.nout = out_set(j.c.ND, j.c.seisout, j.outs);
         if (!j.nout) s = err("seisout=%d has no outputs for ND=%d", j.c.seisout, j.c.ND);
+    }
+    if (!s && j.movout > 0) {
+        movie_vars(&j);
+        const size_t n = (size_t)j.nmov * j.ns * (j.c.NT / j.movout) * n_cells(&j.c);
+        if (n && !(j.mov = (float *)calloc(n, sizeof(float)))) s = err("out of host memory");
     }
     if (!s) {
         const size_t n = (size_t)j.nout * j.allng * j.c.NT;

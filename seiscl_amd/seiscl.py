@@ -319,6 +319,16 @@ class SeisCL:
         rms = rmsnorm = 0.0
         grad = self.gradout == 1
         misfit = not self.inputres and (grad or self.rmsout or self.resout)
+        if self.movout > 0 and grad:
+            m.close()
+            raise SeisCLError("movie output (movout) is available for forward runs")
+        nshot = len(set(self.src_pos[3, :].astype(int)))
+        nframe = int(self.NT) // int(self.movout) if self.movout > 0 else 0
+        # movie variables: output wavefields (the elastic pressure is derived
+        # from the stresses and left out, as in the reference)
+        mov = {n: np.zeros(tuple(int(v) for v in self.N) + (nframe, nshot), np.float32)
+               for n in names if nframe and (n != "p" or self.ND == 22)}
+        mov_shots = []
         if grad:
             if self.back_prop_type != 1:
                 m.close()
@@ -334,8 +344,13 @@ class SeisCL:
             shot = (self.src_pos[:, scol], self.src[:, scol], self.rec_pos[:, rcol])
             if grad and self.inputres:
                 o, _ = m.gradient(*shot, res=[r[:, rcol] for r in self._res_in])
+            elif grad:
+                o = m.forward_save(*shot)
+            elif self.movout > 0:
+                o = self._forward_movie(m, shot, mov, len(mov_shots))
+                mov_shots.append(sid)
             else:
-                o = m.forward_save(*shot) if grad else m.forward(*shot)
+                o = m.forward(*shot)
             if misfit:
                 # var_res_raw with the misfit options, then res_scale: the
                 # adjoint sources, which the reference also writes as <var>res
@@ -350,12 +365,30 @@ class SeisCL:
                     res[n][:, rcol] = a
             for n, a in zip(names, o):
                 out[n][:, rcol] = a
-        self._results = {"out": out, "res": res, "rms": (rms, rmsnorm)}
+        self._results = {"out": out, "res": res, "rms": (rms, rmsnorm), "mov": mov}
         if grad:
             g = m.grad_read()
             self._results["grad"] = {n: g[n] for n in g}
         m.close()
         return ""
+
+    def _forward_movie(self, m, shot, mov, k):
+        """Forward shot k with a snapshot of each movie variable after every
+        step t with (t + 1) % movout == 0 (src/time_stepping.c:272-350)."""
+        m.bind_shot(*shot)
+        H = int(self.FDORDER) // 2
+        inner = tuple(slice(H, -H) for _ in range(len(self.N)))
+        nt = int(self.NT)
+        t = 0
+        for f in range(nt // int(self.movout)):
+            t1 = (f + 1) * int(self.movout)
+            m.run_steps(t, t1 - t)
+            t = t1
+            for n, a in mov.items():
+                a[..., f, k] = m.read_field(n)[inner]
+        if t < nt:
+            m.run_steps(t, nt - t)
+        return m.read_shot()
 
     # ---- outputs (SeisCL.py:507-656) ---------------------------------------
     def read_data(self, workdir=None, filename=None, residuals=False):
@@ -384,6 +417,25 @@ class SeisCL:
             o[:, :self.nab] = 0
             o[:, -self.nab:] = 0
         return o
+
+    def read_movie(self, workdir=None, filename=None):
+        """Snapshots of the output wavefields every movout steps
+        (SeisCL/SeisCL.py:543-570): per variable of to_load_names that is a
+        wavefield, an array [NZ, (NY,) NX, NT // movout, nshots]."""
+        if self.files:
+            path = self._path(workdir, self.file_movout if filename is None else filename)
+            try:
+                have = set(h5mat.names(path))
+            except Exception as e:
+                raise SeisCLError("Could not read data") from e
+            out = [np.transpose(h5mat.read(path, "mov" + w)) for w in self.to_load_names
+                   if "mov" + w in have]
+        else:
+            mov = self._results.get("mov", {})
+            out = [mov[w] for w in self.to_load_names if w in mov]
+        if not out:
+            raise SeisCLError("Could not read movie: variables not found")
+        return out
 
     def read_grad(self, workdir=None, param_names=None, filename=None):
         if self.files:

@@ -102,7 +102,7 @@ def test_cli_input_errors(tmp_path, capfd):
                           ("restype", 2, "restype=2"), ("back_prop_type", 2, "back_prop_type=2")]:
         old = getattr(seis, key)
         setattr(seis, key, val)
-        if key in ("back_prop_type",):
+        if key in ("back_prop_type", "movout"):
             seis.gradout = 1
         seis.write_csts()
         _fails(capfd, [base], msg)

@@ -158,3 +158,38 @@ def test_forward_files_3d(oracle_lib, tmp_path, seisout):
     assert len(out) == len(ref)
     for o, r in zip(out, ref):
         assert rel_l2(o, r) <= 1e-6
+
+
+@pytest.mark.parametrize("files", [True, False], ids=["files", "in_process"])
+def test_movie(oracle_lib, tmp_path, files):
+    """movout (src/time_stepping.c:272-350, src/writehdf5.c:295-310): the
+    wavefields every movout steps, [NZ, NX, NT/movout, shots] after
+    read_movie's transpose; frames equal the oracle's fields after as many
+    steps, bit for bit."""
+    from seiscl_amd.seiscl import SeisCL
+    from seiscl_amd.workloads import layered_model
+    N, NT, mv = np.array([60, 90]), 200, 50
+    args = dict(N=N, ND=2, dh=10.0, dt=1e-3, NT=NT, FDORDER=4, nab=10, seisout=1, f0=15.0,
+                movout=mv)
+    if files:
+        args.update(files=True, spawn=False, workdir=str(tmp_path))
+    seis = SeisCL(**args)
+    seis.surface_acquisition_2d(dg=3, ds=25)
+    model = layered_model(tuple(N), np.random.default_rng(2), nlayers=3)
+    print(f"[cli] movie files={files}", flush=True)
+    seis.set_forward([0, 1], model, withgrad=False)
+    seis.execute()
+    movie = seis.read_movie()
+    assert [m.shape for m in movie] == [(60, 90, NT // mv, 2)] * 2
+    csts = dict(N=N, ND=2, dh=10.0, dt=1e-3, FDORDER=4, nab=10, seisout=1, f0=15.0)
+    for k, sid in enumerate((0, 1)):
+        sc = seis.src_pos[3].astype(int) == sid
+        rc = seis.rec_pos[3].astype(int) == sid
+        for f in (0, NT // mv - 1):
+            nt = (f + 1) * mv
+            _, fl = oracle_lib.forward(model, seis.src_pos[:, sc], seis.src[:nt, sc],
+                                       seis.rec_pos[:, rc], fields=True, NT=nt, **csts)
+            for name, mov in zip(("vx", "vz"), movie):
+                assert np.abs(fl[name]).max() > 0
+                np.testing.assert_array_equal(mov[..., f, k], fl[name][2:-2, 2:-2])
+    seis.close()
