@@ -70,8 +70,11 @@ typedef struct seis_csts {
                         /* 64 normalised-misfit adjoint sources (scalermsnorm)    */
                         /*    scale the trace by o.d, the derivative of           */
                         /*    1/2 |o/|o| - d/|d||^2 (src/residuals.c:331-337 use  */
-                        /*    1 - o.d).                                           */
-                        /* seis_csts_default sets 95; 0 reproduces the reference. */
+                        /*    1 - o.d);                                           */
+                        /* 128 the mu Hessian subtracts the Hessian's own M term  */
+                        /*    (the 2D reference subtracts the gradient's,         */
+                        /*    src/update_adjs2D.cl:456).                          */
+                        /* seis_csts_default sets 223; 0 reproduces the reference.*/
     /* misfit options of var_res_raw (src/residuals.c:58-450), read by       */
     /* seis_residuals and seis_gradient_shot:                                */
     int restype;        /* 0 L2 residuals; 1 the data themselves (rtm_res,      */
@@ -82,6 +85,8 @@ typedef struct seis_csts {
     float fmin, fmax;   /* Butterworth band-pass (order 6, zero phase) of data  */
                         /*   and seismograms before the misfit, 0: off          */
                         /*   (src/butterworth.c:24-86)                          */
+    int Hout;           /* 1: accumulate the approximate Hessian of M and mu    */
+                        /*   with the gradient (src/update_adjs2D.cl:453-456)   */
 } seis_csts;
 
 typedef struct seis_ctx seis_ctx;       /* one GPU: device, streams, allocator   */
@@ -223,6 +228,10 @@ int seis_adjoint(seis_model *m, const float *res);
 /* Gradients [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), in the */
 /* parameter layout (transf_grad, src/calc_grad.c:975-1110).                 */
 int seis_grad_read(seis_model *m, float *grad);
+/* The approximate Hessian accumulated with the gradient when Hout=1,       */
+/* [3][Pi N] in the same order and transform (H<name> of the gradient file,  */
+/* src/writehdf5.c:275-283, src/calc_grad.c:1028-1094).                      */
+int seis_hess_read(seis_model *m, float *hess);
 /* One shot of gradient: forward with boundary saving, residuals against    */
 /* din (or the given adjoint sources res_in, the inputres mode), res_scale, */
 /* adjoint.  out: forward seismograms (may be NULL); rms: [rms, rmsnorm].   */

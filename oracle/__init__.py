@@ -77,7 +77,7 @@ def load():
         lib.or_gradient.argtypes = ([ctypes.POINTER(OrCsts)] + [_FP] * 5 +
                                     [ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
                                      ctypes.c_int, _FP, _FP, _FP,
-                                     ctypes.POINTER(ctypes.c_double), _FP,
+                                     ctypes.POINTER(ctypes.c_double), _FP, _FP,
                                      ctypes.c_char_p, ctypes.c_int])
         lib.or_shell_check.restype = ctypes.c_int
         lib.or_shell_check.argtypes = [ctypes.POINTER(OrCsts), ctypes.POINTER(ctypes.c_long)]
@@ -269,8 +269,9 @@ def check_stability(params: dict, **csts):
 
 
 FIX_RES_SCALE, FIX_ADJ_CPML, FIX_RHO_SCALE, FIX_SURF_SYY, FIX_VISCO_F0 = 1, 2, 4, 8, 16
-FIX_ALL = 95
+FIX_ALL = 223
 FIX_RMSNORM = 64
+FIX_HESS = 128
 HISTORY = 32    # elastic runs also take the forward state from the stored history
 
 
@@ -311,11 +312,12 @@ def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FI
     nv = int(np.prod(pshape))
     nvars = lib.or_nvars(ctypes.byref(c))
     fo = np.zeros(nvars * nv, np.float32) if fields else None
+    hess = np.zeros(3 * n, np.float32) if csts.get("Hout") else None
     err = ctypes.create_string_buffer(256)
     st = lib.or_gradient(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
                          sp.shape[0], fp(sp), fp(s), nrec, fp(rp), fp(d), fp(r),
                          int(gradfix), fp(out), fp(res_out), fp(grad), rms, fp(fo),
-                         err, 256)
+                         fp(hess), err, 256)
     if st:
         raise RuntimeError("oracle: " + err.value.decode())
     names = {0: ("vp", "vs"), 1: ("M", "mu"), 2: ("Ip", "Is"), 3: ("vp", "vs")}[c.param_type]
@@ -324,6 +326,9 @@ def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FI
         g[name] = np.ascontiguousarray(grad[q * n:(q + 1) * n].reshape(shape[::-1]).T)
     result = dict(out=[np.ascontiguousarray(o.T) for o in out], grad=g,
                   res=[np.ascontiguousarray(o.T) for o in res_out], rms=(rms[0], rms[1]))
+    if hess is not None:
+        result["hess"] = {name: np.ascontiguousarray(hess[q * n:(q + 1) * n].reshape(shape[::-1]).T)
+                          for q, name in enumerate((names[0], names[1], "rho"))}
     if fields:
         fd = {}
         for v, name in enumerate(var_names(c)):

@@ -2094,11 +2094,19 @@ static void grad_rho(const sim *S, const sim *A, double *grho) {
 /* lMs, lmus: the moduli of the stress-to-strain conversion (the relaxed   */
 /* ones in the reference; those at f0 for viscoelastic runs with           */
 /* OR_FIX_VISCO_F0).                                                       */
+/* M and mu gradients from the forward stresses S and the adjoint stress  */
+/* increments A->inc (src/update_adjs2D.cl:441-468, src/update_adjs3D.cl:   */
+/* 678-710).  restype 1 (RTM): the M image without the moduli, no mu term. */
+/* hM/hmu (Hout, may be NULL): the approximate Hessian, whose mu part       */
+/* subtracts the Hessian's own M term with hfix (the 2D reference           */
+/* subtracts the gradient's, :456; the 3D reference's expression does not   */
+/* compile, :694-697, and c5 is taken to weigh all three squares).          */
 static void grad_Mmu(const sim *S, const sim *A, const float *lMs, const float *lmus,
-                     double *gM, double *gmu) {
+                     double *gM, double *gmu, double *hM, double *hmu, int hfix) {
     const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
     const long nzi = NZ - 2 * FDOH, nyi = S->ND == 3 ? NY - 2 * FDOH : 1;
     const int y0 = S->ND == 3 ? FDOH : 0, y1 = S->ND == 3 ? NY - FDOH : 1;
+    const int rtm = S->c->restype == 1;
 #pragma omp parallel for schedule(static)
     for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
         for (int gidy = y0; gidy < y1; gidy++)
@@ -2109,18 +2117,36 @@ static void grad_Mmu(const sim *S, const sim *A, const float *lMs, const float *
                 if (S->ND == 2) {
                     float lsxx = A->inc[0][indv], lszz = A->inc[1][indv], lsxz = A->inc[2][indv];
                     float sxx = S->sxx[indv], szz = S->szz[indv], sxz = S->sxz[indv];
+                    if (rtm) {
+                        float dM = (sxx + szz) * (lsxx + lszz);
+                        gM[indp] += -dM;
+                        if (hM) hM[indp] += (float)((sxx + szz) * (sxx + szz));
+                        continue;
+                    }
                     float c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
                     float c3 = 1.0 / (lmu * lmu);
                     float c5 = 0.25 * c3;
                     float dM = c1 * (sxx + szz) * (lsxx + lszz);
                     gM[indp] += -dM;
                     gmu[indp] += -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+                    if (hM) {
+                        float dMH = c1 * (sxx + szz) * (sxx + szz);
+                        hM[indp] += dMH;
+                        hmu[indp] += (float)(c3 * sxz * sxz - (hfix ? dMH : dM) +
+                                             c5 * (sxx - szz) * (sxx - szz));
+                    }
                     continue;
                 }
                 float lsxx = A->inc[0][indv], lsyy = A->inc[1][indv], lszz = A->inc[2][indv];
                 float lsxy = A->inc[3][indv], lsyz = A->inc[4][indv], lsxz = A->inc[5][indv];
                 float sxx = S->sxx[indv], syy = S->syy[indv], szz = S->szz[indv];
                 float sxy = S->sxy[indv], syz = S->syz[indv], sxz = S->sxz[indv];
+                if (rtm) {
+                    float dM = (sxx + syy + szz) * (lsxx + lsyy + lszz);
+                    gM[indp] += -dM;
+                    if (hM) hM[indp] += (float)((sxx + syy + szz) * (sxx + syy + szz));
+                    continue;
+                }
                 float c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
                 float c3 = 1.0 / lmu / lmu;
                 float c5 = 1.0 / 6.0 * c3;
@@ -2129,6 +2155,14 @@ static void grad_Mmu(const sim *S, const sim *A, const float *lMs, const float *
                 gmu[indp] += -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
                              c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
                                    lszz * (2.0 * szz - sxx - syy));
+                if (hM) {
+                    float dMH = c1 * (sxx + syy + szz) * (sxx + syy + szz);
+                    hM[indp] += dMH;
+                    hmu[indp] += c3 * (sxz * sxz + sxy * sxy + syz * syz) - 4.0 / 3 * dMH +
+                                 c5 * ((2.0 * sxx - syy - szz) * (2.0 * sxx - syy - szz) +
+                                       (2.0 * syy - sxx - szz) * (2.0 * syy - sxx - szz) +
+                                       (2.0 * szz - sxx - syy) * (2.0 * szz - sxx - syy));
+                }
             }
 }
 
@@ -2460,6 +2494,34 @@ static void transf_grad(const sim *S, const float *Ms, const float *mus, const d
     }
 }
 
+/* The Hessian through the same transform (src/calc_grad.c:1028-1094), not */
+/* divided by dt; its density part holds only the chain-rule terms.         */
+static void transf_hess(const sim *S, const float *Ms, const float *mus, const double *hM,
+                        const double *hmu, float *out) {
+    const or_csts *c = S->c;
+    const long n = S->np;
+    for (long i = 0; i < n; i++) {
+        double rho = 1.0 / S->rho[i] * c->dt / c->dh;
+        double M = (double)Ms[i] * c->dh / c->dt;
+        double mu = (double)mus[i] * c->dh / c->dt;
+        double hr = 0, hm = hM[i], hu = hmu[i];
+        if (c->param_type == 0 || c->param_type == 2) {
+            hr = hr + M / rho * hm;
+            if (mu > 0) hr = hr + mu / rho * hu;
+            if (c->param_type == 0) {
+                hm = 2.0 * sqrt(rho * M) * hm;
+                hu = 2.0 * sqrt(rho * mu) * hu;
+            } else {
+                hm = 2.0 * sqrt(M / rho) * hm;
+                hu = 2.0 * sqrt(mu / rho) * hu;
+            }
+        }
+        out[i] = (float)hm;
+        out[n + i] = (float)hu;
+        out[2 * n + i] = (float)hr;
+    }
+}
+
 static int adj_alloc(sim *A, const sim *S, char *err, int errlen) {
     /* the adjoint run shares the model and the absorbing profiles; it has   */
     /* its own wavefields and its own zeroed CPML memory                      */
@@ -2490,7 +2552,8 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
                 const float *taup, const float *taus, int nsrc, const float *src_pos,
                 const float *src, int nrec, const float *rec_pos, const float *din,
                 const float *res_in, int gradfix, float *out, float *res_out,
-                float *grad, double *rms, float *fields_out, char *err, int errlen) {
+                float *grad, double *rms, float *fields_out, float *hess, char *err,
+                int errlen) {
     sim S, A;
     char dummy[8];
     if (!err) {
@@ -2645,6 +2708,8 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     }
     double *gM = calloc(S.np, sizeof(double)), *gmu = calloc(S.np, sizeof(double)),
            *grho = calloc(S.np, sizeof(double));
+    double *hM = hess ? calloc(S.np, sizeof(double)) : NULL,
+           *hmu = hess ? calloc(S.np, sizeof(double)) : NULL;
     for (int t = tmax - 1; t > tmin; t--) {
         if (c->freesurf) {
             if (S.ND == 3) surface_adj3D(&A, gradfix & OR_FIX_SURF_SYY);
@@ -2661,7 +2726,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
                        sizeof(float) * S.nv);
             if (S.ND == 3) update_s3D(&A);
             else update_s2D(&A);
-            grad_Mmu(&S, &A, Mg, mug, gM, gmu);
+            grad_Mmu(&S, &A, Mg, mug, gM, gmu, hM, hmu, gradfix & OR_FIX_HESS);
             if (S.ND == 3) update_v3D(&A);
             else update_v2D(&A);
             grad_rho(&S, &A, grho);
@@ -2674,7 +2739,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         inject_shell(&S, &B, bprev, S.ND == 3 ? 3 : 2, S.ND == 3 ? 6 : 3);
         if (S.ND == 3) update_s3D(&A);
         else update_s2D(&A);
-        grad_Mmu(&S, &A, Mg, mug, gM, gmu);
+        grad_Mmu(&S, &A, Mg, mug, gM, gmu, hM, hmu, gradfix & OR_FIX_HESS);
         /* update_adjv: the same for the velocities and the density */
         backprop_v(&S);
         inject_shell(&S, &B, bprev, 0, S.ND == 3 ? 3 : 2);
@@ -2684,6 +2749,9 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         sources(&S, t, -1, nsrc, src_pos, src);
     }
     transf_grad(&S, Mg, mug, gM, gmu, grho, grad, gradfix & OR_FIX_RHO_SCALE);
+    if (hess) transf_hess(&S, Mg, mug, hM, hmu, hess);
+    free(hM);
+    free(hmu);
     if (fields_out) {
         for (int v = 0; v < or_nvars(c); v++)
             memcpy(&fields_out[v * S.nv], named_var(&S, or_varname(c, v)),

@@ -31,6 +31,7 @@ extern "C" {
  * history (to pin that path against backpropagation) */
 #define OR_HISTORY 32
 #define OR_FIX_RMSNORM 64
+#define OR_FIX_HESS 128
 
 typedef struct or_csts {
     int ND;            /* 2: 2D P-SV elastic, 3: 3D, 22: 2D acoustic           */
@@ -138,14 +139,16 @@ int  or_shell_check(const or_csts *c, long *nbnd);
  * grad: [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), gradients
  * accumulated in double.  res_out (optional): the scaled adjoint sources;
  * rms (optional): [rms, rmsnorm]; fields_out (optional): the backpropagated
- * forward wavefields at the end, padded. */
+ * forward wavefields at the end, padded; hess (optional): the approximate
+ * Hessian [3][Pi N] transformed like grad (Hout).  restype 1 accumulates the
+ * RTM image of M instead of the gradient. */
 int  or_gradient(const or_csts *c, const float *p0, const float *p1,
                  const float *rho, const float *taup, const float *taus,
                  int nsrc, const float *src_pos, const float *src,
                  int nrec, const float *rec_pos, const float *din,
                  const float *res_in, int gradfix, float *out,
                  float *res_out, float *grad, double *rms, float *fields_out,
-                 char *err, int errlen);
+                 float *hess, char *err, int errlen);
 
 #ifdef __cplusplus
 }

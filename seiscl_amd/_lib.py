@@ -51,6 +51,7 @@ class SeisCsts(ctypes.Structure):
         ("scalermsnorm", ctypes.c_int),
         ("fmin", ctypes.c_float),
         ("fmax", ctypes.c_float),
+        ("Hout", ctypes.c_int),
     ]
 
 
@@ -86,6 +87,7 @@ SIGNATURES = [
     ("seis_launch_varsout", _I, [_P, _I]),
     ("seis_shot_read", _I, [_P, _FP]),
     ("seis_shot_read_f64", _I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    ("seis_hess_read", _I, [_P, _FP]),
     ("seis_grad_set_memory", _I, [_P, ctypes.c_size_t]),
     ("seis_grad_plan", _I, [_P] + [ctypes.POINTER(_I)] * 4 + [ctypes.POINTER(ctypes.c_size_t)]),
     ("seis_run_steps", _I, [_P, _I, _I]),

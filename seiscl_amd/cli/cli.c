@@ -44,7 +44,7 @@ typedef struct {
     const char *outs[10];
     float *din, *res, *out, *outres;   /* [nout][allng][NT] */
     double rms, rmsnorm;
-    float *grad;                         /* [3][Pi N] */
+    float *grad, *hess;                  /* [3][Pi N] */
     /* movie: the output variables that are wavefields, their snapshots
      * [nmov][ns][NT/movout][Pi N] (z fastest) */
     int nmov;
@@ -165,6 +165,7 @@ static int read_csts(hid_t f, job_t *j) {
     c->scalermsnorm = j->scalermsnorm;
     c->fmin = j->fmin;
     c->fmax = j->fmax;
+    c->Hout = j->Hout;
     /* precision: the reference's FP16=1 is vectorised FP32 (same results as
      * 0); 2 stores fields in half with FP32 arithmetic; 64 selects FP64 */
     switch (j->fp16_file) {
@@ -180,7 +181,6 @@ static int read_csts(hid_t f, job_t *j) {
                    "backpropagation (back_prop_type=1)", j->back_prop_type);
     if (j->movout > 0 && j->gradout)
         return err("movie output (movout) is available for forward runs (gradout=0)");
-    if (j->Hout) return err("Hessian output (Hout) is not available");
     if (j->gradsrcout) return err("source gradient output (gradsrcout) is not available");
     if (j->halfpar) return err("half-precision parameters (halfpar) are not available");
     if (j->restype != 0 && j->restype != 1)
@@ -456,6 +456,11 @@ static int write_outputs(const files_t *fs, job_t *j) {
             if (!par_used(c, k)) continue;
             snprintf(name, sizeof(name), "grad%s", par_name(c->param_type, k) + 1);
             s = write_f32(f, name, j->grad + (size_t)k * n, nd, d);
+            if (!s && j->hess) {
+                /* H<name> (src/writehdf5.c:275-283) */
+                snprintf(name, sizeof(name), "H%s", par_name(c->param_type, k) + 1);
+                s = write_f32(f, name, j->hess + (size_t)k * n, nd, d);
+            }
         }
         seis_h5_close(f);
     }
@@ -620,6 +625,11 @@ static int run(job_t *j) {
         j->grad = (float *)malloc(sizeof(float) * 3 * (size_t)n_cells(c));
         if (!j->grad) s = err("out of host memory");
         else if (seis_grad_read(m, j->grad)) s = err("%s", seis_last_error());
+        if (!s && j->Hout) {
+            j->hess = (float *)malloc(sizeof(float) * 3 * (size_t)n_cells(c));
+            if (!j->hess) s = err("out of host memory");
+            else if (seis_hess_read(m, j->hess)) s = err("%s", seis_last_error());
+        }
     }
 done:
     free(so);
@@ -645,6 +655,7 @@ static void job_free(job_t *j) {
     free(j->out);
     free(j->outres);
     free(j->grad);
+    free(j->hess);
     free(j->mov);
 }
 

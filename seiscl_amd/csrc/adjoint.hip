@@ -84,6 +84,71 @@ __device__ __forceinline__ bool adj_cell(const Layout &g, int nbz, unsigned nb, 
 // ---------------------------------------------------------------------------
 // 2D
 // ---------------------------------------------------------------------------
+// M and mu gradient increments of one cell from the forward stresses s*
+// and the adjoint stress increments l* (src/update_adjs2D.cl:441-468), plus
+// the approximate Hessian (Hout, :453-456) when A.hM is set; restype 1
+// accumulates the RTM image of M instead.  Every increment is taken to
+// physical units in double (factor 1 unless the fields are stored scaled).
+template <typename R>
+__device__ __forceinline__ void grad_mmu_2d(const AdjArgs<R> &A, long i, R sxx, R szz, R sxz,
+                                            R lsxx, R lszz, R lsxz, R lM, R lmu) {
+    if (A.rtm) {
+        const R dM = (sxx + szz) * (lsxx + lszz);
+        A.gM[i] += (double)-dM * A.gscale_s;
+        if (A.hM) A.hM[i] += (double)((sxx + szz) * (sxx + szz)) * A.hscale;
+        return;
+    }
+    const R c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
+    const R c3 = 1.0 / (lmu * lmu);
+    const R c5 = 0.25 * c3;
+    const R dM = c1 * (sxx + szz) * (lsxx + lszz);
+    const R dmu = -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+    A.gM[i] += (double)-dM * A.gscale_s;
+    A.gmu[i] += (double)dmu * A.gscale_s;
+    if (A.hM) {
+        const R dMH = c1 * (sxx + szz) * (sxx + szz);
+        const R hmu = c3 * sxz * sxz - (A.hfix ? dMH : dM) + c5 * (sxx - szz) * (sxx - szz);
+        A.hM[i] += (double)dMH * A.hscale;
+        A.hmu[i] += (double)hmu * A.hscale;
+    }
+}
+
+// The same in 3D (src/update_adjs3D.cl:678-710).  The reference's mu
+// Hessian expression does not compile (one parenthesis too many); here c5
+// weighs the sum of the three squared deviatoric stresses, and the M term is
+// the Hessian's own, as the 2D form with hfix.
+template <typename R>
+__device__ __forceinline__ void grad_mmu_3d(const AdjArgs<R> &A, long i, R sxx, R syy, R szz,
+                                            R sxy, R syz, R sxz, R lsxx, R lsyy, R lszz,
+                                            R lsxy, R lsyz, R lsxz, R lM, R lmu) {
+    if (A.rtm) {
+        const R dM = (sxx + syy + szz) * (lsxx + lsyy + lszz);
+        A.gM[i] += (double)-dM * A.gscale_s;
+        if (A.hM) A.hM[i] += (double)((sxx + syy + szz) * (sxx + syy + szz)) * A.hscale;
+        return;
+    }
+    const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
+    const R c3 = 1.0 / lmu / lmu;
+    const R c5 = 1.0 / 6.0 * c3;
+    const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
+    // evaluated in double from the 4.0 / 3 term on, as the reference's
+    // unsuffixed literals make it
+    const double dmu = -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
+                       c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
+                             lszz * (2.0 * szz - sxx - syy));
+    A.gM[i] += (double)-dM * A.gscale_s;
+    A.gmu[i] += dmu * A.gscale_s;
+    if (A.hM) {
+        const R dMH = c1 * (sxx + syy + szz) * (sxx + syy + szz);
+        const double hmu = c3 * (sxz * sxz + sxy * sxy + syz * syz) - 4.0 / 3 * dMH +
+                           c5 * ((2.0 * sxx - syy - szz) * (2.0 * sxx - syy - szz) +
+                                 (2.0 * syy - sxx - szz) * (2.0 * syy - sxx - szz) +
+                                 (2.0 * szz - sxx - syy) * (2.0 * szz - sxx - syy));
+        A.hM[i] += (double)dMH * A.hscale;
+        A.hmu[i] += hmu * A.hscale;
+    }
+}
+
 template <typename R, int H>
 __global__ __launch_bounds__(256) void k_adj_s2d(AdjArgs<R> A, int nbz, unsigned nb) {
     const FDArgs<R> &f = A.f, &a = A.r;
@@ -140,13 +205,8 @@ __global__ __launch_bounds__(256) void k_adj_s2d(AdjArgs<R> A, int nbz, unsigned
     a.sxx[i] = s3[0];
     a.szz[i] = s3[1];
     a.sxz[i] = s3[2];
-    // 4. M and mu gradients (src/update_adjs2D.cl:441-451)
-    const R c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
-    const R c3 = 1.0 / (lmu * lmu);
-    const R c5 = 0.25 * c3;
-    const R dM = c1 * (sxx + szz) * (lsxx + lszz);
-    A.gM[i] += -dM;
-    A.gmu[i] += -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+    // 4. M and mu gradients (src/update_adjs2D.cl:441-468)
+    grad_mmu_2d(A, i, sxx, szz, sxz, lsxx, lszz, lsxz, lM, lmu);
 }
 
 template <typename R, int H>
@@ -292,15 +352,8 @@ __global__ __launch_bounds__(256) void k_adj_s3d(AdjArgs<R> A, int nbz, unsigned
     a.sxx[i] = s6[3];
     a.syy[i] = s6[4];
     a.szz[i] = s6[5];
-    // M and mu gradients (src/update_adjs3D.cl:679-689)
-    const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
-    const R c3 = 1.0 / lmu / lmu;
-    const R c5 = 1.0 / 6.0 * c3;
-    const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
-    A.gM[i] += -dM;
-    A.gmu[i] += -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
-                c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
-                      lszz * (2.0 * szz - sxx - syy));
+    // M and mu gradients (src/update_adjs3D.cl:678-710)
+    grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz, lM, lmu);
 }
 
 template <typename R, int H>
@@ -453,16 +506,9 @@ __global__ __launch_bounds__(256) void k_adjh_s2d(AdjArgs<R> A, int nbz, unsigne
     a.sxx[i] = s3[0];
     a.szz[i] = s3[1];
     a.sxz[i] = s3[2];
-    // M and mu gradients (src/update_adjs2D.cl:441-451) from the forward state
+    // M and mu gradients (src/update_adjs2D.cl:441-468) from the forward state
     const R sxx = hst<R, S>(A, 2, i), szz = hst<R, S>(A, 3, i), sxz = hst<R, S>(A, 4, i);
-    const R lM = A.Ms[i], lmu = A.mus[i];
-    const R c1 = 1.0 / ((2.0 * lM - 2.0 * lmu) * (2.0 * lM - 2.0 * lmu));
-    const R c3 = 1.0 / (lmu * lmu);
-    const R c5 = 0.25 * c3;
-    const R dM = c1 * (sxx + szz) * (lsxx + lszz);
-    const R dmu = -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
-    A.gM[i] += (double)-dM * A.gscale_s;
-    A.gmu[i] += (double)dmu * A.gscale_s;
+    grad_mmu_2d(A, i, sxx, szz, sxz, lsxx, lszz, lsxz, A.Ms[i], A.mus[i]);
 }
 
 template <typename R, typename S, int H>
@@ -591,21 +637,11 @@ __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigne
     a.sxx[i] = s6[3];
     a.syy[i] = s6[4];
     a.szz[i] = s6[5];
-    // M and mu gradients (src/update_adjs3D.cl:679-689) from the forward state
+    // M and mu gradients (src/update_adjs3D.cl:678-710) from the forward state
     const R sxx = hst<R, S>(A, 3, i), syy = hst<R, S>(A, 4, i), szz = hst<R, S>(A, 5, i);
     const R sxy = hst<R, S>(A, 6, i), syz = hst<R, S>(A, 7, i), sxz = hst<R, S>(A, 8, i);
-    const R lM = A.Ms[i], lmu = A.mus[i];
-    const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
-    const R c3 = 1.0 / lmu / lmu;
-    const R c5 = 1.0 / 6.0 * c3;
-    const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
-    // evaluated in double from the 4.0 / 3 term on, as the reference's
-    // unsuffixed literals make it
-    const double dmu = -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
-                       c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
-                             lszz * (2.0 * szz - sxx - syy));
-    A.gM[i] += (double)-dM * A.gscale_s;
-    A.gmu[i] += dmu * A.gscale_s;
+    grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz, A.Ms[i],
+                A.mus[i]);
 }
 
 template <typename R, typename S, int H>

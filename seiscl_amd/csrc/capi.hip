@@ -136,6 +136,7 @@ struct seis_model {
         long bnd_steps = 0;
         FDArgs<float> r{};          // adjoint wavefields (same model)
         double *gM = nullptr, *gmu = nullptr, *grho = nullptr;   // padded layout
+        double *hM = nullptr, *hmu = nullptr;                    // Hessian (Hout)
         float *res = nullptr;       // [nout][nrec][NT] scaled adjoint sources
         float *stmp = nullptr;      // adjoint free surface: 3 * NX * NY
         size_t res_cap = 0;
@@ -555,6 +556,9 @@ int grad_setup(seis_model *m) {
     double **gd[] = {&G.gM, &G.gmu, &G.grho};
     for (double **p : gd)
         if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
+    if (c.Hout)
+        for (double **p : {&G.hM, &G.hmu})
+            if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
     G.ready = true;
     return 0;
 }
@@ -683,8 +687,8 @@ void seis_csts_default(seis_csts *c) {
     c->f0 = 15;
     c->FL[0] = 15;
     c->seisout = 2;
-    c->gradfix =
-        FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY | FIX_VISCO_F0 | FIX_RMSNORM;
+    c->gradfix = FIX_RES_SCALE | FIX_ADJ_CPML | FIX_RHO_SCALE | FIX_SURF_SYY | FIX_VISCO_F0 |
+                 FIX_RMSNORM | FIX_HESS;
 }
 
 int seis_ctx_create(int device_ordinal, seis_ctx **out) {
@@ -1171,6 +1175,8 @@ int seis_grad_zero(seis_model *m) {
     launch_fill(m->grad.gM, nb, m->ctx->stream);
     launch_fill(m->grad.gmu, nb, m->ctx->stream);
     launch_fill(m->grad.grho, nb, m->ctx->stream);
+    if (m->grad.hM) launch_fill(m->grad.hM, nb, m->ctx->stream);
+    if (m->grad.hmu) launch_fill(m->grad.hmu, nb, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     return 0;
@@ -1562,7 +1568,12 @@ int seis_adjoint(seis_model *m, const float *res) {
         // forward velocities times av, adjoint ones times q
         A.gscale_s = 1.0 / m->as;
         A.gscale_v = 1.0 / ((double)m->av * m->q);
+        A.hscale = 1.0 / ((double)m->as * m->as);
     }
+    A.rtm = m->c.restype == 1;
+    A.hM = G.hM;
+    A.hmu = G.hmu;
+    A.hfix = (m->c.gradfix & FIX_HESS) != 0;
     if (m->c.freesurf && (!A.r.M || !A.r.mu || !G.stmp))
         return fail("seis_adjoint: the adjoint free surface has no moduli on the device");
     const int serial = receivers_share_cells(m, m->rcell_host);
@@ -1613,15 +1624,13 @@ int seis_adjoint(seis_model *m, const float *res) {
     return 0;
 }
 
-int seis_grad_read(seis_model *m, float *grad) {
-    g_err.clear();
-    auto &G = m->grad;
-    if (!G.ready) return fail("seis_grad_read: no gradient accumulated");
+namespace {
+// Device accumulators (padded layout, double) -> parameter layout on the host.
+int accum_read(seis_model *m, const double *const *src, int k, std::vector<double> *acc) {
     const Layout &g = m->g;
     const size_t nf = (size_t)g.nfield, n = (size_t)g.NZ * g.NY * g.NX;
-    std::vector<double> tmp(nf), acc[3];
-    double *src[3] = {G.gM, G.gmu, G.grho};
-    for (int q = 0; q < 3; ++q) {
+    std::vector<double> tmp(nf);
+    for (int q = 0; q < k; ++q) {
         HIP_TRY(hipMemcpyAsync(tmp.data(), src[q], nf * 8, hipMemcpyDeviceToHost, m->ctx->stream));
         HIP_TRY(hipStreamSynchronize(m->ctx->stream));
         acc[q].resize(n);
@@ -1630,9 +1639,33 @@ int seis_grad_read(seis_model *m, float *grad) {
                 for (int z = 0; z < g.NZ; ++z)
                     acc[q][((size_t)x * g.NY + y) * g.NZ + z] = tmp[g.idx(z, y, x)];
     }
+    return 0;
+}
+}  // namespace
+
+int seis_grad_read(seis_model *m, float *grad) {
+    g_err.clear();
+    auto &G = m->grad;
+    if (!G.ready) return fail("seis_grad_read: no gradient accumulated");
+    std::vector<double> acc[3];
+    const double *src[3] = {G.gM, G.gmu, G.grho};
+    if (accum_read(m, src, 3, acc)) return 1;
     const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
     transf_grad(m->c, m->rp, f0 ? m->rp.Mf0 : m->rp.M, f0 ? m->rp.muf0 : m->rp.mu,
                 acc[0].data(), acc[1].data(), acc[2].data(), grad, m->c.gradfix & FIX_RHO_SCALE);
+    return 0;
+}
+
+int seis_hess_read(seis_model *m, float *hess) {
+    g_err.clear();
+    auto &G = m->grad;
+    if (!G.ready || !G.hM) return fail("seis_hess_read: no Hessian accumulated (set Hout=1)");
+    std::vector<double> acc[2];
+    const double *src[2] = {G.hM, G.hmu};
+    if (accum_read(m, src, 2, acc)) return 1;
+    const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
+    transf_hess(m->c, m->rp, f0 ? m->rp.Mf0 : m->rp.M, f0 ? m->rp.muf0 : m->rp.mu,
+                acc[0].data(), acc[1].data(), hess);
     return 0;
 }
 

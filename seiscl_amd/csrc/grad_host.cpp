@@ -283,4 +283,35 @@ void transf_grad(const seis_csts &c, const RefParams &p, const std::vector<float
     }
 }
 
+// The approximate Hessian through the same parameter transform
+// (src/calc_grad.c:1028-1094): not divided by dt; the density term holds
+// only the chain-rule contributions of M and mu (the reference's kernels
+// leave it at zero).
+void transf_hess(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,
+                 const std::vector<float> &mus, const double *hM, const double *hmu,
+                 float *out) {
+    const size_t n = Ms.size();
+    const bool elastic = !mus.empty();
+    for (size_t i = 0; i < n; ++i) {
+        const double rho = 1.0 / p.rho[i] * c.dt / c.dh;
+        const double M = (double)Ms[i] * c.dh / c.dt;
+        const double mu = elastic ? (double)mus[i] * c.dh / c.dt : 0.0;
+        double hr = 0, hm = hM[i], hu = elastic ? hmu[i] : 0.0;
+        if (c.param_type == 0 || c.param_type == 2) {
+            hr = hr + M / rho * hm;
+            if (mu > 0) hr = hr + mu / rho * hu;
+            if (c.param_type == 0) {
+                hm = 2.0 * std::sqrt(rho * M) * hm;
+                hu = 2.0 * std::sqrt(rho * mu) * hu;
+            } else {
+                hm = 2.0 * std::sqrt(M / rho) * hm;
+                hu = 2.0 * std::sqrt(mu / rho) * hu;
+            }
+        }
+        out[i] = (float)hm;
+        out[n + i] = (float)hu;
+        out[2 * n + i] = (float)hr;
+    }
+}
+
 }  // namespace seis

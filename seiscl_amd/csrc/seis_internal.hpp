@@ -199,7 +199,7 @@ struct CkptPlan {
 // Corrections of reference defects, selected by seis_csts.gradfix.
 enum GradFix {
     FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8, FIX_VISCO_F0 = 16,
-    FIX_RMSNORM = 64
+    FIX_RMSNORM = 64, FIX_HESS = 128
 };
 
 // Boundary shell saved during the forward run and re-injected while the
@@ -323,6 +323,15 @@ struct AdjArgs {
     // factors taking the accumulated M/mu and rho increments back to
     // physical units (1 unless the wavefields are stored scaled)
     double gscale_s = 1, gscale_v = 1;
+    // restype 1 (RTM): the M image without the moduli, no mu term
+    // (src/update_adjs2D.cl:459-468, src/update_adjs3D.cl:701-710)
+    int rtm = 0;
+    // approximate Hessian of M and mu (Hout), double, padded layout, or null;
+    // hscale takes it to physical units; hfix: mu's uses the Hessian's own
+    // M term (the reference's 2D kernel subtracts the gradient's)
+    double *hM = nullptr, *hmu = nullptr;
+    double hscale = 1;
+    int hfix = 1;
 };
 
 template <typename R>
@@ -354,6 +363,9 @@ void butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax, 
 int fft_fast_size(int m);
 void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
                int nrec, const float *rec_pos, float *res, int at_receiver);
+void transf_hess(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,
+                 const std::vector<float> &mus, const double *hM, const double *hmu,
+                 float *out);
 void transf_grad(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,
                  const std::vector<float> &mus, const double *gM, const double *gmu,
                  const double *grho, float *out, int rho_fix);

@@ -22,8 +22,8 @@ CSTS_DEFAULTS = dict(
     ND=2, dh=10.0, dt=0.0008, NT=875, L=0, f0=15.0, FL=(15.0,), FDORDER=8,
     MAXRELERROR=1, FP16=0, freesurf=0, abs_type=1, VPPML=3500.0, NPOWER=2.0,
     FPML=15.0, K_MAX_CPML=2.0, nab=16, abpc=6.0, seisout=2, param_type=0,
-    tmax=0.0, tmin=0.0, gradout=0, gradfix=95, restype=0, scalerms=0, scaleshot=0,
-    scalermsnorm=0, fmin=0.0, fmax=0.0,
+    tmax=0.0, tmin=0.0, gradout=0, gradfix=223, restype=0, scalerms=0, scaleshot=0,
+    scalermsnorm=0, fmin=0.0, fmax=0.0, Hout=0,
 )
 
 PARAM_NAMES = {0: ("vp", "vs", "rho"), 1: ("M", "mu", "rho"), 2: ("Ip", "Is", "rho")}
@@ -72,6 +72,7 @@ def make_csts(**kw) -> SeisCsts:
     c.scalermsnorm = int(d["scalermsnorm"])
     c.fmin = float(d["fmin"])
     c.fmax = float(d["fmax"])
+    c.Hout = int(d["Hout"])
     return c
 
 
@@ -309,6 +310,15 @@ class Model:
         check(self.lib.seis_grad_read(self.handle, fptr(g)))
         names = PARAM_NAMES.get(self.csts.param_type, ("vp", "vs", "rho"))
         return {nm: from_c_layout(g[q * n:(q + 1) * n], self.shape) for q, nm in enumerate(names)}
+
+    def hess_read(self) -> dict:
+        """Approximate Hessian {p0, p1, rho} accumulated with the gradient
+        (Hout=1), transformed like the gradient."""
+        n = int(np.prod(self.shape))
+        h = np.zeros(3 * n, dtype=np.float32)
+        check(self.lib.seis_hess_read(self.handle, fptr(h)))
+        names = PARAM_NAMES.get(self.csts.param_type, ("vp", "vs", "rho"))
+        return {nm: from_c_layout(h[q * n:(q + 1) * n], self.shape) for q, nm in enumerate(names)}
 
     def gradient(self, src_pos, src, rec_pos, din=None, res=None):
         """One shot of gradient (added to the accumulators): forward with

@@ -52,7 +52,7 @@ class SeisCL:
     _ENGINE_KEYS = ("ND", "N", "dh", "dt", "NT", "L", "f0", "FL", "FDORDER", "MAXRELERROR",
                     "FP16", "freesurf", "abs_type", "VPPML", "NPOWER", "FPML", "K_MAX_CPML",
                     "nab", "abpc", "seisout", "param_type", "tmax", "tmin", "restype",
-                    "scalerms", "scaleshot", "scalermsnorm", "fmin", "fmax")
+                    "scalerms", "scaleshot", "scalermsnorm", "fmin", "fmax", "Hout")
 
     def __init__(self, N=None, ND=2, dh=10, dt=0.0008, NT=875, L=0, f0=15,
                  FL=np.array(15), FDORDER=8, MAXRELERROR=1, FP16=0,
@@ -66,7 +66,7 @@ class SeisCL:
                  offmin=-float("Inf"), offmax=float("Inf"), inputres=0, restype=0,
                  scalerms=0, scalermsnorm=0, scaleshot=0, seisout=2, resout=0, rmsout=0,
                  movout=0, file="SeisCL", workdir="./seiscl", device: int = 0,
-                 gradfix: int = 95, files: bool = False, spawn: bool = True,
+                 gradfix: int = 223, files: bool = False, spawn: bool = True,
                  progname: str | None = None):
         self.N, self.ND, self.dh, self.dt, self.NT = N, ND, dh, dt, NT
         self.L, self.f0, self.FL = L, f0, FL
@@ -369,6 +369,9 @@ class SeisCL:
         if grad:
             g = m.grad_read()
             self._results["grad"] = {n: g[n] for n in g}
+            if self.Hout:
+                h = m.hess_read()
+                self._results["H"] = {n: h[n] for n in h}
         m.close()
         return ""
 
@@ -455,6 +458,23 @@ class SeisCL:
                 raise SeisCLError(f"no gradient for {name}")
             out.append(self._crop(np.array(self._results["grad"][name])))
         return out
+
+    def read_Hessian(self, workdir=None, param_names=None, filename=None):
+        """The approximate Hessian written with the gradient when Hout=1
+        (SeisCL/SeisCL.py:605-640): H<name> of the gradient file."""
+        if param_names is None:
+            param_names = self.params
+        if self.files:
+            path = self._path(workdir, self.file_gout if filename is None else filename)
+            try:
+                out = [np.transpose(h5mat.read(path, "H" + n)) for n in param_names]
+            except Exception as e:
+                raise SeisCLError("Could not read Hessian") from e
+        else:
+            if "H" not in self._results:
+                raise SeisCLError("Could not read Hessian")
+            out = [self._results["H"][n].copy() for n in param_names]
+        return [self._crop(o) for o in out]
 
     def read_rms(self, workdir=None, filename=None):
         if self.files:

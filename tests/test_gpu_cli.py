@@ -69,20 +69,28 @@ def test_forward_and_gradient_files(oracle_lib, tmp_path):
     seis.write_data({"vx": dobs[0], "vz": dobs[1]})
     print("[cli] gradient files", flush=True)
     seis.rmsout = 1
+    seis.Hout = 1
     seis.set_forward([0, 1], model, withgrad=True)
     seis.execute()
     gout = seis.read_grad()
+    hout = seis.read_Hessian()
     rms, norm = seis.read_rms()
     tot = {k: 0 for k in seis.params}
+    htot = {k: 0 for k in seis.params}
     r_sum = rn_sum = 0.0
     for rc, shot, ref in refs:
-        g = oracle_lib.gradient(model, *shot, din=ref, **csts)
+        g = oracle_lib.gradient(model, *shot, din=ref, Hout=1, **csts)
         for k in tot:
             tot[k] = tot[k] + g["grad"][k].astype(np.float64)
+            htot[k] = htot[k] + g["hess"][k].astype(np.float64)
         r_sum += g["rms"][0]
         rn_sum += g["rms"][1]
     for name, g in zip(seis.params, gout):
         assert rel_l2(g, _cropped(tot[name], seis.nab)) <= 1e-5, name
+    # the approximate Hessian, H<name> of the gradient file
+    for name, h in zip(seis.params, hout):
+        assert np.abs(htot[name]).max() > 0
+        assert rel_l2(h, _cropped(htot[name], seis.nab)) <= 1e-5, name
     # the rms file holds sqrt of the sums (src/writehdf5.c:236-245)
     assert abs(norm - np.sqrt(rn_sum)) <= 1e-5 * np.sqrt(rn_sum)
     assert abs(rms - np.sqrt(r_sum) / np.sqrt(rn_sum)) <= 1e-5 * rms

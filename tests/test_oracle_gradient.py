@@ -294,3 +294,35 @@ def test_gradient_free_surface_fd(oracle_lib, ND):
             gv = oracle_lib.gradient(p, *shot, din=din, gradfix=fix, **csts)["grad"]["vp"]
             errs.append(abs(float(np.sum(gv.astype(np.float64) * d)) / fd - 1))
         assert errs[0] < 0.05 and errs[0] < errs[1], errs
+
+
+@pytest.mark.parametrize("ND", [2, 3])
+def test_rtm_image_and_hessian(oracle_lib, ND):
+    """restype 1 accumulates the M image without the moduli factor c1 and no
+    mu term (src/update_adjs2D.cl:459-468): with the same adjoint sources,
+    the FWI vp kernel is c1 times the RTM one cell by cell, c1 = 1/(2M-2mu)^2
+    in 2D and 1/(3M-4mu)^2 in 3D of the transformed moduli.  The approximate
+    Hessian (Hout) depends on the forward field only: the same for other
+    adjoint sources, non-negative for vp, and again c1 times the RTM one."""
+    csts, p, (sp, s, rp), _ = _smooth_case(ND)
+    csts = dict(csts, NT=120, nthreads=8, Hout=1)
+    rng = np.random.default_rng(3)
+    nout = 3 if ND == 3 else 2
+    res = [rng.standard_normal((120, rp.shape[1])).astype(np.float32) for _ in range(nout)]
+    fwi = oracle_lib.gradient(p, sp, s, rp, res=res, **csts)
+    rtm = oracle_lib.gradient(p, sp, s, rp, res=res, **dict(csts, restype=1))
+    other = oracle_lib.gradient(p, sp, s, rp, res=[2.0 * r[::-1] for r in res], **csts)
+    t = oracle_lib.transform(p, **csts)
+    M, mu = t["M"].astype(np.float64), t["mu"].astype(np.float64)
+    c1 = 1 / (2 * M - 2 * mu) ** 2 if ND == 2 else 1 / (3 * M - 4 * mu) ** 2
+    g, r = fwi["grad"]["vp"].astype(np.float64), rtm["grad"]["vp"].astype(np.float64)
+    big = np.abs(r) > 1e-3 * np.abs(r).max()
+    assert big.sum() > 100
+    np.testing.assert_allclose(g[big], c1[big] * r[big], rtol=2e-4)
+    assert np.abs(rtm["grad"]["vs"]).max() == 0
+    hf, hr = fwi["hess"]["vp"].astype(np.float64), rtm["hess"]["vp"].astype(np.float64)
+    assert hf.min() >= 0 and hf.max() > 0
+    np.testing.assert_array_equal(fwi["hess"]["vp"], other["hess"]["vp"])
+    np.testing.assert_array_equal(fwi["hess"]["vs"], other["hess"]["vs"])
+    bigh = hr > 1e-3 * hr.max()
+    np.testing.assert_allclose(hf[bigh], c1[bigh] * hr[bigh], rtol=2e-4)
