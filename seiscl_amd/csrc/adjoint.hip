@@ -28,6 +28,26 @@ namespace {
 
 constexpr int ABY = 4;   // rows of y (3D) or x (2D) per block
 
+// 3D kernels: non-temporal loads and stores (the nt bit) for the streams a
+// lane touches once (its own cell's fields, memory variables, coefficients,
+// forward history and gradient accumulators), so that the stencil
+// neighbours read by co-resident blocks stay in L2.
+#ifndef SEIS_NT_ADJ
+#define SEIS_NT_ADJ 1
+#endif
+template <typename T>
+__device__ __forceinline__ T own_ld(const T *p) {
+    if constexpr (SEIS_NT_ADJ != 0) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename T>
+__device__ __forceinline__ void own_st(T *p, T v) {
+    if constexpr (SEIS_NT_ADJ != 0) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <typename T>
+__device__ __forceinline__ void own_add(T *p, T v) { own_st(p, own_ld(p) + v); }
+
 // CPML correction of one adjoint derivative on the near/far strip of an
 // axis, with the profile index of the reference kernels (full grid: k or
 // 2*NAB-k; half grid: k or 2*NAB-1-k).
@@ -123,8 +143,8 @@ __device__ __forceinline__ void grad_mmu_3d(const AdjArgs<R> &A, long i, R sxx, 
                                             R lsxy, R lsyz, R lsxz, R lM, R lmu) {
     if (A.rtm) {
         const R dM = (sxx + syy + szz) * (lsxx + lsyy + lszz);
-        A.gM[i] += (double)-dM * A.gscale_s;
-        if (A.hM) A.hM[i] += (double)((sxx + syy + szz) * (sxx + syy + szz)) * A.hscale;
+        own_add(A.gM + i, (double)-dM * A.gscale_s);
+        if (A.hM) own_add(A.hM + i, (double)((sxx + syy + szz) * (sxx + syy + szz)) * A.hscale);
         return;
     }
     const R c1 = 1.0 / (3.0 * lM - 4.0 * lmu) / (3.0 * lM - 4.0 * lmu);
@@ -136,16 +156,16 @@ __device__ __forceinline__ void grad_mmu_3d(const AdjArgs<R> &A, long i, R sxx, 
     const double dmu = -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
                        c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
                              lszz * (2.0 * szz - sxx - syy));
-    A.gM[i] += (double)-dM * A.gscale_s;
-    A.gmu[i] += dmu * A.gscale_s;
+    own_add(A.gM + i, (double)-dM * A.gscale_s);
+    own_add(A.gmu + i, dmu * A.gscale_s);
     if (A.hM) {
         const R dMH = c1 * (sxx + syy + szz) * (sxx + syy + szz);
         const double hmu = c3 * (sxz * sxz + sxy * sxy + syz * syz) - 4.0 / 3 * dMH +
                            c5 * ((2.0 * sxx - syy - szz) * (2.0 * sxx - syy - szz) +
                                  (2.0 * syy - sxx - szz) * (2.0 * syy - sxx - szz) +
                                  (2.0 * szz - sxx - syy) * (2.0 * szz - sxx - syy));
-        A.hM[i] += (double)dMH * A.hscale;
-        A.hmu[i] += hmu * A.hscale;
+        own_add(A.hM + i, (double)dMH * A.hscale);
+        own_add(A.hmu + i, hmu * A.hscale);
     }
 }
 
@@ -272,11 +292,11 @@ __global__ __launch_bounds__(256) void k_adj_s3d(AdjArgs<R> A, int nbz, unsigned
     if (!adj_cell(g, nbz, nb, z, y, x)) return;
     const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
     const R *hc = f.hc;
-    const R fipjp = f.muipjp[i], fjpkp = f.mujpkp[i], fipkp = f.muipkp[i];
-    const R lM = f.M[i], lmu = f.mu[i];
+    const R fipjp = own_ld(f.muipjp + i), fjpkp = own_ld(f.mujpkp + i), fipkp = own_ld(f.muipkp + i);
+    const R lM = own_ld(f.M + i), lmu = own_ld(f.mu + i);
     const R gg = lM, ff = 2.0 * lmu;
-    R sxx = f.sxx[i], syy = f.syy[i], szz = f.szz[i];
-    R sxy = f.sxy[i], syz = f.syz[i], sxz = f.sxz[i];
+    R sxx = own_ld(f.sxx + i), syy = own_ld(f.syy + i), szz = own_ld(f.szz + i);
+    R sxy = own_ld(f.sxy + i), syz = own_ld(f.syz + i), sxz = own_ld(f.sxz + i);
     {
         const R vxx = Dm<H>(f.vx, i, sx, hc), vxy = Dp<H>(f.vx, i, sy, hc), vxz = Dp<H>(f.vx, i, 1, hc);
         const R vyx = Dp<H>(f.vy, i, sx, hc), vyy = Dm<H>(f.vy, i, sy, hc), vyz = Dp<H>(f.vy, i, 1, hc);
@@ -299,12 +319,12 @@ __global__ __launch_bounds__(256) void k_adj_s3d(AdjArgs<R> A, int nbz, unsigned
             syz = A.bnd[7 * n + m];
             sxz = A.bnd[8 * n + m];
         }
-        f.sxx[i] = sxx;
-        f.syy[i] = syy;
-        f.szz[i] = szz;
-        f.sxy[i] = sxy;
-        f.syz[i] = syz;
-        f.sxz[i] = sxz;
+        own_st(f.sxx + i, sxx);
+        own_st(f.syy + i, syy);
+        own_st(f.szz + i, szz);
+        own_st(f.sxy + i, sxy);
+        own_st(f.syz + i, syz);
+        own_st(f.sxz + i, sxz);
     }
     R vxx = Dm<H>(a.vx, i, sx, hc), vxy = Dp<H>(a.vx, i, sy, hc), vxz = Dp<H>(a.vx, i, 1, hc);
     R vyx = Dp<H>(a.vy, i, sx, hc), vyy = Dm<H>(a.vy, i, sy, hc), vyz = Dp<H>(a.vy, i, 1, hc);
@@ -343,15 +363,15 @@ __global__ __launch_bounds__(256) void k_adj_s3d(AdjArgs<R> A, int nbz, unsigned
     const R lsxx = ((gg * vxxyyzz) - (ff * vyyzz));
     const R lsyy = ((gg * vxxyyzz) - (ff * vxxzz));
     const R lszz = ((gg * vxxyyzz) - (ff * vxxyy));
-    R s6[6] = {a.sxy[i] + lsxy, a.syz[i] + lsyz, a.sxz[i] + lsxz,
-               a.sxx[i] + lsxx, a.syy[i] + lsyy, a.szz[i] + lszz};
+    R s6[6] = {own_ld(a.sxy + i) + lsxy, own_ld(a.syz + i) + lsyz, own_ld(a.sxz + i) + lsxz,
+               own_ld(a.sxx + i) + lsxx, own_ld(a.syy + i) + lsyy, own_ld(a.szz + i) + lszz};
     if (a.abs_type == 2) taper_apply(a, z, y, x, s6, 6);
-    a.sxy[i] = s6[0];
-    a.syz[i] = s6[1];
-    a.sxz[i] = s6[2];
-    a.sxx[i] = s6[3];
-    a.syy[i] = s6[4];
-    a.szz[i] = s6[5];
+    own_st(a.sxy + i, s6[0]);
+    own_st(a.syz + i, s6[1]);
+    own_st(a.sxz + i, s6[2]);
+    own_st(a.sxx + i, s6[3]);
+    own_st(a.syy + i, s6[4]);
+    own_st(a.szz + i, s6[5]);
     // M and mu gradients (src/update_adjs3D.cl:678-710)
     grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz, lM, lmu);
 }
@@ -364,8 +384,8 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
     if (!adj_cell(g, nbz, nb, z, y, x)) return;
     const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
     const R *hc = f.hc;
-    const R rip = f.rip[i], rjp = f.rjp[i], rkp = f.rkp[i];
-    R vx = f.vx[i], vy = f.vy[i], vz = f.vz[i];
+    const R rip = own_ld(f.rip + i), rjp = own_ld(f.rjp + i), rkp = own_ld(f.rkp + i);
+    R vx = own_ld(f.vx + i), vy = own_ld(f.vy + i), vz = own_ld(f.vz + i);
     {
         const R szz_z = Dp<H>(f.szz, i, 1, hc), sxx_x = Dp<H>(f.sxx, i, sx, hc);
         const R sxz_x = Dm<H>(f.sxz, i, sx, hc), sxz_z = Dm<H>(f.sxz, i, 1, hc);
@@ -382,9 +402,9 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
             vy = A.bnd[n + m];
             vz = A.bnd[2 * n + m];
         }
-        f.vx[i] = vx;
-        f.vy[i] = vy;
-        f.vz[i] = vz;
+        own_st(f.vx + i, vx);
+        own_st(f.vy + i, vy);
+        own_st(f.vz + i, vz);
     }
     R szz_z = Dp<H>(a.szz, i, 1, hc), sxx_x = Dp<H>(a.sxx, i, sx, hc);
     R sxz_x = Dm<H>(a.sxz, i, sx, hc), sxz_z = Dm<H>(a.sxz, i, 1, hc);
@@ -422,14 +442,14 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
     const R lvx = ((sxx_x + sxy_y + sxz_z) * rip);
     const R lvy = ((syy_y + sxy_x + syz_z) * rjp);
     const R lvz = ((szz_z + sxz_x + syz_y) * rkp);
-    R v3[3] = {a.vx[i] + lvx, a.vy[i] + lvy, a.vz[i] + lvz};
+    R v3[3] = {own_ld(a.vx + i) + lvx, own_ld(a.vy + i) + lvy, own_ld(a.vz + i) + lvz};
     if (a.abs_type == 2) taper_apply(a, z, y, x, v3, 3);
-    a.vx[i] = v3[0];
-    a.vy[i] = v3[1];
-    a.vz[i] = v3[2];
+    own_st(a.vx + i, v3[0]);
+    own_st(a.vy + i, v3[1]);
+    own_st(a.vz + i, v3[2]);
     // density gradient (src/update_adjv3D.cl:481)
     const R d = -vx * lvx - vy * lvy - vz * lvz;
-    A.grho[i] += d;
+    own_add(A.grho + i, (double)d);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,6 +466,12 @@ template <typename R, typename S>
 __device__ __forceinline__ R hst(const AdjArgs<R> &A, int q, long i) {
     if constexpr (std::is_same<S, __half>::value) return (R)__half2float(((const S *)A.st[q])[i]);
     else return (R)((const S *)A.st[q])[i];
+}
+template <typename R, typename S>
+__device__ __forceinline__ R hst_own(const AdjArgs<R> &A, int q, long i) {
+    if constexpr (std::is_same<S, __half>::value)
+        return (R)__half2float(__ushort_as_half(own_ld((const unsigned short *)A.st[q] + i)));
+    else return (R)own_ld((const S *)A.st[q] + i);
 }
 
 template <typename R, typename S, int H, int L>
@@ -591,17 +617,17 @@ __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigne
     const R vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
     const R vxxyyzz = vxx + vyy + vzz, vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
     const R DT2 = a.DT2;
-    const R fipjp = a.fipjp[i], fjpkp = a.fjpkp[i], fipkp = a.fipkp[i];
-    const R gg = a.cg[i], f = a.cf[i], d = a.cd[i], e = a.ce[i];
-    const R dipjp = a.dipjp[i], djpkp = a.djpkp[i], dipkp = a.dipkp[i];
+    const R fipjp = own_ld(a.fipjp + i), fjpkp = own_ld(a.fjpkp + i), fipkp = own_ld(a.fipkp + i);
+    const R gg = own_ld(a.cg + i), f = own_ld(a.cf + i), d = own_ld(a.cd + i), e = own_ld(a.ce + i);
+    const R dipjp = own_ld(a.dipjp + i), djpkp = own_ld(a.djpkp + i), dipkp = own_ld(a.dipkp + i);
     const long nf = g.nfield;
     R rxy[L], ryz[L], rxz[L], rxx[L], ryy[L], rzz[L];
     R sumrxy = 0, sumryz = 0, sumrxz = 0, sumrxx = 0, sumryy = 0, sumrzz = 0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         const long ir = l * nf + i;
-        rxy[l] = a.rxy[ir]; ryz[l] = a.ryz[ir]; rxz[l] = a.rxz[ir];
-        rxx[l] = a.rxx[ir]; ryy[l] = a.ryy[ir]; rzz[l] = a.rzz[ir];
+        rxy[l] = own_ld(a.rxy + ir); ryz[l] = own_ld(a.ryz + ir); rxz[l] = own_ld(a.rxz + ir);
+        rxx[l] = own_ld(a.rxx + ir); ryy[l] = own_ld(a.ryy + ir); rzz[l] = own_ld(a.rzz + ir);
         sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
         sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
     }
@@ -622,26 +648,26 @@ __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigne
         rxx[l] = b * (rxx[l] * c - et * ((e * vxxyyzz) - (d * vyyzz)));
         ryy[l] = b * (ryy[l] * c - et * ((e * vxxyyzz) - (d * vxxzz)));
         rzz[l] = b * (rzz[l] * c - et * ((e * vxxyyzz) - (d * vxxyy)));
-        a.rxy[ir] = rxy[l]; a.ryz[ir] = ryz[l]; a.rxz[ir] = rxz[l];
-        a.rxx[ir] = rxx[l]; a.ryy[ir] = ryy[l]; a.rzz[ir] = rzz[l];
+        own_st(a.rxy + ir, rxy[l]); own_st(a.ryz + ir, ryz[l]); own_st(a.rxz + ir, rxz[l]);
+        own_st(a.rxx + ir, rxx[l]); own_st(a.ryy + ir, ryy[l]); own_st(a.rzz + ir, rzz[l]);
         sumrxy += rxy[l]; sumryz += ryz[l]; sumrxz += rxz[l];
         sumrxx += rxx[l]; sumryy += ryy[l]; sumrzz += rzz[l];
     }
-    R s6[6] = {a.sxy[i] + (lsxy + (DT2 * sumrxy)), a.syz[i] + (lsyz + (DT2 * sumryz)),
-               a.sxz[i] + (lsxz + (DT2 * sumrxz)), a.sxx[i] + (lsxx + (DT2 * sumrxx)),
-               a.syy[i] + (lsyy + (DT2 * sumryy)), a.szz[i] + (lszz + (DT2 * sumrzz))};
+    R s6[6] = {own_ld(a.sxy + i) + (lsxy + (DT2 * sumrxy)), own_ld(a.syz + i) + (lsyz + (DT2 * sumryz)),
+               own_ld(a.sxz + i) + (lsxz + (DT2 * sumrxz)), own_ld(a.sxx + i) + (lsxx + (DT2 * sumrxx)),
+               own_ld(a.syy + i) + (lsyy + (DT2 * sumryy)), own_ld(a.szz + i) + (lszz + (DT2 * sumrzz))};
     if (a.abs_type == 2) taper_apply(a, z, y, x, s6, 6);
-    a.sxy[i] = s6[0];
-    a.syz[i] = s6[1];
-    a.sxz[i] = s6[2];
-    a.sxx[i] = s6[3];
-    a.syy[i] = s6[4];
-    a.szz[i] = s6[5];
+    own_st(a.sxy + i, s6[0]);
+    own_st(a.syz + i, s6[1]);
+    own_st(a.sxz + i, s6[2]);
+    own_st(a.sxx + i, s6[3]);
+    own_st(a.syy + i, s6[4]);
+    own_st(a.szz + i, s6[5]);
     // M and mu gradients (src/update_adjs3D.cl:678-710) from the forward state
-    const R sxx = hst<R, S>(A, 3, i), syy = hst<R, S>(A, 4, i), szz = hst<R, S>(A, 5, i);
-    const R sxy = hst<R, S>(A, 6, i), syz = hst<R, S>(A, 7, i), sxz = hst<R, S>(A, 8, i);
-    grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz, A.Ms[i],
-                A.mus[i]);
+    const R sxx = hst_own<R, S>(A, 3, i), syy = hst_own<R, S>(A, 4, i), szz = hst_own<R, S>(A, 5, i);
+    const R sxy = hst_own<R, S>(A, 6, i), syz = hst_own<R, S>(A, 7, i), sxz = hst_own<R, S>(A, 8, i);
+    grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz,
+                own_ld(A.Ms + i), own_ld(A.mus + i));
 }
 
 template <typename R, typename S, int H>
@@ -652,7 +678,7 @@ __global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigne
     if (!adj_cell(g, nbz, nb, z, y, x)) return;
     const long i = g.idx(z, y, x), sx = g.sx, sy = g.sy;
     const R *hc = a.hc;
-    const R rip = a.rip[i], rjp = a.rjp[i], rkp = a.rkp[i];
+    const R rip = own_ld(a.rip + i), rjp = own_ld(a.rjp + i), rkp = own_ld(a.rkp + i);
     R szz_z = Dp<H>(a.szz, i, 1, hc), sxx_x = Dp<H>(a.sxx, i, sx, hc);
     R sxz_x = Dm<H>(a.sxz, i, sx, hc), sxz_z = Dm<H>(a.sxz, i, 1, hc);
     R syz_y = Dm<H>(a.syz, i, sy, hc), syz_z = Dm<H>(a.syz, i, 1, hc);
@@ -689,14 +715,15 @@ __global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigne
     const R lvx = ((sxx_x + sxy_y + sxz_z) * rip);
     const R lvy = ((syy_y + sxy_x + syz_z) * rjp);
     const R lvz = ((szz_z + sxz_x + syz_y) * rkp);
-    R v3[3] = {a.vx[i] + lvx, a.vy[i] + lvy, a.vz[i] + lvz};
+    R v3[3] = {own_ld(a.vx + i) + lvx, own_ld(a.vy + i) + lvy, own_ld(a.vz + i) + lvz};
     if (a.abs_type == 2) taper_apply(a, z, y, x, v3, 3);
-    a.vx[i] = v3[0];
-    a.vy[i] = v3[1];
-    a.vz[i] = v3[2];
+    own_st(a.vx + i, v3[0]);
+    own_st(a.vy + i, v3[1]);
+    own_st(a.vz + i, v3[2]);
     // density gradient (src/update_adjv3D.cl:481) from the forward state
-    const R d = -hst<R, S>(A, 0, i) * lvx - hst<R, S>(A, 1, i) * lvy - hst<R, S>(A, 2, i) * lvz;
-    A.grho[i] += (double)d * A.gscale_v;
+    const R d = -hst_own<R, S>(A, 0, i) * lvx - hst_own<R, S>(A, 1, i) * lvy -
+                hst_own<R, S>(A, 2, i) * lvz;
+    own_add(A.grho + i, (double)d * A.gscale_v);
 }
 
 // ---------------------------------------------------------------------------

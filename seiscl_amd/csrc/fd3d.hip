@@ -40,6 +40,24 @@ constexpr int BYV = SEIS_BY_V;
 template <typename R, typename S>
 constexpr int bys() { return sizeof(R) == 4 && sizeof(S) == 4 ? SEIS_BY_S : 4; }
 
+// Non-temporal access (the nt bit on loads and stores) for the streams each
+// update touches exactly once: the cell's own fields, its coefficients, its
+// memory variables and the stores.  They then do not displace the stencil
+// neighbours that co-resident blocks still read from L2.  Measured at 256^3
+// (FD4, CPML, free surface) against plain accesses: viscoelastic L=2 -11%
+// per step, elastic -4%, FP16 storage -8%, FP64 -8%.
+// Bits, stress update: 1 the cell's own stresses, 2 coefficients, 4 memory
+// variables (loads), 8 the stores.
+#ifndef SEIS_NT_S
+#define SEIS_NT_S 15
+#endif
+constexpr int kNTS = SEIS_NT_S;
+// velocity update: 1 the cell's own velocities, 2 coefficients, 8 the stores
+#ifndef SEIS_NT_V
+#define SEIS_NT_V 11
+#endif
+constexpr int kNTV = SEIS_NT_V;
+
 // tile -> (bz, y slab, x) with z fastest, then x, then the y slab
 template <int BY>
 __device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int &y, int &x) {
@@ -117,8 +135,16 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
         x1 = pml_load(a, a.psi_sxy_x + ipx, 0, false, iI);
         x2 = pml_load(a, a.psi_sxz_x + ipx, 0, false, iI);
     }
-    const R vx0 = T::ld(vxp, os), vy0 = T::ld(vyp, os), vz0 = T::ld(vzp, os);
-    const R rip = ldo(a.rip, o), rjp = ldo(a.rjp, o), rkp = ldo(a.rkp, o);
+    auto ldv = [](const S *p, unsigned off) -> R {
+        if constexpr (kNTV & 1) return T::ldn_(p, off);
+        else return T::ld(p, off);
+    };
+    auto ldc = [](const R *p, unsigned off) -> R {
+        if constexpr (kNTV & 2) return ldn(p, off);
+        else return ldo(p, off);
+    };
+    const R vx0 = ldv(vxp, os), vy0 = ldv(vyp, os), vz0 = ldv(vzp, os);
+    const R rip = ldc(a.rip, o), rjp = ldc(a.rjp, o), rkp = ldc(a.rkp, o);
     if (pz) {
         pml_apply(sxz_z, a.psi_sxz_z + ipz, z0);
         pml_apply(syz_z, a.psi_syz_z + ipz, z1);
@@ -146,9 +172,15 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
         if (x < NAB) { vx *= t[x]; vy *= t[x]; vz *= t[x]; }
         if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vy *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
     }
-    T::st(vxp, os, vx);
-    T::st(vyp, os, vy);
-    T::st(vzp, os, vz);
+    if constexpr (kNTV & 8) {
+        T::stn_(vxp, os, vx);
+        T::stn_(vyp, os, vy);
+        T::stn_(vzp, os, vz);
+    } else {
+        T::st(vxp, os, vx);
+        T::st(vyp, os, vy);
+        T::st(vzp, os, vz);
+    }
 }
 
 template <typename R, typename S, int H, int L, bool FS>
@@ -208,34 +240,50 @@ __global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz
         x2 = pml_load(a, a.psi_vz_x + ipx, 0, true, iH);
     }
     // the cell's own values and coefficients, before any store
-    R sxy = T::ld(sxyp, os), syz = T::ld(syzp, os), sxz = T::ld(sxzp, os);
-    R sxx = T::ld(sxxp, os), syy = T::ld(syyp, os), szz = T::ld(szzp, os);
+    auto lds = [](const S *p, unsigned off) -> R {
+        if constexpr (kNTS & 1) return T::ldn_(p, off);
+        else return T::ld(p, off);
+    };
+    auto ldr = [](const S *p, unsigned off) -> R {
+        if constexpr (kNTS & 4) return T::ldn_(p, off);
+        else return T::ld(p, off);
+    };
+    auto ldc = [](const R *p, unsigned off) -> R {
+        if constexpr (kNTS & 2) return ldn(p, off);
+        else return ldo(p, off);
+    };
+    auto sts = [](S *p, unsigned off, R v) {
+        if constexpr (kNTS & 8) T::stn_(p, off, v);
+        else T::st(p, off, v);
+    };
+    R sxy = lds(sxyp, os), syz = lds(syzp, os), sxz = lds(sxzp, os);
+    R sxx = lds(sxxp, os), syy = lds(syyp, os), szz = lds(szzp, os);
     R rxy[L > 0 ? L : 1], ryz[L > 0 ? L : 1], rxz[L > 0 ? L : 1];
     R rxx[L > 0 ? L : 1], ryy[L > 0 ? L : 1], rzz[L > 0 ? L : 1];
     R gg, f, d = 0, e = 0;
     R fipjp, fjpkp, fipkp, dipjp = 0, djpkp = 0, dipkp = 0;
     if constexpr (L == 0) {
-        fipjp = ldo(a.muipjp, o);
-        fjpkp = ldo(a.mujpkp, o);
-        fipkp = ldo(a.muipkp, o);
-        gg = ldo(a.M, o);
-        f = (R)2 * ldo(a.mu, o);
+        fipjp = ldc(a.muipjp, o);
+        fjpkp = ldc(a.mujpkp, o);
+        fipkp = ldc(a.muipkp, o);
+        gg = ldc(a.M, o);
+        f = (R)2 * ldc(a.mu, o);
     } else {
-        gg = ldo(a.cg, o);
-        f = ldo(a.cf, o);
-        d = ldo(a.cd, o);
-        e = ldo(a.ce, o);
-        fipjp = ldo(a.fipjp, o);
-        fjpkp = ldo(a.fjpkp, o);
-        fipkp = ldo(a.fipkp, o);
-        dipjp = ldo(a.dipjp, o);
-        djpkp = ldo(a.djpkp, o);
-        dipkp = ldo(a.dipkp, o);
+        gg = ldc(a.cg, o);
+        f = ldc(a.cf, o);
+        d = ldc(a.cd, o);
+        e = ldc(a.ce, o);
+        fipjp = ldc(a.fipjp, o);
+        fjpkp = ldc(a.fjpkp, o);
+        fipkp = ldc(a.fipkp, o);
+        dipjp = ldc(a.dipjp, o);
+        djpkp = ldc(a.djpkp, o);
+        dipkp = ldc(a.dipkp, o);
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             const long ln = l * nf;
-            rxy[l] = T::ld(rxyp + ln, os); ryz[l] = T::ld(ryzp + ln, os); rxz[l] = T::ld(rxzp + ln, os);
-            rxx[l] = T::ld(rxxp + ln, os); ryy[l] = T::ld(ryyp + ln, os); rzz[l] = T::ld(rzzp + ln, os);
+            rxy[l] = ldr(rxyp + ln, os); ryz[l] = ldr(ryzp + ln, os); rxz[l] = ldr(rxzp + ln, os);
+            rxx[l] = ldr(rxxp + ln, os); ryy[l] = ldr(ryyp + ln, os); rzz[l] = ldr(rzzp + ln, os);
         }
     }
     if (pz) {
@@ -354,14 +402,14 @@ __global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz
             }
         }
     }
-    T::st(sxyp, os, sxy); T::st(syzp, os, syz); T::st(sxzp, os, sxz);
-    T::st(sxxp, os, sxx); T::st(syyp, os, syy); T::st(szzp, os, szz);
+    sts(sxyp, os, sxy); sts(syzp, os, syz); sts(sxzp, os, sxz);
+    sts(sxxp, os, sxx); sts(syyp, os, syy); sts(szzp, os, szz);
     if constexpr (L > 0) {
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             const long ln = l * nf;
-            T::st(rxyp + ln, os, rxy[l]); T::st(ryzp + ln, os, ryz[l]); T::st(rxzp + ln, os, rxz[l]);
-            T::st(rxxp + ln, os, rxx[l]); T::st(ryyp + ln, os, ryy[l]); T::st(rzzp + ln, os, rzz[l]);
+            sts(rxyp + ln, os, rxy[l]); sts(ryzp + ln, os, ryz[l]); sts(rxzp + ln, os, rxz[l]);
+            sts(rxxp + ln, os, rxx[l]); sts(ryyp + ln, os, ryy[l]); sts(rzzp + ln, os, rzz[l]);
         }
     }
     if constexpr (FS) {

@@ -126,6 +126,19 @@ __device__ __forceinline__ R DmO(const R *v, unsigned o, long s, const R *hc) {
     return acc;
 }
 
+// Non-temporal variants, for streams a kernel touches once (the cell's own
+// values and coefficients): they bypass the cache's retention so that the
+// stencil neighbours stay resident.
+template <typename R>
+__device__ __forceinline__ R ldn(const R *base, unsigned boff) {
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) + boff));
+}
+template <typename R>
+__device__ __forceinline__ void stn(R *base, unsigned boff, R v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<R *>(reinterpret_cast<char *>(base) + boff));
+}
+
 // Wavefield storage narrower than the compute type (FP16 fields with FP32
 // arithmetic): loads widen exactly, stores round to nearest even.
 template <typename R, typename S>
@@ -134,6 +147,12 @@ struct Store {
         return (R)ldo(base, boff);
     }
     static __device__ __forceinline__ void st(S *base, unsigned boff, R v) { sto(base, boff, (S)v); }
+    static __device__ __forceinline__ R ldn_(const S *base, unsigned boff) {
+        return (R)ldn(base, boff);
+    }
+    static __device__ __forceinline__ void stn_(S *base, unsigned boff, R v) {
+        stn(base, boff, (S)v);
+    }
 };
 template <>
 struct Store<float, __half> {
@@ -142,6 +161,12 @@ struct Store<float, __half> {
     }
     static __device__ __forceinline__ void st(__half *base, unsigned boff, float v) {
         sto(base, boff, __float2half_rn(v));
+    }
+    static __device__ __forceinline__ float ldn_(const __half *base, unsigned boff) {
+        return __half2float(__ushort_as_half(ldn(reinterpret_cast<const unsigned short *>(base), boff)));
+    }
+    static __device__ __forceinline__ void stn_(__half *base, unsigned boff, float v) {
+        stn(reinterpret_cast<unsigned short *>(base), boff, __half_as_ushort(__float2half_rn(v)));
     }
 };
 template <int H, typename R, typename S>
