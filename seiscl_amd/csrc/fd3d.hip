@@ -26,15 +26,17 @@ namespace {
 
 // rows of y per block of the velocity and of the stress update.  The FP32
 // stress update reads three y-derivatives and runs at 4 waves per SIMD
-// either way, so 16 rows (one 1024-lane block per CU) cut its y-halo
-// re-reads; with FP16 storage or FP64 arithmetic 4 rows measure faster (in
-// FP64 a 1024-lane block would cap the kernel at 128 VGPRs and spill).  The
+// either way, so 8 rows (a 512-lane block) cut its y-halo re-reads (256^3,
+// with the non-temporal accesses below: viscoelastic 0.712 ms with 4 rows,
+// 0.699 with 8, 0.709 with 16; elastic 0.339, 0.335, 0.359); with FP16
+// storage or FP64 arithmetic 4 rows measure as fast or faster (in FP64 a
+// 1024-lane block would cap the kernel at 128 VGPRs and spill).  The
 // velocity update keeps 4 rows, the block size that lets it run 6 waves.
 #ifndef SEIS_BY_V
 #define SEIS_BY_V 4
 #endif
 #ifndef SEIS_BY_S
-#define SEIS_BY_S 16
+#define SEIS_BY_S 8
 #endif
 constexpr int BYV = SEIS_BY_V;
 template <typename R, typename S>
@@ -58,16 +60,33 @@ constexpr int kNTS = SEIS_NT_S;
 #endif
 constexpr int kNTV = SEIS_NT_V;
 
-// tile -> (bz, y slab, x) with z fastest, then x, then the y slab
+// Interleave the y slabs over the XCDs (1) or give each XCD a contiguous
+// run of them (0).
+#ifndef SEIS_SLAB_IL
+#define SEIS_SLAB_IL 1
+#endif
+
+// tile -> (bz, y slab, x) with z fastest, then x, then the y slab.  Each XCD
+// receives a contiguous run of tiles, i.e. whole y slabs.  When the slab
+// count is a multiple of 8 the slabs are dealt out round-robin (XCD k gets
+// slabs k, k + 8, ...): a contiguous split would give the two outer XCDs
+// every slab of the y-CPML strips, and the kernel lasts as long as its
+// slowest XCD.
 template <int BY>
-__device__ __forceinline__ bool tile3(int nbz, int NX, unsigned nb, int &z, int &y, int &x) {
+__device__ __forceinline__ bool tile3(int nbz, int NX, int NY, unsigned nb, int &z, int &y,
+                                      int &x) {
     unsigned tile;
     xcd_tile(nb, tile);
     if (tile >= nb) return false;
     const int bz = tile % nbz;
     const unsigned r = tile / nbz;
     x = r % NX;
-    const int by = r / NX;
+    int by = r / NX;
+    const int nby = (NY + BY - 1) / BY;
+    if (SEIS_SLAB_IL && nby % 8 == 0) {
+        const int m = nby / 8;
+        by = (by % m) * 8 + by / m;
+    }
     z = bz * BZ + threadIdx.x;
     y = by * BY + threadIdx.y;
     return true;
@@ -84,7 +103,7 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3<BYV>(nbz, g.NX, nb, z, y, x)) return;
+    if (!tile3<BYV>(nbz, g.NX, g.NY, nb, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy;
@@ -188,7 +207,7 @@ __global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3<bys<R, S>()>(nbz, g.NX, nb, z, y, x)) return;
+    if (!tile3<bys<R, S>()>(nbz, g.NX, g.NY, nb, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy, nf = g.nfield;
