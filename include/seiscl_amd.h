@@ -225,6 +225,17 @@ int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos,
 /* sources res [nout][nrec][NT]: steps tmax-1 .. tmin+1 (src/time_stepping.c:*/
 /* 869-940); adds this shot's gradient to the accumulators.                 */
 int seis_adjoint(seis_model *m, const float *res);
+/* seis_adjoint recording the movie of the gradient run (movout with gradout,*/
+/* src/time_stepping.c:926-929, frame index :317-320): after the adjoint    */
+/* step t with t % movout == 0, frame (t+1)/movout - 1 of each wavefield    */
+/* vars[v] ("vx", "sxx", ...) holds the backpropagated forward wavefield.  */
+/* Viscoelastic runs read the forward state from a recomputed history and  */
+/* record the forward wavefield at the start of step t instead, the state  */
+/* backpropagation reconstructs there.  frames: [nvar][NT/movout][Pi N],    */
+/* interior cells in the parameter layout, physical units; frames no step  */
+/* writes are zero.                                                         */
+int seis_adjoint_movie(seis_model *m, const float *res, int movout, int nvar,
+                       const char *const *vars, float *frames);
 /* Gradients [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), in the */
 /* parameter layout (transf_grad, src/calc_grad.c:975-1110).                 */
 int seis_grad_read(seis_model *m, float *grad);

@@ -78,7 +78,7 @@ def load():
                                     [ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
                                      ctypes.c_int, _FP, _FP, _FP,
                                      ctypes.POINTER(ctypes.c_double), _FP, _FP,
-                                     ctypes.c_char_p, ctypes.c_int])
+                                     ctypes.c_int, _FP, ctypes.c_char_p, ctypes.c_int])
         lib.or_shell_check.restype = ctypes.c_int
         lib.or_shell_check.argtypes = [ctypes.POINTER(OrCsts), ctypes.POINTER(ctypes.c_long)]
         lib.or_nvars.argtypes = [ctypes.POINTER(OrCsts)]
@@ -276,14 +276,16 @@ HISTORY = 32    # elastic runs also take the forward state from the stored histo
 
 
 def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FIX_ALL,
-             nthreads: int = 0, fields: bool = False, **csts):
+             nthreads: int = 0, fields: bool = False, movout: int = 0, **csts):
     """Oracle gradient of one shot by backpropagation.
 
     din: observed seismograms (list of (NT, nrec) in out_names order), or
     res: adjoint sources in the same layout (used as given, then scaled).
     Returns dict(out=[(NT,nrec)...], grad={p0name, p1name, 'rho'} arrays in
     model layout, res=[scaled adjoint sources], rms=(rms, rmsnorm)) and,
-    with fields=True, the backpropagated forward wavefields at the end."""
+    with fields=True, the backpropagated forward wavefields at the end.
+    movout > 0 adds movie={name: (NT/movout, *N) interior frames} of the
+    gradient run (frames no adjoint step writes are zero)."""
     lib = load()
     c = make_csts(nthreads=nthreads, **csts)
     p0, p1, rho, taup, taus = _params(params, c)
@@ -313,11 +315,13 @@ def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FI
     nvars = lib.or_nvars(ctypes.byref(c))
     fo = np.zeros(nvars * nv, np.float32) if fields else None
     hess = np.zeros(3 * n, np.float32) if csts.get("Hout") else None
+    nfr = c.NT // movout if movout > 0 else 0
+    mov = np.zeros(nvars * nfr * nv, np.float32) if nfr else None
     err = ctypes.create_string_buffer(256)
     st = lib.or_gradient(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
                          sp.shape[0], fp(sp), fp(s), nrec, fp(rp), fp(d), fp(r),
                          int(gradfix), fp(out), fp(res_out), fp(grad), rms, fp(fo),
-                         fp(hess), err, 256)
+                         fp(hess), int(movout), fp(mov), err, 256)
     if st:
         raise RuntimeError("oracle: " + err.value.decode())
     names = {0: ("vp", "vs"), 1: ("M", "mu"), 2: ("Ip", "Is"), 3: ("vp", "vs")}[c.param_type]
@@ -334,6 +338,14 @@ def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FI
         for v, name in enumerate(var_names(c)):
             fd[name] = np.ascontiguousarray(fo[v * nv:(v + 1) * nv].reshape(pshape[::-1]).T)
         result["fields"] = fd
+    if mov is not None:
+        inner = tuple(slice(H, -H) for _ in shape)
+        mv = {}
+        for v, name in enumerate(var_names(c)):
+            fr = mov[v * nfr * nv:(v + 1) * nfr * nv].reshape([nfr] + pshape[::-1])
+            fr = np.moveaxis(fr, 0, -1).T            # (nfr, *pshape) with z first
+            mv[name] = np.ascontiguousarray(fr[(slice(None),) + inner])
+        result["movie"] = mv
     return result
 
 

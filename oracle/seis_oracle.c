@@ -2548,12 +2548,24 @@ static void adj_free(sim *A) {
     sim_free(A);
 }
 
+/* Movie frame of a gradient run (src/time_stepping.c:926-929, frame index
+ * of :317-320): every variable, padded, into frame (t+1)/movout - 1 when
+ * t % movout == 0. */
+static void movie_snap(const or_csts *c, sim *S, int movout, float *movie, int t) {
+    if (!movie || movout <= 0 || t % movout) return;
+    const int nfr = c->NT / movout, f = (t + 1) / movout - 1;
+    if (f < 0 || f >= nfr) return;
+    for (int v = 0; v < or_nvars(c); v++)
+        memcpy(movie + ((size_t)v * nfr + f) * S->nv, named_var(S, or_varname(c, v)),
+               sizeof(float) * S->nv);
+}
+
 int or_gradient(const or_csts *c, const float *p0, const float *p1, const float *rho,
                 const float *taup, const float *taus, int nsrc, const float *src_pos,
                 const float *src, int nrec, const float *rec_pos, const float *din,
                 const float *res_in, int gradfix, float *out, float *res_out,
-                float *grad, double *rms, float *fields_out, float *hess, char *err,
-                int errlen) {
+                float *grad, double *rms, float *fields_out, float *hess, int movout,
+                float *movie, char *err, int errlen) {
     sim S, A;
     char dummy[8];
     if (!err) {
@@ -2627,6 +2639,10 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     memset(out, 0, sizeof(float) * (size_t)nout * nrec * c->NT);
     /* forward with boundary saving: src/time_stepping.c:755-835 */
     for (int t = 0; t < tmax; t++) {
+        /* movie of a viscoelastic gradient run: the forward state at the */
+        /* start of step t, what backpropagation would reconstruct after  */
+        /* the adjoint step t                                             */
+        if (visco && t > tmin) movie_snap(c, &S, movout, movie, t);
         sources(&S, t, 1, nsrc, src_pos, src);
         if (visco && t > tmin)
             for (int q = 0; q < nfld; q++)
@@ -2747,6 +2763,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         else update_v2D(&A);
         grad_rho(&S, &A, grho);
         sources(&S, t, -1, nsrc, src_pos, src);
+        movie_snap(c, &S, movout, movie, t);
     }
     transf_grad(&S, Mg, mug, gM, gmu, grho, grad, gradfix & OR_FIX_RHO_SCALE);
     if (hess) transf_hess(&S, Mg, mug, hM, hmu, hess);

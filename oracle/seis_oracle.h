@@ -141,14 +141,20 @@ int  or_shell_check(const or_csts *c, long *nbnd);
  * rms (optional): [rms, rmsnorm]; fields_out (optional): the backpropagated
  * forward wavefields at the end, padded; hess (optional): the approximate
  * Hessian [3][Pi N] transformed like grad (Hout).  restype 1 accumulates the
- * RTM image of M instead of the gradient. */
+ * RTM image of M instead of the gradient.  movie (optional, movout > 0): the
+ * movie of the gradient run, [nvars][NT/movout][padded cells]: after the
+ * adjoint step t with t % movout == 0, frame (t+1)/movout - 1 holds the
+ * backpropagated forward wavefields (src/time_stepping.c:926-929); in
+ * viscoelastic runs, which read the forward state from the history, the
+ * forward wavefields at the start of step t (what backpropagation would
+ * reconstruct there).  Frames no step writes stay as given. */
 int  or_gradient(const or_csts *c, const float *p0, const float *p1,
                  const float *rho, const float *taup, const float *taus,
                  int nsrc, const float *src_pos, const float *src,
                  int nrec, const float *rec_pos, const float *din,
                  const float *res_in, int gradfix, float *out,
                  float *res_out, float *grad, double *rms, float *fields_out,
-                 float *hess, char *err, int errlen);
+                 float *hess, int movout, float *movie, char *err, int errlen);
 
 #ifdef __cplusplus
 }

@@ -179,8 +179,6 @@ static int read_csts(hid_t f, job_t *j) {
     if (j->gradout && j->back_prop_type != 1)
         return err("back_prop_type=%d is not available: gradients are computed by "
                    "backpropagation (back_prop_type=1)", j->back_prop_type);
-    if (j->movout > 0 && j->gradout)
-        return err("movie output (movout) is available for forward runs (gradout=0)");
     if (j->gradsrcout) return err("source gradient output (gradsrcout) is not available");
     if (j->halfpar) return err("half-precision parameters (halfpar) are not available");
     if (j->restype != 0 && j->restype != 1)
@@ -524,6 +522,31 @@ static int forward_movie(seis_model *m, job_t *j, int sh, int nsrc, const float 
     return s ? err("%s", seis_last_error()) : 0;
 }
 
+/* One shot of gradient with the movie of the gradient run (movout with
+ * gradout, src/time_stepping.c:926-929): seis_gradient_shot's steps through
+ * the public entry points, the adjoint recording the frames. */
+static int gradient_movie(seis_model *m, job_t *j, int sh, int nsrc, const float *sp,
+                          const float *src, int nrec, const float *rp, const float *din,
+                          const float *res_in, float *out, float *res, double *rr) {
+    const seis_csts *c = &j->c;
+    const long long n = n_cells(c), nf = c->NT / j->movout;
+    const size_t nres = (size_t)j->nout * nrec * c->NT;
+    float *frames = (float *)malloc(sizeof(float) * (size_t)(j->nmov * nf * n + 1));
+    if (!frames) return err("out of host memory");
+    int s = seis_forward_save(m, nsrc, sp, src, nrec, rp, out);
+    if (!s && res_in) memcpy(res, res_in, sizeof(float) * nres);
+    else if (!s) s = seis_residuals(m, nrec, out, din, res, &rr[0], &rr[1]);
+    if (!s) s = seis_res_scale(m, nrec, rp, res);
+    if (!s) s = seis_adjoint_movie(m, res, j->movout, j->nmov, (const char *const *)j->movs,
+                                   frames);
+    if (!s)
+        for (int v = 0; v < j->nmov; ++v)
+            memcpy(j->mov + ((long long)v * j->ns + sh) * nf * n,
+                   frames + (long long)v * nf * n, sizeof(float) * nf * n);
+    free(frames);
+    return s ? err("%s", seis_last_error()) : 0;
+}
+
 /* ---- the run -------------------------------------------------------------- */
 
 static int run(job_t *j) {
@@ -574,7 +597,18 @@ static int run(job_t *j) {
         fprintf(stdout, "    shot %d of %d: %d source(s), %d receiver(s)\n", sh + 1, j->ns,
                 nsrc, nrec);
         fflush(stdout);
-        if (j->gradout) {
+        if (j->gradout && j->movout > 0 && j->nmov) {
+            double rr[2] = {0, 0};
+            if (gradient_movie(m, j, sh, nsrc, sp, src, nrec, rp, need_din ? sd : NULL,
+                               j->inputres ? sr : NULL, so, sres, rr)) {
+                s = 1;
+                break;
+            }
+            if (need_din) {
+                j->rms += rr[0];
+                j->rmsnorm += rr[1];
+            }
+        } else if (j->gradout) {
             double rr[2] = {0, 0};
             if (seis_gradient_shot(m, nsrc, sp, src, nrec, rp, need_din ? sd : NULL,
                                    j->inputres ? sr : NULL, so, rr)) {

@@ -1028,7 +1028,7 @@ int seis_launch_varsout(seis_model *m, int nt) {
 namespace {
 // The bound shot's seismograms as [nout][nrec][NT] into outf (float) or
 // outd (double) (device: [NT][nout][nrec] of float, or of double in FP64 runs).
-int shot_read(seis_model *m, float *outf, double *outd) {
+static int shot_read(seis_model *m, float *outf, double *outd) {
     const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
     const bool dbl = m->prec == P_FP64;
     std::vector<char> raw((size_t)NT * nout * nrec * (dbl ? 8 : 4));
@@ -1186,7 +1186,7 @@ namespace {
 
 // (Re)allocate a gradient work buffer of at least `bytes` (exact: of exactly
 // `bytes`, freeing it when 0).
-int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes, bool exact) {
+static int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes, bool exact) {
     if (exact ? *have == bytes : *have >= bytes) return 0;
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     if (*buf) {
@@ -1216,7 +1216,7 @@ int grad_buffer(seis_model *m, float **buf, size_t *have, size_t bytes, bool exa
 // backwards over them.  Memory: n0 S + (L-1)(n-1) S + K[L-1] H; work: one
 // forward run plus L recomputations of it.  The plan has the fewest levels
 // that fit the budget B, and of those the least memory.
-bool plan_checkpoints(int T, size_t S, size_t H, size_t B, CkptPlan &best) {
+static bool plan_checkpoints(int T, size_t S, size_t H, size_t B, CkptPlan &best) {
     best = CkptPlan{};
     for (int L = 1; L <= kMaxLevels; ++L) {
         bool found = false;
@@ -1244,19 +1244,19 @@ bool plan_checkpoints(int T, size_t S, size_t H, size_t B, CkptPlan &best) {
     return false;
 }
 
-size_t state_bytes(const std::vector<std::pair<float *, size_t>> &st) {
+static size_t state_bytes(const std::vector<std::pair<float *, size_t>> &st) {
     size_t b = 0;
     for (auto &e : st) b += e.second;
     return b;
 }
-int state_save(const std::vector<std::pair<float *, size_t>> &st, char *dst, hipStream_t s) {
+static int state_save(const std::vector<std::pair<float *, size_t>> &st, char *dst, hipStream_t s) {
     for (auto &e : st) {
         HIP_TRY(hipMemcpyAsync(dst, e.first, e.second, hipMemcpyDeviceToDevice, s));
         dst += e.second;
     }
     return 0;
 }
-int state_load(const std::vector<std::pair<float *, size_t>> &st, const char *src,
+static int state_load(const std::vector<std::pair<float *, size_t>> &st, const char *src,
                hipStream_t s) {
     for (auto &e : st) {
         HIP_TRY(hipMemcpyAsync(e.first, src, e.second, hipMemcpyDeviceToDevice, s));
@@ -1266,7 +1266,7 @@ int state_load(const std::vector<std::pair<float *, size_t>> &st, const char *sr
 }
 
 // One forward step without receivers (the recomputations).
-void step_forward(seis_model *m, int t, hipStream_t st) {
+static void step_forward(seis_model *m, int t, hipStream_t st) {
     step_sources(m, t, 1, st);
     step_v(m, st);
     step_s(m, st);
@@ -1275,7 +1275,7 @@ void step_forward(seis_model *m, int t, hipStream_t st) {
 
 // Viscoelastic forward run for the gradient: the level-1 checkpoints of the
 // plan, then the seismograms.
-int forward_save_visco(seis_model *m, float *out) {
+static int forward_save_visco(seis_model *m, float *out) {
     auto &G = m->grad;
     const seis_csts &c = m->c;
     const int tmax = tmax_of(m);
@@ -1331,6 +1331,37 @@ int forward_save_visco(seis_model *m, float *out) {
     return out ? seis_shot_read(m, out) : 0;
 }
 
+// Movie of a gradient run (movout with gradout, src/time_stepping.c:926-929):
+// after the adjoint step t with t % movout == 0 the forward wavefields go to
+// frame (t+1)/movout - 1 (the frame index of src/time_stepping.c:317-320),
+// interior cells only, frames [nvar][NT/movout][NX][NY][NZ] (z fastest).
+struct MovieOut {
+    int movout = 0, nvar = 0;
+    const char *const *vars = nullptr;
+    float *frames = nullptr;
+};
+
+static int movie_snap(seis_model *m, const MovieOut *mv, int t) {
+    if (!mv || !mv->frames || t % mv->movout) return 0;
+    const int nfr = m->c.NT / mv->movout, f = (t + 1) / mv->movout - 1;
+    if (f < 0 || f >= nfr) return 0;
+    const Layout &g = m->g;
+    const int H = g.H, Hy = g.nd == 3 ? H : 0;
+    const long NZp = g.NZ + 2 * H, NYp = g.NY + 2 * Hy, NXp = g.NX + 2 * H;
+    const size_t ncell = (size_t)g.NZ * g.NY * g.NX;
+    std::vector<float> pad((size_t)NZp * NYp * NXp);
+    for (int v = 0; v < mv->nvar; ++v) {
+        if (seis_read_field(m, mv->vars[v], pad.data())) return 1;
+        float *dst = mv->frames + ((size_t)v * nfr + f) * ncell;
+        for (long x = 0; x < g.NX; ++x)
+            for (long y = 0; y < g.NY; ++y)
+                for (long z = 0; z < g.NZ; ++z)
+                    dst[((size_t)x * g.NY + y) * g.NZ + z] =
+                        pad[((x + H) * NYp + y + Hy) * NZp + z + H];
+    }
+    return 0;
+}
+
 // Viscoelastic adjoint over the plan of forward_save_visco.
 struct ViscoAdj {
     seis_model *m;
@@ -1341,6 +1372,7 @@ struct ViscoAdj {
     int tmin, serial;
     bool half, prof;
     size_t k = 0;               // adjoint steps done (event slots)
+    const MovieOut *mv = nullptr;
 
     // level-l segment [t0, t1), whose starting state is current
     int segment(int l, int t0, int t1) {
@@ -1372,6 +1404,9 @@ struct ViscoAdj {
         hipStream_t st = m->ctx->stream;
         const int NT = c.NT, nout = (int)m->outn.size();
         for (int t = t0; t < t1; ++t) {
+            // movie frame: the recomputed forward state at the start of step
+            // t, what backpropagation reconstructs after the adjoint step t
+            if (t > tmin && movie_snap(m, mv, t)) return 1;
             step_sources(m, t, 1, st);
             char *dst = (char *)G.hist + (size_t)(t - t0) * G.nvb * fb;
             for (size_t q = 0; q < hf.size(); ++q)
@@ -1399,7 +1434,7 @@ struct ViscoAdj {
     }
 };
 
-int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial) {
+static int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial, const MovieOut *mv) {
     auto &G = m->grad;
     const CkptPlan &P = G.plan;
     const int tmax = tmax_of(m), tmin = m->c.tmin;
@@ -1415,6 +1450,7 @@ int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial) {
     V.serial = serial;
     V.half = m->prec == P_FP16;
     V.prof = m->profiling;
+    V.mv = mv;
     A.Ms = G.Ms;
     A.mus = G.mus;
     // with profiling on, events around the two adjoint kernels of every step
@@ -1522,8 +1558,8 @@ int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos, float *r
     return 0;
 }
 
-int seis_adjoint(seis_model *m, const float *res) {
-    g_err.clear();
+namespace {
+static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv) {
     auto &G = m->grad;
     if (!G.saved) return fail("seis_adjoint: no forward run with saved boundaries is bound");
     const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
@@ -1578,7 +1614,7 @@ int seis_adjoint(seis_model *m, const float *res) {
         return fail("seis_adjoint: the adjoint free surface has no moduli on the device");
     const int serial = receivers_share_cells(m, m->rcell_host);
     if (G.visco) {
-        const int r = adjoint_visco(m, A, serial);
+        const int r = adjoint_visco(m, A, serial, mv);
         G.saved = false;
         return r;
     }
@@ -1604,6 +1640,7 @@ int seis_adjoint(seis_model *m, const float *res) {
         launch_adj_v(A, m->nd_case, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
         step_sources(m, t, -1, st);
+        if (movie_snap(m, mv, t)) return 1;
     }
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipGetLastError());
@@ -1624,9 +1661,35 @@ int seis_adjoint(seis_model *m, const float *res) {
     return 0;
 }
 
+}  // namespace
+
+int seis_adjoint(seis_model *m, const float *res) {
+    g_err.clear();
+    return adjoint_run(m, res, nullptr);
+}
+
+int seis_adjoint_movie(seis_model *m, const float *res, int movout, int nvar,
+                       const char *const *vars, float *frames) {
+    g_err.clear();
+    if (movout <= 0 || movout > m->c.NT) return fail("seis_adjoint_movie: movout outside [1, NT]");
+    if (nvar <= 0 || !vars || !frames) return fail("seis_adjoint_movie: no movie variables");
+    for (int v = 0; v < nvar; ++v)
+        if (!vars[v] || !field_ptr(m, vars[v]))
+            return fail(std::string("seis_adjoint_movie: no wavefield named ") +
+                        (vars[v] ? vars[v] : "(null)"));
+    const size_t nf = (size_t)nvar * (m->c.NT / movout) * m->g.NZ * m->g.NY * m->g.NX;
+    std::fill(frames, frames + nf, 0.0f);
+    MovieOut mv;
+    mv.movout = movout;
+    mv.nvar = nvar;
+    mv.vars = vars;
+    mv.frames = frames;
+    return adjoint_run(m, res, &mv);
+}
+
 namespace {
 // Device accumulators (padded layout, double) -> parameter layout on the host.
-int accum_read(seis_model *m, const double *const *src, int k, std::vector<double> *acc) {
+static int accum_read(seis_model *m, const double *const *src, int k, std::vector<double> *acc) {
     const Layout &g = m->g;
     const size_t nf = (size_t)g.nfield, n = (size_t)g.NZ * g.NY * g.NX;
     std::vector<double> tmp(nf);

@@ -276,14 +276,35 @@ class Model:
                                          rp.shape[0], fptr(rp), fptr(out)))
         return [np.ascontiguousarray(o.T) for o in out]
 
-    def adjoint(self, res, scale: bool = True):
+    def adjoint(self, res, scale: bool = True, movout: int = 0, movie_vars=None):
         """Adjoint run of the shot of the last forward_save with adjoint
-        sources res (list of (NT, nrec)); res_scale is applied when scale."""
+        sources res (list of (NT, nrec)); res_scale is applied when scale.
+
+        movout > 0 records the movie of the gradient run (seis_adjoint_movie)
+        for the wavefields movie_vars and returns {name: array [NZ, (NY,) NX,
+        NT // movout]}: frame (t+1)//movout - 1 holds the backpropagated
+        forward wavefield after the adjoint step t, t % movout == 0."""
         rp = self._shot[2]
         r = self._pack(res, rp.shape[0])
         if scale:
             check(self.lib.seis_res_scale(self.handle, rp.shape[0], fptr(rp), fptr(r)))
-        check(self.lib.seis_adjoint(self.handle, fptr(r)))
+        if movout <= 0:
+            check(self.lib.seis_adjoint(self.handle, fptr(r)))
+            return None
+        names = list(movie_vars or [])
+        nfr = int(self.csts.NT) // int(movout)
+        n = int(np.prod(self.shape))
+        frames = np.zeros(len(names) * max(nfr, 1) * n, dtype=np.float32)
+        cn = (ctypes.c_char_p * max(len(names), 1))(*[x.encode() for x in names])
+        check(self.lib.seis_adjoint_movie(self.handle, fptr(r), int(movout), len(names), cn,
+                                          fptr(frames)))
+        out = {}
+        for v, name in enumerate(names):
+            fr = [from_c_layout(frames[(v * nfr + f) * n:(v * nfr + f + 1) * n], self.shape)
+                  for f in range(nfr)]
+            out[name] = np.stack(fr, axis=-1) if fr else np.zeros(tuple(self.shape) + (0,),
+                                                                  np.float32)
+        return out
 
     def res_scale(self, res, rec_pos):
         """res_scale (src/residuals.c:452-570) of adjoint sources res (list of

@@ -319,9 +319,6 @@ class SeisCL:
         rms = rmsnorm = 0.0
         grad = self.gradout == 1
         misfit = not self.inputres and (grad or self.rmsout or self.resout)
-        if self.movout > 0 and grad:
-            m.close()
-            raise SeisCLError("movie output (movout) is available for forward runs")
         nshot = len(set(self.src_pos[3, :].astype(int)))
         nframe = int(self.NT) // int(self.movout) if self.movout > 0 else 0
         # movie variables: output wavefields (the elastic pressure is derived
@@ -342,7 +339,14 @@ class SeisCL:
             scol = self.src_pos[3, :].astype(int) == sid
             rcol = np.where(self.rec_pos[3, :].astype(int) == sid)[0]
             shot = (self.src_pos[:, scol], self.src[:, scol], self.rec_pos[:, rcol])
-            if grad and self.inputres:
+            if grad and self.inputres and mov:
+                # the given adjoint sources, scaled, and the gradient run's movie
+                o = m.forward_save(*shot)
+                r = m.res_scale([r[:, rcol] for r in self._res_in], shot[2])
+                self._movie_frames(m.adjoint(r, scale=False, movout=int(self.movout),
+                                             movie_vars=list(mov)), mov, len(mov_shots))
+                mov_shots.append(sid)
+            elif grad and self.inputres:
                 o, _ = m.gradient(*shot, res=[r[:, rcol] for r in self._res_in])
             elif grad:
                 o = m.forward_save(*shot)
@@ -359,7 +363,12 @@ class SeisCL:
                 rms += r1
                 rmsnorm += r2
                 r = m.res_scale(r, shot[2])
-                if grad:
+                if grad and mov:
+                    # movie of the gradient run (src/time_stepping.c:926-929)
+                    self._movie_frames(m.adjoint(r, scale=False, movout=int(self.movout),
+                                                 movie_vars=list(mov)), mov, len(mov_shots))
+                    mov_shots.append(sid)
+                elif grad:
                     m.adjoint(r, scale=False)
                 for n, a in zip(names, r):
                     res[n][:, rcol] = a
@@ -374,6 +383,12 @@ class SeisCL:
                 self._results["H"] = {n: h[n] for n in h}
         m.close()
         return ""
+
+    @staticmethod
+    def _movie_frames(frames, mov, k):
+        """Frames of shot k from Model.adjoint's movie into the movie arrays."""
+        for n, a in mov.items():
+            a[..., k] = frames[n]
 
     def _forward_movie(self, m, shot, mov, k):
         """Forward shot k with a snapshot of each movie variable after every

@@ -119,6 +119,39 @@ def test_inputres_files(oracle_lib, tmp_path):
         assert rel_l2(g, _cropped(tot[name], seis.nab)) <= 1e-5, name
 
 
+@pytest.mark.parametrize("files", [True, False])
+def test_gradient_movie(oracle_lib, tmp_path, files):
+    """movout with gradout (src/time_stepping.c:926-929): the movie holds the
+    backpropagated forward wavefields of the gradient run, frame
+    (t+1)/movout - 1 after the adjoint step t; equal to the oracle's frames
+    bit for bit, shot by shot, through the files (SeisCL_MPI) and in
+    process; the gradient is unchanged by the recording."""
+    seis, model, _, csts = _fwi_case(tmp_path, movout=50, files=files)
+    seis.prepare_data([0, 1])
+    rng = np.random.default_rng(5)
+    res = [rng.standard_normal((220, seis.rec_pos.shape[1])).astype(np.float32) for _ in range(2)]
+    print(f"[cli] gradient movie files={files}", flush=True)
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.set_backward(res)
+    seis.execute()
+    gout = seis.read_grad()
+    movie = seis.read_movie()
+    assert [m.shape for m in movie] == [(60, 90, 220 // 50, 2)] * 2
+    tot = {k: 0 for k in seis.params}
+    for k, (rc, shot) in enumerate(_shots(seis)):
+        g = oracle_lib.gradient(model, *shot, res=[r[:, rc] for r in res], movout=50, **csts)
+        for name, mov in zip(("vx", "vz"), movie):
+            ref = g["movie"][name]
+            assert np.abs(ref).max() > 0
+            for f in range(220 // 50):
+                np.testing.assert_array_equal(mov[..., f, k], ref[f], err_msg=f"{name} {f}")
+        for p in tot:
+            tot[p] = tot[p] + g["grad"][p].astype(np.float64)
+    for name, gr in zip(seis.params, gout):
+        assert rel_l2(gr, _cropped(tot[name], seis.nab)) <= 1e-5, name
+    seis.close()
+
+
 MISFITS = [dict(), dict(scalermsnorm=1), dict(scaleshot=1, fmin=4.0, fmax=30.0),
            dict(scalerms=1), dict(restype=1)]
 

@@ -122,6 +122,60 @@ def test_adjoint_sources_given(oracle_lib, gpu_ctx, ND):
         np.testing.assert_array_equal(g1[k], g2[k])
 
 
+# (case, tight): tight runs the viscoelastic case under a checkpoint plan
+# with one more level
+MOVIE = [(dict(ND=2, FDORDER=4, abs_type=1), False), (dict(ND=3, FDORDER=4, freesurf=1), False),
+         (dict(ND=2, FDORDER=4, L=2, abs_type=1), False),
+         (dict(ND=2, FDORDER=4, L=2, abs_type=1), True),
+         (dict(ND=3, FDORDER=4, L=1, freesurf=1), False),
+         (dict(ND=3, FDORDER=4, L=1, freesurf=1), True)]
+
+
+@pytest.mark.parametrize("case,tight", MOVIE, ids=lambda v: _ids(v) if isinstance(v, dict) else
+                         ("tight" if v else "default"))
+def test_gradient_movie(oracle_lib, gpu_ctx, case, tight):
+    """Movie of a gradient run (seis_adjoint_movie, src/time_stepping.c:
+    926-929): every frame equals the oracle's bit for bit -- backpropagated
+    forward fields (elastic) or the recomputed forward state (viscoelastic,
+    also under a checkpoint plan with one more level) -- and the gradient
+    equals the one of the run without movie."""
+    from seiscl_amd import Model
+    csts, params, (sp, s, rp), _ = _setup(case)
+    mv = 40
+    print(f"[grad movie] case={_ids(case)} tight={tight}", flush=True)
+    rng = np.random.default_rng(8)
+    nout = 3 if case["ND"] == 3 else 2
+    res = [rng.standard_normal((csts["NT"], rp.shape[1])).astype(np.float32)
+           for _ in range(nout)]
+    ref = oracle_lib.gradient(params, sp, s, rp, res=res, movout=mv, **csts)
+    m = Model(gpu_ctx, params, **csts)
+    names = list(ref["movie"])
+    m.grad_zero()
+    m.forward_save(sp, s, rp)
+    if tight:
+        plan = m.grad_plan()
+        m.set_grad_memory(plan["bytes"] - 1)
+        m.forward_save(sp, s, rp)
+        assert m.grad_plan()["levels"] > plan["levels"]
+    mov = m.adjoint(res, movout=mv, movie_vars=names)
+    g1 = m.grad_read()
+    m.grad_zero()
+    m.forward_save(sp, s, rp)
+    m.adjoint(res)
+    g2 = m.grad_read()
+    m.close()
+    nfr = csts["NT"] // mv
+    for name in names:
+        a = mov[name]
+        b = np.moveaxis(ref["movie"][name], 0, -1)
+        assert a.shape == tuple(csts["N"]) + (nfr,)
+        assert np.abs(b).max() > 0, name
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    for k in g1:
+        np.testing.assert_array_equal(g1[k], g2[k])
+        assert rel_l2(g1[k], ref["grad"][k]) <= TOL, k
+
+
 def test_gradient_accumulates_over_shots(oracle_lib, gpu_ctx):
     from seiscl_amd import Model
     csts, params, (sp, s, rp), true = _setup(dict(ND=2, FDORDER=4))

@@ -326,3 +326,36 @@ def test_rtm_image_and_hessian(oracle_lib, ND):
     np.testing.assert_array_equal(fwi["hess"]["vs"], other["hess"]["vs"])
     bigh = hr > 1e-3 * hr.max()
     np.testing.assert_allclose(hf[bigh], c1[bigh] * hr[bigh], rtol=2e-4)
+
+
+@pytest.mark.parametrize("L", [0, 1])
+def test_gradient_movie_frames(oracle_lib, L):
+    """Movie of a gradient run (src/time_stepping.c:926-929): after the
+    adjoint step t = (f+1)*movout, frame f holds the backpropagated forward
+    field, the forward field after (f+1)*movout steps -- the forward movie's
+    frame f (src/time_stepping.c:819-822).  Elastic runs reconstruct it to
+    backpropagation accuracy; viscoelastic runs, whose adjoint reads the
+    recomputed forward state, record it exactly.  The last frame's step
+    t = NT lies outside the adjoint loop (t <= tmax-1) and stays zero."""
+    if L:
+        csts, p, (sp, s, rp), _ = _visco_case(2, L, 50.0)
+    else:
+        csts, p, (sp, s, rp), _ = _smooth_case(2)
+    NT, mv = csts["NT"], 50
+    r = oracle_lib.gradient(p, sp, s, rp, res=[np.zeros((NT, rp.shape[1]), np.float32)] * 2,
+                            movout=mv, **csts)
+    H, nab = csts["FDORDER"] // 2, csts["nab"]
+    inner = tuple(slice(nab, -nab) for _ in range(2))
+    for name, frames in r["movie"].items():
+        assert frames.shape == (NT // mv,) + tuple(csts["N"])
+        assert np.abs(frames[-1]).max() == 0, name
+        for f in range(NT // mv - 1):
+            nt = (f + 1) * mv
+            _, fl = oracle_lib.forward(p, sp, s[:nt], rp, fields=True, **dict(csts, NT=nt))
+            ref = fl[name][H:-H, H:-H]
+            assert np.abs(ref).max() > 0
+            if L:
+                np.testing.assert_array_equal(frames[f], ref)
+            else:
+                err = np.abs(frames[f][inner] - ref[inner]).max()
+                assert err <= 1e-4 * np.abs(ref[inner]).max(), (name, f, err)
