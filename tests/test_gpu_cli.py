@@ -7,7 +7,8 @@ checked shot by shot against the oracle:
 - gradient files (grad<name>) and the rms file for a two-shot FWI round with
   observed data;
 - the inputres mode (adjoint sources from the residual file);
-- resout: the scaled adjoint sources the reference writes as <var>res.
+- resout: the scaled adjoint sources the reference writes as <var>res;
+- movies (movout) of forward and of gradient runs.
 """
 import os
 
