@@ -59,6 +59,10 @@ def load():
         lib.or_forward.argtypes = [ctypes.POINTER(OrCsts), _FP, _FP, _FP, _FP, _FP,
                                    ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
                                    ctypes.c_char_p, ctypes.c_int]
+        lib.or_forward_from.restype = ctypes.c_int
+        lib.or_forward_from.argtypes = [ctypes.POINTER(OrCsts), _FP, _FP, _FP, _FP, _FP,
+                                        ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
+                                        _FP, ctypes.c_char_p, ctypes.c_int]
         lib.or_last_loop_seconds.restype = ctypes.c_double
         lib.or_last_loop_seconds.argtypes = []
         lib.or_holberg.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -172,11 +176,13 @@ def _params(params, c: OrCsts):
 
 
 def forward(params: dict, src_pos, src, rec_pos, nthreads: int = 0, fields: bool = False,
-            **csts):
+            fields_in: dict | None = None, **csts):
     """Oracle forward modeling of one shot.
 
     Returns a list of (NT, nrec) seismograms in out_names order, plus a dict of
-    final padded wavefields ([NZp, (NYp,) NXp]) when fields=True."""
+    final padded wavefields ([NZp, (NYp,) NXp]) when fields=True.  fields_in
+    (same layout, every wavefield named) starts the run from those fields
+    instead of zeros."""
     lib = load()
     c = make_csts(nthreads=nthreads, **csts)
     p0, p1, rho, taup, taus = _params(params, c)
@@ -190,10 +196,14 @@ def forward(params: dict, src_pos, src, rec_pos, nthreads: int = 0, fields: bool
     pshape = [n + 2 * H for n in shape]
     nvars = lib.or_nvars(ctypes.byref(c))
     fo = np.zeros(nvars * int(np.prod(pshape)), np.float32) if fields else None
+    fi = None
+    if fields_in is not None:
+        fi = np.concatenate([np.ascontiguousarray(np.asarray(fields_in[name], np.float32).T).ravel()
+                             for name in var_names(c)])
     err = ctypes.create_string_buffer(256)
-    st = lib.or_forward(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
-                        sp.shape[0], fp(sp), fp(s), rp.shape[0], fp(rp), fp(out), fp(fo),
-                        err, 256)
+    st = lib.or_forward_from(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
+                             sp.shape[0], fp(sp), fp(s), rp.shape[0], fp(rp), fp(fi), fp(out),
+                             fp(fo), err, 256)
     if st:
         raise RuntimeError("oracle: " + err.value.decode())
     seis = [np.ascontiguousarray(o.T) for o in out]

@@ -1695,6 +1695,19 @@ int or_forward(const or_csts *c, const float *p0, const float *p1,
                int nsrc, const float *src_pos, const float *src, int nrec,
                const float *rec_pos, float *out, float *fields_out, char *err,
                int errlen) {
+    return or_forward_from(c, p0, p1, rho, taup, taus, nsrc, src_pos, src, nrec, rec_pos,
+                           NULL, out, fields_out, err, errlen);
+}
+
+/* The same loop started from given wavefields (or_nvars padded fields in   */
+/* or_varname order) instead of zeros; memory variables and CPML memory     */
+/* start at zero.  Test use: a 3D run on fields copied along y from a 2D    */
+/* state must reproduce the 2D run away from the y edges.                  */
+int or_forward_from(const or_csts *c, const float *p0, const float *p1,
+                    const float *rho, const float *taup, const float *taus,
+                    int nsrc, const float *src_pos, const float *src, int nrec,
+                    const float *rec_pos, const float *fields_in, float *out,
+                    float *fields_out, char *err, int errlen) {
     sim S;
     char dummy[8];
     if (!err) {
@@ -1740,6 +1753,11 @@ int or_forward(const or_csts *c, const float *p0, const float *p1,
         return 1;
     }
     memset(out, 0, sizeof(float) * (size_t)or_nout(c) * nrec * c->NT);
+    if (fields_in) {
+        for (int v = 0; v < or_nvars(c); v++)
+            memcpy(named_var(&S, or_varname(c, v)), &fields_in[v * S.nv],
+                   S.nv * sizeof(float));
+    }
     double t_start = wall_seconds();
     for (int t = 0; t < tmax; t++) {
         sources(&S, t, 1, nsrc, src_pos, src);

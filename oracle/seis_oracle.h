@@ -104,6 +104,13 @@ int  or_forward(const or_csts *c, const float *p0, const float *p1,
                 int nsrc, const float *src_pos, const float *src,
                 int nrec, const float *rec_pos, float *out,
                 float *fields_out, char *err, int errlen);
+/* or_forward started from the padded wavefields fields_in (or_nvars arrays */
+/* in or_varname order); memory variables and CPML memory start at zero.    */
+int  or_forward_from(const or_csts *c, const float *p0, const float *p1,
+                     const float *rho, const float *taup, const float *taus,
+                     int nsrc, const float *src_pos, const float *src, int nrec,
+                     const float *rec_pos, const float *fields_in, float *out,
+                     float *fields_out, char *err, int errlen);
 
 /* wall-clock seconds of the time loop of the last or_forward call */
 double or_last_loop_seconds(void);
