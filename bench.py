@@ -7,20 +7,31 @@ smooth perturbations, 12 Hz Ricker explosion, 64 x 64 surface receivers),
 FP32.  A "step" is one time step of the finite-difference loop over the
 whole grid: sources, velocity update, stress update, free surface and
 receiver sampling.  W warm-up steps run untimed, then K steps are timed with
-the wavefields resident in HBM.
+the wavefields resident in HBM; the shot then runs on to its last step
+(NT = 1500) untimed, so that its seismograms can be compared.
 
 value = grid cells x K / timed seconds  (Gcell-updates/s).
 roofline: the stress update (the dominant kernel), algorithmic bytes per
 launch = 196 B per cell (3 velocities read, 6 stresses and 12 memory
 variables read-modify-write, 10 coefficient arrays read) / its average
 duration from HIP events on the compute stream.
-roofline.traffic: HBM bytes per stress-update launch from rocprofv3 counters
-(FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 512 MiB copy and fill).
+roofline.traffic: HBM bytes per stress-update launch, measured by two
+rocprofv3 --pmc child runs of this script (FETCH_SIZE, then WRITE_SIZE),
+each scaled by what the counter reads for a 512 MiB device copy in the same
+run; they run before this process touches the GPU (--no-traffic skips
+them).
 cpu_baseline: the C oracle on the host cores, same grid, a bounded number of
 steps.
+config.elastic: the same shot without attenuation (L=0), the 3D elastic
+case of the metric, timed the same way with its own roofline fractions.
 config.fp16_storage / config.fp64: the same shot run with FP16 wavefield and
 memory-variable storage (FP32 arithmetic) and in FP64; seismogram errors are
-relative L2 norms against the FP64 run (skip with --no-variants).
+relative L2 norms over the whole shot against the FP64 run (skip with
+--no-variants).
+config.serving: a bounded run of the serving workload (64 seeded 2D
+requests): capacity of the plain and batched paths and a Poisson load at
+the plain path's capacity, shots/h with p50/p99 latency (--no-serving skips
+it; --workload serving runs the full 256 requests).
 """
 from __future__ import annotations
 
@@ -40,20 +51,86 @@ PEAK_HBM_GBS = 8000.0
 BYTES_PER_CELL_S = 196       # stress update, 3D visco L=2 FP32
 BYTES_PER_CELL_V = 60        # velocity update, 3D FP32
 BYTES_PER_CELL_FP16 = 154    # whole step, FP16 fields and memory, FP32 coefficients
-# HBM bytes per stress-update launch at 256^3, from the FETCH_SIZE / WRITE_SIZE
-# rocprofv3 passes summarised in this file (the command is written there)
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "update_s_v_visco3d_256_hbm_counters.md")
+BYTES_PER_CELL_S_EL = 80   # stress update, 3D elastic FP32
+BYTES_PER_CELL_EL = 140     # whole step, 3D elastic FP32
+COPY_BYTES = 512 * 2**20    # calibration copy of the HBM counters
 
 
-def measured_traffic(n):
-    """HBM bytes per k_s3d launch from the committed counter summary (256^3 only)."""
-    if n != 256 or not os.path.exists(TRAFFIC_FILE):
-        return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_launch k_s3d:"):
-            return int(line.split(":")[1])
-    return None
+def traffic_probe(args):
+    """Child run under rocprofv3 --pmc: a 512 MiB device copy (counter
+    calibration), then a few steps of the benchmark shot."""
+    import torch
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    log("probe: calibration copy done")
+    from seiscl_amd import Context, Model
+    from seiscl_amd.workloads import config2
+    csts, params, shots = config2(n=args.n, NT=12, elastic=args.traffic_probe == "elastic")
+    ctx = Context(0)
+    m = Model(ctx, params, **csts)
+    m.bind_shot(*shots[0])
+    m.run_steps(0, 12)
+    ctx.sync()
+    m.close()
+    ctx.close()
+    log("probe: 12 steps done")
+
+
+def measure_traffic(n, elastic=False, timeout=170):
+    """HBM bytes per launch of the stress and velocity updates from two
+    rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE), each counter scaled
+    by its reading for the 512 MiB copy of the same run.  Returns
+    ({kernel: bytes}, note) or (None, reason)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="seis_pmc_")
+        cmd = [exe, "--pmc", counter, "-f", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--traffic-probe", "elastic" if elastic else "visco",
+               "--n", str(n)]
+        log(f"traffic: {counter} pass ({'elastic' if elastic else 'viscoelastic'})")
+        try:
+            r = subprocess.run(cmd, stdout=sys.stderr, stderr=sys.stderr, timeout=timeout,
+                               cwd=ROOT, env=dict(os.environ, TMPDIR="/tmp"))
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 {counter} pass timed out"
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 and not files:
+            return None, f"rocprofv3 {counter} pass failed (rc {r.returncode})"
+        per = {}
+        for f in files:
+            for row in csv.DictReader(open(f)):
+                if row.get("Counter_Name") != counter:
+                    continue
+                per.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [v for k, vals in per.items() if "copyBuffer" in k for v in vals]
+        copy = [v for v in copy if v > 0.25 * COPY_BYTES / 1024]
+        if not copy:
+            return None, f"no calibration copy in the {counter} pass"
+        factor = COPY_BYTES / (1024.0 * max(copy))
+        for name, tag in (("k_s3d", "update_s"), ("k_v3d", "update_v")):
+            vals = [v for k, vs in per.items() if name in k for v in vs]
+            if vals:
+                res.setdefault(tag, {})[counter] = (1024.0 * sum(vals) / len(vals) * factor,
+                                                     factor)
+    out = {k: int(v["FETCH_SIZE"][0] + v["WRITE_SIZE"][0]) for k, v in res.items()
+           if "FETCH_SIZE" in v and "WRITE_SIZE" in v}
+    if "update_s" not in out:
+        return None, "stress-update counters missing"
+    f_fetch = next(iter(res.values()))["FETCH_SIZE"][1]
+    f_write = next(iter(res.values()))["WRITE_SIZE"][1]
+    return out, f"FETCH_SIZE x {f_fetch:.3f} + WRITE_SIZE x {f_write:.3f} (512 MiB copy)"
 
 
 def log(*a):
@@ -434,6 +511,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16-storage and FP64 runs of the same shot")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="forward3d: skip the bounded serving measurement")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="forward3d: skip the rocprofv3 HBM counter passes")
+    ap.add_argument("--traffic-probe", choices=["visco", "elastic"], default=None,
+                    help=argparse.SUPPRESS)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 512 MiB device-to-device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -457,29 +540,70 @@ def main():
     if args.workload == "forward2d":
         return forward2d_main(args)
     args.n = args.n or 256
+    if args.traffic_probe:
+        return traffic_probe(args)
+    forward3d_main(args)
+
+
+def serving_sample(ctx, nreq=64, slots=48):
+    """Bounded serving measurement for the default line: capacity of the
+    plain and batched paths (all requests queued at t=0) and a Poisson load
+    at the plain path's capacity."""
+    from seiscl_amd.serving import (make_requests, poisson_arrivals, serve_batched,
+                                    serve_plain, summarize)
+    from seiscl_amd.workloads import config4
+    work = config4(nreq)
+    serve_plain(ctx, make_requests(work[:4]))
+    cap_p = summarize(serve_plain(ctx, make_requests(work)))
+    cap_b = summarize(serve_batched(ctx, make_requests(work), max_slots=slots))
+    arr = poisson_arrivals(len(work), cap_p["shots_per_hour"] / 3600.0)
+    ld_p = summarize(serve_plain(ctx, make_requests(work, arr)))
+    ld_b = summarize(serve_batched(ctx, make_requests(work, arr), max_slots=slots))
+    keep = ("shots_per_hour", "p50_latency_s", "p99_latency_s", "span_s")
+    pick = lambda d: {k: round(d[k], 4) for k in keep if k in d}   # noqa: E731
+    log(f"serving ({nreq} requests): capacity plain {cap_p['shots_per_hour']:.0f}/h, batched "
+        f"{cap_b['shots_per_hour']:.0f}/h; 1x load p99 plain {ld_p['p99_latency_s']:.2f} s, "
+        f"batched {ld_b['p99_latency_s']:.2f} s")
+    return {
+        "workload": (f"{nreq} seeded 2D acoustic/elastic modeling requests, NZ 100-600, "
+                     "NX 200-1500, NT 1000-5000, FD 4/8, FP32"),
+        "capacity_plain": pick(cap_p), "capacity_batched": pick(cap_b),
+        "speedup_capacity": round(cap_b["shots_per_hour"] / cap_p["shots_per_hour"], 3),
+        "load_1x_plain": pick(ld_p), "load_1x_batched": pick(ld_b),
+    }
+
+
+def forward3d_main(args):
+    from seiscl_amd.workloads import config2
+    W, K = args.warmup, args.steps
+    NT = max(1500, W + K)
+    traffic, traffic_el, tnote = None, None, "skipped (--no-traffic)"
+    if not args.no_traffic:
+        # before this process touches the GPU: the probes are child processes
+        traffic, tnote = measure_traffic(args.n)
+        if traffic is not None:
+            traffic_el, _ = measure_traffic(args.n, elastic=True)
+        log(f"traffic: {traffic} {traffic_el} ({tnote})")
 
     from seiscl_amd import Context, Model
-    from seiscl_amd.workloads import config2
-
     if args.calib_copy:
         import torch
-        x = torch.ones(128 * 2**20, dtype=torch.float32, device="cuda:0")
+        x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
         y = torch.empty_like(x)
         for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
         log("calibration copy: 3 x 512 MiB read + 512 MiB write")
 
-    W, K = args.warmup, args.steps
-    log(f"building workload n={args.n} NT={W + K}")
-    csts, params, shots = config2(n=args.n, NT=W + K)
+    log(f"building workload n={args.n} NT={NT}")
+    csts, params, shots = config2(n=args.n, NT=NT)
     ctx = Context(0)
     ncell = int(np.prod(csts["N"]))
 
-    def run(fp16):
+    def run(fp16, c=csts, p=params, full=True):
         t0 = time.time()
-        m = Model(ctx, params, **dict(csts, FP16=fp16))
-        log(f"FP16={fp16}: model resident in {time.time() - t0:.1f} s, "
+        m = Model(ctx, p, **dict(c, FP16=fp16))
+        log(f"FP16={fp16} L={c['L']}: model resident in {time.time() - t0:.1f} s, "
             f"{m.footprint() / 2**30:.2f} GiB")
         m.bind_shot(*shots[0])
         m.run_steps(0, W)                       # warm-up, untimed
@@ -490,24 +614,48 @@ def main():
         ctx.sync()
         wall = time.perf_counter() - t0
         tm = m.timing()
-        seis = m.read_shot()
+        m.set_profiling(False)
+        seis = None
+        if full and W + K < NT:
+            m.run_steps(W + K, NT - W - K)      # the rest of the shot, untimed
+        if full:
+            seis = m.read_shot()
+            if not all(np.isfinite(x).all() for x in seis):
+                raise RuntimeError(f"non-finite seismograms (FP16={fp16})")
         m.close()
-        if not all(np.isfinite(x).all() for x in seis):
-            raise RuntimeError(f"non-finite seismograms (FP16={fp16})")
         avg_s, avg_v = tm["update_s_ms"] / K, tm["update_v_ms"] / K
-        log(f"FP16={fp16}: timed {K} steps: wall {wall:.3f} s, loop(events) "
+        log(f"FP16={fp16} L={c['L']}: timed {K} steps: wall {wall:.3f} s, loop(events) "
             f"{tm['loop_ms'] / 1e3:.3f} s, update_s {avg_s:.3f} ms, update_v {avg_v:.3f} ms")
         return dict(wall=wall, avg_s=avg_s, avg_v=avg_v, seis=seis,
                     gcell=ncell * K / wall / 1e9)
 
-    main_run = run(0)
+    main_run = run(0, full=not args.no_variants)
     wall, avg_s, avg_v = main_run["wall"], main_run["avg_s"], main_run["avg_v"]
     value = main_run["gcell"]
     achieved = BYTES_PER_CELL_S * ncell / (avg_s / 1e3) / 1e9
+
+    # the metric's 3D elastic case: the same shot and model without attenuation
+    ce, pe, _ = config2(n=args.n, NT=NT, elastic=True)
+    el = run(0, ce, pe, full=False)
+    el_s = BYTES_PER_CELL_S_EL * ncell / (el["avg_s"] / 1e3) / 1e9
+    el_v = BYTES_PER_CELL_V * ncell / (el["avg_v"] / 1e3) / 1e9
+    el_step = BYTES_PER_CELL_EL * ncell / (el["wall"] / K) / 1e9
+    elastic = {
+        "workload": "the same shot without attenuation (L=0): 3D elastic, 256^3, FD4",
+        "gcell_s": round(el["gcell"], 4), "ms_per_step": round(el["wall"] / K * 1e3, 5),
+        "update_s_ms": round(el["avg_s"], 5), "update_v_ms": round(el["avg_v"], 5),
+        "update_s_frac": round(el_s / PEAK_HBM_GBS, 4),
+        "update_v_frac": round(el_v / PEAK_HBM_GBS, 4),
+        "step_frac": round(el_step / PEAK_HBM_GBS, 4),
+        "bytes_per_cell": BYTES_PER_CELL_EL,
+        "traffic": traffic_el,
+    }
+    log(f"elastic: {elastic['gcell_s']} Gcell/s, fractions update_s {elastic['update_s_frac']}, "
+        f"update_v {elastic['update_v_frac']}, step {elastic['step_frac']}")
     variants = {}
     if not args.no_variants:
         # the same shot with FP16 wavefield storage and in FP64; seismogram
-        # error (relative L2 over all traces) against the FP64 run
+        # error (relative L2 over all traces of the whole shot) against FP64
         r16, r64 = run(2), run(64)
 
         def err(a, b):
@@ -516,6 +664,7 @@ def main():
             return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
         variants = {
+            "accuracy_window": f"all {NT} steps of the shot",
             "fp32_rel_l2_vs_fp64": err(main_run["seis"], r64["seis"]),
             "fp16_storage": {"gcell_s": round(r16["gcell"], 4),
                              "ms_per_step": round(r16["wall"] / K * 1e3, 5),
@@ -526,8 +675,11 @@ def main():
             "fp64": {"gcell_s": round(r64["gcell"], 4),
                      "ms_per_step": round(r64["wall"] / K * 1e3, 5)},
         }
-        log(f"accuracy vs FP64: FP32 {variants['fp32_rel_l2_vs_fp64']:.2e}, "
+        log(f"accuracy vs FP64 over {NT} steps: FP32 {variants['fp32_rel_l2_vs_fp64']:.2e}, "
             f"FP16 storage {variants['fp16_storage']['rel_l2_vs_fp64']:.2e}")
+    serving = None
+    if not args.no_serving:
+        serving = serving_sample(ctx)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -545,15 +697,20 @@ def main():
             "workload": (f"3D viscoelastic forward modeling, one shot, {args.n}^3 grid, "
                          "L=2, FD order 4, CPML 16, free surface, 64x64 receivers"),
             "grid": list(csts["N"].tolist()),
-            "fd_order": 4, "L": 2, "precision": "fp32",
+            "fd_order": 4, "L": 2, "precision": "fp32", "NT": NT,
             "update_v_ms": round(avg_v, 5), "update_s_ms": round(avg_s, 5),
             "update_v_gbs": round(BYTES_PER_CELL_V * ncell / (avg_v / 1e3) / 1e9, 1),
+            "traffic_update_v": (traffic or {}).get("update_v"),
+            "traffic_method": tnote,
+            "elastic": elastic,
             **variants,
+            "serving": serving,
         },
         "roofline": {
             "bound": "hbm", "kernel": "update_s (3D viscoelastic stress update)",
             "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(args.n),
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": (traffic or {}).get("update_s"),
         },
     }
     ctx.close()

@@ -140,7 +140,11 @@ static int read_csts(hid_t f, job_t *j) {
     s = s || read_optional(f, "/halfpar", SEIS_H5_I32, &j->halfpar);
     s = s || read_optional(f, "/tmax", SEIS_H5_F32, &tmaxf);
     s = s || read_optional(f, "/tmin", SEIS_H5_F32, &tminf);
+    /* gradient corrections (seis_csts.gradfix): absent -> every correction
+     * (seis_csts_default); 0 reproduces the reference's gradients */
+    s = s || read_optional(f, "/gradfix", SEIS_H5_I32, &c->gradfix);
     if (s) return 1;
+    if (c->gradfix < 0 || c->gradfix > 255) return err("gradfix=%d outside 0..255", c->gradfix);
     const int nd = c->ND == 3 ? 3 : 2;
     if (!seis_h5_exists(f, "/N")) return err("Variable /N is not defined");
     if (seis_h5_numel(f, "/N") != nd) return err("N must hold %d sizes for ND=%d", nd, c->ND);

@@ -270,6 +270,9 @@ class SeisCL:
             if v is None:
                 raise SeisCLError(f"{k} is not set: call set_forward first")
             vals[k] = v
+        # this engine's gradient corrections (absent from the reference's
+        # files; SeisCL_MPI reads them, 0 reproduces the reference)
+        vals["gradfix"] = int(self.gradfix)
         h5mat.savemat(self._path(workdir, self.file_csts if filename is None else filename),
                       vals)
 
@@ -397,15 +400,20 @@ class SeisCL:
         H = int(self.FDORDER) // 2
         inner = tuple(slice(H, -H) for _ in range(len(self.N)))
         nt = int(self.NT)
+        # the run stops at tmax, as seis_forward_shot and SeisCL_MPI's movie do
+        # (times in seconds -> steps in float arithmetic, src/read_hdf5.c:523-526)
+        tmax = int(np.float32(self.tmax) / np.float32(self.dt)) if self.tmax else nt
         t = 0
         for f in range(nt // int(self.movout)):
             t1 = (f + 1) * int(self.movout)
+            if t1 > tmax:
+                break
             m.run_steps(t, t1 - t)
             t = t1
             for n, a in mov.items():
                 a[..., f, k] = m.read_field(n)[inner]
-        if t < nt:
-            m.run_steps(t, nt - t)
+        if t < tmax:
+            m.run_steps(t, tmax - t)
         return m.read_shot()
 
     # ---- outputs (SeisCL.py:507-656) ---------------------------------------

@@ -83,11 +83,13 @@ def config1(NT: int = 2000, seed: int = 1):
     return csts, params, [(src_pos, src, rec_pos)]
 
 
-def config2(n: int = 256, NT: int = 1500, seed: int = 2, nrec_side: int = 64):
+def config2(n: int = 256, NT: int = 1500, seed: int = 2, nrec_side: int = 64,
+            elastic: bool = False):
     """Plain 3D viscoelastic, one shot: n^3, dh 10 m, dt 0.8 ms, FD4, CPML 16,
     free surface, L=2 (FL 5 and 25 Hz, Q=50), layered vp 1800-4500,
     vs = vp/1.8, rho 1800-2600, 3% smooth perturbations, 12 Hz Ricker
-    explosion, 64 x 64 surface receivers."""
+    explosion, 64 x 64 surface receivers.  elastic=True: the same shot and
+    model without attenuation (L=0)."""
     rng = np.random.default_rng(seed)
     N = (n, n, n)
     dh, dt = 10.0, 8e-4
@@ -115,6 +117,9 @@ def config2(n: int = 256, NT: int = 1500, seed: int = 2, nrec_side: int = 64):
                         0 * gx, 0 * gx])
     params = dict(vp=vp.astype(np.float32), vs=vs.astype(np.float32),
                   rho=rho.astype(np.float32), taup=taup, taus=taus)
+    if elastic:
+        csts["L"] = 0
+        del params["taup"], params["taus"]
     return csts, params, [(src_pos, src, rec_pos)]
 
 

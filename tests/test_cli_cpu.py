@@ -69,7 +69,9 @@ def test_files_written_like_seiscl_py(tmp_path):
     seis, model = _seis(tmp_path)
     seis.set_forward([0, 1], model, withgrad=False)
     csts = os.path.join(str(tmp_path), "SeisCL_csts.mat")
-    assert set(h5mat.names(csts)) == set(CSTS_KEYS)
+    # SeisCL.py's keys plus this engine's gradient corrections
+    assert set(h5mat.names(csts)) == set(CSTS_KEYS) | {"gradfix"}
+    assert int(h5mat.read(csts, "gradfix", np.int32)) == 223
     ns, ng = seis.src_pos.shape[1], seis.rec_pos.shape[1]
     assert h5mat.shape(csts, "src_pos") == (ns, 5)
     assert h5mat.shape(csts, "rec_pos") == (ng, 8)

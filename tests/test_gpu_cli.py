@@ -120,6 +120,35 @@ def test_inputres_files(oracle_lib, tmp_path):
         assert rel_l2(g, _cropped(tot[name], seis.nab)) <= 1e-5, name
 
 
+@pytest.mark.parametrize("gradfix", [0, 223])
+def test_gradfix_files(oracle_lib, tmp_path, gradfix):
+    """/gradfix in the constants file selects the gradient corrections at the
+    file boundary: 0 gives the reference's gradients (the oracle's reference
+    mode), 223 the corrected ones; the two differ."""
+    seis, model, true, csts = _fwi_case(tmp_path, gradfix=gradfix)
+    print(f"[cli] gradfix={gradfix} files", flush=True)
+    seis.set_forward([0, 1], true, withgrad=False)
+    seis.execute()
+    dobs = seis.read_data()
+    seis.write_data({"vx": dobs[0], "vz": dobs[1]})
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.execute()
+    gout = seis.read_grad()
+    tot = {k: 0 for k in seis.params}
+    other = {k: 0 for k in seis.params}
+    for rc, shot in _shots(seis):
+        din = [d[:, rc] for d in dobs]
+        g = oracle_lib.gradient(model, *shot, din=din, gradfix=gradfix, **csts)
+        o = oracle_lib.gradient(model, *shot, din=din, gradfix=223 - gradfix, **csts)
+        for k in tot:
+            tot[k] = tot[k] + g["grad"][k].astype(np.float64)
+            other[k] = other[k] + o["grad"][k].astype(np.float64)
+    for name, g in zip(seis.params, gout):
+        ref = _cropped(tot[name], seis.nab)
+        assert rel_l2(g, ref) <= 1e-5, name
+        assert rel_l2(_cropped(other[name], seis.nab), ref) > 1e-3, name
+
+
 @pytest.mark.parametrize("files", [True, False])
 def test_gradient_movie(oracle_lib, tmp_path, files):
     """movout with gradout (src/time_stepping.c:926-929): the movie holds the
@@ -235,3 +264,34 @@ def test_movie(oracle_lib, tmp_path, files):
                 assert np.abs(fl[name]).max() > 0
                 np.testing.assert_array_equal(mov[..., f, k], fl[name][2:-2, 2:-2])
     seis.close()
+
+
+def test_movie_tmax(tmp_path):
+    """With tmax < NT the forward movie stops at tmax: the in-process front
+    end and SeisCL_MPI return the same seismograms and frames (frames past
+    tmax stay zero)."""
+    from seiscl_amd.seiscl import SeisCL
+    from seiscl_amd.workloads import layered_model
+    N, NT, mv, dt = np.array([60, 90]), 200, 40, 1e-3
+    model = layered_model(tuple(N), np.random.default_rng(3), nlayers=3)
+    res = {}
+    for files in (True, False):
+        args = dict(N=N, ND=2, dh=10.0, dt=dt, NT=NT, FDORDER=4, nab=10, seisout=1, f0=15.0,
+                    movout=mv, tmax=130 * dt)
+        if files:
+            args.update(files=True, spawn=False, workdir=str(tmp_path))
+        seis = SeisCL(**args)
+        seis.surface_acquisition_2d(dg=3, ds=25)
+        print(f"[cli] movie tmax files={files}", flush=True)
+        seis.set_forward([0], model, withgrad=False)
+        seis.execute()
+        res[files] = (seis.read_data(), seis.read_movie())
+        seis.close()
+    (d1, m1), (d0, m0) = res[True], res[False]
+    for a, b in zip(d1, d0):
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(m1, m0):
+        np.testing.assert_array_equal(a, b)
+        assert np.abs(a[..., 130 // mv - 1, :]).max() > 0
+        assert np.all(a[..., 130 // mv:, :] == 0)
