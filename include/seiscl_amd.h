@@ -239,6 +239,15 @@ int seis_adjoint_movie(seis_model *m, const float *res, int movout, int nvar,
 /* Gradients [3][Pi N] in param_type order (vp|M|Ip, vs|mu|Is, rho), in the */
 /* parameter layout (transf_grad, src/calc_grad.c:975-1110).                 */
 int seis_grad_read(seis_model *m, float *grad);
+/* Gradients of tau_p and tau_s, [2][Pi N] in the parameter layout, of a    */
+/* viscoelastic model (L > 0, param_type 0-2), at fixed wave speeds (the    */
+/* other parameters of param_type held).  In the reference they exist only */
+/* in the DFT path (grad_coefvisc_0, src/calc_grad.c:111-152, written as    */
+/* gradtaup/gradtaus by src/writehdf5.c:261-283); its time-domain kernels   */
+/* declare and never write them (src/update_adjs3D.cl:66).  Here they are  */
+/* accumulated in double by the time-domain adjoint kernels from the whole */
+/* memory term of each step.                                                */
+int seis_grad_read_tau(seis_model *m, float *gtau);
 /* The approximate Hessian accumulated with the gradient when Hout=1,       */
 /* [3][Pi N] in the same order and transform (H<name> of the gradient file,  */
 /* src/writehdf5.c:275-283, src/calc_grad.c:1028-1094).                      */

@@ -81,7 +81,7 @@ def load():
         lib.or_gradient.argtypes = ([ctypes.POINTER(OrCsts)] + [_FP] * 5 +
                                     [ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
                                      ctypes.c_int, _FP, _FP, _FP,
-                                     ctypes.POINTER(ctypes.c_double), _FP, _FP,
+                                     ctypes.POINTER(ctypes.c_double), _FP, _FP, _FP,
                                      ctypes.c_int, _FP, ctypes.c_char_p, ctypes.c_int])
         lib.or_shell_check.restype = ctypes.c_int
         lib.or_shell_check.argtypes = [ctypes.POINTER(OrCsts), ctypes.POINTER(ctypes.c_long)]
@@ -325,13 +325,14 @@ def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FI
     nvars = lib.or_nvars(ctypes.byref(c))
     fo = np.zeros(nvars * nv, np.float32) if fields else None
     hess = np.zeros(3 * n, np.float32) if csts.get("Hout") else None
+    gtau = np.zeros(2 * n, np.float32) if c.L > 0 and c.param_type <= 2 else None
     nfr = c.NT // movout if movout > 0 else 0
     mov = np.zeros(nvars * nfr * nv, np.float32) if nfr else None
     err = ctypes.create_string_buffer(256)
     st = lib.or_gradient(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
                          sp.shape[0], fp(sp), fp(s), nrec, fp(rp), fp(d), fp(r),
                          int(gradfix), fp(out), fp(res_out), fp(grad), rms, fp(fo),
-                         fp(hess), int(movout), fp(mov), err, 256)
+                         fp(hess), fp(gtau), int(movout), fp(mov), err, 256)
     if st:
         raise RuntimeError("oracle: " + err.value.decode())
     names = {0: ("vp", "vs"), 1: ("M", "mu"), 2: ("Ip", "Is"), 3: ("vp", "vs")}[c.param_type]
@@ -340,6 +341,9 @@ def gradient(params: dict, src_pos, src, rec_pos, din=None, res=None, gradfix=FI
         g[name] = np.ascontiguousarray(grad[q * n:(q + 1) * n].reshape(shape[::-1]).T)
     result = dict(out=[np.ascontiguousarray(o.T) for o in out], grad=g,
                   res=[np.ascontiguousarray(o.T) for o in res_out], rms=(rms[0], rms[1]))
+    if gtau is not None:
+        for q, name in enumerate(("taup", "taus")):
+            g[name] = np.ascontiguousarray(gtau[q * n:(q + 1) * n].reshape(shape[::-1]).T)
     if hess is not None:
         result["hess"] = {name: np.ascontiguousarray(hess[q * n:(q + 1) * n].reshape(shape[::-1]).T)
                           for q, name in enumerate((names[0], names[1], "rho"))}

@@ -514,6 +514,12 @@ typedef struct {
      * cell (2D v: vx vz; 2D s: sxx szz sxz; 3D v: vx vy vz; 3D s: sxx syy
      * szz sxy syz sxz); NULL in forward runs */
     float *inc[6];
+    /* viscoelastic adjoint runs with tau gradients: per cell, the change of */
+    /* each partial stress increment with tau_p (one value, the same for the */
+    /* normal stresses) and with tau_s (per component), both at fixed wave   */
+    /* speeds (2D: p xx zz xz; 3D: p xx yy zz xy yz xz); NULL otherwise      */
+    float *tinc[7];
+    float s0;    /* sum_l w0^2 t_l^2 / (1 + w0^2 t_l^2) of the transform */
 } sim;
 
 static float *zalloc(long n) { return n > 0 ? calloc(n, sizeof(float)) : NULL; }
@@ -624,6 +630,58 @@ static void update_v2D(sim *S) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* tau gradients of the GSLS stress update (no counterpart in the reference's */
+/* time-domain kernels, which declare gradtaup/gradtaus and never write      */
+/* them, src/update_adjs3D.cl:66; its DFT path forms them in                 */
+/* src/calc_grad.c:111-152).  The update of a normal stress is               */
+/*   inc_ii = g th - f (th - v_ii) + Rm_ii,   g = M(1+L tp), f = 2mu(1+L ts), */
+/* with Rm_ii = DT2 sum_l (r_ii,l old + new), the whole memory term of the  */
+/* step (the partial increment the M and mu gradients read holds only the  */
+/* old level; for tau that halves the memory term, and finite differences  */
+/* of the misfit then miss by 60-80%), the memory variables forced by      */
+/* e th - d (th - v_ii), e = M tp/DT, d = 2 mu ts/DT, all through the same  */
+/* relaxation filters: Rm_ii = e F - d F_ii with F the filtered th, and      */
+/* sum_ii Rm_ii = (ND e - (ND-1) d) F.  So F and F_ii follow from the       */
+/* memory sums, and the change of inc with tp (at fixed M) is               */
+/*   M L th + (M/DT) F,                                                     */
+/* with ts (at fixed mu) -2mu L (th - v_ii) - (e F - Rm_ii)/ts, and for a    */
+/* shear component (coefficients mu_ij(1+L ts_ij), mu_ij ts_ij/DT)           */
+/*   mu_ij L w_ij + Rm_ij/ts_ij.                                            */
+/* The wave speeds are held fixed through the relaxation normalisation      */
+/* M = M_f0/(1 + s0 tp) (src/assign_modeling_case.c:415-437): that adds     */
+/* -s0/(1+s0 tp) times the increment's part linear in M (g th + e F), and   */
+/* likewise for mu.  M, mu, tp, ts are recovered from g, f, d, e as the      */
+/* device kernels do.  Cells with ts = 0 drop the memory term of ts.        */
+static void tau_parts_normal(float theta, const float *vii, const float *Rm, int nd, float g,
+                             float f, float d, float e, float L, float DT, float s0, float *Xp,
+                             float *Xs) {
+    float MR = g - L * e * DT;
+    float tp = e * DT / MR;
+    float tmu = f - L * d * DT;
+    float ts = tmu != 0.0f ? d * DT / tmu : 0.0f;
+    float kp = s0 / (1.0f + s0 * tp);
+    float ks = s0 / (1.0f + s0 * ts);
+    float den = nd == 3 ? 3.0f * e - 2.0f * d : 2.0f * e - d;
+    float trm = nd == 3 ? (Rm[0] + Rm[1]) + Rm[2] : Rm[0] + Rm[1];
+    float F = den != 0.0f ? trm / den : 0.0f;
+    *Xp = (MR * L - kp * g) * theta + (MR / DT - kp * e) * F;
+    for (int k = 0; k < nd; k++) {
+        float dev = theta - vii[k];
+        float q = e * F - Rm[k];
+        float Sk = -(f * dev) - q;
+        Xs[k] = -(tmu * L * dev) - (ts > 0.0f ? q / ts : 0.0f) - ks * Sk;
+    }
+}
+
+static float tau_part_shear(float w, float Rm, float fij, float dij, float inc, float L,
+                            float DT, float s0) {
+    float mua = fij - L * dij * DT;
+    float tsa = mua != 0.0f ? dij * DT / mua : 0.0f;
+    float ksa = s0 / (1.0f + s0 * tsa);
+    return mua * L * w + (tsa > 0.0f ? Rm / tsa : 0.0f) - ksa * inc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* 2D P-SV stress update: src/update_s2D.cl:30-326                           */
 /* ------------------------------------------------------------------------ */
 static void update_s2D(sim *S) {
@@ -727,6 +785,7 @@ static void update_s2D(sim *S) {
                     S->inc[1][indv] = lszz;
                     S->inc[2][indv] = lsxz;
                 }
+                const float oxx = sumrxx, ozz = sumrzz, oxz = sumrxz;
                 sumrxz = sumrxx = sumrzz = 0;
                 for (int l = 0; l < LVE; l++) {
                     float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
@@ -744,6 +803,20 @@ static void update_s2D(sim *S) {
                 S->sxz[indv] += lsxz + (DT2 * sumrxz);
                 S->sxx[indv] += lsxx + (DT2 * sumrxx);
                 S->szz[indv] += lszz + (DT2 * sumrzz);
+                if (S->tinc[0]) {
+                    /* the whole increment: memory of the old and new levels */
+                    float vii[2] = {vxx, vzz};
+                    float Rm[2] = {DT2 * oxx + DT2 * sumrxx, DT2 * ozz + DT2 * sumrzz};
+                    float Xp, Xs[2];
+                    tau_parts_normal(vxx + vzz, vii, Rm, 2, g, f, d, e, (float)LVE, DT, S->s0,
+                                     &Xp, Xs);
+                    S->tinc[0][indv] = Xp;
+                    S->tinc[1][indv] = Xs[0];
+                    S->tinc[2][indv] = Xs[1];
+                    S->tinc[3][indv] =
+                        tau_part_shear(vxz + vzx, DT2 * oxz + DT2 * sumrxz, fipkp, dipkp,
+                                       lsxz + (DT2 * sumrxz), (float)LVE, DT, S->s0);
+                }
             }
             if (tap) {
                 float tp = 1.0f;
@@ -1223,6 +1296,8 @@ static void update_s3D(sim *S) {
                         S->inc[4][indv] = lsyz;
                         S->inc[5][indv] = lsxz;
                     }
+                    const float oxy = sumrxy, oyz = sumryz, oxz = sumrxz, oxx = sumrxx,
+                                oyy = sumryy, ozz = sumrzz;
                     sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
                     for (int l = 0; l < LVE; l++) {
                         float b = 1.0 / (1.0 + (S->eta[l] * 0.5));
@@ -1250,6 +1325,28 @@ static void update_s3D(sim *S) {
                     S->sxx[indv] += lsxx + (DT2 * sumrxx);
                     S->syy[indv] += lsyy + (DT2 * sumryy);
                     S->szz[indv] += lszz + (DT2 * sumrzz);
+                    if (S->tinc[0]) {
+                        /* the whole increment: memory of the old and new levels */
+                        float vii[3] = {vxx, vyy, vzz};
+                        float Rm[3] = {DT2 * oxx + DT2 * sumrxx, DT2 * oyy + DT2 * sumryy,
+                                       DT2 * ozz + DT2 * sumrzz};
+                        float Xp, Xs[3];
+                        const float Lf = (float)LVE;
+                        tau_parts_normal(vxxyyzz, vii, Rm, 3, g, f, d, e, Lf, DT, S->s0, &Xp, Xs);
+                        S->tinc[0][indv] = Xp;
+                        S->tinc[1][indv] = Xs[0];
+                        S->tinc[2][indv] = Xs[1];
+                        S->tinc[3][indv] = Xs[2];
+                        S->tinc[4][indv] =
+                            tau_part_shear(vxyyx, DT2 * oxy + DT2 * sumrxy, fipjp, dipjp,
+                                           lsxy + (DT2 * sumrxy), Lf, DT, S->s0);
+                        S->tinc[5][indv] =
+                            tau_part_shear(vyzzy, DT2 * oyz + DT2 * sumryz, fjpkp, djpkp,
+                                           lsyz + (DT2 * sumryz), Lf, DT, S->s0);
+                        S->tinc[6][indv] =
+                            tau_part_shear(vxzzx, DT2 * oxz + DT2 * sumrxz, fipkp, dipkp,
+                                           lsxz + (DT2 * sumrxz), Lf, DT, S->s0);
+                    }
                 }
                 if (tap) {
                     float *V[6] = {&S->sxy[indv], &S->syz[indv], &S->sxz[indv],
@@ -2184,6 +2281,49 @@ static void grad_Mmu(const sim *S, const sim *A, const float *lMs, const float *
             }
 }
 
+/* tau gradients from the forward stresses S and the parts A->tinc: minus  */
+/* the forward strain (the stresses through the compliance of lMs, lmus,    */
+/* as for M and mu) times the change of each increment.                      */
+static void grad_tau(const sim *S, const sim *A, const float *lMs, const float *lmus,
+                     double *gtp, double *gts) {
+    const int FDOH = S->FDOH, NZ = S->NZp, NY = S->NYp, NX = S->NXp;
+    const long nzi = NZ - 2 * FDOH, nyi = S->ND == 3 ? NY - 2 * FDOH : 1;
+    const int y0 = S->ND == 3 ? FDOH : 0, y1 = S->ND == 3 ? NY - FDOH : 1;
+#pragma omp parallel for schedule(static)
+    for (int gidx = FDOH; gidx < NX - FDOH; gidx++)
+        for (int gidy = y0; gidy < y1; gidy++)
+            for (int gidz = FDOH; gidz < NZ - FDOH; gidz++) {
+                long indp = (long)(gidx - FDOH) * nyi * nzi + (long)(gidy - y0) * nzi + (gidz - FDOH);
+                long indv = (long)gidx * NY * NZ + (long)gidy * NZ + gidz;
+                float lM = lMs[indp], lmu = lmus[indp];
+                float lam = lM - 2.0f * lmu;
+                if (S->ND == 2) {
+                    float sxx = S->sxx[indv], szz = S->szz[indv], sxz = S->sxz[indv];
+                    float th = (sxx + szz) / (2.0f * lM - 2.0f * lmu);
+                    gtp[indp] += (double)(-(th * A->tinc[0][indv]));
+                    if (lmu == 0.0f) continue;
+                    float exx = (sxx - lam * th) / (2.0f * lmu);
+                    float ezz = (szz - lam * th) / (2.0f * lmu);
+                    float gxz = sxz / lmu;
+                    gts[indp] += (double)(-((exx * A->tinc[1][indv] + ezz * A->tinc[2][indv]) +
+                                            gxz * A->tinc[3][indv]));
+                    continue;
+                }
+                float sxx = S->sxx[indv], syy = S->syy[indv], szz = S->szz[indv];
+                float sxy = S->sxy[indv], syz = S->syz[indv], sxz = S->sxz[indv];
+                float th = ((sxx + syy) + szz) / (3.0f * lM - 4.0f * lmu);
+                gtp[indp] += (double)(-(th * A->tinc[0][indv]));
+                if (lmu == 0.0f) continue;
+                float exx = (sxx - lam * th) / (2.0f * lmu);
+                float eyy = (syy - lam * th) / (2.0f * lmu);
+                float ezz = (szz - lam * th) / (2.0f * lmu);
+                float n3 = (exx * A->tinc[1][indv] + eyy * A->tinc[2][indv]) + ezz * A->tinc[3][indv];
+                float s3 = ((sxy / lmu) * A->tinc[4][indv] + (syz / lmu) * A->tinc[5][indv]) +
+                           (sxz / lmu) * A->tinc[6][indv];
+                gts[indp] += (double)(-(n3 + s3));
+            }
+}
+
 /* Adjoint of the image-method free surface, applied to the adjoint fields */
 /* at the start of each adjoint step (src/surface2D_adj.cl:35-148,          */
 /* src/surface3D_adj.cl:29-180), elastic.  The reference scatters into the  */
@@ -2553,6 +2693,17 @@ static int adj_alloc(sim *A, const sim *S, char *err, int errlen) {
     A->tausipjp = S->tausipjp; A->tausjpkp = S->tausjpkp;
     int ninc = S->ND == 3 ? 6 : 3;
     for (int q = 0; q < ninc; q++) A->inc[q] = zalloc(S->nv);
+    if (S->L > 0) {
+        /* the tau parts and the relaxation sum of the mu transform */
+        for (int q = 0; q < (S->ND == 3 ? 7 : 4); q++) A->tinc[q] = zalloc(S->nv);
+        float pts[OR_MAXL];
+        for (int l = 0; l < S->L; l++) pts[l] = S->c->dt / S->eta[l];
+        float ws = 2.0 * PI * S->c->f0;
+        float sumu = 0.0;
+        for (int l = 0; l < S->L; l++)
+            sumu += ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));
+        A->s0 = sumu;
+    }
     return 0;
 }
 
@@ -2562,6 +2713,10 @@ static void adj_free(sim *A) {
     for (int q = 0; q < 6; q++) {
         free(A->inc[q]);
         A->inc[q] = NULL;
+    }
+    for (int q = 0; q < 7; q++) {
+        free(A->tinc[q]);
+        A->tinc[q] = NULL;
     }
     sim_free(A);
 }
@@ -2582,7 +2737,8 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
                 const float *taup, const float *taus, int nsrc, const float *src_pos,
                 const float *src, int nrec, const float *rec_pos, const float *din,
                 const float *res_in, int gradfix, float *out, float *res_out,
-                float *grad, double *rms, float *fields_out, float *hess, int movout,
+                float *grad, double *rms, float *fields_out, float *hess, float *gradtau,
+                int movout,
                 float *movie, char *err, int errlen) {
     sim S, A;
     char dummy[8];
@@ -2744,6 +2900,9 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
            *grho = calloc(S.np, sizeof(double));
     double *hM = hess ? calloc(S.np, sizeof(double)) : NULL,
            *hmu = hess ? calloc(S.np, sizeof(double)) : NULL;
+    const int want_tau = c->L > 0 && gradtau && c->restype != 1 && c->param_type <= 2;
+    double *gtp = want_tau ? calloc(S.np, sizeof(double)) : NULL,
+           *gts = want_tau ? calloc(S.np, sizeof(double)) : NULL;
     for (int t = tmax - 1; t > tmin; t--) {
         if (c->freesurf) {
             if (S.ND == 3) surface_adj3D(&A, gradfix & OR_FIX_SURF_SYY);
@@ -2761,6 +2920,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
             if (S.ND == 3) update_s3D(&A);
             else update_s2D(&A);
             grad_Mmu(&S, &A, Mg, mug, gM, gmu, hM, hmu, gradfix & OR_FIX_HESS);
+            if (gtp) grad_tau(&S, &A, Mg, mug, gtp, gts);
             if (S.ND == 3) update_v3D(&A);
             else update_v2D(&A);
             grad_rho(&S, &A, grho);
@@ -2785,6 +2945,18 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     }
     transf_grad(&S, Mg, mug, gM, gmu, grho, grad, gradfix & OR_FIX_RHO_SCALE);
     if (hess) transf_hess(&S, Mg, mug, hM, hmu, hess);
+    if (gradtau) {
+        /* the tau gradients at fixed wave speeds: the accumulators hold     */
+        /* dt^2/dh times dJ/dtau (as the moduli's hold dt^2/dh times the     */
+        /* derivative by the scaled moduli, transf_grad); zero for elastic   */
+        const double k = (double)c->dh / ((double)c->dt * c->dt);
+        for (long i = 0; i < S.np; i++) {
+            gradtau[i] = gtp ? (float)(gtp[i] * k) : 0.0f;
+            gradtau[S.np + i] = gts ? (float)(gts[i] * k) : 0.0f;
+        }
+    }
+    free(gtp);
+    free(gts);
     free(hM);
     free(hmu);
     if (fields_out) {

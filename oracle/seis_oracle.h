@@ -147,7 +147,9 @@ int  or_shell_check(const or_csts *c, long *nbnd);
  * accumulated in double.  res_out (optional): the scaled adjoint sources;
  * rms (optional): [rms, rmsnorm]; fields_out (optional): the backpropagated
  * forward wavefields at the end, padded; hess (optional): the approximate
- * Hessian [3][Pi N] transformed like grad (Hout).  restype 1 accumulates the
+ * Hessian [3][Pi N] transformed like grad (Hout).  gradtau (optional):
+ * [2][Pi N], the gradients of tau_p and tau_s at fixed wave speeds
+ * (viscoelastic runs; zero otherwise).  restype 1 accumulates the
  * RTM image of M instead of the gradient.  movie (optional, movout > 0): the
  * movie of the gradient run, [nvars][NT/movout][padded cells]: after the
  * adjoint step t with t % movout == 0, frame (t+1)/movout - 1 holds the
@@ -161,7 +163,8 @@ int  or_gradient(const or_csts *c, const float *p0, const float *p1,
                  int nrec, const float *rec_pos, const float *din,
                  const float *res_in, int gradfix, float *out,
                  float *res_out, float *grad, double *rms, float *fields_out,
-                 float *hess, int movout, float *movie, char *err, int errlen);
+                 float *hess, float *gradtau, int movout, float *movie, char *err,
+                 int errlen);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ SIGNATURES = [
     ("seis_adjoint", _I, [_P, _FP]),
     ("seis_adjoint_movie", _I, [_P, _FP, _I, _I, ctypes.POINTER(ctypes.c_char_p), _FP]),
     ("seis_grad_read", _I, [_P, _FP]),
+    ("seis_grad_read_tau", _I, [_P, _FP]),
     ("seis_gradient_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP, _FP, _FP,
                                 ctypes.POINTER(ctypes.c_double)]),
     ("seis_batch_create", _I, [_P, _I, ctypes.POINTER(_P)]),

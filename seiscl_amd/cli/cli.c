@@ -454,11 +454,12 @@ static int write_outputs(const files_t *fs, job_t *j) {
         long long d[3];
         for (int i = 0; i < nd; ++i) d[nd - 1 - i] = c->N[i];
         const long long n = n_cells(c);
-        for (int k = 0; k < 3 && !s; ++k) {
+        const int npar = c->L > 0 && c->param_type <= 2 ? 5 : 3;
+        for (int k = 0; k < npar && !s; ++k) {
             if (!par_used(c, k)) continue;
             snprintf(name, sizeof(name), "grad%s", par_name(c->param_type, k) + 1);
             s = write_f32(f, name, j->grad + (size_t)k * n, nd, d);
-            if (!s && j->hess) {
+            if (!s && j->hess && k < 3) {
                 /* H<name> (src/writehdf5.c:275-283) */
                 snprintf(name, sizeof(name), "H%s", par_name(c->param_type, k) + 1);
                 s = write_f32(f, name, j->hess + (size_t)k * n, nd, d);
@@ -660,9 +661,14 @@ static int run(job_t *j) {
             }
     }
     if (!s && j->gradout) {
-        j->grad = (float *)malloc(sizeof(float) * 3 * (size_t)n_cells(c));
+        /* grad<name> of every parameter (src/writehdf5.c:261-283): the
+         * viscoelastic case adds taup and taus */
+        const int ntau = c->L > 0 && c->param_type <= 2 ? 2 : 0;
+        j->grad = (float *)calloc((size_t)(3 + ntau) * n_cells(c), sizeof(float));
         if (!j->grad) s = err("out of host memory");
         else if (seis_grad_read(m, j->grad)) s = err("%s", seis_last_error());
+        else if (ntau && seis_grad_read_tau(m, j->grad + 3 * (size_t)n_cells(c)))
+            s = err("%s", seis_last_error());
         if (!s && j->Hout) {
             j->hess = (float *)malloc(sizeof(float) * 3 * (size_t)n_cells(c));
             if (!j->hess) s = err("out of host memory");

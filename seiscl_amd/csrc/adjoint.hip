@@ -169,6 +169,82 @@ __device__ __forceinline__ void grad_mmu_3d(const AdjArgs<R> &A, long i, R sxx, 
     }
 }
 
+// tau gradients of the GSLS stress update (no counterpart in the
+// reference's time-domain kernels, which declare gradtaup/gradtaus and never
+// write them, src/update_adjs3D.cl:66; its DFT path forms them in
+// src/calc_grad.c:111-152).  A normal-stress update is
+//   inc_ii = g th - f (th - v_ii) + Rm_ii,  g = M(1+L tp), f = 2mu(1+L ts),
+// Rm_ii = DT2 sum_l (r_ii,l old + new): the whole memory term of the step,
+// the memory variables forced by e th - d (th - v_ii), e = M tp/DT,
+// d = 2 mu ts/DT, through the same relaxation filters, so Rm_ii = e F -
+// d F_ii and sum_ii Rm_ii = (ND e - (ND-1) d) F.  The change of the update
+// with tp (at fixed M) is M L th + (M/DT) F, with ts (at fixed mu)
+// -2mu L (th - v_ii) - (e F - Rm_ii)/ts, and for a shear component (pair
+// mu_ij(1+L ts_ij), mu_ij ts_ij/DT) mu_ij L w_ij + Rm_ij/ts_ij.  Holding the
+// wave speeds through the normalisation M = M_f0/(1 + s0 tp) adds
+// -s0/(1+s0 tp) times the part linear in M (likewise for mu).  M, mu, tp, ts
+// are recovered from g, f, d, e.  Cells with ts = 0 drop the memory term of
+// ts.  The gradient is minus the forward strain (the forward stresses
+// through the compliance of the gradient moduli, as for M and mu) times
+// these changes.
+template <typename R>
+__device__ __forceinline__ void tau_parts_normal(R theta, const R *vii, const R *Rm, int nd, R g,
+                                                 R f, R d, R e, R L, R DT, R s0, R &Xp, R *Xs) {
+    const R MR = g - L * e * DT;
+    const R tp = e * DT / MR;
+    const R tmu = f - L * d * DT;
+    const R ts = tmu != R(0) ? d * DT / tmu : R(0);
+    const R kp = s0 / (R(1) + s0 * tp);
+    const R ks = s0 / (R(1) + s0 * ts);
+    const R den = nd == 3 ? R(3) * e - R(2) * d : R(2) * e - d;
+    const R trm = nd == 3 ? (Rm[0] + Rm[1]) + Rm[2] : Rm[0] + Rm[1];
+    const R F = den != R(0) ? trm / den : R(0);
+    Xp = (MR * L - kp * g) * theta + (MR / DT - kp * e) * F;
+    for (int k = 0; k < nd; ++k) {
+        const R dev = theta - vii[k];
+        const R q = e * F - Rm[k];
+        const R Sk = -(f * dev) - q;
+        Xs[k] = -(tmu * L * dev) - (ts > R(0) ? q / ts : R(0)) - ks * Sk;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R tau_part_shear(R w, R Rm, R fij, R dij, R inc, R L, R DT, R s0) {
+    const R mua = fij - L * dij * DT;
+    const R tsa = mua != R(0) ? dij * DT / mua : R(0);
+    const R ksa = s0 / (R(1) + s0 * tsa);
+    return mua * L * w + (tsa > R(0) ? Rm / tsa : R(0)) - ksa * inc;
+}
+
+template <typename R>
+__device__ __forceinline__ void grad_tau_2d(const AdjArgs<R> &A, long i, R sxx, R szz, R sxz,
+                                            R lM, R lmu, R Xp, R Xxx, R Xzz, R Xxz) {
+    const R lam = lM - R(2) * lmu;
+    const R th = (sxx + szz) / (R(2) * lM - R(2) * lmu);
+    A.gtp[i] += (double)(-(th * Xp)) * A.gscale_s;
+    if (lmu == R(0)) return;
+    const R exx = (sxx - lam * th) / (R(2) * lmu);
+    const R ezz = (szz - lam * th) / (R(2) * lmu);
+    const R gxz = sxz / lmu;
+    A.gts[i] += (double)(-((exx * Xxx + ezz * Xzz) + gxz * Xxz)) * A.gscale_s;
+}
+
+template <typename R>
+__device__ __forceinline__ void grad_tau_3d(const AdjArgs<R> &A, long i, R sxx, R syy, R szz,
+                                            R sxy, R syz, R sxz, R lM, R lmu, R Xp,
+                                            const R *X) {
+    const R lam = lM - R(2) * lmu;
+    const R th = ((sxx + syy) + szz) / (R(3) * lM - R(4) * lmu);
+    own_add(A.gtp + i, (double)(-(th * Xp)) * A.gscale_s);
+    if (lmu == R(0)) return;
+    const R exx = (sxx - lam * th) / (R(2) * lmu);
+    const R eyy = (syy - lam * th) / (R(2) * lmu);
+    const R ezz = (szz - lam * th) / (R(2) * lmu);
+    const R n3 = (exx * X[0] + eyy * X[1]) + ezz * X[2];
+    const R s3 = ((sxy / lmu) * X[3] + (syz / lmu) * X[4]) + (sxz / lmu) * X[5];
+    own_add(A.gts + i, (double)(-(n3 + s3)) * A.gscale_s);
+}
+
 template <typename R, int H>
 __global__ __launch_bounds__(256) void k_adj_s2d(AdjArgs<R> A, int nbz, unsigned nb) {
     const FDArgs<R> &f = A.f, &a = A.r;
@@ -515,6 +591,7 @@ __global__ __launch_bounds__(256) void k_adjh_s2d(AdjArgs<R> A, int nbz, unsigne
     const R lsxz = (fipkp * (vxz + vzx)) + (DT2 * sumrxz);
     const R lsxx = ((gg * (vxx + vzz)) - (f * vzz)) + (DT2 * sumrxx);
     const R lszz = ((gg * (vxx + vzz)) - (f * vxx)) + (DT2 * sumrzz);
+    const R oxz = sumrxz, oxx = sumrxx, ozz = sumrzz;
     sumrxz = sumrxx = sumrzz = 0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -534,7 +611,18 @@ __global__ __launch_bounds__(256) void k_adjh_s2d(AdjArgs<R> A, int nbz, unsigne
     a.sxz[i] = s3[2];
     // M and mu gradients (src/update_adjs2D.cl:441-468) from the forward state
     const R sxx = hst<R, S>(A, 2, i), szz = hst<R, S>(A, 3, i), sxz = hst<R, S>(A, 4, i);
-    grad_mmu_2d(A, i, sxx, szz, sxz, lsxx, lszz, lsxz, A.Ms[i], A.mus[i]);
+    const R lM = A.Ms[i], lmu = A.mus[i];
+    grad_mmu_2d(A, i, sxx, szz, sxz, lsxx, lszz, lsxz, lM, lmu);
+    if (A.gtp) {
+        // tau gradients from the whole memory term of the step
+        const R vii[2] = {vxx, vzz};
+        const R Rm[2] = {DT2 * oxx + DT2 * sumrxx, DT2 * ozz + DT2 * sumrzz};
+        R Xp, Xs[2];
+        tau_parts_normal<R>(vxx + vzz, vii, Rm, 2, gg, f, d, e, (R)L, a.DT, A.s0, Xp, Xs);
+        const R Xxz = tau_part_shear<R>(vxz + vzx, DT2 * oxz + DT2 * sumrxz, fipkp, dipkp,
+                                        lsxz + (DT2 * sumrxz), (R)L, a.DT, A.s0);
+        grad_tau_2d(A, i, sxx, szz, sxz, lM, lmu, Xp, Xs[0], Xs[1], Xxz);
+    }
 }
 
 template <typename R, typename S, int H>
@@ -637,6 +725,7 @@ __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigne
     const R lsxx = ((gg * vxxyyzz) - (f * vyyzz)) + (DT2 * sumrxx);
     const R lsyy = ((gg * vxxyyzz) - (f * vxxzz)) + (DT2 * sumryy);
     const R lszz = ((gg * vxxyyzz) - (f * vxxyy)) + (DT2 * sumrzz);
+    const R oxy = sumrxy, oyz = sumryz, oxz = sumrxz, oxx = sumrxx, oyy = sumryy, ozz = sumrzz;
     sumrxy = sumryz = sumrxz = sumrxx = sumryy = sumrzz = 0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -666,8 +755,24 @@ __global__ __launch_bounds__(256) void k_adjh_s3d(AdjArgs<R> A, int nbz, unsigne
     // M and mu gradients (src/update_adjs3D.cl:678-710) from the forward state
     const R sxx = hst_own<R, S>(A, 3, i), syy = hst_own<R, S>(A, 4, i), szz = hst_own<R, S>(A, 5, i);
     const R sxy = hst_own<R, S>(A, 6, i), syz = hst_own<R, S>(A, 7, i), sxz = hst_own<R, S>(A, 8, i);
-    grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz,
-                own_ld(A.Ms + i), own_ld(A.mus + i));
+    const R lM = own_ld(A.Ms + i), lmu = own_ld(A.mus + i);
+    grad_mmu_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lsxx, lsyy, lszz, lsxy, lsyz, lsxz, lM, lmu);
+    if (A.gtp) {
+        // tau gradients from the whole memory term of the step
+        const R vii[3] = {vxx, vyy, vzz};
+        const R Rm[3] = {DT2 * oxx + DT2 * sumrxx, DT2 * oyy + DT2 * sumryy,
+                         DT2 * ozz + DT2 * sumrzz};
+        R Xp, X[6];
+        const R Lr = (R)L;
+        tau_parts_normal<R>(vxxyyzz, vii, Rm, 3, gg, f, d, e, Lr, a.DT, A.s0, Xp, X);
+        X[3] = tau_part_shear<R>(vxyyx, DT2 * oxy + DT2 * sumrxy, fipjp, dipjp,
+                                 lsxy + (DT2 * sumrxy), Lr, a.DT, A.s0);
+        X[4] = tau_part_shear<R>(vyzzy, DT2 * oyz + DT2 * sumryz, fjpkp, djpkp,
+                                 lsyz + (DT2 * sumryz), Lr, a.DT, A.s0);
+        X[5] = tau_part_shear<R>(vxzzx, DT2 * oxz + DT2 * sumrxz, fipkp, dipkp,
+                                 lsxz + (DT2 * sumrxz), Lr, a.DT, A.s0);
+        grad_tau_3d(A, i, sxx, syy, szz, sxy, syz, sxz, lM, lmu, Xp, X);
+    }
 }
 
 template <typename R, typename S, int H>

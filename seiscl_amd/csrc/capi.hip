@@ -137,6 +137,7 @@ struct seis_model {
         FDArgs<float> r{};          // adjoint wavefields (same model)
         double *gM = nullptr, *gmu = nullptr, *grho = nullptr;   // padded layout
         double *hM = nullptr, *hmu = nullptr;                    // Hessian (Hout)
+        double *gtp = nullptr, *gts = nullptr;   // tau_p, tau_s (viscoelastic)
         float *res = nullptr;       // [nout][nrec][NT] scaled adjoint sources
         float *stmp = nullptr;      // adjoint free surface: 3 * NX * NY
         size_t res_cap = 0;
@@ -558,6 +559,10 @@ int grad_setup(seis_model *m) {
         if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
     if (c.Hout)
         for (double **p : {&G.hM, &G.hmu})
+            if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
+    // tau gradients of viscoelastic runs (param_type 3 reads no tau)
+    if (G.visco && c.param_type <= 2)
+        for (double **p : {&G.gtp, &G.gts})
             if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
     G.ready = true;
     return 0;
@@ -1177,6 +1182,8 @@ int seis_grad_zero(seis_model *m) {
     launch_fill(m->grad.grho, nb, m->ctx->stream);
     if (m->grad.hM) launch_fill(m->grad.hM, nb, m->ctx->stream);
     if (m->grad.hmu) launch_fill(m->grad.hmu, nb, m->ctx->stream);
+    if (m->grad.gtp) launch_fill(m->grad.gtp, nb, m->ctx->stream);
+    if (m->grad.gts) launch_fill(m->grad.gts, nb, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     return 0;
@@ -1609,6 +1616,11 @@ static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv) {
     A.rtm = m->c.restype == 1;
     A.hM = G.hM;
     A.hmu = G.hmu;
+    if (G.gtp && !A.rtm) {
+        A.gtp = G.gtp;
+        A.gts = G.gts;
+        A.s0 = relax_sum(m->c);
+    }
     A.hfix = (m->c.gradfix & FIX_HESS) != 0;
     if (m->c.freesurf && (!A.r.M || !A.r.mu || !G.stmp))
         return fail("seis_adjoint: the adjoint free surface has no moduli on the device");
@@ -1716,6 +1728,26 @@ int seis_grad_read(seis_model *m, float *grad) {
     const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
     transf_grad(m->c, m->rp, f0 ? m->rp.Mf0 : m->rp.M, f0 ? m->rp.muf0 : m->rp.mu,
                 acc[0].data(), acc[1].data(), acc[2].data(), grad, m->c.gradfix & FIX_RHO_SCALE);
+    return 0;
+}
+
+int seis_grad_read_tau(seis_model *m, float *gtau) {
+    g_err.clear();
+    auto &G = m->grad;
+    if (!G.ready || !G.gtp)
+        return fail("seis_grad_read_tau: no tau gradient accumulated (viscoelastic runs, "
+                    "param_type 0-2)");
+    std::vector<double> acc[2];
+    const double *src[2] = {G.gtp, G.gts};
+    if (accum_read(m, src, 2, acc)) return 1;
+    // the accumulators hold dt^2/dh times dJ/dtau, as the moduli's hold
+    // dt^2/dh times the derivative by the scaled moduli (transf_grad)
+    const double k = (double)m->c.dh / ((double)m->c.dt * m->c.dt);
+    const size_t n = acc[0].size();
+    for (size_t i = 0; i < n; ++i) {
+        gtau[i] = (float)(acc[0][i] * k);
+        gtau[n + i] = (float)(acc[1][i] * k);
+    }
     return 0;
 }
 

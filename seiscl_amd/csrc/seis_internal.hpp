@@ -78,6 +78,9 @@ void to_padded(const Layout &g, const float *ref, float mul, std::vector<T> &out
 // 252-259, src/update_s2D_acc.cl:343-346): g, f, d, e (f, d for ND 2/3).
 void visco_coeffs(const seis_csts &c, const RefParams &p, float DT, std::vector<float> &g,
                   std::vector<float> &f, std::vector<float> &d, std::vector<float> &e);
+// s0 = sum_l w0^2 t_l^2 / (1 + w0^2 t_l^2), t_l = dt/eta_l: the relaxed
+// moduli are M_f0 / (1 + s0 tau) (src/assign_modeling_case.c:415-437).
+float relax_sum(const seis_csts &c);
 // The staggered shear pair: f_ij = mu_ij (1 + L tau_ij), d_ij = mu_ij tau_ij / DT.
 void visco_pair(const std::vector<float> &mu, const std::vector<float> &ts, int L, float DT,
                 std::vector<float> &ff, std::vector<float> &dd);
@@ -332,6 +335,11 @@ struct AdjArgs {
     double *hM = nullptr, *hmu = nullptr;
     double hscale = 1;
     int hfix = 1;
+    // viscoelastic runs: tau_p and tau_s gradient accumulators (double,
+    // padded layout, scaled by gscale_s like M's) or null, and the
+    // relaxation sum s0 of the transform (relax_sum)
+    double *gtp = nullptr, *gts = nullptr;
+    R s0 = 0;
 };
 
 template <typename R>

@@ -256,6 +256,17 @@ int build_setup(const seis_csts &c, Setup &s, std::string &err) {
     return 0;
 }
 
+float relax_sum(const seis_csts &c) {
+    float s = 0.0;
+    const float ws = 2.0 * kPi * c.f0;
+    for (int l = 0; l < c.L; ++l) {
+        const float eta = (2.0 * kPi * c.FL[l]) * c.dt;
+        const float pts = c.dt / eta;
+        s += ((ws * ws * pts * pts) / (1.0 + ws * ws * pts * pts));
+    }
+    return s;
+}
+
 int transform_params(const seis_csts &c, const float *p0, const float *p1,
                      const float *rho_in, const float *taup_in, const float *taus_in,
                      RefParams &o, std::string &err) {
@@ -334,9 +345,7 @@ int transform_params(const seis_csts &c, const float *p0, const float *p1,
         }
         if (L > 0) {
             o.muf0 = o.mu;
-            float s = 0.0;
-            for (int l = 0; l < L; ++l)
-                s += ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));
+            const float s = relax_sum(c);
 #pragma omp parallel for schedule(static) if (n >= kParCells)
             for (long i = 0; i < n; ++i) o.mu[i] = o.mu[i] / (1.0 + s * o.taus[i]);
         }

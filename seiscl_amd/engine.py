@@ -325,12 +325,19 @@ class Model:
         return [np.ascontiguousarray(x.T) for x in r], (rms.value, rn.value)
 
     def grad_read(self) -> dict:
-        """Accumulated gradients {p0, p1, rho} in the model layout."""
+        """Accumulated gradients {p0, p1, rho} in the model layout, plus
+        {taup, taus} (at fixed wave speeds) for viscoelastic models."""
         n = int(np.prod(self.shape))
         g = np.zeros(3 * n, dtype=np.float32)
         check(self.lib.seis_grad_read(self.handle, fptr(g)))
         names = PARAM_NAMES.get(self.csts.param_type, ("vp", "vs", "rho"))
-        return {nm: from_c_layout(g[q * n:(q + 1) * n], self.shape) for q, nm in enumerate(names)}
+        out = {nm: from_c_layout(g[q * n:(q + 1) * n], self.shape) for q, nm in enumerate(names)}
+        if self.csts.L > 0 and self.csts.param_type <= 2:
+            t = np.zeros(2 * n, dtype=np.float32)
+            check(self.lib.seis_grad_read_tau(self.handle, fptr(t)))
+            out["taup"] = from_c_layout(t[:n], self.shape)
+            out["taus"] = from_c_layout(t[n:], self.shape)
+        return out
 
     def hess_read(self) -> dict:
         """Approximate Hessian {p0, p1, rho} accumulated with the gradient

@@ -295,3 +295,35 @@ def test_movie_tmax(tmp_path):
         np.testing.assert_array_equal(a, b)
         assert np.abs(a[..., 130 // mv - 1, :]).max() > 0
         assert np.all(a[..., 130 // mv:, :] == 0)
+
+
+def test_visco_gradient_files(oracle_lib, tmp_path):
+    """Viscoelastic gradient through SeisCL_MPI: the gradient file holds
+    grad<name> of every parameter, taup and taus included (the reference
+    writes all five, src/writehdf5.c:261-283, and SeisCL.py reads them back
+    for L > 0); each equals the oracle's, shot by shot."""
+    from seiscl_amd.workloads import q_to_tau
+    seis, model, true, csts = _fwi_case(tmp_path, L=2, FL=np.array([8.0, 30.0]))
+    tau = q_to_tau(50.0, 5.0, 40.0, [8.0, 30.0])
+    for p in (model, true):
+        p["taup"] = np.full(p["vp"].shape, tau, np.float32)
+        p["taus"] = np.full(p["vp"].shape, tau, np.float32)
+    csts = dict(csts, L=2, FL=np.array([8.0, 30.0]))
+    print("[cli] visco gradient files", flush=True)
+    seis.set_forward([0, 1], true, withgrad=False)
+    seis.execute()
+    dobs = seis.read_data()
+    seis.write_data({"vx": dobs[0], "vz": dobs[1]})
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.execute()
+    gout = seis.read_grad()
+    assert seis.params == ["vp", "vs", "rho", "taup", "taus"]
+    tot = {k: 0 for k in seis.params}
+    for rc, shot in _shots(seis):
+        g = oracle_lib.gradient(model, *shot, din=[d[:, rc] for d in dobs], **csts)
+        for k in tot:
+            tot[k] = tot[k] + g["grad"][k].astype(np.float64)
+    for name, g in zip(seis.params, gout):
+        ref = _cropped(tot[name], seis.nab)
+        assert np.abs(ref).max() > 0, name
+        assert rel_l2(g, ref) <= 1e-5, name

@@ -1,6 +1,7 @@
 """GPU gradient against the CPU oracle on the same inputs: elastic runs by
 backpropagation with boundary saving, viscoelastic runs from checkpoints
-and recomputed segments (the oracle keeps the whole forward history).
+and recomputed segments (the oracle keeps the whole forward history);
+viscoelastic runs also compare the tau_p and tau_s gradients.
 
 The adjoint kernels follow the oracle's expressions with FMA contraction
 disabled, and both accumulate the per-step gradient increments in double in
@@ -37,6 +38,8 @@ VISCO = (
     + [dict(ND=2, FDORDER=4, L=2, gradfix=f) for f in (0, 15)]
     + [dict(ND=3, FDORDER=4, L=2, abs_type=a) for a in (1, 2)]
     + [dict(ND=3, FDORDER=4, L=1, freesurf=1), dict(ND=3, FDORDER=8, L=2, gradfix=15)]
+    # the benchmark gradient's case: 3D, L=2, FD4, free surface
+    + [dict(ND=3, FDORDER=4, L=2, freesurf=1), dict(ND=3, FDORDER=4, L=2, freesurf=1, gradfix=223)]
 )
 CASES = CASES + VISCO
 
@@ -88,7 +91,8 @@ def test_gradient_matches_oracle(oracle_lib, gpu_ctx, case):
                                   dict(ND=2, FDORDER=4, freesurf=1, abs_type=2),
                                   dict(ND=3, FDORDER=4, abs_type=2, N=(150, 28, 30)),
                                   dict(ND=2, FDORDER=4, L=2, abs_type=1),
-                                  dict(ND=3, FDORDER=4, L=1, freesurf=1)], ids=_ids)
+                                  dict(ND=3, FDORDER=4, L=1, freesurf=1),
+                                  dict(ND=3, FDORDER=4, L=2, freesurf=1)], ids=_ids)
 def test_gradient_bitwise(oracle_lib, gpu_ctx, case):
     ref, out, rms, g = _run_both(oracle_lib, gpu_ctx, case)
     for r, o in zip(ref["out"], out):

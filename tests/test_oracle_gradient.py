@@ -40,7 +40,7 @@ def test_shell_maps(oracle_lib, ND, N, fd, fs):
         assert n == ref
 
 
-def _smooth_case(ND):
+def _smooth_case(ND, sigma=4.0):
     if ND == 2:
         csts, p, shot = small_case(ND=2, FDORDER=4, N=(50, 60), NT=300, seed=5, nab=8)
     else:
@@ -53,7 +53,7 @@ def _smooth_case(ND):
     idx = np.indices(shape).astype(float)
     ctr = [d / 2 for d in shape]
     blob = np.exp(-sum((idx[i] - ctr[i] - (3 if i == 0 else 0)) ** 2
-                       for i in range(len(shape))) / (2 * 4.0 ** 2))
+                       for i in range(len(shape))) / (2 * sigma ** 2))
     return csts, p, shot, blob
 
 
@@ -197,9 +197,9 @@ def test_history_path_equals_backpropagation(oracle_lib, ND):
         assert rel_l2(b["grad"][k][inner], a["grad"][k][inner]) < 1e-4, k
 
 
-def _visco_case(ND, L, Q):
+def _visco_case(ND, L, Q, sigma=4.0):
     from seiscl_amd.workloads import q_to_tau
-    csts, p, shot, blob = _smooth_case(ND)
+    csts, p, shot, blob = _smooth_case(ND, sigma)
     if ND == 2:
         csts, _, shot = small_case(ND=2, FDORDER=4, N=(50, 60), NT=300, seed=5, nab=8, L=L)
     else:
@@ -233,6 +233,30 @@ def test_visco_gradient_directional_fd(oracle_lib, ND):
             assert abs(gr - 1) > 0.05 and abs(gf - 1) < abs(gr - 1), (name, gf, gr)
     # the density term does not involve the moduli: the same with both
     np.testing.assert_array_equal(fixed["rho"] != 0, ref["rho"] != 0)
+
+
+@pytest.mark.parametrize("ND", [2, 3])
+def test_tau_gradient_directional_fd(oracle_lib, ND):
+    """tau_p and tau_s gradients (at fixed wave speeds) against central
+    differences of the misfit along a smooth perturbation of tau, L=2,
+    Q=50: within 5%.  The sensitivities of the staggered shear coefficients
+    are assigned to the cell, so the 3D check uses a wider perturbation
+    (6 cells; at 4 cells tau_s misses by 8.5%).  Measured: 2D 1.3% / 1.4%,
+    3D 1.3% / 3.6%.  The whole memory term of the step is
+    needed: with the partial increment the M and mu gradients read, the
+    2D check misses by 79% and 64%."""
+    csts, p, shot, blob = _visco_case(ND, 2, 50.0, sigma=4.0 if ND == 2 else 6.0)
+    csts["nthreads"] = 8
+    true = {k: (v * (1 + 0.05 * blob)).astype(np.float32) for k, v in p.items()}
+    din = oracle_lib.forward(true, *shot, **csts)
+    g = oracle_lib.gradient(p, *shot, din=din, **csts)["grad"]
+    for name in ("taup", "taus"):
+        fd, d = _directional(oracle_lib, p, shot, din, csts, name, blob, eps=5e-2)
+        gd = float(np.sum(g[name].astype(np.float64) * d))
+        assert abs(gd / fd - 1) < 0.05, (name, gd, fd)
+    # elastic runs report zero tau gradients: the oracle allocates none
+    ce, pe, she, _ = _smooth_case(ND)
+    assert "taup" not in oracle_lib.gradient(pe, *she, res=[0 * x for x in din], **ce)["grad"]
 
 
 def test_visco_gradient_error_grows_with_attenuation(oracle_lib):
