@@ -884,14 +884,25 @@ const char *seis_out_name(const seis_model *m, int i) {
     return m->outn[i].c_str();
 }
 
-int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
-                   const float *rec_pos) {
-    g_err.clear();
+namespace {
+// Host side of a shot: source and receiver cells, the source samples in the
+// model's units (FP16 storage and scaled units fix av and as per shot).
+struct ShotHost {
+    std::vector<int> scell, stype, sk, rcell;
+    std::vector<float> ss;      // scaled source samples (empty: src as given)
+};
+
+static int shot_host(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
+                     const float *rec_pos, ShotHost &h) {
     const seis_csts &c = m->c;
     const Layout &g = m->g;
     const int H = g.H, NT = c.NT;
     const float DH = m->s.DH;
-    std::vector<int> scell(nsrc), stype(nsrc), sk(nsrc), rcell(nrec);
+    h.scell.assign(nsrc, 0);
+    h.stype.assign(nsrc, 0);
+    h.sk.assign(nsrc, 0);
+    h.rcell.assign(nrec, 0);
+    h.ss.clear();
     auto cell = [&](const float *p, int &k, int &out) -> int {
         const int i = (int)(p[0] / DH) + H, j = (int)(p[1] / DH) + H;
         k = (int)(p[2] / DH) + H;
@@ -905,15 +916,76 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
         return 0;
     };
     for (int s = 0; s < nsrc; ++s) {
-        if (cell(&src_pos[5 * s], sk[s], scell[s])) return 1;
-        stype[s] = (int)src_pos[5 * s + 4];
-        if (c.ND == 22 && stype[s] == 100)
+        if (cell(&src_pos[5 * s], h.sk[s], h.scell[s])) return 1;
+        h.stype[s] = (int)src_pos[5 * s + 4];
+        if (c.ND == 22 && h.stype[s] == 100)
             return fail("explosive sources (type 100) are not defined for ND=22; use type 3");
     }
     for (int r = 0; r < nrec; ++r) {
         int k;
-        if (cell(&rec_pos[8 * r], k, rcell[r])) return 1;
+        if (cell(&rec_pos[8 * r], k, h.rcell[r])) return 1;
     }
+    if (nsrc && (m->prec == P_SCALED || m->prec == P_FP16)) {
+        // scaled units: the largest per-step injection (DT * |src|) of the
+        // sources feeding velocities (or, without any, stresses) is stored
+        // as about 2^8; powers of two keep the scaling exact
+        double af = 0, ae = 0;
+        for (int k = 0; k < nsrc; ++k)
+            for (int t = 0; t < NT; ++t) {
+                const double v = std::fabs((double)src[(size_t)k * NT + t]) * c.dt;
+                if (h.stype[k] <= 2) af = std::max(af, v);
+                else ae = std::max(ae, v);
+            }
+        auto pow2 = [](double x) { return std::ldexp(1.0f, (int)std::lround(std::log2(x))); };
+        if (af > 0) {
+            m->av = pow2(256.0 / af);
+            m->as = m->av / m->q;
+        } else if (ae > 0) {
+            m->as = pow2(256.0 / ae);
+            m->av = m->as * m->q;
+        } else {
+            m->av = m->q;
+            m->as = 1;
+        }
+        h.ss.resize((size_t)nsrc * NT);
+        for (int k = 0; k < nsrc; ++k) {
+            const float f = h.stype[k] <= 2 ? m->av : m->as;
+            for (int t = 0; t < NT; ++t) h.ss[(size_t)k * NT + t] = src[(size_t)k * NT + t] * f;
+        }
+        m->unscale.clear();
+        for (int code : m->out_codes)
+            m->unscale.push_back(code <= O_VZ ? 1.0f / m->av : 1.0f / m->as);
+    }
+    return 0;
+}
+
+// Copy a prepared shot into the device buffers of sd (sized for it) on the
+// transfer stream and wait for the copies.
+static int shot_upload(seis_model *m, const ShotHost &h, const float *src, ShotDev &sd) {
+    const int nsrc = (int)h.scell.size(), nrec = (int)h.rcell.size(), NT = m->c.NT;
+    hipStream_t st = m->ctx->xfer;
+    sd.nsrc = nsrc;
+    sd.nrec = nrec;
+    if (nsrc) {
+        HIP_TRY(hipMemcpyAsync(sd.src_cell, h.scell.data(), nsrc * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(sd.src_type, h.stype.data(), nsrc * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(sd.src_k, h.sk.data(), nsrc * 4, hipMemcpyHostToDevice, st));
+        const float *s = h.ss.empty() ? src : h.ss.data();
+        HIP_TRY(hipMemcpyAsync(sd.src, s, (size_t)nsrc * NT * 4, hipMemcpyHostToDevice, st));
+    }
+    if (nrec)
+        HIP_TRY(hipMemcpyAsync(sd.rec_cell, h.rcell.data(), nrec * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+}  // namespace
+
+int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
+                   const float *rec_pos) {
+    g_err.clear();
+    const int NT = m->c.NT;
+    ShotHost h;
+    if (shot_host(m, nsrc, src_pos, src, nrec, rec_pos, h)) return 1;
     seis_ctx *ctx = m->ctx;
     ShotDev &sd = m->shot;
     const int nout = (int)m->outn.size();
@@ -939,52 +1011,8 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
         m->shot_nsrc_cap = ns;
         m->shot_nrec_cap = nr;
     }
-    sd.nsrc = nsrc;
-    sd.nrec = nrec;
-    hipStream_t st = ctx->xfer;   // the host waits for these copies below
-    if (nsrc) {
-        HIP_TRY(hipMemcpyAsync(sd.src_cell, scell.data(), nsrc * 4, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(sd.src_type, stype.data(), nsrc * 4, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(sd.src_k, sk.data(), nsrc * 4, hipMemcpyHostToDevice, st));
-        if (m->prec == P_SCALED || m->prec == P_FP16) {
-            // scaled units: the largest per-step injection (DT * |src|) of the
-            // sources feeding velocities (or, without any, stresses) is stored
-            // as about 2^8; powers of two keep the scaling exact
-            double af = 0, ae = 0;
-            for (int k = 0; k < nsrc; ++k)
-                for (int t = 0; t < NT; ++t) {
-                    const double v = std::fabs((double)src[(size_t)k * NT + t]) * c.dt;
-                    if (stype[k] <= 2) af = std::max(af, v);
-                    else ae = std::max(ae, v);
-                }
-            auto pow2 = [](double x) { return std::ldexp(1.0f, (int)std::lround(std::log2(x))); };
-            if (af > 0) {
-                m->av = pow2(256.0 / af);
-                m->as = m->av / m->q;
-            } else if (ae > 0) {
-                m->as = pow2(256.0 / ae);
-                m->av = m->as * m->q;
-            } else {
-                m->av = m->q;
-                m->as = 1;
-            }
-            std::vector<float> ss((size_t)nsrc * NT);
-            for (int k = 0; k < nsrc; ++k) {
-                const float f = stype[k] <= 2 ? m->av : m->as;
-                for (int t = 0; t < NT; ++t) ss[(size_t)k * NT + t] = src[(size_t)k * NT + t] * f;
-            }
-            HIP_TRY(hipMemcpyAsync(sd.src, ss.data(), ss.size() * 4, hipMemcpyHostToDevice, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            m->unscale.clear();
-            for (int code : m->out_codes)
-                m->unscale.push_back(code <= O_VZ ? 1.0f / m->av : 1.0f / m->as);
-        } else {
-            HIP_TRY(hipMemcpyAsync(sd.src, src, (size_t)nsrc * NT * 4, hipMemcpyHostToDevice, st));
-        }
-    }
-    if (nrec) HIP_TRY(hipMemcpyAsync(sd.rec_cell, rcell.data(), nrec * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    m->rcell_host = rcell;
+    if (shot_upload(m, h, src, sd)) return 1;
+    m->rcell_host = h.rcell;
     m->shot_bound = true;
     m->grad.saved = false;
     return 0;
