@@ -31,20 +31,9 @@ def smooth(shape, corr_len: float, dh: float, rng) -> np.ndarray:
 def q_to_tau(Q: float, fmin: float, fmax: float, FL, Qmin: float = 5.0,
              Qmax: float = 500.0) -> float:
     """tau giving an average 1/Q over [fmin, fmax] for relaxation frequencies
-    FL: the calibration table of QTAU.Q2tau (SeisCL/Q_tau.py:25-131), i.e.
-    Q(w) = (1/tau + sum w^2 t_l^2/(1+w^2 t_l^2)) / sum w t_l/(1+w^2 t_l^2),
-    t_l = 1/(2 pi FL_l), tabulated on 40 log-spaced tau and interpolated
-    linearly in log-log."""
-    taus = [1.0 / f / 2.0 / np.pi for f in np.atleast_1d(FL)]
-    omega = 2 * np.pi * np.arange(fmin, fmax, 0.1)
-    tau = np.linspace(np.log(1 / Qmax / 2), np.log(1 / Qmin * 2), 40)
-    Qi = np.zeros(len(tau))
-    for ii, lt in enumerate(tau):
-        f1 = sum(omega ** 2 * t ** 2 / (1.0 + omega ** 2 * t ** 2) for t in taus)
-        f2 = sum(omega * t / (1.0 + omega ** 2 * t ** 2) for t in taus)
-        q = (1 / np.exp(lt) + f1) / f2
-        Qi[ii] = np.average(1 / q)
-    return float(np.exp(np.interp(np.log(1.0 / Q), np.log(Qi), tau)))
+    FL: QTAU(Qmin, Qmax, fmin, fmax, FL).Q2tau(Q) (SeisCL/Q_tau.py:25-131)."""
+    from .qtau import QTAU
+    return float(QTAU(Qmin, Qmax, fmin, fmax, FL).Q2tau(Q))
 
 
 def layered(shape, nlayers: int, vmin: float, vmax: float, rng, axis: int = 0) -> np.ndarray:

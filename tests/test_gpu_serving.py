@@ -72,3 +72,47 @@ def test_continuous_batching_staggered_arrivals(gpu_ctx):
             np.testing.assert_array_equal(x, y)
     s = summarize(reqs)
     assert s["shots_per_hour"] > 0 and s["p99_latency_s"] >= s["p50_latency_s"]
+
+
+def test_config4_size_requests(oracle_lib, gpu_ctx):
+    """Requests at the serving workload's own sizes (NZ 400-600, NX
+    1000-1500, NT 1000-1200, acoustic and elastic, FD 4 and 8, CPML 16):
+    each large request equals the oracle, and every request of the mix --
+    the large ones and eight drawn like the workload -- gives bit for bit
+    the same seismograms in the batch as alone."""
+    from seiscl_amd import Batch, Model
+    from seiscl_amd.workloads import config4_request
+    rng = np.random.default_rng(40)    # the large ones: elastic FD8, acoustic FD4
+    big = [config4_request(rng, NZ=400, NX=1000, NT=1000),
+           config4_request(rng, NZ=600, NX=1500, NT=1200)]
+    # the workload's draw, with NT cut to 1500 to bound the run
+    mix = [config4_request(rng, NT=int(rng.integers(1000, 1501))) for _ in range(8)]
+    reqs = big + mix
+    print(f"[serving] config-4 sizes: {[tuple(c['N']) + (c['NT'], c['ND']) for c, _, _ in reqs]}",
+          flush=True)
+    alone = []
+    for csts, params, shot in reqs:
+        m = Model(gpu_ctx, params, **csts)
+        alone.append(m.forward(*shot))
+        m.close()
+    for (csts, params, shot), a in zip(big, alone):
+        ref = oracle_lib.forward(params, *shot, nthreads=16, **csts)
+        for r, o in zip(ref, a):
+            assert np.abs(r).max() > 0
+            assert rel_l2(o, r) <= 1e-6
+    batch = Batch(gpu_ctx, len(reqs))
+    models, slots = [], []
+    for csts, params, shot in reqs:
+        m = Model(gpu_ctx, params, **csts)
+        m.bind_shot(*shot)
+        slots.append(batch.add(m))
+        models.append(m)
+    while batch.status()[0]:
+        batch.step(max(1, min(batch.status()[1], 64)))
+    for m, a in zip(models, alone):
+        for x, y in zip(m.read_shot(), a):
+            np.testing.assert_array_equal(x, y)
+    for s, m in zip(slots, models):
+        batch.remove(s)
+        m.close()
+    batch.close()
