@@ -206,6 +206,18 @@ int seis_grad_plan(const seis_model *m, int *levels, int *K, int *n, int *n0,
 int seis_forward_save(seis_model *m, int nsrc, const float *src_pos,
                       const float *src, int nrec, const float *rec_pos,
                       float *out);
+/* The saved forward run of the bound shot as one host buffer, and back:    */
+/* the device side of checkpoint_d2h / checkpoint_h2d (src/time_stepping.c: */
+/* 92-270) for the two-pass inputres flow (:731-745, 851-859).  Elastic     */
+/* runs save the final wavefields and CPML memory with the boundary shell   */
+/* of every step; viscoelastic runs save the forward run's full-state       */
+/* checkpoints.  The buffer starts with a header of this library's own      */
+/* (magic "SEISCKP1", kind, steps, sizes); seis_grad_saved_write checks it  */
+/* against the model and its checkpoint plan, needs the same shot bound     */
+/* (seis_shot_bind) and makes seis_adjoint run without a forward run.       */
+int seis_grad_saved_size(seis_model *m, size_t *bytes);
+int seis_grad_saved_read(seis_model *m, float *host);
+int seis_grad_saved_write(seis_model *m, const float *host);
 /* Residuals (adjoint sources before res_scale) of a shot, [nout][nrec][NT]: */
 /* var_res_raw (src/residuals.c:58-450) with the model's misfit options --    */
 /* Butterworth band-pass (fmin, fmax), rms scalings (scalerms, scaleshot,     */

@@ -95,6 +95,9 @@ SIGNATURES = [
     ("seis_read_field", _I, [_P, ctypes.c_char_p, _FP]),
     ("seis_grad_zero", _I, [_P]),
     ("seis_forward_save", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
+    ("seis_grad_saved_size", _I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    ("seis_grad_saved_read", _I, [_P, _FP]),
+    ("seis_grad_saved_write", _I, [_P, _FP]),
     ("seis_residuals", _I, [_P, _I, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_double)]),
     ("seis_misfit", _I, [ctypes.POINTER(SeisCsts), _I, _I, _FP, _FP, _FP,

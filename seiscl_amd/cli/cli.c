@@ -25,7 +25,7 @@
 
 typedef struct {
     char model[NAMELEN], csts[NAMELEN], dout[NAMELEN], gout[NAMELEN], rms[NAMELEN],
-        movie[NAMELEN], res[NAMELEN], din[NAMELEN];
+        movie[NAMELEN], res[NAMELEN], din[NAMELEN], checkpoint[NAMELEN];
 } files_t;
 
 typedef struct {
@@ -50,6 +50,8 @@ typedef struct {
     int nmov;
     const char *movs[10];
     float *mov;
+    /* checkpoint file of the two-pass inputres flow (src/time_stepping.c:731-745) */
+    const char *ckpt_path;
 } job_t;
 
 static int err(const char *fmt, ...) {
@@ -188,8 +190,6 @@ static int read_csts(hid_t f, job_t *j) {
     if (j->restype != 0 && j->restype != 1)
         return err("restype=%d is not available (0: L2 residuals, 1: RTM)", j->restype);
     if (j->inputres && j->resout) return err("Cannot input and output residuals simultaneously");
-    if (j->inputres && !j->gradout)
-        return err("inputres without gradout (checkpoint files) is not available");
     return 0;
 }
 
@@ -554,6 +554,58 @@ static int gradient_movie(seis_model *m, job_t *j, int sh, int nsrc, const float
 
 /* ---- the run -------------------------------------------------------------- */
 
+/* ---- two-pass inputres flow (src/time_stepping.c:92-270, 731-745, 851-859) --- */
+
+/* Pass 1 (inputres=1, gradout=0): the forward run with its saved state (the
+ * boundary shell, or the viscoelastic checkpoints) and that state written to
+ * the checkpoint file as src<s>_dev0_checkpoint (checkpoint_d2h). */
+static int checkpoint_save(seis_model *m, hid_t f, int sh, int nsrc, const float *sp,
+                           const float *src, int nrec, const float *rp, float *so) {
+    if (seis_forward_save(m, nsrc, sp, src, nrec, rp, so)) return err("%s", seis_last_error());
+    size_t bytes = 0;
+    if (seis_grad_saved_size(m, &bytes)) return err("%s", seis_last_error());
+    float *buf = (float *)malloc(bytes);
+    if (!buf) return err("out of host memory for the checkpoint (%zu bytes)", bytes);
+    int s = 0;
+    if (seis_grad_saved_read(m, buf)) {
+        s = err("%s", seis_last_error());
+    } else {
+        char name[64];
+        snprintf(name, sizeof(name), "src%d_dev0_checkpoint", sh);
+        long long d[1] = {(long long)(bytes / sizeof(float))};
+        s = write_f32(f, name, buf, 1, d);
+    }
+    free(buf);
+    return s;
+}
+
+/* Pass 2 (inputres=1, gradout=1): the shot's saved state read back from the
+ * checkpoint file instead of a forward run (checkpoint_h2d), then res_scale of
+ * the given residuals and the adjoint run.  *found = 0 when the file has no
+ * state for this shot. */
+static int checkpoint_adjoint(seis_model *m, hid_t f, int sh, int nsrc, const float *sp,
+                              const float *src, int nrec, const float *rp, const float *res,
+                              float *sres, int nout, int NT, int *found) {
+    char name[64];
+    snprintf(name, sizeof(name), "src%d_dev0_checkpoint", sh);
+    *found = f >= 0 && seis_h5_exists(f, name) > 0;
+    if (!*found) return 0;
+    const long long n = seis_h5_numel(f, name);
+    if (n <= 0) return err("empty checkpoint %s", name);
+    float *buf = (float *)malloc(sizeof(float) * (size_t)n);
+    if (!buf) return err("out of host memory for the checkpoint");
+    int s = 0;
+    if (seis_h5_read(f, name, SEIS_H5_F32, buf, n)) s = err("%s", seis_h5_last_error());
+    if (!s && (seis_shot_bind(m, nsrc, sp, src, nrec, rp) || seis_grad_saved_write(m, buf)))
+        s = err("%s", seis_last_error());
+    free(buf);
+    if (s) return s;
+    memcpy(sres, res, sizeof(float) * (size_t)nout * nrec * NT);
+    if (seis_res_scale(m, nrec, rp, sres) || seis_adjoint(m, sres))
+        return err("%s", seis_last_error());
+    return 0;
+}
+
 static int run(job_t *j) {
     seis_ctx *ctx = NULL;
     seis_model *m = NULL;
@@ -561,6 +613,7 @@ static int run(job_t *j) {
     const int NT = c->NT;
     int s = 0;
     float *sd = NULL, *sr = NULL, *so = NULL, *sres = NULL;
+    hid_t fck = -1;
     if (seis_ctx_create(0, &ctx)) return err("%s", seis_last_error());
     if (seis_model_create(ctx, c, j->par[0], j->par[1], j->par[2], j->par[3], j->par[4], &m)) {
         s = err("%s", seis_last_error());
@@ -586,6 +639,21 @@ static int run(job_t *j) {
         s = err("out of host memory");
         goto done;
     }
+    /* the checkpoint file: created by pass 1, read by pass 2 (without it,
+     * pass 2 recomputes each shot's forward run) */
+    if (j->inputres && !j->gradout && j->ckpt_path) {
+        remove(j->ckpt_path);
+        if ((fck = seis_h5_create(j->ckpt_path)) < 0) {
+            s = err("Could not create the checkpoint file %s", j->ckpt_path);
+            goto done;
+        }
+    } else if (j->inputres && j->gradout && j->ckpt_path) {
+        FILE *probe = fopen(j->ckpt_path, "rb");
+        if (probe) {
+            fclose(probe);
+            fck = seis_h5_open(j->ckpt_path, 0);
+        }
+    }
     for (int sh = 0; sh < j->ns && !s; ++sh) {
         const int nsrc = j->nsrc[sh], nrec = j->nrec[sh], g0 = j->rec0[sh];
         const float *sp = j->src_pos + (size_t)j->src0[sh] * 5;
@@ -602,7 +670,23 @@ static int run(job_t *j) {
         fprintf(stdout, "    shot %d of %d: %d source(s), %d receiver(s)\n", sh + 1, j->ns,
                 nsrc, nrec);
         fflush(stdout);
-        if (j->gradout && j->movout > 0 && j->nmov) {
+        int from_ckpt = 0;
+        if (j->inputres && j->gradout && !(j->movout > 0 && j->nmov) && fck >= 0) {
+            if (checkpoint_adjoint(m, fck, sh, nsrc, sp, src, nrec, rp, sr, sres, j->nout, NT,
+                                   &from_ckpt)) {
+                s = 1;
+                break;
+            }
+            /* no forward run in this pass: its seismograms are not produced */
+            if (from_ckpt) memset(so, 0, sizeof(float) * (size_t)j->nout * nrec * NT);
+        }
+        if (from_ckpt) {
+        } else if (j->inputres && !j->gradout && fck >= 0 && !(j->movout > 0)) {
+            if (checkpoint_save(m, fck, sh, nsrc, sp, src, nrec, rp, so)) {
+                s = 1;
+                break;
+            }
+        } else if (j->gradout && j->movout > 0 && j->nmov) {
             double rr[2] = {0, 0};
             if (gradient_movie(m, j, sh, nsrc, sp, src, nrec, rp, need_din ? sd : NULL,
                                j->inputres ? sr : NULL, so, sres, rr)) {
@@ -676,6 +760,7 @@ static int run(job_t *j) {
         }
     }
 done:
+    if (fck >= 0) seis_h5_close(fck);
     free(so);
     free(sd);
     free(sr);
@@ -714,6 +799,8 @@ static int file_names(const char *base, const char *din, files_t *fs) {
     NAME(rms, "_rms.mat");
     NAME(movie, "_movie.mat");
     NAME(res, "_res.mat");
+    /* one process: group id 0 (src/SeisCL_MPI.c:120-121) */
+    NAME(checkpoint, "_NP0_checkpoint.mat");
 #undef NAME
     if (din) over |= snprintf(fs->din, NAMELEN, "%s", din) >= NAMELEN;
     else over |= snprintf(fs->din, NAMELEN, "%s_din.mat", base) >= NAMELEN;
@@ -755,6 +842,7 @@ int seis_cli_main(int argc, char **argv) {
 
     job_t j;
     memset(&j, 0, sizeof(j));
+    j.ckpt_path = fs.checkpoint;
     int s = 0;
     hid_t f = seis_h5_open(fs.csts, 0);
     if (f < 0) {
@@ -781,7 +869,7 @@ int seis_cli_main(int argc, char **argv) {
             }
             seis_h5_close(fd);
         }
-    } else if (!s && j.inputres) {
+    } else if (!s && j.inputres && j.gradout) {
         hid_t fr = seis_h5_open(fs.res, 0);
         if (fr < 0) {
             s = err("Could not open the residuals file");

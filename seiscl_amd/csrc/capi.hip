@@ -959,11 +959,12 @@ static int shot_host(seis_model *m, int nsrc, const float *src_pos, const float 
     return 0;
 }
 
-// Copy a prepared shot into the device buffers of sd (sized for it) on the
-// transfer stream and wait for the copies.
-static int shot_upload(seis_model *m, const ShotHost &h, const float *src, ShotDev &sd) {
+// Copy a prepared shot into the device buffers of sd (sized for it) on
+// stream st (the transfer stream by default) and wait for the copies.
+static int shot_upload(seis_model *m, const ShotHost &h, const float *src, ShotDev &sd,
+                       hipStream_t st = nullptr) {
     const int nsrc = (int)h.scell.size(), nrec = (int)h.rcell.size(), NT = m->c.NT;
-    hipStream_t st = m->ctx->xfer;
+    if (!st) st = m->ctx->xfer;
     sd.nsrc = nsrc;
     sd.nrec = nrec;
     if (nsrc) {
@@ -1308,9 +1309,9 @@ static void step_forward(seis_model *m, int t, hipStream_t st) {
     if (m->c.freesurf && !m->a.fuse_surface) step_surface(m, st);
 }
 
-// Viscoelastic forward run for the gradient: the level-1 checkpoints of the
-// plan, then the seismograms.
-static int forward_save_visco(seis_model *m, float *out) {
+// Set-up of a viscoelastic gradient run: the moduli of the gradient
+// formulas, the checkpoint plan and its buffers.
+static int visco_prepare(seis_model *m) {
     auto &G = m->grad;
     const seis_csts &c = m->c;
     const int tmax = tmax_of(m);
@@ -1350,6 +1351,20 @@ static int forward_save_visco(seis_model *m, float *out) {
     if (grad_buffer(m, &G.ckpt, &G.ckpt_bytes, ckb, false)) return 1;
     if (grad_buffer(m, &G.hist, &G.hist_bytes, hib, false)) return 1;
     G.plan = P;
+    return 0;
+}
+
+// Viscoelastic forward run for the gradient: the level-1 checkpoints of the
+// plan, then the seismograms.
+static int forward_save_visco(seis_model *m, float *out) {
+    auto &G = m->grad;
+    const seis_csts &c = m->c;
+    const int tmax = tmax_of(m);
+    hipStream_t st = m->ctx->stream;
+    if (visco_prepare(m)) return 1;
+    const CkptPlan &P = G.plan;
+    const auto state = forward_state(m);
+    const size_t sbytes = state_bytes(state);
     for (int t = 0; t < tmax; ++t) {
         if (t % P.K[0] == 0 &&
             state_save(state, (char *)G.ckpt + (size_t)(t / P.K[0]) * sbytes, st))
@@ -1520,6 +1535,62 @@ static int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial, const Mov
 
 }  // namespace
 
+}  // extern "C"
+
+namespace {
+// Boundary-shell buffer of an elastic gradient run, tmax steps of every
+// saved wavefield.
+static int bnd_prepare(seis_model *m) {
+    auto &G = m->grad;
+    const long need = (long)tmax_of(m);
+    if (G.bnd_steps >= need) return 0;
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    if (G.bnd) {
+        m->bytes -= (size_t)G.bnd_steps * G.nvb * G.sh.n * 4;
+        m->ctx->release(G.bnd);
+        G.bnd = nullptr;
+        G.bnd_steps = 0;
+    }
+    void *p = nullptr;
+    const size_t bytes = (size_t)need * G.nvb * G.sh.n * 4;
+    if (m->ctx->alloc(bytes, &p)) return 1;
+    G.bnd = (float *)p;
+    G.bnd_steps = need;
+    m->bytes += bytes;
+    return 0;
+}
+
+// Header of a saved forward run copied to the host (seis_grad_saved_read).
+struct SavedHeader {
+    char magic[8];          // "SEISCKP1"
+    int32_t kind;           // 0: elastic (final state + boundary), 1: viscoelastic
+    int32_t tmax, n0, K0;
+    uint64_t state_bytes, payload_bytes;
+};
+static_assert(sizeof(SavedHeader) % 4 == 0, "float-sized header");
+
+static int saved_layout(seis_model *m, SavedHeader &h) {
+    auto &G = m->grad;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "SEISCKP1", 8);
+    const auto state = forward_state(m);
+    h.state_bytes = state_bytes(state);
+    h.tmax = tmax_of(m);
+    if (G.visco) {
+        h.kind = 1;
+        h.n0 = G.plan.n0;
+        h.K0 = G.plan.K[0];
+        h.payload_bytes = (uint64_t)h.n0 * h.state_bytes;
+    } else {
+        h.kind = 0;
+        h.payload_bytes = h.state_bytes + (uint64_t)h.tmax * G.nvb * G.sh.n * 4;
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
                       const float *rec_pos, float *out) {
     g_err.clear();
@@ -1531,22 +1602,7 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
         return forward_save_visco(m, out);
     }
     const int tmax = tmax_of(m);
-    const long need = (long)tmax;
-    if (G.bnd_steps < need) {
-        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
-        if (G.bnd) {
-            m->bytes -= (size_t)G.bnd_steps * G.nvb * G.sh.n * 4;
-            m->ctx->release(G.bnd);
-            G.bnd = nullptr;
-            G.bnd_steps = 0;
-        }
-        void *p = nullptr;
-        const size_t bytes = (size_t)need * G.nvb * G.sh.n * 4;
-        if (m->ctx->alloc(bytes, &p)) return 1;
-        G.bnd = (float *)p;
-        G.bnd_steps = need;
-        m->bytes += bytes;
-    }
+    if (bnd_prepare(m)) return 1;
     if (seis_shot_bind(m, nsrc, src_pos, src, nrec, rec_pos)) return 1;
     if (seis_launch_init(m)) return 1;
     hipStream_t st = m->ctx->stream;
@@ -1570,6 +1626,75 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
     HIP_TRY(hipStreamSynchronize(st));
     G.saved = true;
     return out ? seis_shot_read(m, out) : 0;
+}
+
+int seis_grad_saved_size(seis_model *m, size_t *bytes) {
+    g_err.clear();
+    if (!m->grad.saved) return fail("seis_grad_saved_size: no saved forward run is bound");
+    SavedHeader h;
+    saved_layout(m, h);
+    *bytes = sizeof(h) + h.payload_bytes;
+    return 0;
+}
+
+int seis_grad_saved_read(seis_model *m, float *host) {
+    g_err.clear();
+    auto &G = m->grad;
+    if (!G.saved) return fail("seis_grad_saved_read: no saved forward run is bound");
+    SavedHeader h;
+    saved_layout(m, h);
+    std::memcpy(host, &h, sizeof(h));
+    char *dst = (char *)host + sizeof(h);
+    hipStream_t st = m->ctx->stream;
+    if (h.kind == 1) {
+        if (h.payload_bytes)
+            HIP_TRY(hipMemcpyAsync(dst, G.ckpt, h.payload_bytes, hipMemcpyDeviceToHost, st));
+    } else {
+        for (auto &e : forward_state(m)) {
+            HIP_TRY(hipMemcpyAsync(dst, e.first, e.second, hipMemcpyDeviceToHost, st));
+            dst += e.second;
+        }
+        const size_t nb = h.payload_bytes - h.state_bytes;
+        if (nb) HIP_TRY(hipMemcpyAsync(dst, G.bnd, nb, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int seis_grad_saved_write(seis_model *m, const float *host) {
+    g_err.clear();
+    if (!m->shot_bound) return fail("seis_grad_saved_write: bind the shot first");
+    if (grad_setup(m)) return 1;
+    auto &G = m->grad;
+    G.saved = false;
+    SavedHeader h;
+    std::memcpy(&h, host, sizeof(h));
+    if (std::memcmp(h.magic, "SEISCKP1", 8))
+        return fail("seis_grad_saved_write: not a saved forward run of this library");
+    if (G.visco && visco_prepare(m)) return 1;
+    if (!G.visco && bnd_prepare(m)) return 1;
+    SavedHeader want;
+    saved_layout(m, want);
+    if (h.kind != want.kind || h.tmax != want.tmax || h.state_bytes != want.state_bytes ||
+        h.n0 != want.n0 || h.K0 != want.K0 || h.payload_bytes != want.payload_bytes)
+        return fail("seis_grad_saved_write: the saved forward run does not match this model "
+                    "(kind, time steps, state size or checkpoint plan differ)");
+    const char *src = (const char *)host + sizeof(h);
+    hipStream_t st = m->ctx->stream;
+    if (h.kind == 1) {
+        if (h.payload_bytes)
+            HIP_TRY(hipMemcpyAsync(G.ckpt, src, h.payload_bytes, hipMemcpyHostToDevice, st));
+    } else {
+        for (auto &e : forward_state(m)) {
+            HIP_TRY(hipMemcpyAsync(e.first, src, e.second, hipMemcpyHostToDevice, st));
+            src += e.second;
+        }
+        const size_t nb = h.payload_bytes - h.state_bytes;
+        if (nb) HIP_TRY(hipMemcpyAsync(G.bnd, src, nb, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    G.saved = true;
+    return 0;
 }
 
 int seis_misfit(const seis_csts *c, int nout, int nrec, const float *out, const float *din,

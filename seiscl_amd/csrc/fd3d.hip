@@ -18,6 +18,8 @@
 // term and in the same order (FMA contraction disabled in the build).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "fd_common.hpp"
 #include "seis_internal.hpp"
 
@@ -72,20 +74,39 @@ constexpr int kNTV = SEIS_NT_V;
 // slabs k, k + 8, ...): a contiguous split would give the two outer XCDs
 // every slab of the y-CPML strips, and the kernel lasts as long as its
 // slowest XCD.
+// With yg > 1 (nby a multiple of yg), yg adjacent slabs form a group whose
+// tiles are numbered z fastest, then the slab within the group, then x: the
+// blocks an XCD holds at once cover a few x-planes of all the group's slabs,
+// so the y-halo rows one slab reads are the rows its neighbour slab is
+// reading at the same time.  Groups are dealt to the XCDs as slabs are.
 template <int BY>
-__device__ __forceinline__ bool tile3(int nbz, int NX, int NY, unsigned nb, int &z, int &y,
-                                      int &x) {
+__device__ __forceinline__ bool tile3(int nbz, int NX, int NY, unsigned nb, int yg, int &z,
+                                      int &y, int &x) {
     unsigned tile;
     xcd_tile(nb, tile);
     if (tile >= nb) return false;
     const int bz = tile % nbz;
-    const unsigned r = tile / nbz;
-    x = r % NX;
-    int by = r / NX;
+    unsigned r = tile / nbz;
     const int nby = (NY + BY - 1) / BY;
-    if (SEIS_SLAB_IL && nby % 8 == 0) {
-        const int m = nby / 8;
-        by = (by % m) * 8 + by / m;
+    int by;
+    if (yg > 1) {
+        const int sub = r % yg;
+        r /= yg;
+        x = r % NX;
+        int grp = r / NX;
+        const int ng = nby / yg;
+        if (SEIS_SLAB_IL && ng % 8 == 0) {
+            const int m = ng / 8;
+            grp = (grp % m) * 8 + grp / m;
+        }
+        by = grp * yg + sub;
+    } else {
+        x = r % NX;
+        by = r / NX;
+        if (SEIS_SLAB_IL && nby % 8 == 0) {
+            const int m = nby / 8;
+            by = (by % m) * 8 + by / m;
+        }
     }
     z = bz * BZ + threadIdx.x;
     y = by * BY + threadIdx.y;
@@ -99,11 +120,11 @@ constexpr int v3d_waves(int H, int rbytes) { return rbytes == 8 ? 2 : H <= 2 ? 6
 
 template <typename R, typename S, int H>
 __global__ __launch_bounds__(64 * BYV) __attribute__((amdgpu_waves_per_eu(v3d_waves(H, sizeof(R)), 8)))
-void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
+void k_v3d(FDArgs<R> a, int nbz, unsigned nb, int yg) {
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3<BYV>(nbz, g.NX, g.NY, nb, z, y, x)) return;
+    if (!tile3<BYV>(nbz, g.NX, g.NY, nb, yg, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy;
@@ -203,11 +224,11 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb) {
 }
 
 template <typename R, typename S, int H, int L, bool FS>
-__global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz, unsigned nb) {
+__global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz, unsigned nb, int yg) {
     using T = Store<R, S>;
     const Layout &g = a.g;
     int z, y, x;
-    if (!tile3<bys<R, S>()>(nbz, g.NX, g.NY, nb, z, y, x)) return;
+    if (!tile3<bys<R, S>()>(nbz, g.NX, g.NY, nb, yg, z, y, x)) return;
     if (z >= g.NZ || y >= g.NY) return;
     const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
     const long sx = g.sx, sy = g.sy, nf = g.nfield;
@@ -449,12 +470,36 @@ void grid3(const Layout &g, int &nbz, unsigned &nb) {
     nb = (unsigned)nbz * g.NX * nby;
 }
 
+// Slabs per y group (tile3): the largest of want, want/2, ... that divides
+// the slab count; want comes from SEIS_YGROUP_V / SEIS_YGROUP_S when set
+// (tuning), else from the defaults below.
+#ifndef SEIS_YG_V
+#define SEIS_YG_V 1
+#endif
+#ifndef SEIS_YG_S
+#define SEIS_YG_S 1
+#endif
+inline int env_or(const char *name, int def) {
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : def;
+}
+template <int BY>
+int ygroup(const Layout &g, int want) {
+    const int nby = (g.NY + BY - 1) / BY;
+    int G = want < 1 ? 1 : want;
+    while (G > 1 && nby % G) G /= 2;
+    return G;
+}
+
 template <typename R, typename S, int H>
 void run_v3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
     grid3<BYV>(a.g, nbz, nb);
-    hipLaunchKernelGGL((k_v3d<R, S, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BYV), 0, st, a, nbz, nb);
+    static const int want = env_or("SEIS_YGROUP_V", SEIS_YG_V);
+    const int yg = ygroup<BYV>(a.g, want);
+    hipLaunchKernelGGL((k_v3d<R, S, H>), dim3(((nb + 7) / 8) * 8), dim3(BZ, BYV), 0, st, a, nbz,
+                       nb, yg);
 }
 
 template <typename R, typename S, int H, int L>
@@ -462,11 +507,13 @@ void run_s3d(const FDArgs<R> &a, hipStream_t st) {
     int nbz;
     unsigned nb;
     grid3<bys<R, S>()>(a.g, nbz, nb);
+    static const int want = env_or("SEIS_YGROUP_S", SEIS_YG_S);
+    const int yg = ygroup<bys<R, S>()>(a.g, want);
     const dim3 grid(((nb + 7) / 8) * 8), blk(BZ, bys<R, S>());
     if (a.freesurf && a.fuse_surface)
-        hipLaunchKernelGGL((k_s3d<R, S, H, L, true>), grid, blk, 0, st, a, nbz, nb);
+        hipLaunchKernelGGL((k_s3d<R, S, H, L, true>), grid, blk, 0, st, a, nbz, nb, yg);
     else
-        hipLaunchKernelGGL((k_s3d<R, S, H, L, false>), grid, blk, 0, st, a, nbz, nb);
+        hipLaunchKernelGGL((k_s3d<R, S, H, L, false>), grid, blk, 0, st, a, nbz, nb, yg);
 }
 
 template <typename R, typename S, int H>

@@ -276,6 +276,25 @@ class Model:
                                          rp.shape[0], fptr(rp), fptr(out)))
         return [np.ascontiguousarray(o.T) for o in out]
 
+    def saved_read(self) -> np.ndarray:
+        """The saved forward run of the last forward_save as one host array
+        (checkpoint_d2h of src/time_stepping.c:92-168): what seis_adjoint
+        needs, so that a later process can run the adjoint without the
+        forward run."""
+        n = ctypes.c_size_t()
+        check(self.lib.seis_grad_saved_size(self.handle, ctypes.byref(n)))
+        buf = np.zeros(n.value // 4, dtype=np.float32)
+        check(self.lib.seis_grad_saved_read(self.handle, fptr(buf)))
+        return buf
+
+    def saved_write(self, src_pos, src, rec_pos, saved: np.ndarray):
+        """Bind the shot and restore its saved forward run (checkpoint_h2d,
+        src/time_stepping.c:170-270); adjoint() then runs without a forward
+        run."""
+        self.bind_shot(src_pos, src, rec_pos, init=False)
+        buf = np.ascontiguousarray(saved, dtype=np.float32)
+        check(self.lib.seis_grad_saved_write(self.handle, fptr(buf)))
+
     def adjoint(self, res, scale: bool = True, movout: int = 0, movie_vars=None):
         """Adjoint run of the shot of the last forward_save with adjoint
         sources res (list of (NT, nrec)); res_scale is applied when scale.

@@ -99,6 +99,8 @@ class SeisCL:
         self._res_in = None
         self._results = {}
         self._ctx = None
+        # saved forward runs of an inputres pass 1 run in process, by shot id
+        self._saved = {}
 
     # ---- file names (SeisCL.py:355-371) ------------------------------------
     @property
@@ -349,8 +351,20 @@ class SeisCL:
                 self._movie_frames(m.adjoint(r, scale=False, movout=int(self.movout),
                                              movie_vars=list(mov)), mov, len(mov_shots))
                 mov_shots.append(sid)
+            elif grad and self.inputres and sid in self._saved:
+                # pass 2 of the inputres flow: the forward run saved by pass 1
+                # (checkpoint_h2d, src/time_stepping.c:856-859); no seismograms
+                m.saved_write(*shot, self._saved.pop(sid))
+                r = m.res_scale([r[:, rcol] for r in self._res_in], shot[2])
+                m.adjoint(r, scale=False)
+                o = [np.zeros((int(self.NT), rcol.size), np.float32) for _ in names]
             elif grad and self.inputres:
                 o, _ = m.gradient(*shot, res=[r[:, rcol] for r in self._res_in])
+            elif self.inputres and self.movout <= 0:
+                # pass 1 of the inputres flow: forward run with its saved state
+                # kept for pass 2 (checkpoint_d2h, src/time_stepping.c:850-853)
+                o = m.forward_save(*shot)
+                self._saved[sid] = m.saved_read()
             elif grad:
                 o = m.forward_save(*shot)
             elif self.movout > 0:

@@ -327,3 +327,52 @@ def test_visco_gradient_files(oracle_lib, tmp_path):
         ref = _cropped(tot[name], seis.nab)
         assert np.abs(ref).max() > 0, name
         assert rel_l2(g, ref) <= 1e-5, name
+
+
+@pytest.mark.parametrize("files,gradfix,L", [(True, 223, 0), (True, 0, 0), (False, 223, 0),
+                                              (True, 223, 2)])
+def test_inputres_two_pass(oracle_lib, tmp_path, files, gradfix, L):
+    """The reference's two-pass inputres flow (src/time_stepping.c:731-745,
+    850-859): pass 1 (inputres=1, gradout=0) models the shots and writes
+    each shot's saved forward run to <base>_NP0_checkpoint.mat (in process:
+    keeps it); the residuals are made outside; pass 2 (set_backward) reads
+    the checkpoint instead of modeling again and runs the adjoint.  Pass 1's
+    seismograms and pass 2's gradients equal the oracle's, in the reference's
+    gradient mode (gradfix 0) and the corrected one."""
+    kw = dict(inputres=1, gradfix=gradfix, files=files)
+    if L:
+        kw.update(L=L, FL=np.array([8.0, 30.0]))
+    seis, model, _, csts = _fwi_case(tmp_path, **kw)
+    if L:
+        from seiscl_amd.workloads import q_to_tau
+        tau = q_to_tau(50.0, 5.0, 40.0, [8.0, 30.0])
+        model["taup"] = np.full(model["vp"].shape, tau, np.float32)
+        model["taus"] = np.full(model["vp"].shape, tau, np.float32)
+        csts = dict(csts, L=L, FL=np.array([8.0, 30.0]))
+    print(f"[cli] inputres two-pass files={files} gradfix={gradfix} L={L}", flush=True)
+    seis.set_forward([0, 1], model, withgrad=False)
+    seis.execute()
+    ckpt = os.path.join(str(tmp_path), "SeisCL_NP0_checkpoint.mat")
+    assert os.path.exists(ckpt) == files
+    dout = seis.read_data()
+    refs = []
+    for rc, shot in _shots(seis):
+        ref = oracle_lib.forward(model, *shot, **csts)
+        for d, r in zip(dout, ref):
+            assert rel_l2(d[:, rc], r) <= 1e-6
+        refs.append((rc, shot))
+    # adjoint sources made outside the program: the modeled data scaled
+    res = [(0.5 * d).astype(np.float32) for d in dout]
+    seis.set_backward(res)
+    seis.execute()
+    gout = seis.read_grad()
+    tot = {k: 0 for k in seis.params}
+    for rc, shot in refs:
+        g = oracle_lib.gradient(model, *shot, res=[r[:, rc] for r in res], gradfix=gradfix,
+                                **csts)
+        for k in tot:
+            tot[k] = tot[k] + g["grad"][k].astype(np.float64)
+    for name, g in zip(seis.params, gout):
+        ref = _cropped(tot[name], seis.nab)
+        assert np.abs(ref).max() > 0, name
+        assert rel_l2(g, ref) <= 1e-5, (name, rel_l2(g, ref))
