@@ -168,6 +168,26 @@ int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos,
                       const float *src, int nrec, const float *rec_pos,
                       float *out);
 
+/* Size of the fixed device area seis_forward_shot_static works in (one per */
+/* GPU, held until the process ends).                                         */
+#define SEIS_STATIC_SHOT_BYTES ((size_t)64 << 20)
+
+/* seis_forward_shot without allocating device memory.  No counterpart in   */
+/* the reference: for callers that cannot call hipMalloc (code under HIP    */
+/* graph capture, serving loops, harnesses that forbid allocation).  The     */
+/* shot's source and receiver tables, its source samples, the seismogram    */
+/* recording and its transpose live in a static area of                      */
+/* SEIS_STATIC_SHOT_BYTES per GPU; a shot needing more (about                 */
+/* 8 x NT x nrec x outputs bytes, twice that in FP64 runs, plus 4 x NT x nsrc) */
+/* is refused with an error.  Inputs, outputs and results are those of      */
+/* seis_forward_shot; the model's wavefields must already exist              */
+/* (seis_model_create).  Afterwards no shot is bound to the model.  Every    */
+/* call on one GPU shares that GPU's area: calls that overlap on one GPU     */
+/* race.  Models in a serving batch are refused.                             */
+int seis_forward_shot_static(seis_model *m, int nsrc, const float *src_pos,
+                             const float *src, int nrec, const float *rec_pos,
+                             float *out);
+
 /* Copy one wavefield ("vx", "sxx", ...) to the host in the reference's     */
 /* padded layout, Pi(N+FDORDER) floats (used by movout, src/time_stepping.c: */
 /* 272-350, and by parity tests).                                              */

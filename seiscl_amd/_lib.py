@@ -92,6 +92,7 @@ SIGNATURES = [
     ("seis_grad_plan", _I, [_P] + [ctypes.POINTER(_I)] * 4 + [ctypes.POINTER(ctypes.c_size_t)]),
     ("seis_run_steps", _I, [_P, _I, _I]),
     ("seis_forward_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
+    ("seis_forward_shot_static", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),
     ("seis_read_field", _I, [_P, ctypes.c_char_p, _FP]),
     ("seis_grad_zero", _I, [_P]),
     ("seis_forward_save", _I, [_P, _I, _FP, _FP, _I, _FP, _FP]),

@@ -35,8 +35,47 @@ int fail(const std::string &msg) {
 
 }  // namespace
 
+// The fixed device area of seis_forward_shot_static.  In host code the name
+// stands for a host-side shadow, not for device memory: host code uses only
+// the device address that hipGetSymbolAddress returns for the current GPU
+// (kept per device in g_scratch_dev, set up once by seis_ctx_create).
+static __device__ __align__(256) unsigned char scratch[SEIS_STATIC_SHOT_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of this GPU's static area (the device must be current).
+int scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return 0;   // the static path refuses later
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = p;
+    }
+    *out = g_scratch_dev[device];
+    return 0;
+}
+
+// Seismograms recorded time-major ([NT][nout][nrec], float or double) into
+// the boundary's [nout][nrec][NT] floats, in the device area next to them.
+template <typename T>
+__global__ void k_seis_transpose(const T *in, float *out, int NT, int nout, int nrec) {
+    const long n = (long)NT * nout * nrec;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+         k += (long)gridDim.x * blockDim.x) {
+        const long r = k % nrec, o = (k / nrec) % nout, t = k / ((long)nrec * nout);
+        out[(o * nrec + r) * NT + t] = (float)in[k];
+    }
+}
+}  // namespace
+
 struct seis_ctx {
     int device = 0;
+    void *scratch = nullptr;        // device address of this GPU's static shot area
     hipStream_t stream = nullptr;   // compute
     hipStream_t xfer = nullptr;     // model set-up uploads and result copies
     size_t bytes = 0;               // bytes handed out and not released
@@ -126,6 +165,7 @@ struct seis_model {
     RefParams rp;
     bool setting_up = false;         // buffers zero-filled on the transfer stream
     hipEvent_t done_ev = nullptr;    // batch: compute-stream point after the last step
+    bool in_batch = false;           // held by a serving batch slot
     std::vector<int> rcell_host;     // receiver cells of the bound shot
     // gradient state, allocated on first use
     struct Grad {
@@ -705,6 +745,7 @@ int seis_ctx_create(int device_ordinal, seis_ctx **out) {
     auto ctx = std::make_unique<seis_ctx>();
     ctx->device = device_ordinal;
     HIP_TRY(hipSetDevice(device_ordinal));
+    if (scratch_address(device_ordinal, &ctx->scratch)) return 1;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->xfer, hipStreamNonBlocking));
     *out = ctx.release();
@@ -1148,6 +1189,84 @@ int seis_forward_shot(seis_model *m, int nsrc, const float *src_pos, const float
     const int tmax = m->c.tmax > 0 ? std::min(m->c.tmax, m->c.NT) : m->c.NT;
     if (seis_run_steps(m, 0, tmax)) return 1;
     return out ? seis_shot_read(m, out) : 0;
+}
+
+int seis_forward_shot_static(seis_model *m, int nsrc, const float *src_pos, const float *src,
+                             int nrec, const float *rec_pos, float *out) {
+    g_err.clear();
+    if (!m) return fail("seis_forward_shot_static: no model");
+    if (m->in_batch) return fail("seis_forward_shot_static: the model is held by a serving batch");
+    if (nsrc < 0 || nrec < 0) return fail("seis_forward_shot_static: negative source or receiver count");
+    void *base = m->ctx->scratch;
+    if (!base) return fail("seis_forward_shot_static: no static area on this device");
+    ShotHost h;
+    if (shot_host(m, nsrc, src_pos, src, nrec, rec_pos, h)) return 1;
+    // carve the shot's device tables out of the area, 256-byte aligned
+    const int NT = m->c.NT, nout = (int)m->outn.size();
+    const int ns = std::max(nsrc, 1), nr = std::max(nrec, 1);
+    const size_t elem = m->prec == P_FP64 ? 8 : 4;
+    const size_t nseis = (size_t)nr * NT * nout;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_cell = carve(ns * 4), o_type = carve(ns * 4), o_k = carve(ns * 4);
+    const size_t o_src = carve((size_t)ns * NT * 4), o_rec = carve(nr * 4);
+    const size_t o_out = carve(nseis * elem), o_tr = carve(nseis * 4);
+    if (off > SEIS_STATIC_SHOT_BYTES)
+        return fail("seis_forward_shot_static: the shot needs " + std::to_string(off) +
+                    " bytes of device memory, the static area holds " +
+                    std::to_string((size_t)SEIS_STATIC_SHOT_BYTES));
+    char *b = (char *)base;
+    ShotDev sd;
+    sd.src_cell = (int *)(b + o_cell);
+    sd.src_type = (int *)(b + o_type);
+    sd.src_k = (int *)(b + o_k);
+    sd.src = (float *)(b + o_src);
+    sd.rec_cell = (int *)(b + o_rec);
+    sd.out = (float *)(b + o_out);
+    float *tr = (float *)(b + o_tr);
+    hipStream_t st = m->ctx->stream;
+    // the model's own shot buffers are set aside for this run and kept
+    const ShotDev own = m->shot;
+    m->shot = sd;
+    m->shot_bound = false;
+    m->grad.saved = false;
+    int rc = shot_upload(m, h, src, m->shot, st);
+    if (!rc) {
+        zero_state(m);
+        launch_fill(sd.out, nseis * elem, st);
+        const int tmax = m->c.tmax > 0 ? std::min(m->c.tmax, NT) : NT;
+        for (int t = 0; t < tmax; ++t) {
+            // step order of src/time_stepping.c:785-816, as seis_run_steps
+            step_sources(m, t, 1, st);
+            step_v(m, st);
+            step_s(m, st);
+            if (m->c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+            step_varsout(m, t, st);
+        }
+        const long n = (long)nseis;
+        const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+        if (n > 0 && nrec > 0) {
+            if (m->prec == P_FP64)
+                hipLaunchKernelGGL(k_seis_transpose<double>, dim3(blocks), dim3(256), 0, st,
+                                   (const double *)sd.out, tr, NT, nout, nrec);
+            else
+                hipLaunchKernelGGL(k_seis_transpose<float>, dim3(blocks), dim3(256), 0, st,
+                                   (const float *)sd.out, tr, NT, nout, nrec);
+        }
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && out && nrec > 0 && nout > 0)
+            e = hipMemcpyAsync(out, tr, (size_t)nrec * NT * nout * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            rc = fail(std::string("seis_forward_shot_static: ") + hipGetErrorString(e));
+    }
+    m->shot = own;
+    m->shot_bound = false;
+    return rc;
 }
 
 int seis_model_set_profiling(seis_model *m, int on) {
@@ -1971,7 +2090,10 @@ int seis_batch_destroy(seis_batch *b) {
     if (!b) return 0;
     (void)hipStreamSynchronize(b->ctx->stream);
     for (int i = 0; i < b->max_slots; ++i)
-        if (b->model[i]) b->model[i]->done_ev = nullptr;
+        if (b->model[i]) {
+            b->model[i]->done_ev = nullptr;
+            b->model[i]->in_batch = false;
+        }
     for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->table_ev[k]);
     for (auto e : b->step_ev) (void)hipEventDestroy(e);
     b->ctx->release(b->dslots);
@@ -2013,6 +2135,7 @@ int seis_batch_add(seis_batch *b, seis_model *m, int *slot) {
                            b->ctx->stream));
     b->model[k] = m;
     m->done_ev = nullptr;
+    m->in_batch = true;
     b->t[k] = 0;
     b->tmax[k] = sl.tmax;
     b->dirty = true;
@@ -2028,6 +2151,7 @@ int seis_batch_remove(seis_batch *b, int slot) {
                            &b->host_active[slot], sizeof(int), hipMemcpyHostToDevice,
                            b->ctx->stream));
     b->model[slot]->done_ev = nullptr;
+    b->model[slot]->in_batch = false;
     b->model[slot] = nullptr;
     b->dirty = true;
     return 0;
