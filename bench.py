@@ -213,13 +213,17 @@ def forward2d_main(args):
     m.bind_shot(*shots[0])
     m.run_steps(0, NT)                       # warm-up: one whole shot
     m.bind_shot(*shots[0])
-    m.set_profiling(True)
     ctx.sync()
     t0 = time.perf_counter()
-    m.run_steps(0, NT)
+    m.run_steps(0, NT)                       # replayed step graphs, no per-kernel events
     ctx.sync()
     wall = time.perf_counter() - t0
+    # kernel times from a second shot with events around every update
+    m.bind_shot(*shots[0])
+    m.set_profiling(True)
+    m.run_steps(0, NT)
     tm = m.timing()
+    m.set_profiling(False)
     m.close()
     avg_v, avg_s = tm["update_v_ms"] / NT, tm["update_s_ms"] / NT
     ach = 44 * ncell / (avg_s / 1e3) / 1e9
@@ -415,15 +419,23 @@ def gradient3d_main(args):
         "adj_s_ms": round(adj_s, 4), "adj_v_ms": round(adj_v, 4),
         "adj_v_gbs": round(b_v * ncell / (adj_v / 1e3) / 1e9, 1),
     }
-    if args.check_fp32:
-        # the first shot again in FP32 storage: gradient error of the storage
+    def rel(a, b):
+        return {k: float(np.linalg.norm(a[k].astype(np.float64) - b[k])
+                         / np.linalg.norm(b[k].astype(np.float64))) for k in b}
+
+    if args.check_fp32 or args.check_fp64:
+        # the first shot again in FP32 storage (and in FP64): gradient error
+        # of the storage and of FP32 arithmetic, every parameter
         _, g32, p32, _ = run(0, shots[:1])
         _, g1, _, _ = run(args.fp16, shots[:1])
         cfg["fp32_plan"] = p32
-        cfg["rel_l2_vs_fp32"] = {k: float(np.linalg.norm(g1[k].astype(np.float64) - g32[k])
-                                          / np.linalg.norm(g32[k].astype(np.float64)))
-                                 for k in g32}
+        cfg["rel_l2_vs_fp32"] = rel(g1, g32)
         log(f"gradient error against FP32 storage (shot 0): {cfg['rel_l2_vs_fp32']}")
+        if args.check_fp64:
+            _, g64, p64, _ = run(64, shots[:1])
+            cfg["fp64_plan"] = p64
+            cfg["rel_l2_vs_fp64"] = {f"fp16={args.fp16}": rel(g1, g64), "fp32": rel(g32, g64)}
+            log(f"gradient error against FP64 (shot 0): {cfg['rel_l2_vs_fp64']}")
     total = fwd + adj
     out = {
         "metric": METRIC, "value": round(len(shots) * NT * ncell / total / 1e9, 4),
@@ -495,6 +507,8 @@ def main():
                     help="gradient3d: wavefield storage (2 FP16, 1 FP32 scaled, 0 FP32)")
     ap.add_argument("--check-fp32", action="store_true",
                     help="gradient3d: rerun shot 0 with FP32 storage and report the error")
+    ap.add_argument("--check-fp64", action="store_true",
+                    help="gradient3d: also rerun shot 0 in FP64 and report both errors")
     ap.add_argument("--nt", type=int, default=0, help="gradient2d: time steps (default 3000)")
     ap.add_argument("--fd-cells", type=int, default=0,
                     help="gradient2d: finite-difference check on this many cells per parameter")

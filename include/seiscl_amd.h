@@ -129,6 +129,13 @@ int seis_model_destroy(seis_model *m);
 /* Replace the material parameters of an existing model (same grid).          */
 int seis_model_set_params(seis_model *m, const float *p0, const float *p1,
                           const float *rho, const float *taup, const float *taus);
+/* The same with the parameters in device memory of the model's GPU (e.g. */
+/* torch tensors, unpadded in the layout above), ready when called.  The    */
+/* transforms run on the host as for seis_model_set_params (Init_model's    */
+/* host side, src/Init_model.c:40-97), after one device-to-host copy.         */
+int seis_model_set_params_device(seis_model *m, const float *dp0, const float *dp1,
+                                 const float *drho, const float *dtaup,
+                                 const float *dtaus);
 size_t seis_model_footprint(const seis_model *m);
 
 /* number / names of seismogram outputs for the model's seisout              */
@@ -152,6 +159,9 @@ int seis_launch_varsout(seis_model *m, int nt);
 /* D2H of the bound shot's seismograms, [nout][nrec][NT] (reduce_seis,       */
 /* src/time_stepping.c:25-90).                                                */
 int seis_shot_read(seis_model *m, float *out);
+/* The same into device memory of the model's GPU, [nout][nrec][NT] floats  */
+/* (reduce_seis, src/time_stepping.c:25-90, without leaving the GPU).         */
+int seis_shot_read_device(seis_model *m, float *dout);
 /* The same in double, for models running in FP64 (FP16=64), whose receivers */
 /* record in double.  No counterpart in the reference, which has no FP64     */
 /* mode; finite-difference checks of gradients need the digits.             */
@@ -257,6 +267,10 @@ int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos,
 /* sources res [nout][nrec][NT]: steps tmax-1 .. tmin+1 (src/time_stepping.c:*/
 /* 869-940); adds this shot's gradient to the accumulators.                 */
 int seis_adjoint(seis_model *m, const float *res);
+/* seis_adjoint with the adjoint sources in device memory ([nout][nrec][NT],*/
+/* ready when called); res_scale != 0 applies res_scale (as seis_res_scale  */
+/* with the bound shot's receivers) on the device first.                     */
+int seis_adjoint_device(seis_model *m, const float *dres, int res_scale);
 /* seis_adjoint recording the movie of the gradient run (movout with gradout,*/
 /* src/time_stepping.c:926-929, frame index :317-320): after the adjoint    */
 /* step t with t % movout == 0, frame (t+1)/movout - 1 of each wavefield    */
@@ -280,6 +294,10 @@ int seis_grad_read(seis_model *m, float *grad);
 /* accumulated in double by the time-domain adjoint kernels from the whole */
 /* memory term of each step.                                                */
 int seis_grad_read_tau(seis_model *m, float *gtau);
+/* seis_grad_read and (dgtau not NULL) seis_grad_read_tau into device       */
+/* memory of the model's GPU (gradout of src/time_stepping.c:948-1003 for a  */
+/* caller whose tensors live there); the transform runs on the host.         */
+int seis_grad_read_device(seis_model *m, float *dgrad, float *dgtau);
 /* The approximate Hessian accumulated with the gradient when Hout=1,       */
 /* [3][Pi N] in the same order and transform (H<name> of the gradient file,  */
 /* src/writehdf5.c:275-283, src/calc_grad.c:1028-1094).                      */

@@ -75,6 +75,7 @@ SIGNATURES = [
                                ctypes.POINTER(_P)]),
     ("seis_model_destroy", _I, [_P]),
     ("seis_model_set_params", _I, [_P, _FP, _FP, _FP, _FP, _FP]),
+    ("seis_model_set_params_device", _I, [_P, _P, _P, _P, _P, _P]),
     ("seis_model_footprint", ctypes.c_size_t, [_P]),
     ("seis_out_count", _I, [_P]),
     ("seis_out_name", ctypes.c_char_p, [_P, _I]),
@@ -86,6 +87,7 @@ SIGNATURES = [
     ("seis_launch_surface", _I, [_P]),
     ("seis_launch_varsout", _I, [_P, _I]),
     ("seis_shot_read", _I, [_P, _FP]),
+    ("seis_shot_read_device", _I, [_P, _P]),
     ("seis_shot_read_f64", _I, [_P, ctypes.POINTER(ctypes.c_double)]),
     ("seis_hess_read", _I, [_P, _FP]),
     ("seis_grad_set_memory", _I, [_P, ctypes.c_size_t]),
@@ -105,9 +107,11 @@ SIGNATURES = [
                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     ("seis_res_scale", _I, [_P, _I, _FP, _FP]),
     ("seis_adjoint", _I, [_P, _FP]),
+    ("seis_adjoint_device", _I, [_P, _P, _I]),
     ("seis_adjoint_movie", _I, [_P, _FP, _I, _I, ctypes.POINTER(ctypes.c_char_p), _FP]),
     ("seis_grad_read", _I, [_P, _FP]),
     ("seis_grad_read_tau", _I, [_P, _FP]),
+    ("seis_grad_read_device", _I, [_P, _P, _P]),
     ("seis_gradient_shot", _I, [_P, _I, _FP, _FP, _I, _FP, _FP, _FP, _FP,
                                 ctypes.POINTER(ctypes.c_double)]),
     ("seis_batch_create", _I, [_P, _I, ctypes.POINTER(_P)]),
@@ -139,6 +143,15 @@ def load() -> ctypes.CDLL:
         raise SeisError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() or "
             "make -C seiscl_amd/csrc")
+    # The library and PyTorch-ROCm link HIP runtimes of the same soname
+    # (libamdhip64.so.7), so a process holds one of them: the one loaded
+    # first.  With the library's loaded first, torch finds no HIP device; with
+    # torch's loaded first, both run on it.  So torch (when installed) is
+    # loaded first, and device tensors can pass into the C-ABI.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

@@ -23,14 +23,27 @@ from .engine import PARAM_NAMES, Context, Model
 
 class Propagator:
     """A model resident on one GPU plus the bookkeeping the autograd
-    Function needs.  csts are SeisCL.py-style constants (as for Model)."""
+    Function needs.  csts are SeisCL.py-style constants (as for Model).
+
+    ``param_names`` are the differentiable parameters: the three of
+    param_type 0-2 (vp, vs, rho / M, mu, rho / Ip, Is, rho) and, for a
+    viscoelastic model (L > 0), taup and taus.  seismic() also accepts only
+    the first three for a viscoelastic model; taup and taus then keep the
+    values the Propagator was created with."""
 
     def __init__(self, ctx: Context, params: dict, **csts):
         self.model = Model(ctx, params, **csts)
-        self.param_type = self.model.csts.param_type
-        self.param_names = list(PARAM_NAMES.get(self.param_type, ("vp", "vs", "rho")))
-        if self.model.csts.ND == 22:
+        c = self.model.csts
+        self.param_type = c.param_type
+        if c.ND == 22:
             raise ValueError("gradients need the elastic case (ND 2 or 3)")
+        if self.param_type not in PARAM_NAMES:
+            raise ValueError("gradients need param_type 0, 1 or 2")
+        base = list(PARAM_NAMES[self.param_type])
+        self.visco = c.L > 0
+        self.param_names = base + (["taup", "taus"] if self.visco else [])
+        self._fixed_tau = ({n: np.asarray(params[n], np.float32) for n in ("taup", "taus")}
+                           if self.visco else {})
         self._token = 0
         self._holder = None       # token of the forward pass whose last shot is saved
 
@@ -47,14 +60,36 @@ def _np(t: torch.Tensor) -> np.ndarray:
 
 
 class _Seismic(torch.autograd.Function):
+    """Forward: every shot with boundary saving; backward: the adjoint runs
+    (recomputing the forward run of every shot but the last).  With the
+    parameters on the GPU, parameters, seismograms, adjoint sources and
+    gradients stay in device memory (the C-ABI's *_device entry points);
+    with CPU tensors the same runs go through host arrays."""
+
     @staticmethod
     def forward(ctx, prop: Propagator, shots, *params):
-        p = {n: _np(t) for n, t in zip(prop.param_names, params)}
-        prop.model.set_params(p)
+        names = prop.param_names[:len(params)]
+        dev = params[0].is_cuda
+        m = prop.model
+        if dev:
+            p = dict(zip(names, params))
+            for n in prop.param_names[len(params):]:
+                p[n] = torch.from_numpy(prop._fixed_tau[n]).to(params[0].device)
+            p = {n: t.detach() for n, t in p.items()}
+            m.set_params_device(p)
+        else:
+            p = {n: _np(t) for n, t in zip(names, params)}
+            for n in prop.param_names[len(params):]:
+                p[n] = prop._fixed_tau[n]
+            m.set_params(p)
         outs = []
         nout = len(prop.out_names)
         for sp, s, rp in shots:
-            outs.extend(prop.model.forward_save(sp, s, rp))
+            if dev:
+                o = m.forward_save_device(sp, s, rp)
+                outs.extend(o[q].t().contiguous() for q in range(nout))
+            else:
+                outs.extend(torch.from_numpy(o) for o in m.forward_save(sp, s, rp))
         prop._token += 1
         prop._holder = prop._token
         ctx.prop = prop
@@ -62,17 +97,19 @@ class _Seismic(torch.autograd.Function):
         ctx.token = prop._token
         ctx.nout = nout
         ctx.params = p
+        ctx.names = names
+        ctx.dev = dev
         ctx.device = params[0].device
-        return tuple(torch.from_numpy(o).to(ctx.device) for o in outs)
+        return tuple(o.to(ctx.device) for o in outs)
 
     @staticmethod
     def backward(ctx, *grads):
-        prop, shots, nout = ctx.prop, ctx.shots, ctx.nout
+        prop, shots, nout, dev = ctx.prop, ctx.shots, ctx.nout, ctx.dev
         m = prop.model
         # the model may hold other parameters or another shot's boundaries
         fresh = prop._holder == ctx.token
         if not fresh:
-            m.set_params(ctx.params)
+            (m.set_params_device if dev else m.set_params)(ctx.params)
         m.grad_zero()
         for k in reversed(range(len(shots))):
             g = grads[k * nout:(k + 1) * nout]
@@ -80,14 +117,26 @@ class _Seismic(torch.autograd.Function):
                 continue
             sp, s, rp = shots[k]
             nt, nrec = m.csts.NT, np.asarray(rp).shape[1]
-            res = [_np(x) if x is not None else np.zeros((nt, nrec), np.float32) for x in g]
             if not (fresh and k == len(shots) - 1):
-                m.forward_save(sp, s, rp)
-            m.adjoint(res)
+                if dev:
+                    m.forward_save_device(sp, s, rp)
+                else:
+                    m.forward_save(sp, s, rp)
+            if dev:
+                res = torch.stack([x.t() if x is not None else
+                                   torch.zeros((nrec, nt), device=ctx.device) for x in g])
+                m.adjoint_device(res)
+            else:
+                res = [_np(x) if x is not None else np.zeros((nt, nrec), np.float32) for x in g]
+                m.adjoint(res)
             fresh = False
         prop._holder = None
-        gr = m.grad_read()
-        out = [torch.from_numpy(gr[n]).to(ctx.device) for n in prop.param_names]
+        if dev:
+            gr = m.grad_read_device()
+            out = [gr[n] for n in ctx.names]
+        else:
+            gr = m.grad_read()
+            out = [torch.from_numpy(gr[n]).to(ctx.device) for n in ctx.names]
         return (None, None) + tuple(out)
 
 
@@ -96,8 +145,13 @@ def seismic(prop: Propagator, shots, *params):
 
     shots: list of (src_pos (5,ns), src (NT,ns), rec_pos (8,nr)) in
     SeisCL.py layout; params: tensors [NZ, (NY,) NX] in the order of
-    prop.param_names.  Returns a flat tuple of (NT, nrec) tensors, shot by
-    shot, each shot's outputs in prop.out_names order."""
-    if len(params) != len(prop.param_names):
+    prop.param_names (for a viscoelastic model, either all five or only the
+    first three), all on the CPU or all on the model's GPU.  Returns a flat
+    tuple of (NT, nrec) tensors on the parameters' device, shot by shot,
+    each shot's outputs in prop.out_names order."""
+    n = len(params)
+    if n != len(prop.param_names) and not (prop.visco and n == 3):
         raise ValueError(f"expected parameters {prop.param_names}")
+    if len({t.device for t in params}) != 1:
+        raise ValueError("all parameters must be on one device")
     return _Seismic.apply(prop, list(shots), *params)

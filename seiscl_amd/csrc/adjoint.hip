@@ -1064,21 +1064,27 @@ template <typename R>
 void launch_adj_v(const AdjArgs<R> &a, int nd_case, void *stream) {
     SEIS_SWITCH_H(a.f.g.H, (run_adj_v<R, HH>(a, nd_case, (hipStream_t)stream)));
 }
+// FP64 runs (R = double) store their history in double: half is ignored
 template <typename R>
 void launch_adjh_s(const AdjArgs<R> &a, int nd_case, int L, bool half, void *stream) {
-    if (half) {
-        SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, __half, HH>(a, nd_case, L, (hipStream_t)stream)));
-    } else {
-        SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, R, HH>(a, nd_case, L, (hipStream_t)stream)));
+    if constexpr (sizeof(R) == 4) {
+        if (half) {
+            SEIS_SWITCH_H(a.f.g.H,
+                          (run_adjh_s_L<R, __half, HH>(a, nd_case, L, (hipStream_t)stream)));
+            return;
+        }
     }
+    SEIS_SWITCH_H(a.f.g.H, (run_adjh_s_L<R, R, HH>(a, nd_case, L, (hipStream_t)stream)));
 }
 template <typename R>
 void launch_adjh_v(const AdjArgs<R> &a, int nd_case, bool half, void *stream) {
-    if (half) {
-        SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, __half, HH>(a, nd_case, (hipStream_t)stream)));
-    } else {
-        SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, R, HH>(a, nd_case, (hipStream_t)stream)));
+    if constexpr (sizeof(R) == 4) {
+        if (half) {
+            SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, __half, HH>(a, nd_case, (hipStream_t)stream)));
+            return;
+        }
     }
+    SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
 void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream) {
@@ -1107,5 +1113,14 @@ template void launch_adjh_v<float>(const AdjArgs<float> &, int, bool, void *);
 template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *);
 template void launch_residuals<float>(const FDArgs<float> &, int, const ShotDev &, const int *,
                                       int, const float *, int, int, int, void *);
+// FP64 gradient runs
+template void launch_adj_s<double>(const AdjArgs<double> &, int, void *);
+template void launch_surface_adj<double>(const FDArgs<double> &, double *, int, int, void *);
+template void launch_adj_v<double>(const AdjArgs<double> &, int, void *);
+template void launch_adjh_s<double>(const AdjArgs<double> &, int, int, bool, void *);
+template void launch_adjh_v<double>(const AdjArgs<double> &, int, bool, void *);
+template void launch_savebnd<double>(const FDArgs<double> &, const Shell &, int, double *, void *);
+template void launch_residuals<double>(const FDArgs<double> &, int, const ShotDev &, const int *,
+                                       int, const float *, int, int, int, void *);
 
 }  // namespace seis

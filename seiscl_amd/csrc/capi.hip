@@ -62,6 +62,23 @@ int scratch_address(int device, void **out) {
 
 // Seismograms recorded time-major ([NT][nout][nrec], float or double) into
 // the boundary's [nout][nrec][NT] floats, in the device area next to them.
+// Adjoint sources given on the device ([nout][nrec][NT]): res_scale factors
+// applied to t < tmax as (float)((double)r * f), then the velocity scale of
+// scaled-unit runs (qv[o], a float product), as the host path does.
+__global__ void k_res_device(const float *in, float *out, const double *fac, const float *qv,
+                             int nout, int nrec, int NT, int tmax) {
+    const long n = (long)nout * nrec * NT;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+         k += (long)gridDim.x * blockDim.x) {
+        const long tr = k / NT;
+        const int t = (int)(k % NT), o = (int)(tr / nrec);
+        float v = in[k];
+        if (fac && t < tmax) v = (float)((double)v * fac[tr]);
+        if (qv) v = v * qv[o];
+        out[k] = v;
+    }
+}
+
 template <typename T>
 __global__ void k_seis_transpose(const T *in, float *out, int NT, int nout, int nrec) {
     const long n = (long)NT * nout * nrec;
@@ -166,7 +183,13 @@ struct seis_model {
     bool setting_up = false;         // buffers zero-filled on the transfer stream
     hipEvent_t done_ev = nullptr;    // batch: compute-stream point after the last step
     bool in_batch = false;           // held by a serving batch slot
+    // captured forward steps (seis_run_steps, 2D): graph, its key, step counter
+    hipGraph_t sg_graph = nullptr;
+    hipGraphExec_t sg_exec = nullptr;
+    uint64_t sg_key = 0;
+    int *sg_t = nullptr;
     std::vector<int> rcell_host;     // receiver cells of the bound shot
+    std::vector<float> recpos_host;  // receiver positions of the bound shot [nrec][8]
     // gradient state, allocated on first use
     struct Grad {
         bool ready = false;
@@ -174,6 +197,7 @@ struct seis_model {
         int nvb = 0;
         float *bnd = nullptr;       // [tmax][nvb][n]
         long bnd_steps = 0;
+        size_t bnd_elem = 4;        // bytes per saved value (8 in FP64 runs)
         FDArgs<float> r{};          // adjoint wavefields (same model)
         double *gM = nullptr, *gmu = nullptr, *grho = nullptr;   // padded layout
         double *hM = nullptr, *hmu = nullptr;                    // Hessian (Hout)
@@ -181,6 +205,8 @@ struct seis_model {
         float *res = nullptr;       // [nout][nrec][NT] scaled adjoint sources
         float *stmp = nullptr;      // adjoint free surface: 3 * NX * NY
         size_t res_cap = 0;
+        double *res_fac = nullptr;  // [nout][nrec] res_scale factors (device residuals)
+        size_t res_fac_cap = 0;
         std::vector<float *> own_psi;
         bool saved = false;         // a forward run with boundaries is bound
         // viscoelastic runs: full-state checkpoints every K steps and the
@@ -210,9 +236,12 @@ struct seis_model {
     ~seis_model() {
         for (void *p : owned) ctx->release(p);
         void *sp[] = {shot.src_cell, shot.src_type, shot.src_k, shot.src, shot.rec_cell, shot.out,
-                      grad.bnd, grad.res, grad.ckpt, grad.hist};
-        for (void *p : sp) ctx->release(p);
+                      grad.bnd, grad.res, grad.ckpt, grad.hist, grad.res_fac, sg_t};
+        for (void *p : sp)
+            if (p) ctx->release(p);
         for (auto e : evpool) (void)hipEventDestroy(e);
+        if (sg_exec) (void)hipGraphExecDestroy(sg_exec);
+        if (sg_graph) (void)hipGraphDestroy(sg_graph);
     }
 };
 
@@ -502,13 +531,14 @@ FDArgs<double> as_double(const FDArgs<float> &a) {
 }
 
 // One launch of each step kernel in the model's precision.
-void step_sources(seis_model *m, int t, int pdir, hipStream_t st) {
+// tb (may be NULL): device step counter added to t (step graphs)
+void step_sources(seis_model *m, int t, int pdir, hipStream_t st, const int *tb = nullptr) {
     if (m->prec == P_FP16)
-        launch_sources<float, __half>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st);
+        launch_sources<float, __half>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st, tb);
     else if (m->prec == P_FP64)
-        launch_sources<double, double>(m->ad, m->nd_case, m->shot, m->c.NT, t, pdir, st);
+        launch_sources<double, double>(m->ad, m->nd_case, m->shot, m->c.NT, t, pdir, st, tb);
     else
-        launch_sources<float, float>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st);
+        launch_sources<float, float>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st, tb);
 }
 void step_v(seis_model *m, hipStream_t st) {
     if (m->prec == P_FP16) launch_v3d<float, __half>(m->a, st);
@@ -525,16 +555,108 @@ void step_surface(seis_model *m, hipStream_t st) {
     if (m->prec == P_FP64) launch_surface(m->ad, m->nd_case, 1, st);
     else launch_surface(m->a, m->nd_case, 1, st);
 }
-void step_varsout(seis_model *m, int t, hipStream_t st) {
+void step_varsout(seis_model *m, int t, hipStream_t st, const int *tb = nullptr) {
     const int n = (int)m->out_codes.size();
     const float *u = m->unscale.empty() ? nullptr : m->unscale.data();
     if (m->prec == P_FP16)
-        launch_varsout<float, __half>(m->a, m->nd_case, m->shot, m->out_codes.data(), u, n, t, st);
+        launch_varsout<float, __half>(m->a, m->nd_case, m->shot, m->out_codes.data(), u, n, t, st,
+                                      tb);
     else if (m->prec == P_FP64)
         launch_varsout<double, double>(m->ad, m->nd_case, m->shot, m->out_codes.data(), u, n, t,
-                                       st);
+                                       st, tb);
     else
-        launch_varsout<float, float>(m->a, m->nd_case, m->shot, m->out_codes.data(), u, n, t, st);
+        launch_varsout<float, float>(m->a, m->nd_case, m->shot, m->out_codes.data(), u, n, t, st,
+                                     tb);
+}
+
+// varsout of step t_rec and the sources of step t_src in one launch
+void step_io(seis_model *m, int t_rec, int t_src, hipStream_t st, const int *tb = nullptr) {
+    const int n = (int)m->out_codes.size(), NT = m->c.NT;
+    const float *u = m->unscale.empty() ? nullptr : m->unscale.data();
+    const int *oc = m->out_codes.data();
+    if (m->prec == P_FP16)
+        launch_io<float, __half>(m->a, m->nd_case, m->shot, oc, u, n, NT, t_rec, t_src, st, tb);
+    else if (m->prec == P_FP64)
+        launch_io<double, double>(m->ad, m->nd_case, m->shot, oc, u, n, NT, t_rec, t_src, st, tb);
+    else
+        launch_io<float, float>(m->a, m->nd_case, m->shot, oc, u, n, NT, t_rec, t_src, st, tb);
+}
+
+// One forward step t (src/time_stepping.c:785-816) led by varsout of step
+// t-1, which shares the sources' launch: [varsout t-1, sources t], update_v,
+// update_s, the free surface unless fused.  A run of steps ends with
+// step_io(m, last, -1) for its last varsout.  Recording step t-1 again at
+// the start of a run rewrites the same values.
+void step_all(seis_model *m, int t, hipStream_t st, const int *tb = nullptr) {
+    step_io(m, t - 1, t, st, tb);
+    step_v(m, st);
+    step_s(m, st);
+    if (m->c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+}
+
+// What a captured step sequence depends on: the kernel arguments (model
+// arguments, shot tables, outputs and their scales) and the precision.
+uint64_t step_key(const seis_model *m) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(&m->a, sizeof(m->a));
+    mix(&m->ad, sizeof(m->ad));
+    mix(&m->shot, sizeof(m->shot));
+    mix(&m->prec, sizeof(m->prec));
+    mix(m->out_codes.data(), m->out_codes.size() * sizeof(int));
+    mix(m->unscale.data(), m->unscale.size() * sizeof(float));
+    return h;
+}
+
+// Steps per captured graph of forward steps.  2D grids run each kernel in a
+// few microseconds, so host launches, not the GPU, set the pace of a step;
+// replaying a graph of kGraphSteps steps submits them at once.  The step
+// index comes from a device counter that the graph's last node advances.
+constexpr int kGraphSteps = 64;
+
+bool graph_enabled() {
+    static const bool on = [] {
+        const char *v = getenv("SEIS_STEP_GRAPH");
+        return !(v && *v == '0');
+    }();
+    return on;
+}
+
+void graph_reset(seis_model *m) {
+    if (m->sg_exec) (void)hipGraphExecDestroy(m->sg_exec);
+    if (m->sg_graph) (void)hipGraphDestroy(m->sg_graph);
+    m->sg_exec = nullptr;
+    m->sg_graph = nullptr;
+    m->sg_key = 0;
+}
+
+// Capture (or reuse) the graph of kGraphSteps steps for the model's current
+// arguments on stream st.
+int graph_prepare(seis_model *m, hipStream_t st) {
+    const uint64_t key = step_key(m);
+    if (m->sg_exec && m->sg_key == key) return 0;
+    graph_reset(m);
+    if (!m->sg_t) {
+        void *p = nullptr;
+        if (m->ctx->alloc(256, &p)) return 1;
+        m->sg_t = (int *)p;
+    }
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    for (int k = 0; k < kGraphSteps; ++k) step_all(m, k, st, m->sg_t);
+    launch_tcounter(m->sg_t, kGraphSteps, 1, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(std::string("step graph capture: ") + hipGetErrorString(e));
+    }
+    m->sg_graph = g;
+    HIP_TRY(hipGraphInstantiate(&m->sg_exec, g, nullptr, nullptr, 0));
+    m->sg_key = key;
+    return 0;
 }
 
 // Gradient state: adjoint wavefields, their CPML memory (or the forward
@@ -545,11 +667,12 @@ int grad_setup(seis_model *m) {
     const seis_csts &c = m->c;
     if (c.ND == 22)
         return fail("gradients need the elastic or viscoelastic case (ND 2 or 3)");
-    if (m->prec == P_FP64) return fail("gradients are not available in FP64 runs (FP16=64)");
-    if (m->prec != P_FP32 && c.L == 0)
-        return fail("elastic gradients need FP32 wavefields (FP16=0); viscoelastic ones also "
-                    "run with FP16 storage (FP16=2) or scaled units (FP16=1)");
-    if (m->prec != P_FP32 && c.abs_type == 1 && !(c.gradfix & FIX_ADJ_CPML))
+    const bool scaled = m->prec == P_SCALED || m->prec == P_FP16;
+    if (scaled && c.L == 0)
+        return fail("elastic gradients need FP32 or FP64 wavefields (FP16=0 or 64); "
+                    "viscoelastic ones also run with FP16 storage (FP16=2) or scaled units "
+                    "(FP16=1)");
+    if (scaled && c.abs_type == 1 && !(c.gradfix & FIX_ADJ_CPML))
         return fail("gradients with scaled wavefields (FP16=1 or 2) need the adjoint run's own "
                     "CPML memory (gradfix bit 2)");
     G.visco = c.L > 0;
@@ -562,7 +685,9 @@ int grad_setup(seis_model *m) {
     G.nvb = c.ND == 3 ? 9 : 5;
     FDArgs<float> &r = G.r;
     r = m->a;
-    const long nf = m->g.nfield;
+    // adjoint fields, their memory and CPML memory are FP32 (FP64 in FP64 runs)
+    const long ae = m->prec == P_FP64 ? 2 : 1;
+    const long nf = m->g.nfield * ae;
     float **fl[] = {&r.vx, &r.vz, &r.sxx, &r.szz, &r.sxz, &r.vy, &r.syy, &r.sxy, &r.syz};
     const int nfl = c.ND == 3 ? 9 : 5;
     for (int q = 0; q < nfl; ++q)
@@ -579,7 +704,7 @@ int grad_setup(seis_model *m) {
     // of the forward run's final memory (what the reference's kernels bind)
     if (c.abs_type == 1 && ((c.gradfix & FIX_ADJ_CPML) || G.visco)) {
         const long nab2 = 2L * c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
-        const long sxs = nab2 * NY * NZ, sys = NX * nab2 * NZ, szs = NX * NY * nab2;
+        const long sxs = nab2 * NY * NZ * ae, sys = NX * nab2 * NZ * ae, szs = NX * NY * nab2 * ae;
         float **px[] = {&r.psi_sxx_x, &r.psi_sxy_x, &r.psi_sxz_x, &r.psi_vx_x, &r.psi_vy_x,
                         &r.psi_vz_x};
         float **py[] = {&r.psi_sxy_y, &r.psi_syy_y, &r.psi_syz_y, &r.psi_vx_y, &r.psi_vy_y,
@@ -593,17 +718,18 @@ int grad_setup(seis_model *m) {
         for (float **p : pz)
             if (*p && !(*p = m->dalloc(szs))) return 1;
     }
-    if (c.freesurf && !(G.stmp = m->dalloc(3L * m->g.NX * m->g.NY))) return 1;
+    if (c.freesurf && !(G.stmp = m->dalloc(3L * m->g.NX * m->g.NY * ae))) return 1;
+    const long nfd = 2L * m->g.nfield;   // one double per cell
     double **gd[] = {&G.gM, &G.gmu, &G.grho};
     for (double **p : gd)
-        if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
+        if (!(*p = (double *)m->dalloc(nfd))) return 1;
     if (c.Hout)
         for (double **p : {&G.hM, &G.hmu})
-            if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
+            if (!(*p = (double *)m->dalloc(nfd))) return 1;
     // tau gradients of viscoelastic runs (param_type 3 reads no tau)
     if (G.visco && c.param_type <= 2)
         for (double **p : {&G.gtp, &G.gts})
-            if (!(*p = (double *)m->dalloc(2 * nf))) return 1;
+            if (!(*p = (double *)m->dalloc(nfd))) return 1;
     G.ready = true;
     return 0;
 }
@@ -611,7 +737,8 @@ int grad_setup(seis_model *m) {
 // zero the adjoint wavefields and the adjoint run's own CPML memory
 void grad_zero_adjoint(seis_model *m) {
     const FDArgs<float> &r = m->grad.r, &a = m->a;
-    const size_t nf = (size_t)m->g.nfield * 4;
+    const size_t ae = m->prec == P_FP64 ? 8 : 4;   // bytes of an adjoint value
+    const size_t nf = (size_t)m->g.nfield * ae;
     float *fl[] = {r.vx, r.vy, r.vz, r.sxx, r.syy, r.szz, r.sxy, r.syz, r.sxz};
     for (float *p : fl) launch_fill(p, nf, m->ctx->stream);
     float *mv[] = {r.rxx, r.ryy, r.rzz, r.rxy, r.ryz, r.rxz};
@@ -621,7 +748,7 @@ void grad_zero_adjoint(seis_model *m) {
     const bool fix = m->c.gradfix & FIX_ADJ_CPML;
     if (m->c.abs_type == 1 && (fix || m->grad.visco)) {
         const long nab2 = 2L * m->c.nab, NZ = m->g.NZ, NY = m->g.NY, NX = m->g.NX;
-        const size_t sx = nab2 * NY * NZ * 4, sy = NX * nab2 * NZ * 4, sz = NX * NY * nab2 * 4;
+        const size_t sx = nab2 * NY * NZ * ae, sy = NX * nab2 * NZ * ae, sz = NX * NY * nab2 * ae;
         float *px[] = {r.psi_sxx_x, r.psi_sxy_x, r.psi_sxz_x, r.psi_vx_x, r.psi_vy_x, r.psi_vz_x};
         float *py[] = {r.psi_sxy_y, r.psi_syy_y, r.psi_syz_y, r.psi_vx_y, r.psi_vy_y, r.psi_vz_y};
         float *pz[] = {r.psi_sxz_z, r.psi_syz_z, r.psi_szz_z, r.psi_vx_z, r.psi_vy_z, r.psi_vz_z};
@@ -917,6 +1044,23 @@ int seis_model_set_params(seis_model *m, const float *p0, const float *p1, const
     return 0;
 }
 
+int seis_model_set_params_device(seis_model *m, const float *dp0, const float *dp1,
+                                 const float *drho, const float *dtaup, const float *dtaus) {
+    g_err.clear();
+    const size_t n = (size_t)m->g.NZ * m->g.NY * m->g.NX;
+    const float *src[5] = {dp0, dp1, drho, dtaup, dtaus};
+    std::vector<float> h[5];
+    const float *hp[5] = {};
+    for (int k = 0; k < 5; ++k) {
+        if (!src[k]) continue;
+        h[k].resize(n);
+        HIP_TRY(hipMemcpyAsync(h[k].data(), src[k], n * 4, hipMemcpyDeviceToHost, m->ctx->xfer));
+        hp[k] = h[k].data();
+    }
+    HIP_TRY(hipStreamSynchronize(m->ctx->xfer));
+    return seis_model_set_params(m, hp[0], hp[1], hp[2], hp[3], hp[4]);
+}
+
 size_t seis_model_footprint(const seis_model *m) { return m ? m->bytes : 0; }
 
 int seis_out_count(const seis_model *m) { return (int)m->outn.size(); }
@@ -1055,6 +1199,7 @@ int seis_shot_bind(seis_model *m, int nsrc, const float *src_pos, const float *s
     }
     if (shot_upload(m, h, src, sd)) return 1;
     m->rcell_host = h.rcell;
+    m->recpos_host.assign(rec_pos, rec_pos + (size_t)8 * nrec);
     m->shot_bound = true;
     m->grad.saved = false;
     return 0;
@@ -1148,9 +1293,21 @@ int seis_run_steps(seis_model *m, int t0, int nsteps) {
     if (!e0 || !e1) return fail("hipEventCreate failed");
     const bool prof = m->profiling;
     if (prof && !pool_event(m, 2 + 4 * (size_t)nsteps)) return fail("hipEventCreate failed");
+    // 2D runs without per-kernel events replay captured step graphs
+    const bool graph = !prof && m->c.ND != 3 && nsteps >= kGraphSteps && graph_enabled();
+    if (graph && graph_prepare(m, st)) return 1;
     HIP_TRY(hipEventRecord(e0, st));
-    for (int k = 0; k < nsteps; ++k) {
+    int k0 = 0;
+    if (graph) {
+        launch_tcounter(m->sg_t, t0, 0, st);
+        for (; k0 + kGraphSteps <= nsteps; k0 += kGraphSteps) HIP_TRY(hipGraphLaunch(m->sg_exec, st));
+    }
+    for (int k = k0; k < nsteps; ++k) {
         const int t = t0 + k;
+        if (!prof) {
+            step_all(m, t, st);
+            continue;
+        }
         // step order of src/time_stepping.c:785-816
         step_sources(m, t, 1, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
@@ -1162,6 +1319,8 @@ int seis_run_steps(seis_model *m, int t0, int nsteps) {
         if (m->c.freesurf && !m->a.fuse_surface) step_surface(m, st);
         step_varsout(m, t, st);
     }
+    // the last step's varsout (recorded at the start of each step otherwise)
+    if (!prof && nsteps > 0) step_io(m, t0 + nsteps - 1, -1, st);
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventSynchronize(e1));
@@ -1439,12 +1598,19 @@ static int visco_prepare(seis_model *m) {
     const bool f0 = c.gradfix & FIX_VISCO_F0;
     const std::vector<float> &Ms = f0 ? m->rp.Mf0 : m->rp.M, &mus = f0 ? m->rp.muf0 : m->rp.mu;
     std::vector<float> h;
-    to_padded(m->g, Ms.data(), 1.0f, h);
-    HIP_TRY(hipMemcpyAsync(G.Ms, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    to_padded(m->g, mus.data(), 1.0f, h);
-    HIP_TRY(hipMemcpyAsync(G.mus, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<double> hd;
+    for (int q = 0; q < 2; ++q) {
+        to_padded(m->g, (q ? mus : Ms).data(), 1.0f, h);
+        const void *src = h.data();
+        size_t bytes = h.size() * 4;
+        if (m->prec == P_FP64) {   // FP64 runs read the moduli in double
+            hd.assign(h.begin(), h.end());
+            src = hd.data();
+            bytes = hd.size() * 8;
+        }
+        HIP_TRY(hipMemcpyAsync(q ? G.mus : G.Ms, src, bytes, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
     const auto state = forward_state(m);
     const size_t sbytes = state_bytes(state);
     const size_t hbytes = (size_t)G.nvb * m->g.nfield * m->selem;
@@ -1531,10 +1697,13 @@ static int movie_snap(seis_model *m, const MovieOut *mv, int t) {
     return 0;
 }
 
-// Viscoelastic adjoint over the plan of forward_save_visco.
+// Viscoelastic adjoint over the plan of forward_save_visco (R: float, or
+// double in FP64 runs).
+extern "C++" {
+template <typename R>
 struct ViscoAdj {
     seis_model *m;
-    AdjArgs<float> *A;
+    AdjArgs<R> *A;
     std::vector<std::pair<float *, size_t>> state;
     std::vector<float *> hf;    // forward fields the adjoint step reads
     size_t sbytes, fb;          // bytes of a checkpoint, of one stored field
@@ -1586,7 +1755,7 @@ struct ViscoAdj {
         }
         for (int t = t1 - 1; t >= std::max(t0, tmin + 1); --t) {
             if (c.freesurf)
-                launch_surface_adj(A->r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
+                launch_surface_adj(A->r, (R *)G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
             launch_residuals(A->r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t,
                              serial, st);
             const char *h = (const char *)G.hist + (size_t)(t - t0) * G.nvb * fb;
@@ -1603,12 +1772,13 @@ struct ViscoAdj {
     }
 };
 
-static int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial, const MovieOut *mv) {
+template <typename R>
+static int adjoint_visco(seis_model *m, AdjArgs<R> &A, int serial, const MovieOut *mv) {
     auto &G = m->grad;
     const CkptPlan &P = G.plan;
     const int tmax = tmax_of(m), tmin = m->c.tmin;
     hipStream_t st = m->ctx->stream;
-    ViscoAdj V;
+    ViscoAdj<R> V;
     V.m = m;
     V.A = &A;
     V.state = forward_state(m);
@@ -1620,8 +1790,8 @@ static int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial, const Mov
     V.half = m->prec == P_FP16;
     V.prof = m->profiling;
     V.mv = mv;
-    A.Ms = G.Ms;
-    A.mus = G.mus;
+    A.Ms = (const R *)G.Ms;
+    A.mus = (const R *)G.mus;
     // with profiling on, events around the two adjoint kernels of every step
     const size_t nst = (size_t)std::max(0, tmax - 1 - tmin);
     hipEvent_t e0 = pool_event(m, 0), e1 = pool_event(m, 1);
@@ -1651,6 +1821,7 @@ static int adjoint_visco(seis_model *m, AdjArgs<float> &A, int serial, const Mov
         }
     return 0;
 }
+}  // extern "C++"
 
 }  // namespace
 
@@ -1664,17 +1835,19 @@ static int bnd_prepare(seis_model *m) {
     const long need = (long)tmax_of(m);
     if (G.bnd_steps >= need) return 0;
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    const size_t be = m->prec == P_FP64 ? 8 : 4;   // bytes of a saved value
     if (G.bnd) {
-        m->bytes -= (size_t)G.bnd_steps * G.nvb * G.sh.n * 4;
+        m->bytes -= (size_t)G.bnd_steps * G.nvb * G.sh.n * G.bnd_elem;
         m->ctx->release(G.bnd);
         G.bnd = nullptr;
         G.bnd_steps = 0;
     }
     void *p = nullptr;
-    const size_t bytes = (size_t)need * G.nvb * G.sh.n * 4;
+    const size_t bytes = (size_t)need * G.nvb * G.sh.n * be;
     if (m->ctx->alloc(bytes, &p)) return 1;
     G.bnd = (float *)p;
     G.bnd_steps = need;
+    G.bnd_elem = be;
     m->bytes += bytes;
     return 0;
 }
@@ -1702,7 +1875,8 @@ static int saved_layout(seis_model *m, SavedHeader &h) {
         h.payload_bytes = (uint64_t)h.n0 * h.state_bytes;
     } else {
         h.kind = 0;
-        h.payload_bytes = h.state_bytes + (uint64_t)h.tmax * G.nvb * G.sh.n * 4;
+        h.payload_bytes = h.state_bytes + (uint64_t)h.tmax * G.nvb * G.sh.n *
+                                              (m->prec == P_FP64 ? 8 : 4);
     }
     return 0;
 }
@@ -1732,15 +1906,20 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
     // surface runs as its own kernel here
     const int fused = m->a.fuse_surface;
     m->a.fuse_surface = 0;
+    m->ad.fuse_surface = 0;   // the FP64 view of the same arguments
     for (int t = 0; t < tmax; ++t) {
         step_sources(m, t, 1, st);
         step_v(m, st);
         step_s(m, st);
-        launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (size_t)t * per, st);
-        if (m->c.freesurf) launch_surface(m->a, m->nd_case, 1, st);
+        if (m->prec == P_FP64)
+            launch_savebnd(m->ad, G.sh, m->nd_case, (double *)G.bnd + (size_t)t * per, st);
+        else
+            launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (size_t)t * per, st);
+        if (m->c.freesurf) step_surface(m, st);
         step_varsout(m, t, st);
     }
     m->a.fuse_surface = fused;
+    m->ad.fuse_surface = fused;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     G.saved = true;
@@ -1838,42 +2017,24 @@ int seis_res_scale(const seis_model *m, int nrec, const float *rec_pos, float *r
 }
 
 namespace {
-static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv) {
+// The adjoint steps of adjoint_run once the adjoint sources are on the
+// device: R = float, or double in FP64 runs (FP16=64).
+extern "C++" {
+template <typename R>
+static int adjoint_loops(seis_model *m, const MovieOut *mv) {
     auto &G = m->grad;
-    if (!G.saved) return fail("seis_adjoint: no forward run with saved boundaries is bound");
-    const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
+    const int NT = m->c.NT, nout = (int)m->outn.size();
     const int tmax = tmax_of(m), tmin = m->c.tmin;
-    if (tmin < 0 || tmin >= tmax) return fail("tmin outside [0, tmax)");
     hipStream_t st = m->ctx->stream;
-    const size_t nres = (size_t)nout * nrec * NT;
-    if (nres > G.res_cap) {
-        HIP_TRY(hipStreamSynchronize(st));
-        m->ctx->release(G.res);
-        G.res = nullptr;
-        void *p = nullptr;
-        if (m->ctx->alloc(nres * 4, &p)) return 1;
-        G.res = (float *)p;
-        G.res_cap = nres;
-    }
-    // scaled wavefields (FP16=1, 2): the adjoint run keeps its stresses in
-    // physical units and its velocities times q, the units the scaled
-    // coefficients imply, so velocity residuals are injected times q
     const bool scaled = m->prec == P_SCALED || m->prec == P_FP16;
-    std::vector<float> rs;
-    if (scaled && nres) {
-        rs.assign(res, res + nres);
-        for (int o = 0; o < nout; ++o)
-            if (m->out_codes[o] <= O_VZ)
-                for (size_t k = (size_t)o * nrec * NT; k < (size_t)(o + 1) * nrec * NT; ++k)
-                    rs[k] *= m->q;
-        res = rs.data();
+    AdjArgs<R> A;
+    if constexpr (sizeof(R) == 8) {
+        A.f = m->ad;
+        A.r = as_double(adjoint_args(m));
+    } else {
+        A.f = m->a;
+        A.r = adjoint_args(m);
     }
-    if (nres) HIP_TRY(hipMemcpyAsync(G.res, res, nres * 4, hipMemcpyHostToDevice, st));
-    if (!rs.empty()) HIP_TRY(hipStreamSynchronize(st));
-    grad_zero_adjoint(m);
-    AdjArgs<float> A;
-    A.f = m->a;
-    A.r = adjoint_args(m);
     A.gM = G.gM;
     A.gmu = G.gmu;
     A.grho = G.grho;
@@ -1891,7 +2052,7 @@ static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv) {
     if (G.gtp && !A.rtm) {
         A.gtp = G.gtp;
         A.gts = G.gts;
-        A.s0 = relax_sum(m->c);
+        A.s0 = (R)relax_sum(m->c);
     }
     A.hfix = (m->c.gradfix & FIX_HESS) != 0;
     if (m->c.freesurf && (!A.r.M || !A.r.mu || !G.stmp))
@@ -1913,10 +2074,10 @@ static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv) {
     // adjoint loop of src/time_stepping.c:869-940
     for (int t = tmax - 1; t > tmin; --t, ++k) {
         if (m->c.freesurf)
-            launch_surface_adj(A.r, G.stmp, m->nd_case, m->c.gradfix & FIX_SURF_SYY, st);
+            launch_surface_adj(A.r, (R *)G.stmp, m->nd_case, m->c.gradfix & FIX_SURF_SYY, st);
         launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t, serial,
                          st);
-        A.bnd = G.bnd + (size_t)(t - 1) * per;
+        A.bnd = (const R *)G.bnd + (size_t)(t - 1) * per;
         if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
         launch_adj_s(A, m->nd_case, st);
         if (prof) HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
@@ -1945,11 +2106,110 @@ static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv) {
     return 0;
 }
 
+}  // extern "C++"
+
+// dev: res is a device array; dev_scale: apply res_scale to it on the device
+// (the host path receives residuals already scaled by seis_res_scale).
+static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv, bool dev = false,
+                       bool dev_scale = false) {
+    auto &G = m->grad;
+    if (!G.saved) return fail("seis_adjoint: no forward run with saved boundaries is bound");
+    const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
+    const int tmax = tmax_of(m), tmin = m->c.tmin;
+    if (tmin < 0 || tmin >= tmax) return fail("tmin outside [0, tmax)");
+    hipStream_t st = m->ctx->stream;
+    const size_t nres = (size_t)nout * nrec * NT;
+    if (nres > G.res_cap) {
+        HIP_TRY(hipStreamSynchronize(st));
+        m->ctx->release(G.res);
+        G.res = nullptr;
+        void *p = nullptr;
+        if (m->ctx->alloc(nres * 4, &p)) return 1;
+        G.res = (float *)p;
+        G.res_cap = nres;
+    }
+    // scaled wavefields (FP16=1, 2): the adjoint run keeps its stresses in
+    // physical units and its velocities times q, the units the scaled
+    // coefficients imply, so velocity residuals are injected times q
+    const bool scaled = m->prec == P_SCALED || m->prec == P_FP16;
+    std::vector<float> rs;
+    if (dev && nres) {
+        const size_t nfac = (size_t)nout * nrec;
+        const size_t bytes = nfac * 8 + (size_t)nout * 4;
+        if (bytes > G.res_fac_cap) {
+            HIP_TRY(hipStreamSynchronize(st));
+            m->ctx->release(G.res_fac);
+            G.res_fac = nullptr;
+            void *p = nullptr;
+            if (m->ctx->alloc(bytes, &p)) return 1;
+            G.res_fac = (double *)p;
+            G.res_fac_cap = bytes;
+        }
+        std::vector<char> host(bytes);
+        double *fac = (double *)host.data();
+        float *qv = (float *)(host.data() + nfac * 8);
+        if (dev_scale) {
+            if (m->recpos_host.size() != (size_t)8 * nrec)
+                return fail("seis_adjoint_device: no receiver positions for the bound shot");
+            res_scale_factors(m->c, m->rp, m->outn, nrec, m->recpos_host.data(), fac,
+                              m->c.gradfix & FIX_RES_SCALE);
+        }
+        for (int o = 0; o < nout; ++o) qv[o] = scaled && m->out_codes[o] <= O_VZ ? m->q : 1.0f;
+        HIP_TRY(hipMemcpyAsync(G.res_fac, host.data(), bytes, hipMemcpyHostToDevice, st));
+        const int blocks = (int)std::min<size_t>((nres + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_res_device, dim3(blocks), dim3(256), 0, st, res, G.res,
+                           dev_scale ? (const double *)G.res_fac : nullptr,
+                           scaled ? (const float *)(G.res_fac + nfac) : nullptr, nout, nrec, NT,
+                           tmax);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(st));   // the host staging of the factors is released
+    } else if (scaled && nres) {
+        rs.assign(res, res + nres);
+        for (int o = 0; o < nout; ++o)
+            if (m->out_codes[o] <= O_VZ)
+                for (size_t k = (size_t)o * nrec * NT; k < (size_t)(o + 1) * nrec * NT; ++k)
+                    rs[k] *= m->q;
+        res = rs.data();
+    }
+    if (nres && !dev) HIP_TRY(hipMemcpyAsync(G.res, res, nres * 4, hipMemcpyHostToDevice, st));
+    if (!rs.empty()) HIP_TRY(hipStreamSynchronize(st));
+    grad_zero_adjoint(m);
+    return m->prec == P_FP64 ? adjoint_loops<double>(m, mv) : adjoint_loops<float>(m, mv);
+}
+
 }  // namespace
 
 int seis_adjoint(seis_model *m, const float *res) {
     g_err.clear();
     return adjoint_run(m, res, nullptr);
+}
+
+int seis_adjoint_device(seis_model *m, const float *dres, int res_scale_on) {
+    g_err.clear();
+    if (!dres && m->shot.nrec > 0 && !m->outn.empty())
+        return fail("seis_adjoint_device: no residuals");
+    return adjoint_run(m, dres, nullptr, true, res_scale_on != 0);
+}
+
+int seis_shot_read_device(seis_model *m, float *dout) {
+    g_err.clear();
+    if (!m->shot.out) return fail("seis_shot_read_device: no shot bound");
+    const int NT = m->c.NT, nrec = m->shot.nrec, nout = (int)m->outn.size();
+    const long n = (long)NT * nrec * nout;
+    if (n == 0) return 0;
+    if (!dout) return fail("seis_shot_read_device: no output array");
+    hipStream_t st = m->ctx->stream;
+    if (m->done_ev) HIP_TRY(hipStreamWaitEvent(st, m->done_ev, 0));
+    const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+    if (m->prec == P_FP64)
+        hipLaunchKernelGGL(k_seis_transpose<double>, dim3(blocks), dim3(256), 0, st,
+                           (const double *)m->shot.out, dout, NT, nout, nrec);
+    else
+        hipLaunchKernelGGL(k_seis_transpose<float>, dim3(blocks), dim3(256), 0, st,
+                           (const float *)m->shot.out, dout, NT, nout, nrec);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
 }
 
 int seis_adjoint_movie(seis_model *m, const float *res, int movout, int nvar,
@@ -2000,6 +2260,25 @@ int seis_grad_read(seis_model *m, float *grad) {
     const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
     transf_grad(m->c, m->rp, f0 ? m->rp.Mf0 : m->rp.M, f0 ? m->rp.muf0 : m->rp.mu,
                 acc[0].data(), acc[1].data(), acc[2].data(), grad, m->c.gradfix & FIX_RHO_SCALE);
+    return 0;
+}
+
+int seis_grad_read_device(seis_model *m, float *dgrad, float *dgtau) {
+    g_err.clear();
+    const size_t n = (size_t)m->g.NZ * m->g.NY * m->g.NX;
+    hipStream_t st = m->ctx->stream;
+    if (dgrad) {
+        std::vector<float> h(3 * n);
+        if (seis_grad_read(m, h.data())) return 1;
+        HIP_TRY(hipMemcpyAsync(dgrad, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (dgtau) {
+        std::vector<float> h(2 * n);
+        if (seis_grad_read_tau(m, h.data())) return 1;
+        HIP_TRY(hipMemcpyAsync(dgtau, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
     return 0;
 }
 

@@ -436,10 +436,13 @@ __device__ void sources_body(const FDArgs<R> &a, int nd_case, const ShotDev &s, 
     }
 }
 
+// tbase (may be NULL): a device step counter added to nt, so that a captured
+// sequence of steps can be replayed at any starting step
 template <typename R, typename S>
-__global__ void k_sources(FDArgs<R> a, int nd_case, ShotDev s, int NT, int nt, int pdir) {
+__global__ void k_sources(FDArgs<R> a, int nd_case, ShotDev s, int NT, int nt, int pdir,
+                          const int *tbase) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    sources_body<R, S>(a, nd_case, s, NT, nt, pdir);
+    sources_body<R, S>(a, nd_case, s, NT, nt + (tbase ? *tbase : 0), pdir);
 }
 
 struct OutList {
@@ -482,10 +485,32 @@ __device__ void varsout_body(const FDArgs<R> &a, int nd_case, const ShotDev &s, 
 }
 
 template <typename R, typename S>
-__global__ void k_varsout(FDArgs<R> a, int nd_case, ShotDev s, OutList o, int nt) {
+__global__ void k_varsout(FDArgs<R> a, int nd_case, ShotDev s, OutList o, int nt,
+                          const int *tbase) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= s.nrec) return;
-    varsout_body<R, S>(a, nd_case, s, o, nt, r);
+    varsout_body<R, S>(a, nd_case, s, o, nt + (tbase ? *tbase : 0), r);
+}
+
+// Receivers of step t_rec, then the sources of step t_src, in one launch:
+// nothing runs between varsout of step t-1 and the sources of step t
+// (src/time_stepping.c:785-816), so the two share a launch.  One block, so
+// that every receiver has been read before a source writes (a receiver may
+// sit on a source cell).  Steps outside [0, NT) are skipped; tbase as above.
+template <typename R, typename S>
+__global__ __launch_bounds__(1024) void k_io(FDArgs<R> a, int nd_case, ShotDev s, OutList o,
+                                              int NT, int t_rec, int t_src, const int *tbase) {
+    const int tb = tbase ? *tbase : 0;
+    const int tr = t_rec + tb, ts = t_src + tb;
+    if (tr >= 0 && tr < NT)
+        for (int r = threadIdx.x; r < s.nrec; r += blockDim.x) varsout_body<R, S>(a, nd_case, s, o, tr, r);
+    __syncthreads();
+    if (threadIdx.x == 0 && ts >= 0 && ts < NT) sources_body<R, S>(a, nd_case, s, NT, ts, 1);
+}
+
+// the step counter of replayed step sequences: set (add = 0) or advance
+__global__ void k_tcounter(int *t, int v, int add) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t = add ? *t + v : v;
 }
 
 // ---------------------------------------------------------------------------
@@ -649,14 +674,14 @@ void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream) {
 }
 template <typename R, typename S>
 void launch_sources(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT, int nt,
-                    int pdir, void *stream) {
+                    int pdir, void *stream, const int *tbase) {
     if (s.nsrc <= 0) return;
     hipLaunchKernelGGL((k_sources<R, S>), dim3(1), dim3(64), 0, (hipStream_t)stream, a, nd_case,
-                       s, NT, nt, pdir);
+                       s, NT, nt, pdir, tbase);
 }
 template <typename R, typename S>
 void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int *codes,
-                    const float *unscale, int nout, int nt, void *stream) {
+                    const float *unscale, int nout, int nt, void *stream, const int *tbase) {
     if (s.nrec <= 0) return;
     OutList o;
     o.n = nout;
@@ -665,7 +690,26 @@ void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int
         o.unscale[q] = unscale ? unscale[q] : 1.0f;
     }
     hipLaunchKernelGGL((k_varsout<R, S>), dim3((s.nrec + 255) / 256), dim3(256), 0,
-                       (hipStream_t)stream, a, nd_case, s, o, nt);
+                       (hipStream_t)stream, a, nd_case, s, o, nt, tbase);
+}
+
+template <typename R, typename S>
+void launch_io(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int *codes,
+               const float *unscale, int nout, int NT, int t_rec, int t_src, void *stream,
+               const int *tbase) {
+    if (s.nrec <= 0 && s.nsrc <= 0) return;
+    OutList o;
+    o.n = nout;
+    for (int q = 0; q < nout && q < 10; ++q) {
+        o.code[q] = codes[q];
+        o.unscale[q] = unscale ? unscale[q] : 1.0f;
+    }
+    hipLaunchKernelGGL((k_io<R, S>), dim3(1), dim3(1024), 0, (hipStream_t)stream, a, nd_case, s,
+                       o, NT, t_rec, t_src, tbase);
+}
+
+void launch_tcounter(int *t, int v, int add, void *stream) {
+    hipLaunchKernelGGL(k_tcounter, dim3(1), dim3(64), 0, (hipStream_t)stream, t, v, add);
 }
 
 void launch_batch_steps(const BatchSlot *dslots, int nslots, const BatchTable &tab, int maxrec,
@@ -701,16 +745,26 @@ template void launch_update_v<double>(const FDArgs<double> &, int, void *);
 template void launch_update_s<double>(const FDArgs<double> &, int, void *);
 template void launch_surface<double>(const FDArgs<double> &, int, int, void *);
 template void launch_sources<float, float>(const FDArgs<float> &, int, const ShotDev &, int, int,
-                                          int, void *);
+                                          int, void *, const int *);
 template void launch_sources<float, __half>(const FDArgs<float> &, int, const ShotDev &, int, int,
-                                           int, void *);
+                                           int, void *, const int *);
 template void launch_sources<double, double>(const FDArgs<double> &, int, const ShotDev &, int,
-                                            int, int, void *);
+                                            int, int, void *, const int *);
+template void launch_io<float, float>(const FDArgs<float> &, int, const ShotDev &, const int *,
+                                     const float *, int, int, int, int, void *, const int *);
+template void launch_io<float, __half>(const FDArgs<float> &, int, const ShotDev &, const int *,
+                                      const float *, int, int, int, int, void *, const int *);
+template void launch_io<double, double>(const FDArgs<double> &, int, const ShotDev &,
+                                       const int *, const float *, int, int, int, int, void *,
+                                       const int *);
 template void launch_varsout<float, float>(const FDArgs<float> &, int, const ShotDev &,
-                                          const int *, const float *, int, int, void *);
+                                          const int *, const float *, int, int, void *,
+                                          const int *);
 template void launch_varsout<float, __half>(const FDArgs<float> &, int, const ShotDev &,
-                                           const int *, const float *, int, int, void *);
+                                           const int *, const float *, int, int, void *,
+                                           const int *);
 template void launch_varsout<double, double>(const FDArgs<double> &, int, const ShotDev &,
-                                            const int *, const float *, int, int, void *);
+                                            const int *, const float *, int, int, void *,
+                                            const int *);
 
 }  // namespace seis

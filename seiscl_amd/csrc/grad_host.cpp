@@ -211,9 +211,13 @@ double residuals(const seis_csts &c, int nout, int nrec, const float *out, const
 // buoyancy read at (z=x, y=x, x=x) of the receiver's x index as the reference
 // does, or at the receiver's own cell with at_receiver; pressure by
 // -2 (M - mu) dh/dt * dt.
-void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
-               int nrec, const float *rec_pos, float *res, int at_receiver) {
-    const int NT = c.NT, tmax = c.tmax > 0 ? c.tmax : c.NT;
+// The factor of every (output, receiver) trace; 1 where nothing is scaled.
+// Applying it as (float)((double)r * f) equals both of the reference's
+// updates: r *= 1.0 / parscal * dt (double factor) for velocities and
+// r *= parscal * dt (a float factor, whose product with r is exact in
+// double) for pressure.
+void res_scale_factors(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
+                       int nrec, const float *rec_pos, double *fac, int at_receiver) {
     const int NZ = c.N[0], NY = c.ND == 3 ? c.N[1] : 1, NX = c.ND == 3 ? c.N[2] : c.N[1];
     const long np = (long)NZ * NY * NX;
     for (size_t o = 0; o < outn.size(); ++o) {
@@ -224,8 +228,10 @@ void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::st
         else if (name == "vy") par = &p.rjp, isv = true;
         else if (name == "vz") par = &p.rkp, isv = true;
         else if (name == "p" && c.ND != 22) par = &p.M;
-        else continue;
         for (int g = 0; g < nrec; ++g) {
+            double &f = fac[o * nrec + g];
+            f = 1.0;
+            if (!par) continue;
             int x, y, z;
             if (isv && !at_receiver) {
                 x = y = z = (int)(rec_pos[0 + 8 * g] / c.dh);
@@ -236,15 +242,26 @@ void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::st
             }
             const long pos = c.ND == 3 ? (long)x * NZ * NY + (long)y * NZ + z : (long)x * NZ + z;
             if (pos < 0 || pos >= np) continue;
-            float *r = &res[((size_t)o * nrec + g) * NT];
             if (isv) {
                 const float parscal = 1.0 / (*par)[pos] * c.dh / c.dt * powf(2, 0);
-                for (int t = 0; t < tmax; ++t) r[t] *= 1.0 / parscal * c.dt;
+                f = 1.0 / parscal * c.dt;
             } else {
                 const float parscal = -2.0 * ((*par)[pos] - p.mu[pos]) * c.dh / c.dt * powf(2, -0);
-                for (int t = 0; t < tmax; ++t) r[t] *= parscal * c.dt;
+                f = (double)(parscal * c.dt);
             }
         }
+    }
+}
+
+void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
+               int nrec, const float *rec_pos, float *res, int at_receiver) {
+    const int NT = c.NT, tmax = c.tmax > 0 ? c.tmax : c.NT;
+    std::vector<double> fac(outn.size() * (size_t)nrec);
+    res_scale_factors(c, p, outn, nrec, rec_pos, fac.data(), at_receiver);
+    for (size_t k = 0; k < fac.size(); ++k) {
+        if (fac[k] == 1.0) continue;
+        float *r = &res[k * NT];
+        for (int t = 0; t < tmax; ++t) r[t] = (float)((double)r[t] * fac[k]);
     }
 }
 

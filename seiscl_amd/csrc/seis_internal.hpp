@@ -141,13 +141,22 @@ template <typename R>
 void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream);
 // S: storage type of wavefields and memory variables (float, __half or
 // double); R: compute and coefficient type.
+// tbase (may be NULL): device step counter added to nt (replayed step graphs)
 template <typename R, typename S>
 void launch_sources(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT,
-                    int nt, int pdir, void *stream);
+                    int nt, int pdir, void *stream, const int *tbase = nullptr);
 template <typename R, typename S>
 void launch_varsout(const FDArgs<R> &a, int nd_case, const ShotDev &s,
                     const int *out_codes, const float *unscale, int nout, int nt,
-                    void *stream);
+                    void *stream, const int *tbase = nullptr);
+// varsout of step t_rec then the sources of step t_src in one launch (steps
+// outside [0, NT) skipped), each plus *tbase when tbase is given
+template <typename R, typename S>
+void launch_io(const FDArgs<R> &a, int nd_case, const ShotDev &s, const int *out_codes,
+               const float *unscale, int nout, int NT, int t_rec, int t_src, void *stream,
+               const int *tbase = nullptr);
+// *t = v (add 0) or *t += v (add 1), one thread on the stream
+void launch_tcounter(int *t, int v, int add, void *stream);
 void launch_fill(void *ptr, size_t bytes, void *stream);
 template <typename R, typename S>
 void launch_v3d(const FDArgs<R> &a, void *stream);
@@ -371,6 +380,8 @@ void butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax, 
 int fft_fast_size(int m);
 void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
                int nrec, const float *rec_pos, float *res, int at_receiver);
+void res_scale_factors(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
+                       int nrec, const float *rec_pos, double *fac, int at_receiver);
 void transf_hess(const seis_csts &c, const RefParams &p, const std::vector<float> &Ms,
                  const std::vector<float> &mus, const double *hM, const double *hmu,
                  float *out);

@@ -358,6 +358,88 @@ class Model:
             out["taus"] = from_c_layout(t[n:], self.shape)
         return out
 
+    # ---- device-resident variants (torch tensors on the model's GPU) -----
+
+    def _c_layout_device(self, t):
+        """A torch tensor [NZ, (NY,) NX] -> float32, C layout (z fastest),
+        contiguous, on its device."""
+        import torch
+        t = t.detach().to(torch.float32)
+        return t.permute(*reversed(range(t.dim()))).contiguous()
+
+    def set_params_device(self, params: dict):
+        """set_params from torch tensors on the GPU (names as set_params)."""
+        import torch
+        names = PARAM_NAMES.get(self.csts.param_type, ("vpR", "vsR", "rho"))
+        if self.csts.param_type == 3:
+            names = ("vpR", "vsR", "rho")
+        keep = []
+        for k, n in enumerate(names):
+            if self.csts.ND == 22 and k == 1:
+                keep.append(None)
+                continue
+            keep.append(self._c_layout_device(params[n]))
+        if self.csts.L > 0:
+            tp = "vpI" if self.csts.param_type == 3 else "taup"
+            ts = "vsI" if self.csts.param_type == 3 else "taus"
+            keep.append(self._c_layout_device(params[tp]))
+            keep.append(self._c_layout_device(params[ts]) if self.csts.ND != 22 else None)
+        else:
+            keep += [None, None]
+        torch.cuda.current_stream().synchronize()
+        ptr = [ctypes.c_void_p(t.data_ptr()) if t is not None else None for t in keep]
+        check(self.lib.seis_model_set_params_device(self.handle, *ptr))
+
+    def read_shot_device(self):
+        """Seismograms of the bound shot as a torch tensor [nout, nrec, NT]
+        on the GPU (no host round trip)."""
+        import torch
+        nrec = self._shot[2].shape[0]
+        out = torch.empty((len(self.out_names), nrec, self.csts.NT), dtype=torch.float32,
+                          device="cuda")
+        torch.cuda.current_stream().synchronize()
+        check(self.lib.seis_shot_read_device(self.handle, ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def forward_save_device(self, src_pos, src, rec_pos):
+        """forward_save whose seismograms stay on the GPU: [nout, nrec, NT]."""
+        self._shot = shot_arrays(src_pos, src, rec_pos)
+        sp, s, rp = self._shot
+        check(self.lib.seis_forward_save(self.handle, sp.shape[0], fptr(sp), fptr(s),
+                                         rp.shape[0], fptr(rp), None))
+        return self.read_shot_device()
+
+    def adjoint_device(self, res, scale: bool = True):
+        """adjoint() with the adjoint sources a torch tensor [nout, nrec, NT]
+        on the GPU; res_scale runs on the device when scale."""
+        import torch
+        r = res.detach().to(torch.float32).contiguous()
+        torch.cuda.current_stream().synchronize()
+        check(self.lib.seis_adjoint_device(self.handle, ctypes.c_void_p(r.data_ptr()),
+                                           int(bool(scale))))
+
+    def grad_read_device(self) -> dict:
+        """grad_read() as torch tensors on the GPU, in the model layout."""
+        import torch
+        n = int(np.prod(self.shape))
+        g = torch.empty(3 * n, dtype=torch.float32, device="cuda")
+        visco = self.csts.L > 0 and self.csts.param_type <= 2
+        t = torch.empty(2 * n, dtype=torch.float32, device="cuda") if visco else None
+        torch.cuda.current_stream().synchronize()
+        check(self.lib.seis_grad_read_device(
+            self.handle, ctypes.c_void_p(g.data_ptr()),
+            ctypes.c_void_p(t.data_ptr()) if visco else None))
+        rev = tuple(self.shape)[::-1]
+
+        def model_layout(flat):
+            return flat.reshape(rev).permute(*reversed(range(len(rev)))).contiguous()
+        names = PARAM_NAMES.get(self.csts.param_type, ("vp", "vs", "rho"))
+        out = {nm: model_layout(g[q * n:(q + 1) * n]) for q, nm in enumerate(names)}
+        if visco:
+            out["taup"] = model_layout(t[:n])
+            out["taus"] = model_layout(t[n:])
+        return out
+
     def hess_read(self) -> dict:
         """Approximate Hessian {p0, p1, rho} accumulated with the gradient
         (Hout=1), transformed like the gradient."""

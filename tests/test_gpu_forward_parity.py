@@ -93,3 +93,43 @@ def test_config1_full_size_bitwise(oracle_lib, gpu_ctx):
     for o, r in zip(out, ref):
         assert np.abs(r).max() > 0
         np.testing.assert_array_equal(o, r)
+
+
+@pytest.mark.parametrize("case", [dict(ND=2, FDORDER=4, freesurf=1, L=1, NT=300),
+                                  dict(ND=22, FDORDER=8, NT=200),
+                                  dict(ND=2, FDORDER=4, seisout=4, NT=130)], ids=_ids)
+def test_step_graph_replays_equal_eager(oracle_lib, gpu_ctx, case):
+    """2D shots replay captured graphs of 64 steps (profiling off); with
+    events around every kernel (profiling on) the same steps are launched
+    one by one.  Several runs of each kind, interleaved on the same model
+    with host work and synchronisations between them, give the same
+    seismograms bit for bit, and those equal the oracle's; a run of steps
+    started mid-shot (run_steps(t0, n)) replays from that step."""
+    from seiscl_amd import Model
+    csts, params, (sp, s, rp) = small_case(**case)
+    ref = oracle_lib.forward(params, sp, s, rp, **csts)
+    m = Model(gpu_ctx, params, **csts)
+    outs = []
+    for prof in (False, True, False, False, True, False):
+        m.set_profiling(prof)
+        outs.append(m.forward(sp, s, rp))
+        gpu_ctx.sync()
+        assert all(np.isfinite(o).all() for o in outs[-1])
+    # the same shot advanced in uneven pieces
+    m.set_profiling(False)
+    NT = int(csts["NT"])
+    m.bind_shot(sp, s, rp)
+    t = 0
+    for n in (5, 70, 64, NT):
+        n = min(n, NT - t)
+        if n <= 0:
+            break
+        m.run_steps(t, n)
+        t += n
+    outs.append(m.read_shot())
+    m.close()
+    for o in outs:
+        for a, b in zip(o, outs[0]):
+            np.testing.assert_array_equal(a, b)
+    for a, r in zip(outs[0], ref):
+        assert rel_l2(a, r) <= TOL
