@@ -2478,13 +2478,20 @@ int seis_batch_step(seis_batch *b, int nsteps) {
         ids.clear();
         tile0.clear();
         unsigned acc = 0;
+        // tiles per block of the batch passes (SEIS_BATCH_TILES, 1..16)
+        static const int bt = [] {
+            const char *v = getenv("SEIS_BATCH_TILES");
+            const int n = v && *v ? atoi(v) : kBatchTiles;
+            return n < 1 ? 1 : n > 16 ? 16 : n;
+        }();
+        b->table.tiles = bt;
         for (auto &kv : groups)
             for (int i : kv.second) {
                 const Layout &g = b->model[i]->g;
                 ids.push_back(i);
                 tile0.push_back(acc);
                 const unsigned tiles = (unsigned)((g.NZ + 63) / 64) * (unsigned)((g.NX + 3) / 4);
-                acc += (tiles + kBatchTiles - 1) / kBatchTiles;
+                acc += (tiles + bt - 1) / bt;
             }
         tile0.push_back(acc);
         b->table.n = (int)ids.size();

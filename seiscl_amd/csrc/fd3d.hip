@@ -223,185 +223,6 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb, int yg) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// x-marching velocity update.  A block owns a column of 64 (z) x MBY (y)
-// cells and walks XC consecutive x-planes.  Along x each lane keeps a
-// register window of its own column (sxx, sxy, sxz: 2H+1 planes, one new
-// load per plane); in y and z the plane's stresses go through LDS with their
-// halos (syy, sxy, syz in y; szz, sxz, syz in z), so a stencil neighbour is
-// read from HBM or L2 once per block instead of once per lane that needs it.
-// The arithmetic is k_v3d's, term by term.
-// ---------------------------------------------------------------------------
-constexpr int MBY = 4;   // rows of y per marching block
-
-template <int H>
-struct VTile {
-    // [field][row][col]: fields syy, sxy, syz (y stencils), szz, sxz (z);
-    // rows y0-H .. y0+MBY+H-1, columns z0-H .. z0+63+H
-    static constexpr int ROWS = MBY + 2 * H, COLS = 64 + 2 * H;
-};
-
-template <int H, typename R>
-__device__ __forceinline__ R DpL(const R *c, int st, const R *hc) {
-    R acc = hc[1] * (c[st] - c[0]);
-#pragma unroll
-    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (c[k * st] - c[-(k - 1) * st]);
-    return acc;
-}
-template <int H, typename R>
-__device__ __forceinline__ R DmL(const R *c, int st, const R *hc) {
-    R acc = hc[1] * (c[0] - c[-st]);
-#pragma unroll
-    for (int k = 2; k <= H; ++k) acc = acc + hc[k] * (c[(k - 1) * st] - c[-k * st]);
-    return acc;
-}
-
-template <typename R, typename S, int H>
-__global__ __launch_bounds__(64 * MBY) void k_v3d_m(FDArgs<R> a, int nbz, int nby, unsigned nb,
-                                                   int XC) {
-    using T = Store<R, S>;
-    using VT = VTile<H>;
-    constexpr int ROWS = VT::ROWS, COLS = VT::COLS;
-    __shared__ R lds[5][ROWS][COLS];
-    const Layout &g = a.g;
-    unsigned tile;
-    xcd_tile(nb, tile);
-    if (tile >= nb) return;                          // uniform per block
-    const int bz = tile % nbz, by = (tile / nbz) % nby, xc = tile / (nbz * nby);
-    const int tz = threadIdx.x, ty = threadIdx.y;
-    const int NAB = a.nab, NZ = g.NZ, NY = g.NY, NX = g.NX;
-    const int z0 = bz * 64, y0 = by * MBY, z = z0 + tz, y = y0 + ty;
-    const int x0 = xc * XC, x1 = min(x0 + XC, NX);
-    const bool live = z < NZ && y < NY;              // a cell this lane updates
-    const long sx = g.sx;
-    const R *hc = a.hc;
-    const S *szzp = (const S *)a.szz, *sxxp = (const S *)a.sxx, *sxzp = (const S *)a.sxz;
-    const S *syzp = (const S *)a.syz, *syyp = (const S *)a.syy, *sxyp = (const S *)a.sxy;
-    S *vxp = (S *)a.vx, *vyp = (S *)a.vy, *vzp = (S *)a.vz;
-    // loads of padded cells only (z < NZ + H, y < NY + H); others read 0
-    auto ld = [&](const S *p, int zz, int yy, int xx) -> R {
-        if (zz >= NZ + H || yy >= NY + H) return (R)0;
-        return T::ld(p, (unsigned)(g.idx(zz, yy, xx) * sizeof(S)));
-    };
-    // x windows w[j] = f(x - H + j)
-    R wxx[2 * H + 1], wxy[2 * H + 1], wxz[2 * H + 1];
-#pragma unroll
-    for (int j = 0; j < 2 * H; ++j) {
-        wxx[j + 1] = ld(sxxp, z, y, x0 - H + j);
-        wxy[j + 1] = ld(sxyp, z, y, x0 - H + j);
-        wxz[j + 1] = ld(sxzp, z, y, x0 - H + j);
-    }
-    const bool cpml = a.abs_type == 1;
-    const bool pz = cpml && (z > NZ - NAB - 1 || (!a.freesurf && z < NAB));
-    const bool py = cpml && (y < NAB || y > NY - NAB - 1);
-    const int k = z < NAB ? z : z - NZ + 2 * NAB;
-    const int ki = z < NAB ? k : 2 * NAB - k, kh = z < NAB ? k : 2 * NAB - 1 - k;
-    const int j = y < NAB ? y : y - NY + 2 * NAB;
-    const int ji = y < NAB ? j : 2 * NAB - j, jh = y < NAB ? j : 2 * NAB - 1 - j;
-    const int r0 = ty + H, c0 = tz + H;             // own cell in the LDS tile
-    for (int x = x0; x < x1; ++x) {
-#pragma unroll
-        for (int q = 0; q < 2 * H; ++q) {
-            wxx[q] = wxx[q + 1];
-            wxy[q] = wxy[q + 1];
-            wxz[q] = wxz[q + 1];
-        }
-        wxx[2 * H] = ld(sxxp, z, y, x + H);
-        wxy[2 * H] = ld(sxyp, z, y, x + H);
-        wxz[2 * H] = ld(sxzp, z, y, x + H);
-        // the plane's tiles: own cells, then the y and z halos
-        lds[0][r0][c0] = ld(syyp, z, y, x);
-        lds[1][r0][c0] = wxy[H];
-        lds[2][r0][c0] = ld(syzp, z, y, x);
-        lds[3][r0][c0] = ld(szzp, z, y, x);
-        lds[4][r0][c0] = wxz[H];
-        for (int r = ty; r < 2 * H; r += MBY) {
-            const int row = r < H ? r : MBY + r;           // tile row
-            const int yy = y0 - H + row;
-            lds[0][row][c0] = ld(syyp, z, yy, x);
-            lds[1][row][c0] = ld(sxyp, z, yy, x);
-            lds[2][row][c0] = ld(syzp, z, yy, x);
-        }
-        if (tz < 2 * H) {
-            const int col = tz < H ? tz : 64 + tz;         // tile column
-            const int zz = z0 - H + col;
-            lds[2][r0][col] = ld(syzp, zz, y, x);
-            lds[3][r0][col] = ld(szzp, zz, y, x);
-            lds[4][r0][col] = ld(sxzp, zz, y, x);
-        }
-        __syncthreads();
-        if (live) {
-            const bool px = cpml && (x < NAB || x > NX - NAB - 1);
-            R szz_z = DpL<H>(&lds[3][r0][c0], 1, hc);
-            R sxx_x = Dpw<H>(wxx, hc);
-            R sxz_x = Dmw<H>(wxz, hc);
-            R sxz_z = DmL<H>(&lds[4][r0][c0], 1, hc);
-            R syz_y = DmL<H>(&lds[2][r0][c0], COLS, hc);
-            R syz_z = DmL<H>(&lds[2][r0][c0], 1, hc);
-            R syy_y = DpL<H>(&lds[0][r0][c0], COLS, hc);
-            R sxy_x = Dmw<H>(wxy, hc);
-            R sxy_y = DmL<H>(&lds[1][r0][c0], COLS, hc);
-            const long cell = g.idx(z, y, x);
-            const unsigned o = (unsigned)(cell * sizeof(R));
-            const unsigned os = (unsigned)(cell * sizeof(S));
-            const int ii = x < NAB ? x : x - NX + 2 * NAB;
-            const int iI = x < NAB ? ii : 2 * NAB - ii, iH = x < NAB ? ii : 2 * NAB - 1 - ii;
-            const long ipz = (long)x * NY * 2 * NAB + (long)y * 2 * NAB + k;
-            const long ipy = (long)x * 2 * NAB * NZ + (long)j * NZ + z;
-            const long ipx = (long)ii * NY * NZ + (long)y * NZ + z;
-            Pml<R> z0p, z1p, z2p, y0p, y1p, y2p, x0p, x1p, x2p;
-            if (pz) {
-                z0p = pml_load(a, a.psi_sxz_z + ipz, 2, false, ki);
-                z1p = pml_load(a, a.psi_syz_z + ipz, 2, false, ki);
-                z2p = pml_load(a, a.psi_szz_z + ipz, 2, true, kh);
-            }
-            if (py) {
-                y0p = pml_load(a, a.psi_sxy_y + ipy, 1, false, ji);
-                y1p = pml_load(a, a.psi_syy_y + ipy, 1, true, jh);
-                y2p = pml_load(a, a.psi_syz_y + ipy, 1, false, ji);
-            }
-            if (px) {
-                x0p = pml_load(a, a.psi_sxx_x + ipx, 0, true, iH);
-                x1p = pml_load(a, a.psi_sxy_x + ipx, 0, false, iI);
-                x2p = pml_load(a, a.psi_sxz_x + ipx, 0, false, iI);
-            }
-            const R vx0 = T::ldn_(vxp, os), vy0 = T::ldn_(vyp, os), vz0 = T::ldn_(vzp, os);
-            const R rip = ldn(a.rip, o), rjp = ldn(a.rjp, o), rkp = ldn(a.rkp, o);
-            if (pz) {
-                pml_apply(sxz_z, a.psi_sxz_z + ipz, z0p);
-                pml_apply(syz_z, a.psi_syz_z + ipz, z1p);
-                pml_apply(szz_z, a.psi_szz_z + ipz, z2p);
-            }
-            if (py) {
-                pml_apply(sxy_y, a.psi_sxy_y + ipy, y0p);
-                pml_apply(syy_y, a.psi_syy_y + ipy, y1p);
-                pml_apply(syz_y, a.psi_syz_y + ipy, y2p);
-            }
-            if (px) {
-                pml_apply(sxx_x, a.psi_sxx_x + ipx, x0p);
-                pml_apply(sxy_x, a.psi_sxy_x + ipx, x1p);
-                pml_apply(sxz_x, a.psi_sxz_x + ipx, x2p);
-            }
-            R vx = vx0 + ((sxx_x + sxy_y + sxz_z) * rip);
-            R vy = vy0 + ((syy_y + sxy_x + syz_z) * rjp);
-            R vz = vz0 + ((szz_z + sxz_x + syz_y) * rkp);
-            if (a.abs_type == 2) {
-                const R *t = a.taper;
-                if (!a.freesurf && z < NAB) { vx *= t[z]; vy *= t[z]; vz *= t[z]; }
-                if (z > NZ - NAB - 1) { vx *= t[NZ - z - 1]; vy *= t[NZ - z - 1]; vz *= t[NZ - z - 1]; }
-                if (y < NAB) { vx *= t[y]; vy *= t[y]; vz *= t[y]; }
-                if (y > NY - NAB - 1) { vx *= t[NY - y - 1]; vy *= t[NY - y - 1]; vz *= t[NY - y - 1]; }
-                if (x < NAB) { vx *= t[x]; vy *= t[x]; vz *= t[x]; }
-                if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vy *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
-            }
-            T::stn_(vxp, os, vx);
-            T::stn_(vyp, os, vy);
-            T::stn_(vzp, os, vz);
-        }
-        __syncthreads();   // the tiles are rewritten for the next plane
-    }
-}
-
 template <typename R, typename S, int H, int L, bool FS>
 __global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz, unsigned nb, int yg) {
     using T = Store<R, S>;
@@ -651,12 +472,15 @@ void grid3(const Layout &g, int &nbz, unsigned &nb) {
 
 // Slabs per y group (tile3): the largest of want, want/2, ... that divides
 // the slab count; want comes from SEIS_YGROUP_V / SEIS_YGROUP_S when set
-// (tuning), else from the defaults below.
+// (tuning), else from the defaults below.  Stress update: groups of 4 slabs
+// (32 rows) measured 0.661 ms against 0.689 ms ungrouped (viscoelastic
+// 256^3, FD4, CPML, free surface); the velocity update and the elastic
+// stress update measured the same with any grouping.
 #ifndef SEIS_YG_V
 #define SEIS_YG_V 1
 #endif
 #ifndef SEIS_YG_S
-#define SEIS_YG_S 1
+#define SEIS_YG_S 4
 #endif
 inline int env_or(const char *name, int def) {
     const char *v = getenv(name);
@@ -670,30 +494,8 @@ int ygroup(const Layout &g, int want) {
     return G;
 }
 
-// x-planes per marching block, and whether the velocity update marches
-// (SEIS_V3D_MARCH=1; the one-cell-per-lane kernel otherwise)
-#ifndef SEIS_XC
-#define SEIS_XC 32
-#endif
-inline bool v3d_march() {
-    static const bool on = [] {
-        const char *v = getenv("SEIS_V3D_MARCH");
-        return v && *v == '1';
-    }();
-    return on;
-}
-
 template <typename R, typename S, int H>
 void run_v3d(const FDArgs<R> &a, hipStream_t st) {
-    if (v3d_march()) {
-        static const int XC = env_or("SEIS_XC", SEIS_XC);
-        const int nbz = (a.g.NZ + 63) / 64, nby = (a.g.NY + MBY - 1) / MBY;
-        const int nxc = (a.g.NX + XC - 1) / XC;
-        const unsigned nb = (unsigned)nbz * nby * nxc;
-        hipLaunchKernelGGL((k_v3d_m<R, S, H>), dim3(((nb + 7) / 8) * 8), dim3(64, MBY), 0, st, a,
-                           nbz, nby, nb, XC);
-        return;
-    }
     int nbz;
     unsigned nb;
     grid3<BYV>(a.g, nbz, nb);

@@ -540,11 +540,11 @@ __device__ __forceinline__ void batch_cell(const FDArgs<float> &a, int z, int x)
 
 // The tiles [lt0, lt0 + kBatchTiles) of one slot, in its z-fastest tile order.
 template <int H, int KIND>
-__device__ __forceinline__ void batch_tiles(const FDArgs<float> &a, unsigned lt0) {
+__device__ __forceinline__ void batch_tiles(const FDArgs<float> &a, unsigned lt0, int tiles) {
     const int NZ = a.g.NZ, NX = a.g.NX;
     const unsigned nbz = (NZ + BZ - 1) / BZ, nt = nbz * ((NX + BT - 1) / BT);
 #pragma unroll 1
-    for (int i = 0; i < kBatchTiles; ++i) {
+    for (int i = 0; i < tiles; ++i) {
         const unsigned lt = lt0 + i;
         if (lt >= nt) break;
         const int z = (lt % nbz) * BZ + threadIdx.x, x = (lt / nbz) * BT + threadIdx.y;
@@ -568,11 +568,11 @@ __global__ __launch_bounds__(256) void k_batch_pass(const BatchSlot *__restrict_
     const BatchSlot &S = slots[tab.ids[k]];
     const int t = step - S.start;
     if (!S.active || t < 0 || t >= S.tmax) return;
-    const unsigned lt0 = (blk - tab.block0[k]) * kBatchTiles;
+    const unsigned lt0 = (blk - tab.block0[k]) * tab.tiles;
     if (S.nd_case == 22) {
-        SEIS_SWITCH_H(S.a.g.H, (batch_tiles<HH, PASS + 2>(S.a, lt0)));
+        SEIS_SWITCH_H(S.a.g.H, (batch_tiles<HH, PASS + 2>(S.a, lt0, tab.tiles)));
     } else {
-        SEIS_SWITCH_H(S.a.g.H, (batch_tiles<HH, PASS>(S.a, lt0)));
+        SEIS_SWITCH_H(S.a.g.H, (batch_tiles<HH, PASS>(S.a, lt0, tab.tiles)));
     }
 }
 

@@ -187,6 +187,7 @@ constexpr int kBatchTiles = SEIS_BATCH_TILES;
 struct BatchTable {
     int n;
     unsigned nblocks;
+    int tiles = kBatchTiles;   // consecutive tiles per block
     const int *ids;          // [n] slot indices (device)
     const unsigned *block0;  // [n+1] first block of each slot (device)
 };
