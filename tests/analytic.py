@@ -305,3 +305,110 @@ def gsls_inverse_q(tau, FL, freqs):
     tl = 1.0 / (2 * np.pi * np.atleast_1d(np.asarray(FL, np.float64)))
     Mw = 1.0 + tau * sum(1j * w * t / (1.0 + 1j * w * t) for t in tl)
     return Mw.imag / Mw.real
+
+
+# ------------------------------------------------------------------ Lamb 3D
+def lamb_roots(pois):
+    """Roots x1 <= x2 <= x3 of the Rayleigh cubic 16(1-a2)x^3 - 8(3-2a2)x^2
+    + 8x - 1 = 0, a2 = (Cs/Cp)^2; x3 = (Cs/Cr)^2 (Kausel 2012, Proc. R. Soc.
+    A 468, 'Lamb's problem at its simplest', eq. for the Rayleigh function)."""
+    a2 = (1 - 2 * pois) / (2 - 2 * pois)
+    x = np.roots([16 * (1 - a2), -8 * (3 - 2 * a2), 8.0, -1.0])
+    x = np.sort_complex(x)
+    if np.any(np.abs(x.imag) > 1e-12):
+        raise ValueError("complex Rayleigh roots (Poisson's ratio above 0.2631) not handled")
+    return a2, np.sort(x.real)
+
+
+def lamb_step_integral(tau, pois, comp="r"):
+    """Time integral, in dimensionless time tau = t Cs / r, of the surface
+    displacement under a horizontal step point load on the surface of a
+    half space (Kausel 2012): comp 'r' the radial displacement along the
+    load (varies as cos(theta)), 't' the tangential one (as -sin(theta)),
+    both times 2 pi mu r.  The step responses are, with S_j = |tau^2 - x_j|^(1/2)
+    and C_j = (1 - x_j) |a2 - x_j|^(1/2) / prod_{k != j} (x_j - x_k):
+      radial:     (1-nu)/2 tau^2 (C1/S1 + C2/S2 + C3/S3)   tp < tau < 1,
+                  1 + (1-nu) C3 tau^2 / S3                 1 < tau < tr,
+                  1                                         tau > tr;
+      tangential: (1-nu)/2 (1 - C1 S1 - C2 S2 + C3 S3)     tp < tau < 1,
+                  (1-nu) (1 + C3 S3)                        1 < tau < tr,
+                  1 - nu                                    tau > tr,
+    integrated in closed form (the radial one has an integrable singularity
+    at the Rayleigh arrival tr = x3^(1/2))."""
+    a2, (x1, x2, x3) = lamb_roots(pois)
+    xs = (x1, x2, x3)
+    C = [(1 - xj) * np.sqrt(abs(a2 - xj)) / np.prod([xj - xk for xk in xs if xk != xj])
+         for xj in xs]
+    tp, tr = np.sqrt(a2), np.sqrt(x3)
+    nu1 = 1 - pois
+    tau = np.asarray(tau, np.float64)
+
+    def Ij(t, xj):       # int t^2 / sqrt(t^2 - xj) dt
+        s = np.sqrt(np.maximum(t * t - xj, 0.0))
+        return 0.5 * t * s + 0.5 * xj * np.log(t + s)
+
+    def Jj(t, xj):       # int sqrt(t^2 - xj) dt
+        s = np.sqrt(np.maximum(t * t - xj, 0.0))
+        return 0.5 * t * s - 0.5 * xj * np.log(t + s)
+
+    def K3(t):           # int t^2 / sqrt(x3 - t^2) dt
+        t = np.minimum(t, tr)
+        return 0.5 * x3 * np.arcsin(t / tr) - 0.5 * t * np.sqrt(np.maximum(x3 - t * t, 0.0))
+
+    def L3(t):           # int sqrt(x3 - t^2) dt
+        t = np.minimum(t, tr)
+        return 0.5 * t * np.sqrt(np.maximum(x3 - t * t, 0.0)) + 0.5 * x3 * np.arcsin(t / tr)
+
+    def partA(t):        # integral from tp to t, t in [tp, 1]
+        if comp == "r":
+            return 0.5 * nu1 * (C[0] * (Ij(t, x1) - Ij(tp, x1)) + C[1] * (Ij(t, x2) - Ij(tp, x2))
+                                + C[2] * (K3(t) - K3(tp)))
+        return 0.5 * nu1 * ((t - tp) - C[0] * (Jj(t, x1) - Jj(tp, x1))
+                            - C[1] * (Jj(t, x2) - Jj(tp, x2)) + C[2] * (L3(t) - L3(tp)))
+
+    def partB(t):        # integral from 1 to t, t in [1, tr]
+        if comp == "r":
+            return (t - 1.0) + nu1 * C[2] * (K3(t) - K3(1.0))
+        return nu1 * ((t - 1.0) + C[2] * (L3(t) - L3(1.0)))
+
+    out = np.zeros_like(tau)
+    a = (tau > tp) & (tau <= 1.0)
+    out[a] = partA(tau[a])
+    A1 = partA(np.array(1.0))
+    b = (tau > 1.0) & (tau <= tr)
+    out[b] = A1 + partB(tau[b])
+    B1 = partB(np.array(tr))
+    c = tau > tr
+    last = 1.0 if comp == "r" else nu1
+    out[c] = A1 + B1 + last * (tau[c] - tr)
+    return out
+
+
+def ricker_dddot(f0, t):
+    """Third time derivative of ricker_t."""
+    tt = t - 1.5 / f0
+    a = (np.pi * f0) ** 2
+    e = np.exp(-a * tt ** 2)
+    # d/dt of ricker_ddot's (-6a + 24a^2 tt^2 - 8a^3 tt^4) e
+    return (48.0 * a * a * tt - 32.0 * a ** 3 * tt ** 3) * e \
+        - 2.0 * a * tt * (-6.0 * a + 24.0 * a * a * tt ** 2 - 8.0 * a ** 3 * tt ** 4) * e
+
+
+def lamb_shot(offsets, vp, vs, rho, f0, dt, nt, comp="r", shift=V_SHIFT, resamp=20):
+    """Surface particle velocity (nt, len offsets) from a horizontal point
+    force with the Ricker time function on the free surface of a
+    homogeneous half space, up to a common positive factor: v = s''' * W,
+    W the time integral of the step response (lamb_step_integral), sampled
+    at t_n + shift dt on a grid resamp times finer."""
+    pois = 0.5 * (vp * vp - 2 * vs * vs) / (vp * vp - vs * vs)
+    mu = rho * vs * vs
+    dtf = dt / resamp
+    nf = nt * resamp
+    tf = np.arange(nf + 1) * dtf
+    s3 = ricker_dddot(f0, np.arange(nf) * dtf + shift * dt)
+    out = np.zeros((nt, len(offsets)))
+    for i, r in enumerate(offsets):
+        W = lamb_step_integral(tf * vs / r, pois, comp) * (r / vs) / (2 * np.pi * mu * r)
+        v = np.fft.irfft(np.fft.rfft(W[:nf], 2 * nf) * np.fft.rfft(s3, 2 * nf), 2 * nf)[:nf] * dtf
+        out[:, i] = v[::resamp]
+    return out

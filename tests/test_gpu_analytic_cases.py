@@ -1,13 +1,13 @@
 """The HIP path on the cases that pin the oracle's explosive source, pressure
 output, 2D acoustic update and free surface (tests/test_oracle_source_surface_pins.py):
 the GPU equals the oracle on the same inputs and meets the closed forms
-itself; and the 3D viscoelastic benchmark geometry (explosion 10 cells deep,
+itself (Lamb's problem also at the reference's own 300^3 geometry); and the 3D viscoelastic benchmark geometry (explosion 10 cells deep,
 receivers one cell below the free surface) on a reduced grid, past the
 first arrival, in FP32, FP16 storage and FP64."""
 import numpy as np
 import pytest
 
-from analytic import explosion_2d, explosion_3d, garvin_shot
+from analytic import explosion_2d, explosion_3d, garvin_shot, lamb_shot
 from cases import rel_l2
 from seiscl_amd.workloads import config2, ricker
 
@@ -50,6 +50,34 @@ def test_garvin_gpu(oracle_lib, gpu_ctx):
     fd = out[0].astype(np.float64)
     ana = garvin_shot(gx - sx + dh / 2, sz, sz, VP, VS, f0, dt, NT, comp="x")
     assert rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana)) < 6e-2
+
+
+@pytest.mark.parametrize("comp", ["r", "t"])
+def test_lamb_3d_gpu(oracle_lib, gpu_ctx, comp):
+    from test_oracle_source_surface_pins import lamb_case
+    p, shot, csts, k, ana = lamb_case(comp=comp)
+    print(f"[analytic] lamb 3D {comp}", flush=True)
+    ref = oracle_lib.forward(p, *shot, **csts)
+    out = _gpu(gpu_ctx, p, shot, **csts)
+    for r, o in zip(ref, out):
+        assert rel_l2(o, r) <= 1e-6
+    fd = out[k].astype(np.float64)
+    assert rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana)) < 2e-2
+
+
+@pytest.mark.parametrize("comp", ["r", "t"])
+def test_lamb_3d_reference_geometry_gpu(gpu_ctx, comp):
+    """SeisCL/tests/test_analytics.py:234-324 as written: 300^3, dh 2 m,
+    dt 0.25 ms, NT 800, 20 Hz, FD4, CPML 112, force and receivers on the
+    surface (inline: x; crossline: y), offsets 5 dh to the far absorbing
+    layer, bar 6e-2 on unit-norm gathers."""
+    from test_oracle_source_surface_pins import lamb_case
+    p, shot, csts, k, ana = lamb_case(N=300, nab=112, NT=800, comp=comp)
+    print(f"[analytic] lamb 3D reference geometry {comp}", flush=True)
+    fd = _gpu(gpu_ctx, p, shot, **csts)[k].astype(np.float64)
+    err = rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana))
+    print(f"[analytic] lamb 3D 300^3 {comp}: {err:.3e}", flush=True)
+    assert err < 6e-2, err
 
 
 def test_explosion_3d_gpu(oracle_lib, gpu_ctx):

@@ -16,13 +16,21 @@ and free surface against closed forms and exact reductions (CPU only).
   y-invariant state; and plane strain holds exactly on such a state,
   syy = lambda/(2(lambda+mu)) (sxx+szz), at the surface too (with tau_p =
   tau_s also for the viscoelastic correction), which fixes the syy update.
+- Lamb's problem in 3D (SeisCL/tests/test_analytics.py:234-324 setting on
+  a smaller grid): a horizontal point force on the free surface of a
+  homogeneous half space, surface velocities against Kausel's closed-form
+  step response (tests/analytic.py lamb_shot), along the force (radial) and
+  across it (tangential).  It fixes the 3D image method of
+  src/surface3D.cl:39-271; the oracle with the sign of the sxx/syy
+  correction flipped misses the radial gather by 0.42 and the tangential
+  one by 5.7e-2.
 - 2D acoustic pressure reciprocity (uniform modulus, variable density).
 - q_to_tau: the band average of the GSLS 1/Q equals the target.
 """
 import numpy as np
 import pytest
 
-from analytic import explosion_2d, explosion_3d, garvin_shot, gsls_inverse_q
+from analytic import explosion_2d, explosion_3d, garvin_shot, gsls_inverse_q, lamb_shot
 from cases import rel_l2
 from seiscl_amd.workloads import q_to_tau, ricker
 
@@ -126,6 +134,43 @@ def _state_2d(oracle_lib, L, FD=4, NT=90, seed=11):
     rp = np.array([[NX // 2 * dh], [0], [dh], [0], [1], [0], [0], [0]])
     _, f = oracle_lib.forward(p, sp, src, rp, fields=True, **csts)
     return p, csts, f
+
+
+def lamb_case(N=100, nab=16, NT=400, comp="r"):
+    """Lamb's problem: x (comp 'r') or y (comp 't') force at the surface,
+    x = (nab + 2 FD) dh, mid y; receivers of that component along x at the
+    surface, offsets 5 dh ..; dh 2 m, dt 0.25 ms, 20 Hz, FD4, CPML."""
+    dh, dt, f0, FD = 2.0, 0.25e-3, 20.0, 4
+    sx, sy = (nab + 2 * FD) * dh, (N // 2) * dh
+    gx = np.arange(sx + 5 * dh, (N - nab - 2 * FD) * dh, dh)
+    shp = (N, N, N)
+    stype = 0 if comp == "r" else 1
+    sp = np.array([[sx], [sy], [0.0], [0], [stype]])
+    rp = np.stack([gx, 0 * gx + sy, 0 * gx, 0 * gx, np.arange(gx.size) + 1.0, 0 * gx, 0 * gx,
+                   0 * gx])
+    shot = (sp, ricker(f0, NT, dt)[:, None].astype(np.float32), rp)
+    csts = dict(N=np.array(shp), ND=3, dh=dh, dt=dt, NT=NT, FDORDER=FD, abs_type=1, nab=nab,
+                freesurf=1, seisout=1, f0=f0, VPPML=VP, FPML=f0)
+    # force sources act half a step before the velocities they feed are
+    # recorded; the best alignment measured is 0.75 (radial) and 0.5 steps
+    ana = lamb_shot(gx - sx, VP, VS, RHO, f0, dt, NT, comp=comp,
+                    shift=0.75 if comp == "r" else 0.5)
+    return _homog(shp), shot, csts, stype, ana
+
+
+@pytest.mark.parametrize("comp", ["r", "t"])
+def test_lamb_3d(oracle_lib, comp):
+    """Surface velocities against Kausel's solution, both normalised to unit
+    L2 norm: the reference's bar is 6e-2; measured 7.8e-3 (radial) and
+    6.4e-3 (tangential), so the bar here is 2e-2, which the oracle with a
+    flipped free-surface correction misses (0.42, 5.7e-2)."""
+    p, shot, csts, k, ana = lamb_case(comp=comp)
+    fd = oracle_lib.forward(p, *shot, **csts)[k].astype(np.float64)
+    err = rel_l2(fd / np.linalg.norm(fd), ana / np.linalg.norm(ana))
+    assert err < 2e-2, err
+    # without the free surface the same run misses: the surface is seen
+    fd0 = oracle_lib.forward(p, *shot, **dict(csts, freesurf=0))[k].astype(np.float64)
+    assert rel_l2(fd0 / np.linalg.norm(fd0), ana / np.linalg.norm(ana)) > 3 * err
 
 
 @pytest.mark.parametrize("L", [0, 1])
