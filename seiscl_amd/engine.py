@@ -144,6 +144,7 @@ class Context:
         h = ctypes.c_void_p()
         check(self.lib.seis_ctx_create(int(device), ctypes.byref(h)))
         self.handle = h
+        self.device = int(device)
 
     def footprint(self) -> int:
         return int(self.lib.seis_ctx_footprint(self.handle))

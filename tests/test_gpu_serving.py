@@ -116,3 +116,50 @@ def test_config4_size_requests(oracle_lib, gpu_ctx):
         batch.remove(s)
         m.close()
     batch.close()
+
+
+def test_gradient_and_unbatched_requests(gpu_ctx):
+    """Gradient requests, and modeling requests the batch kernels do not
+    cover (viscoelastic, free surface), are served beside the batch on a
+    second context of the GPU: each one's seismograms, gradient and misfit
+    equal those of the same request run alone bit for bit, and the batched
+    requests arriving among them still equal theirs."""
+    from cases import small_case
+    from seiscl_amd import Model
+    from seiscl_amd.serving import Request, batchable, make_requests, run_alone, serve_batched
+    work = _requests(6, seed=5)
+    solo = [small_case(ND=2, FDORDER=4, L=0, NT=150, seed=1),
+            small_case(ND=2, FDORDER=4, L=1, abs_type=1, freesurf=1, NT=150, seed=2),
+            small_case(ND=2, FDORDER=8, L=2, freesurf=1, NT=120, seed=3)]
+    observed = [None] * len(work)
+    for k, (csts, params, shot) in enumerate(solo):
+        work.append((csts, params, shot))
+        if k == 2:
+            observed.append(None)           # modeling outside the batch kernels
+            continue
+        true = {n: (v * 1.03 if n == "vp" else v) for n, v in params.items()}
+        m = Model(gpu_ctx, true, **csts)
+        observed.append(m.forward(*shot))
+        m.close()
+    assert not batchable(work[-1][0]) and batchable(work[0][0])
+    alone = []
+    for i, (csts, params, shot) in enumerate(work):
+        r = Request(i, csts, params, shot, din=observed[i])
+        run_alone(gpu_ctx, r)
+        alone.append(r)
+    arrivals = np.linspace(0.0, 0.04, len(work))
+    order = np.random.default_rng(7).permutation(len(work))   # kinds interleaved in time
+    reqs = serve_batched(gpu_ctx, make_requests(work, arrivals[order], observed), max_slots=4,
+                         chunk=30)
+    kinds = [r.kind for r in sorted(reqs, key=lambda q: q.rid)]
+    print(f"[serving] kinds {kinds}", flush=True)
+    assert kinds.count("gradient") == 2
+    for r, a in zip(sorted(reqs, key=lambda q: q.rid), alone):
+        assert r.finish >= r.arrival
+        for x, y in zip(r.out, a.out):
+            np.testing.assert_array_equal(x, y)
+        assert r.misfit == a.misfit
+        assert sorted(r.grad) == sorted(a.grad)
+        for k in a.grad:
+            assert np.abs(a.grad[k]).max() > 0, k
+            np.testing.assert_array_equal(r.grad[k], a.grad[k])
