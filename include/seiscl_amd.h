@@ -53,7 +53,8 @@ typedef struct seis_csts {
     int tmax;           /* forward steps, 0 -> NT   (src/read_hdf5.c:523-526)    */
     int tmin;           /* first adjoint step bound (src/time_stepping.c:869)     */
     int gradout;        /* 1: keep what the gradient needs                        */
-    int FP16;           /* 0: FP32; 2: FP16 storage, FP32 compute; 64: FP64      */
+    int FP16;           /* 0: FP32; 1: FP32 in the scaled units of 2; 2: FP16    */
+                        /* storage, FP32 compute; 64: FP64                        */
     int gradfix;        /* gradient corrections of reference defects (bit set):   */
                         /*  1 res_scale reads the buoyancy at the receiver's own  */
                         /*    cell (src/residuals.c:487-491 uses x three times);  */
@@ -67,6 +68,9 @@ typedef struct seis_csts {
                         /*    stress-to-strain conversion and the chain rule (the */
                         /*    reference uses the relaxed ones, src/update_adjs2D. */
                         /*    cl:441-451, src/calc_grad.c:1020-1056);             */
+                        /* 32 vs and rho gradients through the staggered averages */
+                        /*    of the transform (the reference gives each node's   */
+                        /*    term to the cell of the same index); not in 223     */
                         /* 64 normalised-misfit adjoint sources (scalermsnorm)    */
                         /*    scale the trace by o.d, the derivative of           */
                         /*    1/2 |o/|o| - d/|d||^2 (src/residuals.c:331-337 use  */
@@ -210,7 +214,7 @@ int seis_read_field(seis_model *m, const char *name, float *host);
 /* memory (save_bnd, src/time_stepping.c:352-395, kept resident instead of   */
 /* the reference's pinned host copy); the adjoint run re-injects it while    */
 /* running the forward wavefield backwards (inject_bnd, :397-424).           */
-/* Viscoelastic (L=1..4; FP32, and in 3D also FP16 storage or scaled units): */
+/* Viscoelastic (L=1..4; FP32, FP64, FP16 storage or scaled units):          */
 /* running the memory variables backwards is unstable, so the forward run    */
 /* keeps full-state checkpoints in device memory and the adjoint recomputes  */
 /* the forward run segment by segment (one or two checkpoint levels, chosen  */

@@ -122,9 +122,16 @@ __device__ __forceinline__ void grad_mmu_2d(const AdjArgs<R> &A, long i, R sxx, 
     const R c3 = 1.0 / (lmu * lmu);
     const R c5 = 0.25 * c3;
     const R dM = c1 * (sxx + szz) * (lsxx + lszz);
-    const R dmu = -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+    if (A.gsh[0]) {
+        // the shear term per staggered node, chained on the host
+        const R dmu = dM - c5 * ((sxx - szz) * (lsxx - lszz));
+        A.gsh[0][i] += (double)(sxz * lsxz) * A.gscale_s;
+        A.gmu[i] += (double)dmu * A.gscale_s;
+    } else {
+        const R dmu = -c3 * (sxz * lsxz) + dM - c5 * ((sxx - szz) * (lsxx - lszz));
+        A.gmu[i] += (double)dmu * A.gscale_s;
+    }
     A.gM[i] += (double)-dM * A.gscale_s;
-    A.gmu[i] += (double)dmu * A.gscale_s;
     if (A.hM) {
         const R dMH = c1 * (sxx + szz) * (sxx + szz);
         const R hmu = c3 * sxz * sxz - (A.hfix ? dMH : dM) + c5 * (sxx - szz) * (sxx - szz);
@@ -153,11 +160,19 @@ __device__ __forceinline__ void grad_mmu_3d(const AdjArgs<R> &A, long i, R sxx, 
     const R dM = c1 * (sxx + syy + szz) * (lsxx + lsyy + lszz);
     // evaluated in double from the 4.0 / 3 term on, as the reference's
     // unsuffixed literals make it
-    const double dmu = -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM -
-                       c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
+    const double dev = c5 * (lsxx * (2.0 * sxx - syy - szz) + lsyy * (2.0 * syy - sxx - szz) +
                              lszz * (2.0 * szz - sxx - syy));
+    if (A.gsh[0]) {
+        // the shear terms per staggered node, chained on the host
+        own_add(A.gsh[0] + i, (double)(sxz * lsxz) * A.gscale_s);
+        own_add(A.gsh[1] + i, (double)(sxy * lsxy) * A.gscale_s);
+        own_add(A.gsh[2] + i, (double)(syz * lsyz) * A.gscale_s);
+        own_add(A.gmu + i, (4.0 / 3 * dM - dev) * A.gscale_s);
+    } else {
+        const double dmu = -c3 * (sxz * lsxz + sxy * lsxy + syz * lsyz) + 4.0 / 3 * dM - dev;
+        own_add(A.gmu + i, dmu * A.gscale_s);
+    }
     own_add(A.gM + i, (double)-dM * A.gscale_s);
-    own_add(A.gmu + i, dmu * A.gscale_s);
     if (A.hM) {
         const R dMH = c1 * (sxx + syy + szz) * (sxx + syy + szz);
         const double hmu = c3 * (sxz * sxz + sxy * sxy + syz * syz) - 4.0 / 3 * dMH +
@@ -353,8 +368,13 @@ __global__ __launch_bounds__(256) void k_adj_v2d(AdjArgs<R> A, int nbz, unsigned
     a.vx[i] = v2[0];
     a.vz[i] = v2[1];
     // density gradient (src/update_adjv2D.cl:326)
-    const R d = -vx * lvx - vz * lvz;
-    A.grho[i] += d;
+    if (A.grz) {
+        A.grho[i] += (double)(-vx * lvx);
+        A.grz[i] += (double)(-(vz * lvz));
+    } else {
+        const R d = -vx * lvx - vz * lvz;
+        A.grho[i] += d;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -524,8 +544,14 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
     own_st(a.vy + i, v3[1]);
     own_st(a.vz + i, v3[2]);
     // density gradient (src/update_adjv3D.cl:481)
-    const R d = -vx * lvx - vy * lvy - vz * lvz;
-    own_add(A.grho + i, (double)d);
+    if (A.grz) {
+        own_add(A.grho + i, (double)(-vx * lvx));
+        own_add(A.gry + i, (double)(-(vy * lvy)));
+        own_add(A.grz + i, (double)(-(vz * lvz)));
+    } else {
+        const R d = -vx * lvx - vy * lvy - vz * lvz;
+        own_add(A.grho + i, (double)d);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -659,8 +685,13 @@ __global__ __launch_bounds__(256) void k_adjh_v2d(AdjArgs<R> A, int nbz, unsigne
     a.vx[i] = v2[0];
     a.vz[i] = v2[1];
     // density gradient (src/update_adjv2D.cl:326) from the forward state
-    const R d = -hst<R, S>(A, 0, i) * lvx - hst<R, S>(A, 1, i) * lvz;
-    A.grho[i] += (double)d * A.gscale_v;
+    if (A.grz) {
+        A.grho[i] += (double)(-hst<R, S>(A, 0, i) * lvx) * A.gscale_v;
+        A.grz[i] += (double)(-(hst<R, S>(A, 1, i) * lvz)) * A.gscale_v;
+    } else {
+        const R d = -hst<R, S>(A, 0, i) * lvx - hst<R, S>(A, 1, i) * lvz;
+        A.grho[i] += (double)d * A.gscale_v;
+    }
 }
 
 template <typename R, typename S, int H, int L>
@@ -826,9 +857,15 @@ __global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigne
     own_st(a.vy + i, v3[1]);
     own_st(a.vz + i, v3[2]);
     // density gradient (src/update_adjv3D.cl:481) from the forward state
-    const R d = -hst_own<R, S>(A, 0, i) * lvx - hst_own<R, S>(A, 1, i) * lvy -
-                hst_own<R, S>(A, 2, i) * lvz;
-    own_add(A.grho + i, (double)d * A.gscale_v);
+    if (A.grz) {
+        own_add(A.grho + i, (double)(-hst_own<R, S>(A, 0, i) * lvx) * A.gscale_v);
+        own_add(A.gry + i, (double)(-(hst_own<R, S>(A, 1, i) * lvy)) * A.gscale_v);
+        own_add(A.grz + i, (double)(-(hst_own<R, S>(A, 2, i) * lvz)) * A.gscale_v);
+    } else {
+        const R d = -hst_own<R, S>(A, 0, i) * lvx - hst_own<R, S>(A, 1, i) * lvy -
+                    hst_own<R, S>(A, 2, i) * lvz;
+        own_add(A.grho + i, (double)d * A.gscale_v);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -202,6 +202,10 @@ struct seis_model {
         double *gM = nullptr, *gmu = nullptr, *grho = nullptr;   // padded layout
         double *hM = nullptr, *hmu = nullptr;                    // Hessian (Hout)
         double *gtp = nullptr, *gts = nullptr;   // tau_p, tau_s (viscoelastic)
+        // gradfix FIX_AVG: per-node shear products (xz, xy, yz) and density
+        // terms of vy and vz (AdjArgs::gsh, gry, grz)
+        double *gsh[3] = {nullptr, nullptr, nullptr};
+        double *gry = nullptr, *grz = nullptr;
         float *res = nullptr;       // [nout][nrec][NT] scaled adjoint sources
         float *stmp = nullptr;      // adjoint free surface: 3 * NX * NY
         size_t res_cap = 0;
@@ -541,18 +545,19 @@ void step_sources(seis_model *m, int t, int pdir, hipStream_t st, const int *tb 
         launch_sources<float, float>(m->a, m->nd_case, m->shot, m->c.NT, t, pdir, st, tb);
 }
 void step_v(seis_model *m, hipStream_t st) {
-    if (m->prec == P_FP16) launch_v3d<float, __half>(m->a, st);
+    if (m->prec == P_FP16) launch_update_v<float, __half>(m->a, m->nd_case, st);
     else if (m->prec == P_FP64) launch_update_v(m->ad, m->nd_case, st);
     else launch_update_v(m->a, m->nd_case, st);
 }
 void step_s(seis_model *m, hipStream_t st) {
-    if (m->prec == P_FP16) launch_s3d<float, __half>(m->a, st);
+    if (m->prec == P_FP16) launch_update_s<float, __half>(m->a, m->nd_case, st);
     else if (m->prec == P_FP64) launch_update_s(m->ad, m->nd_case, st);
     else launch_update_s(m->a, m->nd_case, st);
 }
 // the free surface as its own kernel (2D, and 3D runs that save boundaries)
 void step_surface(seis_model *m, hipStream_t st) {
-    if (m->prec == P_FP64) launch_surface(m->ad, m->nd_case, 1, st);
+    if (m->prec == P_FP16) launch_surface<float, __half>(m->a, m->nd_case, 1, st);
+    else if (m->prec == P_FP64) launch_surface(m->ad, m->nd_case, 1, st);
     else launch_surface(m->a, m->nd_case, 1, st);
 }
 void step_varsout(seis_model *m, int t, hipStream_t st, const int *tb = nullptr) {
@@ -730,6 +735,13 @@ int grad_setup(seis_model *m) {
     if (G.visco && c.param_type <= 2)
         for (double **p : {&G.gtp, &G.gts})
             if (!(*p = (double *)m->dalloc(nfd))) return 1;
+    if ((c.gradfix & FIX_AVG) && c.restype != 1) {
+        const int nsh = c.ND == 3 ? 3 : 1;
+        for (int q = 0; q < nsh; ++q)
+            if (!(G.gsh[q] = (double *)m->dalloc(nfd))) return 1;
+        if (c.ND == 3 && !(G.gry = (double *)m->dalloc(nfd))) return 1;
+        if (!(G.grz = (double *)m->dalloc(nfd))) return 1;
+    }
     G.ready = true;
     return 0;
 }
@@ -922,8 +934,6 @@ int seis_model_create(seis_ctx *ctx, const seis_csts *cin, const float *p0, cons
     if (c.L < 0 || c.L > 4) return fail("L must be in [0, 4]");
     if (c.FP16 != P_FP32 && c.FP16 != P_SCALED && c.FP16 != P_FP16 && c.FP16 != P_FP64)
         return fail("FP16 must be 0 (FP32), 1 (FP32, scaled units), 2 (FP16 storage) or 64 (FP64)");
-    if ((c.FP16 == P_FP16 || c.FP16 == P_SCALED) && c.ND != 3)
-        return fail("FP16 storage is available for ND=3 in this build");
     if (c.NT <= 0) return fail("NT must be positive");
     const int nd = c.ND == 3 ? 3 : 2;
     for (int d = 0; d < nd; ++d) {
@@ -1491,6 +1501,8 @@ int seis_grad_zero(seis_model *m) {
     if (m->grad.hmu) launch_fill(m->grad.hmu, nb, m->ctx->stream);
     if (m->grad.gtp) launch_fill(m->grad.gtp, nb, m->ctx->stream);
     if (m->grad.gts) launch_fill(m->grad.gts, nb, m->ctx->stream);
+    for (double *p : {m->grad.gsh[0], m->grad.gsh[1], m->grad.gsh[2], m->grad.gry, m->grad.grz})
+        if (p) launch_fill(p, nb, m->ctx->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     return 0;
@@ -2054,6 +2066,11 @@ static int adjoint_loops(seis_model *m, const MovieOut *mv) {
         A.gts = G.gts;
         A.s0 = (R)relax_sum(m->c);
     }
+    if (G.grz && !A.rtm) {
+        for (int q = 0; q < 3; ++q) A.gsh[q] = G.gsh[q];
+        A.gry = G.gry;
+        A.grz = G.grz;
+    }
     A.hfix = (m->c.gradfix & FIX_HESS) != 0;
     if (m->c.freesurf && (!A.r.M || !A.r.mu || !G.stmp))
         return fail("seis_adjoint: the adjoint free surface has no moduli on the device");
@@ -2258,6 +2275,22 @@ int seis_grad_read(seis_model *m, float *grad) {
     const double *src[3] = {G.gM, G.gmu, G.grho};
     if (accum_read(m, src, 3, acc)) return 1;
     const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
+    if (G.grz) {
+        // per-node terms through the staggered averages onto the cells
+        std::vector<double> sh[3], rh[3];
+        const double *ssrc[3] = {G.gsh[0], G.gsh[1], G.gsh[2]};
+        const double *rsrc[3] = {G.grho, G.gry, G.grz};
+        const int nq = m->c.ND == 3 ? 3 : 1;
+        if (accum_read(m, ssrc, nq, sh)) return 1;
+        if (m->c.ND == 3) {
+            if (accum_read(m, rsrc, 3, rh)) return 1;
+        } else {
+            rh[0] = acc[2];
+            if (accum_read(m, rsrc + 2, 1, rh + 2)) return 1;
+        }
+        chain_staggered(m->c, m->rp, f0 ? m->rp.muf0 : m->rp.mu, sh, rh, acc[1].data(),
+                        acc[2].data());
+    }
     transf_grad(m->c, m->rp, f0 ? m->rp.Mf0 : m->rp.M, f0 ? m->rp.muf0 : m->rp.mu,
                 acc[0].data(), acc[1].data(), acc[2].data(), grad, m->c.gradfix & FIX_RHO_SCALE);
     return 0;

@@ -23,18 +23,34 @@ namespace seis {
 
 namespace {
 
+// Element i of a field typed R* whose storage is S (FP16 storage: the
+// pointer holds __half values).
+template <typename R, typename S>
+__device__ __forceinline__ R getS(const R *p, long i) {
+    return Store<R, S>::ld((const S *)p + i, 0u);
+}
+template <typename R, typename S>
+__device__ __forceinline__ void putS(R *p, long i, R v) {
+    Store<R, S>::st((S *)p + i, 0u, v);
+}
+
 // ---------------------------------------------------------------------------
 // 2D P-SV: src/update_v2D.cl:32-246, src/update_s2D.cl:30-326
 // ---------------------------------------------------------------------------
-template <typename R, int H>
+//
+// S is the storage type of the wavefields and memory variables (R, or
+// __half with R = float); coefficients and CPML memories are always R.
+template <typename R, typename S, int H>
 __device__ __forceinline__ void cell_v2d(const FDArgs<R> &a, int z, int x) {
+    using T = Store<R, S>;
     const Layout &g = a.g;
     const long i = g.idx(z, 0, x), sx = g.sx;
+    const unsigned os = (unsigned)(i * sizeof(S));
     const R *hc = a.hc;
-    R sxx_x = Dp<H>(a.sxx, i, sx, hc);
-    R szz_z = Dp<H>(a.szz, i, 1, hc);
-    R sxz_z = Dm<H>(a.sxz, i, 1, hc);
-    R sxz_x = Dm<H>(a.sxz, i, sx, hc);
+    R sxx_x = DpS<H>((const S *)a.sxx, os, sx, hc);
+    R szz_z = DpS<H>((const S *)a.szz, os, 1, hc);
+    R sxz_z = DmS<H>((const S *)a.sxz, os, 1, hc);
+    R sxz_x = DmS<H>((const S *)a.sxz, os, sx, hc);
     const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
     if (a.abs_type == 1) {
         if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
@@ -53,8 +69,8 @@ __device__ __forceinline__ void cell_v2d(const FDArgs<R> &a, int z, int x) {
             pml(sxz_x, a.psi_sxz_x + ip, cp(a, 0, CB, iI), cp(a, 0, CA, iI), cp(a, 0, CK, iI));
         }
     }
-    R vx = a.vx[i] + ((sxx_x + sxz_z) * a.rip[i]);
-    R vz = a.vz[i] + ((szz_z + sxz_x) * a.rkp[i]);
+    R vx = T::ld((const S *)a.vx, os) + ((sxx_x + sxz_z) * a.rip[i]);
+    R vz = T::ld((const S *)a.vz, os) + ((szz_z + sxz_x) * a.rkp[i]);
     if (a.abs_type == 2) {
         const R *t = a.taper;
         if (!a.freesurf && z < NAB) { vx *= t[z]; vz *= t[z]; }
@@ -62,11 +78,11 @@ __device__ __forceinline__ void cell_v2d(const FDArgs<R> &a, int z, int x) {
         if (x < NAB) { vx *= t[x]; vz *= t[x]; }
         if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
     }
-    a.vx[i] = vx;
-    a.vz[i] = vz;
+    T::st((S *)a.vx, os, vx);
+    T::st((S *)a.vz, os, vz);
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 __global__ __launch_bounds__(256) void k_update_v2d(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
@@ -74,18 +90,20 @@ __global__ __launch_bounds__(256) void k_update_v2d(FDArgs<R> a, int nbz, unsign
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
     if (z >= a.g.NZ || x >= a.g.NX) return;
-    cell_v2d<R, H>(a, z, x);
+    cell_v2d<R, S, H>(a, z, x);
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 __device__ __forceinline__ void cell_s2d(const FDArgs<R> &a, int z, int x) {
+    using T = Store<R, S>;
     const Layout &g = a.g;
     const long i = g.idx(z, 0, x), sx = g.sx;
+    const unsigned os = (unsigned)(i * sizeof(S));
     const R *hc = a.hc;
-    R vxx = Dm<H>(a.vx, i, sx, hc);
-    R vxz = Dp<H>(a.vx, i, 1, hc);
-    R vzz = Dm<H>(a.vz, i, 1, hc);
-    R vzx = Dp<H>(a.vz, i, sx, hc);
+    R vxx = DmS<H>((const S *)a.vx, os, sx, hc);
+    R vxz = DpS<H>((const S *)a.vx, os, 1, hc);
+    R vzz = DmS<H>((const S *)a.vz, os, 1, hc);
+    R vzx = DpS<H>((const S *)a.vz, os, sx, hc);
     const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
     if (a.abs_type == 1) {
         if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
@@ -104,7 +122,8 @@ __device__ __forceinline__ void cell_s2d(const FDArgs<R> &a, int z, int x) {
             pml(vzx, a.psi_vz_x + ip, cp(a, 0, CBH, iH), cp(a, 0, CAH, iH), cp(a, 0, CKH, iH));
         }
     }
-    R sxz = a.sxz[i], sxx = a.sxx[i], szz = a.szz[i];
+    S *sxzp = (S *)a.sxz, *sxxp = (S *)a.sxx, *szzp = (S *)a.szz;
+    R sxz = T::ld(sxzp, os), sxx = T::ld(sxxp, os), szz = T::ld(szzp, os);
     if constexpr (L == 0) {
         const R fipkp = a.muipkp[i];
         const R f = (R)2 * a.mu[i];
@@ -121,8 +140,10 @@ __device__ __forceinline__ void cell_s2d(const FDArgs<R> &a, int z, int x) {
         R sumrxz = 0, sumrxx = 0, sumrzz = 0;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            const long ir = l * nf + i;
-            rxz[l] = a.rxz[ir]; rxx[l] = a.rxx[ir]; rzz[l] = a.rzz[ir];
+            const long ln = l * nf;
+            rxz[l] = T::ld((const S *)a.rxz + ln, os);
+            rxx[l] = T::ld((const S *)a.rxx + ln, os);
+            rzz[l] = T::ld((const S *)a.rzz + ln, os);
             sumrxz += rxz[l]; sumrxx += rxx[l]; sumrzz += rzz[l];
         }
         const R lsxz = (fipkp * (vxz + vzx)) + (DT2 * sumrxz);
@@ -132,11 +153,13 @@ __device__ __forceinline__ void cell_s2d(const FDArgs<R> &a, int z, int x) {
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             const R b = a.bl[l], c = a.cl[l], et = a.eta[l];
-            const long ir = l * nf + i;
+            const long ln = l * nf;
             rxz[l] = b * (rxz[l] * c - et * (dipkp * (vxz + vzx)));
             rxx[l] = b * (rxx[l] * c - et * ((e * (vxx + vzz)) - (d * vzz)));
             rzz[l] = b * (rzz[l] * c - et * ((e * (vxx + vzz)) - (d * vxx)));
-            a.rxz[ir] = rxz[l]; a.rxx[ir] = rxx[l]; a.rzz[ir] = rzz[l];
+            T::st((S *)a.rxz + ln, os, rxz[l]);
+            T::st((S *)a.rxx + ln, os, rxx[l]);
+            T::st((S *)a.rzz + ln, os, rzz[l]);
             sumrxz += rxz[l]; sumrxx += rxx[l]; sumrzz += rzz[l];
         }
         sxz += lsxz + (DT2 * sumrxz);
@@ -150,12 +173,12 @@ __device__ __forceinline__ void cell_s2d(const FDArgs<R> &a, int z, int x) {
         if (x < NAB) { sxx *= t[x]; szz *= t[x]; sxz *= t[x]; }
         if (x > NX - NAB - 1) { sxx *= t[NX - x - 1]; szz *= t[NX - x - 1]; sxz *= t[NX - x - 1]; }
     }
-    a.sxz[i] = sxz;
-    a.sxx[i] = sxx;
-    a.szz[i] = szz;
+    T::st(sxzp, os, sxz);
+    T::st(sxxp, os, sxx);
+    T::st(szzp, os, szz);
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 __global__ __launch_bounds__(256) void k_update_s2d(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
@@ -163,19 +186,21 @@ __global__ __launch_bounds__(256) void k_update_s2d(FDArgs<R> a, int nbz, unsign
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
     if (z >= a.g.NZ || x >= a.g.NX) return;
-    cell_s2d<R, H, L>(a, z, x);
+    cell_s2d<R, S, H, L>(a, z, x);
 }
 
 // ---------------------------------------------------------------------------
 // 2D acoustic: src/update_v2D_acc.cl:80-325, src/update_s2D_acc.cl:79-402
 // ---------------------------------------------------------------------------
-template <typename R, int H>
+template <typename R, typename S, int H>
 __device__ __forceinline__ void cell_v2d_acc(const FDArgs<R> &a, int z, int x) {
+    using T = Store<R, S>;
     const Layout &g = a.g;
     const long i = g.idx(z, 0, x), sx = g.sx;
+    const unsigned os = (unsigned)(i * sizeof(S));
     const R *hc = a.hc;
-    R p_x = Dp<H>(a.p, i, sx, hc);
-    R p_z = Dp<H>(a.p, i, 1, hc);
+    R p_x = DpS<H>((const S *)a.p, os, sx, hc);
+    R p_z = DpS<H>((const S *)a.p, os, 1, hc);
     const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
     if (a.abs_type == 1) {
         if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
@@ -192,8 +217,8 @@ __device__ __forceinline__ void cell_v2d_acc(const FDArgs<R> &a, int z, int x) {
                 cp(a, 0, CKH, iH));
         }
     }
-    R vx = a.vx[i] + (p_x * a.rip[i]);
-    R vz = a.vz[i] + (p_z * a.rkp[i]);
+    R vx = T::ld((const S *)a.vx, os) + (p_x * a.rip[i]);
+    R vz = T::ld((const S *)a.vz, os) + (p_z * a.rkp[i]);
     if (a.abs_type == 2) {
         const R *t = a.taper;
         // the acoustic kernel tapers the top edge even with a free surface
@@ -202,11 +227,11 @@ __device__ __forceinline__ void cell_v2d_acc(const FDArgs<R> &a, int z, int x) {
         if (x < NAB) { vx *= t[x]; vz *= t[x]; }
         if (x > NX - NAB - 1) { vx *= t[NX - x - 1]; vz *= t[NX - x - 1]; }
     }
-    a.vx[i] = vx;
-    a.vz[i] = vz;
+    T::st((S *)a.vx, os, vx);
+    T::st((S *)a.vz, os, vz);
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 __global__ __launch_bounds__(256) void k_update_v2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
@@ -214,16 +239,18 @@ __global__ __launch_bounds__(256) void k_update_v2d_acc(FDArgs<R> a, int nbz, un
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
     if (z >= a.g.NZ || x >= a.g.NX) return;
-    cell_v2d_acc<R, H>(a, z, x);
+    cell_v2d_acc<R, S, H>(a, z, x);
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 __device__ __forceinline__ void cell_s2d_acc(const FDArgs<R> &a, int z, int x) {
+    using T = Store<R, S>;
     const Layout &g = a.g;
     const long i = g.idx(z, 0, x), sx = g.sx;
+    const unsigned os = (unsigned)(i * sizeof(S));
     const R *hc = a.hc;
-    R vxx = Dm<H>(a.vx, i, sx, hc);
-    R vzz = Dm<H>(a.vz, i, 1, hc);
+    R vxx = DmS<H>((const S *)a.vx, os, sx, hc);
+    R vzz = DmS<H>((const S *)a.vz, os, 1, hc);
     const int NAB = a.nab, NZ = g.NZ, NX = g.NX;
     if (a.abs_type == 1) {
         if (z > NZ - NAB - 1 || (!a.freesurf && z < NAB)) {
@@ -240,7 +267,7 @@ __device__ __forceinline__ void cell_s2d_acc(const FDArgs<R> &a, int z, int x) {
                 cp(a, 0, CK, iI));
         }
     }
-    R p = a.p[i];
+    R p = T::ld((const S *)a.p, os);
     if constexpr (L == 0) {
         p += (a.M[i] * (vxx + vzz));
     } else {
@@ -249,13 +276,13 @@ __device__ __forceinline__ void cell_s2d_acc(const FDArgs<R> &a, int z, int x) {
         R rp[L];
         R sumrp = 0;
 #pragma unroll
-        for (int l = 0; l < L; ++l) { rp[l] = a.rp[l * nf + i]; sumrp += rp[l]; }
+        for (int l = 0; l < L; ++l) { rp[l] = T::ld((const S *)a.rp + l * nf, os); sumrp += rp[l]; }
         const R lp = (gg * (vxx + vzz)) + (DT2 * sumrp);
         sumrp = 0;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             rp[l] = a.bl[l] * (rp[l] * a.cl[l] - a.eta[l] * e * (vxx + vzz));
-            a.rp[l * nf + i] = rp[l];
+            T::st((S *)a.rp + l * nf, os, rp[l]);
             sumrp += rp[l];
         }
         p += lp + (DT2 * sumrp);
@@ -267,10 +294,10 @@ __device__ __forceinline__ void cell_s2d_acc(const FDArgs<R> &a, int z, int x) {
         if (x < NAB) p *= t[x];
         if (x > NX - NAB - 1) p *= t[NX - x - 1];
     }
-    a.p[i] = p;
+    T::st((S *)a.p, os, p);
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 __global__ __launch_bounds__(256) void k_update_s2d_acc(FDArgs<R> a, int nbz, unsigned nb) {
     unsigned tile;
     xcd_tile(nb, tile);
@@ -278,28 +305,30 @@ __global__ __launch_bounds__(256) void k_update_s2d_acc(FDArgs<R> a, int nbz, un
     const int bz = tile % nbz, bx = tile / nbz;
     const int z = bz * BZ + threadIdx.x, x = bx * BT + threadIdx.y;
     if (z >= a.g.NZ || x >= a.g.NX) return;
-    cell_s2d_acc<R, H, L>(a, z, x);
+    cell_s2d_acc<R, S, H, L>(a, z, x);
 }
 
 // ---------------------------------------------------------------------------
 // Free surface by the image method (Levander 1988): src/surface2D.cl:30-193,
 // src/surface3D.cl:39-271.  One lane per surface column.
 // ---------------------------------------------------------------------------
-template <typename R, int H>
+template <typename R, typename S, int H>
 __global__ void k_surface2d(FDArgs<R> a, int pdir) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const Layout &g = a.g;
     if (x >= g.NX) return;
     const long i = g.idx(0, 0, x), sx = g.sx, nf = g.nfield;
     const int L = a.L, NAB = a.nab, NX = g.NX, NZ = g.NZ;
-    a.szz[i] = 0;
-    for (int l = 0; l < L; ++l) a.rzz[l * nf + i] = 0;
+    R *szz = a.szz, *sxz = a.sxz, *sxx = a.sxx, *rzz = a.rzz, *rxx = a.rxx;
+    putS<R, S>(szz, i, 0);
+    for (int l = 0; l < L; ++l) putS<R, S>(rzz, l * nf + i, 0);
     for (int m = 1; m <= H; ++m) {
-        a.szz[i - m] = -a.szz[i + m];
-        a.sxz[i - m] = -a.sxz[i + m - 1];
+        putS<R, S>(szz, i - m, -getS<R, S>(szz, i + m));
+        putS<R, S>(sxz, i - m, -getS<R, S>(sxz, i + m - 1));
     }
-    R vxx = Dm<H>(a.vx, i, sx, a.hc);
-    R vzz = Dm<H>(a.vz, i, 1, a.hc);
+    const unsigned os = (unsigned)(i * sizeof(S));
+    R vxx = DmS<H>((const S *)a.vx, os, sx, a.hc);
+    R vzz = DmS<H>((const S *)a.vz, os, 1, a.hc);
     if (a.abs_type == 1) {
         if (x < NAB) vxx = vxx / cp(a, 0, CK, x) + a.psi_vx_x[(long)x * NZ];
         if (x > NX - NAB - 1) {
@@ -311,14 +340,14 @@ __global__ void k_surface2d(FDArgs<R> a, int pdir) {
     if (L == 0) {
         const R f = a.mu[i] * (R)2, gg = a.M[i];
         const R h = -((gg - f) * (gg - f) * (vxx) / gg) - ((gg - f) * vzz);
-        a.sxx[i] += fp * h;
+        putS<R, S>(sxx, i, getS<R, S>(sxx, i) + fp * h);
     } else {
         const R f = a.cf[i], gg = a.cg[i], d = a.cd[i], e = a.ce[i], DT2 = a.DT2;
         R h = -((gg - f) * (gg - f) * (vxx) / gg) - ((gg - f) * vzz);
         R sump = 0;
-        for (int l = 0; l < L; ++l) sump += a.rxx[l * nf + i];
+        for (int l = 0; l < L; ++l) sump += getS<R, S>(rxx, l * nf + i);
         h += -DT2 * sump;
-        a.sxx[i] += fp * h;
+        R sxxv = getS<R, S>(sxx, i) + fp * h;
         sump = 0;
         for (int l = 0; l < L; ++l) {
             // b = eta/(1+eta/2) and the (f/g - 1.0) factor are doubles in
@@ -327,10 +356,11 @@ __global__ void k_surface2d(FDArgs<R> a, int pdir) {
             const R bb = (R)b;
             const R hh = (R)((double)bb * ((double)(d - e) * ((double)(f / gg) - 1.0) * (double)vxx -
                                           (double)((d - e) * vzz)));
-            a.rxx[l * nf + i] += fp * hh;
-            sump += a.rxx[l * nf + i];
+            const R r = getS<R, S>(rxx, l * nf + i) + fp * hh;
+            putS<R, S>(rxx, l * nf + i, r);
+            sump += getS<R, S>(rxx, l * nf + i);
         }
-        a.sxx[i] += fp * (DT2 * sump);
+        putS<R, S>(sxx, i, sxxv + fp * (DT2 * sump));
     }
 }
 
@@ -399,14 +429,6 @@ __global__ void k_surface3d(FDArgs<R> a, int pdir) {
 // One lane walks all sources in order, so sources sharing a cell add up in
 // a fixed order.
 // ---------------------------------------------------------------------------
-template <typename R, typename S>
-__device__ __forceinline__ R getS(const R *p, long i) {
-    return Store<R, S>::ld((const S *)p, (unsigned)(i * sizeof(S)));
-}
-template <typename R, typename S>
-__device__ __forceinline__ void putS(R *p, long i, R v) {
-    Store<R, S>::st((S *)p, (unsigned)(i * sizeof(S)), v);
-}
 
 // src holds each source already multiplied by the storage scale of the
 // field it feeds (1 unless wavefields are stored scaled).
@@ -532,10 +554,10 @@ __device__ __forceinline__ int batch_find(const unsigned *tile0, int n, unsigned
 // KIND 0: elastic velocity, 1: elastic stress, 2: acoustic velocity, 3: acoustic pressure
 template <int H, int KIND>
 __device__ __forceinline__ void batch_cell(const FDArgs<float> &a, int z, int x) {
-    if constexpr (KIND == 0) cell_v2d<float, H>(a, z, x);
-    else if constexpr (KIND == 1) cell_s2d<float, H, 0>(a, z, x);
-    else if constexpr (KIND == 2) cell_v2d_acc<float, H>(a, z, x);
-    else cell_s2d_acc<float, H, 0>(a, z, x);
+    if constexpr (KIND == 0) cell_v2d<float, float, H>(a, z, x);
+    else if constexpr (KIND == 1) cell_s2d<float, float, H, 0>(a, z, x);
+    else if constexpr (KIND == 2) cell_v2d_acc<float, float, H>(a, z, x);
+    else cell_s2d_acc<float, float, H, 0>(a, z, x);
 }
 
 // The tiles [lt0, lt0 + kBatchTiles) of one slot, in its z-fastest tile order.
@@ -605,7 +627,7 @@ __global__ void k_fill(uint32_t *p, size_t n) {
     for (; i < n; i += stride) p[i] = 0u;
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 void dispatch_v(const FDArgs<R> &a, int nd_case, hipStream_t st) {
     const Layout &g = a.g;
     const int nbz = (g.NZ + BZ - 1) / BZ;
@@ -614,13 +636,13 @@ void dispatch_v(const FDArgs<R> &a, int nd_case, hipStream_t st) {
         const int nbx = (g.NX + BT - 1) / BT;
         const unsigned nb = (unsigned)nbz * nbx;
         if (nd_case == 22)
-            hipLaunchKernelGGL((k_update_v2d_acc<R, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+            hipLaunchKernelGGL((k_update_v2d_acc<R, S, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
         else
-            hipLaunchKernelGGL((k_update_v2d<R, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+            hipLaunchKernelGGL((k_update_v2d<R, S, H>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
     }
 }
 
-template <typename R, int H, int L>
+template <typename R, typename S, int H, int L>
 void dispatch_s_L(const FDArgs<R> &a, int nd_case, hipStream_t st) {
     const Layout &g = a.g;
     const int nbz = (g.NZ + BZ - 1) / BZ;
@@ -629,48 +651,53 @@ void dispatch_s_L(const FDArgs<R> &a, int nd_case, hipStream_t st) {
         const int nbx = (g.NX + BT - 1) / BT;
         const unsigned nb = (unsigned)nbz * nbx;
         if (nd_case == 22)
-            hipLaunchKernelGGL((k_update_s2d_acc<R, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+            hipLaunchKernelGGL((k_update_s2d_acc<R, S, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
         else
-            hipLaunchKernelGGL((k_update_s2d<R, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
+            hipLaunchKernelGGL((k_update_s2d<R, S, H, L>), dim3(((nb + 7) / 8) * 8), blk, 0, st, a, nbz, nb);
     }
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 void dispatch_s(const FDArgs<R> &a, int nd_case, hipStream_t st) {
     switch (a.L) {
-        case 0: dispatch_s_L<R, H, 0>(a, nd_case, st); break;
-        case 1: dispatch_s_L<R, H, 1>(a, nd_case, st); break;
-        case 2: dispatch_s_L<R, H, 2>(a, nd_case, st); break;
-        case 3: dispatch_s_L<R, H, 3>(a, nd_case, st); break;
-        case 4: dispatch_s_L<R, H, 4>(a, nd_case, st); break;
+        case 0: dispatch_s_L<R, S, H, 0>(a, nd_case, st); break;
+        case 1: dispatch_s_L<R, S, H, 1>(a, nd_case, st); break;
+        case 2: dispatch_s_L<R, S, H, 2>(a, nd_case, st); break;
+        case 3: dispatch_s_L<R, S, H, 3>(a, nd_case, st); break;
+        case 4: dispatch_s_L<R, S, H, 4>(a, nd_case, st); break;
         default: break;   // rejected at model creation
     }
 }
 
-template <typename R, int H>
+template <typename R, typename S, int H>
 void dispatch_surface(const FDArgs<R> &a, int nd_case, int pdir, hipStream_t st) {
     const Layout &g = a.g;
-    if (nd_case == 3)
-        hipLaunchKernelGGL((k_surface3d<R, H>), dim3((g.NY + 63) / 64, g.NX), dim3(64), 0, st, a, pdir);
-    else
-        hipLaunchKernelGGL((k_surface2d<R, H>), dim3((g.NX + 63) / 64), dim3(64), 0, st, a, pdir);
+    if (nd_case == 3) {
+        // 3D runs with FP16 storage fuse the surface into the stress update
+        if constexpr (std::is_same<R, S>::value)
+            hipLaunchKernelGGL((k_surface3d<R, H>), dim3((g.NY + 63) / 64, g.NX), dim3(64), 0, st,
+                               a, pdir);
+    } else {
+        hipLaunchKernelGGL((k_surface2d<R, S, H>), dim3((g.NX + 63) / 64), dim3(64), 0, st, a,
+                           pdir);
+    }
 }
 
 }  // namespace
 
-template <typename R>
+template <typename R, typename S>
 void launch_update_v(const FDArgs<R> &a, int nd_case, void *stream) {
-    if (nd_case == 3) return launch_v3d<R, R>(a, stream);
-    SEIS_SWITCH_H(a.g.H, (dispatch_v<R, HH>(a, nd_case, (hipStream_t)stream)));
+    if (nd_case == 3) return launch_v3d<R, S>(a, stream);
+    SEIS_SWITCH_H(a.g.H, (dispatch_v<R, S, HH>(a, nd_case, (hipStream_t)stream)));
 }
-template <typename R>
+template <typename R, typename S>
 void launch_update_s(const FDArgs<R> &a, int nd_case, void *stream) {
-    if (nd_case == 3) return launch_s3d<R, R>(a, stream);
-    SEIS_SWITCH_H(a.g.H, (dispatch_s<R, HH>(a, nd_case, (hipStream_t)stream)));
+    if (nd_case == 3) return launch_s3d<R, S>(a, stream);
+    SEIS_SWITCH_H(a.g.H, (dispatch_s<R, S, HH>(a, nd_case, (hipStream_t)stream)));
 }
-template <typename R>
+template <typename R, typename S>
 void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream) {
-    SEIS_SWITCH_H(a.g.H, (dispatch_surface<R, HH>(a, nd_case, pdir, (hipStream_t)stream)));
+    SEIS_SWITCH_H(a.g.H, (dispatch_surface<R, S, HH>(a, nd_case, pdir, (hipStream_t)stream)));
 }
 template <typename R, typename S>
 void launch_sources(const FDArgs<R> &a, int nd_case, const ShotDev &s, int NT, int nt,
@@ -738,12 +765,15 @@ void launch_fill(void *ptr, size_t bytes, void *stream) {
     hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint32_t *)ptr, n);
 }
 
-template void launch_update_v<float>(const FDArgs<float> &, int, void *);
-template void launch_update_s<float>(const FDArgs<float> &, int, void *);
-template void launch_surface<float>(const FDArgs<float> &, int, int, void *);
-template void launch_update_v<double>(const FDArgs<double> &, int, void *);
-template void launch_update_s<double>(const FDArgs<double> &, int, void *);
-template void launch_surface<double>(const FDArgs<double> &, int, int, void *);
+template void launch_update_v<float, float>(const FDArgs<float> &, int, void *);
+template void launch_update_s<float, float>(const FDArgs<float> &, int, void *);
+template void launch_surface<float, float>(const FDArgs<float> &, int, int, void *);
+template void launch_update_v<float, __half>(const FDArgs<float> &, int, void *);
+template void launch_update_s<float, __half>(const FDArgs<float> &, int, void *);
+template void launch_surface<float, __half>(const FDArgs<float> &, int, int, void *);
+template void launch_update_v<double, double>(const FDArgs<double> &, int, void *);
+template void launch_update_s<double, double>(const FDArgs<double> &, int, void *);
+template void launch_surface<double, double>(const FDArgs<double> &, int, int, void *);
 template void launch_sources<float, float>(const FDArgs<float> &, int, const ShotDev &, int, int,
                                           int, void *, const int *);
 template void launch_sources<float, __half>(const FDArgs<float> &, int, const ShotDev &, int, int,

@@ -84,6 +84,14 @@ float relax_sum(const seis_csts &c);
 // The staggered shear pair: f_ij = mu_ij (1 + L tau_ij), d_ij = mu_ij tau_ij / DT.
 void visco_pair(const std::vector<float> &mu, const std::vector<float> &ts, int L, float DT,
                 std::vector<float> &ff, std::vector<float> &dd);
+// Gradients through the staggered averages of transform_params (gradfix
+// FIX_AVG), unpadded layout.  sh: per-node shear products (xz, xy, yz), rh:
+// per-node density terms (vx, vy, vz nodes); gmu gets the chained shear terms
+// added (with the moduli mus of the gradient formulas), grho is replaced by
+// the chained density terms.
+void chain_staggered(const seis_csts &c, const RefParams &p, const std::vector<float> &mus,
+                     const std::vector<double> *sh, const std::vector<double> *rh, double *gmu,
+                     double *grho);
 
 // Output names of seisout (SeisCL.py to_load_names order).
 std::vector<std::string> out_names(const seis_csts &c);
@@ -132,12 +140,15 @@ struct ShotDev {
     float *out = nullptr;      // [NT][nout][nrec] (time-major on the device)
 };
 
-// Launchers implemented in the .hip kernel files.
-template <typename R>
+// Launchers implemented in the .hip kernel files.  S: storage type of the
+// wavefields and memory variables (R, or __half with R = float).  With FP16
+// storage, 3D runs fuse the free surface into the stress update and
+// launch_surface does nothing for them.
+template <typename R, typename S = R>
 void launch_update_v(const FDArgs<R> &a, int nd_case, void *stream);
-template <typename R>
+template <typename R, typename S = R>
 void launch_update_s(const FDArgs<R> &a, int nd_case, void *stream);
-template <typename R>
+template <typename R, typename S = R>
 void launch_surface(const FDArgs<R> &a, int nd_case, int pdir, void *stream);
 // S: storage type of wavefields and memory variables (float, __half or
 // double); R: compute and coefficient type.
@@ -212,7 +223,7 @@ struct CkptPlan {
 // Corrections of reference defects, selected by seis_csts.gradfix.
 enum GradFix {
     FIX_RES_SCALE = 1, FIX_ADJ_CPML = 2, FIX_RHO_SCALE = 4, FIX_SURF_SYY = 8, FIX_VISCO_F0 = 16,
-    FIX_RMSNORM = 64, FIX_HESS = 128
+    FIX_AVG = 32, FIX_RMSNORM = 64, FIX_HESS = 128
 };
 
 // Boundary shell saved during the forward run and re-injected while the
@@ -350,6 +361,15 @@ struct AdjArgs {
     // relaxation sum s0 of the transform (relax_sum)
     double *gtp = nullptr, *gts = nullptr;
     R s0 = 0;
+    // gradfix FIX_AVG: the terms of parameters that reach the update through
+    // staggered averages, kept per staggered node for the host to chain
+    // through the averages (chain_staggered): the shear-stress products
+    // sigma_ij times its adjoint increment (xz, xy, yz; not divided by mu^2)
+    // and the density terms of vy and vz (vx's stays in grho), scaled like
+    // the accumulators they leave.  Null: the reference's attribution of
+    // every term to the cell of the same index.
+    double *gsh[3] = {nullptr, nullptr, nullptr};
+    double *gry = nullptr, *grz = nullptr;
 };
 
 template <typename R>

@@ -496,4 +496,103 @@ std::vector<std::string> out_names(const seis_csts &c) {
     return {};
 }
 
+
+// ---------------------------------------------------------------------------
+// Gradients through the staggered averages (gradfix FIX_AVG): transposes of
+// avg2 and avg4, dJ/dp_c = sum over the staggered nodes s that cell c enters
+// of dJ/dp_s dp_s/dp_c, with the node classes of those functions: averaged
+// nodes, nodes copied from their own cell, and (avg4's second edge) nodes
+// left at zero, which depend on no cell.
+// ---------------------------------------------------------------------------
+namespace {
+// 1 where avg4 copies the node from its cell (its edge loops)
+std::vector<char> avg4_copied(int NZ, int NY, int NX, const int d[2][3]) {
+    std::vector<char> cp((size_t)NZ * NY * NX, 0);
+    int x0 = 0, y0 = 0, z0 = 0;
+    for (int e = 0; e < 2; ++e) {
+        if (d[e][2]) x0 = NX - 1;
+        else if (d[e][1]) y0 = NY - 1;
+        if (d[e][0]) z0 = NZ - 1;
+        for (int x = x0; x < NX; ++x)
+            for (int y = y0; y < NY; ++y)
+                for (int z = z0; z < NZ; ++z) cp[lin(z, y, x, NZ, NY)] = 1;
+    }
+    return cp;
+}
+}  // namespace
+
+void chain_density(const std::vector<double> &G, const std::vector<float> &b,
+                   const std::vector<float> &bs, int NZ, int NY, int NX, int dz, int dy, int dx,
+                   double *out) {
+    // bs = avg2(b) along (dz, dy, dx); rho_s = k / bs, so
+    // drho_s/drho_c = (1/2) (b_c / bs_s)^2 for an averaged node, 1 for a copy
+    for (int x = 0; x < NX; ++x)
+        for (int y = 0; y < NY; ++y)
+            for (int z = 0; z < NZ; ++z) {
+                const long s = lin(z, y, x, NZ, NY);
+                const bool avg = x < NX - dx && y < NY - dy && z < NZ - dz;
+                if (!avg) {
+                    out[s] += G[s];
+                    continue;
+                }
+                const long s2 = lin(z + dz, y + dy, x + dx, NZ, NY);
+                const double q = (double)bs[s];
+                if (q == 0) continue;
+                const double r1 = (double)b[s] / q, r2 = (double)b[s2] / q;
+                out[s] += 0.5 * r1 * r1 * G[s];
+                out[s2] += 0.5 * r2 * r2 * G[s];
+            }
+}
+
+void chain_shear(const std::vector<double> &P, const std::vector<float> &mus, int NZ, int NY,
+                 int NX, const int d[2][3], double *out) {
+    // P_s: the shear stress times its adjoint increment, summed over time;
+    // dJ/dmu_s = -P_s / mu_s^2 and the harmonic mean gives dmu_s/dmu_c =
+    // mu_s^2 / (4 mu_c^2): -P_s / (4 mu_c^2) per cell of an averaged node,
+    // -P_s / mu_c^2 for a copied one (the gradient formulas' own moduli mus)
+    const std::vector<char> cp = avg4_copied(NZ, NY, NX, d);
+    const int ez = d[0][0] + d[1][0], ey = d[0][1] + d[1][1], ex = d[0][2] + d[1][2];
+    auto add = [&](long c, double v) {
+        const double m = mus[c];
+        if (m != 0) out[c] += -v / (m * m);
+    };
+    for (int x = 0; x < NX; ++x)
+        for (int y = 0; y < NY; ++y)
+            for (int z = 0; z < NZ; ++z) {
+                const long s = lin(z, y, x, NZ, NY);
+                if (cp[s]) {
+                    add(s, P[s]);
+                    continue;
+                }
+                if (!(x < NX - ex && y < NY - ey && z < NZ - ez)) continue;   // left at zero
+                const double q = 0.25 * P[s];
+                add(s, q);
+                add(lin(z + d[0][0], y + d[0][1], x + d[0][2], NZ, NY), q);
+                add(lin(z + d[1][0], y + d[1][1], x + d[1][2], NZ, NY), q);
+                add(lin(z + ez, y + ey, x + ex, NZ, NY), q);
+            }
+}
+
+void chain_staggered(const seis_csts &c, const RefParams &p, const std::vector<float> &mus,
+                     const std::vector<double> *sh, const std::vector<double> *rh, double *gmu,
+                     double *grho) {
+    const int NZ = c.N[0], NY = c.ND == 3 ? c.N[1] : 1, NX = c.ND == 3 ? c.N[2] : c.N[1];
+    const size_t n = (size_t)NZ * NY * NX;
+    // density: rh[0] vx (x nodes), rh[1] vy (y), rh[2] vz (z); grho is
+    // rebuilt from them
+    std::fill(grho, grho + n, 0.0);
+    chain_density(rh[0], p.rho, p.rip, NZ, NY, NX, 0, 0, 1, grho);
+    if (c.ND == 3) chain_density(rh[1], p.rho, p.rjp, NZ, NY, NX, 0, 1, 0, grho);
+    chain_density(rh[2], p.rho, p.rkp, NZ, NY, NX, 1, 0, 0, grho);
+    // shear: sh[0] xz, sh[1] xy, sh[2] yz, added to the normal-stress part
+    const int dxz[2][3] = {{0, 0, 1}, {1, 0, 0}};
+    const int dxy[2][3] = {{0, 0, 1}, {0, 1, 0}};
+    const int dyz[2][3] = {{0, 1, 0}, {1, 0, 0}};
+    chain_shear(sh[0], mus, NZ, NY, NX, dxz, gmu);
+    if (c.ND == 3) {
+        chain_shear(sh[1], mus, NZ, NY, NX, dxy, gmu);
+        chain_shear(sh[2], mus, NZ, NY, NX, dyz, gmu);
+    }
+}
+
 }  // namespace seis

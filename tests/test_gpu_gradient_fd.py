@@ -16,13 +16,17 @@ Finite differences: central, h = 1e-3 of the cell's value, of the misfit
 J = 1/2 sum (out - d)^2 from FP64 runs of the same kernels (receivers
 recording in double), so the difference quotient carries no FP32 rounding.
 The gradient (FP32 fields, double accumulation) is the reference's
-correlation formula, not the exact derivative of the discrete scheme:
-vp enters one cell's modulus, so it matches cell by cell; vs and rho also
-enter staggered averages over neighbouring cells (harmonic mu, arithmetic
-buoyancy), which the formula attributes to the cell itself.  Tolerances,
-as relative L2 error over the checked cells: vp 3%, vs 10%, rho 30%
-(a CPU run of the oracle on a 100 x 200 version gave about 0.5%, 5% and
-20%).
+correlation formula.  vp enters one cell's modulus, so it matches cell by
+cell; vs and rho also enter staggered averages over neighbouring cells
+(harmonic mu, arithmetic buoyancy).  With gradfix 223 (the default) the
+formula attributes those terms to the cell of the same index, as the
+reference does: tolerances, as relative L2 error over the checked cells, vp
+3%, vs 10%, rho 30% (a CPU run of the oracle on a 100 x 200 version gave
+about 0.5%, 5% and 20%; MI355X: under 1e-4, 3.4% and 20.8%).  gradfix
+255 adds bit 32, which keeps those terms per staggered node and chains them
+through the averages onto the cells: the gradient is then the derivative of
+the discrete misfit, and all three agree to 1e-3 on 20 cells each (MI355X:
+under 1e-4 for all three).
 """
 import numpy as np
 import pytest
@@ -32,8 +36,9 @@ from cases import rel_l2
 pytestmark = pytest.mark.gpu
 
 NT = 3000
-TOL = {"vp": 0.03, "vs": 0.10, "rho": 0.30}
-NCELLS = {"vp": 20, "vs": 10, "rho": 10}
+# gradfix -> (tolerance, cells checked) per parameter
+MODES = {223: ({"vp": 0.03, "vs": 0.10, "rho": 0.30}, {"vp": 20, "vs": 10, "rho": 10}),
+         255: ({"vp": 1e-3, "vs": 1e-3, "rho": 1e-3}, {"vp": 20, "vs": 20, "rho": 20})}
 
 
 def _case(nt=NT):
@@ -87,12 +92,16 @@ def test_config3_gradient_matches_oracle(oracle_lib, gpu_ctx):
         assert rel_l2(g[k], ref["grad"][k]) <= 1e-5, k
 
 
-def test_config3_gradient_finite_differences(gpu_ctx):
+@pytest.mark.parametrize("gradfix", sorted(MODES))
+def test_config3_gradient_finite_differences(gpu_ctx, gradfix):
     import torch
     from seiscl_amd import Model
     from seiscl_amd.autograd import Propagator, seismic
+    TOL, NCELLS = MODES[gradfix]
     csts, true, start, shots = _case()
+    print(f"[grad fd] gradfix={gradfix}", flush=True)
     din = _observed(gpu_ctx, true, csts, shots)
+    csts = dict(csts, gradfix=gradfix)
     prop = Propagator(gpu_ctx, start, **csts)
     params = [torch.tensor(start[n], requires_grad=True) for n in prop.param_names]
     outs = seismic(prop, shots, *params)
@@ -122,6 +131,6 @@ def test_config3_gradient_finite_differences(gpu_ctx):
             print(f"{name} ({iz},{ix}): fd {fd[-1]:.5e} grad {gr[-1]:.5e}", flush=True)
         fd, gr = np.array(fd), np.array(gr)
         err = float(np.linalg.norm(gr - fd) / np.linalg.norm(fd))
-        print(f"{name}: relative L2 error over {len(fd)} cells {err:.4f}", flush=True)
+        print(f"{name}: relative L2 error over {len(fd)} cells {err:.3e}", flush=True)
         assert err <= TOL[name], (name, err)
     m64.close()

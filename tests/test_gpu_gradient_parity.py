@@ -200,12 +200,14 @@ def test_gradient_accumulates_over_shots(oracle_lib, gpu_ctx):
         assert rel_l2(g[k], tot) <= 1e-5, k
 
 
-# ---- viscoelastic gradients with scaled or FP16 wavefield storage (3D) and
-# ---- the checkpoint plans
+# ---- viscoelastic gradients with scaled or FP16 wavefield storage and the
+# ---- checkpoint plans
 
 VISCO_SCALED = [dict(ND=3, FDORDER=4, L=2, abs_type=1),
                 dict(ND=3, FDORDER=4, L=2, abs_type=1, freesurf=1),
-                dict(ND=3, FDORDER=4, L=1, abs_type=2, freesurf=1)]
+                dict(ND=3, FDORDER=4, L=1, abs_type=2, freesurf=1),
+                dict(ND=2, FDORDER=4, L=2, abs_type=1),
+                dict(ND=2, FDORDER=8, L=1, abs_type=1, freesurf=1)]
 
 
 def _gpu_grad(gpu_ctx, case, FP16=0, mem=0):
@@ -353,3 +355,35 @@ def test_rtm_and_hessian_match_oracle(oracle_lib, gpu_ctx, case):
         for k in ref["hess"]:
             np.testing.assert_array_equal(h[k], ref["hess"][k])
         assert np.abs(ref["hess"]["vp"]).max() > 0
+
+
+@pytest.mark.parametrize("case", [dict(ND=2, FDORDER=4), dict(ND=3, FDORDER=4, freesurf=1),
+                                  dict(ND=2, FDORDER=4, L=1, abs_type=1),
+                                  dict(ND=3, FDORDER=4, L=2, abs_type=1)], ids=_ids)
+def test_staggered_chain_conserves_totals(gpu_ctx, case):
+    """gradfix bit 32 moves the shear and density terms from the cell of
+    the same index to the cells each staggered node averages over, with
+    weights that sum to one over a node's cells when the model is
+    homogeneous: on a homogeneous starting model the totals of the vs and
+    rho gradients equal those of the reference's attribution (to rounding)
+    while the cell values differ, and the vp gradient, which enters no
+    staggered average, is the same bit for bit."""
+    from seiscl_amd import Model
+    csts, params, (sp, s, rp), true = _setup(dict(case, gradfix=223))
+    homog = {k: np.full_like(v, float(np.median(v))) for k, v in params.items()}
+    m = Model(gpu_ctx, true, **csts)
+    din = m.forward(sp, s, rp)
+    m.close()
+    g = {}
+    for gf in (223, 255):
+        print(f"[grad chain] case={_ids(case)} gradfix={gf}", flush=True)
+        m = Model(gpu_ctx, homog, **dict(csts, gradfix=gf))
+        m.grad_zero()
+        m.gradient(sp, s, rp, din=din)
+        g[gf] = m.grad_read()
+        m.close()
+    np.testing.assert_array_equal(g[255]["vp"], g[223]["vp"])
+    for k in ("vs", "rho"):
+        a, b = g[255][k].astype(np.float64), g[223][k].astype(np.float64)
+        assert abs(a.sum() - b.sum()) <= 1e-5 * np.abs(b).sum(), k
+        assert rel_l2(a, b) > 1e-3, k

@@ -1,4 +1,5 @@
-"""3D precision modes of the GPU kernels (seis_csts.FP16).
+"""Precision modes of the GPU kernels (seis_csts.FP16), 3D, 2D P-SV and 2D
+acoustic.
 
   FP16=1  FP32 storage in the scaled units used for FP16: the scaling is by
           powers of two, so results must equal the FP32 oracle (up to
@@ -19,7 +20,11 @@ pytestmark = pytest.mark.gpu
 CASES = ([dict(ND=3, FDORDER=o, abs_type=a, freesurf=f, L=l)
           for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
          # several z tiles tall
-         + [dict(ND=3, FDORDER=4, abs_type=1, freesurf=1, L=2, N=(150, 28, 30))])
+         + [dict(ND=3, FDORDER=4, abs_type=1, freesurf=1, L=2, N=(150, 28, 30))]
+         # 2D P-SV (src/update_{v,s}2D_half2.cl) and 2D acoustic
+         + [dict(ND=2, FDORDER=o, abs_type=a, freesurf=f, L=l)
+            for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
+         + [dict(ND=22, FDORDER=o, abs_type=a, L=l) for o, a, l in ((4, 1, 0), (8, 2, 1))])
 
 
 def _ids(c):
@@ -37,10 +42,11 @@ def _run(gpu_ctx, csts, params, shot, fp16, fields=()):
 
 @pytest.mark.parametrize("case", CASES, ids=_ids)
 def test_scaled_units_equal_fp32(oracle_lib, gpu_ctx, case):
-    csts, params, shot = small_case(NT=100, **case)
+    csts, params, shot = small_case(NT=100 if case["ND"] == 3 else 150, **case)
     print(f"[precision] scaled case={_ids(case)}", flush=True)
     ref, rf = oracle_lib.forward(params, *shot, fields=True, **csts)
-    out, f = _run(gpu_ctx, csts, params, shot, 1, fields=("vx", "szz"))
+    fields = ("vx", "p") if case["ND"] == 22 else ("vx", "szz")
+    out, f = _run(gpu_ctx, csts, params, shot, 1, fields=fields)
     for r, o in zip(ref, out):
         assert rel_l2(o, r) <= 1e-6
     for k in f:
@@ -49,7 +55,7 @@ def test_scaled_units_equal_fp32(oracle_lib, gpu_ctx, case):
 
 @pytest.mark.parametrize("case", CASES, ids=_ids)
 def test_fp32_and_fp16_against_fp64(gpu_ctx, case):
-    csts, params, shot = small_case(NT=120, **case)
+    csts, params, shot = small_case(NT=120 if case["ND"] == 3 else 150, **case)
     print(f"[precision] fp64/fp16 case={_ids(case)}", flush=True)
     o64, _ = _run(gpu_ctx, csts, params, shot, 64)
     o32, _ = _run(gpu_ctx, csts, params, shot, 0)
