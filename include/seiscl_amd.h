@@ -306,6 +306,21 @@ int seis_grad_read_device(seis_model *m, float *dgrad, float *dgtau);
 /* [3][Pi N] in the same order and transform (H<name> of the gradient file,  */
 /* src/writehdf5.c:275-283, src/calc_grad.c:1028-1094).                      */
 int seis_hess_read(seis_model *m, float *hess);
+/* DFT-domain gradient (back_prop_type 2; src/calc_grad.c:377-968, the     */
+/* savefreqs kernel src/automatic_kernels.c:1030-1128, sampling             */
+/* src/assign_modeling_case.c:549-570, src/time_stepping.c:658-672,767-780, */
+/* 869-965): the model's next gradient runs keep the DFT of every wavefield */
+/* and memory variable at the nfreqs frequencies freqs (Hz, gradfreqs)     */
+/* every DTNYQ = ceil(0.0156/(fmax dt)) steps of the forward run, propagate */
+/* the adjoint wavefield alone and keep its DFT, and form the gradient of  */
+/* the parameterisation (vp|M|Ip, vs|mu|Is, rho, and taup, taus for L>0)    */
+/* from the two sets of spectra; seis_grad_read / seis_grad_read_tau then  */
+/* return it as accumulated (no transf_grad), and seis_residuals /         */
+/* seis_gradient_shot report the rms at these frequencies                 */
+/* (src/residuals.c:364-403).  nfreqs = 0 returns to backpropagation; zero */
+/* the accumulators (seis_grad_zero) after switching.  FP32 models, ND 2   */
+/* or 3, L2 misfit without trace scaling, no Hessian, no RTM.              */
+int seis_grad_dft(seis_model *m, int nfreqs, const float *freqs);
 /* One shot of gradient: forward with boundary saving, residuals against    */
 /* din (or the given adjoint sources res_in, the inputres mode), res_scale, */
 /* adjoint.  out: forward seismograms (may be NULL); rms: [rms, rmsnorm].   */

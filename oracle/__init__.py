@@ -83,6 +83,18 @@ def load():
                                      ctypes.c_int, _FP, _FP, _FP,
                                      ctypes.POINTER(ctypes.c_double), _FP, _FP, _FP,
                                      ctypes.c_int, _FP, ctypes.c_char_p, ctypes.c_int])
+        _D = ctypes.POINTER(ctypes.c_double)
+        lib.or_gradient_dft.restype = ctypes.c_int
+        lib.or_gradient_dft.argtypes = ([ctypes.POINTER(OrCsts)] + [_FP] * 5 +
+                                        [ctypes.c_int, _FP, _FP, ctypes.c_int, _FP, _FP, _FP,
+                                         ctypes.c_int, ctypes.c_int, _FP, _FP, _D, _D, _FP,
+                                         ctypes.c_char_p, ctypes.c_int])
+        lib.or_dft_sampling.restype = ctypes.c_int
+        lib.or_dft_sampling.argtypes = [ctypes.POINTER(OrCsts), ctypes.c_int, _FP,
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int), _FP, _FP]
+        lib.or_dft_coef.restype = None
+        lib.or_dft_coef.argtypes = [_D, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 8
         lib.or_shell_check.restype = ctypes.c_int
         lib.or_shell_check.argtypes = [ctypes.POINTER(OrCsts), ctypes.POINTER(ctypes.c_long)]
         lib.or_nvars.argtypes = [ctypes.POINTER(OrCsts)]
@@ -370,3 +382,80 @@ def shell_check(**csts):
     n = ctypes.c_long()
     bad = lib.or_shell_check(ctypes.byref(c), ctypes.byref(n))
     return int(bad), int(n.value)
+
+
+def dft_sampling(freqs, **csts):
+    """(DTNYQ, NTNYQ, df, bins) of the DFT gradient (or_dft_sampling)."""
+    lib = load()
+    c = make_csts(**csts)
+    f = np.ascontiguousarray(np.asarray(freqs, np.float32).ravel())
+    bins = np.zeros(f.size, np.float32)
+    dt, nt, df = ctypes.c_int(), ctypes.c_int(), np.zeros(1, np.float32)
+    st = lib.or_dft_sampling(ctypes.byref(c), f.size, fp(f), ctypes.byref(dt), ctypes.byref(nt),
+                             fp(df), fp(bins))
+    if st == 1:
+        raise RuntimeError("oracle: bad gradfreqs")
+    return dt.value, nt.value, float(df[0]), bins
+
+
+def dft_coef(pt, visco, M, mu, taup, taus, rho, ND, L, al):
+    """The 24 scalar-product coefficients of one cell (or_dft_coef)."""
+    c = (ctypes.c_double * 24)()
+    load().or_dft_coef(c, int(pt), int(visco), M, mu, taup, taus, rho, ND, L, al)
+    return np.array(c[:])
+
+
+def gradient_dft(params: dict, src_pos, src, rec_pos, freqs, din=None, res=None,
+                 gradfix=FIX_ALL, nthreads: int = 0, spectra: bool = False, **csts):
+    """Oracle DFT-domain gradient of one shot (back_prop_type 2).
+
+    Returns dict(out=[(NT,nrec)...], grad={p0name, p1name, 'rho'[, 'taup',
+    'taus']} in model layout (double), rms=(rms, rmsnorm) at the gradient
+    frequencies) and, with spectra=True, spectra: complex array
+    [nq, nf, *padded N] of the forward DFTs (or_gradient_dft)."""
+    lib = load()
+    c = make_csts(nthreads=nthreads, **csts)
+    p0, p1, rho, taup, taus = _params(params, c)
+    sp = np.ascontiguousarray(np.asarray(src_pos, np.float32).T)
+    s = np.ascontiguousarray(np.asarray(src, np.float32).T)
+    rp = np.ascontiguousarray(np.asarray(rec_pos, np.float32).T)
+    nout = lib.or_nout(ctypes.byref(c))
+    nrec = rp.shape[0]
+
+    def pack(lst):
+        if lst is None:
+            return None
+        a = np.stack([np.asarray(x, np.float32).T for x in lst])
+        return np.ascontiguousarray(a.reshape(nout, nrec, c.NT))
+
+    f = np.ascontiguousarray(np.asarray(freqs, np.float32).ravel())
+    out = np.zeros((nout, nrec, c.NT), np.float32)
+    shape = [int(v) for v in np.atleast_1d(csts["N"])]
+    n = int(np.prod(shape))
+    grad = np.zeros(5 * n, np.float64)
+    rms = (ctypes.c_double * 2)()
+    H = c.FDORDER // 2
+    pshape = [v + 2 * H for v in shape]
+    nv = int(np.prod(pshape))
+    nq = (9 + 6 * c.L) if c.ND == 3 else (5 + 3 * c.L)
+    so = np.zeros(nq * f.size * nv * 2, np.float32) if spectra else None
+    err = ctypes.create_string_buffer(256)
+    st = lib.or_gradient_dft(ctypes.byref(c), fp(p0), fp(p1), fp(rho), fp(taup), fp(taus),
+                             sp.shape[0], fp(sp), fp(s), nrec, fp(rp), fp(pack(din)),
+                             fp(pack(res)), int(gradfix), f.size, fp(f), fp(out),
+                             grad.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), rms,
+                             fp(so), err, 256)
+    if st:
+        raise RuntimeError("oracle: " + err.value.decode())
+    names = {0: ("vp", "vs"), 1: ("M", "mu"), 2: ("Ip", "Is"), 3: ("vp", "vs")}[c.param_type]
+    keys = [names[0], names[1], "rho"] + (["taup", "taus"] if c.L > 0 else [])
+    g = {k: np.ascontiguousarray(grad[q * n:(q + 1) * n].reshape(shape[::-1]).T)
+         for q, k in enumerate(keys)}
+    result = dict(out=[np.ascontiguousarray(o.T) for o in out], grad=g, rms=(rms[0], rms[1]))
+    if spectra:
+        z = so.reshape(nq, f.size, nv, 2)
+        z = z[..., 0] + 1j * z[..., 1]
+        result["spectra"] = np.moveaxis(z.reshape([nq, f.size] + pshape[::-1]),
+                                        list(range(2, 2 + len(shape))),
+                                        list(range(2 + len(shape) - 1, 1, -1)))
+    return result

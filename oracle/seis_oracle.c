@@ -2721,6 +2721,25 @@ static void adj_free(sim *A) {
     sim_free(A);
 }
 
+/* The reference's adjoint kernels update the forward CPML memory, which   */
+/* still holds the end of the forward run: the adjoint starts from a copy.  */
+static void adj_share_cpml(sim *A, const sim *S) {
+    float *const *fw[] = {&S->psi_sxx_x, &S->psi_sxy_x, &S->psi_sxz_x, &S->psi_sxy_y,
+                          &S->psi_syy_y, &S->psi_syz_y, &S->psi_sxz_z, &S->psi_syz_z,
+                          &S->psi_szz_z, &S->psi_vx_x,  &S->psi_vy_x,  &S->psi_vz_x,
+                          &S->psi_vx_y,  &S->psi_vy_y,  &S->psi_vz_y,  &S->psi_vx_z,
+                          &S->psi_vy_z,  &S->psi_vz_z};
+    float **aw[] = {&A->psi_sxx_x, &A->psi_sxy_x, &A->psi_sxz_x, &A->psi_sxy_y, &A->psi_syy_y,
+                    &A->psi_syz_y, &A->psi_sxz_z, &A->psi_syz_z, &A->psi_szz_z, &A->psi_vx_x,
+                    &A->psi_vy_x,  &A->psi_vz_x,  &A->psi_vx_y,  &A->psi_vy_y,  &A->psi_vz_y,
+                    &A->psi_vx_z,  &A->psi_vy_z,  &A->psi_vz_z};
+    long nab2 = 2 * S->NAB, nyi = S->ND == 3 ? S->NY : 1;
+    long sizes[3] = {nab2 * nyi * S->NZ, (long)S->NX * nab2 * S->NZ, (long)S->NX * nyi * nab2};
+    const int axis[18] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 1, 1, 1, 2, 2, 2};
+    for (int q = 0; q < 18; q++)
+        if (*fw[q] && *aw[q]) memcpy(*aw[q], *fw[q], sizeof(float) * sizes[axis[q]]);
+}
+
 /* Movie frame of a gradient run (src/time_stepping.c:926-929, frame index
  * of :317-320): every variable, padded, into frame (t+1)/movout - 1 when
  * t % movout == 0. */
@@ -2856,23 +2875,7 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
         sim_free(&S);
         return 1;
     }
-    if (!(gradfix & OR_FIX_ADJ_CPML)) {
-        /* the reference's adjoint kernels update the forward CPML memory,   */
-        /* which still holds the end of the forward run                      */
-        float **fw[] = {&S.psi_sxx_x, &S.psi_sxy_x, &S.psi_sxz_x, &S.psi_sxy_y, &S.psi_syy_y,
-                        &S.psi_syz_y, &S.psi_sxz_z, &S.psi_syz_z, &S.psi_szz_z, &S.psi_vx_x,
-                        &S.psi_vy_x, &S.psi_vz_x, &S.psi_vx_y, &S.psi_vy_y, &S.psi_vz_y,
-                        &S.psi_vx_z, &S.psi_vy_z, &S.psi_vz_z};
-        float **aw[] = {&A.psi_sxx_x, &A.psi_sxy_x, &A.psi_sxz_x, &A.psi_sxy_y, &A.psi_syy_y,
-                        &A.psi_syz_y, &A.psi_sxz_z, &A.psi_syz_z, &A.psi_szz_z, &A.psi_vx_x,
-                        &A.psi_vy_x, &A.psi_vz_x, &A.psi_vx_y, &A.psi_vy_y, &A.psi_vz_y,
-                        &A.psi_vx_z, &A.psi_vy_z, &A.psi_vz_z};
-        long nab2 = 2 * S.NAB, nyi = S.ND == 3 ? S.NY : 1;
-        long sizes[3] = {nab2 * nyi * S.NZ, (long)S.NX * nab2 * S.NZ, (long)S.NX * nyi * nab2};
-        const int axis[18] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 1, 1, 1, 2, 2, 2};
-        for (int q = 0; q < 18; q++)
-            if (*fw[q] && *aw[q]) memcpy(*aw[q], *fw[q], sizeof(float) * sizes[axis[q]]);
-    }
+    if (!(gradfix & OR_FIX_ADJ_CPML)) adj_share_cpml(&A, &S);
     /* Moduli of the gradient formulas.  The reference uses the relaxed     */
     /* moduli M, mu for the stress-to-strain conversion and the chain rule   */
     /* (src/update_adjs2D.cl:441-451, src/calc_grad.c:1020-1056); for a       */
@@ -2971,6 +2974,521 @@ int or_gradient(const or_csts *c, const float *p0, const float *p1, const float 
     free(muf);
     free(bnd);
     free(hist);
+    free(res);
+    adj_free(&A);
+    sim_free(&S);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* DFT-domain gradient (BACK_PROP_TYPE=2).                                   */
+/*   sampling: src/assign_modeling_case.c:549-570 (gradfreqsn);               */
+/*   forward: the DFT of every wavefield and memory variable before the      */
+/*     sources of each sampled step (src/time_stepping.c:767-780, savefreqs  */
+/*     src/automatic_kernels.c:1030-1128);                                    */
+/*   adjoint: the adjoint wavefield alone (no backpropagation), its DFT after */
+/*     the updates of each sampled step (src/time_stepping.c:869-925);        */
+/*   gradient: the correlation of the two spectra per frequency with the     */
+/*     coefficients of the parameterisation (src/calc_grad.c:111-376,        */
+/*     377-968); no transf_grad (:982-1002 runs for BACK_PROP_TYPE=1 only);  */
+/*   misfit: rms at the gradient frequencies (src/residuals.c:364-403).       */
+/* ------------------------------------------------------------------------ */
+int or_dft_sampling(const or_csts *c, int nf, const float *freqs, int *dtnyq, int *ntnyq,
+                    float *df, float *bins) {
+    float fmax = 0;
+    for (int f = 0; f < nf; f++)
+        if (freqs[f] > fmax) fmax = freqs[f];
+    if (nf <= 0 || fmax <= 0) return 1;
+    const int tmax = c->tmax > 0 ? c->tmax : c->NT;
+    *dtnyq = (int)ceil(0.0156 / fmax / c->dt);
+    *ntnyq = (tmax - c->tmin) / *dtnyq + 1;
+    *df = (float)(1.0 / *ntnyq / c->dt / *dtnyq);
+    for (int f = 0; f < nf; f++) {
+        bins[f] = (float)floor(freqs[f] / *df);
+        /* bin 0 (below df) would divide the memory-variable terms by 0 */
+        if (bins[f] < 1) return 2;
+    }
+    return 0;
+}
+
+/* factor of sample thist at every frequency: dtnyq dt e^{-i pi 2 bin thist / ntnyq} */
+static void dft_fact(const or_csts *c, int nf, const float *bins, int dtnyq, int ntnyq,
+                     int thist, float *fact) {
+    const float s = (float)dtnyq * c->dt;
+    for (int f = 0; f < nf; f++) {
+        const double a = 2.0 * bins[f] * thist / ntnyq;
+        fact[2 * f] = (float)(s * cos(PI * a));
+        fact[2 * f + 1] = (float)(-s * sin(PI * a));
+    }
+}
+
+/* the arrays a DFT covers, in order: velocities and stresses, then each  */
+/* memory-variable component's L slabs                                      */
+static int dft_arrays(sim *S, float **q) {
+    int n = 0;
+    if (S->ND == 3) {
+        float *f[9] = {S->vx, S->vy, S->vz, S->sxx, S->syy, S->szz, S->sxy, S->syz, S->sxz};
+        for (int k = 0; k < 9; k++) q[n++] = f[k];
+        float *r[6] = {S->rxx, S->ryy, S->rzz, S->rxy, S->ryz, S->rxz};
+        for (int k = 0; k < 6 && S->L > 0; k++)
+            for (int l = 0; l < S->L; l++) q[n++] = r[k] + (long)l * S->nv;
+    } else {
+        float *f[5] = {S->vx, S->vz, S->sxx, S->szz, S->sxz};
+        for (int k = 0; k < 5; k++) q[n++] = f[k];
+        float *r[3] = {S->rxx, S->rzz, S->rxz};
+        for (int k = 0; k < 3 && S->L > 0; k++)
+            for (int l = 0; l < S->L; l++) q[n++] = r[k] + (long)l * S->nv;
+    }
+    return n;
+}
+
+/* F[q][f][cell] (complex) += fact[f] u_q[cell] over the padded arrays */
+static void dft_acc(sim *S, float *F, int nf, const float *fact) {
+    float *q[9 + 6 * OR_MAXL];
+    const int nq = dft_arrays(S, q);
+    const long nv = S->nv;
+    for (int a = 0; a < nq; a++)
+        for (int f = 0; f < nf; f++) {
+            float *Fq = F + ((long)a * nf + f) * nv * 2;
+            const float fx = fact[2 * f], fy = fact[2 * f + 1];
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < nv; i++) {
+                const float u = q[a][i];
+                Fq[2 * i] += fx * u;
+                Fq[2 * i + 1] += fy * u;
+            }
+        }
+}
+
+typedef struct {
+    float x, y;
+} cfl;
+static inline cfl c_add(cfl a, cfl b) { return (cfl){a.x + b.x, a.y + b.y}; }
+static inline cfl c_sub(cfl a, cfl b) { return (cfl){a.x - b.x, a.y - b.y}; }
+static inline cfl c_add3(cfl a, cfl b, cfl c) { return (cfl){a.x + b.x + c.x, a.y + b.y + c.y}; }
+static inline cfl c_sub3(cfl a, cfl b, cfl c) { return (cfl){a.x - b.x - c.x, a.y - b.y - c.y}; }
+/* the spectrum of the time integral: a / (i w) */
+static inline cfl c_integ(cfl a, float w) { return (cfl){a.y / w, -a.x / w}; }
+/* Im(conj(b) a) */
+static inline float c_itreal(cfl a, cfl b) { return a.y * b.x - a.x * b.y; }
+/* the memory-variable correlation: ts Re(conj(b) a) + Im(conj(a) b) / w */
+static inline float c_rm(cfl a, cfl b, float ts, float w) {
+    return ts * (a.x * b.x + a.y * b.y) + (a.x * b.y - a.y * b.x) / w;
+}
+
+/* The 24 coefficients of the scalar products for one cell
+ * (src/calc_grad.c:111-376: grad_coefvisc_0..3, grad_coefelast_0..2), in
+ * double.  M, mu, rho are the transformed parameters (rho the buoyancy
+ * dt/(dh rho)), ND and L as floats, al = sum_l (w0 tau_l)^2/(1+(w0 tau_l)^2).
+ * The square roots are the reference's float ones. */
+void or_dft_coef(double *c, int pt, int visco, float M, float mu, float tp, float ts, float rho,
+                 float ND, float L, float al) {
+    for (int k = 0; k < 24; k++) c[k] = 0;
+    const double Md = M, mud = mu, rd = rho, n3 = (ND + 1.0) / 3.0, Lal = L - al;
+    const double kp = 1.0 + L * tp, ks = 1.0 + L * ts, ap = 1.0 + al * tp, as = 1.0 + al * ts;
+    const double mu2 = mud * mud, as2 = as * as, ap2 = ap * ap;
+    if (!visco) {
+        const double F = (ND * Md - 2.0 * (ND - 1.0) * mud) * (ND * Md - 2.0 * (ND - 1.0) * mud);
+        if (pt == 1) {
+            c[0] = 1.0 / F;
+            c[2] = 1.0 / mu2;
+            c[3] = n3 / F;
+            c[4] = 1.0 / (2 * ND * mu2);
+            return;
+        }
+        const double A = pt == 0 ? 2.0 * sqrtf(rho * M) : 2.0 * sqrtf(M / rho);
+        const double B = pt == 0 ? 2.0 * sqrtf(rho * mu) : 2.0 * sqrtf(mu / rho);
+        c[0] = A / F;
+        c[2] = B / mu2;
+        c[3] = B * n3 / F;
+        c[4] = B / (2 * ND * mu2);
+        if (pt == 0) {
+            c[16] = Md / rd / F;
+            c[18] = mud / rd / mu2;
+            c[19] = mud / rd * n3 / F;
+            c[20] = mud / rd / (2 * ND * mu2);
+        } else {
+            c[16] = -Md / rd * kp * ap * as2;
+            c[18] = -mud / rd / mu2;
+            c[19] = -mud / rd * n3 / F;
+            c[20] = -mud / rd / (2 * ND * mu2);
+        }
+        return;
+    }
+    const double g1 = ND * Md * kp * as - 2.0 * (ND - 1.0) * mud * ks * ap;
+    const double g2 = ND * Md * tp * as - 2.0 * (ND - 1.0) * mud * ts * ap;
+    const double f1 = g1 * g1, f2 = g2 * g2;
+    /* the tau parts (c8..c15) and the unweighted moduli parts */
+    const double t8 = Md * Lal * as2 / f1, t9 = Md * as2 / f2;
+    const double t10 = Lal / (mud * ks * ks), t11 = n3 * mud * Lal * ap2 / f1;
+    const double t12 = Lal / (2 * ND * mud * ks * ks), t13 = 1.0 / (mud * ts * ts);
+    const double t14 = n3 * mud * ap2 / f2, t15 = 1.0 / (2 * ND * mud * ts * ts);
+    const double u[8] = {kp * ap * as2 / f1,        tp * ap * as2 / f2,
+                         as / (mu2 * ks),           n3 * ks * as * ap2 / f1,
+                         as / (2.0 * ND * mu2 * ks), as / (mu2 * ts),
+                         n3 * ts * as * ap2 / f2,   as / (2.0 * ND * mu2 * ts)};
+    double A = 1, B = 1;
+    if (pt == 0 || pt == 3) {
+        A = 2.0 * sqrtf(rho * M);
+        B = 2.0 * sqrtf(rho * mu);
+    } else if (pt == 2) {
+        A = 2.0 * sqrtf(M / rho);
+        B = 2.0 * sqrtf(mu / rho);
+    }
+    for (int k = 0; k < 8; k++) c[k] = (k < 2 ? A : B) * u[k];
+    const double t[8] = {t8, t9, t10, t11, t12, t13, t14, t15};
+    for (int k = 0; k < 8; k++) c[8 + k] = t[k];
+    if (pt == 3) {
+        /* taup is the Q parameter: its chain through vp = sqrt(M/rho)(1+taup) */
+        const double sm = sqrtf(M / rho), R = tp / sm, Rp = (1.0 + tp) / sm;
+        const double w[8] = {Md * Lal * as2, Md * as2, Lal / (mud * ks * ks),
+                             n3 * mud * Lal * ap2, Lal / (2 * ND * mud * ks * ks),
+                             1.0 / (mud * ts * ts), n3 * mud * ap2, 1.0 / (2 * ND * mud * ts * ts)};
+        const double dv[8] = {f1, f2, 1, f1, 1, 1, f2, 1};
+        const double P[8] = {A * kp * ap * as2, A * tp * ap * as2, B * as / (mu2 * ks),
+                             B * n3 * ks * as * ap2, B * as / (2.0 * ND * mu2 * ks),
+                             B * as / (mu2 * ts), B * n3 * ts * as * ap2,
+                             B * as / (2.0 * ND * mu2 * ts)};
+        for (int k = 0; k < 8; k++) {
+            c[k] = (P[k] - R * w[k]) / dv[k];
+            c[8 + k] = (-P[k] + Rp * w[k]) / dv[k];
+        }
+    }
+    if (pt == 1) return;
+    const double s = pt == 2 ? -1.0 : 1.0, m_r = Md / rd, u_r = mud / rd;
+    c[16] = s * m_r * u[0];
+    c[17] = s * m_r * u[1];
+    for (int k = 2; k < 8; k++) c[16 + k] = s * u_r * u[k];
+}
+
+/* Gradient increments of one shot from the forward (Ff) and adjoint (Fa)
+ * spectra (src/calc_grad.c:608-968): per interior cell and frequency, the
+ * memory-variable spectra are integrated out of the stress spectra, the
+ * scalar products dot[0..8] formed, and grad[5][np] (param order, then
+ * taup, taus) incremented in double.  The reference's 3D memory-variable
+ * terms are kept as written: ryy_mxxzz and rzz_mxxyy repeat rxx_myyzz, and
+ * the adjoint syz spectrum is corrected with the forward ryz. */
+static void dft_grad(sim *S, const float *Ff, const float *Fa, int nf, const float *bins,
+                     float df, int ntnyq, double *grad) {
+    const or_csts *c = S->c;
+    const int ND3 = S->ND == 3, L = S->L, nq = ND3 ? 9 + 6 * L : 5 + 3 * L;
+    const long nv = S->nv, np = S->np;
+    const float NDf = (float)S->ND, Lf = (float)L;
+    float tausig[OR_MAXL];
+    float al = 0;
+    const float w0 = 2.0 * PI * c->f0;
+    for (int l = 0; l < L; l++) {
+        tausig[l] = 1.0 / (2.0 * PI * c->FL[l]);
+        const double r = w0 / (2.0 * PI * c->FL[l]);
+        al += r * r / (1.0 + r * r);
+    }
+    const int pt = c->param_type;
+    const int H = S->FDOH;
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < np; m++) {
+        int i, j, k;
+        if (ND3) {
+            i = (int)(m / ((long)S->NY * S->NZ));
+            j = (int)(m / S->NZ % S->NY);
+            k = (int)(m % S->NZ);
+        } else {
+            i = (int)(m / S->NZ);
+            j = 0;
+            k = (int)(m % S->NZ);
+        }
+        const long iv = ND3 ? ((long)(i + H) * S->NYp + (j + H)) * S->NZp + (k + H)
+                            : (long)(i + H) * S->NZp + (k + H);
+        double cf[24];
+        or_dft_coef(cf, pt, L > 0, S->M[m], S->mu[m], L > 0 ? S->taup[m] : 0.f,
+                    L > 0 ? S->taus[m] : 0.f, S->rho[m], NDf, Lf, al);
+        if (S->mu[m] < 1) {
+            for (int n = 2; n < 8; n++) cf[n] = 0;
+            for (int n = 10; n < 16; n++) cf[n] = 0;
+            for (int n = 18; n < 24; n++) cf[n] = 0;
+        }
+        for (int f = 0; f < nf; f++) {
+            const float w = (float)(2.0 * PI * df * bins[f]);
+            cfl A[9 + 6 * OR_MAXL], B[9 + 6 * OR_MAXL];   /* forward, adjoint */
+            for (int q = 0; q < nq; q++) {
+                const long o = (((long)q * nf + f) * nv + iv) * 2;
+                A[q] = (cfl){Ff[o], Ff[o + 1]};
+                B[q] = (cfl){Fa[o], Fa[o + 1]};
+            }
+            double dot[9] = {0};
+            if (ND3) {
+                /* A/B: vx vy vz sxx syy szz sxy syz sxz, rxx.. ryy.. rzz.. rxy.. ryz.. rxz.. */
+                cfl *sxx = &A[3], *syy = &A[4], *szz = &A[5], *sxy = &A[6], *syz = &A[7],
+                    *sxz = &A[8];
+                cfl *sxxr = &B[3], *syyr = &B[4], *szzr = &B[5], *sxyr = &B[6], *syzr = &B[7],
+                    *sxzr = &B[8];
+                for (int l = 0; l < L; l++) {
+                    const cfl rxx = A[9 + l], ryy = A[9 + L + l], rzz = A[9 + 2 * L + l],
+                              rxy = A[9 + 3 * L + l], ryz = A[9 + 4 * L + l],
+                              rxz = A[9 + 5 * L + l];
+                    const cfl rxxr = B[9 + l], ryyr = B[9 + L + l], rzzr = B[9 + 2 * L + l],
+                              rxyr = B[9 + 3 * L + l], ryzr = B[9 + 4 * L + l],
+                              rxzr = B[9 + 5 * L + l];
+                    *sxx = c_sub(*sxx, c_integ(rxx, w));
+                    *szz = c_sub(*szz, c_integ(rzz, w));
+                    *syy = c_sub(*syy, c_integ(ryy, w));
+                    *sxz = c_sub(*sxz, c_integ(rxz, w));
+                    *sxy = c_sub(*sxy, c_integ(rxy, w));
+                    *syz = c_sub(*syz, c_integ(ryz, w));
+                    *sxxr = c_sub(*sxxr, c_integ(rxxr, w));
+                    *szzr = c_sub(*szzr, c_integ(rzzr, w));
+                    *syyr = c_sub(*syyr, c_integ(ryyr, w));
+                    *sxzr = c_sub(*sxzr, c_integ(rxzr, w));
+                    *sxyr = c_sub(*sxyr, c_integ(rxyr, w));
+                    *syzr = c_sub(*syzr, c_integ(ryz, w));
+                    const cfl rsum = c_add3(rxx, ryy, rzz), rsumr = c_add3(rxxr, ryyr, rzzr);
+                    const cfl rdev = c_sub3(rxx, ryy, rzz);
+                    dot[1] += c_rm(rsumr, rsum, tausig[l], w) / ntnyq;
+                    dot[5] += (c_rm(rxyr, rxy, tausig[l], w) + c_rm(rxzr, rxz, tausig[l], w) +
+                               c_rm(ryzr, ryz, tausig[l], w)) /
+                              ntnyq;
+                    dot[6] = dot[1];
+                    dot[7] += (c_rm(rxxr, rdev, tausig[l], w) + c_rm(ryyr, rdev, tausig[l], w) +
+                               c_rm(rzzr, rdev, tausig[l], w)) /
+                              ntnyq;
+                }
+                const cfl ssum = c_add3(*sxx, *syy, *szz), ssumr = c_add3(*sxxr, *syyr, *szzr);
+                const cfl dxx = c_sub3(*sxx, *syy, *szz), dyy = c_sub3(*syy, *sxx, *szz),
+                          dzz = c_sub3(*szz, *sxx, *syy);
+                dot[0] = w * c_itreal(ssumr, ssum) / ntnyq;
+                dot[2] = w * (c_itreal(*sxyr, *sxy) + c_itreal(*sxzr, *sxz) +
+                              c_itreal(*syzr, *syz)) / ntnyq;
+                dot[3] = dot[0];
+                dot[4] = w * (c_itreal(*sxxr, dxx) + c_itreal(*syyr, dyy) +
+                              c_itreal(*szzr, dzz)) / ntnyq;
+                dot[8] = w * (c_itreal(B[0], A[0]) + c_itreal(B[1], A[1]) +
+                              c_itreal(B[2], A[2])) / ntnyq;
+            } else {
+                /* A/B: vx vz sxx szz sxz, rxx.. rzz.. rxz.. */
+                cfl *sxx = &A[2], *szz = &A[3], *sxz = &A[4];
+                cfl *sxxr = &B[2], *szzr = &B[3], *sxzr = &B[4];
+                for (int l = 0; l < L; l++) {
+                    const cfl rxx = A[5 + l], rzz = A[5 + L + l], rxz = A[5 + 2 * L + l];
+                    const cfl rxxr = B[5 + l], rzzr = B[5 + L + l], rxzr = B[5 + 2 * L + l];
+                    *sxx = c_sub(*sxx, c_integ(rxx, w));
+                    *szz = c_sub(*szz, c_integ(rzz, w));
+                    *sxz = c_sub(*sxz, c_integ(rxz, w));
+                    *sxxr = c_sub(*sxxr, c_integ(rxxr, w));
+                    *szzr = c_sub(*szzr, c_integ(rzzr, w));
+                    *sxzr = c_sub(*sxzr, c_integ(rxzr, w));
+                    const cfl rsum = c_add(rxx, rzz), rsumr = c_add(rxxr, rzzr);
+                    const cfl dxx = c_sub(rxx, rzz), dzz = c_sub(rzz, rxx);
+                    dot[1] += c_rm(rsumr, rsum, tausig[l], w) / ntnyq;
+                    dot[5] += (c_rm(rxzr, rxz, tausig[l], w)) / ntnyq;
+                    dot[6] = dot[1];
+                    dot[7] += (c_rm(rxxr, dxx, tausig[l], w) + c_rm(rzzr, dzz, tausig[l], w)) /
+                              ntnyq;
+                }
+                const cfl ssum = c_add(*sxx, *szz), ssumr = c_add(*sxxr, *szzr);
+                const cfl dxx = c_sub(*sxx, *szz), dzz = c_sub(*szz, *sxx);
+                dot[0] = w * c_itreal(ssumr, ssum) / ntnyq;
+                dot[2] = w * (c_itreal(*sxzr, *sxz)) / ntnyq;
+                dot[3] = dot[0];
+                dot[4] = w * (c_itreal(*sxxr, dxx) + c_itreal(*szzr, dzz)) / ntnyq;
+                dot[8] = w * (c_itreal(B[0], A[0]) + c_itreal(B[1], A[1])) / ntnyq;
+            }
+            grad[m] += -cf[0] * dot[0] + cf[1] * dot[1];
+            grad[np + m] += -cf[2] * dot[2] + cf[3] * dot[3] - cf[4] * dot[4] + cf[5] * dot[5] -
+                            cf[6] * dot[6] + cf[7] * dot[7];
+            if (L > 0) {
+                grad[3 * np + m] += -cf[8] * dot[0] + cf[9] * dot[1];
+                grad[4 * np + m] += -cf[10] * dot[2] + cf[11] * dot[3] - cf[12] * dot[4] +
+                                    cf[13] * dot[5] - cf[14] * dot[6] + cf[15] * dot[7];
+            }
+            grad[2 * np + m] += -dot[8] + cf[16] * dot[0] - cf[17] * dot[1] + cf[18] * dot[2] -
+                                cf[19] * dot[3] + cf[20] * dot[4] - cf[21] * dot[5] +
+                                cf[22] * dot[6] - cf[23] * dot[7];
+        }
+    }
+}
+
+/* rms and rmsnorm at the gradient frequencies (src/residuals.c:364-403):   */
+/* the traces -out + din and din over t < tmax, band-passed as for the     */
+/* residuals, zero-padded to nfft (the smallest 2,3,5-smooth length >=     */
+/* tmax); rms sums |X(k)|^2 at k = (int)(freq nfft dt + 1) of every output  */
+/* and receiver, with the DFT bin evaluated directly in double.             */
+int or_dft_rms(const or_csts *c, int nout, int nrec, const float *out, const float *din,
+               int nf, const float *freqs, double *rms) {
+    const int NT = c->NT, tmax = c->tmax > 0 ? c->tmax : c->NT;
+    int nfft = tmax < 1 ? 1 : tmax;
+    for (;; nfft++) {
+        int k = nfft;
+        while (k % 2 == 0) k /= 2;
+        while (k % 3 == 0) k /= 3;
+        while (k % 5 == 0) k /= 5;
+        if (k == 1) break;
+    }
+    int kb[OR_MAXF];
+    if (nf > OR_MAXF) return 1;
+    for (int f = 0; f < nf; f++) {
+        kb[f] = (int)(freqs[f] * nfft * c->dt + 1);
+        if (kb[f] > nfft / 2) return 1;
+    }
+    const long n = (long)nout * nrec * NT;
+    float *vo = malloc(sizeof(float) * (n ? n : 1)), *vi = malloc(sizeof(float) * (n ? n : 1));
+    memcpy(vo, out, sizeof(float) * n);
+    memcpy(vi, din, sizeof(float) * n);
+    if (c->fmax > 0 || c->fmin > 0)
+        for (int o = 0; o < nout; o++) {
+            or_butterworth(vo + (long)o * nrec * NT, c->fmin, c->fmax, c->dt, NT, tmax, nrec, 6);
+            or_butterworth(vi + (long)o * nrec * NT, c->fmin, c->fmax, c->dt, NT, tmax, nrec, 6);
+        }
+    rms[0] = rms[1] = 0;
+    for (int g = 0; g < nrec; g++)
+        for (int o = 0; o < nout; o++) {
+            const float *a = vo + ((long)o * nrec + g) * NT, *b = vi + ((long)o * nrec + g) * NT;
+            for (int f = 0; f < nf; f++) {
+                double rr = 0, ri = 0, nr = 0, ni = 0;
+                for (int t = 0; t < tmax; t++) {
+                    const double ph = 2.0 * PI * (double)(((long)kb[f] * t) % nfft) / nfft;
+                    const double cs = cos(ph), sn = sin(ph);
+                    const float r = -a[t] + b[t];
+                    rr += r * cs;
+                    ri -= r * sn;
+                    nr += b[t] * cs;
+                    ni -= b[t] * sn;
+                }
+                rms[0] += rr * rr + ri * ri;
+                rms[1] += nr * nr + ni * ni;
+            }
+        }
+    free(vo);
+    free(vi);
+    return 0;
+}
+
+int or_gradient_dft(const or_csts *c, const float *p0, const float *p1, const float *rho,
+                    const float *taup, const float *taus, int nsrc, const float *src_pos,
+                    const float *src, int nrec, const float *rec_pos, const float *din,
+                    const float *res_in, int gradfix, int nf, const float *freqs, float *out,
+                    double *grad, double *rms, float *dft_out, char *err, int errlen) {
+    sim S, A;
+    char dummy[8];
+    if (!err) {
+        err = dummy;
+        errlen = sizeof(dummy);
+    }
+    if (c->ND != 2 && c->ND != 3) {
+        snprintf(err, errlen, "the DFT gradient needs the elastic or viscoelastic case (ND 2 or 3)");
+        return 1;
+    }
+    if (c->param_type == 3 && c->L == 0) {
+        snprintf(err, errlen, "param_type 3 needs a viscoelastic model (L > 0)");
+        return 1;
+    }
+    int dtnyq, ntnyq;
+    float df, bins[OR_MAXF];
+    const int smp = nf > 0 && nf <= OR_MAXF
+                        ? or_dft_sampling(c, nf, freqs, &dtnyq, &ntnyq, &df, bins) : 1;
+    if (smp == 1) {
+        snprintf(err, errlen, "gradfreqs: 1 to %d positive frequencies", OR_MAXF);
+        return 1;
+    }
+    if (smp == 2) {
+        snprintf(err, errlen, "gradfreqs below the frequency step of the record (bin 0)");
+        return 1;
+    }
+    if (sim_init(&S, c, err, errlen)) {
+        sim_free(&S);
+        return 1;
+    }
+#ifdef _OPENMP
+    if (c->nthreads > 0) omp_set_num_threads(c->nthreads);
+#endif
+    or_transform(c, p0, p1, rho, taup, taus, S.M, S.mu, S.rho, S.rip, S.rjp, S.rkp,
+                 S.muipkp, S.muipjp, S.mujpkp, S.taup, S.taus, S.tausipkp,
+                 S.tausipjp, S.tausjpkp);
+    float dts;
+    if (or_check_stability(c, S.M, S.mu, S.rho, &dts)) {
+        snprintf(err, errlen, "Time step too large, to be stable, set dt<%f", dts);
+        sim_free(&S);
+        return 1;
+    }
+    if (check_positions(&S, nsrc, src_pos, 5, err, errlen) ||
+        check_positions(&S, nrec, rec_pos, 8, err, errlen)) {
+        sim_free(&S);
+        return 1;
+    }
+    const int tmax = c->tmax > 0 ? c->tmax : c->NT, tmin = c->tmin, nout = or_nout(c);
+    const int nq = S.ND == 3 ? 9 + 6 * S.L : 5 + 3 * S.L;
+    const size_t nF = (size_t)nq * nf * S.nv * 2;
+    float *Ff = calloc(nF, sizeof(float)), *Fa = calloc(nF, sizeof(float));
+    float fact[2 * OR_MAXF];
+    memset(out, 0, sizeof(float) * (size_t)nout * nrec * c->NT);
+    /* forward: the DFT of the state before the sources of each sampled step */
+    for (int t = 0; t < tmax; t++) {
+        if (t >= tmin && (t - tmin) % dtnyq == 0) {
+            dft_fact(c, nf, bins, dtnyq, ntnyq, (t - tmin) / dtnyq, fact);
+            dft_acc(&S, Ff, nf, fact);
+        }
+        sources(&S, t, 1, nsrc, src_pos, src);
+        if (S.ND == 3) {
+            update_v3D(&S);
+            update_s3D(&S);
+            if (c->freesurf) surface3D(&S, 1);
+        } else {
+            update_v2D(&S);
+            update_s2D(&S);
+            if (c->freesurf) surface2D(&S, 1);
+        }
+        varsout(&S, t, nrec, rec_pos, out);
+    }
+    if (dft_out) memcpy(dft_out, Ff, nF * sizeof(float));
+    size_t nres = (size_t)nout * nrec * c->NT;
+    float *res = calloc(nres ? nres : 1, sizeof(float));
+    double rr[2] = {0, 0};
+    if (res_in) memcpy(res, res_in, nres * sizeof(float));
+    else {
+        or_residuals(c, nout, nrec, out, din, res, rr, gradfix);
+        if (or_dft_rms(c, nout, nrec, out, din, nf, freqs, rr)) {
+            snprintf(err, errlen, "gradfreqs above the Nyquist frequency of the record");
+            free(Ff);
+            free(Fa);
+            free(res);
+            sim_free(&S);
+            return 1;
+        }
+    }
+    res_scale(&S, nrec, rec_pos, res, gradfix & OR_FIX_RES_SCALE);
+    if (rms) {
+        rms[0] = rr[0];
+        rms[1] = rr[1];
+    }
+    if (adj_alloc(&A, &S, err, errlen)) {
+        free(Ff);
+        free(Fa);
+        free(res);
+        sim_free(&S);
+        return 1;
+    }
+    if (!(gradfix & OR_FIX_ADJ_CPML)) adj_share_cpml(&A, &S);
+    /* adjoint: the adjoint wavefield alone, its DFT after the updates */
+    for (int t = tmax - 1; t > tmin; t--) {
+        if (c->freesurf) {
+            if (S.ND == 3) surface_adj3D(&A, gradfix & OR_FIX_SURF_SYY);
+            else surface_adj2D(&A);
+        }
+        residuals_inject(&A, t, nrec, rec_pos, res);
+        if (S.ND == 3) {
+            update_s3D(&A);
+            update_v3D(&A);
+        } else {
+            update_s2D(&A);
+            update_v2D(&A);
+        }
+        if ((t - tmin) % dtnyq == 0) {
+            dft_fact(c, nf, bins, dtnyq, ntnyq, (t - tmin) / dtnyq, fact);
+            dft_acc(&A, Fa, nf, fact);
+        }
+    }
+    memset(grad, 0, sizeof(double) * 5 * S.np);
+    dft_grad(&S, Ff, Fa, nf, bins, df, ntnyq, grad);
+    free(Ff);
+    free(Fa);
     free(res);
     adj_free(&A);
     sim_free(&S);

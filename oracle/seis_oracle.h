@@ -22,6 +22,7 @@ extern "C" {
 #endif
 
 #define OR_MAXL 8
+#define OR_MAXF 512   /* gradient frequencies of the DFT gradient */
 #define OR_FIX_RES_SCALE 1
 #define OR_FIX_ADJ_CPML 2
 #define OR_FIX_RHO_SCALE 4
@@ -165,6 +166,32 @@ int  or_gradient(const or_csts *c, const float *p0, const float *p1,
                  float *res_out, float *grad, double *rms, float *fields_out,
                  float *hess, float *gradtau, int movout, float *movie, char *err,
                  int errlen);
+
+/* DFT-domain gradient of one shot (BACK_PROP_TYPE=2; src/calc_grad.c:
+ * 377-968, src/automatic_kernels.c:1030-1128, src/assign_modeling_case.c:
+ * 549-570, src/time_stepping.c:658-672, 767-780, 869-965), ND 2 or 3: the
+ * forward run accumulates the DFT of every wavefield and memory variable at
+ * the nf frequencies freqs (Hz) every DTNYQ steps from tmin, before the
+ * sources; the adjoint run propagates the adjoint wavefield alone and
+ * accumulates its DFT after the updates; the gradient correlates the two
+ * spectra per cell and frequency with the coefficients of param_type.
+ * grad: [5][Pi N] (param order, then taup, taus), double, not transformed
+ * further (transf_grad applies to BACK_PROP_TYPE=1 only).  rms (optional):
+ * [rms, rmsnorm] at the gradient frequencies.  dft_out (optional): the
+ * forward spectra [nq][nf][padded cells][2], nq = 5 + 3L (2D: vx vz sxx szz
+ * sxz, then rxx rzz rxz by l) or 9 + 6L (3D: vx vy vz sxx syy szz sxy syz
+ * sxz, then rxx ryy rzz rxy ryz rxz by l). */
+int  or_dft_sampling(const or_csts *c, int nf, const float *freqs, int *dtnyq, int *ntnyq,
+                     float *df, float *bins);
+void or_dft_coef(double *c, int pt, int visco, float M, float mu, float tp, float ts, float rho,
+                 float ND, float L, float al);
+int  or_dft_rms(const or_csts *c, int nout, int nrec, const float *out, const float *din,
+                int nf, const float *freqs, double *rms);
+int  or_gradient_dft(const or_csts *c, const float *p0, const float *p1, const float *rho,
+                     const float *taup, const float *taus, int nsrc, const float *src_pos,
+                     const float *src, int nrec, const float *rec_pos, const float *din,
+                     const float *res_in, int gradfix, int nf, const float *freqs, float *out,
+                     double *grad, double *rms, float *dft_out, char *err, int errlen);
 
 #ifdef __cplusplus
 }

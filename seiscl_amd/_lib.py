@@ -109,6 +109,7 @@ SIGNATURES = [
     ("seis_adjoint", _I, [_P, _FP]),
     ("seis_adjoint_device", _I, [_P, _P, _I]),
     ("seis_adjoint_movie", _I, [_P, _FP, _I, _I, ctypes.POINTER(ctypes.c_char_p), _FP]),
+    ("seis_grad_dft", _I, [_P, _I, _FP]),
     ("seis_grad_read", _I, [_P, _FP]),
     ("seis_grad_read_tau", _I, [_P, _FP]),
     ("seis_grad_read_device", _I, [_P, _P, _P]),

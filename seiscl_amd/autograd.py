@@ -29,7 +29,13 @@ class Propagator:
     param_type 0-2 (vp, vs, rho / M, mu, rho / Ip, Is, rho) and, for a
     viscoelastic model (L > 0), taup and taus.  seismic() also accepts only
     the first three for a viscoelastic model; taup and taus then keep the
-    values the Propagator was created with."""
+    values the Propagator was created with.
+
+    The gradient is the reference's correlation formula with the
+    corrections of csts["gradfix"] (default 223).  gradfix=255 adds bit 32,
+    which chains the vs and rho terms through the staggered averages of the
+    parameter transform: the backward pass is then the derivative of the
+    discrete misfit (finite differences agree to 1e-4)."""
 
     def __init__(self, ctx: Context, params: dict, **csts):
         self.model = Model(ctx, params, **csts)

@@ -33,6 +33,9 @@ typedef struct {
     int gradout, gradsrcout, resout, rmsout, movout, Hout, inputres, back_prop_type;
     int scalerms, scaleshot, scalermsnorm, restype, halfpar, fp16_file;
     float fmin, fmax;
+    /* back_prop_type 2: the gradient frequencies (/gradfreqs, Hz) */
+    int nfreqs;
+    float *gradfreqs;
     /* sources and receivers, file order (src/read_hdf5.c:307-459) */
     int allns, allng, ns;
     float *src_pos, *src, *rec_pos;
@@ -184,10 +187,24 @@ static int read_csts(hid_t f, job_t *j) {
         default: return err("FP16=%d is not a precision mode: use 0, 1, 2, 3 or 64",
                             j->fp16_file);
     }
+    /* gradients by backpropagation (1) or from the DFT of the wavefields at
+     * the frequencies /gradfreqs (2, src/assign_modeling_case.c:1189-1190) */
+    if (j->gradout && j->back_prop_type != 1 && j->back_prop_type != 2)
+        return err("back_prop_type=%d is not available: 1 (backpropagation) or 2 (DFT)",
+                   j->back_prop_type);
+    if (j->gradout && j->back_prop_type == 2) {
+        if (!seis_h5_exists(f, "/gradfreqs")) return err("Variable /gradfreqs is not defined");
+        const long long nf = seis_h5_numel(f, "/gradfreqs");
+        if (nf < 1) return err("gradfreqs holds no frequency");
+        j->nfreqs = (int)nf;
+        j->gradfreqs = (float *)malloc(sizeof(float) * (size_t)nf);
+        if (!j->gradfreqs) return err("out of host memory");
+        if (seis_h5_read(f, "/gradfreqs", SEIS_H5_F32, j->gradfreqs, nf))
+            return err("%s", seis_h5_last_error());
+        if (j->inputres) return err("inputres needs back_prop_type=1");
+        if (j->movout) return err("movout with gradout needs back_prop_type=1");
+    }
     /* features outside this engine fail loudly rather than being ignored */
-    if (j->gradout && j->back_prop_type != 1)
-        return err("back_prop_type=%d is not available: gradients are computed by "
-                   "backpropagation (back_prop_type=1)", j->back_prop_type);
     if (j->gradsrcout) return err("source gradient output (gradsrcout) is not available");
     if (j->halfpar) return err("half-precision parameters (halfpar) are not available");
     if (j->restype != 0 && j->restype != 1)
@@ -626,6 +643,10 @@ static int run(job_t *j) {
         s = err("internal: output count mismatch");
         goto done;
     }
+    if (j->gradout && j->back_prop_type == 2 && seis_grad_dft(m, j->nfreqs, j->gradfreqs)) {
+        s = err("%s", seis_last_error());
+        goto done;
+    }
     if (j->gradout && seis_grad_zero(m)) {
         s = err("%s", seis_last_error());
         goto done;
@@ -774,6 +795,7 @@ done:
 }
 
 static void job_free(job_t *j) {
+    free(j->gradfreqs);
     free(j->src_pos);
     free(j->src);
     free(j->rec_pos);

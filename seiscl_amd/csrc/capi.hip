@@ -223,6 +223,19 @@ struct seis_model {
         size_t mem_limit = 0;       // budget of ckpt + hist; 0: 90% of free memory
         float *Ms = nullptr, *mus = nullptr;   // moduli of the gradient formulas
     } grad;
+    // DFT-domain gradient (seis_grad_dft): sampling of the spectra, the
+    // factor table, forward and adjoint spectra, the cell parameters
+    struct Dft {
+        int nf = 0;                    // 0: gradients by backpropagation
+        std::vector<float> freqs, bins, w;
+        int dtnyq = 1, ntnyq = 1, nq = 0;
+        float df = 0;
+        float *fact = nullptr;         // [ntnyq][nf] complex
+        float *Ff = nullptr, *Fa = nullptr;   // [nq][nf][nfield] complex
+        float *par = nullptr;          // [5][N]: M mu taup taus rho, unpadded
+        float *w_dev = nullptr;        // [nf] angular frequencies
+        size_t cap = 0;                // nf the buffers hold
+    } dft;
 
     float *dalloc(long n) {
         void *p = nullptr;
@@ -1892,15 +1905,235 @@ static int saved_layout(seis_model *m, SavedHeader &h) {
     }
     return 0;
 }
+
+// ---- DFT-domain gradient (back_prop_type 2) --------------------------------
+// Sampling of src/assign_modeling_case.c:549-570: every DTNYQ steps from
+// tmin, NTNYQ samples, frequency bins of the record at df.
+int dft_sampling(seis_model *m) {
+    auto &D = m->dft;
+    const seis_csts &c = m->c;
+    float fmax = 0;
+    for (float f : D.freqs) fmax = std::max(fmax, f);
+    const int tmax = tmax_of(m);
+    D.dtnyq = (int)std::ceil(0.0156 / fmax / c.dt);
+    D.ntnyq = (tmax - c.tmin) / D.dtnyq + 1;
+    D.df = (float)(1.0 / D.ntnyq / c.dt / D.dtnyq);
+    D.bins.resize(D.nf);
+    D.w.resize(D.nf);
+    for (int f = 0; f < D.nf; ++f) {
+        D.bins[f] = (float)std::floor(D.freqs[f] / D.df);
+        D.w[f] = (float)(2.0 * kPiDft * D.df * D.bins[f]);
+    }
+    return 0;
+}
+
+// The arrays whose spectra are kept, in the order of the gradient formula.
+DftArrays dft_arrays(const seis_model *m, const FDArgs<float> &a) {
+    DftArrays q;
+    const long nf = m->g.nfield;
+    const int L = m->c.L;
+    if (m->c.ND == 3) {
+        const float *f[9] = {a.vx, a.vy, a.vz, a.sxx, a.syy, a.szz, a.sxy, a.syz, a.sxz};
+        for (const float *p : f) q.p[q.n++] = p;
+        const float *r[6] = {a.rxx, a.ryy, a.rzz, a.rxy, a.ryz, a.rxz};
+        for (int k = 0; k < 6 && L > 0; ++k)
+            for (int l = 0; l < L; ++l) q.p[q.n++] = r[k] + (long)l * nf;
+    } else {
+        const float *f[5] = {a.vx, a.vz, a.sxx, a.szz, a.sxz};
+        for (const float *p : f) q.p[q.n++] = p;
+        const float *r[3] = {a.rxx, a.rzz, a.rxz};
+        for (int k = 0; k < 3 && L > 0; ++k)
+            for (int l = 0; l < L; ++l) q.p[q.n++] = r[k] + (long)l * nf;
+    }
+    return q;
+}
+
+// Spectra, factor table and cell parameters of the model (once per
+// frequency set); the spectra are zeroed for each forward run.
+int dft_prepare(seis_model *m) {
+    auto &D = m->dft;
+    auto &G = m->grad;
+    const seis_csts &c = m->c;
+    dft_sampling(m);
+    D.nq = c.ND == 3 ? 9 + 6 * c.L : 5 + 3 * c.L;
+    hipStream_t st = m->ctx->stream;
+    const long nfd = m->g.nfield;
+    if (D.cap != (size_t)D.nf || !D.Ff) {
+        // buffers of an earlier frequency set stay with the model until it
+        // is destroyed
+        const long n = 2L * D.nq * D.nf * nfd;
+        if (!(D.Ff = m->dalloc(n)) || !(D.Fa = m->dalloc(n))) return 1;
+        if (!(D.w_dev = m->dalloc(D.nf))) return 1;
+        if (!(D.fact = m->dalloc(2L * D.ntnyq * D.nf))) return 1;
+        D.cap = D.nf;
+    }
+    std::vector<float> fact(2 * (size_t)D.ntnyq * D.nf);
+    const float sc = (float)D.dtnyq * c.dt;
+    for (int th = 0; th < D.ntnyq; ++th)
+        for (int f = 0; f < D.nf; ++f) {
+            const double a = 2.0 * D.bins[f] * th / D.ntnyq;
+            fact[2 * ((size_t)th * D.nf + f)] = (float)(sc * std::cos(kPiDft * a));
+            fact[2 * ((size_t)th * D.nf + f) + 1] = (float)(-sc * std::sin(kPiDft * a));
+        }
+    HIP_TRY(hipMemcpyAsync(D.fact, fact.data(), fact.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(D.w_dev, D.w.data(), D.nf * 4, hipMemcpyHostToDevice, st));
+    const size_t n = (size_t)m->g.NZ * m->g.NY * m->g.NX;
+    if (!D.par && !(D.par = m->dalloc(5L * n))) return 1;
+    std::vector<float> par(5 * n, 0.f);
+    const RefParams &p = m->rp;
+    std::copy(p.M.begin(), p.M.end(), par.begin());
+    if (!p.mu.empty()) std::copy(p.mu.begin(), p.mu.end(), par.begin() + n);
+    if (c.L > 0) {
+        std::copy(p.taup.begin(), p.taup.end(), par.begin() + 2 * n);
+        std::copy(p.taus.begin(), p.taus.end(), par.begin() + 3 * n);
+    }
+    std::copy(p.rho.begin(), p.rho.end(), par.begin() + 4 * n);
+    HIP_TRY(hipMemcpyAsync(D.par, par.data(), par.size() * 4, hipMemcpyHostToDevice, st));
+    launch_fill(D.Ff, 8 * (size_t)D.nq * D.nf * nfd, st);
+    launch_fill(D.Fa, 8 * (size_t)D.nq * D.nf * nfd, st);
+    // tau gradients of every viscoelastic parameterisation
+    if (c.L > 0 && !G.gtp) {
+        const long nd2 = 2L * nfd;
+        for (double **q : {&G.gtp, &G.gts}) {
+            if (!(*q = (double *)m->dalloc(nd2))) return 1;
+            launch_fill(*q, 8 * (size_t)nfd, st);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Forward run with the spectra of the state before the sources of every
+// sampled step (src/time_stepping.c:767-780).
+int forward_dft(seis_model *m, float *out) {
+    auto &D = m->dft;
+    const seis_csts &c = m->c;
+    const int tmax = tmax_of(m), tmin = c.tmin;
+    hipStream_t st = m->ctx->stream;
+    const DftArrays q = dft_arrays(m, m->a);
+    for (int t = 0; t < tmax; ++t) {
+        if (t >= tmin && (t - tmin) % D.dtnyq == 0)
+            launch_dft_acc(q, D.Ff, D.fact + 2L * ((t - tmin) / D.dtnyq) * D.nf, D.nf, m->g, st);
+        step_sources(m, t, 1, st);
+        step_v(m, st);
+        step_s(m, st);
+        if (c.freesurf && !m->a.fuse_surface) step_surface(m, st);
+        step_varsout(m, t, st);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    m->grad.saved = true;
+    return out ? seis_shot_read(m, out) : 0;
+}
+
+// Adjoint run of the adjoint wavefield alone (src/time_stepping.c:869-925
+// with BACK_PROP_TYPE 2: surface_adj, residuals, the adjoint updates, the
+// spectra after them), then the gradient from the two sets of spectra.
+int adjoint_dft(seis_model *m) {
+    auto &D = m->dft;
+    auto &G = m->grad;
+    const seis_csts &c = m->c;
+    const int NT = c.NT, nout = (int)m->outn.size(), tmax = tmax_of(m), tmin = c.tmin;
+    hipStream_t st = m->ctx->stream;
+    FDArgs<float> r = adjoint_args(m);
+    r.fuse_surface = 0;
+    const DftArrays q = dft_arrays(m, r);
+    const int serial = receivers_share_cells(m, m->rcell_host);
+    for (int t = tmax - 1; t > tmin; --t) {
+        if (c.freesurf) launch_surface_adj(r, G.stmp, m->nd_case, c.gradfix & FIX_SURF_SYY, st);
+        launch_residuals(r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t, serial,
+                         st);
+        launch_update_s(r, m->nd_case, st);
+        launch_update_v(r, m->nd_case, st);
+        if ((t - tmin) % D.dtnyq == 0)
+            launch_dft_acc(q, D.Fa, D.fact + 2L * ((t - tmin) / D.dtnyq) * D.nf, D.nf, m->g, st);
+    }
+    DftGrad d;
+    d.Ff = D.Ff;
+    d.Fa = D.Fa;
+    d.par = D.par;
+    d.w = D.w_dev;
+    d.gM = G.gM;
+    d.gmu = G.gmu;
+    d.grho = G.grho;
+    d.gtp = G.gtp;
+    d.gts = G.gts;
+    d.nf = D.nf;
+    d.L = c.L;
+    d.pt = c.param_type;
+    d.ntnyq = D.ntnyq;
+    float al = 0;
+    const float w0 = 2.0 * kPiDft * c.f0;
+    for (int l = 0; l < c.L; ++l) {
+        d.tausig[l] = 1.0 / (2.0 * kPiDft * c.FL[l]);
+        const double rr = w0 / (2.0 * kPiDft * c.FL[l]);
+        al += rr * rr / (1.0 + rr * rr);
+    }
+    d.al = al;
+    launch_dft_grad(d, m->g, st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    G.saved = false;
+    return 0;
+}
 }  // namespace
 
 extern "C" {
+
+int seis_grad_dft(seis_model *m, int nfreqs, const float *freqs) {
+    g_err.clear();
+    if (!m) return fail("seis_grad_dft: null model");
+    if (nfreqs == 0) {
+        m->dft.nf = 0;
+        return 0;
+    }
+    const seis_csts &c = m->c;
+    if (nfreqs < 0 || nfreqs > kMaxDftFreqs || !freqs)
+        return fail("seis_grad_dft: give 1 to 512 gradient frequencies");
+    for (int f = 0; f < nfreqs; ++f)
+        if (!(freqs[f] > 0)) return fail("seis_grad_dft: gradient frequencies must be positive");
+    if (c.ND != 2 && c.ND != 3)
+        return fail("seis_grad_dft: the DFT gradient needs the elastic or viscoelastic case "
+                    "(ND 2 or 3)");
+    if (m->prec != P_FP32) return fail("seis_grad_dft: the DFT gradient runs in FP32 (FP16=0)");
+    if (c.param_type == 3 && c.L == 0)
+        return fail("seis_grad_dft: param_type 3 needs a viscoelastic model (L > 0)");
+    if (c.restype != 0 || c.Hout || c.scalerms || c.scaleshot || c.scalermsnorm)
+        return fail("seis_grad_dft: the DFT gradient takes the L2 misfit without trace scaling, "
+                    "no Hessian and no RTM (restype 0, Hout 0, scalerms/scaleshot/scalermsnorm 0)");
+    m->dft.freqs.assign(freqs, freqs + nfreqs);
+    m->dft.nf = nfreqs;
+    dft_sampling(m);
+    // bin 0 (a frequency below the record's step df) would divide the
+    // memory-variable terms by zero
+    for (float b : m->dft.bins)
+        if (b < 1) {
+            m->dft.nf = 0;
+            return fail("seis_grad_dft: gradient frequencies below the frequency step of the "
+                        "record (bin 0)");
+        }
+    // the rms bins must stay below the Nyquist frequency of the record
+    const int tmax = tmax_of(m), nfft = fft_fast_size(tmax);
+    for (float f : m->dft.freqs)
+        if ((int)(f * nfft * c.dt + 1) > nfft / 2) {
+            m->dft.nf = 0;
+            return fail("seis_grad_dft: gradient frequencies above the Nyquist frequency of the "
+                        "record");
+        }
+    return 0;
+}
 
 int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float *src, int nrec,
                       const float *rec_pos, float *out) {
     g_err.clear();
     if (grad_setup(m)) return 1;
     auto &G = m->grad;
+    if (m->dft.nf) {
+        if (dft_prepare(m)) return 1;
+        if (seis_shot_bind(m, nsrc, src_pos, src, nrec, rec_pos)) return 1;
+        if (seis_launch_init(m)) return 1;
+        return forward_dft(m, out);
+    }
     if (G.visco) {
         if (seis_shot_bind(m, nsrc, src_pos, src, nrec, rec_pos)) return 1;
         if (seis_launch_init(m)) return 1;
@@ -2018,7 +2251,8 @@ int seis_misfit(const seis_csts *c, int nout, int nrec, const float *out, const 
 
 int seis_residuals(const seis_model *m, int nrec, const float *out, const float *din, float *res,
                    double *rms, double *rmsnorm) {
-    const double r = residuals(m->c, (int)m->outn.size(), nrec, out, din, res, rmsnorm);
+    double r = residuals(m->c, (int)m->outn.size(), nrec, out, din, res, rmsnorm);
+    if (m->dft.nf) r = dft_rms(m->c, (int)m->outn.size(), nrec, out, din, m->dft.freqs, rmsnorm);
     if (rms) *rms = r;
     return 0;
 }
@@ -2191,6 +2425,10 @@ static int adjoint_run(seis_model *m, const float *res, const MovieOut *mv, bool
     if (nres && !dev) HIP_TRY(hipMemcpyAsync(G.res, res, nres * 4, hipMemcpyHostToDevice, st));
     if (!rs.empty()) HIP_TRY(hipStreamSynchronize(st));
     grad_zero_adjoint(m);
+    if (m->dft.nf) {
+        if (mv) return fail("seis_adjoint_movie: no movie of a DFT-gradient run");
+        return adjoint_dft(m);
+    }
     return m->prec == P_FP64 ? adjoint_loops<double>(m, mv) : adjoint_loops<float>(m, mv);
 }
 
@@ -2274,6 +2512,13 @@ int seis_grad_read(seis_model *m, float *grad) {
     std::vector<double> acc[3];
     const double *src[3] = {G.gM, G.gmu, G.grho};
     if (accum_read(m, src, 3, acc)) return 1;
+    if (m->dft.nf) {
+        // the DFT gradient is already in the parameterisation (no transf_grad)
+        const size_t n = acc[0].size();
+        for (int q = 0; q < 3; ++q)
+            for (size_t i = 0; i < n; ++i) grad[q * n + i] = (float)acc[q][i];
+        return 0;
+    }
     const bool f0 = G.visco && (m->c.gradfix & FIX_VISCO_F0);
     if (G.grz) {
         // per-node terms through the staggered averages onto the cells
@@ -2324,6 +2569,14 @@ int seis_grad_read_tau(seis_model *m, float *gtau) {
     std::vector<double> acc[2];
     const double *src[2] = {G.gtp, G.gts};
     if (accum_read(m, src, 2, acc)) return 1;
+    if (m->dft.nf) {
+        const size_t n = acc[0].size();
+        for (size_t i = 0; i < n; ++i) {
+            gtau[i] = (float)acc[0][i];
+            gtau[n + i] = (float)acc[1][i];
+        }
+        return 0;
+    }
     // the accumulators hold dt^2/dh times dJ/dtau, as the moduli's hold
     // dt^2/dh times the derivative by the scaled moduli (transf_grad)
     const double k = (double)m->c.dh / ((double)m->c.dt * m->c.dt);
@@ -2359,6 +2612,8 @@ int seis_gradient_shot(seis_model *m, int nsrc, const float *src_pos, const floa
     double rr[2] = {0, 0};
     if (res_in) std::copy(res_in, res_in + res.size(), res.begin());
     else rr[0] = residuals(m->c, nout, nrec, o.data(), din, res.data(), &rr[1]);
+    if (!res_in && m->dft.nf)
+        rr[0] = dft_rms(m->c, nout, nrec, o.data(), din, m->dft.freqs, &rr[1]);
     res_scale(m->c, m->rp, m->outn, nrec, rec_pos, res.data(), m->c.gradfix & FIX_RES_SCALE);
     if (seis_adjoint(m, res.data())) return 1;
     if (out) std::copy(o.begin(), o.end(), out);

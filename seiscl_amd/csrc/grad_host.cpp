@@ -206,6 +206,43 @@ double residuals(const seis_csts &c, int nout, int nrec, const float *out, const
     return rms;
 }
 
+double dft_rms(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
+               const std::vector<float> &freqs, double *rmsnorm) {
+    const int NT = c.NT, tmax = c.tmax > 0 ? c.tmax : c.NT, nfft = fft_fast_size(tmax);
+    const size_t n = (size_t)nout * nrec * NT;
+    std::vector<float> vo(out, out + n), vi(din, din + n);
+    if (c.fmax > 0 || c.fmin > 0)
+        for (int o = 0; o < nout; ++o) {
+            butterworth(vo.data() + (size_t)o * nrec * NT, c.fmin, c.fmax, c.dt, NT, tmax, nrec, 6);
+            butterworth(vi.data() + (size_t)o * nrec * NT, c.fmin, c.fmax, c.dt, NT, tmax, nrec, 6);
+        }
+    std::vector<int> kb;
+    for (float f : freqs) kb.push_back((int)(f * nfft * c.dt + 1));
+    const double pi = 3.141592653589793238462643383279502884197169;
+    double rms = 0, rnorm = 0;
+    for (int g = 0; g < nrec; ++g)
+        for (int o = 0; o < nout; ++o) {
+            const float *a = vo.data() + ((size_t)o * nrec + g) * NT;
+            const float *b = vi.data() + ((size_t)o * nrec + g) * NT;
+            for (int k : kb) {
+                double rr = 0, ri = 0, nr = 0, ni = 0;
+                for (int t = 0; t < tmax; ++t) {
+                    const double ph = 2.0 * pi * (double)(((long)k * t) % nfft) / nfft;
+                    const double cs = std::cos(ph), sn = std::sin(ph);
+                    const float r = -a[t] + b[t];
+                    rr += r * cs;
+                    ri -= r * sn;
+                    nr += b[t] * cs;
+                    ni -= b[t] * sn;
+                }
+                rms += rr * rr + ri * ri;
+                rnorm += nr * nr + ni * ni;
+            }
+        }
+    if (rmsnorm) *rmsnorm = rnorm;
+    return rms;
+}
+
 // res_scale (src/residuals.c:452-570), FP32 storage (par_scale 0).
 // Velocity outputs are scaled by dt / (dh / (dt * buoyancy)), with the
 // buoyancy read at (z=x, y=x, x=x) of the receiver's x index as the reference

@@ -27,6 +27,8 @@ namespace seis {
 
 constexpr int kMaxL = SEIS_MAXL;
 constexpr int kMaxNab = 240;
+constexpr int kMaxDftFreqs = 512;   // gradient frequencies of the DFT gradient
+constexpr double kPiDft = 3.141592653589793238462643383279502884197169;
 
 struct Layout {
     int nd = 2;            // 2 or 3 (acoustic is 2)
@@ -372,6 +374,26 @@ struct AdjArgs {
     double *gry = nullptr, *grz = nullptr;
 };
 
+// DFT-domain gradient (dft.hip).  The arrays whose spectra are kept:
+// velocities, stresses, then each memory-variable component's L slabs.
+struct DftArrays {
+    int n = 0;
+    const float *p[9 + 6 * kMaxL] = {};
+};
+// F (complex float pairs) [n][nf][nfield] += fact[f] u; fact: nf pairs
+void launch_dft_acc(const DftArrays &q, float *F, const float *fact, int nf, const Layout &g,
+                    void *stream);
+struct DftGrad {
+    const void *Ff = nullptr, *Fa = nullptr;   // forward, adjoint spectra
+    const float *par = nullptr;   // [5][N] unpadded: M mu taup taus rho (buoyancy)
+    const float *w = nullptr;     // [nf] angular frequencies 2 pi df bin
+    double *gM = nullptr, *gmu = nullptr, *grho = nullptr, *gtp = nullptr, *gts = nullptr;
+    int nf = 0, L = 0, pt = 0, ntnyq = 1;
+    float al = 0;
+    float tausig[kMaxL] = {};
+};
+void launch_dft_grad(const DftGrad &d, const Layout &g, void *stream);
+
 template <typename R>
 void launch_adj_s(const AdjArgs<R> &a, int nd_case, void *stream);
 template <typename R>
@@ -399,6 +421,12 @@ double residuals(const seis_csts &c, int nout, int nrec, const float *out, const
 void butterworth(float *data, float fcl, float fch, float dt, int NT, int tmax, int ntrace,
                  int order);
 int fft_fast_size(int m);
+// rms and rmsnorm at the DFT gradient's frequencies (src/residuals.c:364-403):
+// |X(k)|^2 of the traces -out + din and din (band-passed like the
+// residuals, zero-padded to fft_fast_size(tmax)) at k = (int)(f nfft dt + 1),
+// the DFT bin evaluated directly in double.
+double dft_rms(const seis_csts &c, int nout, int nrec, const float *out, const float *din,
+               const std::vector<float> &freqs, double *rmsnorm);
 void res_scale(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,
                int nrec, const float *rec_pos, float *res, int at_receiver);
 void res_scale_factors(const seis_csts &c, const RefParams &p, const std::vector<std::string> &outn,

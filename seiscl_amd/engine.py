@@ -450,6 +450,13 @@ class Model:
         names = PARAM_NAMES.get(self.csts.param_type, ("vp", "vs", "rho"))
         return {nm: from_c_layout(h[q * n:(q + 1) * n], self.shape) for q, nm in enumerate(names)}
 
+    def grad_dft(self, freqs=None):
+        """Select the DFT-domain gradient (back_prop_type 2) at the given
+        frequencies (Hz) for the next gradient runs; None or an empty list
+        returns to backpropagation (seis_grad_dft)."""
+        f = np.ascontiguousarray(np.asarray([] if freqs is None else freqs, np.float32).ravel())
+        check(self.lib.seis_grad_dft(self.handle, int(f.size), fptr(f) if f.size else None))
+
     def gradient(self, src_pos, src, rec_pos, din=None, res=None):
         """One shot of gradient (added to the accumulators): forward with
         boundary saving, residuals against din (or adjoint sources res),

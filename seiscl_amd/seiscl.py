@@ -332,9 +332,16 @@ class SeisCL:
                for n in names if nframe and (n != "p" or self.ND == 22)}
         mov_shots = []
         if grad:
-            if self.back_prop_type != 1:
+            if self.back_prop_type not in (1, 2):
                 m.close()
-                raise SeisCLError("only back_prop_type=1 (backpropagation) is available")
+                raise SeisCLError("back_prop_type must be 1 (backpropagation) or 2 (DFT)")
+            if self.back_prop_type == 2:
+                # gradient from the DFT of the wavefields at gradfreqs
+                # (src/assign_modeling_case.c:549-570, src/calc_grad.c:377-968)
+                if self.inputres or self.movout > 0:
+                    m.close()
+                    raise SeisCLError("inputres and movout need back_prop_type=1")
+                m.grad_dft(np.asarray(self.gradfreqs, np.float32).ravel())
             m.grad_zero()
         if misfit and (self._din is None or any(n not in self._din for n in names)):
             m.close()

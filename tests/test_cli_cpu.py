@@ -100,7 +100,8 @@ def test_cli_input_errors(tmp_path, capfd):
     csts = base + "_csts.mat"
     # features outside this engine are refused, not ignored
     for key, val, msg in [("gradsrcout", 1, "source gradient"), ("FP16", 5, "FP16=5 is not a precision mode"),
-                          ("restype", 2, "restype=2"), ("back_prop_type", 2, "back_prop_type=2")]:
+                          ("restype", 2, "restype=2"), ("back_prop_type", 3, "back_prop_type=3"),
+                          ("back_prop_type", 2, "gradfreqs holds no frequency")]:
         old = getattr(seis, key)
         setattr(seis, key, val)
         if key == "back_prop_type":

@@ -376,3 +376,31 @@ def test_inputres_two_pass(oracle_lib, tmp_path, files, gradfix, L):
         ref = _cropped(tot[name], seis.nab)
         assert np.abs(ref).max() > 0, name
         assert rel_l2(g, ref) <= 1e-5, (name, rel_l2(g, ref))
+
+
+@pytest.mark.parametrize("files", [True, False])
+def test_dft_gradient_files(oracle_lib, tmp_path, files):
+    """back_prop_type=2 with /gradfreqs (src/calc_grad.c:377-968): the DFT
+    gradient through the files (SeisCL_MPI) and in process, two shots,
+    against the oracle's DFT gradient summed over the shots."""
+    freqs = np.array([6.0, 12.0, 20.0], np.float32)
+    seis, model, true, csts = _fwi_case(tmp_path, files=files, back_prop_type=2,
+                                        gradfreqs=freqs)
+    print(f"[cli] DFT gradient files={files}", flush=True)
+    seis.set_forward([0, 1], true, withgrad=False)
+    seis.execute()
+    dobs = seis.read_data()
+    seis.write_data({"vx": dobs[0], "vz": dobs[1]})
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.execute()
+    gout = seis.read_grad()
+    tot = {k: 0 for k in seis.params}
+    for rc, shot in _shots(seis):
+        din = [d[:, rc] for d in dobs]
+        g = oracle_lib.gradient_dft(model, *shot, freqs, din=din, **csts)
+        for k in tot:
+            tot[k] = tot[k] + g["grad"][k]
+    for name, g in zip(seis.params, gout):
+        ref = _cropped(tot[name], seis.nab)
+        assert np.abs(ref).max() > 0, name
+        assert rel_l2(g, ref) <= 1e-5, name
