@@ -178,6 +178,7 @@ def serving_main(args):
         loads[f"{f}x"] = {"plain": p, "batched": b}
         log(f"load {f}x: plain {p['shots_per_hour']:.0f}/h p99 {p['p99_latency_s']:.2f} s; "
             f"batched {b['shots_per_hour']:.0f}/h p99 {b['p99_latency_s']:.2f} s")
+    mixed = serving_mixed(ctx, work, args.grad_share, args.slots) if args.grad_share > 0 else None
     ctx.close()
     value = cap_batch["shots_per_hour"]
     out = {
@@ -193,6 +194,7 @@ def serving_main(args):
             "capacity_plain": cap_plain, "capacity_batched": cap_batch,
             "speedup_capacity": cap_batch["shots_per_hour"] / cap_plain["shots_per_hour"],
             "loads": loads,
+            "with_gradients": mixed,
         },
     }
     print(json.dumps(out), flush=True)
@@ -514,6 +516,9 @@ def main():
                     help="gradient2d: finite-difference check on this many cells per parameter")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
     ap.add_argument("--slots", type=int, default=48, help="serving: batch slots")
+    ap.add_argument("--grad-share", type=float, default=0.125,
+                    help="serving: share of the elastic requests sent as gradient requests "
+                         "in the mixed measurement (0: none)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="forward3d: time steps (1500); gradient2d: whole gradients (2)")
@@ -559,6 +564,40 @@ def main():
     forward3d_main(args)
 
 
+def gradient_mix(ctx, work, share):
+    """Observed seismograms for a share of the elastic requests (a model 2%
+    faster in vp), which makes them gradient requests; untimed set-up."""
+    from seiscl_amd import Model
+    observed = [None] * len(work)
+    el = [i for i, (c, _, _) in enumerate(work) if int(c["ND"]) == 2]
+    step = max(1, int(round(1.0 / share))) if share > 0 else 0
+    for i in (el[::step] if step else []):
+        csts, params, shot = work[i]
+        true = {k: (v * 1.02 if k == "vp" else v) for k, v in params.items()}
+        m = Model(ctx, true, **csts)
+        observed[i] = m.forward(*shot)
+        m.close()
+    return observed
+
+
+def serving_mixed(ctx, work, share, slots):
+    """Capacity with gradient requests in the mix: the plain path runs every
+    request alone in arrival order; the batched path batches the modeling
+    requests and runs the gradient requests beside the batch."""
+    from seiscl_amd.serving import make_requests, serve_batched, serve_plain, summarize
+    obs = gradient_mix(ctx, work, share)
+    ngrad = sum(o is not None for o in obs)
+    p = summarize(serve_plain(ctx, make_requests(work, observed=obs)))
+    b = summarize(serve_batched(ctx, make_requests(work, observed=obs), max_slots=slots))
+    keep = ("shots_per_hour", "p50_latency_s", "p99_latency_s", "span_s")
+    pick = lambda d: {k: round(d[k], 4) for k in keep if k in d}   # noqa: E731
+    log(f"mixed ({ngrad} gradient requests of {len(work)}): plain {p['shots_per_hour']:.0f}/h, "
+        f"batched {b['shots_per_hour']:.0f}/h")
+    return {"gradient_requests": ngrad, "requests": len(work), "capacity_plain": pick(p),
+            "capacity_batched": pick(b),
+            "speedup_capacity": round(b["shots_per_hour"] / p["shots_per_hour"], 3)}
+
+
 def serving_sample(ctx, nreq=64, slots=48):
     """Bounded serving measurement for the default line: capacity of the
     plain and batched paths (all requests queued at t=0) and a Poisson load
@@ -584,6 +623,7 @@ def serving_sample(ctx, nreq=64, slots=48):
         "capacity_plain": pick(cap_p), "capacity_batched": pick(cap_b),
         "speedup_capacity": round(cap_b["shots_per_hour"] / cap_p["shots_per_hour"], 3),
         "load_1x_plain": pick(ld_p), "load_1x_batched": pick(ld_b),
+        "with_gradients": serving_mixed(ctx, work, 0.125, slots),
     }
 
 
