@@ -271,6 +271,8 @@ def gradient_main(args):
     from seiscl_amd.autograd import Propagator, seismic
     from seiscl_amd.workloads import config3
     csts, true, start, shots = config3(NT=args.nt or 3000)
+    if args.gradfix is not None:
+        csts = dict(csts, gradfix=args.gradfix)
     NT, ncell = int(csts["NT"]), int(np.prod(csts["N"]))
     ctx = Context(0)
     t0 = time.time()
@@ -317,6 +319,7 @@ def gradient_main(args):
         "workload": ("2D elastic FWI gradient through torch.autograd, 300x800, FD8, NT "
                      f"{NT}, 16 shots, CPML 16, boundary-saving recomputation"),
         "grid": [int(v) for v in csts["N"]], "fd_order": 8, "shots": len(shots), "NT": NT,
+        "gradfix": int(csts.get("gradfix", 223)),
         "forward_s": round(fwd, 4), "backward_s": round(bwd, 4), "misfit_s": round(misfit, 4),
         "forward_gcell_s": round(cells / fwd / 1e9, 3),
         "backward_gcell_s": round(cells / bwd / 1e9, 3),
@@ -494,6 +497,7 @@ def fd_check(ctx, csts, start, shots, din, grad, ncells):
         res[name] = {"cells": ncells,
                      "rel_l2": float(np.linalg.norm(gr - fd) / np.linalg.norm(fd)),
                      "median_ratio": float(np.median(gr / fd))}
+        log(f"finite differences {name}: rel L2 {res[name]['rel_l2']:.3e} over {ncells} cells")
         log(f"finite differences {name}: {res[name]}")
     m64.close()
     return res
@@ -512,6 +516,9 @@ def main():
     ap.add_argument("--check-fp64", action="store_true",
                     help="gradient3d: also rerun shot 0 in FP64 and report both errors")
     ap.add_argument("--nt", type=int, default=0, help="gradient2d: time steps (default 3000)")
+    ap.add_argument("--gradfix", type=int, default=None,
+                    help="gradient2d: seis_csts.gradfix (default 223; 255 adds the chain "
+                         "through the staggered averages)")
     ap.add_argument("--fd-cells", type=int, default=0,
                     help="gradient2d: finite-difference check on this many cells per parameter")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
