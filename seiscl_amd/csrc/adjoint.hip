@@ -101,6 +101,14 @@ __device__ __forceinline__ bool adj_cell(const Layout &g, int nbz, unsigned nb, 
     return z < g.NZ && x < g.NX;
 }
 
+// The boundary shell of this step: A.bnd, or, in replayed step graphs,
+// A.bnd + (*A.tb + A.bofs) steps (a device step counter plus the step's
+// offset within the graph).
+template <typename R>
+__device__ __forceinline__ const R *adj_bnd(const AdjArgs<R> &A) {
+    return A.tb ? A.bnd + ((long)*A.tb + A.bofs) * A.bper : A.bnd;
+}
+
 // ---------------------------------------------------------------------------
 // 2D
 // ---------------------------------------------------------------------------
@@ -281,9 +289,10 @@ __global__ __launch_bounds__(256) void k_adj_s2d(AdjArgs<R> A, int nbz, unsigned
         const long m = A.sh.inject(z + H, 0, x + H);
         if (m >= 0) {
             const long n = A.sh.n;
-            sxx = A.bnd[2 * n + m];
-            szz = A.bnd[3 * n + m];
-            sxz = A.bnd[4 * n + m];
+            const R *bnd = adj_bnd(A);
+            sxx = bnd[2 * n + m];
+            szz = bnd[3 * n + m];
+            sxz = bnd[4 * n + m];
         }
         f.sxx[i] = sxx;
         f.szz[i] = szz;
@@ -337,8 +346,9 @@ __global__ __launch_bounds__(256) void k_adj_v2d(AdjArgs<R> A, int nbz, unsigned
         vz -= ((szz_z + sxz_x) * rkp);
         const long m = A.sh.inject(z + H, 0, x + H);
         if (m >= 0) {
-            vx = A.bnd[m];
-            vz = A.bnd[A.sh.n + m];
+            const R *bnd = adj_bnd(A);
+            vx = bnd[m];
+            vz = bnd[A.sh.n + m];
         }
         f.vx[i] = vx;
         f.vz[i] = vz;
@@ -408,12 +418,13 @@ __global__ __launch_bounds__(256) void k_adj_s3d(AdjArgs<R> A, int nbz, unsigned
         const long m = A.sh.inject(z + H, y + H, x + H);
         if (m >= 0) {
             const long n = A.sh.n;
-            sxx = A.bnd[3 * n + m];
-            syy = A.bnd[4 * n + m];
-            szz = A.bnd[5 * n + m];
-            sxy = A.bnd[6 * n + m];
-            syz = A.bnd[7 * n + m];
-            sxz = A.bnd[8 * n + m];
+            const R *bnd = adj_bnd(A);
+            sxx = bnd[3 * n + m];
+            syy = bnd[4 * n + m];
+            szz = bnd[5 * n + m];
+            sxy = bnd[6 * n + m];
+            syz = bnd[7 * n + m];
+            sxz = bnd[8 * n + m];
         }
         own_st(f.sxx + i, sxx);
         own_st(f.syy + i, syy);
@@ -494,9 +505,10 @@ __global__ __launch_bounds__(256) void k_adj_v3d(AdjArgs<R> A, int nbz, unsigned
         const long m = A.sh.inject(z + H, y + H, x + H);
         if (m >= 0) {
             const long n = A.sh.n;
-            vx = A.bnd[m];
-            vy = A.bnd[n + m];
-            vz = A.bnd[2 * n + m];
+            const R *bnd = adj_bnd(A);
+            vx = bnd[m];
+            vy = bnd[n + m];
+            vz = bnd[2 * n + m];
         }
         own_st(f.vx + i, vx);
         own_st(f.vy + i, vy);
@@ -872,9 +884,10 @@ __global__ __launch_bounds__(256) void k_adjh_v3d(AdjArgs<R> A, int nbz, unsigne
 // Boundary saving and adjoint sources
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_savebnd(FDArgs<R> f, Shell sh, R *bnd) {
+__global__ void k_savebnd(FDArgs<R> f, Shell sh, R *bnd, const int *tb, long per) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= sh.n) return;
+    if (tb) bnd += (long)*tb * per;   // replayed step graphs: a device step counter
     int k, j, i;
     sh.cell(gid, k, j, i);
     const int H = f.g.H;
@@ -922,7 +935,8 @@ struct CodeList {
 // serial: one lane walks every receiver in order (receivers may share cells)
 template <typename R>
 __global__ void k_residuals(FDArgs<R> a, int nd_case, ShotDev s, CodeList o, const float *res,
-                            int NT, int nt, int serial) {
+                            int NT, int nt, int serial, const int *tbase) {
+    if (tbase) nt += *tbase;
     const int r0 = blockIdx.x * blockDim.x + threadIdx.x;
     const int r1 = serial ? s.nrec : r0 + 1;
     if (serial && r0 != 0) return;
@@ -1124,22 +1138,25 @@ void launch_adjh_v(const AdjArgs<R> &a, int nd_case, bool half, void *stream) {
     SEIS_SWITCH_H(a.f.g.H, (run_adjh_v<R, R, HH>(a, nd_case, (hipStream_t)stream)));
 }
 template <typename R>
-void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream) {
+void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream,
+                    const int *tb, long per) {
     (void)nd_case;
     if (sh.n <= 0) return;
     const unsigned blocks = (unsigned)((sh.n + 255) / 256);
-    hipLaunchKernelGGL((k_savebnd<R>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, f, sh, bnd);
+    hipLaunchKernelGGL((k_savebnd<R>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, f, sh, bnd,
+                       tb, per);
 }
 template <typename R>
 void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const int *codes,
-                      int nout, const float *res, int NT, int nt, int serial, void *stream) {
+                      int nout, const float *res, int NT, int nt, int serial, void *stream,
+                      const int *tbase) {
     if (s.nrec <= 0) return;
     CodeList o;
     o.n = nout;
     for (int q = 0; q < nout && q < 10; ++q) o.code[q] = codes[q];
     const unsigned blocks = serial ? 1u : (unsigned)((s.nrec + 255) / 256);
     hipLaunchKernelGGL((k_residuals<R>), dim3(blocks), dim3(serial ? 64 : 256), 0,
-                       (hipStream_t)stream, r, nd_case, s, o, res, NT, nt, serial);
+                       (hipStream_t)stream, r, nd_case, s, o, res, NT, nt, serial, tbase);
 }
 
 template void launch_adj_s<float>(const AdjArgs<float> &, int, void *);
@@ -1147,17 +1164,19 @@ template void launch_surface_adj<float>(const FDArgs<float> &, float *, int, int
 template void launch_adj_v<float>(const AdjArgs<float> &, int, void *);
 template void launch_adjh_s<float>(const AdjArgs<float> &, int, int, bool, void *);
 template void launch_adjh_v<float>(const AdjArgs<float> &, int, bool, void *);
-template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *);
+template void launch_savebnd<float>(const FDArgs<float> &, const Shell &, int, float *, void *,
+                                    const int *, long);
 template void launch_residuals<float>(const FDArgs<float> &, int, const ShotDev &, const int *,
-                                      int, const float *, int, int, int, void *);
+                                      int, const float *, int, int, int, void *, const int *);
 // FP64 gradient runs
 template void launch_adj_s<double>(const AdjArgs<double> &, int, void *);
 template void launch_surface_adj<double>(const FDArgs<double> &, double *, int, int, void *);
 template void launch_adj_v<double>(const AdjArgs<double> &, int, void *);
 template void launch_adjh_s<double>(const AdjArgs<double> &, int, int, bool, void *);
 template void launch_adjh_v<double>(const AdjArgs<double> &, int, bool, void *);
-template void launch_savebnd<double>(const FDArgs<double> &, const Shell &, int, double *, void *);
+template void launch_savebnd<double>(const FDArgs<double> &, const Shell &, int, double *, void *,
+                                     const int *, long);
 template void launch_residuals<double>(const FDArgs<double> &, int, const ShotDev &, const int *,
-                                       int, const float *, int, int, int, void *);
+                                       int, const float *, int, int, int, void *, const int *);
 
 }  // namespace seis

@@ -188,6 +188,20 @@ struct seis_model {
     hipGraphExec_t sg_exec = nullptr;
     uint64_t sg_key = 0;
     int *sg_t = nullptr;
+    // captured steps of the 2D elastic gradient: the forward run with
+    // boundary saving and the adjoint run (same step counter)
+    struct Graph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        uint64_t key = 0;
+        void reset() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+            e = nullptr;
+            g = nullptr;
+            key = 0;
+        }
+    } fs_graph, ad_graph;
     std::vector<int> rcell_host;     // receiver cells of the bound shot
     std::vector<float> recpos_host;  // receiver positions of the bound shot [nrec][8]
     // gradient state, allocated on first use
@@ -259,6 +273,8 @@ struct seis_model {
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (sg_exec) (void)hipGraphExecDestroy(sg_exec);
         if (sg_graph) (void)hipGraphDestroy(sg_graph);
+        fs_graph.reset();
+        ad_graph.reset();
     }
 };
 
@@ -651,17 +667,49 @@ void graph_reset(seis_model *m) {
     m->sg_key = 0;
 }
 
+// The device step counter of replayed step graphs.
+int step_counter(seis_model *m) {
+    if (m->sg_t) return 0;
+    void *p = nullptr;
+    if (m->ctx->alloc(256, &p)) return 1;
+    m->sg_t = (int *)p;
+    return 0;
+}
+
+// key mixed with n more bytes
+uint64_t key_mix(uint64_t h, const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// Capture body() on stream st into G unless G already holds a graph of the
+// same key.  body only enqueues kernels.
+template <typename F>
+int capture_steps(seis_model::Graph &G, uint64_t key, hipStream_t st, F &&body) {
+    if (G.e && G.key == key) return 0;
+    G.reset();
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(std::string("step graph capture: ") + hipGetErrorString(e));
+    }
+    G.g = g;
+    HIP_TRY(hipGraphInstantiate(&G.e, g, nullptr, nullptr, 0));
+    G.key = key;
+    return 0;
+}
+
 // Capture (or reuse) the graph of kGraphSteps steps for the model's current
 // arguments on stream st.
 int graph_prepare(seis_model *m, hipStream_t st) {
     const uint64_t key = step_key(m);
     if (m->sg_exec && m->sg_key == key) return 0;
     graph_reset(m);
-    if (!m->sg_t) {
-        void *p = nullptr;
-        if (m->ctx->alloc(256, &p)) return 1;
-        m->sg_t = (int *)p;
-    }
+    if (step_counter(m)) return 1;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     for (int k = 0; k < kGraphSteps; ++k) step_all(m, k, st, m->sg_t);
     launch_tcounter(m->sg_t, kGraphSteps, 1, st);
@@ -2152,17 +2200,39 @@ int seis_forward_save(seis_model *m, int nsrc, const float *src_pos, const float
     const int fused = m->a.fuse_surface;
     m->a.fuse_surface = 0;
     m->ad.fuse_surface = 0;   // the FP64 view of the same arguments
-    for (int t = 0; t < tmax; ++t) {
-        step_sources(m, t, 1, st);
+    // step t (relative to *tb when tb is given): varsout of t-1 with the
+    // sources of t, the updates, the shell of t, the free surface
+    auto step = [&](int t, const int *tb) {
+        step_io(m, t - 1, t, st, tb);
         step_v(m, st);
         step_s(m, st);
         if (m->prec == P_FP64)
-            launch_savebnd(m->ad, G.sh, m->nd_case, (double *)G.bnd + (size_t)t * per, st);
+            launch_savebnd(m->ad, G.sh, m->nd_case, (double *)G.bnd + (long)t * (long)per, st,
+                           tb, (long)per);
         else
-            launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (size_t)t * per, st);
+            launch_savebnd(m->a, G.sh, m->nd_case, G.bnd + (long)t * (long)per, st, tb,
+                           (long)per);
         if (m->c.freesurf) step_surface(m, st);
-        step_varsout(m, t, st);
+    };
+    int t0 = 0;
+    // 2D: runs of kGraphSteps steps replayed from a captured graph
+    if (m->c.ND != 3 && !m->profiling && tmax >= kGraphSteps && graph_enabled()) {
+        if (step_counter(m)) return 1;
+        uint64_t key = step_key(m);
+        key = key_mix(key, &G.bnd, sizeof(G.bnd));
+        key = key_mix(key, &G.sh, sizeof(G.sh));
+        key = key_mix(key, &per, sizeof(per));
+        if (capture_steps(m->fs_graph, key, st, [&] {
+                for (int k = 0; k < kGraphSteps; ++k) step(k, m->sg_t);
+                launch_tcounter(m->sg_t, kGraphSteps, 1, st);
+            }))
+            return 1;
+        launch_tcounter(m->sg_t, 0, 0, st);
+        for (; t0 + kGraphSteps <= tmax; t0 += kGraphSteps)
+            HIP_TRY(hipGraphLaunch(m->fs_graph.e, st));
     }
+    for (int t = t0; t < tmax; ++t) step(t, nullptr);
+    step_io(m, tmax - 1, -1, st);   // varsout of the last step
     m->a.fuse_surface = fused;
     m->ad.fuse_surface = fused;
     HIP_TRY(hipGetLastError());
@@ -2322,20 +2392,66 @@ static int adjoint_loops(seis_model *m, const MovieOut *mv) {
     if (!e0 || !e1 || (prof && !pool_event(m, 1 + 4 * nst))) return fail("hipEventCreate failed");
     HIP_TRY(hipEventRecord(e0, st));
     size_t k = 0;
-    // adjoint loop of src/time_stepping.c:869-940
-    for (int t = tmax - 1; t > tmin; --t, ++k) {
+    const R *bnd0 = (const R *)G.bnd;
+    // adjoint step t (relative to *tb when tb is given; the shell of step
+    // t-1), src/time_stepping.c:869-940
+    auto step = [&](int t, const int *tb) {
         if (m->c.freesurf)
             launch_surface_adj(A.r, (R *)G.stmp, m->nd_case, m->c.gradfix & FIX_SURF_SYY, st);
         launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t, serial,
-                         st);
-        A.bnd = (const R *)G.bnd + (size_t)(t - 1) * per;
-        if (prof) HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
+                         st, tb);
+        A.tb = tb;
+        A.bper = (long)per;
+        A.bofs = t - 1;
+        A.bnd = tb ? bnd0 : bnd0 + (long)(t - 1) * (long)per;
         launch_adj_s(A, m->nd_case, st);
-        if (prof) HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
-        if (prof) HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
         launch_adj_v(A, m->nd_case, st);
-        if (prof) HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
-        step_sources(m, t, -1, st);
+        step_sources(m, t, -1, st, tb);
+    };
+    int t = tmax - 1;
+    // 2D without profiling or movie: runs of kGraphSteps adjoint steps
+    // replayed from a captured graph, the counter holding the run's first t
+    if (m->c.ND != 3 && !prof && !mv && tmax - 1 - tmin >= kGraphSteps && graph_enabled()) {
+        if (step_counter(m)) return 1;
+        AdjArgs<R> K = A;
+        K.bnd = bnd0;
+        K.tb = m->sg_t;
+        K.bper = (long)per;
+        uint64_t key = step_key(m);
+        key = key_mix(key, &K, sizeof(K));
+        key = key_mix(key, &G.res, sizeof(G.res));
+        key = key_mix(key, &G.stmp, sizeof(G.stmp));
+        key = key_mix(key, &serial, sizeof(serial));
+        key = key_mix(key, &NT, sizeof(NT));
+        const int ss = sizeof(R);
+        key = key_mix(key, &ss, sizeof(ss));
+        if (capture_steps(m->ad_graph, key, st, [&] {
+                for (int j = 0; j < kGraphSteps; ++j) step(-j, m->sg_t);
+                launch_tcounter(m->sg_t, -kGraphSteps, 1, st);
+            }))
+            return 1;
+        launch_tcounter(m->sg_t, t, 0, st);
+        for (; t - kGraphSteps >= tmin; t -= kGraphSteps, k += kGraphSteps)
+            HIP_TRY(hipGraphLaunch(m->ad_graph.e, st));
+    }
+    for (; t > tmin; --t, ++k) {
+        if (!prof) {
+            step(t, nullptr);
+        } else {
+            if (m->c.freesurf)
+                launch_surface_adj(A.r, (R *)G.stmp, m->nd_case, m->c.gradfix & FIX_SURF_SYY, st);
+            launch_residuals(A.r, m->nd_case, m->shot, m->out_codes.data(), nout, G.res, NT, t,
+                             serial, st);
+            A.tb = nullptr;
+            A.bnd = bnd0 + (long)(t - 1) * (long)per;
+            HIP_TRY(hipEventRecord(m->evpool[2 + 4 * k], st));
+            launch_adj_s(A, m->nd_case, st);
+            HIP_TRY(hipEventRecord(m->evpool[3 + 4 * k], st));
+            HIP_TRY(hipEventRecord(m->evpool[4 + 4 * k], st));
+            launch_adj_v(A, m->nd_case, st);
+            HIP_TRY(hipEventRecord(m->evpool[5 + 4 * k], st));
+            step_sources(m, t, -1, st);
+        }
         if (movie_snap(m, mv, t)) return 1;
     }
     HIP_TRY(hipEventRecord(e1, st));

@@ -372,6 +372,9 @@ struct AdjArgs {
     // every term to the cell of the same index.
     double *gsh[3] = {nullptr, nullptr, nullptr};
     double *gry = nullptr, *grz = nullptr;
+    // replayed step graphs: the shell read is bnd + (*tb + bofs) * bper
+    const int *tb = nullptr;
+    long bofs = 0, bper = 0;
 };
 
 // DFT-domain gradient (dft.hip).  The arrays whose spectra are kept:
@@ -409,11 +412,15 @@ template <typename R>
 void launch_surface_adj(const FDArgs<R> &r, R *tmp, int nd_case, int syy_fix, void *stream);
 // bnd[q][g] = field_q(shell cell g), q over the shell variables
 template <typename R>
-void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream);
+// tb (may be NULL): the shell goes to bnd + *tb * per (replayed step graphs)
+void launch_savebnd(const FDArgs<R> &f, const Shell &sh, int nd_case, R *bnd, void *stream,
+                    const int *tb = nullptr, long per = 0);
 // adjoint sources of step nt into the adjoint wavefields; res [nout][nrec][NT]
 template <typename R>
+// tbase (may be NULL): device step counter added to nt (replayed step graphs)
 void launch_residuals(const FDArgs<R> &r, int nd_case, const ShotDev &s, const int *out_codes,
-                      int nout, const float *res, int NT, int nt, int serial, void *stream);
+                      int nout, const float *res, int NT, int nt, int serial, void *stream,
+                      const int *tbase = nullptr);
 
 // Host side of the gradient (src/residuals.c, src/calc_grad.c:975-1110).
 double residuals(const seis_csts &c, int nout, int nrec, const float *out, const float *din,

@@ -387,3 +387,42 @@ def test_staggered_chain_conserves_totals(gpu_ctx, case):
         a, b = g[255][k].astype(np.float64), g[223][k].astype(np.float64)
         assert abs(a.sum() - b.sum()) <= 1e-5 * np.abs(b).sum(), k
         assert rel_l2(a, b) > 1e-3, k
+
+
+@pytest.mark.parametrize("case", [dict(ND=2, FDORDER=4, abs_type=1),
+                                  dict(ND=2, FDORDER=8, freesurf=1),
+                                  dict(ND=2, FDORDER=4, abs_type=2, FP16=64),
+                                  dict(ND=2, FDORDER=4, tmin=0.0305)], ids=_ids)
+def test_gradient_graph_replays_equal_eager(oracle_lib, gpu_ctx, case):
+    """2D elastic gradients replay captured graphs of 64 forward steps with
+    boundary saving and of 64 adjoint steps (profiling off); with profiling
+    on every kernel is launched one by one.  Interleaved runs of both kinds
+    on the same model give the same seismograms and gradients bit for bit,
+    and (FP32) those equal the oracle's."""
+    from seiscl_amd import Model
+    case = dict(case)
+    fp16 = case.pop("FP16", 0)
+    tmin = case.pop("tmin", 0)
+    csts, params, (sp, s, rp), true = _setup(case)
+    if tmin:
+        csts["tmin"] = tmin
+    print(f"[grad graph] case={_ids(case)} FP16={fp16} tmin={tmin}", flush=True)
+    din = oracle_lib.forward(true, sp, s, rp, **csts)
+    m = Model(gpu_ctx, params, **dict(csts, FP16=fp16))
+    runs = []
+    for prof in (False, True, False, True, False):
+        m.set_profiling(prof)
+        m.grad_zero()
+        out, _ = m.gradient(sp, s, rp, din=din)
+        runs.append((out, m.grad_read()))
+        gpu_ctx.sync()
+    m.close()
+    for out, g in runs[1:]:
+        for a, b in zip(out, runs[0][0]):
+            np.testing.assert_array_equal(a, b)
+        for k in g:
+            np.testing.assert_array_equal(g[k], runs[0][1][k])
+    if fp16 == 0:
+        ref = oracle_lib.gradient(params, sp, s, rp, din=din, **csts)
+        for k in ref["grad"]:
+            assert rel_l2(runs[0][1][k], ref["grad"][k]) <= TOL, k
