@@ -205,7 +205,6 @@ static int read_csts(hid_t f, job_t *j) {
         if (j->movout) return err("movout with gradout needs back_prop_type=1");
     }
     /* features outside this engine fail loudly rather than being ignored */
-    if (j->gradsrcout) return err("source gradient output (gradsrcout) is not available");
     if (j->halfpar) return err("half-precision parameters (halfpar) are not available");
     if (j->restype != 0 && j->restype != 1)
         return err("restype=%d is not available (0: L2 residuals, 1: RTM)", j->restype);
@@ -484,6 +483,18 @@ static int write_outputs(const files_t *fs, job_t *j) {
                 snprintf(name, sizeof(name), "H%s", par_name(c->param_type, k) + 1);
                 s = write_f32(f, name, j->hess + (size_t)k * n, nd, d);
             }
+        }
+        if (!s && j->gradsrcout) {
+            /* /gradsrc [allns][NT] (src/writehdf5.c:285-288): the reference's
+             * source-gradient terms are commented out of its adjoint kernels
+             * (src/update_adjv2D.cl:341-357, src/update_adjv3D.cl:500-520,
+             * src/update_adjs2D.cl:480-495), so the array it writes holds the
+             * zeros of init_gradsrc; this build writes the same */
+            long long ds[2] = {j->allns, c->NT};
+            float *z = (float *)calloc((size_t)j->allns * c->NT + 1, sizeof(float));
+            if (!z) s = err("out of host memory");
+            else s = write_f32(f, "/gradsrc", z, 2, ds);
+            free(z);
         }
         seis_h5_close(f);
     }

@@ -99,7 +99,7 @@ def test_cli_input_errors(tmp_path, capfd):
     seis.set_forward([0, 1], model, withgrad=False)
     csts = base + "_csts.mat"
     # features outside this engine are refused, not ignored
-    for key, val, msg in [("gradsrcout", 1, "source gradient"), ("FP16", 5, "FP16=5 is not a precision mode"),
+    for key, val, msg in [("FP16", 5, "FP16=5 is not a precision mode"),
                           ("restype", 2, "restype=2"), ("back_prop_type", 3, "back_prop_type=3"),
                           ("back_prop_type", 2, "gradfreqs holds no frequency")]:
         old = getattr(seis, key)

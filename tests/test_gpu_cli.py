@@ -404,3 +404,26 @@ def test_dft_gradient_files(oracle_lib, tmp_path, files):
         ref = _cropped(tot[name], seis.nab)
         assert np.abs(ref).max() > 0, name
         assert rel_l2(g, ref) <= 1e-5, name
+
+
+def test_gradsrc_file(tmp_path):
+    """gradsrcout=1 writes /gradsrc [allns][NT] into the gradient file
+    (src/writehdf5.c:285-288).  The reference's source-gradient terms are
+    commented out of its adjoint kernels, so its array holds the zeros of
+    init_gradsrc; the file here holds the same, next to the usual
+    gradients."""
+    from seiscl_amd import h5mat
+    seis, model, true, csts = _fwi_case(tmp_path, gradsrcout=1)
+    seis.set_forward([0, 1], true, withgrad=False)
+    seis.execute()
+    dobs = seis.read_data()
+    seis.write_data({"vx": dobs[0], "vz": dobs[1]})
+    seis.set_forward([0, 1], model, withgrad=True)
+    seis.execute()
+    path = os.path.join(seis.workdir, seis.file_gout)
+    g = h5mat.loadmat(path)
+    assert "gradsrc" in g and "gradvp" in g
+    nsrc = int(np.sum(np.isin(seis.src_pos[3].astype(int), [0, 1])))
+    assert g["gradsrc"].size == nsrc * int(csts["NT"])
+    assert not np.any(g["gradsrc"])
+    assert np.abs(g["gradvp"]).max() > 0
