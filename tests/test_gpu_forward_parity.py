@@ -37,6 +37,11 @@ CASES = (
     + [dict(ND=3, FDORDER=o, abs_type=a, freesurf=f, L=l, N=(150, 28, 30))
        for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
     + [dict(ND=3, FDORDER=4, abs_type=0, freesurf=f, N=(150, 28, 30)) for f in (0, 1)]
+    # z extents that are multiples of 64: every wave full, z-derivatives
+    # taken from wave neighbours
+    + [dict(ND=3, FDORDER=o, abs_type=a, freesurf=f, L=l, N=(128, 20, 22))
+       for o in (2, 4, 8, 12) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
+    + [dict(ND=3, FDORDER=4, abs_type=0, freesurf=f, N=(64, 20, 22)) for f in (0, 1)]
 )
 
 

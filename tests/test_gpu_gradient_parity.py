@@ -30,6 +30,7 @@ CASES = (
     + [dict(ND=3, FDORDER=4, freesurf=1, gradfix=f) for f in (15, 7)]
     # tall 3D grid: several z tiles of 64 cells
     + [dict(ND=3, FDORDER=4, abs_type=a, N=(150, 28, 30)) for a in (1, 2)]
+    + [dict(ND=3, FDORDER=4, abs_type=1, freesurf=f, N=(128, 20, 22)) for f in (0, 1)]
 )
 # viscoelastic: forward state from checkpoints and recomputed segments
 VISCO = (
@@ -40,6 +41,7 @@ VISCO = (
     + [dict(ND=3, FDORDER=4, L=1, freesurf=1), dict(ND=3, FDORDER=8, L=2, gradfix=15)]
     # the benchmark gradient's case: 3D, L=2, FD4, free surface
     + [dict(ND=3, FDORDER=4, L=2, freesurf=1), dict(ND=3, FDORDER=4, L=2, freesurf=1, gradfix=223)]
+    + [dict(ND=3, FDORDER=4, L=2, freesurf=1, N=(128, 20, 22))]
 )
 CASES = CASES + VISCO
 

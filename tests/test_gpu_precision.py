@@ -21,6 +21,8 @@ CASES = ([dict(ND=3, FDORDER=o, abs_type=a, freesurf=f, L=l)
           for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
          # several z tiles tall
          + [dict(ND=3, FDORDER=4, abs_type=1, freesurf=1, L=2, N=(150, 28, 30))]
+         + [dict(ND=3, FDORDER=o, abs_type=1, freesurf=1, L=l, N=(128, 20, 22))
+            for o in (4, 8) for l in (0, 2)]
          # 2D P-SV (src/update_{v,s}2D_half2.cl) and 2D acoustic
          + [dict(ND=2, FDORDER=o, abs_type=a, freesurf=f, L=l)
             for o in (4, 8) for a in (1, 2) for f in (0, 1) for l in (0, 2)]
