@@ -62,6 +62,12 @@ constexpr int kNTS = SEIS_NT_S;
 #endif
 constexpr int kNTV = SEIS_NT_V;
 
+// CPML memory variables of both updates: non-temporal (1) or plain (0).
+#ifndef SEIS_NT_PSI
+#define SEIS_NT_PSI 1
+#endif
+constexpr bool kNTP = SEIS_NT_PSI;
+
 // Interleave the y slabs over the XCDs (1) or give each XCD a contiguous
 // run of them (0).
 #ifndef SEIS_SLAB_IL
@@ -161,19 +167,19 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb, int yg) {
     const long ipx = (long)ii * NY * NZ + (long)y * NZ + z;
     Pml<R> z0, z1, z2, y0, y1, y2, x0, x1, x2;
     if (pz) {
-        z0 = pml_load(a, a.psi_sxz_z + ipz, 2, false, ki);
-        z1 = pml_load(a, a.psi_syz_z + ipz, 2, false, ki);
-        z2 = pml_load(a, a.psi_szz_z + ipz, 2, true, kh);
+        z0 = pml_load<kNTP>(a, a.psi_sxz_z + ipz, 2, false, ki);
+        z1 = pml_load<kNTP>(a, a.psi_syz_z + ipz, 2, false, ki);
+        z2 = pml_load<kNTP>(a, a.psi_szz_z + ipz, 2, true, kh);
     }
     if (py) {
-        y0 = pml_load(a, a.psi_sxy_y + ipy, 1, false, ji);
-        y1 = pml_load(a, a.psi_syy_y + ipy, 1, true, jh);
-        y2 = pml_load(a, a.psi_syz_y + ipy, 1, false, ji);
+        y0 = pml_load<kNTP>(a, a.psi_sxy_y + ipy, 1, false, ji);
+        y1 = pml_load<kNTP>(a, a.psi_syy_y + ipy, 1, true, jh);
+        y2 = pml_load<kNTP>(a, a.psi_syz_y + ipy, 1, false, ji);
     }
     if (px) {
-        x0 = pml_load(a, a.psi_sxx_x + ipx, 0, true, iH);
-        x1 = pml_load(a, a.psi_sxy_x + ipx, 0, false, iI);
-        x2 = pml_load(a, a.psi_sxz_x + ipx, 0, false, iI);
+        x0 = pml_load<kNTP>(a, a.psi_sxx_x + ipx, 0, true, iH);
+        x1 = pml_load<kNTP>(a, a.psi_sxy_x + ipx, 0, false, iI);
+        x2 = pml_load<kNTP>(a, a.psi_sxz_x + ipx, 0, false, iI);
     }
     auto ldv = [](const S *p, unsigned off) -> R {
         if constexpr (kNTV & 1) return T::ldn_(p, off);
@@ -186,19 +192,19 @@ void k_v3d(FDArgs<R> a, int nbz, unsigned nb, int yg) {
     const R vx0 = ldv(vxp, os), vy0 = ldv(vyp, os), vz0 = ldv(vzp, os);
     const R rip = ldc(a.rip, o), rjp = ldc(a.rjp, o), rkp = ldc(a.rkp, o);
     if (pz) {
-        pml_apply(sxz_z, a.psi_sxz_z + ipz, z0);
-        pml_apply(syz_z, a.psi_syz_z + ipz, z1);
-        pml_apply(szz_z, a.psi_szz_z + ipz, z2);
+        pml_apply<kNTP>(sxz_z, a.psi_sxz_z + ipz, z0);
+        pml_apply<kNTP>(syz_z, a.psi_syz_z + ipz, z1);
+        pml_apply<kNTP>(szz_z, a.psi_szz_z + ipz, z2);
     }
     if (py) {
-        pml_apply(sxy_y, a.psi_sxy_y + ipy, y0);
-        pml_apply(syy_y, a.psi_syy_y + ipy, y1);
-        pml_apply(syz_y, a.psi_syz_y + ipy, y2);
+        pml_apply<kNTP>(sxy_y, a.psi_sxy_y + ipy, y0);
+        pml_apply<kNTP>(syy_y, a.psi_syy_y + ipy, y1);
+        pml_apply<kNTP>(syz_y, a.psi_syz_y + ipy, y2);
     }
     if (px) {
-        pml_apply(sxx_x, a.psi_sxx_x + ipx, x0);
-        pml_apply(sxy_x, a.psi_sxy_x + ipx, x1);
-        pml_apply(sxz_x, a.psi_sxz_x + ipx, x2);
+        pml_apply<kNTP>(sxx_x, a.psi_sxx_x + ipx, x0);
+        pml_apply<kNTP>(sxy_x, a.psi_sxy_x + ipx, x1);
+        pml_apply<kNTP>(sxz_x, a.psi_sxz_x + ipx, x2);
     }
     R vx = vx0 + ((sxx_x + sxy_y + sxz_z) * rip);
     R vy = vy0 + ((syy_y + sxy_x + syz_z) * rjp);
@@ -265,19 +271,19 @@ __global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz
     const long ipx = (long)ii * NY * NZ + (long)y * NZ + z;
     Pml<R> z0, z1, z2, y0, y1, y2, x0, x1, x2;
     if (pz) {
-        z0 = pml_load(a, a.psi_vx_z + ipz, 2, true, kh);
-        z1 = pml_load(a, a.psi_vy_z + ipz, 2, true, kh);
-        z2 = pml_load(a, a.psi_vz_z + ipz, 2, false, ki);
+        z0 = pml_load<kNTP>(a, a.psi_vx_z + ipz, 2, true, kh);
+        z1 = pml_load<kNTP>(a, a.psi_vy_z + ipz, 2, true, kh);
+        z2 = pml_load<kNTP>(a, a.psi_vz_z + ipz, 2, false, ki);
     }
     if (py) {
-        y0 = pml_load(a, a.psi_vx_y + ipy, 1, true, jh);
-        y1 = pml_load(a, a.psi_vy_y + ipy, 1, false, ji);
-        y2 = pml_load(a, a.psi_vz_y + ipy, 1, true, jh);
+        y0 = pml_load<kNTP>(a, a.psi_vx_y + ipy, 1, true, jh);
+        y1 = pml_load<kNTP>(a, a.psi_vy_y + ipy, 1, false, ji);
+        y2 = pml_load<kNTP>(a, a.psi_vz_y + ipy, 1, true, jh);
     }
     if (px) {
-        x0 = pml_load(a, a.psi_vx_x + ipx, 0, false, iI);
-        x1 = pml_load(a, a.psi_vy_x + ipx, 0, true, iH);
-        x2 = pml_load(a, a.psi_vz_x + ipx, 0, true, iH);
+        x0 = pml_load<kNTP>(a, a.psi_vx_x + ipx, 0, false, iI);
+        x1 = pml_load<kNTP>(a, a.psi_vy_x + ipx, 0, true, iH);
+        x2 = pml_load<kNTP>(a, a.psi_vz_x + ipx, 0, true, iH);
     }
     // the cell's own values and coefficients, before any store
     auto lds = [](const S *p, unsigned off) -> R {
@@ -327,19 +333,19 @@ __global__ __launch_bounds__((64 * bys<R, S>())) void k_s3d(FDArgs<R> a, int nbz
         }
     }
     if (pz) {
-        pml_apply(vxz, a.psi_vx_z + ipz, z0);
-        pml_apply(vyz, a.psi_vy_z + ipz, z1);
-        pml_apply(vzz, a.psi_vz_z + ipz, z2);
+        pml_apply<kNTP>(vxz, a.psi_vx_z + ipz, z0);
+        pml_apply<kNTP>(vyz, a.psi_vy_z + ipz, z1);
+        pml_apply<kNTP>(vzz, a.psi_vz_z + ipz, z2);
     }
     if (py) {
-        pml_apply(vxy, a.psi_vx_y + ipy, y0);
-        pml_apply(vyy, a.psi_vy_y + ipy, y1);
-        pml_apply(vzy, a.psi_vz_y + ipy, y2);
+        pml_apply<kNTP>(vxy, a.psi_vx_y + ipy, y0);
+        pml_apply<kNTP>(vyy, a.psi_vy_y + ipy, y1);
+        pml_apply<kNTP>(vzy, a.psi_vz_y + ipy, y2);
     }
     if (px) {
-        pml_apply(vxx, a.psi_vx_x + ipx, x0);
-        pml_apply(vyx, a.psi_vy_x + ipx, x1);
-        pml_apply(vzx, a.psi_vz_x + ipx, x2);
+        pml_apply<kNTP>(vxx, a.psi_vx_x + ipx, x0);
+        pml_apply<kNTP>(vyx, a.psi_vy_x + ipx, x1);
+        pml_apply<kNTP>(vzx, a.psi_vz_x + ipx, x2);
     }
     const R vxyyx = vxy + vyx;
     const R vyzzy = vyz + vzy;

@@ -49,21 +49,24 @@ template <typename R>
 struct Pml {
     R psi, a, b, K;
 };
-template <typename R>
+// NT: non-temporal access to the memory variable.
+template <bool NT = false, typename R>
 __device__ __forceinline__ Pml<R> pml_load(const FDArgs<R> &a, const R *psi, int axis, bool half,
                                            int idx) {
     Pml<R> p;
     const int k0 = half ? CKH : CK;
-    p.psi = *psi;
+    if constexpr (NT) p.psi = __builtin_nontemporal_load(psi);
+    else p.psi = *psi;
     p.K = cp(a, axis, k0, idx);
     p.a = cp(a, axis, k0 + 1, idx);
     p.b = cp(a, axis, k0 + 2, idx);
     return p;
 }
-template <typename R>
+template <bool NT = false, typename R>
 __device__ __forceinline__ void pml_apply(R &d, R *psi, const Pml<R> &p) {
     const R ps = p.b * p.psi + p.a * d;
-    *psi = ps;
+    if constexpr (NT) __builtin_nontemporal_store(ps, psi);
+    else *psi = ps;
     d = d / p.K + ps;
 }
 
