@@ -13,6 +13,13 @@
  *   - parameters and gradients are unpadded: indp = x*NY*NZ + y*NZ + z;
  *   - memory variables: r[l*NXp*NYp*NZp + indv];
  *   - CPML strips: src/update_v3D.cl:23-31.
+ *
+ * Pins: tests/test_oracle_*.py and tests/test_misfit_cpu.py check every
+ * function against closed forms, invariants or the reference's own test
+ * geometry.  Parity unpinned: the DFT gradient's per-cell coefficients
+ * beyond what its Parseval correlation with the time-domain gradient fixes,
+ * and the reference's 3D memory-variable index quirks that its restatement
+ * keeps (tests/test_oracle_dft.py names them).
  */
 #ifndef SEIS_ORACLE_H
 #define SEIS_ORACLE_H

@@ -8,9 +8,9 @@ Pinned: the band-pass (exact-bin sinusoids are scaled by the filter's
 H(f); arbitrary traces agree with numpy's rfft restatement of the same
 periodic-extension filter), restype 1 (residuals = data), scaleshot (a
 single shot-wide factor 1/|d|), scalermsnorm with the derivative fix
-(finite differences of 1/2 sum |o/|o| - d/|d||^2).  scalerms has no
-independent pin: library and oracle agree bit for bit, and its reference
-form does not vanish at a perfect fit (noted in the test below).
+(finite differences of 1/2 sum |o/|o| - d/|d||^2), scalerms by a closed
+form reduced by hand from the reference's steps (the reference states no
+objective for it, and its source does not vanish at a perfect fit).
 """
 import numpy as np
 import pytest
@@ -114,6 +114,32 @@ def test_scalermsnorm_is_the_derivative(oracle_lib):
             fd = (J(o + e) - J(o - e)) / (2 * h)
             assert fixed[t, g] == pytest.approx(fd, rel=1e-4, abs=1e-7)
     assert np.linalg.norm(ref - fixed) > 0.1 * np.linalg.norm(fixed)
+
+
+def test_scalerms_closed_form(oracle_lib):
+    """scalerms (src/residuals.c:123-186, 209-237, 327-341) reduced by hand
+    to a closed form in double.  With per-receiver norms |o_g|, |d_g| over
+    all outputs and samples and the shot norm |D|: the outputs are scaled by
+    c_g = |d_g| / (|o_g| |D|) and the data by 1/|D|; the factor that
+    multiplies the normalised misfit is |D|^2 (the data scale is overwritten
+    by 1/|D| before it is read), so the source is
+    c_g (c_g (|d_g|^2 - |d_g| <o_g, d_g> / |o_g|) o_g - d_g / |D|),
+    the scaled data have unit energy (rmsnorm = 1) and every scaled output
+    trace has its scaled data trace's energy."""
+    out, din = _traces(NT=300, nr=4, nout=2, seed=3, noise=0.4)
+    res, (rms, rmsnorm) = oracle_lib.residuals(out, din, NT=300, seisout=1, scalerms=1, **BASE)
+    o = np.stack([x.astype(np.float64) for x in out])          # (nout, NT, nr)
+    d = np.stack([x.astype(np.float64) for x in din])
+    no = np.sqrt(np.sum(o ** 2, axis=(0, 1)))
+    nd = np.sqrt(np.sum(d ** 2, axis=(0, 1)))
+    nD = np.sqrt(np.sum(d ** 2))
+    c = nd / (no * nD)
+    inner = np.sum(o * d, axis=(0, 1))
+    want = c * (c * (nd ** 2 - nd * inner / no) * o - d / nD)
+    got = np.stack(res).astype(np.float64)
+    assert np.linalg.norm(got - want) <= 1e-4 * np.linalg.norm(want)   # float sums: 7e-6
+    assert rmsnorm == pytest.approx(1.0, rel=1e-5)
+    assert rms == pytest.approx(float(np.sum((d / nD - c * o) ** 2)), rel=1e-4)
 
 
 def test_scalerms_reference_form(oracle_lib):
